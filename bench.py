@@ -1,0 +1,119 @@
+"""Bake benchmark: SDF voxel-queries/s of the plain single-mesh bake on one MI355X.
+
+Workload (one step): a 327,680-triangle displaced icosphere (subdivision 7, seeded 8-mode radial
+displacement) already resident in HBM is baked to a dense 256^3 FP32 signed-distance grid over
+[-0.8, 0.8]^3 plus a 512^3 bit-packed occupancy grid: weld + closedness check, LBVH build,
+occupancy, +z column crossings, per-voxel closest triangle + sign.
+
+Prints ONE JSON line.  `roofline` describes the dominant kernel (the per-voxel query); its FLOPs
+are algorithmic: traversal counters from one counted run times per-test FLOP constants.
+`cpu_baseline` times the oracle's all-core CPU bake (same exact math) on a bounded z-slab sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "italy-rs_amd"))
+
+METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake latency"
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published)
+PEAK_HBM_GBS = 8000.0
+# FLOPs per test, counted from the kernel source (ib_device.hpp): point/box distance 3x(sub, mul,
+# add); closest point on triangle, face-region path (the longest): 77
+FLOP_BOX = 9
+FLOP_TRI = 77
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-slabs", type=int, default=6, help="z-slabs of the 256^3 grid timed on the CPU")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the bake shards by mesh or "
+              "brick across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    from italy_bake import bake, meshgen
+
+    mesh, cfg = meshgen.workload_plain_large()
+    sdf = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["sdf_res"])
+    occ = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["occ_res"])
+    nvox = sdf.nx * sdf.ny * sdf.nz
+    ctx = bake.Context(0)
+    d_xyz = ctx.alloc(mesh.xyz.nbytes)
+    ctx.upload(d_xyz, mesh.xyz)
+    d_dist = ctx.alloc(4 * nvox)
+    d_occ = ctx.alloc(4 * ((occ.nx + 31) // 32) * occ.ny * occ.nz)
+
+    def step():
+        ctx.bake_dense_device(d_xyz, mesh.ntri, sdf, d_dist, occ, d_occ)
+
+    for _ in range(args.warmup):
+        step()
+    ctx.synchronize()
+    sdf_ms, total_ms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = ctx.stats()  # event timings of this bake (waits for its last event)
+        sdf_ms.append(st["ms_sdf"])
+        total_ms.append(st["ms_total"])
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = nvox / (ms_per_step * 1e-3)
+
+    # algorithmic work of the query kernel from one counted run
+    counters = ctx.query_counters(d_xyz, mesh.ntri, sdf) if hasattr(ctx, "query_counters") else None
+    sdf_avg_ms = float(np.mean(sdf_ms))
+    roofline = {"bound": "valu", "achieved": None, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": None,
+                "traffic": None, "kernel": "k_sdf", "kernel_ms": sdf_avg_ms}
+    if counters:
+        flops = counters["box_tests"] * FLOP_BOX + counters["tri_tests"] * FLOP_TRI
+        achieved = flops / (sdf_avg_ms * 1e-3) / 1e12
+        roofline.update(achieved=achieved, frac=achieved / PEAK_FP32_TFLOPS, flops_per_launch=flops,
+                        box_tests=counters["box_tests"], tri_tests=counters["tri_tests"])
+    alg_bytes = 4 * nvox + mesh.xyz.nbytes
+    roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        cores = min(16, os.cpu_count() or 1)
+        z0 = sdf.nz // 2 - args.cpu_slabs // 2
+        _, cst = ib_oracle.bake_dense(mesh.xyz, list(sdf.origin), sdf.h, sdf.nx, sdf.ny, sdf.nz, z0,
+                                      z0 + args.cpu_slabs, threads=cores)
+        cpu = {"value": cst["voxels"] / cst["seconds"], "unit": "voxel-queries/s", "cores": cst["threads"],
+               "kind": "port",
+               "sample": f"{args.cpu_slabs} central z-slabs of the 256^3 grid ({cst['voxels']} voxels), "
+                         f"LBVH build included, {cst['seconds']:.2f} s"}
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "voxel-queries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "plain single-mesh bake: 327,680-triangle displaced icosphere (level 7, seed "
+                               "0x17A1E5), 256^3 FP32 SDF + 512^3 bit-packed occupancy over [-0.8,0.8]^3",
+                   "triangles": mesh.ntri, "sdf_res": sdf.nx, "occ_res": occ.nx, "parallelism": "single GPU"},
+        "device_ms_per_bake": float(np.mean(total_ms)),
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,127 @@
+/*
+ * italy_bake.h -- C ABI of the MI355X (gfx950) mesh -> SDF / occupancy bake.
+ *
+ * Drop-in boundary for the reference's src/convert bake path.  Plain pointers and sizes only,
+ * no exceptions across the boundary: every entry point returns an ib_status and leaves a
+ * message for ib_last_error().  The C++ wrappers in include/italy_bake.hpp rethrow non-zero
+ * statuses as std::runtime_error, which is the reference's error convention
+ * (src/render/optix_check.h:13-48).
+ *
+ * Conventions shared with the reference:
+ *   - input is a triangle soup: 9 floats per triangle, 3 vertices in winding order
+ *     (src/io/mesh_asset.h:34,46), with caller-supplied bounds (mesh_asset.h:43-44);
+ *   - grids are cell-centred, value of cell (x,y,z) sampled at origin + (i + 0.5) * h and stored
+ *     at index (z*ny + y)*nx + x (src/convert/sdf_grid.h:35, src/render/kernels/sdf_bake.cu:34-37,71);
+ *   - distances are negative inside (sdf_bake.cu:69).  "Inside" means |generalized winding
+ *     number| > 0.5, which agrees with the reference's ray-parity vote on closed meshes of either
+ *     winding.
+ *   - occupancy is the reference's surface voxelization (src/convert/voxelize.cpp:52-107 with the
+ *     triangle/box test of src/convert/tri_box_overlap.h:72-114), bit-exact.
+ */
+#ifndef ITALY_BAKE_H
+#define ITALY_BAKE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  IB_OK = 0,
+  IB_ERR_INVALID_ARGUMENT = 1,
+  IB_ERR_INVALID_MESH = 2,   /* non-finite coordinates */
+  IB_ERR_OUT_OF_BUDGET = 3,  /* the request does not fit the context's HBM budget */
+  IB_ERR_DEVICE = 4,         /* a HIP call failed; ib_last_error() names it */
+  IB_ERR_NO_DEVICE = 5,
+  IB_ERR_UNSUPPORTED = 6,
+  IB_ERR_CAPACITY = 7,       /* caller buffer too small; the needed size was written back */
+  IB_ERR_NOT_READY = 8,      /* served job still running */
+  IB_ERR_INTERNAL = 9
+} ib_status;
+
+typedef struct ib_context ib_context;
+
+/* Cell-centred grid: cell (x,y,z) spans origin + [i, i+1) * h. */
+typedef struct {
+  float origin[3];
+  float h;
+  int32_t nx, ny, nz;
+  int32_t reserved;
+} ib_grid;
+
+/* Per-bake counters and device timings (milliseconds, HIP events on the bake stream). */
+typedef struct {
+  double ms_total;       /* first kernel start .. last kernel end */
+  double ms_ingest;      /* weld + closedness */
+  double ms_bvh;         /* LBVH build */
+  double ms_occupancy;
+  double ms_columns;     /* +z column crossings */
+  double ms_sdf;         /* closest-triangle + sign per voxel */
+  uint64_t triangles;
+  uint64_t voxels;       /* SDF voxel queries */
+  uint64_t occ_cells;    /* occupancy cells covered by the grid */
+  int32_t bvh_depth;
+  int32_t deep_stack;    /* 1 if the global-memory traversal stack was used */
+  int64_t boundary_edges;
+  uint64_t column_overflows;
+  uint64_t rechecked;    /* voxels whose winding number was re-evaluated in FP64 */
+  uint64_t ties;         /* voxels decided by the ray-parity tie-break */
+  uint64_t reserved[6];
+} ib_stats;
+
+/* ---- context ------------------------------------------------------------------------------ */
+/* device: HIP ordinal.  hbm_budget_bytes: cap on what the context may allocate (0 = 90% of free). */
+ib_status ib_create(int device, uint64_t hbm_budget_bytes, ib_context **out);
+void ib_destroy(ib_context *ctx);
+const char *ib_last_error(void);
+const char *ib_version(void);
+
+/* ---- framing (pure host arithmetic, identical to the reference) ----------------------------- */
+/* grid of italy::bakeSdf(mesh, resolution): src/convert/sdf_baker.cpp:139-148 */
+ib_status ib_sdf_framing(const float bmin[3], const float bmax[3], int32_t resolution, ib_grid *grid);
+/* origin and voxel size of italy::voxelizeMesh(mesh, resolution): src/convert/voxelize.cpp:54-57 */
+ib_status ib_vox_framing(const float bmin[3], const float bmax[3], int32_t resolution, float origin[3],
+                         float *voxel_size);
+
+/* ---- drop-in entry points (host buffers) ---------------------------------------------------- */
+/* Replaces the distance part of italy::bakeSdf(const MeshAsset&, int) (src/convert/sdf_baker.h:8,
+ * body sdf_baker.cpp:137-310): frames the grid as the reference does, writes nx*ny*nz signed
+ * distances.  Pass distances = NULL to only receive the grid. */
+ib_status ib_bake_sdf(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                      int32_t resolution, ib_grid *grid, float *distances, uint64_t capacity);
+
+/* Replaces italy::voxelizeMesh(const MeshAsset&, int) (src/convert/voxelize.h:8, body
+ * voxelize.cpp:52-107): writes the occupied cells (x,y,z int triples) in the reference's first-touch
+ * order (triangle index, then z, y, x).  cell_tri receives, per cell, the first touching triangle
+ * (may be NULL).  On IB_ERR_CAPACITY *count holds the number of cells needed. */
+ib_status ib_voxelize(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                      int32_t resolution, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count);
+
+/* Per-cell material splat of bakeSdf (src/convert/sdf_baker.cpp:76-128): per-triangle material
+ * (r,g,b,metallic,roughness: 5 floats each) averaged over every grid cell the triangle overlaps,
+ * accumulated in triangle order; cells without triangles get (1,1,1,0,1). */
+ib_status ib_splat_materials(ib_context *ctx, const float *xyz, uint64_t ntri, const float *tri_material,
+                             const ib_grid *grid, float *base_color, float *metallic, float *roughness);
+
+/* ---- explicit grids (the workload entry points) -------------------------------------------- */
+/* Dense bake of one mesh: sdf grid (may be NULL) and occupancy grid (may be NULL).  Host buffers. */
+ib_status ib_bake_dense(ib_context *ctx, const float *xyz, uint64_t ntri, const ib_grid *sdf, float *distances,
+                        const ib_grid *occ, uint32_t *occ_words);
+/* Same with device-resident input and outputs, enqueued on `stream` (hipStream_t, NULL = the
+ * context's stream).  occ_words holds ny*nz*ceil(nx/32) words; bit x&31 of word
+ * ((z*ny + y)*ceil(nx/32) + x/32). */
+ib_status ib_bake_dense_device(ib_context *ctx, const float *d_xyz, uint64_t ntri, const ib_grid *sdf,
+                               float *d_distances, const ib_grid *occ, uint32_t *d_occ_words, void *stream);
+
+/* ---- device memory helpers (for callers without a GPU framework) --------------------------- */
+ib_status ib_device_alloc(ib_context *ctx, uint64_t bytes, void **d_ptr);
+ib_status ib_device_free(ib_context *ctx, void *d_ptr);
+ib_status ib_memcpy_h2d(ib_context *ctx, void *d_dst, const void *src, uint64_t bytes);
+ib_status ib_memcpy_d2h(ib_context *ctx, void *dst, const void *d_src, uint64_t bytes);
+ib_status ib_synchronize(ib_context *ctx);
+ib_status ib_last_stats(ib_context *ctx, ib_stats *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

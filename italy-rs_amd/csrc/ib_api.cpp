@@ -1,0 +1,599 @@
+// C ABI of the bake (include/italy_bake.h): context, device buffers, bake orchestration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "ib_kernels.hpp"
+#include "italy_bake.h"
+
+using namespace ib;
+
+namespace {
+
+thread_local std::string g_err;
+
+ib_status fail(ib_status s, const std::string &msg) {
+  g_err = msg;
+  return s;
+}
+
+#define IB_HIP(call)                                                                                  \
+  do {                                                                                                \
+    const hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) return fail(IB_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define IB_TRY(expr)                    \
+  do {                                  \
+    const ib_status s_ = (expr);        \
+    if (s_ != IB_OK) return s_;         \
+  } while (0)
+
+struct Buf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace
+
+enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_COUNT };
+
+struct ib_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0;
+  uint64_t allocated = 0;
+  std::unordered_map<std::string, Buf> bufs;
+  std::unordered_map<void *, size_t> user;
+  Scalars *host_sc = nullptr;  // pinned
+  hipEvent_t ev[EV_COUNT] = {};
+  bool ev_valid[EV_COUNT] = {};
+  ib_stats stats{};
+  std::mutex mu;
+
+  ib_status alloc_raw(size_t bytes, void **p) {
+    if (allocated + bytes > budget)
+      return fail(IB_ERR_OUT_OF_BUDGET, "allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
+                                            std::to_string(allocated) + " of " + std::to_string(budget) + " in use)");
+    const hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess) return fail(IB_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocated += bytes;
+    return IB_OK;
+  }
+  void free_raw(void *p, size_t bytes) {
+    if (p) {
+      (void)hipFree(p);
+      allocated -= bytes;
+    }
+  }
+  // named scratch that only grows; contents are not preserved across growth
+  template <typename T>
+  ib_status scratch(const char *name, size_t count, T **out) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    Buf &b = bufs[name];
+    if (b.bytes < bytes) {
+      free_raw(b.p, b.bytes);
+      b.p = nullptr;
+      b.bytes = 0;
+      const size_t want = bytes + bytes / 4;
+      void *p = nullptr;
+      ib_status st = alloc_raw(want, &p);
+      if (st != IB_OK) {
+        st = alloc_raw(bytes, &p);
+        if (st != IB_OK) return st;
+        b.bytes = bytes;
+      } else {
+        b.bytes = want;
+      }
+      b.p = p;
+    }
+    *out = static_cast<T *>(b.p);
+    return IB_OK;
+  }
+  void record(EventId e, hipStream_t s) {
+    (void)hipEventRecord(ev[e], s);
+    ev_valid[e] = true;
+  }
+};
+
+namespace {
+
+uint32_t pow2_at_least(uint64_t v) {
+  uint64_t c = 1024;
+  while (c < v) c <<= 1;
+  return (uint32_t)c;
+}
+
+bool grid_ok(const ib_grid *g) {
+  return g && g->nx > 0 && g->ny > 0 && g->nz > 0 && std::isfinite(g->h) && g->h > 0.0f &&
+         std::isfinite(g->origin[0]) && std::isfinite(g->origin[1]) && std::isfinite(g->origin[2]);
+}
+
+GridArgs to_args(const ib_grid &g) { return GridArgs{g.origin[0], g.origin[1], g.origin[2], g.h, g.nx, g.ny, g.nz}; }
+
+// Hole caps: the unmatched directed edges form closed chains (every vertex has as many outgoing as
+// incoming unmatched edges); each chain is closed by a fan from the centroid of its vertices.  Then
+// (mesh + reversed caps) is a closed surface, and w_mesh = w_closed + w_caps.
+void build_caps(const std::vector<int2> &edges, const std::vector<float> &soup, std::vector<float> &caps) {
+  caps.clear();
+  std::unordered_map<int, std::vector<int>> out;  // vertex -> indices of edges starting there
+  out.reserve(edges.size() * 2);
+  for (size_t i = 0; i < edges.size(); ++i) out[edges[i].x].push_back((int)i);
+  std::vector<char> used(edges.size(), 0);
+  std::vector<int> loop;
+  for (size_t s = 0; s < edges.size(); ++s) {
+    if (used[s]) continue;
+    loop.clear();
+    int e = (int)s;
+    const int start = edges[s].x;
+    for (;;) {
+      used[e] = 1;
+      loop.push_back(e);
+      const int v = edges[e].y;
+      if (v == start) break;
+      auto &lst = out[v];
+      int next = -1;
+      while (!lst.empty()) {
+        const int c = lst.back();
+        lst.pop_back();
+        if (!used[c]) {
+          next = c;
+          break;
+        }
+      }
+      if (next < 0) break;  // cannot happen for a boundary cycle; close the fan anyway
+      e = next;
+    }
+    double cx = 0, cy = 0, cz = 0;
+    for (int id : loop) {
+      const int v = edges[id].x;
+      cx += soup[3 * (size_t)v];
+      cy += soup[3 * (size_t)v + 1];
+      cz += soup[3 * (size_t)v + 2];
+    }
+    const float apex[3] = {(float)(cx / loop.size()), (float)(cy / loop.size()), (float)(cz / loop.size())};
+    for (int id : loop) {
+      const int a = edges[id].x, b = edges[id].y;
+      caps.insert(caps.end(), {apex[0], apex[1], apex[2], soup[3 * (size_t)a], soup[3 * (size_t)a + 1],
+                               soup[3 * (size_t)a + 2], soup[3 * (size_t)b], soup[3 * (size_t)b + 1],
+                               soup[3 * (size_t)b + 2]});
+    }
+  }
+}
+
+float ev_ms(ib_context *c, EventId a, EventId b) {
+  if (!c->ev_valid[a] || !c->ev_valid[b]) return 0.0f;
+  float ms = 0.0f;
+  if (hipEventElapsedTime(&ms, c->ev[a], c->ev[b]) != hipSuccess) return 0.0f;
+  return ms;
+}
+
+struct MeshDev {
+  int n = 0;
+  Node *nodes = nullptr;
+  float4 *tris = nullptr;
+  int32_t *order = nullptr;
+  float *caps = nullptr;
+  int ncap = 0;
+  int depth = 0;
+};
+
+// ingest + LBVH of one mesh whose soup is already on the device; synchronises once to read the
+// scalars (validity, depth, boundary edges) and, for open meshes, to build the hole caps.
+ib_status build_mesh(ib_context *c, const float *d_xyz, int n, hipStream_t s, const ib_grid *occ,
+                     uint32_t *d_occ, MeshDev &m) {
+  Scalars *sc;
+  IB_TRY(c->scratch("sc", 1, &sc));
+  // reset scalars: cmin = all ones, the rest zero
+  fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
+  fill_u32(sc->cmin, 0xffffffffu, 3, s);
+  c->record(EV_START, s);
+  // weld + closedness
+  const uint32_t vcap = pow2_at_least(2ull * 3 * n), ecap = pow2_at_least(2ull * 3 * n);
+  int32_t *slots, *minid, *cslot, *enet;
+  unsigned long long *ekeys;
+  int2 *bnd;
+  IB_TRY(c->scratch("w_slots", vcap, &slots));
+  IB_TRY(c->scratch("w_minid", vcap, &minid));
+  IB_TRY(c->scratch("w_cslot", 3ull * n, &cslot));
+  IB_TRY(c->scratch("e_keys", ecap, &ekeys));
+  IB_TRY(c->scratch("e_net", ecap, &enet));
+  IB_TRY(c->scratch("e_bnd", 3ull * n, &bnd));
+  fill_u32(reinterpret_cast<uint32_t *>(slots), 0xffffffffu, vcap, s);
+  fill_u32(reinterpret_cast<uint32_t *>(minid), 0x7fffffffu, vcap, s);
+  fill_u32(reinterpret_cast<uint32_t *>(ekeys), 0xffffffffu, 2ull * ecap, s);
+  fill_u32(reinterpret_cast<uint32_t *>(enet), 0u, ecap, s);
+  launch_weld(d_xyz, n, slots, minid, vcap - 1, cslot, ekeys, enet, ecap - 1, sc, bnd, 3u * n, s);
+  c->record(EV_INGEST, s);
+  // LBVH
+  uint32_t *codes_in, *codes;
+  int32_t *idx, *leaf_parent;
+  uint32_t *visits;
+  float *root_box;
+  IB_TRY(c->scratch("codes_in", n, &codes_in));
+  IB_TRY(c->scratch("codes", n, &codes));
+  IB_TRY(c->scratch("idx", n, &idx));
+  IB_TRY(c->scratch("order", n, &m.order));
+  IB_TRY(c->scratch("nodes", std::max(n - 1, 1), &m.nodes));
+  IB_TRY(c->scratch("leaf_parent", n, &leaf_parent));
+  IB_TRY(c->scratch("tris", 3ull * n, &m.tris));
+  IB_TRY(c->scratch("visits", n, &visits));
+  IB_TRY(c->scratch("root_box", 8, &root_box));
+  launch_prep(d_xyz, n, sc, s);
+  launch_morton(d_xyz, n, sc, codes_in, idx, s);
+  size_t temp_bytes = 0;
+  IB_HIP(sort_pairs(nullptr, temp_bytes, codes_in, codes, idx, m.order, n, s));
+  void *temp;
+  IB_TRY(c->scratch("sort_temp", temp_bytes, reinterpret_cast<char **>(&temp)));
+  IB_HIP(sort_pairs(temp, temp_bytes, codes_in, codes, idx, m.order, n, s));
+  launch_karras(codes, n, m.nodes, leaf_parent, s);
+  launch_gather(d_xyz, m.order, n, m.tris, s);
+  fill_u32(visits, 0u, n, s);
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, s);
+  c->record(EV_BVH, s);
+  if (occ && d_occ) {
+    const uint64_t words = (uint64_t)((occ->nx + 31) / 32) * occ->ny * occ->nz;
+    fill_u32(d_occ, 0u, words, s);
+    launch_occupancy(d_xyz, n, to_args(*occ), d_occ, s);
+  }
+  c->record(EV_OCC, s);
+  IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  const Scalars h = *c->host_sc;
+  if (h.flags & kFlagNonFinite) return fail(IB_ERR_INVALID_MESH, "mesh has non-finite vertex coordinates");
+  m.n = n;
+  m.depth = (int)h.depth;
+  if (m.depth > kMaxDepth) return fail(IB_ERR_INTERNAL, "BVH parent chain does not reach the root");
+  if (m.depth + 2 > kStack)
+    return fail(IB_ERR_UNSUPPORTED, "BVH depth " + std::to_string(m.depth) + " exceeds the column traversal stack");
+  c->stats.bvh_depth = m.depth;
+  c->stats.boundary_edges = h.boundary;
+  m.ncap = 0;
+  m.caps = nullptr;
+  if (h.boundary > 0) {
+    std::vector<int2> edges(std::min<uint32_t>(h.boundary, 3u * n));
+    IB_HIP(hipMemcpy(edges.data(), bnd, edges.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    std::vector<float> soup(9ull * n);
+    IB_HIP(hipMemcpy(soup.data(), d_xyz, soup.size() * sizeof(float), hipMemcpyDeviceToHost));
+    std::sort(edges.begin(), edges.end(), [](int2 a, int2 b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+    std::vector<float> caps;
+    build_caps(edges, soup, caps);
+    m.ncap = (int)(caps.size() / 9);
+    IB_TRY(c->scratch("caps", caps.size(), &m.caps));
+    IB_HIP(hipMemcpyAsync(m.caps, caps.data(), caps.size() * sizeof(float), hipMemcpyHostToDevice, s));
+    IB_HIP(hipStreamSynchronize(s));
+  }
+  return IB_OK;
+}
+
+ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *d_out, hipStream_t s) {
+  Scalars *sc;
+  IB_TRY(c->scratch("sc", 1, &sc));
+  const uint64_t ncol = (uint64_t)g.nx * g.ny;
+  ColumnArgs ca{};
+  ca.nodes = m.nodes;
+  ca.nnodes = std::max(m.n - 1, 1);
+  ca.tris = m.tris;
+  ca.caps = m.caps;
+  ca.ncap = m.ncap;
+  ca.g = to_args(g);
+  ca.sc = sc;
+  IB_TRY(c->scratch("col_cross", ncol * kColCap, &ca.cross));
+  IB_TRY(c->scratch("col_count", ncol, &ca.count));
+  launch_columns(ca, s);
+  c->record(EV_COLS, s);
+  SdfArgs sa{};
+  sa.col = ca;
+  sa.out = d_out;
+  const bool deep = m.depth + 2 > kStack;
+  c->stats.deep_stack = deep ? 1 : 0;
+  if (deep) {
+    const uint64_t threads = (uint64_t)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4) * 256;
+    sa.gstack_depth = m.depth + 2;
+    IB_TRY(c->scratch("gstack", threads * sa.gstack_depth, &sa.gstack));
+    IB_TRY(c->scratch("gstackd", threads * sa.gstack_depth, &sa.gstackd));
+  }
+  launch_sdf(sa, deep, s);
+  c->record(EV_SDF, s);
+  return IB_OK;
+}
+
+// empty mesh: no surface anywhere
+ib_status fill_empty(ib_context *c, const ib_grid *sdf, float *d_dist, const ib_grid *occ, uint32_t *d_occ,
+                     hipStream_t s) {
+  (void)c;
+  if (sdf && d_dist)
+    fill_u32(reinterpret_cast<uint32_t *>(d_dist), 0x7f800000u, (uint64_t)sdf->nx * sdf->ny * sdf->nz, s);
+  if (occ && d_occ) fill_u32(d_occ, 0u, (uint64_t)((occ->nx + 31) / 32) * occ->ny * occ->nz, s);
+  return IB_OK;
+}
+
+void reset_stats(ib_context *c) {
+  c->stats = ib_stats{};
+  for (bool &v : c->ev_valid) v = false;
+}
+
+ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib_grid *sdf, float *d_dist,
+                      const ib_grid *occ, uint32_t *d_occ, hipStream_t s) {
+  if (sdf && !grid_ok(sdf)) return fail(IB_ERR_INVALID_ARGUMENT, "invalid SDF grid");
+  if (occ && !grid_ok(occ)) return fail(IB_ERR_INVALID_ARGUMENT, "invalid occupancy grid");
+  if ((sdf && !d_dist) || (occ && !d_occ)) return fail(IB_ERR_INVALID_ARGUMENT, "missing output buffer");
+  if (ntri > (uint64_t)(INT_MAX / 9)) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  reset_stats(c);
+  c->stats.triangles = ntri;
+  c->stats.voxels = sdf ? (uint64_t)sdf->nx * sdf->ny * sdf->nz : 0;
+  c->stats.occ_cells = occ ? (uint64_t)occ->nx * occ->ny * occ->nz : 0;
+  if (ntri == 0) return fill_empty(c, sdf, d_dist, occ, d_occ, s);
+  if (!d_xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+  MeshDev m;
+  IB_TRY(build_mesh(c, d_xyz, (int)ntri, s, occ, d_occ, m));
+  if (sdf) IB_TRY(query_dense(c, m, *sdf, d_dist, s));
+  return IB_OK;
+}
+
+}  // namespace
+
+// ============================================================================================
+// C ABI
+// ============================================================================================
+extern "C" {
+
+const char *ib_last_error(void) { return g_err.c_str(); }
+const char *ib_version(void) { return "italy-bake 0.1 gfx950"; }
+
+ib_status ib_create(int device, uint64_t budget, ib_context **out) {
+  if (!out) return fail(IB_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(IB_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= count) return fail(IB_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+  IB_HIP(hipSetDevice(device));
+  auto *c = new ib_context();
+  c->device = device;
+  size_t freeb = 0, total = 0;
+  if (hipMemGetInfo(&freeb, &total) != hipSuccess) {
+    delete c;
+    return fail(IB_ERR_DEVICE, "hipMemGetInfo failed");
+  }
+  c->budget = budget ? budget : (uint64_t)(0.9 * (double)freeb);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipHostMalloc((void **)&c->host_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return fail(IB_ERR_DEVICE, "stream / pinned allocation failed");
+  }
+  for (auto &e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return IB_OK;
+}
+
+void ib_destroy(ib_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &kv : c->bufs) (void)hipFree(kv.second.p);
+  for (auto &kv : c->user) (void)hipFree(kv.first);
+  for (auto &e : c->ev) (void)hipEventDestroy(e);
+  (void)hipHostFree(c->host_sc);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+ib_status ib_sdf_framing(const float bmin[3], const float bmax[3], int32_t res, ib_grid *g) {
+  if (!bmin || !bmax || !g) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  // sdf_baker.cpp:139-148
+  float ext[3];
+  for (int k = 0; k < 3; ++k) ext[k] = bmax[k] - bmin[k];
+  const float longest = std::max({ext[0], ext[1], ext[2], 1e-6f});
+  const float vs = longest / static_cast<float>(std::max(res, 1));
+  const float pad = vs * 2.0f;
+  const float pad2 = pad * 2.0f;
+  for (int k = 0; k < 3; ++k) g->origin[k] = bmin[k] - pad;
+  g->nx = static_cast<int>(std::ceil((ext[0] + pad2) / vs)) + 1;
+  g->ny = static_cast<int>(std::ceil((ext[1] + pad2) / vs)) + 1;
+  g->nz = static_cast<int>(std::ceil((ext[2] + pad2) / vs)) + 1;
+  g->h = vs;
+  g->reserved = 0;
+  return IB_OK;
+}
+
+ib_status ib_vox_framing(const float bmin[3], const float bmax[3], int32_t res, float origin[3], float *vs) {
+  if (!bmin || !bmax || !origin || !vs) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  // voxelize.cpp:54-57
+  float ext[3];
+  for (int k = 0; k < 3; ++k) ext[k] = bmax[k] - bmin[k];
+  const float longest = std::max({ext[0], ext[1], ext[2], 1e-6f});
+  *vs = longest / static_cast<float>(std::max(res, 1));
+  for (int k = 0; k < 3; ++k) origin[k] = bmin[k];
+  return IB_OK;
+}
+
+ib_status ib_bake_dense_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib_grid *sdf, float *d_dist,
+                               const ib_grid *occ, uint32_t *d_occ, void *stream) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  return bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s);
+}
+
+ib_status ib_bake_dense(ib_context *c, const float *xyz, uint64_t ntri, const ib_grid *sdf, float *dist,
+                        const ib_grid *occ, uint32_t *occ_words) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  if (ntri && !xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+  if ((sdf && !dist) || (occ && !occ_words)) return fail(IB_ERR_INVALID_ARGUMENT, "missing output buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  float *d_xyz = nullptr, *d_dist = nullptr;
+  uint32_t *d_occ = nullptr;
+  const uint64_t nvox = sdf ? (uint64_t)sdf->nx * sdf->ny * sdf->nz : 0;
+  const uint64_t nwords = occ ? (uint64_t)((occ->nx + 31) / 32) * occ->ny * occ->nz : 0;
+  IB_TRY(c->scratch("in_xyz", std::max<uint64_t>(9 * ntri, 1), &d_xyz));
+  if (sdf) IB_TRY(c->scratch("out_dist", nvox, &d_dist));
+  if (occ) IB_TRY(c->scratch("out_occ", nwords, &d_occ));
+  if (ntri) IB_HIP(hipMemcpyAsync(d_xyz, xyz, 9 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
+  IB_TRY(bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s));
+  if (sdf) IB_HIP(hipMemcpyAsync(dist, d_dist, nvox * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (occ) IB_HIP(hipMemcpyAsync(occ_words, d_occ, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
+ib_status ib_bake_sdf(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                      int32_t res, ib_grid *grid, float *dist, uint64_t capacity) {
+  if (!c || !grid) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  IB_TRY(ib_sdf_framing(bmin, bmax, res, grid));
+  if (!dist) return IB_OK;
+  const uint64_t nvox = (uint64_t)grid->nx * grid->ny * grid->nz;
+  if (capacity < nvox) return fail(IB_ERR_CAPACITY, "distance buffer holds " + std::to_string(capacity) +
+                                                        " floats, the grid needs " + std::to_string(nvox));
+  return ib_bake_dense(c, xyz, ntri, grid, dist, nullptr, nullptr);
+}
+
+ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                      int32_t res, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count) {
+  if (!c || !count) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  if (ntri && !xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+  *count = 0;
+  if (ntri == 0) return IB_OK;
+  if (res <= 0)
+    return fail(IB_ERR_INVALID_ARGUMENT, "resolution must be >= 1 (the reference clamps cells to resolution-1)");
+  if ((uint64_t)res * res * res >= (1ull << 32)) return fail(IB_ERR_UNSUPPORTED, "resolution too large");
+  if (ntri > (uint64_t)(INT_MAX / 9)) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  ib_grid g{};
+  IB_TRY(ib_vox_framing(bmin, bmax, res, g.origin, &g.h));
+  g.nx = g.ny = g.nz = res;
+  const uint64_t ncell = (uint64_t)res * res * res;
+  float *d_xyz;
+  int32_t *first;
+  Scalars *sc;
+  IB_TRY(c->scratch("in_xyz", 9 * ntri, &d_xyz));
+  IB_TRY(c->scratch("vx_first", ncell, &first));
+  IB_TRY(c->scratch("sc", 1, &sc));
+  IB_HIP(hipMemcpyAsync(d_xyz, xyz, 9 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
+  fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
+  fill_u32(reinterpret_cast<uint32_t *>(first), 0x7fffffffu, ncell, s);
+  launch_first_touch(d_xyz, (int)ntri, to_args(g), first, s);
+  // compact the occupied cells into (first triangle, linear cell) keys and read the count back
+  unsigned long long *keys, *sorted;
+  IB_TRY(c->scratch("vx_keys", ncell, &keys));
+  launch_first_compact(first, ncell, sc, keys, s);
+  IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  const uint32_t n = c->host_sc->occ_set;
+  *count = n;
+  if (!cells) return IB_OK;
+  if (capacity < n) return fail(IB_ERR_CAPACITY, "cell buffer holds " + std::to_string(capacity) + " cells, " +
+                                                     std::to_string(n) + " are occupied");
+  IB_TRY(c->scratch("vx_sorted", std::max<uint32_t>(n, 1), &sorted));
+  size_t tb = 0;
+  IB_HIP(sort_keys64(nullptr, tb, keys, sorted, n, s));
+  char *temp;
+  IB_TRY(c->scratch("vx_temp", tb, &temp));
+  IB_HIP(sort_keys64(temp, tb, keys, sorted, n, s));
+  int32_t *d_cells, *d_tri = nullptr;
+  IB_TRY(c->scratch("vx_cells", 3ull * std::max<uint32_t>(n, 1), &d_cells));
+  if (cell_tri) IB_TRY(c->scratch("vx_tri", std::max<uint32_t>(n, 1), &d_tri));
+  launch_unpack_cells(sorted, n, res, res, d_cells, d_tri, s);
+  IB_HIP(hipMemcpyAsync(cells, d_cells, 3ull * n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (cell_tri) IB_HIP(hipMemcpyAsync(cell_tri, d_tri, (uint64_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
+ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, const float *tri_material,
+                             const ib_grid *grid, float *base_color, float *metallic, float *roughness) {
+  (void)c;
+  (void)xyz;
+  (void)ntri;
+  (void)tri_material;
+  (void)grid;
+  (void)base_color;
+  (void)metallic;
+  (void)roughness;
+  return fail(IB_ERR_UNSUPPORTED, "material splat is not built yet");
+}
+
+ib_status ib_device_alloc(ib_context *c, uint64_t bytes, void **p) {
+  if (!c || !p) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  IB_TRY(c->alloc_raw(bytes, p));
+  c->user[*p] = bytes;
+  return IB_OK;
+}
+
+ib_status ib_device_free(ib_context *c, void *p) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  auto it = c->user.find(p);
+  if (it == c->user.end()) return fail(IB_ERR_INVALID_ARGUMENT, "pointer was not allocated by ib_device_alloc");
+  c->free_raw(p, it->second);
+  c->user.erase(it);
+  return IB_OK;
+}
+
+ib_status ib_memcpy_h2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_HIP(hipSetDevice(c->device));
+  IB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return IB_OK;
+}
+
+ib_status ib_memcpy_d2h(ib_context *c, void *dst, const void *src, uint64_t bytes) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_HIP(hipSetDevice(c->device));
+  IB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return IB_OK;
+}
+
+ib_status ib_synchronize(ib_context *c) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_HIP(hipSetDevice(c->device));
+  IB_HIP(hipStreamSynchronize(c->stream));
+  IB_HIP(hipDeviceSynchronize());
+  return IB_OK;
+}
+
+ib_status ib_last_stats(ib_context *c, ib_stats *out) {
+  if (!c || !out) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  std::lock_guard<std::mutex> lk(c->mu);
+  for (int e = EV_COUNT - 1; e >= 0; --e)
+    if (c->ev_valid[e]) {
+      IB_HIP(hipEventSynchronize(c->ev[e]));
+      break;
+    }
+  ib_stats st = c->stats;
+  st.ms_ingest = ev_ms(c, EV_START, EV_INGEST);
+  st.ms_bvh = ev_ms(c, EV_INGEST, EV_BVH);
+  st.ms_occupancy = ev_ms(c, EV_BVH, EV_OCC);
+  st.ms_columns = ev_ms(c, EV_OCC, EV_COLS);
+  st.ms_sdf = ev_ms(c, EV_COLS, EV_SDF);
+  st.ms_total = c->ev_valid[EV_SDF] ? ev_ms(c, EV_START, EV_SDF) : ev_ms(c, EV_START, EV_OCC);
+  // device-side counters of the last bake
+  Scalars *sc = nullptr;
+  auto it = c->bufs.find("sc");
+  if (it != c->bufs.end()) {
+    sc = static_cast<Scalars *>(it->second.p);
+    Scalars h;
+    IB_HIP(hipMemcpy(&h, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
+    st.column_overflows = h.col_overflow;
+    st.rechecked = h.rechecked;
+    st.ties = h.ties;
+  }
+  *out = st;
+  return IB_OK;
+}
+
+}  // extern "C"

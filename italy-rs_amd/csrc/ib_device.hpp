@@ -1,0 +1,323 @@
+// Device-side geometry for the bake kernels (gfx950).  Every translation unit that includes this
+// is compiled with -ffp-contract=off, so each a*b+c below is two correctly rounded operations and
+// the triangle/box test is bit-identical to the host restatement of tri_box_overlap.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ib {
+
+constexpr int kStack = 64;        // traversal stack entries (>= LBVH depth, checked at build)
+constexpr int kColCap = 32;       // crossings stored per +z column before the per-voxel fallback
+constexpr uint32_t kSliver = 1u;  // triangle flag: distance evaluated in FP64
+
+// Internal LBVH node: both children's boxes inline so one 64-byte record decides both children.
+// child >= 0: internal node index, child < 0: ~leaf (sorted triangle position).
+struct alignas(16) Node {
+  float lo0[3], hi0[3];
+  float lo1[3], hi1[3];
+  int32_t c0, c1, parent, pad;
+};
+static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
+
+// ---------------------------------------------------------------------------------------------
+// triangle / AABB separating axis test: same expression trees as src/convert/tri_box_overlap.h
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+struct F3 {
+  float x, y, z;
+};
+
+__device__ __forceinline__ bool sat_axis(float p0, float p1, float rad) {
+  const float mn = smin(p0, p1), mx = smax(p0, p1);
+  return !(mn > rad || mx < -rad);
+}
+
+// box centre c, half size bh (all components equal here), triangle t0,t1,t2
+__device__ __forceinline__ bool tri_box(F3 c, float bh, F3 t0, F3 t1, F3 t2) {
+  const F3 v0 = {t0.x - c.x, t0.y - c.y, t0.z - c.z};
+  const F3 v1 = {t1.x - c.x, t1.y - c.y, t1.z - c.z};
+  const F3 v2 = {t2.x - c.x, t2.y - c.y, t2.z - c.z};
+  const F3 e0 = {v1.x - v0.x, v1.y - v0.y, v1.z - v0.z};
+  const F3 e1 = {v2.x - v1.x, v2.y - v1.y, v2.z - v1.z};
+  const F3 e2 = {v0.x - v2.x, v0.y - v2.y, v0.z - v2.z};
+  float fex = fabsf(e0.x), fey = fabsf(e0.y), fez = fabsf(e0.z);
+  // X01(a=e.z, b=e.y): p = a*v.y - b*v.z over v0,v2 ; rad = fa*h + fb*h
+  if (!sat_axis(e0.z * v0.y - e0.y * v0.z, e0.z * v2.y - e0.y * v2.z, fez * bh + fey * bh)) return false;
+  // Y02(a=e.z, b=e.x): p = -a*v.x + b*v.z over v0,v2
+  if (!sat_axis(-e0.z * v0.x + e0.x * v0.z, -e0.z * v2.x + e0.x * v2.z, fez * bh + fex * bh)) return false;
+  // Z12(a=e.y, b=e.x): p = a*v.x - b*v.y over v1,v2
+  if (!sat_axis(e0.y * v1.x - e0.x * v1.y, e0.y * v2.x - e0.x * v2.y, fey * bh + fex * bh)) return false;
+  fex = fabsf(e1.x);
+  fey = fabsf(e1.y);
+  fez = fabsf(e1.z);
+  if (!sat_axis(e1.z * v0.y - e1.y * v0.z, e1.z * v2.y - e1.y * v2.z, fez * bh + fey * bh)) return false;
+  if (!sat_axis(-e1.z * v0.x + e1.x * v0.z, -e1.z * v2.x + e1.x * v2.z, fez * bh + fex * bh)) return false;
+  // Z0 over v0,v1
+  if (!sat_axis(e1.y * v0.x - e1.x * v0.y, e1.y * v1.x - e1.x * v1.y, fey * bh + fex * bh)) return false;
+  fex = fabsf(e2.x);
+  fey = fabsf(e2.y);
+  fez = fabsf(e2.z);
+  // X2 over v0,v1
+  if (!sat_axis(e2.z * v0.y - e2.y * v0.z, e2.z * v1.y - e2.y * v1.z, fez * bh + fey * bh)) return false;
+  // Y1 over v0,v1
+  if (!sat_axis(-e2.z * v0.x + e2.x * v0.z, -e2.z * v1.x + e2.x * v1.z, fez * bh + fex * bh)) return false;
+  // Z12 over v1,v2
+  if (!sat_axis(e2.y * v1.x - e2.x * v1.y, e2.y * v2.x - e2.x * v2.y, fey * bh + fex * bh)) return false;
+  {
+    const float mn = smin(smin(v0.x, v1.x), v2.x), mx = smax(smax(v0.x, v1.x), v2.x);
+    if (mn > bh || mx < -bh) return false;
+  }
+  {
+    const float mn = smin(smin(v0.y, v1.y), v2.y), mx = smax(smax(v0.y, v1.y), v2.y);
+    if (mn > bh || mx < -bh) return false;
+  }
+  {
+    const float mn = smin(smin(v0.z, v1.z), v2.z), mx = smax(smax(v0.z, v1.z), v2.z);
+    if (mn > bh || mx < -bh) return false;
+  }
+  // plane test with n = cross(e0, e1) (glm order) and dot = (x + y) + z
+  const F3 n = {e0.y * e1.z - e1.y * e0.z, e0.z * e1.x - e1.z * e0.x, e0.x * e1.y - e1.x * e0.y};
+  F3 vmin, vmax;
+  if (n.x > 0.0f) { vmin.x = -bh - v0.x; vmax.x = bh - v0.x; } else { vmin.x = bh - v0.x; vmax.x = -bh - v0.x; }
+  if (n.y > 0.0f) { vmin.y = -bh - v0.y; vmax.y = bh - v0.y; } else { vmin.y = bh - v0.y; vmax.y = -bh - v0.y; }
+  if (n.z > 0.0f) { vmin.z = -bh - v0.z; vmax.z = bh - v0.z; } else { vmin.z = bh - v0.z; vmax.z = -bh - v0.z; }
+  {
+    const float tx = n.x * vmin.x, ty = n.y * vmin.y, tz = n.z * vmin.z;
+    if ((tx + ty) + tz > 0.0f) return false;
+  }
+  const float tx = n.x * vmax.x, ty = n.y * vmax.y, tz = n.z * vmax.z;
+  return (tx + ty) + tz >= 0.0f;
+}
+
+// ---------------------------------------------------------------------------------------------
+// closest point distance
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+__device__ __forceinline__ float box_d2(F3 q, const float *lo, const float *hi) {
+  float s = 0.0f;
+  float d = 0.0f;
+  if (q.x < lo[0]) d = lo[0] - q.x; else if (q.x > hi[0]) d = q.x - hi[0];
+  s += d * d;
+  d = 0.0f;
+  if (q.y < lo[1]) d = lo[1] - q.y; else if (q.y > hi[1]) d = q.y - hi[1];
+  s += d * d;
+  d = 0.0f;
+  if (q.z < lo[2]) d = lo[2] - q.z; else if (q.z > hi[2]) d = q.z - hi[2];
+  s += d * d;
+  return s;
+}
+
+__device__ __forceinline__ double dot3d(double ax, double ay, double az, double bx, double by, double bz) {
+  return ax * bx + ay * by + az * bz;
+}
+
+__device__ __forceinline__ double seg_d2(double ax, double ay, double az, double bx, double by, double bz) {
+  const double ex = bx - ax, ey = by - ay, ez = bz - az;
+  const double l = dot3d(ex, ey, ez, ex, ey, ez);
+  double t = 0.0;
+  if (l > 0.0) {
+    t = -dot3d(ax, ay, az, ex, ey, ez) / l;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  }
+  const double rx = ax + t * ex, ry = ay + t * ey, rz = az + t * ez;
+  return dot3d(rx, ry, rz, rx, ry, rz);
+}
+
+// FP64 squared distance (degenerate-safe): min over the three edges and the interior projection
+__device__ __noinline__ double tri_d2_f64(F3 q, F3 p0, F3 p1, F3 p2) {
+  const double ax = (double)p0.x - q.x, ay = (double)p0.y - q.y, az = (double)p0.z - q.z;
+  const double bx = (double)p1.x - q.x, by = (double)p1.y - q.y, bz = (double)p1.z - q.z;
+  const double cx = (double)p2.x - q.x, cy = (double)p2.y - q.y, cz = (double)p2.z - q.z;
+  double best = seg_d2(ax, ay, az, bx, by, bz);
+  best = fmin(best, seg_d2(bx, by, bz, cx, cy, cz));
+  best = fmin(best, seg_d2(cx, cy, cz, ax, ay, az));
+  const double abx = bx - ax, aby = by - ay, abz = bz - az;
+  const double acx = cx - ax, acy = cy - ay, acz = cz - az;
+  const double nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
+  const double nn = dot3d(nx, ny, nz, nx, ny, nz);
+  if (nn > 0.0) {
+    const double s = dot3d(nx, ny, nz, ax, ay, az) / nn;
+    const double px = s * nx, py = s * ny, pz = s * nz;
+    const double pax = ax - px, pay = ay - py, paz = az - pz;
+    const double pbx = bx - px, pby = by - py, pbz = bz - pz;
+    const double pcx = cx - px, pcy = cy - py, pcz = cz - pz;
+    const double u = dot3d(pay * pbz - paz * pby, paz * pbx - pax * pbz, pax * pby - pay * pbx, nx, ny, nz);
+    const double v = dot3d(pby * pcz - pbz * pcy, pbz * pcx - pbx * pcz, pbx * pcy - pby * pcx, nx, ny, nz);
+    const double w = dot3d(pcy * paz - pcz * pay, pcz * pax - pcx * paz, pcx * pay - pcy * pax, nx, ny, nz);
+    if (u >= 0.0 && v >= 0.0 && w >= 0.0) {
+      const double s2 = dot3d(nx, ny, nz, ax, ay, az);
+      best = fmin(best, s2 * s2 / nn);
+    }
+  }
+  return best;
+}
+
+// FP32 squared distance in coordinates relative to q: Voronoi regions after Ericson (2005) 5.1.5,
+// the face region by plane distance (n.a)^2 / |n|^2.
+__device__ __forceinline__ float tri_d2_f32(F3 q, F3 p0, F3 p1, F3 p2) {
+  const F3 a = {p0.x - q.x, p0.y - q.y, p0.z - q.z};
+  const F3 b = {p1.x - q.x, p1.y - q.y, p1.z - q.z};
+  const F3 c = {p2.x - q.x, p2.y - q.y, p2.z - q.z};
+  const F3 ab = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+  const F3 ac = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+  const F3 ap = {-a.x, -a.y, -a.z};
+  const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return dot3(a, a);
+  const F3 bp = {-b.x, -b.y, -b.z};
+  const float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return dot3(b, b);
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float v = d1 / (d1 - d3);
+    const F3 r = {a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z};
+    return dot3(r, r);
+  }
+  const F3 cp = {-c.x, -c.y, -c.z};
+  const float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return dot3(c, c);
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+    const F3 r = {a.x + w * ac.x, a.y + w * ac.y, a.z + w * ac.z};
+    return dot3(r, r);
+  }
+  const float va = d3 * d6 - d5 * d4;
+  const float e43 = d4 - d3, e56 = d5 - d6;
+  if (va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f) {
+    const float w = e43 / (e43 + e56);
+    const F3 bc = {p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
+    const F3 r = {b.x + w * bc.x, b.y + w * bc.y, b.z + w * bc.z};
+    return dot3(r, r);
+  }
+  const F3 n = {ab.y * ac.z - ab.z * ac.y, ab.z * ac.x - ab.x * ac.z, ab.x * ac.y - ab.y * ac.x};
+  const float s = dot3(n, a);
+  return (s * s) / dot3(n, n);
+}
+
+// triangle record: 3 x float4 = (v0, flags), (v1, original index), (v2, 0)
+__device__ __forceinline__ float tri_d2(F3 q, const float4 *t) {
+  const float4 a = t[0], b = t[1], c = t[2];
+  const F3 p0 = {a.x, a.y, a.z}, p1 = {b.x, b.y, b.z}, p2 = {c.x, c.y, c.z};
+  if (__float_as_uint(a.w) & kSliver) return (float)tri_d2_f64(q, p0, p1, p2);
+  return tri_d2_f32(q, p0, p1, p2);
+}
+
+// ---------------------------------------------------------------------------------------------
+// watertight crossings (Woop, Benthin, Wald 2013) with exact FP64 edge signs and a half-open tie
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ int tie_pos(float dx, float dy) { return (dy > 0.0f || (dy == 0.0f && dx > 0.0f)) ? 1 : -1; }
+
+// sign of px*qy - py*qx (exact: products of floats are exact in FP64)
+__device__ __forceinline__ int edge_sign(float px, float py, float qx, float qy, double &e) {
+  e = (double)px * (double)qy - (double)py * (double)qx;
+  return e > 0.0 ? 1 : (e < 0.0 ? -1 : 0);
+}
+
+// 2D crossing test of a +z line at (X,Y) with triangle p0,p1,p2.  Returns 0 (miss) or the winding
+// contribution (+1 where the triangle normal has +z component, i.e. the line leaves the solid);
+// *zhit receives the z of the crossing.
+__device__ __forceinline__ int column_cross(float X, float Y, F3 p0, F3 p1, F3 p2, float *zhit) {
+  const float ax = p0.x - X, ay = p0.y - Y;
+  const float bx = p1.x - X, by = p1.y - Y;
+  const float cx = p2.x - X, cy = p2.y - Y;
+  double U, V, W;
+  int su = edge_sign(cx, cy, bx, by, U);  // edge B->C
+  int sv = edge_sign(ax, ay, cx, cy, V);  // edge C->A
+  int sw = edge_sign(bx, by, ax, ay, W);  // edge A->B
+  if (su == 0) su = tie_pos(cx - bx, cy - by);
+  if (sv == 0) sv = tie_pos(ax - cx, ay - cy);
+  if (sw == 0) sw = tie_pos(bx - ax, by - ay);
+  if (su != sv || sv != sw) return 0;
+  const double det = U + V + W;
+  if (det == 0.0) return 0;
+  *zhit = (float)((U * (double)p0.z + V * (double)p1.z + W * (double)p2.z) / det);
+  return -su;
+}
+
+// general-direction watertight ray test for the parity tie-break: 1 if hit at t > 0
+struct Shear {
+  int kx, ky, kz;
+  float sx, sy, sz;
+};
+
+__device__ __forceinline__ Shear make_shear(float dx, float dy, float dz) {
+  const float d[3] = {dx, dy, dz};
+  int kz = 0;
+  if (fabsf(d[1]) > fabsf(d[kz])) kz = 1;
+  if (fabsf(d[2]) > fabsf(d[kz])) kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  if (d[kz] < 0.0f) {
+    const int t = kx;
+    kx = ky;
+    ky = t;
+  }
+  Shear s;
+  s.kx = kx;
+  s.ky = ky;
+  s.kz = kz;
+  s.sx = d[kx] / d[kz];
+  s.sy = d[ky] / d[kz];
+  s.sz = 1.0f / d[kz];
+  return s;
+}
+
+__device__ __forceinline__ float comp(F3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ bool ray_hit(F3 o, const Shear &r, F3 p0, F3 p1, F3 p2) {
+  const F3 A = {p0.x - o.x, p0.y - o.y, p0.z - o.z};
+  const F3 B = {p1.x - o.x, p1.y - o.y, p1.z - o.z};
+  const F3 C = {p2.x - o.x, p2.y - o.y, p2.z - o.z};
+  const float ax = comp(A, r.kx) - r.sx * comp(A, r.kz), ay = comp(A, r.ky) - r.sy * comp(A, r.kz);
+  const float bx = comp(B, r.kx) - r.sx * comp(B, r.kz), by = comp(B, r.ky) - r.sy * comp(B, r.kz);
+  const float cx = comp(C, r.kx) - r.sx * comp(C, r.kz), cy = comp(C, r.ky) - r.sy * comp(C, r.kz);
+  double U, V, W;
+  int su = edge_sign(cx, cy, bx, by, U);
+  int sv = edge_sign(ax, ay, cx, cy, V);
+  int sw = edge_sign(bx, by, ax, ay, W);
+  if (su == 0) su = tie_pos(cx - bx, cy - by);
+  if (sv == 0) sv = tie_pos(ax - cx, ay - cy);
+  if (sw == 0) sw = tie_pos(bx - ax, by - ay);
+  if (su != sv || sv != sw) return false;
+  const double det = U + V + W;
+  if (det == 0.0) return false;
+  const double az = (double)r.sz * comp(A, r.kz), bz = (double)r.sz * comp(B, r.kz), cz = (double)r.sz * comp(C, r.kz);
+  const double t = (U * az + V * bz + W * cz) / det;
+  return t > 0.0;
+}
+
+// the five parity directions: +z first, then four fixed generic unit vectors
+__device__ __constant__ const float kParityDirs[15] = {
+    0.0f,          0.0f,          1.0f,          0.5773502588f,  0.5773502588f,
+    -0.5773502588f, -0.6666666865f, 0.3333333433f, 0.6666666865f,  0.2672612369f,
+    -0.8017837405f, -0.5345224738f, -0.4850712717f, -0.7276068926f, 0.4850712717f};
+
+// ordered-int encoding of floats for atomic min/max
+__device__ __forceinline__ uint32_t f2o(float f) {
+  const uint32_t u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float o2f(uint32_t o) {
+  const uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  union {
+    uint32_t u;
+    float f;
+  } c;
+  c.u = u;
+  return c.f;
+}
+
+__device__ __forceinline__ uint32_t expand10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+}  // namespace ib

@@ -1,0 +1,765 @@
+// Bake kernels for gfx950: mesh ingest (weld + closedness), LBVH build (Karras 2012), surface
+// occupancy, +z column crossings and the per-voxel closest-triangle + winding-number query.
+#include <cstring>
+
+#include "ib_kernels.hpp"
+
+#include <rocprim/rocprim.hpp>
+
+namespace ib {
+
+// ============================================================================================
+// small utilities
+// ============================================================================================
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+__device__ __forceinline__ uint32_t canon(float f) { return f == 0.0f ? 0u : __float_as_uint(f); }
+
+__device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
+  uint64_t x = ((uint64_t)a << 32 | b) ^ ((uint64_t)c * 0x9E3779B97F4A7C15ULL);
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return (uint32_t)x;
+}
+
+__device__ __forceinline__ uint32_t hash64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return (uint32_t)x;
+}
+
+// ============================================================================================
+// ingest: finite check + centroid bounds (block reduction, then 6 ordered-int atomics)
+// ============================================================================================
+__global__ __launch_bounds__(256) void k_prep(const float *__restrict__ xyz, int n, Scalars *sc) {
+  __shared__ uint32_t red[6][256];
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+  if (i < n) {
+    const float *t = xyz + 9 * (size_t)i;
+    bool finite = true;
+    for (int k = 0; k < 9; ++k) finite &= isfinite(t[k]);
+    if (!finite) atomicOr(&sc->flags, kFlagNonFinite);
+    for (int k = 0; k < 3; ++k) {
+      const float c = ((t[k] + t[3 + k]) + t[6 + k]) / 3.0f;
+      mn[k] = f2o(c);
+      mx[k] = f2o(c);
+    }
+  }
+  for (int k = 0; k < 3; ++k) {
+    red[k][threadIdx.x] = mn[k];
+    red[3 + k][threadIdx.x] = mx[k];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int k = 0; k < 3; ++k) {
+        red[k][threadIdx.x] = min(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        red[3 + k][threadIdx.x] = max(red[3 + k][threadIdx.x], red[3 + k][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&sc->cmin[k], red[k][0]);
+      atomicMax(&sc->cmax[k], red[3 + k][0]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ xyz, int n, const Scalars *sc,
+                                                uint32_t *codes, int32_t *idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float *t = xyz + 9 * (size_t)i;
+  uint32_t q[3];
+  for (int k = 0; k < 3; ++k) {
+    const float lo = o2f(sc->cmin[k]), hi = o2f(sc->cmax[k]);
+    const float ext = hi - lo;
+    const float c = ((t[k] + t[3 + k]) + t[6 + k]) / 3.0f;
+    const float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
+    float s = u * 1024.0f;
+    s = s < 0.0f ? 0.0f : (s > 1023.0f ? 1023.0f : s);
+    q[k] = (uint32_t)s;
+  }
+  codes[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  idx[i] = i;
+}
+
+// ============================================================================================
+// LBVH topology (Karras 2012, "Maximizing parallelism in the construction of BVHs, octrees and
+// k-d trees"): keys are (code << 32 | sorted position) so equal codes still split uniquely.
+// ============================================================================================
+__device__ __forceinline__ int kdelta(const uint32_t *codes, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint64_t a = ((uint64_t)codes[i] << 32) | (uint32_t)i;
+  const uint64_t b = ((uint64_t)codes[j] << 32) | (uint32_t)j;
+  return __clzll((long long)(a ^ b));
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ codes, int n, Node *nodes,
+                                                int32_t *leaf_parent) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = kdelta(codes, n, i, i - d);
+  int lmax = 2;
+  while (kdelta(codes, n, i, i + lmax * d) > dmin) lmax *= 2;
+  int l = 0;
+  for (int t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(codes, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = kdelta(codes, n, i, j);
+  int s = 0, t = l;
+  do {
+    t = (t + 1) >> 1;
+    if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
+  } while (t > 1);
+  const int gamma = i + s * d + (d < 0 ? d : 0);
+  const int lo = min(i, j), hi = max(i, j);
+  const int lc = (lo == gamma) ? ~gamma : gamma;
+  const int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  nodes[i].c0 = lc;
+  nodes[i].c1 = rc;
+  if (lc >= 0) nodes[lc].parent = i; else leaf_parent[~lc] = i;
+  if (rc >= 0) nodes[rc].parent = i; else leaf_parent[~rc] = i;
+  if (i == 0) nodes[0].parent = -1;
+}
+
+// sorted triangle records (3 x float4) with the sliver flag
+__global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, const int32_t *__restrict__ order,
+                                                int n, float4 *tris) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int o = order[i];
+  const float *t = xyz + 9 * (size_t)o;
+  const float abx = t[3] - t[0], aby = t[4] - t[1], abz = t[5] - t[2];
+  const float acx = t[6] - t[0], acy = t[7] - t[1], acz = t[8] - t[2];
+  const float nx = aby * acz - abz * acy, ny = abz * acx - abx * acz, nz = abx * acy - aby * acx;
+  const float nn = nx * nx + ny * ny + nz * nz;
+  const float l1 = abx * abx + aby * aby + abz * abz, l2 = acx * acx + acy * acy + acz * acz;
+  const uint32_t flags = (nn > 1e-4f * (l1 * l2)) ? 0u : kSliver;
+  tris[3 * i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(flags));
+  tris[3 * i + 1] = make_float4(t[3], t[4], t[5], __int_as_float(o));
+  tris[3 * i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+}
+
+// bottom-up refit: each leaf writes its box into its parent's child slot; the second thread to
+// reach a node unions the two child boxes and continues upwards.
+__global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, int n, Node *nodes,
+                                               const int32_t *__restrict__ leaf_parent, uint32_t *visits,
+                                               float *root_box) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+  float lo[3] = {smin(smin(a.x, b.x), c.x), smin(smin(a.y, b.y), c.y), smin(smin(a.z, b.z), c.z)};
+  float hi[3] = {smax(smax(a.x, b.x), c.x), smax(smax(a.y, b.y), c.y), smax(smax(a.z, b.z), c.z)};
+  int child = ~i;
+  int p = leaf_parent[i];
+  for (int guard = 0; p >= 0 && guard < kMaxDepth; ++guard) {
+    Node *nd = nodes + p;
+    const bool left = (nd->c0 == child);
+    float *dlo = left ? nd->lo0 : nd->lo1;
+    float *dhi = left ? nd->hi0 : nd->hi1;
+    for (int k = 0; k < 3; ++k) {
+      __hip_atomic_store(&dlo[k], lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&dhi[k], hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __threadfence();
+    const uint32_t old = atomicAdd(&visits[p], 1u);
+    if (old == 0) return;  // the sibling is still on its way; it finishes this node
+    __threadfence();
+    for (int k = 0; k < 3; ++k) {
+      const float l0 = __hip_atomic_load(&nd->lo0[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const float l1 = __hip_atomic_load(&nd->lo1[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const float h0 = __hip_atomic_load(&nd->hi0[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const float h1 = __hip_atomic_load(&nd->hi1[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      lo[k] = smin(l0, l1);
+      hi[k] = smax(h0, h1);
+    }
+    child = p;
+    p = nd->parent;
+  }
+  for (int k = 0; k < 3; ++k) {
+    root_box[k] = lo[k];
+    root_box[3 + k] = hi[k];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_depth(const Node *__restrict__ nodes, const int32_t *__restrict__ leaf_parent,
+                                               int n, Scalars *sc) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  int d = 0;
+  int p = leaf_parent[i];
+  while (p >= 0 && d < kMaxDepth) {
+    ++d;
+    p = nodes[p].parent;
+  }
+  atomicMax(&sc->depth, (uint32_t)(p >= 0 ? kMaxDepth + 1 : d));
+}
+
+// one triangle: a single node whose second child is an empty box
+__global__ void k_single(const float4 *__restrict__ tris, Node *nodes, float *root_box) {
+  const float4 a = tris[0], b = tris[1], c = tris[2];
+  Node nd;
+  nd.lo0[0] = smin(smin(a.x, b.x), c.x);
+  nd.lo0[1] = smin(smin(a.y, b.y), c.y);
+  nd.lo0[2] = smin(smin(a.z, b.z), c.z);
+  nd.hi0[0] = smax(smax(a.x, b.x), c.x);
+  nd.hi0[1] = smax(smax(a.y, b.y), c.y);
+  nd.hi0[2] = smax(smax(a.z, b.z), c.z);
+  for (int k = 0; k < 3; ++k) {
+    nd.lo1[k] = INFINITY;
+    nd.hi1[k] = -INFINITY;
+    root_box[k] = nd.lo0[k];
+    root_box[3 + k] = nd.hi0[k];
+  }
+  nd.c0 = ~0;
+  nd.c1 = ~0;
+  nd.parent = -1;
+  nd.pad = 0;
+  nodes[0] = nd;
+}
+
+// ============================================================================================
+// weld by exact position (-0 == +0) and count unmatched directed edges
+// ============================================================================================
+__global__ __launch_bounds__(256) void k_weld_insert(const float *__restrict__ xyz, int ncorner, int32_t *slots,
+                                                     uint32_t mask, int32_t *cslot) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncorner) return;
+  const uint32_t a = canon(xyz[3 * i]), b = canon(xyz[3 * i + 1]), c = canon(xyz[3 * i + 2]);
+  uint32_t s = hash3(a, b, c) & mask;
+  for (;;) {
+    int cur = slots[s];
+    if (cur < 0) {
+      cur = atomicCAS(&slots[s], -1, i);
+      if (cur < 0) break;
+    }
+    if (canon(xyz[3 * cur]) == a && canon(xyz[3 * cur + 1]) == b && canon(xyz[3 * cur + 2]) == c) break;
+    s = (s + 1) & mask;
+  }
+  cslot[i] = (int32_t)s;
+}
+
+__global__ __launch_bounds__(256) void k_weld_min(const int32_t *__restrict__ cslot, int ncorner, int32_t *minid) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncorner) return;
+  atomicMin(&minid[cslot[i]], i);
+}
+
+__global__ __launch_bounds__(256) void k_edge_insert(const int32_t *__restrict__ cslot,
+                                                     const int32_t *__restrict__ minid, int ntri,
+                                                     unsigned long long *keys, int32_t *net, uint32_t mask) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntri) return;
+  int v[3];
+  for (int k = 0; k < 3; ++k) v[k] = minid[cslot[3 * t + k]];
+  for (int k = 0; k < 3; ++k) {
+    const int a = v[k], b = v[k == 2 ? 0 : k + 1];
+    if (a == b) continue;
+    const uint32_t lo = (uint32_t)min(a, b), hi = (uint32_t)max(a, b);
+    const unsigned long long key = ((unsigned long long)lo << 32) | hi;
+    uint32_t s = hash64(key) & mask;
+    for (;;) {
+      unsigned long long cur = keys[s];
+      if (cur == ~0ull) {
+        cur = atomicCAS(&keys[s], ~0ull, key);
+        if (cur == ~0ull) break;
+      }
+      if (cur == key) break;
+      s = (s + 1) & mask;
+    }
+    atomicAdd(&net[s], a < b ? 1 : -1);
+  }
+}
+
+// boundary edges: emit each unmatched directed edge (as corner indices = representative vertex
+// ids) with its multiplicity
+__global__ __launch_bounds__(256) void k_edge_collect(const unsigned long long *__restrict__ keys,
+                                                      const int32_t *__restrict__ net, uint32_t cap, Scalars *sc,
+                                                      int2 *out, uint32_t out_cap) {
+  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
+  if (s >= cap) return;
+  const unsigned long long key = keys[s];
+  if (key == ~0ull) return;
+  const int m = net[s];
+  if (m == 0) return;
+  const int lo = (int)(key >> 32), hi = (int)(key & 0xffffffffu);
+  const int cnt = m > 0 ? m : -m;
+  const uint32_t pos = atomicAdd(&sc->boundary, (uint32_t)cnt);
+  for (int k = 0; k < cnt; ++k)
+    if (pos + k < out_cap) out[pos + k] = m > 0 ? make_int2(lo, hi) : make_int2(hi, lo);
+}
+
+// ============================================================================================
+// surface occupancy: voxelize.cpp:64-100 per triangle, bit-packed with atomicOr
+// ============================================================================================
+// inclusive cell range of a triangle's AABB: floor((min - origin) / h) clamped with
+// glm::clamp(x, 0, n - 1) = min(max(x, 0), n - 1) on every axis (voxelize.cpp:70-77)
+__device__ __forceinline__ void tri_cells(const float *p, const GridArgs &g, int lo[3], int hi[3]) {
+  const int maxc[3] = {g.nx - 1, g.ny - 1, g.nz - 1};
+  const float org[3] = {g.ox, g.oy, g.oz};
+  for (int k = 0; k < 3; ++k) {
+    const float mn = smin(smin(p[k], p[3 + k]), p[6 + k]);
+    const float mx = smax(smax(p[k], p[3 + k]), p[6 + k]);
+    int a = (int)floorf((mn - org[k]) / g.h);
+    int b = (int)floorf((mx - org[k]) / g.h);
+    a = a < 0 ? 0 : a;
+    a = a > maxc[k] ? maxc[k] : a;
+    b = b < 0 ? 0 : b;
+    b = b > maxc[k] ? maxc[k] : b;
+    lo[k] = a;
+    hi[k] = b;
+  }
+}
+
+// cell centre origin + (cell + 0.5) * h (voxelize.cpp:84), half size h * 0.5 (:85)
+__device__ __forceinline__ F3 cell_centre(const GridArgs &g, int x, int y, int z) {
+  return {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+}
+
+__global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ xyz, int n, GridArgs g,
+                                                   uint32_t *words) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float *p = xyz + 9 * (size_t)t;
+  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
+  int lo[3], hi[3];
+  tri_cells(p, g, lo, hi);
+  const float half = g.h * 0.5f;
+  const int wpr = (g.nx + 31) >> 5;
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x)
+        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2))
+          atomicOr(&words[((size_t)z * g.ny + y) * wpr + (x >> 5)], 1u << (x & 31));
+}
+
+// first touching triangle per cell (the order key of voxelize.cpp:88-96)
+__global__ __launch_bounds__(256) void k_first_touch(const float *__restrict__ xyz, int n, GridArgs g,
+                                                     int32_t *first) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float *p = xyz + 9 * (size_t)t;
+  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
+  int lo[3], hi[3];
+  tri_cells(p, g, lo, hi);
+  const float half = g.h * 0.5f;
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x)
+        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2))
+          atomicMin(&first[((size_t)z * g.ny + y) * g.nx + x], t);
+}
+
+__global__ __launch_bounds__(256) void k_first_compact(const int32_t *__restrict__ first, uint64_t ncell,
+                                                       Scalars *sc, unsigned long long *keys) {
+  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= ncell) return;
+  const int32_t t = first[i];
+  if (t == 0x7fffffff) return;
+  const uint32_t pos = atomicAdd(&sc->occ_set, 1u);
+  keys[pos] = ((unsigned long long)(uint32_t)t << 32) | (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_unpack_cells(const unsigned long long *__restrict__ keys, uint32_t n,
+                                                      int nx, int ny, int32_t *cells, int32_t *tri) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long key = keys[k];
+  const uint32_t i = (uint32_t)(key & 0xffffffffu);
+  cells[3 * k + 0] = (int32_t)(i % (uint32_t)nx);
+  cells[3 * k + 1] = (int32_t)((i / (uint32_t)nx) % (uint32_t)ny);
+  cells[3 * k + 2] = (int32_t)(i / ((uint32_t)nx * (uint32_t)ny));
+  if (tri) tri[k] = (int32_t)(key >> 32);
+}
+
+// ============================================================================================
+// +z column crossings through the mesh and (reversed) hole caps
+// ============================================================================================
+__device__ __forceinline__ F3 f3(float4 v) { return {v.x, v.y, v.z}; }
+
+struct ColumnSink {
+  float2 *dst;
+  int cnt;
+  __device__ void add(float z, int s) {
+    if (cnt < kColCap) dst[cnt] = make_float2(z, (float)s);
+    ++cnt;
+  }
+};
+
+// visits every leaf whose box contains (X,Y) in xy
+template <typename F>
+__device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes, int nnodes, float X, float Y,
+                                                  F &&leaf) {
+  int stack[kStack + 2];
+  int sp = 0;
+  int cur = 0;
+  for (int visits = 0; visits <= nnodes; ++visits) {
+    const Node nd = nodes[cur];
+    const bool in0 = X >= nd.lo0[0] && X <= nd.hi0[0] && Y >= nd.lo0[1] && Y <= nd.hi0[1];
+    const bool in1 = X >= nd.lo1[0] && X <= nd.hi1[0] && Y >= nd.lo1[1] && Y <= nd.hi1[1];
+    int next = -1;
+    if (in0) {
+      if (nd.c0 < 0) leaf(~nd.c0); else next = nd.c0;
+    }
+    if (in1) {
+      if (nd.c1 < 0) leaf(~nd.c1);
+      else if (next < 0) next = nd.c1;
+      else if (sp < kStack) stack[sp++] = nd.c1;
+    }
+    if (next >= 0) {
+      cur = next;
+      continue;
+    }
+    if (sp == 0) break;
+    cur = stack[--sp];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= a.g.nx || y >= a.g.ny) return;
+  const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
+  const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
+  const size_t col = (size_t)y * a.g.nx + x;
+  ColumnSink sink{a.cross + col * kColCap, 0};
+  for_column_leaves(a.nodes, a.nnodes, X, Y, [&](int leaf) {
+    const float4 *t = a.tris + 3 * leaf;
+    float z;
+    const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
+    if (s) sink.add(z, s);
+  });
+  for (int c = 0; c < a.ncap; ++c) {  // caps enter the closed surface reversed
+    const float *t = a.caps + 9 * c;
+    float z;
+    const int s = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
+    if (s) sink.add(z, -s);
+  }
+  a.count[col] = sink.cnt;
+  if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
+}
+
+// winding (integer part) of the closed surface mesh + reversed caps for a voxel whose column
+// overflowed the stored crossings: recount crossings above q.z
+__device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y, float qz) {
+  int w = 0;
+  for_column_leaves(a.nodes, a.nnodes, X, Y, [&](int leaf) {
+    const float4 *t = a.tris + 3 * leaf;
+    float z;
+    const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
+    if (s && z > qz) w += s;
+  });
+  for (int c = 0; c < a.ncap; ++c) {
+    const float *t = a.caps + 9 * c;
+    float z;
+    const int s = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
+    if (s && z > qz) w -= s;
+  }
+  return w;
+}
+
+// ============================================================================================
+// hole caps: solid angles (Van Oosterom & Strackee 1983)
+// ============================================================================================
+__device__ __forceinline__ float solid_angle_f32(F3 q, const float *t) {
+  const F3 a = {t[0] - q.x, t[1] - q.y, t[2] - q.z};
+  const F3 b = {t[3] - q.x, t[4] - q.y, t[5] - q.z};
+  const F3 c = {t[6] - q.x, t[7] - q.y, t[8] - q.z};
+  const float la = sqrtf(dot3(a, a)), lb = sqrtf(dot3(b, b)), lc = sqrtf(dot3(c, c));
+  const F3 bc = {b.y * c.z - b.z * c.y, b.z * c.x - b.x * c.z, b.x * c.y - b.y * c.x};
+  const float det = dot3(a, bc);
+  const float den = la * lb * lc + dot3(a, b) * lc + dot3(b, c) * la + dot3(c, a) * lb;
+  return 2.0f * atan2f(det, den);
+}
+
+__device__ __noinline__ double cap_winding_f64(F3 q, const float *caps, int ncap) {
+  double sum = 0.0, comp = 0.0;
+  for (int i = 0; i < ncap; ++i) {
+    const float *t = caps + 9 * i;
+    const double ax = (double)t[0] - q.x, ay = (double)t[1] - q.y, az = (double)t[2] - q.z;
+    const double bx = (double)t[3] - q.x, by = (double)t[4] - q.y, bz = (double)t[5] - q.z;
+    const double cx = (double)t[6] - q.x, cy = (double)t[7] - q.y, cz = (double)t[8] - q.z;
+    const double la = sqrt(dot3d(ax, ay, az, ax, ay, az)), lb = sqrt(dot3d(bx, by, bz, bx, by, bz)),
+                 lc = sqrt(dot3d(cx, cy, cz, cx, cy, cz));
+    const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+    const double den = la * lb * lc + dot3d(ax, ay, az, bx, by, bz) * lc + dot3d(bx, by, bz, cx, cy, cz) * la +
+                       dot3d(cx, cy, cz, ax, ay, az) * lb;
+    const double om = 2.0 * atan2(det, den);
+    const double s = sum + om;
+    comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
+    sum = s;
+  }
+  return (sum + comp) * (1.0 / (4.0 * M_PI));
+}
+
+// conservative ray / box slab test for the parity tie-break (boxes inflated by a relative margin)
+__device__ __forceinline__ bool ray_box(F3 o, F3 d, const float *lo, const float *hi) {
+  float t0 = 0.0f, t1 = INFINITY;
+  const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+  for (int k = 0; k < 3; ++k) {
+    const float m = 1e-5f * (fabsf(lo[k]) + fabsf(hi[k])) + 1e-30f;
+    const float l = lo[k] - m, h = hi[k] + m;
+    if (dd[k] == 0.0f) {
+      if (oo[k] < l || oo[k] > h) return false;
+      continue;
+    }
+    float a = (l - oo[k]) / dd[k], b = (h - oo[k]) / dd[k];
+    if (a > b) {
+      const float t = a;
+      a = b;
+      b = t;
+    }
+    t0 = fmaxf(t0, a * (1.0f - 1e-5f));
+    t1 = fminf(t1, b * (1.0f + 1e-5f));
+  }
+  return t0 <= t1;
+}
+
+__device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
+                                           F3 q) {
+  int votes = 0;
+  for (int v = 0; v < 5; ++v) {
+    const F3 d = {kParityDirs[3 * v], kParityDirs[3 * v + 1], kParityDirs[3 * v + 2]};
+    const Shear sh = make_shear(d.x, d.y, d.z);
+    int hits = 0;
+    int stack[kStack + 2];
+    int sp = 0, cur = 0;
+    for (int visits = 0; visits <= nnodes; ++visits) {
+      const Node nd = nodes[cur];
+      int next = -1;
+      if (ray_box(q, d, nd.lo0, nd.hi0)) {
+        if (nd.c0 < 0) {
+          const float4 *t = tris + 3 * (~nd.c0);
+          hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+        } else {
+          next = nd.c0;
+        }
+      }
+      if (ray_box(q, d, nd.lo1, nd.hi1)) {
+        if (nd.c1 < 0) {
+          if (nd.c1 != nd.c0) {
+            const float4 *t = tris + 3 * (~nd.c1);
+            hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+          }
+        } else if (next < 0) {
+          next = nd.c1;
+        } else if (sp < kStack) {
+          stack[sp++] = nd.c1;
+        }
+      }
+      if (next >= 0) {
+        cur = next;
+        continue;
+      }
+      if (sp == 0) break;
+      cur = stack[--sp];
+    }
+    if (hits & 1) ++votes;
+  }
+  return votes * 2 > 5;
+}
+
+// ============================================================================================
+// per-voxel closest triangle + sign.  One wave = one 4x4x4 brick of voxels; a block of 4 waves
+// covers an 8x8x4 region.
+// ============================================================================================
+template <bool DEEP>
+__device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
+                                            F3 q, int *gstack, float *gstackd, int cap) {
+  float best = INFINITY;
+  int lstack[DEEP ? 1 : kStack];
+  float lstackd[DEEP ? 1 : kStack];
+  int *st = DEEP ? gstack : lstack;
+  float *sd = DEEP ? gstackd : lstackd;
+  int sp = 0;
+  int cur = 0;
+  for (int visits = 0; visits <= nnodes; ++visits) {
+    const Node nd = nodes[cur];
+    const float d0 = box_d2(q, nd.lo0, nd.hi0);
+    const float d1 = box_d2(q, nd.lo1, nd.hi1);
+    if (nd.c0 < 0 && d0 <= best) best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
+    if (nd.c1 < 0 && d1 <= best) best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+    const bool g0 = nd.c0 >= 0 && d0 <= best;
+    const bool g1 = nd.c1 >= 0 && d1 <= best;
+    if (g0 && g1) {
+      const bool first0 = d0 <= d1;
+      if (sp < cap) {
+        st[sp] = first0 ? nd.c1 : nd.c0;
+        sd[sp] = first0 ? d1 : d0;
+        ++sp;
+      }
+      cur = first0 ? nd.c0 : nd.c1;
+      continue;
+    }
+    if (g0) {
+      cur = nd.c0;
+      continue;
+    }
+    if (g1) {
+      cur = nd.c1;
+      continue;
+    }
+    cur = -1;
+    while (sp > 0) {
+      --sp;
+      if (sd[sp] <= best) {
+        cur = st[sp];
+        break;
+      }
+    }
+    if (cur < 0) break;
+  }
+  return best;
+}
+
+template <bool DEEP>
+__global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int x = blockIdx.x * 8 + (wv & 1) * 4 + (lane & 3);
+  const int y = blockIdx.y * 8 + (wv >> 1) * 4 + ((lane >> 2) & 3);
+  const int z = blockIdx.z * 4 + (lane >> 4);
+  const GridArgs &g = a.col.g;
+  if (x >= g.nx || y >= g.ny || z >= g.nz) return;
+  const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+  const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
+  int *gs = nullptr;
+  float *gsd = nullptr;
+  if (DEEP) {
+    const size_t tid = (((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
+    gs = a.gstack + tid * a.gstack_depth;
+    gsd = a.gstackd + tid * a.gstack_depth;
+  }
+  const float best = closest_d2<DEEP>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd, DEEP ? a.gstack_depth : kStack);
+  // integer winding of the closed surface (mesh + reversed caps) from the crossings above q
+  const size_t col = (size_t)y * g.nx + x;
+  const int cnt = a.col.count[col];
+  int wi = 0;
+  if (cnt <= kColCap) {
+    const float2 *cr = a.col.cross + col * kColCap;
+    for (int k = 0; k < cnt; ++k) {
+      const float2 c = cr[k];
+      if (c.x > q.z) wi += (int)c.y;
+    }
+  } else {
+    wi = column_direct(a.col, q.x, q.y, q.z);
+  }
+  bool inside;
+  if (a.col.ncap == 0) {
+    inside = wi != 0;
+  } else {
+    // w = wi + w_caps; FP32 Neumaier sum, FP64 recheck inside the 0.05 band, parity for ties
+    float sum = 0.0f, cmp = 0.0f;
+    for (int c = 0; c < a.col.ncap; ++c) {
+      const float om = solid_angle_f32(q, a.col.caps + 9 * c);
+      const float s = sum + om;
+      cmp += (fabsf(sum) >= fabsf(om)) ? (sum - s) + om : (om - s) + sum;
+      sum = s;
+    }
+    const float w32 = (float)wi + (sum + cmp) * (float)(1.0 / (4.0 * M_PI));
+    const float m32 = fabsf(fabsf(w32) - 0.5f);
+    if (m32 >= 0.05f) {
+      inside = fabsf(w32) > 0.5f;
+    } else {
+      atomicAdd(&a.col.sc->rechecked, 1u);
+      const double w64 = (double)wi + cap_winding_f64(q, a.col.caps, a.col.ncap);
+      if (fabs(fabs(w64) - 0.5) >= kTieEps) {
+        inside = fabs(w64) > 0.5;
+      } else {
+        atomicAdd(&a.col.sc->ties, 1u);
+        inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q);
+      }
+    }
+  }
+  const float d = sqrtf(best);
+  a.out[vox] = inside ? -d : d;
+}
+
+template __global__ void k_sdf<false>(SdfArgs);
+template __global__ void k_sdf<true>(SdfArgs);
+
+// ============================================================================================
+// host-side launchers
+// ============================================================================================
+static inline unsigned nb(uint64_t n, unsigned b = 256) { return (unsigned)((n + b - 1) / b); }
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+  if (!n) return;
+  const unsigned blocks = (unsigned)std::min<uint64_t>(nb(n), 65535u * 4);
+  k_fill_u32<<<blocks, 256, 0, s>>>(p, v, n);
+}
+
+void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s) { k_prep<<<nb(n), 256, 0, s>>>(xyz, n, sc); }
+void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, int32_t *idx, hipStream_t s) {
+  k_morton<<<nb(n), 256, 0, s>>>(xyz, n, sc, codes, idx);
+}
+
+hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                      int32_t *vout, int n, hipStream_t s) {
+  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, (unsigned)n, 0u, 30u, s);
+}
+
+void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s) {
+  if (n > 1) k_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, nodes, leaf_parent);
+}
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, hipStream_t s) {
+  k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris);
+}
+void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
+                  float *root_box, Scalars *sc, hipStream_t s) {
+  if (n == 1) {
+    k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
+    return;
+  }
+  k_refit<<<nb(n), 256, 0, s>>>(tris, n, nodes, leaf_parent, visits, root_box);
+  k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
+}
+void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
+                 unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
+                 hipStream_t s) {
+  const int nc = 3 * ntri;
+  k_weld_insert<<<nb(nc), 256, 0, s>>>(xyz, nc, slots, vmask, cslot);
+  k_weld_min<<<nb(nc), 256, 0, s>>>(cslot, nc, minid);
+  k_edge_insert<<<nb(ntri), 256, 0, s>>>(cslot, minid, ntri, ekeys, enet, emask);
+  k_edge_collect<<<nb((uint64_t)emask + 1), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
+}
+void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
+  k_occupancy<<<nb(n), 256, 0, s>>>(xyz, n, g, words);
+}
+void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s) {
+  k_first_touch<<<nb(n), 256, 0, s>>>(xyz, n, g, first);
+}
+void launch_first_compact(const int32_t *first, uint64_t ncell, Scalars *sc, unsigned long long *keys,
+                          hipStream_t s) {
+  k_first_compact<<<nb(ncell), 256, 0, s>>>(first, ncell, sc, keys);
+}
+hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                       uint32_t n, hipStream_t s) {
+  return rocprim::radix_sort_keys(temp, temp_bytes, kin, kout, n, 0u, 64u, s);
+}
+void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
+                         hipStream_t s) {
+  if (n) k_unpack_cells<<<nb(n), 256, 0, s>>>(keys, n, nx, ny, cells, tri);
+}
+void launch_columns(const ColumnArgs &a, hipStream_t s) {
+  dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
+  k_columns<<<grid, 256, 0, s>>>(a);
+}
+void launch_sdf(const SdfArgs &a, bool deep, hipStream_t s) {
+  dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
+  if (deep) k_sdf<true><<<grid, 256, 0, s>>>(a);
+  else k_sdf<false><<<grid, 256, 0, s>>>(a);
+}
+
+}  // namespace ib

@@ -1,0 +1,82 @@
+// Kernel argument blocks and host launchers shared by ib_kernels.hip and ib_api.cpp.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "ib_device.hpp"
+
+namespace ib {
+
+// |w| within this of 0.5 after the FP64 recheck -> ray parity decides
+constexpr double kTieEps = 1e-6;
+
+constexpr uint32_t kFlagNonFinite = 1u;
+// parent-chain walks stop here; a deeper chain means a malformed tree and is reported as depth
+constexpr int kMaxDepth = 4096;
+
+// device-side scalars of one bake (reset before every bake)
+struct Scalars {
+  uint32_t cmin[3], cmax[3];  // centroid bounds, ordered-int encoded
+  uint32_t flags;
+  uint32_t depth;
+  uint32_t boundary;      // unmatched directed edges (with multiplicity)
+  uint32_t col_overflow;  // columns with more than kColCap crossings
+  uint32_t rechecked;     // voxels re-evaluated in FP64
+  uint32_t ties;          // voxels decided by ray parity
+  uint32_t occ_set;       // occupied cells (first-touch voxelizer)
+  uint32_t pad[3];
+};
+
+struct GridArgs {
+  float ox, oy, oz, h;
+  int nx, ny, nz;
+};
+
+struct ColumnArgs {
+  const Node *nodes;
+  int nnodes;         // internal nodes (bounds every traversal loop)
+  const float4 *tris;
+  const float *caps;  // 9 floats per cap triangle
+  int ncap;
+  GridArgs g;
+  float2 *cross;      // kColCap per column: (z, +-1)
+  int32_t *count;     // per column
+  Scalars *sc;
+};
+
+struct SdfArgs {
+  ColumnArgs col;
+  float *out;
+  int *gstack;
+  float *gstackd;
+  int gstack_depth;
+};
+
+void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s);
+void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, int32_t *idx, hipStream_t s);
+hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                      int32_t *vout, int n, hipStream_t s);
+void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s);
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, hipStream_t s);
+void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
+                  float *root_box, Scalars *sc, hipStream_t s);
+void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
+                 unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
+                 hipStream_t s);
+void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
+void launch_columns(const ColumnArgs &a, hipStream_t s);
+void launch_sdf(const SdfArgs &a, bool deep, hipStream_t s);
+
+// first-touch voxelizer (drop-in voxelizeMesh order)
+void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s);
+void launch_first_compact(const int32_t *first, uint64_t ncell, Scalars *sc, unsigned long long *keys,
+                          hipStream_t s);
+hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                       uint32_t n, hipStream_t s);
+void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
+                         hipStream_t s);
+
+}  // namespace ib

@@ -1,0 +1,1 @@
+"""Host-side mirror of the reference's bake entry points over the C-ABI library."""

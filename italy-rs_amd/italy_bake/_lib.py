@@ -1,0 +1,103 @@
+"""ctypes binding of lib/libitaly_bake.so (C ABI declared in include/italy_bake.h)."""
+import ctypes
+import os
+
+from ._paths import LIB_DIR
+
+LIB_PATH = os.path.join(LIB_DIR, "libitaly_bake.so")
+
+IB_OK = 0
+STATUS_NAMES = {
+    0: "IB_OK", 1: "IB_ERR_INVALID_ARGUMENT", 2: "IB_ERR_INVALID_MESH", 3: "IB_ERR_OUT_OF_BUDGET",
+    4: "IB_ERR_DEVICE", 5: "IB_ERR_NO_DEVICE", 6: "IB_ERR_UNSUPPORTED", 7: "IB_ERR_CAPACITY",
+    8: "IB_ERR_NOT_READY", 9: "IB_ERR_INTERNAL",
+}
+
+# every symbol include/italy_bake.h declares
+EXPORTS = [
+    "ib_create", "ib_destroy", "ib_last_error", "ib_version", "ib_sdf_framing", "ib_vox_framing", "ib_bake_sdf",
+    "ib_voxelize", "ib_splat_materials", "ib_bake_dense", "ib_bake_dense_device", "ib_device_alloc",
+    "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_synchronize", "ib_last_stats",
+]
+
+
+class BakeError(RuntimeError):
+    """A non-zero ib_status, raised like the reference's CUDA_CHECK/OPTIX_CHECK runtime_error."""
+
+    def __init__(self, status, message):
+        self.status = status
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_float * 3), ("h", ctypes.c_float), ("nx", ctypes.c_int32),
+                ("ny", ctypes.c_int32), ("nz", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+    @classmethod
+    def make(cls, origin, h, nx, ny, nz):
+        g = cls()
+        for k in range(3):
+            g.origin[k] = float(origin[k])
+        g.h = float(h)
+        g.nx, g.ny, g.nz = int(nx), int(ny), int(nz)
+        return g
+
+    def shape(self):
+        return (self.nz, self.ny, self.nx)
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [
+        ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double), ("ms_bvh", ctypes.c_double),
+        ("ms_occupancy", ctypes.c_double), ("ms_columns", ctypes.c_double), ("ms_sdf", ctypes.c_double),
+        ("triangles", ctypes.c_uint64), ("voxels", ctypes.c_uint64), ("occ_cells", ctypes.c_uint64),
+        ("bvh_depth", ctypes.c_int32), ("deep_stack", ctypes.c_int32), ("boundary_edges", ctypes.c_int64),
+        ("column_overflows", ctypes.c_uint64), ("rechecked", ctypes.c_uint64), ("ties", ctypes.c_uint64),
+        ("reserved", ctypes.c_uint64 * 6),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int32
+_f3 = ctypes.POINTER(ctypes.c_float)
+
+
+def load():
+    """Load the HIP library; raises if it is not built (there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() (there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    G = ctypes.POINTER(Grid)
+    lib.ib_create.argtypes = [ctypes.c_int, _u64, ctypes.POINTER(_vp)]
+    lib.ib_destroy.argtypes = [_vp]
+    lib.ib_destroy.restype = None
+    lib.ib_last_error.restype = ctypes.c_char_p
+    lib.ib_version.restype = ctypes.c_char_p
+    lib.ib_sdf_framing.argtypes = [_f3, _f3, _i32, G]
+    lib.ib_vox_framing.argtypes = [_f3, _f3, _i32, _f3, _f3]
+    lib.ib_bake_sdf.argtypes = [_vp, _vp, _u64, _f3, _f3, _i32, G, _vp, _u64]
+    lib.ib_voxelize.argtypes = [_vp, _vp, _u64, _f3, _f3, _i32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+    lib.ib_splat_materials.argtypes = [_vp, _vp, _u64, _vp, G, _vp, _vp, _vp]
+    lib.ib_bake_dense.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp]
+    lib.ib_bake_dense_device.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp, _vp]
+    lib.ib_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
+    lib.ib_device_free.argtypes = [_vp, _vp]
+    lib.ib_memcpy_h2d.argtypes = [_vp, _vp, _vp, _u64]
+    lib.ib_memcpy_d2h.argtypes = [_vp, _vp, _vp, _u64]
+    lib.ib_synchronize.argtypes = [_vp]
+    lib.ib_last_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != IB_OK:
+        raise BakeError(status, load().ib_last_error().decode(errors="replace"))

@@ -1,0 +1,197 @@
+"""Host-side mirror of the reference bake entry points over the HIP C ABI.
+
+    bakeSdf(mesh, resolution)      -> SdfGrid      src/convert/sdf_baker.h:8
+    voxelizeMesh(mesh, resolution) -> VoxelGrid    src/convert/voxelize.h:8
+
+plus the explicit-grid bake the workloads use (Context.bake_dense / bake_dense_device).
+Every call runs the gfx950 kernels; a missing library raises (there is no CPU fallback).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import BakeError, Grid, Stats, check
+
+
+def _f3(v):
+    a = (ctypes.c_float * 3)()
+    for k in range(3):
+        a[k] = float(v[k])
+    return a
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class Context:
+    """One device context: stream, growable device scratch, HBM budget (ib_create)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        lib = _lib.load()
+        h = ctypes.c_void_p()
+        check(lib.ib_create(int(device), int(hbm_budget_bytes), ctypes.byref(h)))
+        self._h = h
+        self.lib = lib
+
+    def close(self):
+        if self._h:
+            self.lib.ib_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ---- explicit grids ---------------------------------------------------------------------
+    def bake_dense(self, xyz, sdf_grid=None, occ_grid=None):
+        """Bake one soup (ntri,3,3) float32: returns (distances (nz,ny,nx) or None, occ words or None)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        ntri = xyz.size // 9
+        dist = np.empty(sdf_grid.shape(), np.float32) if sdf_grid is not None else None
+        occ = None
+        if occ_grid is not None:
+            occ = np.empty(((occ_grid.nx + 31) // 32) * occ_grid.ny * occ_grid.nz, np.uint32)
+        check(self.lib.ib_bake_dense(self._h, _ptr(xyz) if ntri else None, ntri,
+                                     ctypes.byref(sdf_grid) if sdf_grid is not None else None, _ptr(dist),
+                                     ctypes.byref(occ_grid) if occ_grid is not None else None, _ptr(occ)))
+        return dist, occ
+
+    def bake_dense_device(self, d_xyz, ntri, sdf_grid, d_dist, occ_grid=None, d_occ=None, stream=None):
+        check(self.lib.ib_bake_dense_device(self._h, ctypes.c_void_p(d_xyz), int(ntri),
+                                            ctypes.byref(sdf_grid) if sdf_grid is not None else None,
+                                            ctypes.c_void_p(d_dist) if d_dist else None,
+                                            ctypes.byref(occ_grid) if occ_grid is not None else None,
+                                            ctypes.c_void_p(d_occ) if d_occ else None,
+                                            ctypes.c_void_p(stream) if stream else None))
+
+    def stats(self):
+        st = Stats()
+        check(self.lib.ib_last_stats(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    # ---- device memory ----------------------------------------------------------------------
+    def alloc(self, nbytes):
+        p = ctypes.c_void_p()
+        check(self.lib.ib_device_alloc(self._h, int(nbytes), ctypes.byref(p)))
+        return p.value
+
+    def free(self, p):
+        check(self.lib.ib_device_free(self._h, ctypes.c_void_p(p)))
+
+    def upload(self, d_ptr, arr):
+        arr = np.ascontiguousarray(arr)
+        check(self.lib.ib_memcpy_h2d(self._h, ctypes.c_void_p(d_ptr), _ptr(arr), arr.nbytes))
+
+    def download(self, d_ptr, arr):
+        check(self.lib.ib_memcpy_d2h(self._h, _ptr(arr), ctypes.c_void_p(d_ptr), arr.nbytes))
+        return arr
+
+    def synchronize(self):
+        check(self.lib.ib_synchronize(self._h))
+
+
+# ---- the reference's data types and free functions --------------------------------------------
+class SdfGrid:
+    """src/convert/sdf_grid.h:21-36: origin, voxelSize, nx, ny, nz, distances (x fastest)."""
+
+    def __init__(self, grid, distances):
+        self.origin = np.array(list(grid.origin), np.float32)
+        self.voxelSize = np.float32(grid.h)
+        self.nx, self.ny, self.nz = grid.nx, grid.ny, grid.nz
+        self.distances = distances.reshape(-1)
+        self.baseColor = np.empty((0, 3), np.float32)
+        self.metallic = np.empty(0, np.float32)
+        self.roughness = np.empty(0, np.float32)
+
+    def index(self, x, y, z):
+        return (z * self.ny + y) * self.nx + x
+
+    def boundsMax(self):
+        return self.origin + np.array([self.nx, self.ny, self.nz], np.float32) * self.voxelSize
+
+
+class VoxelGrid:
+    """src/convert/voxel_grid.h:9-17: origin, voxelSize, cells in first-touch order."""
+
+    def __init__(self, origin, voxel_size, cells, first_tri):
+        self.origin = np.asarray(origin, np.float32)
+        self.voxelSize = np.float32(voxel_size)
+        self.cells = cells
+        self.first_triangle = first_tri
+
+    def cellMin(self, c):
+        return self.origin + np.asarray(c, np.float32) * self.voxelSize
+
+    def cellMax(self, c):
+        return self.cellMin(c) + self.voxelSize
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context()
+    return _default_ctx
+
+
+def sdf_framing(bmin, bmax, resolution):
+    g = Grid()
+    check(_lib.load().ib_sdf_framing(_f3(bmin), _f3(bmax), int(resolution), ctypes.byref(g)))
+    return g
+
+
+def bakeSdf(mesh, resolution, ctx=None):
+    """italy::bakeSdf (src/convert/sdf_baker.h:8): reference framing, exact signed distances."""
+    ctx = ctx or default_context()
+    g = sdf_framing(mesh.bmin, mesh.bmax, resolution)
+    dist = np.empty(g.nx * g.ny * g.nz, np.float32)
+    xyz = np.ascontiguousarray(mesh.xyz, np.float32)
+    check(ctx.lib.ib_bake_sdf(ctx.handle, _ptr(xyz) if mesh.ntri else None, mesh.ntri, _f3(mesh.bmin),
+                              _f3(mesh.bmax), int(resolution), ctypes.byref(g), _ptr(dist), dist.size))
+    return SdfGrid(g, dist)
+
+
+def voxelizeMesh(mesh, resolution, ctx=None):
+    """italy::voxelizeMesh (src/convert/voxelize.h:8): occupied cells in first-touch order."""
+    ctx = ctx or default_context()
+    lib = ctx.lib
+    xyz = np.ascontiguousarray(mesh.xyz, np.float32)
+    origin = (ctypes.c_float * 3)()
+    vs = ctypes.c_float()
+    check(lib.ib_vox_framing(_f3(mesh.bmin), _f3(mesh.bmax), int(resolution), origin, ctypes.byref(vs)))
+    count = ctypes.c_uint64()
+    check(lib.ib_voxelize(ctx.handle, _ptr(xyz) if mesh.ntri else None, mesh.ntri, _f3(mesh.bmin), _f3(mesh.bmax),
+                          int(resolution), None, None, 0, ctypes.byref(count)))
+    n = count.value
+    cells = np.zeros((max(n, 1), 3), np.int32)
+    tri = np.zeros(max(n, 1), np.int32)
+    if n:
+        check(lib.ib_voxelize(ctx.handle, _ptr(xyz), mesh.ntri, _f3(mesh.bmin), _f3(mesh.bmax), int(resolution),
+                              _ptr(cells), _ptr(tri), n, ctypes.byref(count)))
+    return VoxelGrid(list(origin), vs.value, cells[:n], tri[:n])
+
+
+def dense_grid(lo, hi, res):
+    """Cell-centred res^3 grid over [lo, hi]^3."""
+    h = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(res))
+    return Grid.make((lo, lo, lo), h, res, res, res)
+
+
+__all__ = ["Context", "SdfGrid", "VoxelGrid", "bakeSdf", "voxelizeMesh", "sdf_framing", "dense_grid", "BakeError",
+           "Grid"]

@@ -1,0 +1,109 @@
+"""Seeded synthetic meshes (triangle soups) for the bake workloads, via libitaly_meshgen.so."""
+import ctypes
+import os
+
+import numpy as np
+
+from ._paths import PKG_ROOT
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+
+
+class _Soup(ctypes.Structure):
+    _fields_ = [("xyz", _f32p), ("ntri", ctypes.c_int64), ("bmin", ctypes.c_float * 3),
+                ("bmax", ctypes.c_float * 3)]
+
+
+class Mesh:
+    """Triangle soup: positions (ntri, 3, 3) float32 and caller bounds, as MeshAsset
+    (src/io/mesh_asset.h:33-54) holds them."""
+
+    def __init__(self, xyz, bmin, bmax):
+        self.xyz = np.ascontiguousarray(xyz, dtype=np.float32).reshape(-1, 3, 3)
+        self.bmin = np.asarray(bmin, np.float32).copy()
+        self.bmax = np.asarray(bmax, np.float32).copy()
+
+    @property
+    def ntri(self):
+        return self.xyz.shape[0]
+
+    def __repr__(self):
+        return f"Mesh(ntri={self.ntri}, bmin={self.bmin.tolist()}, bmax={self.bmax.tolist()})"
+
+
+_lib = None
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        path = os.path.join(PKG_ROOT, "lib", "libitaly_meshgen.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        lib = ctypes.CDLL(path)
+        S = ctypes.POINTER(_Soup)
+        lib.ib_gen_uv_sphere.argtypes = [ctypes.c_float, ctypes.c_int, ctypes.c_int, S]
+        lib.ib_gen_cube_shell.argtypes = [S]
+        lib.ib_gen_icosphere.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_uint64, S]
+        lib.ib_gen_displaced.argtypes = [ctypes.c_int, ctypes.c_uint64, S]
+        lib.ib_gen_decimated.argtypes = [ctypes.c_int64, ctypes.c_uint64, S]
+        lib.ib_gen_stress.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, S]
+        lib.ib_soup_free.argtypes = [S]
+        _lib = lib
+    return _lib
+
+
+def _take(fn, *args):
+    s = _Soup()
+    rc = fn(*args, ctypes.byref(s))
+    if rc != 0:
+        raise ValueError(f"mesh generator failed ({rc})")
+    n = int(s.ntri)
+    xyz = np.ctypeslib.as_array(s.xyz, (max(n, 1) * 9,)).copy()[: n * 9]
+    m = Mesh(xyz, list(s.bmin), list(s.bmax))
+    _load().ib_soup_free(ctypes.byref(s))
+    return m
+
+
+def uv_sphere(r=1.0, lon=32, lat=16):
+    """The UV sphere of tests/sdf_sign_test.cpp:18-42 (bounds set to +-r as the test does)."""
+    return _take(_load().ib_gen_uv_sphere, float(r), int(lon), int(lat))
+
+
+def cube_shell():
+    """The unit cube shell of tests/voxelize_test.cpp:11-33."""
+    return _take(_load().ib_gen_cube_shell)
+
+
+def icosphere(level, radius=0.5, jitter=0.0, seed=0x17A1E5):
+    return _take(_load().ib_gen_icosphere, int(level), float(radius), float(jitter), int(seed))
+
+
+def displaced(level, seed=0x17A1E5):
+    return _take(_load().ib_gen_displaced, int(level), int(seed))
+
+
+def decimated(target, seed):
+    return _take(_load().ib_gen_decimated, int(target), int(seed))
+
+
+def stress(seed=0x17A1E5, ncomp=64, remove_frac=0.02):
+    return _take(_load().ib_gen_stress, int(seed), int(ncomp), float(remove_frac))
+
+
+# ---- the workloads -----------------------------------------------------------------------
+BASE_SEED = 0x17A1E5
+
+
+def workload_plain_small():
+    """5,120-triangle icosphere (level 4, r 0.5) with radial jitter U(+-1e-3) of the welded
+    vertices; baked to 64^3 over [-0.6, 0.6]^3 (SDF and occupancy)."""
+    m = icosphere(4, 0.5, 1e-3, BASE_SEED)
+    return m, dict(lo=-0.6, hi=0.6, sdf_res=64, occ_res=64)
+
+
+def workload_plain_large():
+    """327,680-triangle displaced icosphere (level 7); 256^3 SDF and 512^3 occupancy over
+    [-0.8, 0.8]^3."""
+    m = displaced(7, BASE_SEED)
+    return m, dict(lo=-0.8, hi=0.8, sdf_res=256, occ_res=512)

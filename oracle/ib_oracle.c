@@ -1,0 +1,1077 @@
+/*
+ * CPU oracle for the mesh -> SDF / occupancy bake path.  TEST INFRASTRUCTURE ONLY:
+ * loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+ * checker, never by the product.  Build: oracle/Makefile (-O2 -ffp-contract=off).
+ * Reference anchors are given per function; see ib_oracle.h for the list.
+ */
+#include "ib_oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include <unistd.h>
+
+/* std::min / std::max semantics for two floats (tri_box_overlap.h:32,39,...) */
+static inline float smin(float a, float b) { return (b < a) ? b : a; }
+static inline float smax(float a, float b) { return (a < b) ? b : a; }
+/* glm::dot(vec3) = (x*x' + y*y') + z*z' */
+static inline float gdot(const float a[3], const float b[3]) {
+  const float tx = a[0] * b[0], ty = a[1] * b[1], tz = a[2] * b[2];
+  return tx + ty + tz;
+}
+
+/* ============================================================================
+ * a10: triBoxOverlap  (tri_box_overlap.h:12-114), expression trees kept literally
+ * ========================================================================== */
+static int plane_test(const float n[3], const float v[3], const float mb[3]) { /* :12-27 */
+  float vmin[3], vmax[3];
+  for (int q = 0; q < 3; ++q) {
+    const float x = v[q];
+    if (n[q] > 0.0f) {
+      vmin[q] = -mb[q] - x;
+      vmax[q] = mb[q] - x;
+    } else {
+      vmin[q] = mb[q] - x;
+      vmax[q] = -mb[q] - x;
+    }
+  }
+  if (gdot(n, vmin) > 0.0f) return 0;
+  return gdot(n, vmax) >= 0.0f;
+}
+/* :29-35 */
+static int ax_x01(float a, float b, float fa, float fb, const float v0[3], const float v2[3], const float bh[3]) {
+  const float p0 = a * v0[1] - b * v0[2], p2 = a * v2[1] - b * v2[2];
+  const float mn = smin(p0, p2), mx = smax(p0, p2);
+  const float rad = fa * bh[1] + fb * bh[2];
+  return !(mn > rad || mx < -rad);
+}
+/* :36-42 */
+static int ax_x2(float a, float b, float fa, float fb, const float v0[3], const float v1[3], const float bh[3]) {
+  const float p0 = a * v0[1] - b * v0[2], p1 = a * v1[1] - b * v1[2];
+  const float mn = smin(p0, p1), mx = smax(p0, p1);
+  const float rad = fa * bh[1] + fb * bh[2];
+  return !(mn > rad || mx < -rad);
+}
+/* :43-49 */
+static int ax_y02(float a, float b, float fa, float fb, const float v0[3], const float v2[3], const float bh[3]) {
+  const float p0 = -a * v0[0] + b * v0[2], p2 = -a * v2[0] + b * v2[2];
+  const float mn = smin(p0, p2), mx = smax(p0, p2);
+  const float rad = fa * bh[0] + fb * bh[2];
+  return !(mn > rad || mx < -rad);
+}
+/* :50-56 */
+static int ax_y1(float a, float b, float fa, float fb, const float v0[3], const float v1[3], const float bh[3]) {
+  const float p0 = -a * v0[0] + b * v0[2], p1 = -a * v1[0] + b * v1[2];
+  const float mn = smin(p0, p1), mx = smax(p0, p1);
+  const float rad = fa * bh[0] + fb * bh[2];
+  return !(mn > rad || mx < -rad);
+}
+/* :57-63 */
+static int ax_z12(float a, float b, float fa, float fb, const float v1[3], const float v2[3], const float bh[3]) {
+  const float p1 = a * v1[0] - b * v1[1], p2 = a * v2[0] - b * v2[1];
+  const float mn = smin(p1, p2), mx = smax(p1, p2);
+  const float rad = fa * bh[0] + fb * bh[1];
+  return !(mn > rad || mx < -rad);
+}
+/* :64-70 */
+static int ax_z0(float a, float b, float fa, float fb, const float v0[3], const float v1[3], const float bh[3]) {
+  const float p0 = a * v0[0] - b * v0[1], p1 = a * v1[0] - b * v1[1];
+  const float mn = smin(p0, p1), mx = smax(p0, p1);
+  const float rad = fa * bh[0] + fb * bh[1];
+  return !(mn > rad || mx < -rad);
+}
+
+int orc_tri_box_overlap(const float c[3], const float bh[3], const float t0[3], const float t1[3], const float t2[3]) {
+  float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3];
+  for (int k = 0; k < 3; ++k) { /* :74-75 */
+    v0[k] = t0[k] - c[k];
+    v1[k] = t1[k] - c[k];
+    v2[k] = t2[k] - c[k];
+  }
+  for (int k = 0; k < 3; ++k) {
+    e0[k] = v1[k] - v0[k];
+    e1[k] = v2[k] - v1[k];
+    e2[k] = v0[k] - v2[k];
+  }
+  float fex = fabsf(e0[0]), fey = fabsf(e0[1]), fez = fabsf(e0[2]); /* :77-83 */
+  if (!ax_x01(e0[2], e0[1], fez, fey, v0, v2, bh)) return 0;
+  if (!ax_y02(e0[2], e0[0], fez, fex, v0, v2, bh)) return 0;
+  if (!ax_z12(e0[1], e0[0], fey, fex, v1, v2, bh)) return 0;
+  fex = fabsf(e1[0]); /* :85-93 */
+  fey = fabsf(e1[1]);
+  fez = fabsf(e1[2]);
+  if (!ax_x01(e1[2], e1[1], fez, fey, v0, v2, bh)) return 0;
+  if (!ax_y02(e1[2], e1[0], fez, fex, v0, v2, bh)) return 0;
+  if (!ax_z0(e1[1], e1[0], fey, fex, v0, v1, bh)) return 0;
+  fex = fabsf(e2[0]); /* :95-103 */
+  fey = fabsf(e2[1]);
+  fez = fabsf(e2[2]);
+  if (!ax_x2(e2[2], e2[1], fez, fey, v0, v1, bh)) return 0;
+  if (!ax_y1(e2[2], e2[0], fez, fex, v0, v1, bh)) return 0;
+  if (!ax_z12(e2[1], e2[0], fey, fex, v1, v2, bh)) return 0;
+  for (int axis = 0; axis < 3; ++axis) { /* :105-110 */
+    const float mn = smin(smin(v0[axis], v1[axis]), v2[axis]);
+    const float mx = smax(smax(v0[axis], v1[axis]), v2[axis]);
+    if (mn > bh[axis] || mx < -bh[axis]) return 0;
+  }
+  /* :112-113  glm::cross */
+  const float n[3] = {e0[1] * e1[2] - e1[1] * e0[2], e0[2] * e1[0] - e1[2] * e0[0], e0[0] * e1[1] - e1[0] * e0[1]};
+  return plane_test(n, v0, bh);
+}
+
+/* ============================================================================
+ * framing
+ * ========================================================================== */
+static float longest_axis(const float bmin[3], const float bmax[3], float ext[3]) {
+  for (int k = 0; k < 3; ++k) ext[k] = bmax[k] - bmin[k];
+  /* std::max({ex, ey, ez, 1e-6f}) */
+  float m = ext[0];
+  if (m < ext[1]) m = ext[1];
+  if (m < ext[2]) m = ext[2];
+  if (m < 1e-6f) m = 1e-6f;
+  return m;
+}
+
+void orc_sdf_framing(const float bmin[3], const float bmax[3], int res, float origin[3], float *h, int n[3]) {
+  float ext[3];
+  const float longest = longest_axis(bmin, bmax, ext); /* sdf_baker.cpp:139-140 */
+  const float vs = longest / (float)(res > 1 ? res : 1); /* :141 */
+  const float pad = vs * 2.0f;                           /* :143-144 kPad = 2 */
+  const float pad2 = pad * 2.0f;                         /* :145 */
+  for (int k = 0; k < 3; ++k) {
+    origin[k] = bmin[k] - pad;
+    const float padded = ext[k] + pad2;
+    n[k] = (int)ceilf(padded / vs) + 1; /* :146-148 */
+  }
+  *h = vs;
+}
+
+void orc_vox_framing(const float bmin[3], const float bmax[3], int res, float origin[3], float *h) {
+  float ext[3];
+  const float longest = longest_axis(bmin, bmax, ext); /* voxelize.cpp:54-55 */
+  *h = longest / (float)(res > 1 ? res : 1);           /* :56 */
+  for (int k = 0; k < 3; ++k) origin[k] = bmin[k];     /* :57 */
+}
+
+/* per-triangle inclusive cell range, voxelize.cpp:70-77 / sdf_baker.cpp:95-101 */
+static void cell_range(const float *t, const float origin[3], float h, const int maxc[3], int lo[3], int hi[3]) {
+  for (int k = 0; k < 3; ++k) {
+    const float mn = smin(smin(t[k], t[3 + k]), t[6 + k]);
+    const float mx = smax(smax(t[k], t[3 + k]), t[6 + k]);
+    int a = (int)floorf((mn - origin[k]) / h);
+    int b = (int)floorf((mx - origin[k]) / h);
+    /* glm::clamp(x, 0, m) = min(max(x, 0), m): res <= 0 gives -1, as the reference does */
+    a = a < 0 ? 0 : a;
+    a = a > maxc[k] ? maxc[k] : a;
+    b = b < 0 ? 0 : b;
+    b = b > maxc[k] ? maxc[k] : b;
+    lo[k] = a;
+    hi[k] = b;
+  }
+}
+
+static void cell_center(const float origin[3], float h, int x, int y, int z, float c[3]) {
+  c[0] = origin[0] + ((float)x + 0.5f) * h; /* voxelize.cpp:84 */
+  c[1] = origin[1] + ((float)y + 0.5f) * h;
+  c[2] = origin[2] + ((float)z + 0.5f) * h;
+}
+
+/* ============================================================================
+ * a9: voxelizeMesh (first-touch order), voxelize.cpp:52-107
+ * ========================================================================== */
+typedef struct {
+  int64_t *keys; /* packed cell key, -1 empty */
+  int64_t *vals;
+  size_t cap;
+} i64map;
+
+static uint64_t mix64(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33;
+  return x;
+}
+static void map_init(i64map *m, size_t cap) {
+  size_t c = 16;
+  while (c < cap * 2) c <<= 1;
+  m->cap = c;
+  m->keys = (int64_t *)malloc(c * sizeof(int64_t));
+  m->vals = (int64_t *)malloc(c * sizeof(int64_t));
+  for (size_t i = 0; i < c; ++i) m->keys[i] = -1;
+}
+static void map_free(i64map *m) {
+  free(m->keys);
+  free(m->vals);
+}
+static void map_grow(i64map *m);
+/* returns pointer to value; *inserted set when new */
+static int64_t *map_get(i64map *m, int64_t key, int *inserted, size_t *count) {
+  if ((*count + 1) * 2 > m->cap) map_grow(m);
+  size_t i = mix64((uint64_t)key) & (m->cap - 1);
+  for (;;) {
+    if (m->keys[i] == key) {
+      *inserted = 0;
+      return &m->vals[i];
+    }
+    if (m->keys[i] == -1) {
+      m->keys[i] = key;
+      *inserted = 1;
+      (*count)++;
+      return &m->vals[i];
+    }
+    i = (i + 1) & (m->cap - 1);
+  }
+}
+static void map_grow(i64map *m) {
+  i64map n;
+  map_init(&n, m->cap);
+  for (size_t i = 0; i < m->cap; ++i)
+    if (m->keys[i] != -1) {
+      size_t j = mix64((uint64_t)m->keys[i]) & (n.cap - 1);
+      while (n.keys[j] != -1) j = (j + 1) & (n.cap - 1);
+      n.keys[j] = m->keys[i];
+      n.vals[j] = m->vals[i];
+    }
+  map_free(m);
+  *m = n;
+}
+
+size_t orc_voxelize_first_touch(const float *xyz, size_t ntri, const float *colors, const float bmin[3],
+                                const float bmax[3], int res, int32_t *cells, float *cell_colors, size_t cap) {
+  float origin[3], h;
+  orc_vox_framing(bmin, bmax, res, origin, &h);
+  const int maxc[3] = {res - 1, res - 1, res - 1};
+  const float half[3] = {h * 0.5f, h * 0.5f, h * 0.5f}; /* :85 */
+  i64map map;
+  map_init(&map, 1024);
+  size_t count = 0;
+  size_t ncells = 0;
+  size_t scap = 1024;
+  float *csum = (float *)malloc(scap * 3 * sizeof(float));
+  int *ccnt = (int *)malloc(scap * sizeof(int));
+  int32_t *tmpcells = (int32_t *)malloc(scap * 3 * sizeof(int32_t));
+  for (size_t t = 0; t < ntri; ++t) {
+    const float *tri = xyz + 9 * t;
+    int lo[3], hi[3];
+    cell_range(tri, origin, h, maxc, lo, hi);
+    const float col[3] = {colors ? colors[3 * t] : 1.0f, colors ? colors[3 * t + 1] : 1.0f,
+                          colors ? colors[3 * t + 2] : 1.0f};
+    for (int z = lo[2]; z <= hi[2]; ++z)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          float c[3];
+          cell_center(origin, h, x, y, z, c);
+          if (!orc_tri_box_overlap(c, half, tri, tri + 3, tri + 6)) continue;
+          /* cells may be negative only when res <= 0; pack with an offset */
+          const int64_t key = (((int64_t)z + 0x100000) << 42) | (((int64_t)y + 0x100000) << 21) | ((int64_t)x + 0x100000);
+          int ins;
+          int64_t *v = map_get(&map, key, &ins, &count);
+          if (ins) {
+            if (ncells == scap) {
+              scap *= 2;
+              csum = (float *)realloc(csum, scap * 3 * sizeof(float));
+              ccnt = (int *)realloc(ccnt, scap * sizeof(int));
+              tmpcells = (int32_t *)realloc(tmpcells, scap * 3 * sizeof(int32_t));
+            }
+            *v = (int64_t)ncells;
+            tmpcells[3 * ncells] = x;
+            tmpcells[3 * ncells + 1] = y;
+            tmpcells[3 * ncells + 2] = z;
+            csum[3 * ncells] = col[0];
+            csum[3 * ncells + 1] = col[1];
+            csum[3 * ncells + 2] = col[2];
+            ccnt[ncells] = 1;
+            ncells++;
+          } else {
+            const size_t i = (size_t)*v;
+            csum[3 * i] += col[0];
+            csum[3 * i + 1] += col[1];
+            csum[3 * i + 2] += col[2];
+            ccnt[i] += 1;
+          }
+        }
+  }
+  const size_t n = ncells < cap ? ncells : cap;
+  for (size_t i = 0; i < n; ++i) {
+    if (cells) {
+      cells[3 * i] = tmpcells[3 * i];
+      cells[3 * i + 1] = tmpcells[3 * i + 1];
+      cells[3 * i + 2] = tmpcells[3 * i + 2];
+    }
+    if (cell_colors) { /* :102-104 */
+      const float cnt = (float)ccnt[i];
+      cell_colors[3 * i] = csum[3 * i] / cnt;
+      cell_colors[3 * i + 1] = csum[3 * i + 1] / cnt;
+      cell_colors[3 * i + 2] = csum[3 * i + 2] / cnt;
+    }
+  }
+  free(csum);
+  free(ccnt);
+  free(tmpcells);
+  map_free(&map);
+  return ncells;
+}
+
+void orc_occupancy(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
+                   uint32_t *words) {
+  const int maxc[3] = {nx - 1, ny - 1, nz - 1};
+  const float half[3] = {h * 0.5f, h * 0.5f, h * 0.5f};
+  const int wpr = (nx + 31) / 32;
+  for (size_t t = 0; t < ntri; ++t) {
+    const float *tri = xyz + 9 * t;
+    int lo[3], hi[3];
+    cell_range(tri, origin, h, maxc, lo, hi);
+    for (int z = lo[2]; z <= hi[2]; ++z)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          float c[3];
+          cell_center(origin, h, x, y, z, c);
+          if (orc_tri_box_overlap(c, half, tri, tri + 3, tri + 6))
+            words[((size_t)z * ny + y) * wpr + (x >> 5)] |= 1u << (x & 31);
+        }
+  }
+}
+
+/* ============================================================================
+ * distance
+ * ========================================================================== */
+int orc_tri_is_sliver(const float *t) {
+  const float ab[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+  const float ac[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+  const float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  const float nn = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+  const float l1 = ab[0] * ab[0] + ab[1] * ab[1] + ab[2] * ab[2];
+  const float l2 = ac[0] * ac[0] + ac[1] * ac[1] + ac[2] * ac[2];
+  /* sin^2(angle at v0) < 1e-4, or any zero-length edge */
+  return !(nn > 1e-4f * (l1 * l2));
+}
+
+static inline float dot3f(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+static inline double dot3d(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* Closest-point regions after Ericson, Real-Time Collision Detection (2005) 5.1.5, evaluated in
+ * coordinates relative to the query; the face region uses the plane distance (n.a)^2/|n|^2. */
+float orc_tri_dist2_f32(const float q[3], const float *t) {
+  if (orc_tri_is_sliver(t)) {
+    const double qd[3] = {q[0], q[1], q[2]};
+    return (float)orc_tri_dist2_f64(qd, t);
+  }
+  const float a[3] = {t[0] - q[0], t[1] - q[1], t[2] - q[2]};
+  const float b[3] = {t[3] - q[0], t[4] - q[1], t[5] - q[2]};
+  const float c[3] = {t[6] - q[0], t[7] - q[1], t[8] - q[2]};
+  const float ab[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+  const float ac[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+  const float ap[3] = {-a[0], -a[1], -a[2]};
+  const float d1 = dot3f(ab, ap), d2 = dot3f(ac, ap);
+  if (d1 <= 0.0f && d2 <= 0.0f) return dot3f(a, a);
+  const float bp[3] = {-b[0], -b[1], -b[2]};
+  const float d3 = dot3f(ab, bp), d4 = dot3f(ac, bp);
+  if (d3 >= 0.0f && d4 <= d3) return dot3f(b, b);
+  const float vc = d1 * d4 - d3 * d2;
+  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
+    const float v = d1 / (d1 - d3);
+    const float r[3] = {a[0] + v * ab[0], a[1] + v * ab[1], a[2] + v * ab[2]};
+    return dot3f(r, r);
+  }
+  const float cp[3] = {-c[0], -c[1], -c[2]};
+  const float d5 = dot3f(ab, cp), d6 = dot3f(ac, cp);
+  if (d6 >= 0.0f && d5 <= d6) return dot3f(c, c);
+  const float vb = d5 * d2 - d1 * d6;
+  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
+    const float w = d2 / (d2 - d6);
+    const float r[3] = {a[0] + w * ac[0], a[1] + w * ac[1], a[2] + w * ac[2]};
+    return dot3f(r, r);
+  }
+  const float va = d3 * d6 - d5 * d4;
+  const float e43 = d4 - d3, e56 = d5 - d6;
+  if (va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f) {
+    const float w = e43 / (e43 + e56);
+    const float bc[3] = {t[6] - t[3], t[7] - t[4], t[8] - t[5]};
+    const float r[3] = {b[0] + w * bc[0], b[1] + w * bc[1], b[2] + w * bc[2]};
+    return dot3f(r, r);
+  }
+  const float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  const float s = dot3f(n, a);
+  return (s * s) / dot3f(n, n);
+}
+
+static double seg_dist2_d(const double a[3], const double b[3]) {
+  const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double l = dot3d(ab, ab);
+  double t = 0.0;
+  if (l > 0.0) {
+    t = -dot3d(a, ab) / l;
+    t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
+  }
+  const double r[3] = {a[0] + t * ab[0], a[1] + t * ab[1], a[2] + t * ab[2]};
+  return dot3d(r, r);
+}
+
+double orc_tri_dist2_f64(const double q[3], const float *t) {
+  const double a[3] = {(double)t[0] - q[0], (double)t[1] - q[1], (double)t[2] - q[2]};
+  const double b[3] = {(double)t[3] - q[0], (double)t[4] - q[1], (double)t[5] - q[2]};
+  const double c[3] = {(double)t[6] - q[0], (double)t[7] - q[1], (double)t[8] - q[2]};
+  double best = seg_dist2_d(a, b);
+  double d = seg_dist2_d(b, c);
+  if (d < best) best = d;
+  d = seg_dist2_d(c, a);
+  if (d < best) best = d;
+  const double ab[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  const double ac[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  const double n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+  const double nn = dot3d(n, n);
+  if (nn > 0.0) {
+    /* projection of the origin (the query) inside the triangle? */
+    const double s = dot3d(n, a) / nn;
+    const double p[3] = {s * n[0], s * n[1], s * n[2]};
+    const double pa[3] = {a[0] - p[0], a[1] - p[1], a[2] - p[2]};
+    const double pb[3] = {b[0] - p[0], b[1] - p[1], b[2] - p[2]};
+    const double pc[3] = {c[0] - p[0], c[1] - p[1], c[2] - p[2]};
+    double x[3];
+    x[0] = pa[1] * pb[2] - pa[2] * pb[1];
+    x[1] = pa[2] * pb[0] - pa[0] * pb[2];
+    x[2] = pa[0] * pb[1] - pa[1] * pb[0];
+    const double u = dot3d(x, n);
+    x[0] = pb[1] * pc[2] - pb[2] * pc[1];
+    x[1] = pb[2] * pc[0] - pb[0] * pc[2];
+    x[2] = pb[0] * pc[1] - pb[1] * pc[0];
+    const double v = dot3d(x, n);
+    x[0] = pc[1] * pa[2] - pc[2] * pa[1];
+    x[1] = pc[2] * pa[0] - pc[0] * pa[2];
+    x[2] = pc[0] * pa[1] - pc[1] * pa[0];
+    const double w = dot3d(x, n);
+    if (u >= 0.0 && v >= 0.0 && w >= 0.0) {
+      const double s2 = dot3d(n, a);
+      const double pd = s2 * s2 / nn;
+      if (pd < best) best = pd;
+    }
+  }
+  return best;
+}
+
+/* ============================================================================
+ * winding number / crossings / parity
+ * ========================================================================== */
+double orc_gwn_f64(const double q[3], const float *xyz, size_t ntri) {
+  double sum = 0.0, comp = 0.0; /* Neumaier */
+  for (size_t t = 0; t < ntri; ++t) {
+    const float *v = xyz + 9 * t;
+    const double a[3] = {(double)v[0] - q[0], (double)v[1] - q[1], (double)v[2] - q[2]};
+    const double b[3] = {(double)v[3] - q[0], (double)v[4] - q[1], (double)v[5] - q[2]};
+    const double c[3] = {(double)v[6] - q[0], (double)v[7] - q[1], (double)v[8] - q[2]};
+    const double la = sqrt(dot3d(a, a)), lb = sqrt(dot3d(b, b)), lc = sqrt(dot3d(c, c));
+    const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+    const double det = dot3d(a, bc);
+    const double den = la * lb * lc + dot3d(a, b) * lc + dot3d(b, c) * la + dot3d(c, a) * lb;
+    const double om = 2.0 * atan2(det, den);
+    const double s = sum + om;
+    if (fabs(sum) >= fabs(om))
+      comp += (sum - s) + om;
+    else
+      comp += (om - s) + sum;
+    sum = s;
+  }
+  return (sum + comp) / (4.0 * M_PI);
+}
+
+/* Half-open tie rule for an exactly-zero edge function: positive iff the edge direction D points
+ * into the upper half plane (D.y > 0, or D.y == 0 and D.x > 0).  Antisymmetric in D, so the two
+ * triangles sharing an edge never both claim a point on it. */
+static inline int tie_pos(float dx, float dy) { return dy > 0.0f || (dy == 0.0f && dx > 0.0f); }
+/* sign of e = px*qy - py*qx, evaluated in FP64: the float products are exact there, so the sign
+ * is exact and swapping the edge's endpoints negates it exactly */
+static inline int edge_sign(float px, float py, float qx, float qy, float dx, float dy) {
+  const double ed = (double)px * (double)qy - (double)py * (double)qx;
+  if (ed > 0.0) return 1;
+  if (ed < 0.0) return -1;
+  return tie_pos(dx, dy) ? 1 : -1;
+}
+
+/* Watertight ray / triangle (Woop, Benthin, Wald 2013).  Returns 0 for no hit, else the common
+ * sign s of the edge functions; *t receives the hit distance (only meaningful if hit). */
+typedef struct {
+  int kx, ky, kz;
+  float sx, sy, sz;
+} ray_shear;
+
+static void make_shear(const float d[3], ray_shear *r) {
+  int kz = 0;
+  if (fabsf(d[1]) > fabsf(d[kz])) kz = 1;
+  if (fabsf(d[2]) > fabsf(d[kz])) kz = 2;
+  int kx = kz + 1 == 3 ? 0 : kz + 1;
+  int ky = kx + 1 == 3 ? 0 : kx + 1;
+  if (d[kz] < 0.0f) {
+    const int tmp = kx;
+    kx = ky;
+    ky = tmp;
+  }
+  r->kx = kx;
+  r->ky = ky;
+  r->kz = kz;
+  r->sx = d[kx] / d[kz];
+  r->sy = d[ky] / d[kz];
+  r->sz = 1.0f / d[kz];
+}
+
+static int ray_tri(const float o[3], const ray_shear *r, const float *v, double *tout) {
+  float A[3], B[3], C[3];
+  for (int k = 0; k < 3; ++k) {
+    A[k] = v[k] - o[k];
+    B[k] = v[3 + k] - o[k];
+    C[k] = v[6 + k] - o[k];
+  }
+  const float ax = A[r->kx] - r->sx * A[r->kz], ay = A[r->ky] - r->sy * A[r->kz];
+  const float bx = B[r->kx] - r->sx * B[r->kz], by = B[r->ky] - r->sy * B[r->kz];
+  const float cx = C[r->kx] - r->sx * C[r->kz], cy = C[r->ky] - r->sy * C[r->kz];
+  /* U: edge B->C, V: edge C->A, W: edge A->B */
+  const int su = edge_sign(cx, cy, bx, by, cx - bx, cy - by);
+  const int sv = edge_sign(ax, ay, cx, cy, ax - cx, ay - cy);
+  const int sw = edge_sign(bx, by, ax, ay, bx - ax, by - ay);
+  if (su != sv || sv != sw) return 0;
+  const double U = (double)cx * by - (double)cy * bx;
+  const double V = (double)ax * cy - (double)ay * cx;
+  const double W = (double)bx * ay - (double)by * ax;
+  const double det = U + V + W;
+  if (det == 0.0) return 0;
+  const double az = (double)r->sz * A[r->kz], bz = (double)r->sz * B[r->kz], cz = (double)r->sz * C[r->kz];
+  *tout = (U * az + V * bz + W * cz) / det;
+  return su;
+}
+
+int orc_crossings_up(const float q[3], const float *xyz, size_t ntri) {
+  const float d[3] = {0.0f, 0.0f, 1.0f};
+  ray_shear r;
+  make_shear(d, &r);
+  int w = 0;
+  for (size_t t = 0; t < ntri; ++t) {
+    double th;
+    const int s = ray_tri(q, &r, xyz + 9 * t, &th);
+    if (s != 0 && th > 0.0) w += -s; /* s<0 <=> normal.z>0 (exit) -> +1 */
+  }
+  return w;
+}
+
+void orc_parity_dirs(float dirs[15]) {
+  /* +z first (sdf_bake.cu:56), then four fixed generic unit directions */
+  static const float tab[15] = {0.0f,         0.0f,         1.0f,         0.5773502588f, 0.5773502588f,
+                                -0.5773502588f, -0.6666666865f, 0.3333333433f, 0.6666666865f, 0.2672612369f,
+                                -0.8017837405f, -0.5345224738f, -0.4850712717f, -0.7276068926f, 0.4850712717f};
+  memcpy(dirs, tab, sizeof(tab));
+}
+
+int orc_parity_inside(const float q[3], const float *xyz, size_t ntri) {
+  float dirs[15];
+  orc_parity_dirs(dirs);
+  int votes = 0;
+  for (int v = 0; v < 5; ++v) {
+    ray_shear r;
+    make_shear(dirs + 3 * v, &r);
+    int hits = 0;
+    for (size_t t = 0; t < ntri; ++t) {
+      double th;
+      if (ray_tri(q, &r, xyz + 9 * t, &th) != 0 && th > 0.0) hits++;
+    }
+    if (hits & 1) votes++; /* sdf_bake.cu:66-67 */
+  }
+  return votes * 2 > 5; /* sdf_bake.cu:69 */
+}
+
+/* ============================================================================
+ * LBVH (Karras 2012)
+ * ========================================================================== */
+static uint32_t expand10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+static void centroid(const float *t, float c[3]) {
+  for (int k = 0; k < 3; ++k) c[k] = ((t[k] + t[3 + k]) + t[6 + k]) / 3.0f;
+}
+
+void orc_morton_codes(const float *xyz, size_t n, uint32_t *codes) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < n; ++i) {
+    float c[3];
+    centroid(xyz + 9 * i, c);
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = c[k] < lo[k] ? c[k] : lo[k];
+      hi[k] = c[k] > hi[k] ? c[k] : hi[k];
+    }
+  }
+  float ext[3];
+  for (int k = 0; k < 3; ++k) ext[k] = hi[k] - lo[k];
+  for (size_t i = 0; i < n; ++i) {
+    float c[3];
+    centroid(xyz + 9 * i, c);
+    uint32_t q[3];
+    for (int k = 0; k < 3; ++k) {
+      const float u = ext[k] > 0.0f ? (c[k] - lo[k]) / ext[k] : 0.5f;
+      float s = u * 1024.0f;
+      s = s < 0.0f ? 0.0f : (s > 1023.0f ? 1023.0f : s);
+      q[k] = (uint32_t)s;
+    }
+    codes[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  }
+}
+
+static int clz64(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
+
+typedef struct {
+  const uint32_t *codes;
+  int n;
+} karras_ctx;
+
+static int kdelta(const karras_ctx *k, int i, int j) {
+  if (j < 0 || j >= k->n) return -1;
+  const uint64_t a = ((uint64_t)k->codes[i] << 32) | (uint32_t)i;
+  const uint64_t b = ((uint64_t)k->codes[j] << 32) | (uint32_t)j;
+  return clz64(a ^ b);
+}
+
+static int cmp_pair(const void *pa, const void *pb) {
+  const uint64_t a = *(const uint64_t *)pa, b = *(const uint64_t *)pb;
+  return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+int orc_bvh_build(const float *xyz, size_t ntri, orc_bvh *b) {
+  memset(b, 0, sizeof(*b));
+  const int n = (int)ntri;
+  b->n = n;
+  if (n <= 0) return -1;
+  uint32_t *codes = (uint32_t *)malloc(sizeof(uint32_t) * n);
+  orc_morton_codes(xyz, ntri, codes);
+  /* stable sort by code: sort (code<<32 | index) */
+  uint64_t *kv = (uint64_t *)malloc(sizeof(uint64_t) * n);
+  for (int i = 0; i < n; ++i) kv[i] = ((uint64_t)codes[i] << 32) | (uint32_t)i;
+  qsort(kv, n, sizeof(uint64_t), cmp_pair);
+  b->order = (int32_t *)malloc(sizeof(int32_t) * n);
+  b->codes = (uint32_t *)malloc(sizeof(uint32_t) * n);
+  for (int i = 0; i < n; ++i) {
+    b->order[i] = (int32_t)(kv[i] & 0xffffffffu);
+    b->codes[i] = (uint32_t)(kv[i] >> 32);
+  }
+  free(kv);
+  free(codes);
+  const int ni = n - 1;
+  b->left = (int32_t *)malloc(sizeof(int32_t) * (ni > 0 ? ni : 1));
+  b->right = (int32_t *)malloc(sizeof(int32_t) * (ni > 0 ? ni : 1));
+  b->parent = (int32_t *)malloc(sizeof(int32_t) * (2 * n - 1));
+  b->lo = (float *)malloc(sizeof(float) * 3 * (2 * n - 1));
+  b->hi = (float *)malloc(sizeof(float) * 3 * (2 * n - 1));
+  for (int i = 0; i < 2 * n - 1; ++i) b->parent[i] = -1;
+  karras_ctx k = {b->codes, n};
+  for (int i = 0; i < ni; ++i) {
+    const int d = (kdelta(&k, i, i + 1) - kdelta(&k, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = kdelta(&k, i, i - d);
+    int lmax = 2;
+    while (kdelta(&k, i, i + lmax * d) > dmin) lmax *= 2;
+    int l = 0;
+    for (int t = lmax / 2; t >= 1; t /= 2)
+      if (kdelta(&k, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = kdelta(&k, i, j);
+    int s = 0, t = l;
+    do {
+      t = (t + 1) >> 1;
+      if (kdelta(&k, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int gamma = i + s * d + (d < 0 ? d : 0);
+    const int lo = i < j ? i : j, hi = i < j ? j : i;
+    const int lc = (lo == gamma) ? ~gamma : gamma;
+    const int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+    b->left[i] = lc;
+    b->right[i] = rc;
+    b->parent[lc >= 0 ? lc : ni + ~lc] = i;
+    b->parent[rc >= 0 ? rc : ni + ~rc] = i;
+  }
+  /* leaf boxes */
+  for (int i = 0; i < n; ++i) {
+    const float *t = xyz + 9 * (size_t)b->order[i];
+    float *lo = b->lo + 3 * (ni + i), *hi = b->hi + 3 * (ni + i);
+    for (int c = 0; c < 3; ++c) {
+      lo[c] = smin(smin(t[c], t[3 + c]), t[6 + c]);
+      hi[c] = smax(smax(t[c], t[3 + c]), t[6 + c]);
+    }
+  }
+  /* refit: post-order by explicit stack */
+  if (ni > 0) {
+    int *st = (int *)malloc(sizeof(int) * (2 * n + 8));
+    char *done = (char *)calloc(ni, 1);
+    int sp = 0;
+    st[sp++] = 0;
+    while (sp) {
+      const int x = st[sp - 1];
+      const int c[2] = {b->left[x], b->right[x]};
+      int pushed = 0;
+      if (!done[x]) {
+        done[x] = 1;
+        for (int q = 0; q < 2; ++q)
+          if (c[q] >= 0) {
+            st[sp++] = c[q];
+            pushed = 1;
+          }
+        if (pushed) continue;
+      }
+      sp--;
+      const int ia = c[0] >= 0 ? c[0] : ni + ~c[0], ib = c[1] >= 0 ? c[1] : ni + ~c[1];
+      for (int q = 0; q < 3; ++q) {
+        b->lo[3 * x + q] = smin(b->lo[3 * ia + q], b->lo[3 * ib + q]);
+        b->hi[3 * x + q] = smax(b->hi[3 * ia + q], b->hi[3 * ib + q]);
+      }
+    }
+    free(st);
+    free(done);
+  }
+  /* depth */
+  int depth = 0;
+  for (int i = 0; i < n; ++i) {
+    int dd = 0, x = ni + i;
+    while (b->parent[x] >= 0) {
+      x = b->parent[x];
+      dd++;
+    }
+    if (dd > depth) depth = dd;
+  }
+  b->depth = depth;
+  return 0;
+}
+
+void orc_bvh_free(orc_bvh *b) {
+  free(b->order);
+  free(b->codes);
+  free(b->left);
+  free(b->right);
+  free(b->parent);
+  free(b->lo);
+  free(b->hi);
+  memset(b, 0, sizeof(*b));
+}
+
+/* ============================================================================
+ * dense CPU bake
+ * ========================================================================== */
+static inline float box_dist2(const float q[3], const float *lo, const float *hi) {
+  float s = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    float d = 0.0f;
+    if (q[k] < lo[k]) d = lo[k] - q[k];
+    else if (q[k] > hi[k]) d = q[k] - hi[k];
+    s += d * d;
+  }
+  return s;
+}
+
+typedef struct {
+  const float *tris; /* sorted soup, 9 floats */
+  const orc_bvh *b;
+} qctx;
+
+static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t *tests) {
+  const orc_bvh *b = c->b;
+  const int ni = b->n - 1;
+  float best = INFINITY;
+  if (ni == 0) {
+    (*tests)++;
+    return orc_tri_dist2_f32(q, c->tris);
+  }
+  int stack[256];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp) {
+    const int x = stack[--sp];
+    const int ch[2] = {b->left[x], b->right[x]};
+    float dd[2];
+    for (int k = 0; k < 2; ++k) {
+      const int id = ch[k] >= 0 ? ch[k] : ni + ~ch[k];
+      dd[k] = box_dist2(q, b->lo + 3 * id, b->hi + 3 * id);
+      (*nodes)++;
+    }
+    const int first = dd[1] < dd[0] ? 1 : 0;
+    for (int o = 1; o >= 0; --o) { /* push far first so near is popped first */
+      const int k = o ? 1 - first : first;
+      if (!(dd[k] <= best)) continue;
+      if (ch[k] < 0) {
+        (*tests)++;
+        const float d2 = orc_tri_dist2_f32(q, c->tris + 9 * (size_t)(~ch[k]));
+        if (d2 < best) best = d2;
+      } else {
+        stack[sp++] = ch[k];
+      }
+    }
+  }
+  return best;
+}
+
+typedef struct {
+  float z;
+  int s;
+} crossing;
+
+static int cmp_cross(const void *a, const void *b) {
+  const float za = ((const crossing *)a)->z, zb = ((const crossing *)b)->z;
+  return za < zb ? -1 : (za > zb ? 1 : 0);
+}
+
+/* all crossings of the vertical line through (x,y): exact column z and sign */
+static int column_crossings(const qctx *c, float x, float y, crossing **out, int *cap, uint64_t *tests) {
+  const orc_bvh *b = c->b;
+  const int ni = b->n - 1;
+  const float o[3] = {x, y, 0.0f};
+  const float d[3] = {0.0f, 0.0f, 1.0f};
+  ray_shear r;
+  make_shear(d, &r);
+  int cnt = 0;
+  int stack[256];
+  int sp = 0;
+  if (ni == 0) stack[sp++] = ~0;
+  else stack[sp++] = 0;
+  while (sp) {
+    const int x0 = stack[--sp];
+    const int id = x0 >= 0 ? x0 : ni + ~x0;
+    const float *lo = b->lo + 3 * id, *hi = b->hi + 3 * id;
+    if (!(x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1])) continue;
+    if (x0 >= 0) {
+      stack[sp++] = b->left[x0];
+      stack[sp++] = b->right[x0];
+      continue;
+    }
+    (*tests)++;
+    double th;
+    const int s = ray_tri(o, &r, c->tris + 9 * (size_t)(~x0), &th);
+    if (s == 0) continue;
+    if (cnt == *cap) {
+      *cap = *cap ? *cap * 2 : 16;
+      *out = (crossing *)realloc(*out, sizeof(crossing) * (*cap));
+    }
+    (*out)[cnt].z = (float)th;
+    (*out)[cnt].s = -s;
+    cnt++;
+  }
+  qsort(*out, cnt, sizeof(crossing), cmp_cross);
+  return cnt;
+}
+
+typedef struct {
+  const qctx *c;
+  const float *origin;
+  float h;
+  int nx, ny, nz, z0, z1;
+  float *dist;
+  int next_row;
+  pthread_mutex_t *mu;
+  uint64_t nodes, tests, ctests, inside;
+} bake_job;
+
+typedef struct {
+  bake_job *job;
+  uint64_t nodes, tests, ctests, inside;
+} bake_thread;
+
+static void *bake_thread_fn(void *arg) {
+  bake_thread *th = (bake_thread *)arg;
+  bake_job *j = th->job;
+  crossing *cr = NULL;
+  int cap = 0;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const int y = j->next_row++;
+    pthread_mutex_unlock(j->mu);
+    if (y >= j->ny) break;
+    const float py = j->origin[1] + ((float)y + 0.5f) * j->h;
+    for (int x = 0; x < j->nx; ++x) {
+      const float px = j->origin[0] + ((float)x + 0.5f) * j->h;
+      const int nc = column_crossings(j->c, px, py, &cr, &cap, &th->ctests);
+      for (int z = j->z0; z < j->z1; ++z) {
+        const float q[3] = {px, py, j->origin[2] + ((float)z + 0.5f) * j->h};
+        const float d2 = closest2(j->c, q, &th->nodes, &th->tests);
+        int w = 0;
+        for (int k = 0; k < nc; ++k)
+          if (cr[k].z > q[2]) w += cr[k].s;
+        const float d = sqrtf(d2);
+        const int inside = w != 0; /* |w| > 0.5: orientation-agnostic, like the parity sign */
+        th->inside += (uint64_t)inside;
+        j->dist[((size_t)z * j->ny + y) * j->nx + x] = inside ? -d : d;
+      }
+    }
+  }
+  free(cr);
+  return NULL;
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz, int z0,
+                   int z1, int threads, float *dist, orc_bake_stats *st) {
+  if (orc_boundary_edge_count(xyz, ntri) != 0) return -1;
+  const double t0 = now_s();
+  orc_bvh b;
+  if (orc_bvh_build(xyz, ntri, &b) != 0) return -2;
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  qctx c = {sorted, &b};
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (z0 < 0) z0 = 0;
+  if (z1 > nz || z1 < 0) z1 = nz;
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  bake_job job = {&c, origin, h, nx, ny, nz, z0, z1, dist, 0, &mu, 0, 0, 0, 0};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  bake_thread *ths = (bake_thread *)calloc(threads, sizeof(bake_thread));
+  for (int i = 0; i < threads; ++i) {
+    ths[i].job = &job;
+    pthread_create(&tid[i], NULL, bake_thread_fn, &ths[i]);
+  }
+  uint64_t nodes = 0, tests = 0, ctests = 0, inside = 0;
+  for (int i = 0; i < threads; ++i) {
+    pthread_join(tid[i], NULL);
+    nodes += ths[i].nodes;
+    tests += ths[i].tests;
+    ctests += ths[i].ctests;
+    inside += ths[i].inside;
+  }
+  const double t1 = now_s();
+  if (st) {
+    st->voxels = (uint64_t)nx * ny * (uint64_t)(z1 - z0);
+    st->node_visits = nodes;
+    st->tri_tests = tests;
+    st->column_tri_tests = ctests;
+    st->inside = inside;
+    st->seconds = t1 - t0;
+    st->threads = threads;
+  }
+  free(tid);
+  free(ths);
+  free(sorted);
+  orc_bvh_free(&b);
+  return 0;
+}
+
+typedef struct {
+  const float *xyz;
+  size_t ntri;
+  const double *pts;
+  size_t npts;
+  double *d, *w;
+  size_t next;
+  pthread_mutex_t *mu;
+} f64_job;
+
+static void *f64_fn(void *arg) {
+  f64_job *j = (f64_job *)arg;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 64;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 64 < j->npts ? i0 + 64 : j->npts;
+    for (size_t i = i0; i < i1; ++i) {
+      const double *q = j->pts + 3 * i;
+      double best = INFINITY;
+      for (size_t t = 0; t < j->ntri; ++t) {
+        const double d2 = orc_tri_dist2_f64(q, j->xyz + 9 * t);
+        if (d2 < best) best = d2;
+      }
+      const double w = orc_gwn_f64(q, j->xyz, j->ntri);
+      j->w[i] = w;
+      j->d[i] = fabs(w) > 0.5 ? -sqrt(best) : sqrt(best);
+    }
+  }
+  return NULL;
+}
+
+void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_d,
+                        double *out_w) {
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  f64_job j = {xyz, ntri, pts, npts, out_d, out_w, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, f64_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+}
+
+/* ============================================================================
+ * closedness: weld by exact value (-0 == +0), cancel a->b against b->a
+ * ========================================================================== */
+static uint32_t canon_bits(float f) {
+  if (f == 0.0f) f = 0.0f;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  return u;
+}
+
+int64_t orc_boundary_edge_count(const float *xyz, size_t ntri) {
+  /* weld: map position -> id */
+  const size_t nv = 3 * ntri;
+  size_t cap = 16;
+  while (cap < nv * 2) cap <<= 1;
+  uint32_t *kx = (uint32_t *)malloc(cap * 4), *ky = (uint32_t *)malloc(cap * 4), *kz = (uint32_t *)malloc(cap * 4);
+  int64_t *kid = (int64_t *)malloc(cap * 8);
+  for (size_t i = 0; i < cap; ++i) kid[i] = -1;
+  int64_t *vid = (int64_t *)malloc(nv * 8);
+  int64_t nextid = 0;
+  for (size_t i = 0; i < nv; ++i) {
+    const uint32_t a = canon_bits(xyz[3 * i]), b = canon_bits(xyz[3 * i + 1]), c = canon_bits(xyz[3 * i + 2]);
+    size_t s = mix64(((uint64_t)a << 32 | b) ^ mix64(c)) & (cap - 1);
+    for (;;) {
+      if (kid[s] == -1) {
+        kx[s] = a;
+        ky[s] = b;
+        kz[s] = c;
+        kid[s] = nextid++;
+        break;
+      }
+      if (kx[s] == a && ky[s] == b && kz[s] == c) break;
+      s = (s + 1) & (cap - 1);
+    }
+    vid[i] = kid[s];
+  }
+  free(kx);
+  free(ky);
+  free(kz);
+  free(kid);
+  /* net directed count per undirected edge */
+  i64map m;
+  map_init(&m, nv);
+  size_t count = 0;
+  for (size_t t = 0; t < ntri; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const int64_t a = vid[3 * t + k], b = vid[3 * t + (k + 1) % 3];
+      if (a == b) continue;
+      const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+      int ins;
+      int64_t *v = map_get(&m, (lo << 31) ^ hi, &ins, &count);
+      if (ins) *v = 0;
+      *v += (a < b) ? 1 : -1;
+    }
+  int64_t bnd = 0;
+  for (size_t i = 0; i < m.cap; ++i)
+    if (m.keys[i] != -1) bnd += m.vals[i] < 0 ? -m.vals[i] : m.vals[i];
+  map_free(&m);
+  free(vid);
+  return bnd;
+}
+
+/* FP32 brute force unsigned squared distance: min over all triangles of orc_tri_dist2_f32 */
+void orc_udf2_f32_brute(const float *xyz, size_t ntri, const float *pts, size_t npts, float *out) {
+  for (size_t i = 0; i < npts; ++i) {
+    float best = INFINITY;
+    for (size_t t = 0; t < ntri; ++t) {
+      const float d2 = orc_tri_dist2_f32(pts + 3 * i, xyz + 9 * t);
+      if (d2 < best) best = d2;
+    }
+    out[i] = best;
+  }
+}

@@ -1,0 +1,123 @@
+/*
+ * CPU oracle for the mesh -> SDF / occupancy bake path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product (italy-rs_amd/) never links, loads or calls it.
+ *
+ * It restates, in plain C99 compiled with -ffp-contract=off:
+ *   - triBoxOverlap          src/convert/tri_box_overlap.h:12-114 (literal expression trees)
+ *   - voxelizeMesh           src/convert/voxelize.cpp:52-107 (framing, clamp, first-touch order)
+ *   - bakeSdf grid framing   src/convert/sdf_baker.cpp:137-148
+ *   - cell-centre sampling   src/render/kernels/sdf_bake.cu:34-37, index :71, sign :69
+ *   - parity sign vote       src/render/kernels/sdf_bake.cu:53-69 (+z first, majority of 5)
+ * and the exact quantities the new bake computes:
+ *   - closest-point distance (FP32 in query-local coordinates, FP64 for slivers; FP64 reference)
+ *   - generalized winding number (FP64 brute-force solid angles, Van Oosterom-Strackee 1983)
+ *   - signed watertight crossing count along +z columns (Woop et al. 2013 edge functions)
+ *   - LBVH (Karras 2012) over 30-bit Morton codes, same integer topology the device builds.
+ *
+ * How each function is pinned is listed in oracle/PINNING.md; a function without a
+ * pin there is "parity unpinned".
+ */
+#ifndef IB_ORACLE_H
+#define IB_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- a10: triangle / AABB separating-axis test ------------------------------ */
+int orc_tri_box_overlap(const float center[3], const float half[3], const float t0[3], const float t1[3],
+                        const float t2[3]);
+
+/* ---- a1 / a9: framing -------------------------------------------------------- */
+/* bakeSdf framing: sdf_baker.cpp:139-148 */
+void orc_sdf_framing(const float bmin[3], const float bmax[3], int resolution, float origin[3], float *h, int n[3]);
+/* voxelizeMesh framing: voxelize.cpp:54-57 */
+void orc_vox_framing(const float bmin[3], const float bmax[3], int resolution, float origin[3], float *h);
+
+/* ---- a9: voxelizeMesh, first-touch order --------------------------------------
+ * xyz: 9 floats per triangle (soup).  colors: 3 floats per triangle (may be NULL -> 1,1,1).
+ * Writes up to cap cells (3 ints each, x,y,z) and their averaged colours; returns the count.
+ * res_clamp is resolution-1 on every axis, as voxelize.cpp:72-77. */
+size_t orc_voxelize_first_touch(const float *xyz, size_t ntri, const float *colors, const float bmin[3],
+                                const float bmax[3], int resolution, int32_t *cells, float *cell_colors, size_t cap);
+
+/* Bit-packed occupancy over an explicit grid (origin, h, nx,ny,nz): same per-triangle cell range
+ * (clamped to [0,n-1] per axis), same centre/half, same SAT.  Bit x&31 of word
+ * ((z*ny + y)*((nx+31)/32) + x/32).  words must hold ny*nz*((nx+31)/32) zeroed words. */
+void orc_occupancy(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
+                   uint32_t *words);
+
+/* ---- distance ----------------------------------------------------------------- */
+/* 1 if the triangle takes the FP64 path (near-degenerate: sin^2 of its corner angle < 1e-4). */
+int orc_tri_is_sliver(const float *tri9);
+/* squared distance point->triangle: FP32 in query-local coordinates, FP64 for slivers */
+float orc_tri_dist2_f32(const float q[3], const float *tri9);
+/* squared distance point->triangle, all FP64 (min over edges + interior projection) */
+double orc_tri_dist2_f64(const double q[3], const float *tri9);
+
+/* ---- winding number ----------------------------------------------------------- */
+/* FP64 brute force generalized winding number: sum of solid angles / 4pi */
+double orc_gwn_f64(const double q[3], const float *xyz, size_t ntri);
+/* Signed crossing count of the +z ray from (x,y,z) with the triangles (Woop edge functions with
+ * exact zero handling and a half-open tie rule).  For a closed consistently oriented mesh this
+ * equals the winding number of the point. */
+int orc_crossings_up(const float q[3], const float *xyz, size_t ntri);
+/* Parity vote (sdf_bake.cu:53-69 structure): 5 directions, dir 0 = +z, the rest from a fixed
+ * table; each ray counts hits with t > 0; inside if the majority of rays see odd counts. */
+int orc_parity_inside(const float q[3], const float *xyz, size_t ntri);
+/* the 5 tie-break directions used by both the oracle and the device code */
+void orc_parity_dirs(float dirs[15]);
+
+/* ---- LBVH (Karras 2012) --------------------------------------------------------- */
+typedef struct {
+  int32_t n;          /* triangles */
+  int32_t *order;     /* n: sorted position -> original triangle index */
+  uint32_t *codes;    /* n: sorted Morton codes */
+  int32_t *left;      /* n-1: child of internal node i; >=0 internal, <0 = ~leaf */
+  int32_t *right;     /* n-1 */
+  int32_t *parent;    /* 2n-1: parent of internal (0..n-2) and leaves (n-1+leaf) */
+  float *lo, *hi;     /* (2n-1)*3: boxes of internal nodes then leaves */
+  int32_t depth;      /* max leaf depth (root = 0) */
+} orc_bvh;
+
+int orc_bvh_build(const float *xyz, size_t ntri, orc_bvh *out);
+void orc_bvh_free(orc_bvh *b);
+/* Morton code of each triangle centroid (unsorted), same arithmetic as the device build */
+void orc_morton_codes(const float *xyz, size_t ntri, uint32_t *codes);
+
+/* ---- full CPU bake (the host-core baseline) ------------------------------------ */
+typedef struct {
+  uint64_t voxels;
+  uint64_t node_visits;   /* distance traversal box tests */
+  uint64_t tri_tests;     /* distance triangle tests */
+  uint64_t column_tri_tests;
+  uint64_t inside;
+  double seconds;
+  int threads;
+} orc_bake_stats;
+
+/* Dense SDF over an explicit cell-centred grid (index (z*ny+y)*nx+x, negative inside), using the
+ * LBVH for the closest triangle and +z column crossings for the sign.  Only closed meshes are
+ * signed exactly this way; the function returns -1 if the mesh has boundary edges.
+ * z_begin/z_end restrict the slabs computed (for bounded samples); threads<=0 -> all cores. */
+int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
+                   int z_begin, int z_end, int threads, float *dist, orc_bake_stats *stats);
+
+/* FP64 brute-force signed distance at arbitrary points (inside where |GWN| > 0.5, FP64) */
+void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_dist,
+                        double *out_w);
+
+/* FP32 brute force: out[i] = min over triangles of orc_tri_dist2_f32(pts[i], tri) */
+void orc_udf2_f32_brute(const float *xyz, size_t ntri, const float *pts, size_t npts, float *out);
+
+/* number of directed edges left unmatched after exact-position welding (0 => closed) */
+int64_t orc_boundary_edge_count(const float *xyz, size_t ntri);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,253 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's
+cpu_baseline leg as the checker.  The product package never imports this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+
+class BakeStats(ctypes.Structure):
+    _fields_ = [
+        ("voxels", ctypes.c_uint64),
+        ("node_visits", ctypes.c_uint64),
+        ("tri_tests", ctypes.c_uint64),
+        ("column_tri_tests", ctypes.c_uint64),
+        ("inside", ctypes.c_uint64),
+        ("seconds", ctypes.c_double),
+        ("threads", ctypes.c_int),
+    ]
+
+
+class Bvh(ctypes.Structure):
+    _fields_ = [
+        ("n", ctypes.c_int32),
+        ("order", _i32p),
+        ("codes", _u32p),
+        ("left", _i32p),
+        ("right", _i32p),
+        ("parent", _i32p),
+        ("lo", _f32p),
+        ("hi", _f32p),
+        ("depth", ctypes.c_int32),
+    ]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        build()
+    lib = ctypes.CDLL(_LIB)
+    lib.orc_tri_box_overlap.argtypes = [_f32p] * 5
+    lib.orc_tri_box_overlap.restype = ctypes.c_int
+    lib.orc_sdf_framing.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, _f32p, _i32p]
+    lib.orc_vox_framing.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, _f32p]
+    lib.orc_voxelize_first_touch.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, _f32p, ctypes.c_int, _i32p,
+                                             _f32p, ctypes.c_size_t]
+    lib.orc_voxelize_first_touch.restype = ctypes.c_size_t
+    lib.orc_occupancy.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _u32p]
+    lib.orc_tri_is_sliver.argtypes = [_f32p]
+    lib.orc_tri_dist2_f32.argtypes = [_f32p, _f32p]
+    lib.orc_tri_dist2_f32.restype = ctypes.c_float
+    lib.orc_tri_dist2_f64.argtypes = [_f64p, _f32p]
+    lib.orc_tri_dist2_f64.restype = ctypes.c_double
+    lib.orc_gwn_f64.argtypes = [_f64p, _f32p, ctypes.c_size_t]
+    lib.orc_gwn_f64.restype = ctypes.c_double
+    lib.orc_crossings_up.argtypes = [_f32p, _f32p, ctypes.c_size_t]
+    lib.orc_parity_inside.argtypes = [_f32p, _f32p, ctypes.c_size_t]
+    lib.orc_parity_dirs.argtypes = [_f32p]
+    lib.orc_bvh_build.argtypes = [_f32p, ctypes.c_size_t, ctypes.POINTER(Bvh)]
+    lib.orc_bvh_free.argtypes = [ctypes.POINTER(Bvh)]
+    lib.orc_morton_codes.argtypes = [_f32p, ctypes.c_size_t, _u32p]
+    lib.orc_bake_dense.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p,
+                                   ctypes.POINTER(BakeStats)]
+    lib.orc_sdf_f64_points.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p, _f64p]
+    lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
+    lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
+    lib.orc_boundary_edge_count.restype = ctypes.c_int64
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def tri_box_overlap(center, half, t0, t1, t2):
+    args = [_f32(x) for x in (center, half, t0, t1, t2)]
+    return bool(lib().orc_tri_box_overlap(*[_p(a, _f32p) for a in args]))
+
+
+def sdf_framing(bmin, bmax, res):
+    o = np.zeros(3, np.float32)
+    h = np.zeros(1, np.float32)
+    n = np.zeros(3, np.int32)
+    lib().orc_sdf_framing(_p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p), int(res), _p(o, _f32p), _p(h, _f32p),
+                          _p(n, _i32p))
+    return o, np.float32(h[0]), tuple(int(v) for v in n)
+
+
+def vox_framing(bmin, bmax, res):
+    o = np.zeros(3, np.float32)
+    h = np.zeros(1, np.float32)
+    lib().orc_vox_framing(_p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p), int(res), _p(o, _f32p), _p(h, _f32p))
+    return o, np.float32(h[0])
+
+
+def voxelize_first_touch(xyz, bmin, bmax, res, colors=None):
+    xyz = _f32(xyz)
+    ntri = xyz.size // 9
+    cp = _p(_f32(colors), _f32p) if colors is not None else None
+    n = lib().orc_voxelize_first_touch(_p(xyz, _f32p), ntri, cp, _p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p),
+                                       int(res), None, None, 0)
+    cells = np.zeros((max(n, 1), 3), np.int32)
+    cols = np.zeros((max(n, 1), 3), np.float32)
+    lib().orc_voxelize_first_touch(_p(xyz, _f32p), ntri, cp, _p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p),
+                                   int(res), _p(cells, _i32p), _p(cols, _f32p), n)
+    return cells[:n], cols[:n]
+
+
+def occupancy(xyz, origin, h, nx, ny, nz):
+    xyz = _f32(xyz)
+    words = np.zeros(((nx + 31) // 32) * ny * nz, np.uint32)
+    lib().orc_occupancy(_p(xyz, _f32p), xyz.size // 9, _p(_f32(origin), _f32p), float(h), nx, ny, nz,
+                        _p(words, _u32p))
+    return words
+
+
+def tri_dist2_f32(q, tri):
+    return float(lib().orc_tri_dist2_f32(_p(_f32(q), _f32p), _p(_f32(tri), _f32p)))
+
+
+def tri_dist2_f64(q, tri):
+    return float(lib().orc_tri_dist2_f64(_p(np.ascontiguousarray(q, np.float64), _f64p), _p(_f32(tri), _f32p)))
+
+
+def tri_is_sliver(tri):
+    return bool(lib().orc_tri_is_sliver(_p(_f32(tri), _f32p)))
+
+
+def gwn_f64(q, xyz):
+    xyz = _f32(xyz)
+    return float(lib().orc_gwn_f64(_p(np.ascontiguousarray(q, np.float64), _f64p), _p(xyz, _f32p), xyz.size // 9))
+
+
+def crossings_up(q, xyz):
+    xyz = _f32(xyz)
+    return int(lib().orc_crossings_up(_p(_f32(q), _f32p), _p(xyz, _f32p), xyz.size // 9))
+
+
+def parity_inside(q, xyz):
+    xyz = _f32(xyz)
+    return bool(lib().orc_parity_inside(_p(_f32(q), _f32p), _p(xyz, _f32p), xyz.size // 9))
+
+
+def parity_dirs():
+    d = np.zeros(15, np.float32)
+    lib().orc_parity_dirs(_p(d, _f32p))
+    return d.reshape(5, 3)
+
+
+def morton_codes(xyz):
+    xyz = _f32(xyz)
+    out = np.zeros(xyz.size // 9, np.uint32)
+    lib().orc_morton_codes(_p(xyz, _f32p), xyz.size // 9, _p(out, _u32p))
+    return out
+
+
+def bvh(xyz):
+    """Karras LBVH: returns dict of numpy arrays (order, codes, left, right, parent, lo, hi, depth)."""
+    xyz = _f32(xyz)
+    b = Bvh()
+    n = xyz.size // 9
+    if lib().orc_bvh_build(_p(xyz, _f32p), n, ctypes.byref(b)) != 0:
+        raise ValueError("empty mesh")
+    ni = max(n - 1, 0)
+    out = {
+        "order": np.ctypeslib.as_array(b.order, (n,)).copy(),
+        "codes": np.ctypeslib.as_array(b.codes, (n,)).copy(),
+        "left": np.ctypeslib.as_array(b.left, (max(ni, 1),)).copy()[:ni],
+        "right": np.ctypeslib.as_array(b.right, (max(ni, 1),)).copy()[:ni],
+        "parent": np.ctypeslib.as_array(b.parent, (2 * n - 1,)).copy(),
+        "lo": np.ctypeslib.as_array(b.lo, (3 * (2 * n - 1),)).copy().reshape(-1, 3),
+        "hi": np.ctypeslib.as_array(b.hi, (3 * (2 * n - 1),)).copy().reshape(-1, 3),
+        "depth": int(b.depth),
+    }
+    lib().orc_bvh_free(ctypes.byref(b))
+    return out
+
+
+def bake_dense(xyz, origin, h, nx, ny, nz, z0=0, z1=-1, threads=0):
+    """Dense signed distance grid (z, y, x) with the oracle's BVH bake; None if the mesh is open."""
+    xyz = _f32(xyz)
+    dist = np.full((nz, ny, nx), np.nan, np.float32)
+    st = BakeStats()
+    rc = lib().orc_bake_dense(_p(xyz, _f32p), xyz.size // 9, _p(_f32(origin), _f32p), float(h), nx, ny, nz, z0, z1,
+                              threads, _p(dist, _f32p), ctypes.byref(st))
+    if rc != 0:
+        return None, None
+    return dist, {k: getattr(st, k) for k, _ in BakeStats._fields_}
+
+
+def sdf_f64_points(xyz, pts, threads=0):
+    """FP64 brute-force signed distance (sign from |w| > 0.5) and winding number at points."""
+    xyz = _f32(xyz)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    d = np.zeros(len(pts), np.float64)
+    w = np.zeros(len(pts), np.float64)
+    lib().orc_sdf_f64_points(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f64p), len(pts), threads, _p(d, _f64p),
+                             _p(w, _f64p))
+    return d, w
+
+
+def boundary_edge_count(xyz):
+    xyz = _f32(xyz)
+    return int(lib().orc_boundary_edge_count(_p(xyz, _f32p), xyz.size // 9))
+
+
+def udf2_f32_brute(xyz, pts):
+    xyz = _f32(xyz)
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float32)
+    lib().orc_udf2_f32_brute(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f32p), len(pts), _p(out, _f32p))
+    return out
+
+
+def cell_centres(origin, h, nx, ny, nz):
+    """(nz, ny, nx, 3) float32 centres origin + (i + 0.5) * h, evaluated in float32 as the bake does."""
+    o = np.asarray(origin, np.float32)
+    h = np.float32(h)
+    zs = o[2] + (np.arange(nz, dtype=np.float32) + np.float32(0.5)) * h
+    ys = o[1] + (np.arange(ny, dtype=np.float32) + np.float32(0.5)) * h
+    xs = o[0] + (np.arange(nx, dtype=np.float32) + np.float32(0.5)) * h
+    zz, yy, xx = np.meshgrid(zs, ys, xs, indexing="ij")
+    return np.stack([xx, yy, zz], -1).astype(np.float32)

@@ -1,0 +1,291 @@
+"""Pins for the CPU oracle (runs without a GPU).
+
+Each oracle function is checked against something that does not restate its own formula: the
+reference tests' known answers, closed forms, an independent algorithm (polygon clipping, bitwise
+Morton interleave, FP64 brute force) or an invariant, chosen so that a plausible mistake (wrong
+axis, sign, operand, dropped term) fails.
+"""
+import math
+
+import numpy as np
+import pytest
+
+
+# ------------------------------------------------------------------------------------------
+# triBoxOverlap (src/convert/tri_box_overlap.h:72-114)
+# ------------------------------------------------------------------------------------------
+def _clip_nonempty(tri, lo, hi):
+    """Exact-ish FP64 triangle/box intersection by Sutherland-Hodgman clipping (closed box)."""
+    poly = [np.asarray(p, np.float64) for p in tri]
+    for axis in range(3):
+        for bound, sgn in ((lo[axis], 1.0), (hi[axis], -1.0)):
+            out = []
+            n = len(poly)
+            for i in range(n):
+                a, b = poly[i], poly[(i + 1) % n]
+                da, db = sgn * (a[axis] - bound), sgn * (b[axis] - bound)
+                if da >= 0:
+                    out.append(a)
+                if (da >= 0) != (db >= 0):
+                    t = da / (da - db)
+                    out.append(a + t * (b - a))
+            poly = out
+            if not poly:
+                return False
+    return True
+
+
+def test_tri_box_against_polygon_clipping(oracle):
+    rng = np.random.default_rng(1)
+    agree = 0
+    for _ in range(4000):
+        c = rng.uniform(-1, 1, 3).astype(np.float32)
+        h = np.float32(rng.uniform(0.05, 0.5))
+        tri = (c + rng.uniform(-1.2, 1.2, (3, 3)) * rng.uniform(0.1, 1.0)).astype(np.float32)
+        # skip near-ties: the answer must be the same for a slightly grown and shrunk box
+        eps = 1e-4 * float(h)
+        grown = _clip_nonempty(tri, c - h - eps, c + h + eps)
+        shrunk = _clip_nonempty(tri, c - h + eps, c + h - eps)
+        if grown != shrunk:
+            continue
+        got = oracle.tri_box_overlap(c, [h, h, h], tri[0], tri[1], tri[2])
+        assert got == grown, (c, h, tri)
+        agree += 1
+    assert agree > 3500
+
+
+def test_tri_box_touching_counts_as_overlap(oracle):
+    # a face lying exactly on the plane x = 0.5 touches both cells that share it (`>` / `<` tests)
+    tri = np.array([[0.5, 0.1, 0.1], [0.5, 0.4, 0.1], [0.5, 0.1, 0.4]], np.float32)
+    for cx in (0.25, 0.75):
+        assert oracle.tri_box_overlap([cx, 0.25, 0.25], [0.25] * 3, *tri)
+    assert not oracle.tri_box_overlap([1.25, 0.25, 0.25], [0.25] * 3, *tri)
+
+
+def test_voxelize_reference_known_answers(oracle, meshgen):
+    """tests/voxelize_test.cpp:37-58: cube shell at resolution 8."""
+    cube = meshgen.cube_shell()
+    cells, _ = oracle.voxelize_first_touch(cube.xyz, cube.bmin, cube.bmax, 8)
+    assert 0 < len(cells) <= 6 * 8 * 8
+    s = {tuple(c) for c in cells.tolist()}
+    assert (0, 0, 0) in s
+    assert (4, 4, 4) not in s
+    assert len(s) == len(cells)  # each cell once
+
+
+def test_voxelize_first_touch_order_brute_force(oracle, meshgen):
+    """First-touch order (voxelize.cpp:88-96) == a brute-force replay over every cell of the grid."""
+    m = meshgen.icosphere(1, 0.5, 0.0)
+    res = 6
+    # bounds offset so that no vertex sits exactly on a cell face: there the reference's AABB cell
+    # range and an all-cells scan legitimately differ (a face-touching neighbour outside the range)
+    bmin, bmax = m.bmin - np.float32(0.0131), m.bmax + np.float32(0.0217)
+    origin, h = oracle.vox_framing(bmin, bmax, res)
+    half = [h * np.float32(0.5)] * 3
+    order, seen = [], set()
+    for t in range(m.ntri):
+        tri = m.xyz[t]
+        for z in range(res):
+            for y in range(res):
+                for x in range(res):
+                    c = origin + (np.array([x, y, z], np.float32) + np.float32(0.5)) * h
+                    if (x, y, z) not in seen and oracle.tri_box_overlap(c, half, *tri):
+                        seen.add((x, y, z))
+                        order.append((x, y, z))
+    cells, _ = oracle.voxelize_first_touch(m.xyz, bmin, bmax, res)
+    assert [tuple(c) for c in cells.tolist()] == order
+
+
+def test_occupancy_contains_every_vertex_cell(oracle, meshgen):
+    m = meshgen.icosphere(3, 0.5, 1e-3)
+    n, lo, hi = 32, -0.6, 0.6
+    h = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(n))
+    words = oracle.occupancy(m.xyz, [lo] * 3, h, n, n, n)
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little").reshape(n, n, n)
+    v = m.xyz.reshape(-1, 3)
+    idx = np.floor((v - np.float32(lo)) / h).astype(int)
+    # vertices strictly inside a cell (not on a face) must mark it
+    frac = (v - np.float32(lo)) / h - idx
+    ok = np.all((frac > 1e-3) & (frac < 1 - 1e-3), axis=1)
+    assert ok.sum() > 100
+    assert bits[idx[ok, 2], idx[ok, 1], idx[ok, 0]].all()
+
+
+# ------------------------------------------------------------------------------------------
+# framing (sdf_baker.cpp:137-148, voxelize.cpp:54-57)
+# ------------------------------------------------------------------------------------------
+def test_sdf_framing_known_values(oracle):
+    o, h, n = oracle.sdf_framing([-1, -1, -1], [1, 1, 1], 24)
+    assert h == np.float32(2.0) / np.float32(24.0)
+    assert n == (29, 29, 29)  # ceil((2 + 4h) / h) + 1
+    assert np.allclose(o, -1 - 2 * h)
+    o, h, n = oracle.sdf_framing([0, 0, 0], [2, 1, 0.5], 8)
+    assert h == np.float32(0.25) and n == (13, 9, 7)
+    o, h, n = oracle.sdf_framing([0, 0, 0], [0, 0, 0], 0)  # degenerate: longest clamps to 1e-6, res to 1
+    assert h == np.float32(1e-6) and n == (5, 5, 5)
+
+
+# ------------------------------------------------------------------------------------------
+# distance
+# ------------------------------------------------------------------------------------------
+TRI = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+
+
+@pytest.mark.parametrize("q,d2", [
+    ((0.2, 0.2, 0.5), 0.25),       # face region: height above the plane
+    ((-1, -1, 0), 2.0),            # vertex a
+    ((2, -1, 0), 2.0),             # vertex b
+    ((-1, 2, 0.5), 1.0 + 1.0 + 0.25),  # vertex c
+    ((0.5, -1, 0.3), 1.09),        # edge ab: distance to the line y = 0
+    ((1, 1, 0), 0.5),              # edge bc: line x + y = 1
+    ((-0.5, 0.3, -0.2), 0.25 + 0.04),  # edge ca: line x = 0
+])
+def test_point_triangle_regions(oracle, q, d2):
+    assert oracle.tri_dist2_f64(q, TRI) == pytest.approx(d2, rel=1e-12)
+    assert oracle.tri_dist2_f32(q, TRI) == pytest.approx(d2, rel=1e-6)
+
+
+def test_degenerate_triangles(oracle):
+    seg = np.array([[0, 0, 0], [1, 0, 0], [2, 0, 0]], np.float32)  # collinear
+    assert oracle.tri_is_sliver(seg)
+    assert oracle.tri_dist2_f32([1, 1, 0], seg) == pytest.approx(1.0)
+    assert oracle.tri_dist2_f32([3, 0, 0], seg) == pytest.approx(1.0)
+    pt = np.array([[1, 1, 1]] * 3, np.float32)  # all three vertices equal
+    assert oracle.tri_dist2_f32([1, 1, 3], pt) == pytest.approx(4.0)
+    assert not oracle.tri_is_sliver(TRI)
+
+
+def test_f32_distance_matches_f64(oracle):
+    rng = np.random.default_rng(2)
+    worst = 0.0
+    for _ in range(3000):
+        tri = rng.uniform(-1, 1, (3, 3)).astype(np.float32)
+        q = rng.uniform(-1.5, 1.5, 3).astype(np.float32)
+        d64 = math.sqrt(oracle.tri_dist2_f64(q.astype(np.float64), tri))
+        d32 = math.sqrt(oracle.tri_dist2_f32(q, tri))
+        worst = max(worst, abs(d32 - d64))
+    assert worst < 2e-6
+
+
+def test_box_mesh_distance_is_sdbox(oracle, meshgen):
+    """The cube shell's exact SDF is sdBox (src/convert/sdf_primitives.h:12-15)."""
+    cube = meshgen.cube_shell()
+    rng = np.random.default_rng(3)
+    pts = rng.uniform(-0.5, 1.5, (400, 3))
+    d, w = oracle.sdf_f64_points(cube.xyz, pts)
+    q = np.abs(pts - 0.5) - 0.5
+    sd = np.linalg.norm(np.maximum(q, 0), axis=1) + np.minimum(q.max(axis=1), 0)
+    assert np.max(np.abs(d - sd)) < 1e-6
+    d32 = np.sqrt(oracle.udf2_f32_brute(cube.xyz, pts.astype(np.float32)))
+    assert np.max(np.abs(d32 - np.abs(sd))) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------
+# winding number, crossings, parity
+# ------------------------------------------------------------------------------------------
+def test_gwn_closed_forms(oracle, meshgen):
+    octant = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1]], np.float32)
+    assert oracle.gwn_f64([0, 0, 0], octant) == pytest.approx(1 / 8, abs=1e-12)
+    assert oracle.gwn_f64([0, 0, 0], octant[::-1].copy()) == pytest.approx(-1 / 8, abs=1e-12)
+    cube = meshgen.cube_shell()  # the reference's cube shell is wound inward
+    assert oracle.gwn_f64([0.5, 0.5, 0.5], cube.xyz) == pytest.approx(-1.0, abs=1e-12)
+    # one face of the cube seen from the centre subtends 4 pi / 6
+    assert oracle.gwn_f64([0.5, 0.5, 0.5], cube.xyz[:2]) == pytest.approx(-1 / 6, abs=1e-12)
+    assert abs(oracle.gwn_f64([50, 40, 30], cube.xyz)) < 1e-9
+
+
+def test_crossings_and_parity_match_gwn_on_closed_meshes(oracle, meshgen):
+    rng = np.random.default_rng(4)
+    for m in (meshgen.icosphere(3, 0.5, 1e-3), meshgen.displaced(3, 11), meshgen.cube_shell(),
+              meshgen.decimated(600, 5)):
+        assert oracle.boundary_edge_count(m.xyz) == 0
+        lo, hi = m.bmin - 0.2, m.bmax + 0.2
+        pts = rng.uniform(lo, hi, (150, 3)).astype(np.float32)
+        d, w = oracle.sdf_f64_points(m.xyz, pts)
+        for p, dd, ww in zip(pts, d, w):
+            if abs(dd) < 1e-4:
+                continue
+            assert abs(ww - round(ww)) < 1e-9
+            assert oracle.crossings_up(p, m.xyz) == round(ww)
+            assert oracle.parity_inside(p, m.xyz) == (abs(ww) > 0.5)
+
+
+def test_parity_dirs_are_unit_and_start_with_plus_z(oracle):
+    d = oracle.parity_dirs()
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    assert d[0].tolist() == [0.0, 0.0, 1.0]  # sdf_bake.cu:56
+
+
+def test_crossings_watertight_through_vertices(oracle, meshgen):
+    """Columns through exact mesh vertices and edges still count each crossing once."""
+    m = meshgen.icosphere(2, 0.5, 0.0)
+    v = np.unique(m.xyz.reshape(-1, 3), axis=0)
+    for p in v[:60]:
+        for zoff in (-2.0, 2.0):
+            q = np.array([p[0], p[1], zoff], np.float32)
+            c = oracle.crossings_up(q, m.xyz)
+            assert c == 0  # a full column through a closed surface enters and leaves
+    # midpoints of edges, from inside
+    q = np.array([0.0, 0.0, 0.0], np.float32)
+    assert oracle.crossings_up(q, m.xyz) == 1
+
+
+# ------------------------------------------------------------------------------------------
+# LBVH
+# ------------------------------------------------------------------------------------------
+def _morton_bitwise(q):
+    code = 0
+    for bit in range(10):
+        for k, shift in ((0, 2), (1, 1), (2, 0)):
+            code |= ((int(q[k]) >> bit) & 1) << (3 * bit + shift)
+    return code
+
+
+def test_morton_codes_independent(oracle, meshgen):
+    m = meshgen.displaced(2, 3)
+    codes = oracle.morton_codes(m.xyz)
+    c = ((m.xyz[:, 0] + m.xyz[:, 1]) + m.xyz[:, 2]) / np.float32(3.0)
+    lo, hi = c.min(0), c.max(0)
+    u = (c - lo) / (hi - lo)
+    q = np.clip(u * np.float32(1024.0), 0, 1023).astype(np.uint32)
+    assert [int(x) for x in codes] == [_morton_bitwise(r) for r in q]
+
+
+def test_lbvh_invariants(oracle, meshgen):
+    m = meshgen.displaced(3, 9)
+    b = oracle.bvh(m.xyz)
+    n = m.ntri
+    assert np.all(np.diff(b["codes"].astype(np.int64)) >= 0)
+    assert sorted(b["order"].tolist()) == list(range(n))
+    # every node except the root has exactly one parent; leaves reached once
+    ch = np.concatenate([b["left"], b["right"]])
+    ids = np.where(ch >= 0, ch, (n - 1) + ~ch)
+    assert sorted(ids.tolist()) == list(range(1, 2 * n - 1))
+    for i in range(n - 1):
+        for c in (b["left"][i], b["right"][i]):
+            j = c if c >= 0 else (n - 1) + ~c
+            assert np.all(b["lo"][i] <= b["lo"][j]) and np.all(b["hi"][i] >= b["hi"][j])
+    assert b["depth"] < 64
+
+
+# ------------------------------------------------------------------------------------------
+# dense CPU bake (the host-core baseline)
+# ------------------------------------------------------------------------------------------
+def test_dense_bake_equals_brute_force(oracle, meshgen):
+    m = meshgen.displaced(2, 21)
+    n, lo, hi = 12, -0.6, 0.6
+    h = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(n))
+    d, st = oracle.bake_dense(m.xyz, [lo] * 3, h, n, n, n, threads=2)
+    pts = oracle.cell_centres([lo] * 3, h, n, n, n).reshape(-1, 3)
+    brute = np.sqrt(oracle.udf2_f32_brute(m.xyz, pts)).reshape(n, n, n)
+    assert np.array_equal(np.abs(d), brute)  # bit-exact: same per-triangle FP32 values, min is exact
+    d64, w = oracle.sdf_f64_points(m.xyz, pts.astype(np.float64))
+    assert np.array_equal(d.reshape(-1) < 0, np.abs(w) > 0.5)
+    assert np.max(np.abs(d.reshape(-1) - d64)) < 1e-6
+
+
+def test_open_mesh_rejected_by_closed_bake(oracle, meshgen):
+    u = meshgen.uv_sphere()
+    assert oracle.boundary_edge_count(u.xyz) == 64  # seam crack + south-pole ring (float trig)
+    d, st = oracle.bake_dense(u.xyz, [-1.2] * 3, 0.1, 4, 4, 4)
+    assert d is None
