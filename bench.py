@@ -36,6 +36,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-slabs", type=int, default=6, help="z-slabs of the 256^3 grid timed on the CPU")
+    ap.add_argument("--variant", type=int, default=None, help="query kernel: 0 per-voxel, 1 per-brick wave")
     return ap.parse_args()
 
 
@@ -52,6 +53,8 @@ def main():
     occ = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["occ_res"])
     nvox = sdf.nx * sdf.ny * sdf.nz
     ctx = bake.Context(0)
+    if args.variant is not None:
+        ctx.set_option(2, args.variant)
     d_xyz = ctx.alloc(mesh.xyz.nbytes)
     ctx.upload(d_xyz, mesh.xyz)
     d_dist = ctx.alloc(4 * nvox)

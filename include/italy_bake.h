@@ -66,8 +66,18 @@ typedef struct {
   uint64_t column_overflows;
   uint64_t rechecked;    /* voxels whose winding number was re-evaluated in FP64 */
   uint64_t ties;         /* voxels decided by the ray-parity tie-break */
-  uint64_t reserved[6];
+  uint64_t box_tests;    /* point/box distance tests of the closest-triangle traversal (counting mode) */
+  uint64_t tri_tests;    /* point/triangle distance tests (counting mode) */
+  uint64_t reserved[4];
 } ib_stats;
+
+/* Options for ib_set_option. */
+typedef enum {
+  IB_OPT_COUNT_WORK = 1, /* value != 0: bakes run the counting variant of the query kernel and fill
+                            ib_stats.box_tests / tri_tests (slower; for roofline accounting) */
+  IB_OPT_QUERY_VARIANT = 2 /* closest-triangle kernel: 0 = one traversal per voxel, 1 = one traversal per
+                              4x4x4 brick shared by the wave (default); results are identical */
+} ib_option;
 
 /* ---- context ------------------------------------------------------------------------------ */
 /* device: HIP ordinal.  hbm_budget_bytes: cap on what the context may allocate (0 = 90% of free). */
@@ -120,6 +130,7 @@ ib_status ib_memcpy_h2d(ib_context *ctx, void *d_dst, const void *src, uint64_t 
 ib_status ib_memcpy_d2h(ib_context *ctx, void *dst, const void *d_src, uint64_t bytes);
 ib_status ib_synchronize(ib_context *ctx);
 ib_status ib_last_stats(ib_context *ctx, ib_stats *out);
+ib_status ib_set_option(ib_context *ctx, int32_t option, int64_t value);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,8 @@ struct ib_context {
   hipEvent_t ev[EV_COUNT] = {};
   bool ev_valid[EV_COUNT] = {};
   ib_stats stats{};
+  bool count_work = false;
+  int query_variant = 1;
   std::mutex mu;
 
   ib_status alloc_raw(size_t bytes, void **p) {
@@ -301,7 +303,7 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_TRY(c->scratch("gstack", threads * sa.gstack_depth, &sa.gstack));
     IB_TRY(c->scratch("gstackd", threads * sa.gstack_depth, &sa.gstackd));
   }
-  launch_sdf(sa, deep, s);
+  launch_sdf(sa, deep, c->count_work, c->query_variant, s);
   c->record(EV_SDF, s);
   return IB_OK;
 }
@@ -514,15 +516,71 @@ ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const floa
 
 ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, const float *tri_material,
                              const ib_grid *grid, float *base_color, float *metallic, float *roughness) {
-  (void)c;
-  (void)xyz;
-  (void)ntri;
-  (void)tri_material;
-  (void)grid;
-  (void)base_color;
-  (void)metallic;
-  (void)roughness;
-  return fail(IB_ERR_UNSUPPORTED, "material splat is not built yet");
+  if (!c || !grid || !base_color || !metallic || !roughness) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  if (ntri && (!xyz || !tri_material)) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+  if (!grid_ok(grid)) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid");
+  const uint64_t ncell = (uint64_t)grid->nx * grid->ny * grid->nz;
+  if (ncell >= (1ull << 32)) return fail(IB_ERR_UNSUPPORTED, "grid too large for 32-bit cell keys");
+  if (ntri > (uint64_t)(INT_MAX / 9)) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  constexpr int K = 5;
+  float *d_out;
+  IB_TRY(c->scratch("sp_out", (K + 0) * ncell, &d_out));
+  // defaults of sdf_baker.cpp:77-80: colour 1, metallic 0, roughness 1
+  fill_u32(reinterpret_cast<uint32_t *>(d_out), 0x3f800000u, 3 * ncell, s);
+  fill_u32(reinterpret_cast<uint32_t *>(d_out) + 3 * ncell, 0u, ncell, s);
+  fill_u32(reinterpret_cast<uint32_t *>(d_out) + 4 * ncell, 0x3f800000u, ncell, s);
+  if (ntri) {
+    const int n = (int)ntri;
+    float *d_xyz, *d_attr;
+    uint32_t *counts, *offsets;
+    IB_TRY(c->scratch("in_xyz", 9ull * n, &d_xyz));
+    IB_TRY(c->scratch("sp_attr", (uint64_t)K * n, &d_attr));
+    IB_TRY(c->scratch("sp_counts", (uint64_t)n + 1, &counts));
+    IB_TRY(c->scratch("sp_offsets", (uint64_t)n + 1, &offsets));
+    IB_HIP(hipMemcpyAsync(d_xyz, xyz, 9ull * n * sizeof(float), hipMemcpyHostToDevice, s));
+    IB_HIP(hipMemcpyAsync(d_attr, tri_material, (uint64_t)K * n * sizeof(float), hipMemcpyHostToDevice, s));
+    launch_splat_count(d_xyz, n, to_args(*grid), counts, s);
+    fill_u32(counts + n, 0u, 1, s);
+    size_t tb = 0;
+    IB_HIP(exclusive_scan_u32(nullptr, tb, counts, offsets, (uint32_t)n + 1, s));
+    char *temp;
+    IB_TRY(c->scratch("sp_temp", tb, &temp));
+    IB_HIP(exclusive_scan_u32(temp, tb, counts, offsets, (uint32_t)n + 1, s));
+    uint32_t npairs = 0;
+    IB_HIP(hipMemcpyAsync(&npairs, offsets + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    IB_HIP(hipStreamSynchronize(s));
+    if (npairs) {
+      uint32_t *cell, *cell_s;
+      int32_t *tri, *tri_s;
+      IB_TRY(c->scratch("sp_cell", npairs, &cell));
+      IB_TRY(c->scratch("sp_cell_s", npairs, &cell_s));
+      IB_TRY(c->scratch("sp_tri", npairs, &tri));
+      IB_TRY(c->scratch("sp_tri_s", npairs, &tri_s));
+      launch_splat_emit(d_xyz, n, to_args(*grid), offsets, cell, tri, s);
+      int bits = 1;
+      while (bits < 32 && (1ull << bits) < ncell) ++bits;
+      tb = 0;
+      IB_HIP(sort_pairs_u32(nullptr, tb, cell, cell_s, tri, tri_s, npairs, bits, s));
+      IB_TRY(c->scratch("sp_temp", tb, &temp));
+      IB_HIP(sort_pairs_u32(temp, tb, cell, cell_s, tri, tri_s, npairs, bits, s));
+      launch_splat_reduce(cell_s, tri_s, npairs, d_attr, K, d_out, ncell, s);
+    }
+  }
+  // planes on the device; the host layout interleaves rgb per cell (std::vector<glm::vec3>)
+  std::vector<float> planes(K * ncell);
+  IB_HIP(hipMemcpyAsync(planes.data(), d_out, K * ncell * sizeof(float), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  for (uint64_t i = 0; i < ncell; ++i) {
+    base_color[3 * i] = planes[i];
+    base_color[3 * i + 1] = planes[ncell + i];
+    base_color[3 * i + 2] = planes[2 * ncell + i];
+    metallic[i] = planes[3 * ncell + i];
+    roughness[i] = planes[4 * ncell + i];
+  }
+  return IB_OK;
 }
 
 ib_status ib_device_alloc(ib_context *c, uint64_t bytes, void **p) {
@@ -591,9 +649,27 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
     st.column_overflows = h.col_overflow;
     st.rechecked = h.rechecked;
     st.ties = h.ties;
+    st.box_tests = h.box_tests;
+    st.tri_tests = h.tri_tests;
   }
   *out = st;
   return IB_OK;
+}
+
+ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  switch (option) {
+    case IB_OPT_COUNT_WORK:
+      c->count_work = value != 0;
+      return IB_OK;
+    case IB_OPT_QUERY_VARIANT:
+      if (value < 0 || value > 1) return fail(IB_ERR_INVALID_ARGUMENT, "query variant must be 0 or 1");
+      c->query_variant = (int)value;
+      return IB_OK;
+    default:
+      return fail(IB_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));
+  }
 }
 
 }  // extern "C"

@@ -362,6 +362,65 @@ __global__ __launch_bounds__(256) void k_first_touch(const float *__restrict__ x
           atomicMin(&first[((size_t)z * g.ny + y) * g.nx + x], t);
 }
 
+// per-cell attribute splat (sdf_baker.cpp:76-128, voxelize.cpp:94-104): every (triangle, cell)
+// overlap pair in triangle order, stably sorted by cell, then summed sequentially per cell
+__global__ __launch_bounds__(256) void k_splat_count(const float *__restrict__ xyz, int n, GridArgs g,
+                                                     uint32_t *counts) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float *p = xyz + 9 * (size_t)t;
+  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
+  int lo[3], hi[3];
+  tri_cells(p, g, lo, hi);
+  const float half = g.h * 0.5f;
+  uint32_t c = 0;
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x) c += tri_box(cell_centre(g, x, y, z), half, t0, t1, t2) ? 1u : 0u;
+  counts[t] = c;
+}
+
+__global__ __launch_bounds__(256) void k_splat_emit(const float *__restrict__ xyz, int n, GridArgs g,
+                                                    const uint32_t *__restrict__ offsets, uint32_t *cell,
+                                                    int32_t *tri) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float *p = xyz + 9 * (size_t)t;
+  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
+  int lo[3], hi[3];
+  tri_cells(p, g, lo, hi);
+  const float half = g.h * 0.5f;
+  uint32_t o = offsets[t];
+  for (int z = lo[2]; z <= hi[2]; ++z)
+    for (int y = lo[1]; y <= hi[1]; ++y)
+      for (int x = lo[0]; x <= hi[0]; ++x)
+        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2)) {
+          cell[o] = (uint32_t)(((uint64_t)z * g.ny + y) * g.nx + x);
+          tri[o] = t;
+          ++o;
+        }
+}
+
+// out: per cell (k sums followed by the count as float), written only for touched cells
+__global__ __launch_bounds__(256) void k_splat_reduce(const uint32_t *__restrict__ cell, const int32_t *__restrict__ tri,
+                                                      uint32_t npairs, const float *__restrict__ attr, int k,
+                                                      float *out, uint64_t ncell) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= npairs) return;
+  const uint32_t c = cell[i];
+  if (i > 0 && cell[i - 1] == c) return;  // not the first pair of its cell
+  float sum[8];
+  for (int j = 0; j < 8; ++j) sum[j] = 0.0f;
+  uint32_t cnt = 0;
+  for (uint32_t p = i; p < npairs && cell[p] == c; ++p) {
+    const float *a = attr + (size_t)k * tri[p];
+    for (int j = 0; j < k && j < 8; ++j) sum[j] += a[j];
+    ++cnt;
+  }
+  const float fc = (float)cnt;
+  for (int j = 0; j < k && j < 8; ++j) out[(size_t)j * ncell + c] = sum[j] / fc;
+}
+
 __global__ __launch_bounds__(256) void k_first_compact(const int32_t *__restrict__ first, uint64_t ncell,
                                                        Scalars *sc, unsigned long long *keys) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -575,9 +634,13 @@ __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int n
 // per-voxel closest triangle + sign.  One wave = one 4x4x4 brick of voxels; a block of 4 waves
 // covers an 8x8x4 region.
 // ============================================================================================
-template <bool DEEP>
+struct WorkCount {
+  uint32_t box = 0, tri = 0;
+};
+
+template <bool DEEP, bool COUNT>
 __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
-                                            F3 q, int *gstack, float *gstackd, int cap) {
+                                            F3 q, int *gstack, float *gstackd, int cap, WorkCount &wc) {
   float best = INFINITY;
   int lstack[DEEP ? 1 : kStack];
   float lstackd[DEEP ? 1 : kStack];
@@ -589,8 +652,15 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const Node nd = nodes[cur];
     const float d0 = box_d2(q, nd.lo0, nd.hi0);
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
-    if (nd.c0 < 0 && d0 <= best) best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
-    if (nd.c1 < 0 && d1 <= best) best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+    if (COUNT) wc.box += 2;
+    if (nd.c0 < 0 && d0 <= best) {
+      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
+      if (COUNT) wc.tri++;
+    }
+    if (nd.c1 < 0 && d1 <= best) {
+      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+      if (COUNT) wc.tri++;
+    }
     const bool g0 = nd.c0 >= 0 && d0 <= best;
     const bool g1 = nd.c1 >= 0 && d1 <= best;
     if (g0 && g1) {
@@ -624,7 +694,9 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
   return best;
 }
 
-template <bool DEEP>
+__device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best);
+
+template <bool DEEP, bool COUNT>
 __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int x = blockIdx.x * 8 + (wv & 1) * 4 + (lane & 3);
@@ -633,7 +705,6 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   const GridArgs &g = a.col.g;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
   const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
-  const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
   int *gs = nullptr;
   float *gsd = nullptr;
   if (DEEP) {
@@ -641,8 +712,20 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
     gs = a.gstack + tid * a.gstack_depth;
     gsd = a.gstackd + tid * a.gstack_depth;
   }
-  const float best = closest_d2<DEEP>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd, DEEP ? a.gstack_depth : kStack);
-  // integer winding of the closed surface (mesh + reversed caps) from the crossings above q
+  WorkCount wc;
+  const float best =
+      closest_d2<DEEP, COUNT>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd, DEEP ? a.gstack_depth : kStack, wc);
+  if (COUNT) {
+    atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
+    atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
+  }
+  finish_voxel(a, q, x, y, z, best);
+}
+
+// sign from the winding number and store: w = (crossings above q of mesh + reversed caps) + w_caps
+__device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
+  const GridArgs &g = a.col.g;
+  const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
   const size_t col = (size_t)y * g.nx + x;
   const int cnt = a.col.count[col];
   int wi = 0;
@@ -686,8 +769,84 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   a.out[vox] = inside ? -d : d;
 }
 
-template __global__ void k_sdf<false>(SdfArgs);
-template __global__ void k_sdf<true>(SdfArgs);
+// --------------------------------------------------------------------------------------------
+// Wave-coherent variant: the 64 lanes of a wave (one 4x4x4 brick) walk the tree together.  A
+// child is entered if ANY lane's closest-so-far distance reaches its box; node and triangle
+// records are loaded once per wave (uniform addresses -> scalar loads) and the traversal stack is
+// wave-uniform in LDS.  Every lane evaluates every triangle the brick visits, so the per-voxel
+// minimum is the same as the per-lane traversal's (an exact min over a superset of candidates).
+// --------------------------------------------------------------------------------------------
+template <bool COUNT>
+__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
+  __shared__ int sstack[4][kStack];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const GridArgs &g = a.col.g;
+  const int bx = blockIdx.x * 8 + (wv & 1) * 4, by = blockIdx.y * 8 + (wv >> 1) * 4, bz = blockIdx.z * 4;
+  const int x = bx + (lane & 3), y = by + ((lane >> 2) & 3), z = bz + (lane >> 4);
+  if (bx >= g.nx || by >= g.ny) return;  // whole brick outside (wave-uniform)
+  const bool valid = x < g.nx && y < g.ny && z < g.nz;
+  // lanes past the grid edge shadow the brick's first voxel so they never widen the search
+  const int xs = valid ? x : bx, ys = valid ? y : by, zs = valid ? z : bz;
+  const F3 q = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
+  // brick centre orders the children (same value in every lane)
+  const F3 bc = {g.ox + (float)(bx + 2) * g.h, g.oy + (float)(by + 2) * g.h, g.oz + (float)(bz + 2) * g.h};
+  const Node *__restrict__ nodes = a.col.nodes;
+  const float4 *__restrict__ tris = a.col.tris;
+  float best = INFINITY;
+  WorkCount wc;
+  int sp = 0;
+  int cur = 0;
+  for (int visits = 0; visits <= a.col.nnodes; ++visits) {
+    const Node nd = nodes[cur];
+    const float d0 = box_d2(q, nd.lo0, nd.hi0);
+    const float d1 = box_d2(q, nd.lo1, nd.hi1);
+    if (COUNT) wc.box += 2;
+    const bool any0 = __ballot(d0 <= best) != 0;
+    const bool any1 = __ballot(d1 <= best) != 0;
+    if (nd.c0 < 0 && any0) {
+      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
+      if (COUNT) wc.tri++;
+    }
+    if (nd.c1 < 0 && any1) {
+      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+      if (COUNT) wc.tri++;
+    }
+    const bool g0 = nd.c0 >= 0 && any0;
+    const bool g1 = nd.c1 >= 0 && any1;
+    if (g0 && g1) {
+      const int near0 = __builtin_amdgcn_readfirstlane(box_d2(bc, nd.lo0, nd.hi0) <= box_d2(bc, nd.lo1, nd.hi1));
+      if (sp < kStack) {
+        if (lane == 0) sstack[wv][sp] = near0 ? nd.c1 : nd.c0;
+        ++sp;
+      }
+      cur = near0 ? nd.c0 : nd.c1;
+      continue;
+    }
+    if (g0) {
+      cur = nd.c0;
+      continue;
+    }
+    if (g1) {
+      cur = nd.c1;
+      continue;
+    }
+    if (sp == 0) break;
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    cur = __builtin_amdgcn_readfirstlane(sstack[wv][sp]);
+  }
+  if (COUNT && valid) {
+    atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
+    atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
+  }
+  if (!valid) return;
+  finish_voxel(a, q, x, y, z, best);
+}
+
+template __global__ void k_sdf<false, false>(SdfArgs);
+template __global__ void k_sdf<true, false>(SdfArgs);
+template __global__ void k_sdf<false, true>(SdfArgs);
+template __global__ void k_sdf<true, true>(SdfArgs);
 
 // ============================================================================================
 // host-side launchers
@@ -737,6 +896,25 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
   k_occupancy<<<nb(n), 256, 0, s>>>(xyz, n, g, words);
 }
+void launch_splat_count(const float *xyz, int n, const GridArgs &g, uint32_t *counts, hipStream_t s) {
+  k_splat_count<<<nb(n), 256, 0, s>>>(xyz, n, g, counts);
+}
+hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
+                              hipStream_t s) {
+  return rocprim::exclusive_scan(temp, temp_bytes, in, out, 0u, n, rocprim::plus<uint32_t>(), s);
+}
+void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_t *offsets, uint32_t *cell,
+                       int32_t *tri, hipStream_t s) {
+  k_splat_emit<<<nb(n), 256, 0, s>>>(xyz, n, g, offsets, cell, tri);
+}
+hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                          int32_t *vout, uint32_t n, int end_bit, hipStream_t s) {
+  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, s);
+}
+void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
+                         float *out, uint64_t ncell, hipStream_t s) {
+  if (npairs) k_splat_reduce<<<nb(npairs), 256, 0, s>>>(cell, tri, npairs, attr, k, out, ncell);
+}
 void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s) {
   k_first_touch<<<nb(n), 256, 0, s>>>(xyz, n, g, first);
 }
@@ -756,10 +934,23 @@ void launch_columns(const ColumnArgs &a, hipStream_t s) {
   dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
   k_columns<<<grid, 256, 0, s>>>(a);
 }
-void launch_sdf(const SdfArgs &a, bool deep, hipStream_t s) {
+template __global__ void k_sdf_wave<false>(SdfArgs);
+template __global__ void k_sdf_wave<true>(SdfArgs);
+
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, hipStream_t s) {
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
-  if (deep) k_sdf<true><<<grid, 256, 0, s>>>(a);
-  else k_sdf<false><<<grid, 256, 0, s>>>(a);
+  if (variant == 1 && !deep) {
+    if (count) k_sdf_wave<true><<<grid, 256, 0, s>>>(a);
+    else k_sdf_wave<false><<<grid, 256, 0, s>>>(a);
+    return;
+  }
+  if (count) {
+    if (deep) k_sdf<true, true><<<grid, 256, 0, s>>>(a);
+    else k_sdf<false, true><<<grid, 256, 0, s>>>(a);
+  } else {
+    if (deep) k_sdf<true, false><<<grid, 256, 0, s>>>(a);
+    else k_sdf<false, false><<<grid, 256, 0, s>>>(a);
+  }
 }
 
 }  // namespace ib

@@ -26,7 +26,9 @@ struct Scalars {
   uint32_t rechecked;     // voxels re-evaluated in FP64
   uint32_t ties;          // voxels decided by ray parity
   uint32_t occ_set;       // occupied cells (first-touch voxelizer)
-  uint32_t pad[3];
+  uint32_t pad;
+  unsigned long long box_tests;  // counting mode only
+  unsigned long long tri_tests;
 };
 
 struct GridArgs {
@@ -68,7 +70,8 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
                  hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
-void launch_sdf(const SdfArgs &a, bool deep, hipStream_t s);
+// variant 0: per-lane traversal, 1: wave-coherent brick traversal
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, hipStream_t s);
 
 // first-touch voxelizer (drop-in voxelizeMesh order)
 void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s);
@@ -78,5 +81,16 @@ hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long 
                        uint32_t n, hipStream_t s);
 void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
                          hipStream_t s);
+
+// per-cell attribute splat
+void launch_splat_count(const float *xyz, int n, const GridArgs &g, uint32_t *counts, hipStream_t s);
+hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
+                              hipStream_t s);
+void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_t *offsets, uint32_t *cell,
+                       int32_t *tri, hipStream_t s);
+hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                          int32_t *vout, uint32_t n, int end_bit, hipStream_t s);
+void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
+                         float *out, uint64_t ncell, hipStream_t s);
 
 }  // namespace ib

@@ -17,7 +17,7 @@ STATUS_NAMES = {
 EXPORTS = [
     "ib_create", "ib_destroy", "ib_last_error", "ib_version", "ib_sdf_framing", "ib_vox_framing", "ib_bake_sdf",
     "ib_voxelize", "ib_splat_materials", "ib_bake_dense", "ib_bake_dense_device", "ib_device_alloc",
-    "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_synchronize", "ib_last_stats",
+    "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_synchronize", "ib_last_stats", "ib_set_option",
 ]
 
 
@@ -53,7 +53,7 @@ class Stats(ctypes.Structure):
         ("triangles", ctypes.c_uint64), ("voxels", ctypes.c_uint64), ("occ_cells", ctypes.c_uint64),
         ("bvh_depth", ctypes.c_int32), ("deep_stack", ctypes.c_int32), ("boundary_edges", ctypes.c_int64),
         ("column_overflows", ctypes.c_uint64), ("rechecked", ctypes.c_uint64), ("ties", ctypes.c_uint64),
-        ("reserved", ctypes.c_uint64 * 6),
+        ("box_tests", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4),
     ]
 
     def as_dict(self):
@@ -94,6 +94,7 @@ def load():
     lib.ib_memcpy_d2h.argtypes = [_vp, _vp, _vp, _u64]
     lib.ib_synchronize.argtypes = [_vp]
     lib.ib_last_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
+    lib.ib_set_option.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64]
     _lib = lib
     return lib
 

@@ -78,6 +78,21 @@ class Context:
                                             ctypes.c_void_p(d_occ) if d_occ else None,
                                             ctypes.c_void_p(stream) if stream else None))
 
+    def set_option(self, option, value):
+        check(self.lib.ib_set_option(self._h, int(option), int(value)))
+
+    def query_counters(self, d_xyz, ntri, sdf_grid):
+        """Run one counting bake and return the traversal counters of the query kernel."""
+        d_out = self.alloc(4 * sdf_grid.nx * sdf_grid.ny * sdf_grid.nz)
+        try:
+            self.set_option(1, 1)
+            self.bake_dense_device(d_xyz, ntri, sdf_grid, d_out)
+            st = self.stats()
+        finally:
+            self.set_option(1, 0)
+            self.free(d_out)
+        return {"box_tests": st["box_tests"], "tri_tests": st["tri_tests"]}
+
     def stats(self):
         st = Stats()
         check(self.lib.ib_last_stats(self._h, ctypes.byref(st)))
