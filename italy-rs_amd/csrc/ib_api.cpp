@@ -186,6 +186,7 @@ struct MeshDev {
   float *caps = nullptr;
   int ncap = 0;
   int depth = 0;
+  bool slivers = false;
 };
 
 // ingest + LBVH of one mesh whose soup is already on the device; synchronises once to read the
@@ -237,7 +238,7 @@ ib_status build_mesh(ib_context *c, const float *d_xyz, int n, hipStream_t s, co
   IB_TRY(c->scratch("sort_temp", temp_bytes, reinterpret_cast<char **>(&temp)));
   IB_HIP(sort_pairs(temp, temp_bytes, codes_in, codes, idx, m.order, n, s));
   launch_karras(codes, n, m.nodes, leaf_parent, s);
-  launch_gather(d_xyz, m.order, n, m.tris, s);
+  launch_gather(d_xyz, m.order, n, m.tris, sc, s);
   fill_u32(visits, 0u, n, s);
   launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, s);
   c->record(EV_BVH, s);
@@ -253,6 +254,7 @@ ib_status build_mesh(ib_context *c, const float *d_xyz, int n, hipStream_t s, co
   if (h.flags & kFlagNonFinite) return fail(IB_ERR_INVALID_MESH, "mesh has non-finite vertex coordinates");
   m.n = n;
   m.depth = (int)h.depth;
+  m.slivers = h.slivers > 0;
   if (m.depth > kMaxDepth) return fail(IB_ERR_INTERNAL, "BVH parent chain does not reach the root");
   if (m.depth + 2 > kStack)
     return fail(IB_ERR_UNSUPPORTED, "BVH depth " + std::to_string(m.depth) + " exceeds the column traversal stack");
@@ -295,6 +297,7 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
   SdfArgs sa{};
   sa.col = ca;
   sa.out = d_out;
+  IB_TRY(c->scratch("amb_list", (uint64_t)g.nx * g.ny * g.nz, &sa.amb_list));
   const bool deep = m.depth + 2 > kStack;
   c->stats.deep_stack = deep ? 1 : 0;
   if (deep) {
@@ -303,7 +306,8 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_TRY(c->scratch("gstack", threads * sa.gstack_depth, &sa.gstack));
     IB_TRY(c->scratch("gstackd", threads * sa.gstack_depth, &sa.gstackd));
   }
-  launch_sdf(sa, deep, c->count_work, c->query_variant, s);
+  launch_sdf(sa, deep, c->count_work, c->query_variant, m.slivers, s);
+  launch_resolve(sa, s);
   c->record(EV_SDF, s);
   return IB_OK;
 }

@@ -198,11 +198,14 @@ __device__ __forceinline__ float tri_d2_f32(F3 q, F3 p0, F3 p1, F3 p2) {
   return (s * s) / dot3(n, n);
 }
 
-// triangle record: 3 x float4 = (v0, flags), (v1, original index), (v2, 0)
+// triangle record: 3 x float4 = (v0, flags), (v1, original index), (v2, 0).  SLIVERS = false is
+// only instantiated for meshes without sliver triangles (the host counts them at build time), so
+// that kernel carries no FP64 code.
+template <bool SLIVERS>
 __device__ __forceinline__ float tri_d2(F3 q, const float4 *t) {
   const float4 a = t[0], b = t[1], c = t[2];
   const F3 p0 = {a.x, a.y, a.z}, p1 = {b.x, b.y, b.z}, p2 = {c.x, c.y, c.z};
-  if (__float_as_uint(a.w) & kSliver) return (float)tri_d2_f64(q, p0, p1, p2);
+  if (SLIVERS && (__float_as_uint(a.w) & kSliver)) return (float)tri_d2_f64(q, p0, p1, p2);
   return tri_d2_f32(q, p0, p1, p2);
 }
 

@@ -136,7 +136,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
 
 // sorted triangle records (3 x float4) with the sliver flag
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, const int32_t *__restrict__ order,
-                                                int n, float4 *tris) {
+                                                int n, float4 *tris, Scalars *sc) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int o = order[i];
@@ -147,6 +147,7 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, c
   const float nn = nx * nx + ny * ny + nz * nz;
   const float l1 = abx * abx + aby * aby + abz * abz, l2 = acx * acx + acy * acy + acz * acz;
   const uint32_t flags = (nn > 1e-4f * (l1 * l2)) ? 0u : kSliver;
+  if (flags) atomicAdd(&sc->slivers, 1u);
   tris[3 * i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(flags));
   tris[3 * i + 1] = make_float4(t[3], t[4], t[5], __int_as_float(o));
   tris[3 * i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
@@ -638,7 +639,7 @@ struct WorkCount {
   uint32_t box = 0, tri = 0;
 };
 
-template <bool DEEP, bool COUNT>
+template <bool DEEP, bool COUNT, bool SLIVERS>
 __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
                                             F3 q, int *gstack, float *gstackd, int cap, WorkCount &wc) {
   float best = INFINITY;
@@ -654,11 +655,11 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
     if (COUNT) wc.box += 2;
     if (nd.c0 < 0 && d0 <= best) {
-      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
+      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c0)));
       if (COUNT) wc.tri++;
     }
     if (nd.c1 < 0 && d1 <= best) {
-      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c1)));
       if (COUNT) wc.tri++;
     }
     const bool g0 = nd.c0 >= 0 && d0 <= best;
@@ -694,9 +695,10 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
   return best;
 }
 
+template <bool CAPS>
 __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best);
 
-template <bool DEEP, bool COUNT>
+template <bool DEEP, bool COUNT, bool SLIVERS, bool CAPS>
 __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int x = blockIdx.x * 8 + (wv & 1) * 4 + (lane & 3);
@@ -713,47 +715,83 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
     gsd = a.gstackd + tid * a.gstack_depth;
   }
   WorkCount wc;
-  const float best =
-      closest_d2<DEEP, COUNT>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd, DEEP ? a.gstack_depth : kStack, wc);
+  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd,
+                                                     DEEP ? a.gstack_depth : kStack, wc);
   if (COUNT) {
     atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
     atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
   }
-  finish_voxel(a, q, x, y, z, best);
+  finish_voxel<CAPS>(a, q, x, y, z, best);
 }
 
-// sign from the winding number and store: w = (crossings above q of mesh + reversed caps) + w_caps
+// Sign from the winding number and store.  w = (signed crossings above q of the closed surface
+// mesh + reversed caps, from the column list) + (solid angle sum of the caps)/4pi.  Voxels whose
+// column overflowed the stored crossings, or whose FP32 |w| lies within 0.05 of 0.5, are stored
+// unsigned and listed for the resolve kernel.
+template <bool CAPS>
 __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
   const GridArgs &g = a.col.g;
   const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
   const size_t col = (size_t)y * g.nx + x;
+  const float d = sqrtf(best);
   const int cnt = a.col.count[col];
-  int wi = 0;
-  if (cnt <= kColCap) {
+  bool ambiguous = cnt > kColCap;
+  bool inside = false;
+  if (!ambiguous) {
     const float2 *cr = a.col.cross + col * kColCap;
+    int wi = 0;
     for (int k = 0; k < cnt; ++k) {
       const float2 c = cr[k];
       if (c.x > q.z) wi += (int)c.y;
     }
-  } else {
-    wi = column_direct(a.col, q.x, q.y, q.z);
-  }
-  bool inside;
-  if (a.col.ncap == 0) {
-    inside = wi != 0;
-  } else {
-    // w = wi + w_caps; FP32 Neumaier sum, FP64 recheck inside the 0.05 band, parity for ties
-    float sum = 0.0f, cmp = 0.0f;
-    for (int c = 0; c < a.col.ncap; ++c) {
-      const float om = solid_angle_f32(q, a.col.caps + 9 * c);
-      const float s = sum + om;
-      cmp += (fabsf(sum) >= fabsf(om)) ? (sum - s) + om : (om - s) + sum;
-      sum = s;
+    if (!CAPS) {
+      inside = wi != 0;
+    } else {
+      float sum = 0.0f, cmp = 0.0f;  // Neumaier
+      for (int c = 0; c < a.col.ncap; ++c) {
+        const float om = solid_angle_f32(q, a.col.caps + 9 * c);
+        const float t = sum + om;
+        cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
+        sum = t;
+      }
+      const float w32 = (float)wi + (sum + cmp) * (float)(1.0 / (4.0 * M_PI));
+      if (fabsf(fabsf(w32) - 0.5f) >= 0.05f) inside = fabsf(w32) > 0.5f;
+      else ambiguous = true;
     }
-    const float w32 = (float)wi + (sum + cmp) * (float)(1.0 / (4.0 * M_PI));
-    const float m32 = fabsf(fabsf(w32) - 0.5f);
-    if (m32 >= 0.05f) {
-      inside = fabsf(w32) > 0.5f;
+  }
+  if (ambiguous) {
+    a.out[vox] = d;
+    const uint32_t pos = atomicAdd(&a.col.sc->ambiguous, 1u);
+    a.amb_list[pos] = (uint32_t)vox;
+  } else {
+    a.out[vox] = inside ? -d : d;
+  }
+}
+
+// FP64 recheck / parity / overflow recount for the listed voxels (grid-stride over the count the
+// query kernel left in the scalars)
+__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
+  const uint32_t n = a.col.sc->ambiguous;
+  const GridArgs &g = a.col.g;
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const uint32_t vox = a.amb_list[i];
+    const int x = (int)(vox % (uint32_t)g.nx);
+    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
+    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
+    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+    const size_t col = (size_t)y * g.nx + x;
+    const int cnt = a.col.count[col];
+    int wi = 0;
+    if (cnt <= kColCap) {
+      const float2 *cr = a.col.cross + col * kColCap;
+      for (int k = 0; k < cnt; ++k)
+        if (cr[k].x > q.z) wi += (int)cr[k].y;
+    } else {
+      wi = column_direct(a.col, q.x, q.y, q.z);
+    }
+    bool inside;
+    if (a.col.ncap == 0) {
+      inside = wi != 0;
     } else {
       atomicAdd(&a.col.sc->rechecked, 1u);
       const double w64 = (double)wi + cap_winding_f64(q, a.col.caps, a.col.ncap);
@@ -764,9 +802,9 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
         inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q);
       }
     }
+    const float d = a.out[vox];
+    a.out[vox] = inside ? -d : d;
   }
-  const float d = sqrtf(best);
-  a.out[vox] = inside ? -d : d;
 }
 
 // --------------------------------------------------------------------------------------------
@@ -776,7 +814,7 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
 // wave-uniform in LDS.  Every lane evaluates every triangle the brick visits, so the per-voxel
 // minimum is the same as the per-lane traversal's (an exact min over a superset of candidates).
 // --------------------------------------------------------------------------------------------
-template <bool COUNT>
+template <bool COUNT, bool SLIVERS, bool CAPS>
 __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
   __shared__ int sstack[4][kStack];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -804,11 +842,11 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
     const bool any0 = __ballot(d0 <= best) != 0;
     const bool any1 = __ballot(d1 <= best) != 0;
     if (nd.c0 < 0 && any0) {
-      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c0)));
+      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c0)));
       if (COUNT) wc.tri++;
     }
     if (nd.c1 < 0 && any1) {
-      best = fminf(best, tri_d2(q, tris + 3 * (~nd.c1)));
+      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c1)));
       if (COUNT) wc.tri++;
     }
     const bool g0 = nd.c0 >= 0 && any0;
@@ -840,13 +878,8 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
     atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
   }
   if (!valid) return;
-  finish_voxel(a, q, x, y, z, best);
+  finish_voxel<CAPS>(a, q, x, y, z, best);
 }
-
-template __global__ void k_sdf<false, false>(SdfArgs);
-template __global__ void k_sdf<true, false>(SdfArgs);
-template __global__ void k_sdf<false, true>(SdfArgs);
-template __global__ void k_sdf<true, true>(SdfArgs);
 
 // ============================================================================================
 // host-side launchers
@@ -872,8 +905,8 @@ hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint3
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s) {
   if (n > 1) k_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, nodes, leaf_parent);
 }
-void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, hipStream_t s) {
-  k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris);
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, Scalars *sc, hipStream_t s) {
+  k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, sc);
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
                   float *root_box, Scalars *sc, hipStream_t s) {
@@ -934,23 +967,29 @@ void launch_columns(const ColumnArgs &a, hipStream_t s) {
   dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
   k_columns<<<grid, 256, 0, s>>>(a);
 }
-template __global__ void k_sdf_wave<false>(SdfArgs);
-template __global__ void k_sdf_wave<true>(SdfArgs);
+template <bool COUNT, bool SLIVERS, bool CAPS>
+static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, dim3 grid, hipStream_t s) {
+  if (variant == 1 && !deep) k_sdf_wave<COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
+  else if (deep) k_sdf<true, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
+  else k_sdf<false, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
+}
 
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, hipStream_t s) {
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, hipStream_t s) {
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
-  if (variant == 1 && !deep) {
-    if (count) k_sdf_wave<true><<<grid, 256, 0, s>>>(a);
-    else k_sdf_wave<false><<<grid, 256, 0, s>>>(a);
-    return;
-  }
-  if (count) {
-    if (deep) k_sdf<true, true><<<grid, 256, 0, s>>>(a);
-    else k_sdf<false, true><<<grid, 256, 0, s>>>(a);
-  } else {
-    if (deep) k_sdf<true, false><<<grid, 256, 0, s>>>(a);
-    else k_sdf<false, false><<<grid, 256, 0, s>>>(a);
+  const bool caps = a.col.ncap > 0;
+  const int key = (count ? 4 : 0) | (slivers ? 2 : 0) | (caps ? 1 : 0);
+  switch (key) {
+    case 0: launch_sdf_t<false, false, false>(a, deep, variant, grid, s); break;
+    case 1: launch_sdf_t<false, false, true>(a, deep, variant, grid, s); break;
+    case 2: launch_sdf_t<false, true, false>(a, deep, variant, grid, s); break;
+    case 3: launch_sdf_t<false, true, true>(a, deep, variant, grid, s); break;
+    case 4: launch_sdf_t<true, false, false>(a, deep, variant, grid, s); break;
+    case 5: launch_sdf_t<true, false, true>(a, deep, variant, grid, s); break;
+    case 6: launch_sdf_t<true, true, false>(a, deep, variant, grid, s); break;
+    default: launch_sdf_t<true, true, true>(a, deep, variant, grid, s); break;
   }
 }
+
+void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, s>>>(a); }
 
 }  // namespace ib

@@ -26,7 +26,9 @@ struct Scalars {
   uint32_t rechecked;     // voxels re-evaluated in FP64
   uint32_t ties;          // voxels decided by ray parity
   uint32_t occ_set;       // occupied cells (first-touch voxelizer)
-  uint32_t pad;
+  uint32_t slivers;       // triangles taking the FP64 distance path
+  uint32_t ambiguous;     // voxels handed to the resolve kernel (column overflow or |w| near 0.5)
+  uint32_t pad[3];
   unsigned long long box_tests;  // counting mode only
   unsigned long long tri_tests;
 };
@@ -51,6 +53,7 @@ struct ColumnArgs {
 struct SdfArgs {
   ColumnArgs col;
   float *out;
+  uint32_t *amb_list;  // voxels whose sign the resolve kernel decides (capacity: all voxels)
   int *gstack;
   float *gstackd;
   int gstack_depth;
@@ -62,7 +65,7 @@ void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, 
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                       int32_t *vout, int n, hipStream_t s);
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s);
-void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, hipStream_t s);
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, Scalars *sc, hipStream_t s);
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
                   float *root_box, Scalars *sc, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
@@ -70,8 +73,11 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
                  hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
-// variant 0: per-lane traversal, 1: wave-coherent brick traversal
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, hipStream_t s);
+// variant 0: per-lane traversal, 1: wave-coherent brick traversal; slivers: the mesh has triangles
+// on the FP64 distance path
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, hipStream_t s);
+// FP64 winding recheck, parity tie-break and column-overflow recount for the listed voxels
+void launch_resolve(const SdfArgs &a, hipStream_t s);
 
 // first-touch voxelizer (drop-in voxelizeMesh order)
 void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s);

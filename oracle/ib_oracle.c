@@ -863,7 +863,7 @@ typedef struct {
   const qctx *c;
   const float *origin;
   float h;
-  int nx, ny, nz, z0, z1;
+  int nx, ny, nz, z0, z1, zstep;
   float *dist;
   int next_row;
   pthread_mutex_t *mu;
@@ -889,7 +889,7 @@ static void *bake_thread_fn(void *arg) {
     for (int x = 0; x < j->nx; ++x) {
       const float px = j->origin[0] + ((float)x + 0.5f) * j->h;
       const int nc = column_crossings(j->c, px, py, &cr, &cap, &th->ctests);
-      for (int z = j->z0; z < j->z1; ++z) {
+      for (int z = j->z0; z < j->z1; z += j->zstep) {
         const float q[3] = {px, py, j->origin[2] + ((float)z + 0.5f) * j->h};
         const float d2 = closest2(j->c, q, &th->nodes, &th->tests);
         int w = 0;
@@ -914,6 +914,12 @@ static double now_s(void) {
 
 int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz, int z0,
                    int z1, int threads, float *dist, orc_bake_stats *st) {
+  return orc_bake_dense_strided(xyz, ntri, origin, h, nx, ny, nz, z0, z1, 1, threads, dist, st);
+}
+
+int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
+                           int z0, int z1, int zstep, int threads, float *dist, orc_bake_stats *st) {
+  if (zstep < 1) zstep = 1;
   if (orc_boundary_edge_count(xyz, ntri) != 0) return -1;
   const double t0 = now_s();
   orc_bvh b;
@@ -925,7 +931,7 @@ int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h
   if (z0 < 0) z0 = 0;
   if (z1 > nz || z1 < 0) z1 = nz;
   pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
-  bake_job job = {&c, origin, h, nx, ny, nz, z0, z1, dist, 0, &mu, 0, 0, 0, 0};
+  bake_job job = {&c, origin, h, nx, ny, nz, z0, z1, zstep, dist, 0, &mu, 0, 0, 0, 0};
   pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
   bake_thread *ths = (bake_thread *)calloc(threads, sizeof(bake_thread));
   for (int i = 0; i < threads; ++i) {
@@ -942,7 +948,7 @@ int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h
   }
   const double t1 = now_s();
   if (st) {
-    st->voxels = (uint64_t)nx * ny * (uint64_t)(z1 - z0);
+    st->voxels = (uint64_t)nx * ny * (uint64_t)((z1 - z0 + zstep - 1) / zstep);
     st->node_visits = nodes;
     st->tri_tests = tests;
     st->column_tri_tests = ctests;

@@ -107,6 +107,10 @@ typedef struct {
 int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
                    int z_begin, int z_end, int threads, float *dist, orc_bake_stats *stats);
 
+/* same, computing only every zstep-th slab of [z_begin, z_end) */
+int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
+                           int z_begin, int z_end, int zstep, int threads, float *dist, orc_bake_stats *stats);
+
 /* FP64 brute-force signed distance at arbitrary points (inside where |GWN| > 0.5, FP64) */
 void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_dist,
                         double *out_w);

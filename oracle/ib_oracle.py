@@ -77,6 +77,9 @@ def _load():
     lib.orc_bake_dense.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p,
                                    ctypes.POINTER(BakeStats)]
+    lib.orc_bake_dense_strided.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, _f32p, ctypes.POINTER(BakeStats)]
     lib.orc_sdf_f64_points.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p, _f64p]
     lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
     lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
@@ -206,13 +209,14 @@ def bvh(xyz):
     return out
 
 
-def bake_dense(xyz, origin, h, nx, ny, nz, z0=0, z1=-1, threads=0):
-    """Dense signed distance grid (z, y, x) with the oracle's BVH bake; None if the mesh is open."""
+def bake_dense(xyz, origin, h, nx, ny, nz, z0=0, z1=-1, threads=0, zstep=1):
+    """Dense signed distance grid (z, y, x) with the oracle's BVH bake; None if the mesh is open.
+    Only every zstep-th slab of [z0, z1) is computed (the others stay NaN)."""
     xyz = _f32(xyz)
     dist = np.full((nz, ny, nx), np.nan, np.float32)
     st = BakeStats()
-    rc = lib().orc_bake_dense(_p(xyz, _f32p), xyz.size // 9, _p(_f32(origin), _f32p), float(h), nx, ny, nz, z0, z1,
-                              threads, _p(dist, _f32p), ctypes.byref(st))
+    rc = lib().orc_bake_dense_strided(_p(xyz, _f32p), xyz.size // 9, _p(_f32(origin), _f32p), float(h), nx, ny, nz,
+                                      z0, z1, zstep, threads, _p(dist, _f32p), ctypes.byref(st))
     if rc != 0:
         return None, None
     return dist, {k: getattr(st, k) for k, _ in BakeStats._fields_}

@@ -104,7 +104,7 @@ def test_plain_large_occupancy_bitexact(ctx, oracle, meshgen):
     _, occ = ctx.bake_dense(mesh.xyz, None, og)
     ref = oracle.occupancy(mesh.xyz, list(og.origin), og.h, og.nx, og.ny, og.nz)
     assert np.array_equal(occ, ref)
-    assert int(np.unpackbits(occ.view(np.uint8)).sum()) > 1_000_000
+    assert int(np.unpackbits(occ.view(np.uint8)).sum()) > 300_000  # ~ surface area / h^2
 
 
 def test_plain_large_sdf(ctx, oracle, meshgen):
