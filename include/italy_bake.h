@@ -75,9 +75,22 @@ typedef struct {
 typedef enum {
   IB_OPT_COUNT_WORK = 1, /* value != 0: bakes run the counting variant of the query kernel and fill
                             ib_stats.box_tests / tri_tests (slower; for roofline accounting) */
-  IB_OPT_QUERY_VARIANT = 2 /* closest-triangle kernel: 0 = one traversal per voxel, 1 = one traversal per
-                              4x4x4 brick shared by the wave (default); results are identical */
+  IB_OPT_QUERY_VARIANT = 2, /* closest-triangle kernel: 0 = one traversal per voxel, 1 = one traversal per
+                               brick shared by the wave (default); results are identical */
+  IB_OPT_BRICK_SHAPE = 3,   /* voxels of one wave: 0 = 4x4x4 (default), 1 = 8x4x2, 2 = 8x8x1, 3 = 16x4x1 */
+  IB_OPT_BLOCKS_PER_CU = 4  /* persistent blocks per compute unit in batched bakes (default 8) */
 } ib_option;
+
+/* One mesh of a batch: device soup, optional dense SDF grid (sdf.nx == 0: none) and optional
+ * occupancy grid (occ.nx == 0: none), device outputs laid out as for ib_bake_dense_device. */
+typedef struct {
+  const float *d_xyz;
+  uint64_t ntri;
+  ib_grid sdf;
+  float *d_distances;
+  ib_grid occ;
+  uint32_t *d_occ_words;
+} ib_job;
 
 /* ---- context ------------------------------------------------------------------------------ */
 /* device: HIP ordinal.  hbm_budget_bytes: cap on what the context may allocate (0 = 90% of free). */
@@ -122,6 +135,19 @@ ib_status ib_bake_dense(ib_context *ctx, const float *xyz, uint64_t ntri, const 
  * ((z*ny + y)*ceil(nx/32) + x/32). */
 ib_status ib_bake_dense_device(ib_context *ctx, const float *d_xyz, uint64_t ntri, const ib_grid *sdf,
                                float *d_distances, const ib_grid *occ, uint32_t *d_occ_words, void *stream);
+
+/* Batched bake: every job's LBVH is built, then ONE persistent launch drains the brick tiles of
+ * all jobs from a device work queue.  Synchronous: returns when every output is written.  Results
+ * are identical to baking each job alone with ib_bake_dense_device (no counterpart in the
+ * reference, which bakes one mesh per call: src/convert/sdf_baker.cpp:137-310). */
+ib_status ib_bake_batch_device(ib_context *ctx, const ib_job *jobs, uint32_t njobs, void *stream);
+
+/* Autotuner for batched bakes: bakes every sample_stride-th job of `jobs` with each brick shape
+ * (IB_OPT_BRICK_SHAPE) and persistent-block count (IB_OPT_BLOCKS_PER_CU), times the persistent
+ * query launch with HIP events, and leaves the fastest setting on the context.  The choice changes
+ * speed only, never results.  Outputs (may be NULL): the chosen shape, blocks per CU and time. */
+ib_status ib_autotune_batch(ib_context *ctx, const ib_job *jobs, uint32_t njobs, uint32_t sample_stride,
+                            int32_t *out_shape, int32_t *out_blocks, double *out_ms);
 
 /* ---- device memory helpers (for callers without a GPU framework) --------------------------- */
 ib_status ib_device_alloc(ib_context *ctx, uint64_t bytes, void **d_ptr);

@@ -762,7 +762,7 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
   if (ambiguous) {
     a.out[vox] = d;
     const uint32_t pos = atomicAdd(&a.col.sc->ambiguous, 1u);
-    a.amb_list[pos] = (uint32_t)vox;
+    if (pos < a.amb_cap) a.amb_list[pos] = (uint32_t)vox;  // the host re-bakes on overflow
   } else {
     a.out[vox] = inside ? -d : d;
   }
@@ -770,10 +770,10 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
 
 // FP64 recheck / parity / overflow recount for the listed voxels (grid-stride over the count the
 // query kernel left in the scalars)
-__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
-  const uint32_t n = a.col.sc->ambiguous;
+__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t block, uint32_t nblocks) {
+  const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
   const GridArgs &g = a.col.g;
-  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+  for (uint32_t i = block * 256 + threadIdx.x; i < n; i += nblocks * 256) {
     const uint32_t vox = a.amb_list[i];
     const int x = (int)(vox % (uint32_t)g.nx);
     const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
@@ -807,6 +807,8 @@ __global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
   }
 }
 
+__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) { resolve_range(a, blockIdx.x, gridDim.x); }
+
 // --------------------------------------------------------------------------------------------
 // Wave-coherent variant: the 64 lanes of a wave (one 4x4x4 brick) walk the tree together.  A
 // child is entered if ANY lane's closest-so-far distance reaches its box; node and triangle
@@ -814,20 +816,22 @@ __global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
 // wave-uniform in LDS.  Every lane evaluates every triangle the brick visits, so the per-voxel
 // minimum is the same as the per-lane traversal's (an exact min over a superset of candidates).
 // --------------------------------------------------------------------------------------------
+// One wave = one brick of sx * sy * sz = 64 voxels with corner (bx, by, bz); lane l is voxel
+// (l % sx, (l / sx) % sy, l / (sx * sy)) of the brick.  sstack is the wave's LDS stack.
 template <bool COUNT, bool SLIVERS, bool CAPS>
-__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
-  __shared__ int sstack[4][kStack];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int bz, int sx, int sy, int *sstack) {
+  const int lane = threadIdx.x & 63;
   const GridArgs &g = a.col.g;
-  const int bx = blockIdx.x * 8 + (wv & 1) * 4, by = blockIdx.y * 8 + (wv >> 1) * 4, bz = blockIdx.z * 4;
-  const int x = bx + (lane & 3), y = by + ((lane >> 2) & 3), z = bz + (lane >> 4);
-  if (bx >= g.nx || by >= g.ny) return;  // whole brick outside (wave-uniform)
+  if (bx >= g.nx || by >= g.ny || bz >= g.nz) return;  // whole brick outside (wave-uniform)
+  const int sz = 64 / (sx * sy);
+  const int x = bx + lane % sx, y = by + (lane / sx) % sy, z = bz + lane / (sx * sy);
   const bool valid = x < g.nx && y < g.ny && z < g.nz;
   // lanes past the grid edge shadow the brick's first voxel so they never widen the search
   const int xs = valid ? x : bx, ys = valid ? y : by, zs = valid ? z : bz;
   const F3 q = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
   // brick centre orders the children (same value in every lane)
-  const F3 bc = {g.ox + (float)(bx + 2) * g.h, g.oy + (float)(by + 2) * g.h, g.oz + (float)(bz + 2) * g.h};
+  const F3 bc = {g.ox + ((float)bx + 0.5f * (float)sx) * g.h, g.oy + ((float)by + 0.5f * (float)sy) * g.h,
+                 g.oz + ((float)bz + 0.5f * (float)sz) * g.h};
   const Node *__restrict__ nodes = a.col.nodes;
   const float4 *__restrict__ tris = a.col.tris;
   float best = INFINITY;
@@ -854,7 +858,7 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
     if (g0 && g1) {
       const int near0 = __builtin_amdgcn_readfirstlane(box_d2(bc, nd.lo0, nd.hi0) <= box_d2(bc, nd.lo1, nd.hi1));
       if (sp < kStack) {
-        if (lane == 0) sstack[wv][sp] = near0 ? nd.c1 : nd.c0;
+        if (lane == 0) sstack[sp] = near0 ? nd.c1 : nd.c0;
         ++sp;
       }
       cur = near0 ? nd.c0 : nd.c1;
@@ -871,7 +875,7 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
     if (sp == 0) break;
     --sp;
     __builtin_amdgcn_wave_barrier();
-    cur = __builtin_amdgcn_readfirstlane(sstack[wv][sp]);
+    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
   }
   if (COUNT && valid) {
     atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
@@ -879,6 +883,54 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a) {
   }
   if (!valid) return;
   finish_voxel<CAPS>(a, q, x, y, z, best);
+}
+
+// brick shapes a wave can take (sx, sy; sz = 64 / (sx * sy)); the 4 waves of a block tile 2 x 2
+__device__ __constant__ const int kShapes[4][2] = {{4, 4}, {8, 4}, {8, 8}, {16, 4}};
+
+template <bool COUNT, bool SLIVERS, bool CAPS>
+__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a, int shape) {
+  __shared__ int sstack[4][kStack];
+  const int wv = threadIdx.x >> 6;
+  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 / (sx * sy);
+  const int bx = blockIdx.x * 2 * sx + (wv & 1) * sx, by = blockIdx.y * 2 * sy + (wv >> 1) * sy, bz = blockIdx.z * sz;
+  wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, sx, sy, sstack[wv]);
+}
+
+// Persistent variant for a batch of meshes: blocks take block-tiles from one device work queue
+// (tile_prefix[j] = first tile of job j) until the queue is empty.
+template <bool COUNT, bool SLIVERS, bool CAPS>
+__global__ __launch_bounds__(256) void k_sdf_batch(BatchArgs t) {
+  __shared__ int sstack[4][kStack];
+  __shared__ uint32_t s_tile;
+  const int wv = threadIdx.x >> 6;
+  const int sx = kShapes[t.shape][0], sy = kShapes[t.shape][1], sz = 64 / (sx * sy);
+  const uint32_t total = t.tile_prefix[t.njobs];
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = atomicAdd(t.head, 1u);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    __syncthreads();
+    if (tile >= total) break;
+    int lo = 0, hi = (int)t.njobs - 1;  // job owning this tile
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (t.tile_prefix[mid] <= tile) lo = mid; else hi = mid - 1;
+    }
+    const SdfArgs &a = t.jobs[lo];
+    const uint32_t local = tile - t.tile_prefix[lo];
+    const GridArgs &g = a.col.g;
+    const uint32_t tx = (g.nx + 2 * sx - 1) / (2 * sx), ty = (g.ny + 2 * sy - 1) / (2 * sy);
+    const int bx = (int)(local % tx) * 2 * sx + (wv & 1) * sx;
+    const int by = (int)((local / tx) % ty) * 2 * sy + (wv >> 1) * sy;
+    const int bz = (int)(local / (tx * ty)) * sz;
+    wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, sx, sy, sstack[wv]);
+  }
+}
+
+// resolve for every job of a batch: blockIdx.y = job
+__global__ __launch_bounds__(256) void k_resolve_batch(const SdfArgs *jobs) {
+  resolve_range(jobs[blockIdx.y], blockIdx.x, gridDim.x);
 }
 
 // ============================================================================================
@@ -967,29 +1019,57 @@ void launch_columns(const ColumnArgs &a, hipStream_t s) {
   dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
   k_columns<<<grid, 256, 0, s>>>(a);
 }
+static const int kShapeHost[4][2] = {{4, 4}, {8, 4}, {8, 8}, {16, 4}};
+
+void tile_dims(const GridArgs &g, int shape, uint32_t d[3]) {
+  const int sx = kShapeHost[shape][0], sy = kShapeHost[shape][1], sz = 64 / (sx * sy);
+  d[0] = nb(g.nx, 2 * sx);
+  d[1] = nb(g.ny, 2 * sy);
+  d[2] = nb(g.nz, sz);
+}
+
 template <bool COUNT, bool SLIVERS, bool CAPS>
-static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, dim3 grid, hipStream_t s) {
-  if (variant == 1 && !deep) k_sdf_wave<COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
-  else if (deep) k_sdf<true, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
+static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, hipStream_t s) {
+  uint32_t d[3];
+  tile_dims(a.col.g, shape, d);
+  if (variant == 1 && !deep) {
+    k_sdf_wave<COUNT, SLIVERS, CAPS><<<dim3(d[0], d[1], d[2]), 256, 0, s>>>(a, shape);
+    return;
+  }
+  dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
+  if (deep) k_sdf<true, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
   else k_sdf<false, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
 }
 
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, hipStream_t s) {
-  dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
-  const bool caps = a.col.ncap > 0;
-  const int key = (count ? 4 : 0) | (slivers ? 2 : 0) | (caps ? 1 : 0);
-  switch (key) {
-    case 0: launch_sdf_t<false, false, false>(a, deep, variant, grid, s); break;
-    case 1: launch_sdf_t<false, false, true>(a, deep, variant, grid, s); break;
-    case 2: launch_sdf_t<false, true, false>(a, deep, variant, grid, s); break;
-    case 3: launch_sdf_t<false, true, true>(a, deep, variant, grid, s); break;
-    case 4: launch_sdf_t<true, false, false>(a, deep, variant, grid, s); break;
-    case 5: launch_sdf_t<true, false, true>(a, deep, variant, grid, s); break;
-    case 6: launch_sdf_t<true, true, false>(a, deep, variant, grid, s); break;
-    default: launch_sdf_t<true, true, true>(a, deep, variant, grid, s); break;
+#define IB_DISPATCH(fn, count, slivers, caps, ...)                              \
+  switch (((count) ? 4 : 0) | ((slivers) ? 2 : 0) | ((caps) ? 1 : 0)) {        \
+    case 0: fn<false, false, false>(__VA_ARGS__); break;                        \
+    case 1: fn<false, false, true>(__VA_ARGS__); break;                         \
+    case 2: fn<false, true, false>(__VA_ARGS__); break;                         \
+    case 3: fn<false, true, true>(__VA_ARGS__); break;                          \
+    case 4: fn<true, false, false>(__VA_ARGS__); break;                         \
+    case 5: fn<true, false, true>(__VA_ARGS__); break;                          \
+    case 6: fn<true, true, false>(__VA_ARGS__); break;                          \
+    default: fn<true, true, true>(__VA_ARGS__); break;                          \
   }
+
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, hipStream_t s) {
+  IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, deep, variant, shape, s);
 }
 
 void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, s>>>(a); }
+
+template <bool COUNT, bool SLIVERS, bool CAPS>
+static void launch_batch_t(const BatchArgs &t, uint32_t blocks, hipStream_t s) {
+  k_sdf_batch<COUNT, SLIVERS, CAPS><<<blocks, 256, 0, s>>>(t);
+}
+
+void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s) {
+  IB_DISPATCH(launch_batch_t, count, slivers, caps, t, blocks, s);
+}
+
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s) {
+  if (njobs) k_resolve_batch<<<dim3(16, njobs), 256, 0, s>>>(jobs);
+}
 
 }  // namespace ib

@@ -53,7 +53,8 @@ struct ColumnArgs {
 struct SdfArgs {
   ColumnArgs col;
   float *out;
-  uint32_t *amb_list;  // voxels whose sign the resolve kernel decides (capacity: all voxels)
+  uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
+  uint32_t amb_cap;    // capacity of amb_list
   int *gstack;
   float *gstackd;
   int gstack_depth;
@@ -73,9 +74,22 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
                  hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
+// a batch of query jobs drained by one persistent launch
+struct BatchArgs {
+  const SdfArgs *jobs;          // device array
+  const uint32_t *tile_prefix;  // njobs + 1 (device)
+  uint32_t njobs;
+  uint32_t *head;               // work-queue counter (zeroed before the launch)
+  int shape;                    // brick shape index
+};
+
+// block-tiles of a grid for a brick shape (x, y, z)
+void tile_dims(const GridArgs &g, int shape, uint32_t d[3]);
 // variant 0: per-lane traversal, 1: wave-coherent brick traversal; slivers: the mesh has triangles
-// on the FP64 distance path
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, hipStream_t s);
+// on the FP64 distance path; shape: brick shape of the wave (0: 4x4x4, 1: 8x4x2, 2: 8x8x1, 3: 16x4x1)
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, hipStream_t s);
+void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s);
 // FP64 winding recheck, parity tie-break and column-overflow recount for the listed voxels
 void launch_resolve(const SdfArgs &a, hipStream_t s);
 

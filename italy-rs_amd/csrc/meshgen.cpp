@@ -2,6 +2,8 @@
 #include "meshgen.h"
 
 #include <algorithm>
+#include <atomic>
+#include <thread>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -475,6 +477,21 @@ int ib_gen_stress(uint64_t seed, int ncomp, double remove_frac, ib_soup *out) {
   emit(kept, out);
   set_bounds(out);
   return 0;
+}
+
+int ib_gen_decimated_batch(int n, const int64_t *targets, const uint64_t *seeds, int threads, ib_soup *outs) {
+  if (n < 0 || (n && (!targets || !seeds || !outs))) return -1;
+  if (threads <= 0) threads = (int)std::thread::hardware_concurrency();
+  threads = std::max(1, std::min(threads, n));
+  std::atomic<int> next{0}, err{0};
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&] {
+      for (int i = next++; i < n; i = next++)
+        if (ib_gen_decimated(targets[i], seeds[i], &outs[i]) != 0) err = 1;
+    });
+  for (auto &th : pool) th.join();
+  return err.load() ? -1 : 0;
 }
 
 void ib_soup_free(ib_soup *s) {

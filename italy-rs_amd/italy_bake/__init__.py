@@ -1,2 +1,2 @@
 """Host-side mirror of the reference's bake entry points over the C-ABI library."""
-from ._lib import BakeError, Grid  # noqa: F401
+from ._lib import BakeError, Grid, Job  # noqa: F401

@@ -11,7 +11,12 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import BakeError, Grid, Stats, check
+from ._lib import BakeError, Grid, Job, Stats, check
+
+OPT_COUNT_WORK = 1
+OPT_QUERY_VARIANT = 2
+OPT_BRICK_SHAPE = 3
+OPT_BLOCKS_PER_CU = 4
 
 
 def _f3(v):
@@ -78,6 +83,19 @@ class Context:
                                             ctypes.c_void_p(d_occ) if d_occ else None,
                                             ctypes.c_void_p(stream) if stream else None))
 
+    def bake_batch_device(self, jobs, stream=None):
+        """One batched bake (ib_bake_batch_device) of a list of Job structures; synchronous."""
+        arr = (Job * len(jobs))(*jobs)
+        check(self.lib.ib_bake_batch_device(self._h, arr, len(jobs), ctypes.c_void_p(stream) if stream else None))
+
+    def autotune_batch(self, jobs, sample_stride=16):
+        """ib_autotune_batch: returns (brick shape, blocks per CU, query ms of the sample)."""
+        arr = (Job * len(jobs))(*jobs)
+        shape, blocks, ms = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_double()
+        check(self.lib.ib_autotune_batch(self._h, arr, len(jobs), int(sample_stride), ctypes.byref(shape),
+                                         ctypes.byref(blocks), ctypes.byref(ms)))
+        return shape.value, blocks.value, ms.value
+
     def set_option(self, option, value):
         check(self.lib.ib_set_option(self._h, int(option), int(value)))
 
@@ -85,11 +103,13 @@ class Context:
         """Run one counting bake and return the traversal counters of the query kernel."""
         d_out = self.alloc(4 * sdf_grid.nx * sdf_grid.ny * sdf_grid.nz)
         try:
-            self.set_option(1, 1)
+            self.set_option(OPT_COUNT_WORK, 1)
+            self.set_option(OPT_QUERY_VARIANT, 0)  # per-voxel traversal: the algorithm's own work
             self.bake_dense_device(d_xyz, ntri, sdf_grid, d_out)
             st = self.stats()
         finally:
-            self.set_option(1, 0)
+            self.set_option(OPT_COUNT_WORK, 0)
+            self.set_option(OPT_QUERY_VARIANT, 1)
             self.free(d_out)
         return {"box_tests": st["box_tests"], "tri_tests": st["tri_tests"]}
 
@@ -209,4 +229,4 @@ def dense_grid(lo, hi, res):
 
 
 __all__ = ["Context", "SdfGrid", "VoxelGrid", "bakeSdf", "voxelizeMesh", "sdf_framing", "dense_grid", "BakeError",
-           "Grid"]
+           "Grid", "Job"]

@@ -49,6 +49,8 @@ def _load():
         lib.ib_gen_decimated.argtypes = [ctypes.c_int64, ctypes.c_uint64, S]
         lib.ib_gen_stress.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, S]
         lib.ib_soup_free.argtypes = [S]
+        lib.ib_gen_decimated_batch.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                               ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, S]
         _lib = lib
     return _lib
 
@@ -107,3 +109,33 @@ def workload_plain_large():
     [-0.8, 0.8]^3."""
     m = displaced(7, BASE_SEED)
     return m, dict(lo=-0.8, hi=0.8, sdf_res=256, occ_res=512)
+
+
+def decimated_batch(targets, seeds, threads=0):
+    """Decimated displaced icospheres generated in parallel on the host."""
+    n = len(targets)
+    t = (ctypes.c_int64 * n)(*[int(x) for x in targets])
+    sd = (ctypes.c_uint64 * n)(*[int(x) for x in seeds])
+    outs = (_Soup * n)()
+    lib = _load()
+    if lib.ib_gen_decimated_batch(n, t, sd, int(threads), outs) != 0:
+        raise ValueError("mesh generator failed")
+    meshes = []
+    for i in range(n):
+        k = int(outs[i].ntri)
+        xyz = np.ctypeslib.as_array(outs[i].xyz, (max(k, 1) * 9,)).copy()[: k * 9]
+        meshes.append(Mesh(xyz, list(outs[i].bmin), list(outs[i].bmax)))
+        lib.ib_soup_free(ctypes.byref(outs[i]))
+    return meshes
+
+
+def workload_batched(n=512, seed=BASE_SEED + 4096, threads=0):
+    """n displaced icospheres, triangle counts log-uniform in [500, 200k] (subdivision level, then
+    random edge collapses down to the count), each with a dense SDF + occupancy grid over
+    [-0.8, 0.8]^3 at a resolution drawn uniformly from {32, 64, 128, 256}.
+    Returns (meshes, resolutions)."""
+    rng = np.random.default_rng(seed)
+    targets = np.exp(rng.uniform(np.log(500.0), np.log(200000.0), n)).astype(np.int64)
+    res = rng.choice([32, 64, 128, 256], n)
+    seeds = [seed + 1 + i for i in range(n)]
+    return decimated_batch(targets, seeds, threads), [int(r) for r in res]

@@ -1,0 +1,130 @@
+// The reference's two hot-path tests, run through the C++ drop-ins of include/italy_bake.hpp:
+//   tests/sdf_sign_test.cpp:46-82  (UV sphere r=1, 32x16, bakeSdf res 24, sign + magnitude)
+//   tests/voxelize_test.cpp:37-58  (unit cube shell, voxelizeMesh res 8, membership)
+// Checks print a line and set a non-zero exit status instead of relying on assert.
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "italy_bake.hpp"
+
+namespace {
+
+struct vec3 {  // three packed floats, like glm::vec3
+  float x, y, z;
+};
+
+struct Mesh {  // the fields of italy::MeshAsset the bake reads
+  std::vector<vec3> positions;
+  vec3 boundsMin{0, 0, 0}, boundsMax{0, 0, 0};
+};
+
+int failures = 0;
+void expect(bool ok, const char *what) {
+  if (!ok) {
+    std::printf("FAILED: %s\n", what);
+    ++failures;
+  }
+}
+
+Mesh makeUvSphere(float r, int lonSegs, int latSegs) {  // sdf_sign_test.cpp:18-42
+  Mesh mesh;
+  auto vertex = [&](int lat, int lon) {
+    const float theta = static_cast<float>(lat) / latSegs * static_cast<float>(M_PI);
+    const float phi = static_cast<float>(lon) / lonSegs * 2.0f * static_cast<float>(M_PI);
+    return vec3{r * std::sin(theta) * std::cos(phi), r * std::cos(theta), r * std::sin(theta) * std::sin(phi)};
+  };
+  for (int lat = 0; lat < latSegs; ++lat)
+    for (int lon = 0; lon < lonSegs; ++lon) {
+      const vec3 a = vertex(lat, lon), b = vertex(lat + 1, lon), c = vertex(lat + 1, lon + 1), d = vertex(lat, lon + 1);
+      for (vec3 v : {a, b, c, a, c, d}) mesh.positions.push_back(v);
+    }
+  mesh.boundsMin = {-r, -r, -r};
+  mesh.boundsMax = {r, r, r};
+  return mesh;
+}
+
+void addQuad(Mesh &mesh, vec3 a, vec3 b, vec3 c, vec3 d) {  // voxelize_test.cpp:11-17
+  for (vec3 v : {a, b, c, a, c, d}) mesh.positions.push_back(v);
+}
+
+Mesh makeUnitCubeShell() {  // voxelize_test.cpp:19-33
+  Mesh mesh;
+  const vec3 lo{0, 0, 0}, hi{1, 1, 1};
+  addQuad(mesh, {lo.x, lo.y, lo.z}, {hi.x, lo.y, lo.z}, {hi.x, hi.y, lo.z}, {lo.x, hi.y, lo.z});
+  addQuad(mesh, {lo.x, lo.y, hi.z}, {lo.x, hi.y, hi.z}, {hi.x, hi.y, hi.z}, {hi.x, lo.y, hi.z});
+  addQuad(mesh, {lo.x, lo.y, lo.z}, {lo.x, hi.y, lo.z}, {lo.x, hi.y, hi.z}, {lo.x, lo.y, hi.z});
+  addQuad(mesh, {hi.x, lo.y, lo.z}, {hi.x, lo.y, hi.z}, {hi.x, hi.y, hi.z}, {hi.x, hi.y, lo.z});
+  addQuad(mesh, {lo.x, lo.y, lo.z}, {lo.x, lo.y, hi.z}, {hi.x, lo.y, hi.z}, {hi.x, lo.y, lo.z});
+  addQuad(mesh, {lo.x, hi.y, lo.z}, {hi.x, hi.y, lo.z}, {hi.x, hi.y, hi.z}, {lo.x, hi.y, hi.z});
+  mesh.boundsMin = lo;
+  mesh.boundsMax = hi;
+  return mesh;
+}
+
+void sdfSignTest() {
+  const float radius = 1.0f;
+  const Mesh sphere = makeUvSphere(radius, 32, 16);
+  const italy_bake::SdfGrid grid = italy_bake::bakeSdf(sphere, 24);
+  expect(!grid.distances.empty(), "distances non-empty");
+  const float tolerance = 1.5f * grid.voxelSize;
+  size_t checked = 0, wrongSign = 0;
+  double maxAbsError = 0.0;
+  for (int z = 0; z < grid.nz; ++z)
+    for (int y = 0; y < grid.ny; ++y)
+      for (int x = 0; x < grid.nx; ++x) {
+        const float cx = grid.origin[0] + (static_cast<float>(x) + 0.5f) * grid.voxelSize;
+        const float cy = grid.origin[1] + (static_cast<float>(y) + 0.5f) * grid.voxelSize;
+        const float cz = grid.origin[2] + (static_cast<float>(z) + 0.5f) * grid.voxelSize;
+        const float analytic = std::sqrt(cx * cx + cy * cy + cz * cz) - radius;
+        const float baked = grid.distances[grid.index(x, y, z)];
+        if (std::fabs(analytic) > tolerance) {
+          ++checked;
+          if ((analytic < 0.0f) != (baked < 0.0f)) ++wrongSign;
+          maxAbsError = std::max(maxAbsError, static_cast<double>(std::fabs(baked - analytic)));
+        }
+      }
+  std::printf("sdf_sign_test: %zu/%zu cells checked away from the surface, max |error| = %.4f (tolerance %.4f)\n",
+              checked - wrongSign, checked, maxAbsError, static_cast<double>(tolerance));
+  expect(checked > 0, "cells outside the tolerance band");
+  expect(wrongSign == 0, "no sign disagreement with the analytic sphere");
+  expect(maxAbsError < 2.0 * tolerance, "magnitude within 3h of the analytic sphere");
+  // materials: the default material everywhere it was splatted, defaults elsewhere
+  expect(grid.baseColor.size() == grid.distances.size(), "per-cell materials present");
+  bool matOk = true;
+  for (size_t i = 0; i < grid.baseColor.size(); ++i)
+    matOk &= grid.baseColor[i][0] == 1.0f && grid.metallic[i] == 0.0f && grid.roughness[i] == 1.0f;
+  expect(matOk, "default material");
+}
+
+void voxelizeTest() {
+  const int resolution = 8;
+  const Mesh cube = makeUnitCubeShell();
+  const auto grid = italy_bake::voxelizeMesh(cube, resolution);
+  const size_t maxShellCells = static_cast<size_t>(6 * resolution * resolution);
+  expect(!grid.cells.empty(), "cells non-empty");
+  expect(grid.cells.size() <= maxShellCells, "at most the shell's cells");
+  bool foundCorner = false, foundCenter = false;
+  for (const auto &c : grid.cells) {
+    if (c[0] == 0 && c[1] == 0 && c[2] == 0) foundCorner = true;
+    if (c[0] == 4 && c[1] == 4 && c[2] == 4) foundCenter = true;
+  }
+  expect(foundCorner, "corner cell should be on the shell");
+  expect(!foundCenter, "center cell is interior");
+  expect(grid.colors.size() == grid.cells.size(), "one colour per cell");
+  std::printf("voxelize_test: %zu cells\n", grid.cells.size());
+}
+
+}  // namespace
+
+int main() {
+  try {
+    sdfSignTest();
+    voxelizeTest();
+  } catch (const std::exception &e) {
+    std::printf("FAILED: exception %s\n", e.what());
+    return 2;
+  }
+  std::printf(failures ? "dropin_test: FAILED\n" : "dropin_test: OK\n");
+  return failures ? 1 : 0;
+}
