@@ -81,8 +81,12 @@ typedef enum {
   IB_OPT_BLOCKS_PER_CU = 4  /* persistent blocks per compute unit in batched bakes (default 8) */
 } ib_option;
 
-/* One mesh of a batch: device soup, optional dense SDF grid (sdf.nx == 0: none) and optional
- * occupancy grid (occ.nx == 0: none), device outputs laid out as for ib_bake_dense_device. */
+/* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy
+ * grid (occ.nx == 0: none), device outputs laid out as for ib_bake_dense_device.
+ * Narrow band (band_voxels > 0, grid dims divisible by 8): only the 8^3 bricks that can hold a voxel
+ * with |d| <= band_voxels * h are baked.  d_brick_index gets, per brick (x fastest), its slot in the
+ * pool or -1; d_distances is the pool: slot * 512 + (z%8 * 8 + y%8) * 8 + x%8; nbricks is written
+ * back (on IB_ERR_CAPACITY it is the size the pool needs). */
 typedef struct {
   const float *d_xyz;
   uint64_t ntri;
@@ -90,6 +94,11 @@ typedef struct {
   float *d_distances;
   ib_grid occ;
   uint32_t *d_occ_words;
+  int32_t band_voxels;
+  int32_t reserved;
+  int32_t *d_brick_index;
+  uint64_t pool_bricks;
+  uint64_t nbricks;
 } ib_job;
 
 /* ---- context ------------------------------------------------------------------------------ */
@@ -140,7 +149,12 @@ ib_status ib_bake_dense_device(ib_context *ctx, const float *d_xyz, uint64_t ntr
  * all jobs from a device work queue.  Synchronous: returns when every output is written.  Results
  * are identical to baking each job alone with ib_bake_dense_device (no counterpart in the
  * reference, which bakes one mesh per call: src/convert/sdf_baker.cpp:137-310). */
-ib_status ib_bake_batch_device(ib_context *ctx, const ib_job *jobs, uint32_t njobs, void *stream);
+ib_status ib_bake_batch_device(ib_context *ctx, ib_job *jobs, uint32_t njobs, void *stream);
+
+/* Narrow-band bake of one mesh (the configs' sparse-brick output): a batch of one band job. */
+ib_status ib_bake_band_device(ib_context *ctx, const float *d_xyz, uint64_t ntri, const ib_grid *grid,
+                              int32_t band_voxels, int32_t *d_brick_index, float *d_pool, uint64_t pool_bricks,
+                              uint64_t *nbricks, void *stream);
 
 /* Autotuner for batched bakes: bakes every sample_stride-th job of `jobs` with each brick shape
  * (IB_OPT_BRICK_SHAPE) and persistent-block count (IB_OPT_BLOCKS_PER_CU), times the persistent

@@ -176,6 +176,94 @@ void build_caps(const std::vector<int2> &edges, const std::vector<float> &soup, 
   }
 }
 
+// Hierarchy over the cap triangles: median splits on the longest centroid axis, leaves of <= 4
+// triangles.  Node bounds are computed in double and rounded outwards so that the device's FP32
+// bound area / dmin^2 stays an upper bound.
+int cap_node(std::vector<float> &tris, std::vector<int> &ids, int lo, int hi, std::vector<CapNode> &nodes,
+             std::vector<float> &out) {
+  const int me = (int)nodes.size();
+  nodes.emplace_back();
+  double cmin[3] = {1e300, 1e300, 1e300}, cmax[3] = {-1e300, -1e300, -1e300};
+  double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
+  double area = 0, dip[3] = {0, 0, 0}, ac[3] = {0, 0, 0};
+  for (int i = lo; i < hi; ++i) {
+    const float *t = &tris[9 * (size_t)ids[i]];
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+      const double cen = ((double)t[k] + t[3 + k] + t[6 + k]) / 3.0;
+      cmin[k] = std::min(cmin[k], cen);
+      cmax[k] = std::max(cmax[k], cen);
+      for (int v = 0; v < 3; ++v) {
+        bl[k] = std::min(bl[k], (double)t[3 * v + k]);
+        bh[k] = std::max(bh[k], (double)t[3 * v + k]);
+      }
+      e1[k] = (double)t[3 + k] - t[k];
+      e2[k] = (double)t[6 + k] - t[k];
+    }
+    const double n[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                         0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
+    const double a = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    area += a;
+    for (int k = 0; k < 3; ++k) {
+      dip[k] += n[k];
+      ac[k] += a * ((double)t[k] + t[3 + k] + t[6 + k]) / 3.0;
+    }
+  }
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = area > 0 ? ac[k] / area : 0.5 * (bl[k] + bh[k]);
+  double r = 0;
+  for (int i = lo; i < hi; ++i) {
+    const float *t = &tris[9 * (size_t)ids[i]];
+    for (int v = 0; v < 3; ++v) {
+      const double dx = t[3 * v] - c[0], dy = t[3 * v + 1] - c[1], dz = t[3 * v + 2] - c[2];
+      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+    }
+  }
+  CapNode nd{};
+  for (int k = 0; k < 3; ++k) {
+    nd.c[k] = (float)c[k];
+    nd.dip[k] = (float)dip[k];
+    nd.lo[k] = std::nextafter((float)bl[k], -INFINITY);
+    nd.hi[k] = std::nextafter((float)bh[k], INFINITY);
+  }
+  // the float centre moved by at most one ulp per axis: widen r by that, then by a relative margin
+  nd.r = (float)((r + 1e-6 * (std::fabs(c[0]) + std::fabs(c[1]) + std::fabs(c[2]))) * (1.0 + 1e-5));
+  nd.area = (float)(area * (1.0 + 1e-5));
+  if (hi - lo <= 4) {
+    nd.left = -1 - (int)(out.size() / 9);
+    nd.right = hi - lo;
+    for (int i = lo; i < hi; ++i) out.insert(out.end(), &tris[9 * (size_t)ids[i]], &tris[9 * (size_t)ids[i]] + 9);
+    nodes[me] = nd;
+    return me;
+  }
+  int axis = 0;
+  for (int k = 1; k < 3; ++k)
+    if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
+  const int mid = (lo + hi) / 2;
+  auto key = [&](int id) {
+    const float *t = &tris[9 * (size_t)id];
+    return ((double)t[axis] + t[3 + axis] + t[6 + axis]);
+  };
+  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
+                   [&](int a, int b) { return key(a) != key(b) ? key(a) < key(b) : a < b; });
+  const int l = cap_node(tris, ids, lo, mid, nodes, out);
+  const int rr = cap_node(tris, ids, mid, hi, nodes, out);
+  nd.left = l;
+  nd.right = rr;
+  nodes[me] = nd;
+  return me;
+}
+
+void build_cap_hierarchy(std::vector<float> &tris, std::vector<CapNode> &nodes, std::vector<float> &ordered) {
+  nodes.clear();
+  ordered.clear();
+  const int n = (int)(tris.size() / 9);
+  if (n == 0) return;
+  std::vector<int> ids(n);
+  for (int i = 0; i < n; ++i) ids[i] = i;
+  cap_node(tris, ids, 0, n, nodes, ordered);
+}
+
 float ev_ms(ib_context *c, EventId a, EventId b) {
   if (!c->ev_valid[a] || !c->ev_valid[b]) return 0.0f;
   float ms = 0.0f;
@@ -209,6 +297,8 @@ struct MeshDev {
   float4 *tris = nullptr;
   float *caps = nullptr;
   int ncap = 0;
+  CapNode *cap_nodes = nullptr;
+  int ncapnode = 0;
   int depth = 0;
   bool slivers = false;
 };
@@ -310,11 +400,16 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   IB_HIP(hipMemcpyAsync(soup.data(), d_xyz, soup.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
   std::sort(edges.begin(), edges.end(), [](int2 a, int2 b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
-  std::vector<float> caps;
+  std::vector<float> caps, ordered;
   build_caps(edges, soup, caps);
-  m.ncap = (int)(caps.size() / 9);
-  IB_TRY(take(c, arena, "caps", caps.size(), &m.caps));
-  IB_HIP(hipMemcpyAsync(m.caps, caps.data(), caps.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  std::vector<CapNode> nodes;
+  build_cap_hierarchy(caps, nodes, ordered);
+  m.ncap = (int)(ordered.size() / 9);
+  m.ncapnode = (int)nodes.size();
+  IB_TRY(take(c, arena, "caps", ordered.size(), &m.caps));
+  IB_TRY(take(c, arena, "cap_nodes", nodes.size(), &m.cap_nodes));
+  IB_HIP(hipMemcpyAsync(m.caps, ordered.data(), ordered.size() * sizeof(float), hipMemcpyHostToDevice, s));
+  IB_HIP(hipMemcpyAsync(m.cap_nodes, nodes.data(), nodes.size() * sizeof(CapNode), hipMemcpyHostToDevice, s));
   IB_HIP(hipStreamSynchronize(s));
   return IB_OK;
 }
@@ -329,6 +424,8 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.tris = m.tris;
   ca.caps = m.caps;
   ca.ncap = m.ncap;
+  ca.cap_nodes = m.cap_nodes;
+  ca.ncapnode = m.ncapnode;
   ca.g = to_args(g);
   ca.sc = sc;
   IB_TRY(take(c, arena, "col_cross", ncol * kColCap, &ca.cross));
@@ -725,23 +822,37 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   return IB_OK;
 }
 
-ib_status ib_bake_batch_device(ib_context *c, const ib_job *jobs, uint32_t njobs, void *stream) {
+// Batched bake (dense and narrow-band jobs): build every LBVH, one readback; narrow-band jobs get
+// a coarse unsigned pass at brick centres and a brick compaction (second readback for the brick
+// counts); then columns, ONE persistent query launch over all tiles, one resolve launch.
+ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void *stream) {
   if (!c || (njobs && !jobs)) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
   reset_stats(c);
-  for (uint32_t j = 0; j < njobs; ++j) {
-    const ib_job &jb = jobs[j];
-    const bool has_sdf = jb.sdf.nx != 0, has_occ = jb.occ.nx != 0;
-    if ((has_sdf && (!grid_ok(&jb.sdf) || !jb.d_distances)) || (has_occ && (!grid_ok(&jb.occ) || !jb.d_occ_words)))
-      return fail(IB_ERR_INVALID_ARGUMENT, "job " + std::to_string(j) + ": invalid grid or missing output");
-    if (jb.ntri > (uint64_t)(INT_MAX / 9) || (jb.ntri && !jb.d_xyz))
-      return fail(IB_ERR_INVALID_ARGUMENT, "job " + std::to_string(j) + ": invalid triangle buffer");
-    if (has_sdf && (uint64_t)jb.sdf.nx * jb.sdf.ny * jb.sdf.nz >= (1ull << 32))
-      return fail(IB_ERR_UNSUPPORTED, "job " + std::to_string(j) + ": grid too large");
-  }
   if (njobs > 65535) return fail(IB_ERR_UNSUPPORTED, "at most 65535 jobs per batch");
+  for (uint32_t j = 0; j < njobs; ++j) {
+    ib_job &jb = jobs[j];
+    const std::string tag = "job " + std::to_string(j) + ": ";
+    const bool has_sdf = jb.sdf.nx != 0, has_occ = jb.occ.nx != 0;
+    // a narrow-band job may pass no pool with pool_bricks = 0 to learn the pool size (IB_ERR_CAPACITY)
+    const bool sizing = jb.band_voxels > 0 && jb.pool_bricks == 0;
+    if ((has_sdf && (!grid_ok(&jb.sdf) || (!jb.d_distances && !sizing))) ||
+        (has_occ && (!grid_ok(&jb.occ) || !jb.d_occ_words)))
+      return fail(IB_ERR_INVALID_ARGUMENT, tag + "invalid grid or missing output");
+    if (jb.ntri > (uint64_t)(INT_MAX / 9) || (jb.ntri && !jb.d_xyz))
+      return fail(IB_ERR_INVALID_ARGUMENT, tag + "invalid triangle buffer");
+    if (has_sdf && (uint64_t)jb.sdf.nx * jb.sdf.ny * jb.sdf.nz >= (1ull << 32))
+      return fail(IB_ERR_UNSUPPORTED, tag + "grid too large");
+    if (jb.band_voxels < 0) return fail(IB_ERR_INVALID_ARGUMENT, tag + "negative band");
+    if (jb.band_voxels > 0) {
+      if (!has_sdf || (jb.sdf.nx | jb.sdf.ny | jb.sdf.nz) & 7)
+        return fail(IB_ERR_INVALID_ARGUMENT, tag + "a narrow band needs an SDF grid with dims divisible by 8");
+      if (!jb.d_brick_index) return fail(IB_ERR_INVALID_ARGUMENT, tag + "missing brick index");
+    }
+    jb.nbricks = 0;
+  }
   if (c->batch_sc_cap < njobs) {
     if (c->host_batch_sc) (void)hipHostFree(c->host_batch_sc);
     c->host_batch_sc = nullptr;
@@ -768,10 +879,17 @@ ib_status ib_bake_batch_device(ib_context *c, const ib_job *jobs, uint32_t njobs
   IB_HIP(hipStreamSynchronize(s));
   bool any_slivers = false, any_caps = false;
   for (uint32_t j = 0; j < njobs; ++j) {
-    const ib_job &jb = jobs[j];
+    ib_job &jb = jobs[j];
     if (jb.ntri == 0) {
-      IB_TRY(fill_empty(c, jb.sdf.nx ? &jb.sdf : nullptr, jb.d_distances, jb.occ.nx ? &jb.occ : nullptr,
-                        jb.d_occ_words, s));
+      if (jb.band_voxels > 0) {  // no surface: no bricks
+        const ib_grid &g = jb.sdf;
+        fill_u32(reinterpret_cast<uint32_t *>(jb.d_brick_index), 0xffffffffu,
+                 (uint64_t)(g.nx / 8) * (g.ny / 8) * (g.nz / 8), s);
+        if (jb.occ.nx) IB_TRY(fill_empty(c, nullptr, nullptr, &jb.occ, jb.d_occ_words, s));
+      } else {
+        IB_TRY(fill_empty(c, jb.sdf.nx ? &jb.sdf : nullptr, jb.d_distances, jb.occ.nx ? &jb.occ : nullptr,
+                          jb.d_occ_words, s));
+      }
       continue;
     }
     const ib_status st = finish_build(c, jb.d_xyz, (int)jb.ntri, c->host_batch_sc[j], mesh[j], &arena, false, s);
@@ -783,21 +901,104 @@ ib_status ib_bake_batch_device(ib_context *c, const ib_job *jobs, uint32_t njobs
     c->stats.bvh_depth = std::max(c->stats.bvh_depth, mesh[j].depth);
   }
   c->record(EV_OCC, s);
-  // 2. columns + per-job query blocks, tile prefix
+  // 2. narrow band: unsigned distance at every brick centre (a cell-centred grid of spacing 8h),
+  //    bricks within band*h + 3.5*sqrt(3)*h of the surface (1-Lipschitz: every voxel of the band)
+  std::vector<uint32_t> band_jobs;
+  std::vector<uint32_t *> band_bricks(njobs, nullptr), band_count(njobs, nullptr);
+  for (uint32_t j = 0; j < njobs; ++j)
+    if (jobs[j].band_voxels > 0 && jobs[j].ntri) band_jobs.push_back(j);
+  if (!band_jobs.empty()) {
+    std::vector<SdfArgs> coarse;
+    std::vector<uint32_t> cprefix(1, 0);
+    std::vector<float *> cdist(njobs, nullptr);
+    for (uint32_t j : band_jobs) {
+      const ib_grid &g = jobs[j].sdf;
+      ib_grid cg = g;
+      cg.h = g.h * 8.0f;
+      cg.nx = g.nx / 8;
+      cg.ny = g.ny / 8;
+      cg.nz = g.nz / 8;
+      IB_TRY(arena.get((uint64_t)cg.nx * cg.ny * cg.nz, &cdist[j]));
+      SdfArgs sa{};
+      sa.col.nodes = mesh[j].nodes;
+      sa.col.nnodes = std::max(mesh[j].n - 1, 1);
+      sa.col.tris = mesh[j].tris;
+      sa.col.g = to_args(cg);
+      sa.col.sc = scs + j;
+      sa.out = cdist[j];
+      sa.unsigned_only = 1;
+      uint32_t d[3];
+      tile_dims(sa.col.g, 0, d);
+      coarse.push_back(sa);
+      cprefix.push_back(cprefix.back() + d[0] * d[1] * d[2]);
+    }
+    SdfArgs *d_c;
+    uint32_t *d_cp, *d_head;
+    IB_TRY(arena.get(coarse.size(), &d_c));
+    IB_TRY(arena.get(cprefix.size(), &d_cp));
+    IB_TRY(arena.get(1, &d_head));
+    IB_HIP(hipMemcpyAsync(d_c, coarse.data(), coarse.size() * sizeof(SdfArgs), hipMemcpyHostToDevice, s));
+    IB_HIP(hipMemcpyAsync(d_cp, cprefix.data(), cprefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    fill_u32(d_head, 0u, 1, s);
+    BatchArgs t{d_c, d_cp, (uint32_t)coarse.size(), d_head, 0};
+    launch_sdf_batch(t, std::max(1u, std::min<uint32_t>(cprefix.back(), (uint32_t)(c->cus * 8))), false, any_slivers,
+                     false, s);
+    std::vector<uint32_t> host_counts(band_jobs.size());
+    uint32_t *d_counts;
+    IB_TRY(arena.get(band_jobs.size(), &d_counts));
+    for (size_t k = 0; k < band_jobs.size(); ++k) {
+      const uint32_t j = band_jobs[k];
+      const ib_grid &g = jobs[j].sdf;
+      const uint32_t nbrick = (uint32_t)((g.nx / 8) * (g.ny / 8) * (g.nz / 8));
+      const float thresh = g.h * ((float)jobs[j].band_voxels + 6.0621778f);  // 3.5 * sqrt(3)
+      uint32_t *flags, *slots;
+      IB_TRY(arena.get(nbrick + 1, &flags));
+      IB_TRY(arena.get(nbrick + 1, &slots));
+      IB_TRY(arena.get(nbrick, &band_bricks[j]));
+      launch_band_flags(cdist[j], nbrick, thresh, flags, s);
+      size_t tb = 0;
+      IB_HIP(exclusive_scan_u32(nullptr, tb, flags, slots, nbrick, s));
+      char *temp;
+      IB_TRY(c->scratch("band_scan", tb, &temp));
+      IB_HIP(exclusive_scan_u32(temp, tb, flags, slots, nbrick, s));
+      launch_band_scatter(flags, slots, nbrick, jobs[j].d_brick_index, band_bricks[j], d_counts + k, s);
+    }
+    IB_HIP(hipMemcpyAsync(host_counts.data(), d_counts, host_counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                          s));
+    IB_HIP(hipStreamSynchronize(s));
+    bool short_pool = false;
+    for (size_t k = 0; k < band_jobs.size(); ++k) {
+      ib_job &jb = jobs[band_jobs[k]];
+      jb.nbricks = host_counts[k];
+      if (jb.nbricks > jb.pool_bricks) short_pool = true;
+    }
+    if (short_pool) return fail(IB_ERR_CAPACITY, "a brick pool is smaller than its narrow band (see nbricks)");
+  }
+  // 3. columns + per-job query blocks, tile prefix
   std::vector<SdfArgs> args;
   std::vector<uint32_t> owner, prefix(1, 0);
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
     if (jb.ntri == 0 || jb.sdf.nx == 0) continue;
-    const uint64_t nvox = (uint64_t)jb.sdf.nx * jb.sdf.ny * jb.sdf.nz;
+    const bool band = jb.band_voxels > 0;
+    const uint64_t nvox = band ? jb.nbricks * 512ull : (uint64_t)jb.sdf.nx * jb.sdf.ny * jb.sdf.nz;
+    if (band && jb.nbricks == 0) continue;
     SdfArgs sa;
     const uint32_t amb_cap = (uint32_t)std::min<uint64_t>(nvox, nvox / 16 + 4096);
     IB_TRY(prepare_query(c, mesh[j], jb.sdf, jb.d_distances, scs + j, &arena, amb_cap, sa, s));
-    uint32_t d[3];
-    tile_dims(to_args(jb.sdf), c->brick_shape, d);
+    uint32_t tiles;
+    if (band) {
+      sa.band_index = jb.d_brick_index;
+      sa.band_bricks = band_bricks[j];
+      tiles = (uint32_t)(2 * jb.nbricks);
+    } else {
+      uint32_t d[3];
+      tile_dims(to_args(jb.sdf), c->brick_shape, d);
+      tiles = d[0] * d[1] * d[2];
+    }
     args.push_back(sa);
     owner.push_back(j);
-    prefix.push_back(prefix.back() + d[0] * d[1] * d[2]);
+    prefix.push_back(prefix.back() + tiles);
     c->stats.voxels += nvox;
   }
   c->record(EV_COLS, s);
@@ -818,7 +1019,7 @@ ib_status ib_bake_batch_device(ib_context *c, const ib_job *jobs, uint32_t njobs
   c->record(EV_SDF, s);
   IB_HIP(hipMemcpyAsync(c->host_batch_sc, scs, sizeof(Scalars) * njobs, hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
-  // 3. a job whose ambiguous-voxel list overflowed is baked again on its own (full-size list)
+  // 4. a job whose ambiguous-voxel list overflowed is queried again on its own with a full list
   for (size_t k = 0; k < args.size(); ++k) {
     const uint32_t j = owner[k];
     const Scalars &h = c->host_batch_sc[j];
@@ -828,24 +1029,51 @@ ib_status ib_bake_batch_device(ib_context *c, const ib_job *jobs, uint32_t njobs
     c->stats.box_tests += h.box_tests;
     c->stats.tri_tests += h.tri_tests;
     if (h.ambiguous > args[k].amb_cap) {
+      SdfArgs sa = args[k];
       const ib_job &jb = jobs[j];
-      MeshDev m;
-      Scalars *sc;
-      IB_TRY(c->scratch("sc", 1, &sc));
-      IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, sc, m, nullptr, s));
-      IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-      IB_HIP(hipStreamSynchronize(s));
-      IB_TRY(finish_build(c, jb.d_xyz, (int)jb.ntri, *c->host_sc, m, nullptr, true, s));
-      IB_TRY(query_dense(c, m, jb.sdf, jb.d_distances, s));
+      const uint64_t nvox = jb.band_voxels > 0 ? jb.nbricks * 512ull : (uint64_t)jb.sdf.nx * jb.sdf.ny * jb.sdf.nz;
+      fill_u32(reinterpret_cast<uint32_t *>(scs + j), 0u, sizeof(Scalars) / 4, s);
+      sa.amb_cap = (uint32_t)nvox;
+      IB_TRY(arena.get(nvox, &sa.amb_list));
+      SdfArgs *d_one;
+      uint32_t *d_pf, *d_hd;
+      IB_TRY(arena.get(1, &d_one));
+      IB_TRY(arena.get(2, &d_pf));
+      IB_TRY(arena.get(1, &d_hd));
+      const uint32_t pf[2] = {0, prefix[k + 1] - prefix[k]};
+      IB_HIP(hipMemcpyAsync(d_one, &sa, sizeof(SdfArgs), hipMemcpyHostToDevice, s));
+      IB_HIP(hipMemcpyAsync(d_pf, pf, sizeof(pf), hipMemcpyHostToDevice, s));
+      fill_u32(d_hd, 0u, 1, s);
+      BatchArgs t{d_one, d_pf, 1, d_hd, jb.band_voxels > 0 ? 0 : c->brick_shape};
+      launch_sdf_batch(t, std::max(1u, std::min<uint32_t>(pf[1], (uint32_t)(c->cus * c->blocks_per_cu))), false,
+                       any_slivers, any_caps, s);
+      launch_resolve_batch(d_one, 1, s);
       IB_HIP(hipStreamSynchronize(s));
     }
   }
   return IB_OK;
 }
 
+ib_status ib_bake_band_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib_grid *grid,
+                              int32_t band_voxels, int32_t *d_brick_index, float *d_pool, uint64_t pool_bricks,
+                              uint64_t *nbricks, void *stream) {
+  if (!grid || band_voxels <= 0 || !nbricks) return fail(IB_ERR_INVALID_ARGUMENT, "invalid narrow-band request");
+  ib_job jb{};
+  jb.d_xyz = d_xyz;
+  jb.ntri = ntri;
+  jb.sdf = *grid;
+  jb.d_distances = d_pool;
+  jb.band_voxels = band_voxels;
+  jb.d_brick_index = d_brick_index;
+  jb.pool_bricks = pool_bricks;
+  const ib_status st = ib_bake_batch_device(c, &jb, 1, stream);
+  *nbricks = jb.nbricks;
+  return st;
+}
 
 ib_status ib_autotune_batch(ib_context *c, const ib_job *jobs, uint32_t njobs, uint32_t sample_stride,
                             int32_t *out_shape, int32_t *out_blocks, double *out_ms) {
+  // the band pass uses fixed 4x4x4 sub-bricks; the shape and block count tune the dense tiles
   if (!c || (njobs && !jobs)) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
   if (sample_stride == 0) sample_stride = 1;
   std::vector<ib_job> sample;

@@ -20,6 +20,22 @@ struct alignas(16) Node {
 };
 static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
 
+// Node of the hole-cap hierarchy (built on the host).  left >= 0: children left/right;
+// left < 0: leaf holding triangles [-1 - left, -1 - left + right) of the reordered cap soup.
+// (c, r) bounds every vertex below; area = sum of |triangle area|; dip = sum of area vectors.
+struct alignas(16) CapNode {
+  float c[3], r;
+  float dip[3], area;
+  float lo[3];
+  int32_t left;
+  float hi[3];
+  int32_t right;
+};
+static_assert(sizeof(CapNode) == 64, "cap node record is one 64-byte line");
+constexpr int kCapStack = 48;
+// a cap node farther than this bound on |its solid angle| / 4pi is not evaluated, only bounded
+constexpr float kCapTau = 1e-3f;
+
 // ---------------------------------------------------------------------------------------------
 // triangle / AABB separating axis test: same expression trees as src/convert/tri_box_overlap.h
 // ---------------------------------------------------------------------------------------------

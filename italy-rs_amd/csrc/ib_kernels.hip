@@ -487,6 +487,26 @@ __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes
   }
 }
 
+// every cap triangle whose box contains (X, Y) in xy
+template <typename F>
+__device__ __forceinline__ void for_column_caps(const ColumnArgs &a, float X, float Y, F &&tri) {
+  if (a.ncap == 0) return;
+  int stack[kCapStack];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const CapNode nd = a.cap_nodes[stack[--sp]];
+    if (!(X >= nd.lo[0] && X <= nd.hi[0] && Y >= nd.lo[1] && Y <= nd.hi[1])) continue;
+    if (nd.left < 0) {
+      const int first = -1 - nd.left;
+      for (int k = 0; k < nd.right; ++k) tri(a.caps + 9 * (first + k));
+    } else if (sp + 2 <= kCapStack) {
+      stack[sp++] = nd.left;
+      stack[sp++] = nd.right;
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
   const int x = blockIdx.x * 16 + (threadIdx.x & 15);
   const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
@@ -501,12 +521,11 @@ __global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
     if (s) sink.add(z, s);
   });
-  for (int c = 0; c < a.ncap; ++c) {  // caps enter the closed surface reversed
-    const float *t = a.caps + 9 * c;
+  for_column_caps(a, X, Y, [&](const float *t) {  // caps enter the closed surface reversed
     float z;
     const int s = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
     if (s) sink.add(z, -s);
-  }
+  });
   a.count[col] = sink.cnt;
   if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
 }
@@ -521,12 +540,11 @@ __device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y,
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
     if (s && z > qz) w += s;
   });
-  for (int c = 0; c < a.ncap; ++c) {
-    const float *t = a.caps + 9 * c;
+  for_column_caps(a, X, Y, [&](const float *t) {
     float z;
     const int s = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
     if (s && z > qz) w -= s;
-  }
+  });
   return w;
 }
 
@@ -542,6 +560,45 @@ __device__ __forceinline__ float solid_angle_f32(F3 q, const float *t) {
   const float det = dot3(a, bc);
   const float den = la * lb * lc + dot3(a, b) * lc + dot3(b, c) * la + dot3(c, a) * lb;
   return 2.0f * atan2f(det, den);
+}
+
+// Cap winding with a rigorous error bound: caps near q are summed exactly (FP32 solid angles,
+// Neumaier), every farther node only contributes its bound |Omega| <= area / dmin^2 (a surface of
+// that area seen from at least dmin) to `bound`, in winding units.
+__device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, float &w, float &bound) {
+  const float inv4pi = (float)(1.0 / (4.0 * M_PI));
+  float sum = 0.0f, cmp = 0.0f, ub = 0.0f;
+  int stack[kCapStack];
+  int sp = 0;
+  stack[sp++] = 0;
+  while (sp > 0) {
+    const CapNode nd = a.cap_nodes[stack[--sp]];
+    const float dx = q.x - nd.c[0], dy = q.y - nd.c[1], dz = q.z - nd.c[2];
+    const float dmin = sqrtf(dx * dx + dy * dy + dz * dz) - nd.r;
+    if (dmin > 0.0f) {
+      const float b = nd.area * inv4pi / (dmin * dmin);
+      if (b <= kCapTau) {
+        ub += b;
+        continue;
+      }
+    }
+    if (nd.left < 0) {
+      const int first = -1 - nd.left;
+      for (int k = 0; k < nd.right; ++k) {
+        const float om = solid_angle_f32(q, a.caps + 9 * (first + k));
+        const float t = sum + om;
+        cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
+        sum = t;
+      }
+    } else if (sp + 2 <= kCapStack) {
+      stack[sp++] = nd.left;
+      stack[sp++] = nd.right;
+    } else {
+      ub += 1.0f;  // stack exhausted: give up the bound, the voxel goes to the FP64 recheck
+    }
+  }
+  w = (sum + cmp) * inv4pi;
+  bound = ub * 1.0001f + 1e-5f;  // + FP32 rounding of the bound and the sum
 }
 
 __device__ __noinline__ double cap_winding_f64(F3 q, const float *caps, int ncap) {
@@ -728,12 +785,25 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
 // mesh + reversed caps, from the column list) + (solid angle sum of the caps)/4pi.  Voxels whose
 // column overflowed the stored crossings, or whose FP32 |w| lies within 0.05 of 0.5, are stored
 // unsigned and listed for the resolve kernel.
+// output address of voxel (x, y, z): dense index, or slot * 512 + offset inside its 8^3 brick
+__device__ __forceinline__ size_t out_index(const SdfArgs &a, int x, int y, int z) {
+  const GridArgs &g = a.col.g;
+  if (!a.band_index) return ((size_t)z * g.ny + y) * g.nx + x;
+  const size_t b = ((size_t)(z >> 3) * (size_t)(g.ny >> 3) + (size_t)(y >> 3)) * (size_t)(g.nx >> 3) + (x >> 3);
+  return (size_t)a.band_index[b] * 512 + (size_t)((((z & 7) << 3) | (y & 7)) << 3 | (x & 7));
+}
+
 template <bool CAPS>
 __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
   const GridArgs &g = a.col.g;
   const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
   const size_t col = (size_t)y * g.nx + x;
   const float d = sqrtf(best);
+  const size_t oi = out_index(a, x, y, z);
+  if (a.unsigned_only) {
+    a.out[oi] = d;
+    return;
+  }
   const int cnt = a.col.count[col];
   bool ambiguous = cnt > kColCap;
   bool inside = false;
@@ -747,24 +817,19 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
     if (!CAPS) {
       inside = wi != 0;
     } else {
-      float sum = 0.0f, cmp = 0.0f;  // Neumaier
-      for (int c = 0; c < a.col.ncap; ++c) {
-        const float om = solid_angle_f32(q, a.col.caps + 9 * c);
-        const float t = sum + om;
-        cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
-        sum = t;
-      }
-      const float w32 = (float)wi + (sum + cmp) * (float)(1.0 / (4.0 * M_PI));
-      if (fabsf(fabsf(w32) - 0.5f) >= 0.05f) inside = fabsf(w32) > 0.5f;
+      float wc, bound;
+      cap_winding_bounded(a.col, q, wc, bound);
+      const float w32 = (float)wi + wc;
+      if (fabsf(fabsf(w32) - 0.5f) >= 0.05f + bound) inside = fabsf(w32) > 0.5f;
       else ambiguous = true;
     }
   }
   if (ambiguous) {
-    a.out[vox] = d;
+    a.out[oi] = d;
     const uint32_t pos = atomicAdd(&a.col.sc->ambiguous, 1u);
     if (pos < a.amb_cap) a.amb_list[pos] = (uint32_t)vox;  // the host re-bakes on overflow
   } else {
-    a.out[vox] = inside ? -d : d;
+    a.out[oi] = inside ? -d : d;
   }
 }
 
@@ -802,8 +867,9 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t block, 
         inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q);
       }
     }
-    const float d = a.out[vox];
-    a.out[vox] = inside ? -d : d;
+    const size_t oi = out_index(a, x, y, z);
+    const float d = a.out[oi];
+    a.out[oi] = inside ? -d : d;
   }
 }
 
@@ -920,6 +986,16 @@ __global__ __launch_bounds__(256) void k_sdf_batch(BatchArgs t) {
     const SdfArgs &a = t.jobs[lo];
     const uint32_t local = tile - t.tile_prefix[lo];
     const GridArgs &g = a.col.g;
+    if (a.band_bricks) {  // narrow band: 2 block-tiles per 8^3 brick, 4x4x4 sub-brick per wave
+      const uint32_t b = a.band_bricks[local >> 1];
+      const uint32_t nbx = (uint32_t)g.nx >> 3, nby = (uint32_t)g.ny >> 3;
+      const int sub = (int)((local & 1u) << 2) | wv;
+      const int bx = (int)(b % nbx) * 8 + (sub & 1) * 4;
+      const int by = (int)((b / nbx) % nby) * 8 + ((sub >> 1) & 1) * 4;
+      const int bz = (int)(b / (nbx * nby)) * 8 + (sub >> 2) * 4;
+      wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, 4, 4, sstack[wv]);
+      continue;
+    }
     const uint32_t tx = (g.nx + 2 * sx - 1) / (2 * sx), ty = (g.ny + 2 * sy - 1) / (2 * sy);
     const int bx = (int)(local % tx) * 2 * sx + (wv & 1) * sx;
     const int by = (int)((local / tx) % ty) * 2 * sy + (wv >> 1) * sy;
@@ -1066,6 +1142,31 @@ static void launch_batch_t(const BatchArgs &t, uint32_t blocks, hipStream_t s) {
 
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s) {
   IB_DISPATCH(launch_batch_t, count, slivers, caps, t, blocks, s);
+}
+
+__global__ __launch_bounds__(256) void k_band_flags(const float *__restrict__ coarse, uint32_t n, float thresh,
+                                                    uint32_t *flags) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) flags[i] = coarse[i] <= thresh ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_band_scatter(const uint32_t *__restrict__ flags,
+                                                      const uint32_t *__restrict__ slots, uint32_t n, int32_t *index,
+                                                      uint32_t *bricks, uint32_t *count) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  index[i] = flags[i] ? (int32_t)slots[i] : -1;
+  if (flags[i]) bricks[slots[i]] = i;
+  if (i == n - 1) *count = slots[i] + flags[i];
+}
+
+void launch_band_flags(const float *coarse, uint32_t nbrick, float thresh, uint32_t *flags, hipStream_t s) {
+  if (nbrick) k_band_flags<<<nb(nbrick), 256, 0, s>>>(coarse, nbrick, thresh, flags);
+}
+
+void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t nbrick, int32_t *index,
+                         uint32_t *bricks, uint32_t *count, hipStream_t s) {
+  if (nbrick) k_band_scatter<<<nb(nbrick), 256, 0, s>>>(flags, slots, nbrick, index, bricks, count);
 }
 
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s) {

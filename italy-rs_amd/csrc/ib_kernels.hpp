@@ -42,8 +42,10 @@ struct ColumnArgs {
   const Node *nodes;
   int nnodes;         // internal nodes (bounds every traversal loop)
   const float4 *tris;
-  const float *caps;  // 9 floats per cap triangle
+  const float *caps;  // 9 floats per cap triangle, in cap-hierarchy leaf order
   int ncap;
+  const CapNode *cap_nodes;
+  int ncapnode;
   GridArgs g;
   float2 *cross;      // kColCap per column: (z, +-1)
   int32_t *count;     // per column
@@ -53,6 +55,9 @@ struct ColumnArgs {
 struct SdfArgs {
   ColumnArgs col;
   float *out;
+  const int32_t *band_index;    // narrow band: slot of every 8^3 brick of the grid (-1: none); null: dense out
+  const uint32_t *band_bricks;  // narrow band: brick linear index of every slot (2 block-tiles per brick)
+  int unsigned_only;            // 1: store |d| only (brick selection pass; no columns needed)
   uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
   uint32_t amb_cap;    // capacity of amb_list
   int *gstack;
@@ -90,6 +95,10 @@ void tile_dims(const GridArgs &g, int shape, uint32_t d[3]);
 void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, hipStream_t s);
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s);
+// narrow band: bricks whose centre distance (from the coarse pass) is <= thresh
+void launch_band_flags(const float *coarse, uint32_t nbrick, float thresh, uint32_t *flags, hipStream_t s);
+void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t nbrick, int32_t *index,
+                         uint32_t *bricks, uint32_t *count, hipStream_t s);
 // FP64 winding recheck, parity tie-break and column-overflow recount for the listed voxels
 void launch_resolve(const SdfArgs &a, hipStream_t s);
 

@@ -18,7 +18,7 @@ EXPORTS = [
     "ib_create", "ib_destroy", "ib_last_error", "ib_version", "ib_sdf_framing", "ib_vox_framing", "ib_bake_sdf",
     "ib_voxelize", "ib_splat_materials", "ib_bake_dense", "ib_bake_dense_device", "ib_device_alloc",
     "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_synchronize", "ib_last_stats", "ib_set_option",
-    "ib_bake_batch_device", "ib_autotune_batch",
+    "ib_bake_batch_device", "ib_autotune_batch", "ib_bake_band_device",
 ]
 
 
@@ -64,7 +64,9 @@ class Stats(ctypes.Structure):
 class Job(ctypes.Structure):
     """ib_job: one mesh of a batched bake (device pointers)."""
     _fields_ = [("d_xyz", ctypes.c_void_p), ("ntri", ctypes.c_uint64), ("sdf", Grid), ("d_distances", ctypes.c_void_p),
-                ("occ", Grid), ("d_occ_words", ctypes.c_void_p)]
+                ("occ", Grid), ("d_occ_words", ctypes.c_void_p), ("band_voxels", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("d_brick_index", ctypes.c_void_p), ("pool_bricks", ctypes.c_uint64),
+                ("nbricks", ctypes.c_uint64)]
 
 
 _lib = None
@@ -103,6 +105,7 @@ def load():
     lib.ib_last_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     lib.ib_set_option.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64]
     lib.ib_bake_batch_device.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, _vp]
+    lib.ib_bake_band_device.argtypes = [_vp, _vp, _u64, G, ctypes.c_int32, _vp, _vp, _u64, ctypes.POINTER(_u64), _vp]
     lib.ib_autotune_batch.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_double)]

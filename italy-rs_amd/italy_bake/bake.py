@@ -84,9 +84,40 @@ class Context:
                                             ctypes.c_void_p(stream) if stream else None))
 
     def bake_batch_device(self, jobs, stream=None):
-        """One batched bake (ib_bake_batch_device) of a list of Job structures; synchronous."""
+        """One batched bake (ib_bake_batch_device) of a list of Job structures; synchronous.
+        Narrow-band jobs get their nbricks written back into the list."""
         arr = (Job * len(jobs))(*jobs)
-        check(self.lib.ib_bake_batch_device(self._h, arr, len(jobs), ctypes.c_void_p(stream) if stream else None))
+        st = self.lib.ib_bake_batch_device(self._h, arr, len(jobs), ctypes.c_void_p(stream) if stream else None)
+        for i in range(len(jobs)):
+            jobs[i].nbricks = arr[i].nbricks
+        check(st)
+
+    def bake_band(self, xyz, grid, band_voxels):
+        """Narrow-band bake of one host soup: returns (brick index (nz/8, ny/8, nx/8) int32, pool
+        (nbricks, 8, 8, 8) float32)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        ntri = xyz.size // 9
+        nbrick = (grid.nx // 8) * (grid.ny // 8) * (grid.nz // 8)
+        d_xyz = self.alloc(max(xyz.nbytes, 4))
+        d_idx = self.alloc(4 * max(nbrick, 1))
+        try:
+            if ntri:
+                self.upload(d_xyz, xyz)
+            n = ctypes.c_uint64()
+            cap = max(nbrick, 1)  # worst case: every brick
+            d_pool = self.alloc(4 * 512 * cap)
+            try:
+                check(self.lib.ib_bake_band_device(self._h, ctypes.c_void_p(d_xyz), ntri, ctypes.byref(grid),
+                                                   int(band_voxels), ctypes.c_void_p(d_idx),
+                                                   ctypes.c_void_p(d_pool), cap, ctypes.byref(n), None))
+                pool = self.download(d_pool, np.empty((cap, 8, 8, 8), np.float32))[: n.value]
+            finally:
+                self.free(d_pool)
+            index = self.download(d_idx, np.empty((grid.nz // 8, grid.ny // 8, grid.nx // 8), np.int32))
+        finally:
+            self.free(d_xyz)
+            self.free(d_idx)
+        return index, pool
 
     def autotune_batch(self, jobs, sample_stride=16):
         """ib_autotune_batch: returns (brick shape, blocks per CU, query ms of the sample)."""

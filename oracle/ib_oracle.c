@@ -1081,3 +1081,47 @@ void orc_udf2_f32_brute(const float *xyz, size_t ntri, const float *pts, size_t 
     out[i] = best;
   }
 }
+
+/* FP32 unsigned distance at arbitrary points through the LBVH (same per-triangle values as the
+ * brute force; pruning never drops a candidate with box distance <= best) */
+typedef struct {
+  const qctx *c;
+  const float *pts;
+  size_t npts;
+  float *out;
+  size_t next;
+  pthread_mutex_t *mu;
+} udf_job;
+
+static void *udf_fn(void *arg) {
+  udf_job *j = (udf_job *)arg;
+  uint64_t nodes = 0, tests = 0;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 256;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 256 < j->npts ? i0 + 256 : j->npts;
+    for (size_t i = i0; i < i1; ++i) j->out[i] = sqrtf(closest2(j->c, j->pts + 3 * i, &nodes, &tests));
+  }
+  return NULL;
+}
+
+int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out) {
+  orc_bvh b;
+  if (orc_bvh_build(xyz, ntri, &b) != 0) return -1;
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  qctx c = {sorted, &b};
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  udf_job j = {&c, pts, npts, out, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, udf_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+  free(sorted);
+  orc_bvh_free(&b);
+  return 0;
+}

@@ -118,6 +118,9 @@ void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t
 /* FP32 brute force: out[i] = min over triangles of orc_tri_dist2_f32(pts[i], tri) */
 void orc_udf2_f32_brute(const float *xyz, size_t ntri, const float *pts, size_t npts, float *out);
 
+/* FP32 unsigned distance at points through the LBVH (equals the FP32 brute force) */
+int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out);
+
 /* number of directed edges left unmatched after exact-position welding (0 => closed) */
 int64_t orc_boundary_edge_count(const float *xyz, size_t ntri);
 

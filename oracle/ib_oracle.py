@@ -82,6 +82,7 @@ def _load():
                                            ctypes.c_int, _f32p, ctypes.POINTER(BakeStats)]
     lib.orc_sdf_f64_points.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p, _f64p]
     lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
+    lib.orc_udf_points_f32.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
     lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
     lib.orc_boundary_edge_count.restype = ctypes.c_int64
     return lib
@@ -255,3 +256,22 @@ def cell_centres(origin, h, nx, ny, nz):
     xs = o[0] + (np.arange(nx, dtype=np.float32) + np.float32(0.5)) * h
     zz, yy, xx = np.meshgrid(zs, ys, xs, indexing="ij")
     return np.stack([xx, yy, zz], -1).astype(np.float32)
+
+
+def udf_points(xyz, pts, threads=0):
+    """FP32 unsigned distance at points (LBVH; equal to udf2_f32_brute's square root)."""
+    xyz = _f32(xyz)
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float32)
+    lib().orc_udf_points_f32(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f32p), len(pts), threads, _p(out, _f32p))
+    return out
+
+
+def band_bricks(xyz, origin, h, nx, ny, nz, band):
+    """Bricks (8^3) a narrow band of +-band voxels keeps: unsigned distance at the brick centre
+    (cell centres of the grid of spacing 8h) <= h * (band + 3.5 sqrt 3), FP32 as the bake does it."""
+    h8 = np.float32(h) * np.float32(8.0)
+    c = cell_centres(origin, h8, nx // 8, ny // 8, nz // 8).reshape(-1, 3)
+    d = udf_points(xyz, c)
+    thresh = np.float32(h) * (np.float32(band) + np.float32(6.0621778))
+    return d <= thresh
