@@ -490,7 +490,7 @@ __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes
 // every cap triangle whose box contains (X, Y) in xy
 template <typename F>
 __device__ __forceinline__ void for_column_caps(const ColumnArgs &a, float X, float Y, F &&tri) {
-  if (a.ncap == 0) return;
+  if (a.ncap == 0 || a.ncapnode == 0 || !a.cap_nodes) return;
   int stack[kCapStack];
   int sp = 0;
   stack[sp++] = 0;
@@ -566,6 +566,10 @@ __device__ __forceinline__ float solid_angle_f32(F3 q, const float *t) {
 // Neumaier), every farther node only contributes its bound |Omega| <= area / dmin^2 (a surface of
 // that area seen from at least dmin) to `bound`, in winding units.
 __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, float &w, float &bound) {
+  w = 0.0f;
+  bound = 0.0f;
+  // a batch runs the capped kernel for every job once one job has caps: closed meshes have none
+  if (a.ncap == 0 || a.ncapnode == 0 || !a.cap_nodes) return;
   const float inv4pi = (float)(1.0 / (4.0 * M_PI));
   float sum = 0.0f, cmp = 0.0f, ub = 0.0f;
   int stack[kCapStack];
