@@ -11,7 +11,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import BakeError, Grid, Job, Stats, check
+from ._lib import BakeError, Grid, Job, Stats, check  # noqa: F401
 
 OPT_COUNT_WORK = 1
 OPT_QUERY_VARIANT = 2

@@ -139,3 +139,11 @@ def workload_batched(n=512, seed=BASE_SEED + 4096, threads=0):
     res = rng.choice([32, 64, 128, 256], n)
     seeds = [seed + 1 + i for i in range(n)]
     return decimated_batch(targets, seeds, threads), [int(r) for r in res]
+
+
+def workload_stress(seed=BASE_SEED):
+    """64 seeded closed components (32 tori of 64x32 segments, 32 boxes of 32x32 quads per face)
+    with random rotations and positions in [-0.7, 0.7]^3, overlaps allowed, 2% of triangles removed
+    at random; baked as a 384^3 narrow band of +-4 voxels in 8^3 bricks over [-1, 1]^3."""
+    m = stress(seed, 64, 0.02)
+    return m, dict(lo=-1.0, hi=1.0, res=384, band=4)
