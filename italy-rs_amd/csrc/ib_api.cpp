@@ -58,6 +58,7 @@ struct ib_context {
   bool ev_valid[EV_COUNT] = {};
   ib_stats stats{};
   bool count_work = false;
+  bool stats_in_sc = false;    // last bake was a plain bake: its device counters are in the "sc" scratch
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
@@ -473,6 +474,7 @@ ib_status fill_empty(ib_context *c, const ib_grid *sdf, float *d_dist, const ib_
 
 void reset_stats(ib_context *c) {
   c->stats = ib_stats{};
+  c->stats_in_sc = false;
   for (bool &v : c->ev_valid) v = false;
 }
 
@@ -491,6 +493,7 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
   MeshDev m;
   Scalars *sc;
   IB_TRY(c->scratch("sc", 1, &sc));
+  c->stats_in_sc = true;
   c->record(EV_START, s);
   IB_TRY(enqueue_build(c, d_xyz, (int)ntri, sc, m, nullptr, s));
   if (occ && d_occ) enqueue_occupancy(d_xyz, (int)ntri, *occ, d_occ, s);
@@ -808,7 +811,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   // device-side counters of the last bake
   Scalars *sc = nullptr;
   auto it = c->bufs.find("sc");
-  if (it != c->bufs.end()) {
+  if (c->stats_in_sc && it != c->bufs.end()) {  // batched bakes accumulate their counters on the host
     sc = static_cast<Scalars *>(it->second.p);
     Scalars h;
     IB_HIP(hipMemcpy(&h, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
