@@ -163,6 +163,56 @@ ib_status ib_bake_band_device(ib_context *ctx, const float *d_xyz, uint64_t ntri
 ib_status ib_autotune_batch(ib_context *ctx, const ib_job *jobs, uint32_t njobs, uint32_t sample_stride,
                             int32_t *out_shape, int32_t *out_blocks, double *out_ms);
 
+/* ---- served bakes ----------------------------------------------------------------------------
+ * A server admits requests first-come-first-served while their device footprint fits its HBM budget,
+ * bakes whatever is admitted in batches on `workers` contexts at once (each on its own stream; a
+ * request that arrives while batches run joins the next free worker's batch), and keeps every
+ * result resident on the device until ib_server_release.  The soup passed to submit is read when
+ * the request starts and must stay valid until the ticket is done.  No counterpart in the
+ * reference, whose bake is a synchronous call on the UI thread (src/app/main.cpp:250-259). */
+typedef struct ib_server ib_server;
+
+typedef struct {
+  const float *xyz;      /* host soup, 9 floats per triangle */
+  uint64_t ntri;
+  ib_grid sdf;           /* sdf.nx == 0: no SDF */
+  int32_t band_voxels;   /* 0: dense SDF; > 0: narrow band in 8^3 bricks (grid dims divisible by 8) */
+  int32_t reserved;
+  ib_grid occ;           /* occ.nx == 0: no occupancy */
+} ib_request;
+
+typedef struct {
+  int32_t state;         /* 0 queued, 1 running, 2 done, 3 failed */
+  int32_t status;        /* ib_status of a failed request */
+  double t_submit, t_admit, t_start, t_done;  /* seconds on the server's steady clock */
+  uint64_t nbricks;      /* narrow band: bricks in the result */
+  uint64_t resident_bytes;
+  int32_t worker;
+  int32_t reserved;
+} ib_ticket_info;
+
+typedef struct {
+  uint64_t budget, in_use, cached, pending, ready, running_batches, batches, completed;
+} ib_server_counters;
+
+ib_status ib_server_create(int device, uint64_t hbm_budget_bytes, int32_t workers, int32_t max_batch,
+                           ib_server **out);
+void ib_server_destroy(ib_server *srv);
+const char *ib_server_last_error(void);
+ib_status ib_server_submit(ib_server *srv, const ib_request *req, uint64_t *ticket);
+ib_status ib_server_info(ib_server *srv, uint64_t ticket, ib_ticket_info *info);
+ib_status ib_server_wait(ib_server *srv, uint64_t ticket, double timeout_s);
+ib_status ib_server_drain(ib_server *srv, double timeout_s);
+/* host copies of a finished result: distances (dense grid, or nbricks * 512 pool floats), the brick
+ * index of a narrow band, the occupancy words; any pointer may be NULL */
+ib_status ib_server_fetch(ib_server *srv, uint64_t ticket, float *distances, uint64_t capacity, int32_t *brick_index,
+                          uint32_t *occ_words);
+/* device pointers of a finished result (valid until release) */
+ib_status ib_server_device_result(ib_server *srv, uint64_t ticket, const float **d_distances,
+                                  const int32_t **d_brick_index, const uint32_t **d_occ_words);
+ib_status ib_server_release(ib_server *srv, uint64_t ticket);
+ib_status ib_server_stats(ib_server *srv, ib_server_counters *out);
+
 /* ---- device memory helpers (for callers without a GPU framework) --------------------------- */
 ib_status ib_device_alloc(ib_context *ctx, uint64_t bytes, void **d_ptr);
 ib_status ib_device_free(ib_context *ctx, void *d_ptr);

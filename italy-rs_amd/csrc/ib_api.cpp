@@ -67,20 +67,44 @@ struct ib_context {
   uint32_t batch_sc_cap = 0;
   std::mutex mu;
 
+  // freed blocks are kept per exact size and handed out again, so that repeated batches do not
+  // call hipMalloc / hipFree (which can serialise work queued on other streams)
+  std::unordered_map<size_t, std::vector<void *>> free_blocks;
+  size_t cached = 0;
   ib_status alloc_raw(size_t bytes, void **p) {
     if (allocated + bytes > budget)
       return fail(IB_ERR_OUT_OF_BUDGET, "allocation of " + std::to_string(bytes) + " bytes exceeds the HBM budget (" +
                                             std::to_string(allocated) + " of " + std::to_string(budget) + " in use)");
-    const hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    auto it = free_blocks.find(bytes);
+    if (it != free_blocks.end() && !it->second.empty()) {
+      *p = it->second.back();
+      it->second.pop_back();
+      cached -= bytes;
+      allocated += bytes;
+      return IB_OK;
+    }
+    hipError_t e = hipMalloc(p, bytes ? bytes : 1);
+    if (e != hipSuccess && cached) {  // out of device memory: return the cache and retry once
+      release_cache();
+      e = hipMalloc(p, bytes ? bytes : 1);
+    }
     if (e != hipSuccess) return fail(IB_ERR_DEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
     allocated += bytes;
     return IB_OK;
   }
   void free_raw(void *p, size_t bytes) {
     if (p) {
-      (void)hipFree(p);
+      free_blocks[bytes].push_back(p);
+      cached += bytes;
       allocated -= bytes;
+      if (cached > (budget >> 2)) release_cache();
     }
+  }
+  void release_cache() {
+    for (auto &kv : free_blocks)
+      for (void *q : kv.second) (void)hipFree(q);
+    free_blocks.clear();
+    cached = 0;
   }
   // named scratch that only grows; contents are not preserved across growth
   template <typename T>
@@ -550,6 +574,7 @@ void ib_destroy(ib_context *c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto &kv : c->bufs) (void)hipFree(kv.second.p);
   for (auto &kv : c->user) (void)hipFree(kv.first);
+  c->release_cache();
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   (void)hipHostFree(c->host_sc);
   if (c->host_batch_sc) (void)hipHostFree(c->host_batch_sc);

@@ -18,7 +18,9 @@ EXPORTS = [
     "ib_create", "ib_destroy", "ib_last_error", "ib_version", "ib_sdf_framing", "ib_vox_framing", "ib_bake_sdf",
     "ib_voxelize", "ib_splat_materials", "ib_bake_dense", "ib_bake_dense_device", "ib_device_alloc",
     "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_synchronize", "ib_last_stats", "ib_set_option",
-    "ib_bake_batch_device", "ib_autotune_batch", "ib_bake_band_device",
+    "ib_bake_batch_device", "ib_autotune_batch", "ib_bake_band_device", "ib_server_create", "ib_server_destroy",
+    "ib_server_last_error", "ib_server_submit", "ib_server_info", "ib_server_wait", "ib_server_drain",
+    "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
 ]
 
 
