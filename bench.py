@@ -39,7 +39,57 @@ def parse():
     ap.add_argument("--variant", type=int, default=None, help="query kernel: 0 per-voxel, 1 per-brick wave")
     ap.add_argument("--batch-meshes", type=int, default=512, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--no-band", action="store_true", help="skip the narrow-band precision-stress bake")
+    ap.add_argument("--serve-seconds", type=float, default=8.0,
+                    help="arrival window of the served workload on the batched mesh distribution (0: skip)")
+    ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
+                    help="arrival window of the served precision-stress + large-job mix (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=200.0)
     return ap.parse_args()
+
+
+def served_section(args, batched):
+    """Open-loop Poisson arrivals into the bake server (3 worker streams, HBM budget 200 GB).
+
+    batched_distribution: requests drawn from the batched workload's distribution (dense SDF +
+      occupancy, 500..200k triangles, 32..256^3): the plain per-mesh bake of the same meshes is the
+      comparison point.
+    stress_mix: 95% precision-stress unions (384^3 narrow band) + 5% large jobs (2M-triangle displaced
+      icosphere, 512^3 narrow band), the other reading of the served configuration; far beyond one
+      GPU's rate at 200 req/s, so its latency is queueing."""
+    from italy_bake import bake, meshgen
+    from italy_bake.serve import Server, open_loop
+
+    out = {}
+    budget = 200 << 30
+    if args.serve_seconds > 0:
+        meshes, res = meshgen.workload_batched(128, seed=meshgen.BASE_SEED + 9000)
+        reqs = [(m.xyz, bake.dense_grid(-0.8, 0.8, r), 0, bake.dense_grid(-0.8, 0.8, r)) for m, r in zip(meshes, res)]
+        with Server(hbm_budget_bytes=budget, workers=3, max_batch=256) as srv:
+            open_loop(srv, reqs[:8], rate_hz=50, duration_s=0.5, seed=7)  # warm the worker contexts
+            r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_seconds, seed=1)
+        r.pop("_latencies")
+        r["workload"] = ("requests drawn from 128 meshes of the batched distribution (dense SDF + occupancy at "
+                         "32/64/128/256^3)")
+        if batched:
+            r["plain_per_mesh_meshes_per_s"] = batched["plain_meshes_per_s"]
+        out["batched_distribution"] = r
+    if args.serve_stress_seconds > 0:
+        stress = [meshgen.stress(meshgen.BASE_SEED + 100 + i, 64, 0.02) for i in range(6)]
+        big = meshgen.decimated(2_000_000, meshgen.BASE_SEED + 200)
+        g384 = bake.dense_grid(-1.0, 1.0, 384)
+        g512 = bake.dense_grid(-0.8, 0.8, 512)
+        # 95% / 5%: 19 stress entries per large entry in the uniform draw
+        reqs = [(m.xyz, g384, 4, None) for m in stress] * 19 + [(big.xyz, g512, 4, None)] * 6
+        with Server(hbm_budget_bytes=budget, workers=3, max_batch=64) as srv:
+            r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_stress_seconds, seed=2)
+        lat = r.pop("_latencies")
+        big_lat = [l for k, l in lat if reqs[k][1] is g512]
+        r["large_jobs"] = len(big_lat)
+        r["large_job_latency_s_max"] = max(big_lat) if big_lat else None
+        r["workload"] = ("95% precision-stress unions (64 tori/boxes, 2% removed, 384^3 narrow band +-4) + 5% "
+                         "2M-triangle displaced icospheres (512^3 narrow band +-4)")
+        out["stress_mix"] = r
+    return out or None
 
 
 def band_section(ctx, steps=3):
@@ -186,6 +236,7 @@ def main():
 
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None
+    served = served_section(args, batched)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -208,7 +259,9 @@ def main():
                                "0x17A1E5), 256^3 FP32 SDF + 512^3 bit-packed occupancy over [-0.8,0.8]^3",
                    "triangles": mesh.ntri, "sdf_res": sdf.nx, "occ_res": occ.nx, "parallelism": "single GPU"},
         "device_ms_per_bake": float(np.mean(total_ms)),
-        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band,
+        "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band, "served": served,
+        "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
+        if served and served.get("batched_distribution") else None,
     }
     print(json.dumps(out))
 
