@@ -23,10 +23,12 @@ sys.path.insert(0, os.path.join(ROOT, "italy-rs_amd"))
 METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake latency"
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published)
 PEAK_HBM_GBS = 8000.0
-# FLOPs per test, counted from the kernel source (ib_device.hpp): point/box distance 3x(sub, mul,
-# add); closest point on triangle, face-region path (the longest): 77
-FLOP_BOX = 9
-FLOP_TRI = 77
+# FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max
+# and compares not counted): point/box distance 6 sub + 3 mul + 2 add = 11; branch-free point/
+# triangle distance 9 sub + 3 segments x (dot 5 + scale 1 + fma point 6 + square 5) + 3 edge-normal
+# dots x 5 + plane dot 5 + square and scale 2 = 82
+FLOP_BOX = 11
+FLOP_TRI = 82
 
 
 def parse():
@@ -44,6 +46,9 @@ def parse():
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
                     help="arrival window of the served precision-stress + large-job mix (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain device-to-device copy of this many MiB (a known-size kernel that "
+                         "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
     return ap.parse_args()
 
 
@@ -233,6 +238,15 @@ def main():
                         box_tests=counters["box_tests"], tri_tests=counters["tri_tests"])
     alg_bytes = 4 * nvox + mesh.xyz.nbytes
     roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
+
+    if args.copy_calibration_mib > 0:
+        import torch
+
+        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None

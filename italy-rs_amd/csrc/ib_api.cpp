@@ -62,6 +62,7 @@ struct ib_context {
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
+  bool tile_order = true;      // dense per-brick queries launch their costliest tiles first
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
   uint32_t batch_sc_cap = 0;
@@ -320,6 +321,7 @@ struct MeshDev {
   int n = 0;
   Node *nodes = nullptr;
   float4 *tris = nullptr;
+  TriRec *recs = nullptr;
   float *caps = nullptr;
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
@@ -375,6 +377,7 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(c->scratch("root_box", 8, &root_box));
   IB_TRY(take(c, arena, "nodes", std::max(n - 1, 1), &m.nodes));
   IB_TRY(take(c, arena, "tris", 3ull * n, &m.tris));
+  IB_TRY(take(c, arena, "trirecs", (size_t)n, &m.recs));
   launch_prep(d_xyz, n, sc, s);
   launch_morton(d_xyz, n, sc, codes_in, idx, s);
   size_t temp_bytes = 0;
@@ -383,7 +386,7 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(c->scratch("sort_temp", temp_bytes, &temp));
   IB_HIP(sort_pairs(temp, temp_bytes, codes_in, codes, idx, order, n, s));
   launch_karras(codes, n, m.nodes, leaf_parent, s);
-  launch_gather(d_xyz, order, n, m.tris, sc, s);
+  launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits, 0u, n, s);
   launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, s);
   m.n = n;
@@ -447,6 +450,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.nodes = m.nodes;
   ca.nnodes = std::max(m.n - 1, 1);
   ca.tris = m.tris;
+  ca.recs = m.recs;
   ca.caps = m.caps;
   ca.ncap = m.ncap;
   ca.cap_nodes = m.cap_nodes;
@@ -480,7 +484,27 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_TRY(c->scratch("gstack", threads * sa.gstack_depth, &sa.gstack));
     IB_TRY(c->scratch("gstackd", threads * sa.gstack_depth, &sa.gstackd));
   }
-  launch_sdf(sa, deep, c->count_work, c->query_variant, m.slivers, c->brick_shape, s);
+  const uint32_t *order = nullptr;
+  if (c->query_variant == 1 && !deep && c->tile_order) {
+    // costliest block-tiles first: sort the tiles by the column-crossing cost estimate
+    uint32_t d[3];
+    tile_dims(sa.col.g, c->brick_shape, d);
+    const uint32_t nt = d[0] * d[1] * d[2];
+    uint32_t *keys_in, *keys;
+    int32_t *ids_in, *ids;
+    IB_TRY(c->scratch("tile_keys_in", nt, &keys_in));
+    IB_TRY(c->scratch("tile_keys", nt, &keys));
+    IB_TRY(c->scratch("tile_ids_in", nt, &ids_in));
+    IB_TRY(c->scratch("tile_ids", nt, &ids));
+    launch_tile_keys(sa.col, c->brick_shape, keys_in, ids_in, s);
+    size_t temp_bytes = 0;
+    IB_HIP(sort_pairs_u32(nullptr, temp_bytes, keys_in, keys, ids_in, ids, nt, 32, s));
+    char *temp;
+    IB_TRY(c->scratch("tile_sort_temp", temp_bytes, &temp));
+    IB_HIP(sort_pairs_u32(temp, temp_bytes, keys_in, keys, ids_in, ids, nt, 32, s));
+    order = reinterpret_cast<const uint32_t *>(ids);
+  }
+  launch_sdf(sa, deep, c->count_work, c->query_variant, m.slivers, c->brick_shape, order, s);
   launch_resolve(sa, s);
   c->record(EV_SDF, s);
   return IB_OK;
@@ -951,6 +975,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       sa.col.nodes = mesh[j].nodes;
       sa.col.nnodes = std::max(mesh[j].n - 1, 1);
       sa.col.tris = mesh[j].tris;
+      sa.col.recs = mesh[j].recs;
       sa.col.g = to_args(cg);
       sa.col.sc = scs + j;
       sa.out = cdist[j];
@@ -1156,6 +1181,9 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
     case IB_OPT_BLOCKS_PER_CU:
       if (value < 1 || value > 64) return fail(IB_ERR_INVALID_ARGUMENT, "blocks per CU must be 1..64");
       c->blocks_per_cu = (int)value;
+      return IB_OK;
+    case IB_OPT_TILE_ORDER:
+      c->tile_order = value != 0;
       return IB_OK;
     default:
       return fail(IB_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));

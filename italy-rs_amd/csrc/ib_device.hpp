@@ -113,18 +113,15 @@ __device__ __forceinline__ bool tri_box(F3 c, float bh, F3 t0, F3 t1, F3 t2) {
 // ---------------------------------------------------------------------------------------------
 __device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
+// Squared distance from q to the box [lo, hi] (lo <= hi).  Per axis the gap is lo - q below the
+// box, q - hi above it and 0 inside; max(lo - q, q - hi, 0) gives exactly those values (each
+// difference is one rounded subtraction), so this branch-free form is bit-identical to the
+// three-way branch and maps to one v_max3_f32 per axis.
 __device__ __forceinline__ float box_d2(F3 q, const float *lo, const float *hi) {
-  float s = 0.0f;
-  float d = 0.0f;
-  if (q.x < lo[0]) d = lo[0] - q.x; else if (q.x > hi[0]) d = q.x - hi[0];
-  s += d * d;
-  d = 0.0f;
-  if (q.y < lo[1]) d = lo[1] - q.y; else if (q.y > hi[1]) d = q.y - hi[1];
-  s += d * d;
-  d = 0.0f;
-  if (q.z < lo[2]) d = lo[2] - q.z; else if (q.z > hi[2]) d = q.z - hi[2];
-  s += d * d;
-  return s;
+  const float dx = fmaxf(fmaxf(lo[0] - q.x, q.x - hi[0]), 0.0f);
+  const float dy = fmaxf(fmaxf(lo[1] - q.y, q.y - hi[1]), 0.0f);
+  const float dz = fmaxf(fmaxf(lo[2] - q.z, q.z - hi[2]), 0.0f);
+  return (dx * dx + dy * dy) + dz * dz;
 }
 
 __device__ __forceinline__ double dot3d(double ax, double ay, double az, double bx, double by, double bz) {
@@ -172,57 +169,81 @@ __device__ __noinline__ double tri_d2_f64(F3 q, F3 p0, F3 p1, F3 p2) {
   return best;
 }
 
-// FP32 squared distance in coordinates relative to q: Voronoi regions after Ericson (2005) 5.1.5,
-// the face region by plane distance (n.a)^2 / |n|^2.
-__device__ __forceinline__ float tri_d2_f32(F3 q, F3 p0, F3 p1, F3 p2) {
-  const F3 a = {p0.x - q.x, p0.y - q.y, p0.z - q.z};
-  const F3 b = {p1.x - q.x, p1.y - q.y, p1.z - q.z};
-  const F3 c = {p2.x - q.x, p2.y - q.y, p2.z - q.z};
-  const F3 ab = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
-  const F3 ac = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
-  const F3 ap = {-a.x, -a.y, -a.z};
-  const float d1 = dot3(ab, ap), d2 = dot3(ac, ap);
-  if (d1 <= 0.0f && d2 <= 0.0f) return dot3(a, a);
-  const F3 bp = {-b.x, -b.y, -b.z};
-  const float d3 = dot3(ab, bp), d4 = dot3(ac, bp);
-  if (d3 >= 0.0f && d4 <= d3) return dot3(b, b);
-  const float vc = d1 * d4 - d3 * d2;
-  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    const float v = d1 / (d1 - d3);
-    const F3 r = {a.x + v * ab.x, a.y + v * ab.y, a.z + v * ab.z};
-    return dot3(r, r);
-  }
-  const F3 cp = {-c.x, -c.y, -c.z};
-  const float d5 = dot3(ab, cp), d6 = dot3(ac, cp);
-  if (d6 >= 0.0f && d5 <= d6) return dot3(c, c);
-  const float vb = d5 * d2 - d1 * d6;
-  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    const float w = d2 / (d2 - d6);
-    const F3 r = {a.x + w * ac.x, a.y + w * ac.y, a.z + w * ac.z};
-    return dot3(r, r);
-  }
-  const float va = d3 * d6 - d5 * d4;
-  const float e43 = d4 - d3, e56 = d5 - d6;
-  if (va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f) {
-    const float w = e43 / (e43 + e56);
-    const F3 bc = {p2.x - p1.x, p2.y - p1.y, p2.z - p1.z};
-    const F3 r = {b.x + w * bc.x, b.y + w * bc.y, b.z + w * bc.z};
-    return dot3(r, r);
-  }
-  const F3 n = {ab.y * ac.z - ab.z * ac.y, ab.z * ac.x - ab.x * ac.z, ab.x * ac.y - ab.y * ac.x};
-  const float s = dot3(n, a);
-  return (s * s) / dot3(n, n);
+// Distance record of one triangle (144 bytes), built once per mesh from the three vertices:
+//   e_i = p_{i+1} - p_i                       edge vectors (p_3 = p_0)
+//   n = cross(e_0, p_2 - p_0)                 plane normal (unnormalised), inv_nn = 1 / |n|^2
+//   nie_i = -1 / |e_i|^2                      segment parameter scale
+//   m_i = cross(n, e_i)                       in-plane normal of edge i, pointing into the triangle
+// with cross(a, b)_x = fma(a_y, b_z, -(a_z * b_y)) (and cyclic) and dot(a, b) = fma(a_z, b_z,
+// fma(a_y, b_y, a_x * b_x)).  The record is read at wave-uniform addresses (scalar loads).
+struct alignas(16) TriRec {
+  float p[3][3];
+  float e[3][3];
+  float n[3], inv_nn;
+  float nie[3];
+  float m[3][3];
+  uint32_t flags;
+};
+static_assert(sizeof(TriRec) == 144, "distance record is 9 x 16 bytes");
+
+__device__ __forceinline__ float fdot(F3 a, F3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
+__device__ __forceinline__ F3 fcross(F3 a, F3 b) {
+  return {fmaf(a.y, b.z, -(a.z * b.y)), fmaf(a.z, b.x, -(a.x * b.z)), fmaf(a.x, b.y, -(a.y * b.x))};
 }
 
-// triangle record: 3 x float4 = (v0, flags), (v1, original index), (v2, 0).  SLIVERS = false is
-// only instantiated for meshes without sliver triangles (the host counts them at build time), so
-// that kernel carries no FP64 code.
+__device__ __forceinline__ void make_trirec(F3 p0, F3 p1, F3 p2, uint32_t flags, TriRec &r) {
+  const F3 P[3] = {p0, p1, p2};
+  F3 E[3];
+  for (int i = 0; i < 3; ++i) {
+    const F3 a = P[i], b = P[i == 2 ? 0 : i + 1];
+    E[i] = {b.x - a.x, b.y - a.y, b.z - a.z};
+  }
+  const F3 ac = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+  const F3 n = fcross(E[0], ac);
+  for (int i = 0; i < 3; ++i) {
+    r.p[i][0] = P[i].x; r.p[i][1] = P[i].y; r.p[i][2] = P[i].z;
+    r.e[i][0] = E[i].x; r.e[i][1] = E[i].y; r.e[i][2] = E[i].z;
+    r.nie[i] = -1.0f / fdot(E[i], E[i]);
+    const F3 m = fcross(n, E[i]);
+    r.m[i][0] = m.x; r.m[i][1] = m.y; r.m[i][2] = m.z;
+  }
+  r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
+  r.inv_nn = 1.0f / fdot(n, n);
+  r.flags = flags;
+}
+
+// FP32 squared distance from q to a non-sliver triangle, branch free.  With a_i = p_i - q:
+//   segment i: t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = a_i + t e_i (fma), d_i = |r|^2
+//   face:      inside = max_i dot(m_i, a_i) <= 0, d_f = (dot(n, a_0)^2) * inv_nn
+//   result     min(d_0, d_1, d_2, inside ? d_f : inf)
+// |r|^2 is summed as (x^2 + y^2) + z^2 without fusion, the order of the box distance, so a vertex
+// that is a box corner gives the same value as its box.
+__device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
+  F3 a[3];
+  for (int i = 0; i < 3; ++i) a[i] = {t.p[i][0] - q.x, t.p[i][1] - q.y, t.p[i][2] - q.z};
+  float d = INFINITY;
+  for (int i = 0; i < 3; ++i) {
+    const F3 e = {t.e[i][0], t.e[i][1], t.e[i][2]};
+    const float u = fminf(fmaxf(fdot(a[i], e) * t.nie[i], 0.0f), 1.0f);
+    const F3 r = {fmaf(u, e.x, a[i].x), fmaf(u, e.y, a[i].y), fmaf(u, e.z, a[i].z)};
+    d = fminf(d, (r.x * r.x + r.y * r.y) + r.z * r.z);
+  }
+  const float k0 = fdot({t.m[0][0], t.m[0][1], t.m[0][2]}, a[0]);
+  const float k1 = fdot({t.m[1][0], t.m[1][1], t.m[1][2]}, a[1]);
+  const float k2 = fdot({t.m[2][0], t.m[2][1], t.m[2][2]}, a[2]);
+  const float s = fdot({t.n[0], t.n[1], t.n[2]}, a[0]);
+  const float df = (s * s) * t.inv_nn;
+  return fmaxf(fmaxf(k0, k1), k2) <= 0.0f ? fminf(d, df) : d;
+}
+
+// SLIVERS = false is only instantiated for meshes without sliver triangles (the host counts them
+// at build time), so that kernel carries no FP64 code.
 template <bool SLIVERS>
-__device__ __forceinline__ float tri_d2(F3 q, const float4 *t) {
-  const float4 a = t[0], b = t[1], c = t[2];
-  const F3 p0 = {a.x, a.y, a.z}, p1 = {b.x, b.y, b.z}, p2 = {c.x, c.y, c.z};
-  if (SLIVERS && (__float_as_uint(a.w) & kSliver)) return (float)tri_d2_f64(q, p0, p1, p2);
-  return tri_d2_f32(q, p0, p1, p2);
+__device__ __forceinline__ float tri_d2(F3 q, const TriRec &t) {
+  if (SLIVERS && (t.flags & kSliver))
+    return (float)tri_d2_f64(q, {t.p[0][0], t.p[0][1], t.p[0][2]}, {t.p[1][0], t.p[1][1], t.p[1][2]},
+                             {t.p[2][0], t.p[2][1], t.p[2][2]});
+  return tri_d2_f32(q, t);
 }
 
 // ---------------------------------------------------------------------------------------------

@@ -134,9 +134,9 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
   if (i == 0) nodes[0].parent = -1;
 }
 
-// sorted triangle records (3 x float4) with the sliver flag
+// sorted triangle records (3 x float4) with the sliver flag, and their distance records
 __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, const int32_t *__restrict__ order,
-                                                int n, float4 *tris, Scalars *sc) {
+                                                int n, float4 *tris, TriRec *recs, Scalars *sc) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const int o = order[i];
@@ -151,6 +151,9 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, c
   tris[3 * i + 0] = make_float4(t[0], t[1], t[2], __uint_as_float(flags));
   tris[3 * i + 1] = make_float4(t[3], t[4], t[5], __int_as_float(o));
   tris[3 * i + 2] = make_float4(t[6], t[7], t[8], 0.0f);
+  TriRec r;
+  make_trirec({t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, flags, r);
+  recs[i] = r;
 }
 
 // bottom-up refit: each leaf writes its box into its parent's child slot; the second thread to
@@ -701,7 +704,7 @@ struct WorkCount {
 };
 
 template <bool DEEP, bool COUNT, bool SLIVERS>
-__device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
+__device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const TriRec *__restrict__ recs,
                                             F3 q, int *gstack, float *gstackd, int cap, WorkCount &wc) {
   float best = INFINITY;
   int lstack[DEEP ? 1 : kStack];
@@ -716,11 +719,11 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
     if (COUNT) wc.box += 2;
     if (nd.c0 < 0 && d0 <= best) {
-      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c0)));
+      best = fminf(best, tri_d2<SLIVERS>(q, recs[~nd.c0]));
       if (COUNT) wc.tri++;
     }
     if (nd.c1 < 0 && d1 <= best) {
-      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c1)));
+      best = fminf(best, tri_d2<SLIVERS>(q, recs[~nd.c1]));
       if (COUNT) wc.tri++;
     }
     const bool g0 = nd.c0 >= 0 && d0 <= best;
@@ -776,7 +779,7 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
     gsd = a.gstackd + tid * a.gstack_depth;
   }
   WorkCount wc;
-  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.tris, q, gs, gsd,
+  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.recs, q, gs, gsd,
                                                      DEEP ? a.gstack_depth : kStack, wc);
   if (COUNT) {
     atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
@@ -888,6 +891,36 @@ __global__ __launch_bounds__(256) void k_resolve(SdfArgs a) { resolve_range(a, b
 // --------------------------------------------------------------------------------------------
 // One wave = one brick of sx * sy * sz = 64 voxels with corner (bx, by, bz); lane l is voxel
 // (l % sx, (l / sx) % sy, l / (sx * sy)) of the brick.  sstack is the wave's LDS stack.
+// Node and triangle records read at wave-uniform addresses go through the constant address space,
+// so they become scalar loads (s_load) into SGPRs instead of per-lane vector loads.  The records are
+// read as plain 4-float vectors (no constructors, valid in either address space).
+typedef float V4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(4))) V4 CV4;
+
+__device__ __forceinline__ Node load_node_uniform(const Node *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  const V4 a = r[0], b = r[1], c = r[2], d = r[3];
+  Node n;
+  n.lo0[0] = a.x; n.lo0[1] = a.y; n.lo0[2] = a.z; n.hi0[0] = a.w;
+  n.hi0[1] = b.x; n.hi0[2] = b.y; n.lo1[0] = b.z; n.lo1[1] = b.w;
+  n.lo1[2] = c.x; n.hi1[0] = c.y; n.hi1[1] = c.z; n.hi1[2] = c.w;
+  n.c0 = __float_as_int(d.x); n.c1 = __float_as_int(d.y); n.parent = __float_as_int(d.z); n.pad = 0;
+  return n;
+}
+
+union TriRecBits {
+  V4 v[9];
+  TriRec r;
+};
+
+__device__ __forceinline__ TriRec load_trirec_uniform(const TriRec *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  TriRecBits u;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) u.v[k] = r[k];
+  return u.r;
+}
+
 template <bool COUNT, bool SLIVERS, bool CAPS>
 __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int bz, int sx, int sy, int *sstack) {
   const int lane = threadIdx.x & 63;
@@ -899,34 +932,35 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   // lanes past the grid edge shadow the brick's first voxel so they never widen the search
   const int xs = valid ? x : bx, ys = valid ? y : by, zs = valid ? z : bz;
   const F3 q = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
-  // brick centre orders the children (same value in every lane)
-  const F3 bc = {g.ox + ((float)bx + 0.5f * (float)sx) * g.h, g.oy + ((float)by + 0.5f * (float)sy) * g.h,
-                 g.oz + ((float)bz + 0.5f * (float)sz) * g.h};
+  // the lane of voxel (sx/2, sy/2, sz/2) orders the children: its two box distances are read with
+  // v_readlane and compared as integers (non-negative floats order like their bit patterns)
+  const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (sz >> 1) * sx * sy);
   const Node *__restrict__ nodes = a.col.nodes;
-  const float4 *__restrict__ tris = a.col.tris;
+  const TriRec *__restrict__ recs = a.col.recs;
   float best = INFINITY;
   WorkCount wc;
   int sp = 0;
   int cur = 0;
   for (int visits = 0; visits <= a.col.nnodes; ++visits) {
-    const Node nd = nodes[cur];
+    const Node nd = load_node_uniform(nodes, cur);
     const float d0 = box_d2(q, nd.lo0, nd.hi0);
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
     if (COUNT) wc.box += 2;
     const bool any0 = __ballot(d0 <= best) != 0;
     const bool any1 = __ballot(d1 <= best) != 0;
     if (nd.c0 < 0 && any0) {
-      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c0)));
+      best = fminf(best, tri_d2<SLIVERS>(q, load_trirec_uniform(recs, ~nd.c0)));
       if (COUNT) wc.tri++;
     }
     if (nd.c1 < 0 && any1) {
-      best = fminf(best, tri_d2<SLIVERS>(q, tris + 3 * (~nd.c1)));
+      best = fminf(best, tri_d2<SLIVERS>(q, load_trirec_uniform(recs, ~nd.c1)));
       if (COUNT) wc.tri++;
     }
     const bool g0 = nd.c0 >= 0 && any0;
     const bool g1 = nd.c1 >= 0 && any1;
     if (g0 && g1) {
-      const int near0 = __builtin_amdgcn_readfirstlane(box_d2(bc, nd.lo0, nd.hi0) <= box_d2(bc, nd.lo1, nd.hi1));
+      const bool near0 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d0), centre) <=
+                         (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d1), centre);
       if (sp < kStack) {
         if (lane == 0) sstack[sp] = near0 ? nd.c1 : nd.c0;
         ++sp;
@@ -958,13 +992,54 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
 // brick shapes a wave can take (sx, sy; sz = 64 / (sx * sy)); the 4 waves of a block tile 2 x 2
 __device__ __constant__ const int kShapes[4][2] = {{4, 4}, {8, 4}, {8, 8}, {16, 4}};
 
+// One block per block-tile (2 x 2 bricks); `order`, when given, lists the tiles in launch order
+// (most expensive first, from k_tile_keys), so the long-running tiles do not form the tail.
 template <bool COUNT, bool SLIVERS, bool CAPS>
-__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a, int shape) {
+__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a, int shape, const uint32_t *__restrict__ order,
+                                                  uint32_t tx_n, uint32_t ty_n) {
   __shared__ int sstack[4][kStack];
   const int wv = threadIdx.x >> 6;
   const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 / (sx * sy);
-  const int bx = blockIdx.x * 2 * sx + (wv & 1) * sx, by = blockIdx.y * 2 * sy + (wv >> 1) * sy, bz = blockIdx.z * sz;
+  const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
+  const int bx = (int)(tile % tx_n) * 2 * sx + (wv & 1) * sx;
+  const int by = (int)((tile / tx_n) % ty_n) * 2 * sy + (wv >> 1) * sy;
+  const int bz = (int)(tile / (tx_n * ty_n)) * sz;
   wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, sx, sy, sstack[wv]);
+}
+
+// Launch-order key of every block-tile: an estimate of its query cost from the column crossings
+// of the tile's centre column.  u = vertical distance from the tile centre to the nearest crossing
+// (an upper bound of the distance there); voxels inside the surface see it in every direction and
+// cost most, so their key is 4u.  A column that misses the mesh gets half the grid height, an
+// overflowed column +inf.  keys = ~bits(key) so an ascending sort puts the costliest tile first.
+__global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint32_t tx_n, uint32_t ty_n,
+                                                   uint32_t ntiles, uint32_t *keys, int32_t *ids) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntiles) return;
+  const GridArgs &g = a.g;
+  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 / (sx * sy);
+  const int xc = min((int)(t % tx_n) * 2 * sx + sx, g.nx - 1);
+  const int yc = min((int)((t / tx_n) % ty_n) * 2 * sy + sy, g.ny - 1);
+  const float zc = g.oz + ((float)((int)(t / (tx_n * ty_n)) * sz) + 0.5f * (float)sz) * g.h;
+  const size_t col = (size_t)yc * g.nx + xc;
+  const int cnt = a.count[col];
+  float key;
+  if (cnt > kColCap) {
+    key = INFINITY;
+  } else if (cnt == 0) {
+    key = 0.5f * (float)g.nz * g.h;
+  } else {
+    float u = INFINITY;
+    int w = 0;
+    for (int k = 0; k < cnt; ++k) {
+      const float2 c = a.cross[col * kColCap + k];
+      u = fminf(u, fabsf(c.x - zc));
+      if (c.x > zc) w += (int)c.y;
+    }
+    key = w != 0 ? 4.0f * u : u;
+  }
+  keys[t] = ~__float_as_uint(key);
+  ids[t] = (int32_t)t;
 }
 
 // Persistent variant for a batch of meshes: blocks take block-tiles from one device work queue
@@ -1037,8 +1112,9 @@ hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint3
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s) {
   if (n > 1) k_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, nodes, leaf_parent);
 }
-void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, Scalars *sc, hipStream_t s) {
-  k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, sc);
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, TriRec *recs, Scalars *sc,
+                   hipStream_t s) {
+  k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, recs, sc);
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
                   float *root_box, Scalars *sc, hipStream_t s) {
@@ -1108,12 +1184,20 @@ void tile_dims(const GridArgs &g, int shape, uint32_t d[3]) {
   d[2] = nb(g.nz, sz);
 }
 
+void launch_tile_keys(const ColumnArgs &a, int shape, uint32_t *keys, int32_t *ids, hipStream_t s) {
+  uint32_t d[3];
+  tile_dims(a.g, shape, d);
+  const uint32_t n = d[0] * d[1] * d[2];
+  k_tile_keys<<<nb(n), 256, 0, s>>>(a, shape, d[0], d[1], n, keys, ids);
+}
+
 template <bool COUNT, bool SLIVERS, bool CAPS>
-static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, hipStream_t s) {
+static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, const uint32_t *order,
+                         hipStream_t s) {
   uint32_t d[3];
   tile_dims(a.col.g, shape, d);
   if (variant == 1 && !deep) {
-    k_sdf_wave<COUNT, SLIVERS, CAPS><<<dim3(d[0], d[1], d[2]), 256, 0, s>>>(a, shape);
+    k_sdf_wave<COUNT, SLIVERS, CAPS><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
     return;
   }
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
@@ -1133,8 +1217,9 @@ static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, hi
     default: fn<true, true, true>(__VA_ARGS__); break;                          \
   }
 
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, hipStream_t s) {
-  IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, deep, variant, shape, s);
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, const uint32_t *order,
+                hipStream_t s) {
+  IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, deep, variant, shape, order, s);
 }
 
 void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, s>>>(a); }

@@ -42,6 +42,7 @@ struct ColumnArgs {
   const Node *nodes;
   int nnodes;         // internal nodes (bounds every traversal loop)
   const float4 *tris;
+  const TriRec *recs;  // distance records, same order as tris
   const float *caps;  // 9 floats per cap triangle, in cap-hierarchy leaf order
   int ncap;
   const CapNode *cap_nodes;
@@ -71,7 +72,8 @@ void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, 
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                       int32_t *vout, int n, hipStream_t s);
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s);
-void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, Scalars *sc, hipStream_t s);
+void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, TriRec *recs, Scalars *sc,
+                   hipStream_t s);
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
                   float *root_box, Scalars *sc, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
@@ -92,7 +94,10 @@ struct BatchArgs {
 void tile_dims(const GridArgs &g, int shape, uint32_t d[3]);
 // variant 0: per-lane traversal, 1: wave-coherent brick traversal; slivers: the mesh has triangles
 // on the FP64 distance path; shape: brick shape of the wave (0: 4x4x4, 1: 8x4x2, 2: 8x8x1, 3: 16x4x1)
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, hipStream_t s);
+void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, const uint32_t *order,
+                hipStream_t s);
+// launch-order keys (ascending = costliest first) and tile ids of the per-brick query's block-tiles
+void launch_tile_keys(const ColumnArgs &a, int shape, uint32_t *keys, int32_t *ids, hipStream_t s);
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s);
 // narrow band: bricks whose centre distance (from the coarse pass) is <= thresh

@@ -353,50 +353,65 @@ int orc_tri_is_sliver(const float *t) {
 static inline float dot3f(const float a[3], const float b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 static inline double dot3d(const double a[3], const double b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
 
-/* Closest-point regions after Ericson, Real-Time Collision Detection (2005) 5.1.5, evaluated in
- * coordinates relative to the query; the face region uses the plane distance (n.a)^2/|n|^2. */
-float orc_tri_dist2_f32(const float q[3], const float *t) {
-  if (orc_tri_is_sliver(t)) {
-    const double qd[3] = {q[0], q[1], q[2]};
-    return (float)orc_tri_dist2_f64(qd, t);
+/* Distance record of a triangle and the branch-free FP32 distance built on it.  This is the
+ * closest-point query behind the bake's distances (the reference finds the closest triangle with
+ * OptiX, src/convert/sdf_baker.cpp:150-305, and its shaders are not part of the tree); the result is
+ * pinned against the FP64 distance (orc_tri_dist2_f64) and closed forms in the oracle tests.
+ *   e_i = p_{i+1} - p_i, n = cross(e_0, p_2 - p_0), inv_nn = 1/|n|^2, nie_i = -1/|e_i|^2,
+ *   m_i = cross(n, e_i) (in-plane normal of edge i, pointing into the triangle);
+ *   cross(a, b)_x = fmaf(a_y, b_z, -(a_z * b_y)) (cyclic), dot(a, b) = fmaf(a_z, b_z, fmaf(a_y, b_y, a_x * b_x)).
+ * Query, with a_i = p_i - q: segment i t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = fmaf(t, e_i, a_i),
+ * d_i = (r_x^2 + r_y^2) + r_z^2; face d_f = (s * s) * inv_nn with s = dot(n, a_0), used when
+ * max_i dot(m_i, a_i) <= 0; result min(d_0, d_1, d_2[, d_f]). */
+static inline float fdot3(const float a[3], const float b[3]) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
+static inline void fcross3(const float a[3], const float b[3], float o[3]) {
+  o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
+  o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
+  o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
+}
+
+void orc_trirec_make(const float *t, orc_trirec *r) {
+  for (int i = 0; i < 3; ++i) {
+    const float *a = t + 3 * i, *b = t + 3 * ((i + 1) % 3);
+    for (int k = 0; k < 3; ++k) {
+      r->p[i][k] = a[k];
+      r->e[i][k] = b[k] - a[k];
+    }
   }
-  const float a[3] = {t[0] - q[0], t[1] - q[1], t[2] - q[2]};
-  const float b[3] = {t[3] - q[0], t[4] - q[1], t[5] - q[2]};
-  const float c[3] = {t[6] - q[0], t[7] - q[1], t[8] - q[2]};
-  const float ab[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
   const float ac[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
-  const float ap[3] = {-a[0], -a[1], -a[2]};
-  const float d1 = dot3f(ab, ap), d2 = dot3f(ac, ap);
-  if (d1 <= 0.0f && d2 <= 0.0f) return dot3f(a, a);
-  const float bp[3] = {-b[0], -b[1], -b[2]};
-  const float d3 = dot3f(ab, bp), d4 = dot3f(ac, bp);
-  if (d3 >= 0.0f && d4 <= d3) return dot3f(b, b);
-  const float vc = d1 * d4 - d3 * d2;
-  if (vc <= 0.0f && d1 >= 0.0f && d3 <= 0.0f) {
-    const float v = d1 / (d1 - d3);
-    const float r[3] = {a[0] + v * ab[0], a[1] + v * ab[1], a[2] + v * ab[2]};
-    return dot3f(r, r);
+  fcross3(r->e[0], ac, r->n);
+  for (int i = 0; i < 3; ++i) {
+    r->nie[i] = -1.0f / fdot3(r->e[i], r->e[i]);
+    fcross3(r->n, r->e[i], r->m[i]);
   }
-  const float cp[3] = {-c[0], -c[1], -c[2]};
-  const float d5 = dot3f(ab, cp), d6 = dot3f(ac, cp);
-  if (d6 >= 0.0f && d5 <= d6) return dot3f(c, c);
-  const float vb = d5 * d2 - d1 * d6;
-  if (vb <= 0.0f && d2 >= 0.0f && d6 <= 0.0f) {
-    const float w = d2 / (d2 - d6);
-    const float r[3] = {a[0] + w * ac[0], a[1] + w * ac[1], a[2] + w * ac[2]};
-    return dot3f(r, r);
+  r->inv_nn = 1.0f / fdot3(r->n, r->n);
+  r->sliver = orc_tri_is_sliver(t);
+}
+
+float orc_trirec_dist2(const float q[3], const orc_trirec *r) {
+  if (r->sliver) {
+    const double qd[3] = {q[0], q[1], q[2]};
+    return (float)orc_tri_dist2_f64(qd, &r->p[0][0]);
   }
-  const float va = d3 * d6 - d5 * d4;
-  const float e43 = d4 - d3, e56 = d5 - d6;
-  if (va <= 0.0f && e43 >= 0.0f && e56 >= 0.0f) {
-    const float w = e43 / (e43 + e56);
-    const float bc[3] = {t[6] - t[3], t[7] - t[4], t[8] - t[5]};
-    const float r[3] = {b[0] + w * bc[0], b[1] + w * bc[1], b[2] + w * bc[2]};
-    return dot3f(r, r);
+  float a[3][3];
+  for (int i = 0; i < 3; ++i)
+    for (int k = 0; k < 3; ++k) a[i][k] = r->p[i][k] - q[k];
+  float d = INFINITY;
+  for (int i = 0; i < 3; ++i) {
+    const float u = fminf(fmaxf(fdot3(a[i], r->e[i]) * r->nie[i], 0.0f), 1.0f);
+    const float x = fmaf(u, r->e[i][0], a[i][0]), y = fmaf(u, r->e[i][1], a[i][1]), z = fmaf(u, r->e[i][2], a[i][2]);
+    d = fminf(d, (x * x + y * y) + z * z);
   }
-  const float n[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
-  const float s = dot3f(n, a);
-  return (s * s) / dot3f(n, n);
+  const float k0 = fdot3(r->m[0], a[0]), k1 = fdot3(r->m[1], a[1]), k2 = fdot3(r->m[2], a[2]);
+  const float s = fdot3(r->n, a[0]);
+  const float df = (s * s) * r->inv_nn;
+  return fmaxf(fmaxf(k0, k1), k2) <= 0.0f ? fminf(d, df) : d;
+}
+
+float orc_tri_dist2_f32(const float q[3], const float *t) {
+  orc_trirec r;
+  orc_trirec_make(t, &r);
+  return orc_trirec_dist2(q, &r);
 }
 
 static double seg_dist2_d(const double a[3], const double b[3]) {
@@ -770,9 +785,16 @@ static inline float box_dist2(const float q[3], const float *lo, const float *hi
 }
 
 typedef struct {
-  const float *tris; /* sorted soup, 9 floats */
+  const float *tris;       /* sorted soup, 9 floats */
   const orc_bvh *b;
+  const orc_trirec *recs;  /* distance records of the sorted soup */
 } qctx;
+
+static orc_trirec *make_recs(const float *sorted, size_t ntri) {
+  orc_trirec *r = (orc_trirec *)malloc(sizeof(orc_trirec) * (ntri ? ntri : 1));
+  for (size_t i = 0; i < ntri; ++i) orc_trirec_make(sorted + 9 * i, r + i);
+  return r;
+}
 
 static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t *tests) {
   const orc_bvh *b = c->b;
@@ -780,7 +802,7 @@ static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t
   float best = INFINITY;
   if (ni == 0) {
     (*tests)++;
-    return orc_tri_dist2_f32(q, c->tris);
+    return orc_trirec_dist2(q, c->recs);
   }
   int stack[256];
   int sp = 0;
@@ -800,7 +822,7 @@ static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t
       if (!(dd[k] <= best)) continue;
       if (ch[k] < 0) {
         (*tests)++;
-        const float d2 = orc_tri_dist2_f32(q, c->tris + 9 * (size_t)(~ch[k]));
+        const float d2 = orc_trirec_dist2(q, c->recs + (size_t)(~ch[k]));
         if (d2 < best) best = d2;
       } else {
         stack[sp++] = ch[k];
@@ -926,7 +948,8 @@ int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3],
   if (orc_bvh_build(xyz, ntri, &b) != 0) return -2;
   float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
   for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
-  qctx c = {sorted, &b};
+  orc_trirec *recs = make_recs(sorted, ntri);
+  qctx c = {sorted, &b, recs};
   if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
   if (z0 < 0) z0 = 0;
   if (z1 > nz || z1 < 0) z1 = nz;
@@ -959,6 +982,7 @@ int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3],
   free(tid);
   free(ths);
   free(sorted);
+  free(recs);
   orc_bvh_free(&b);
   return 0;
 }
@@ -1113,7 +1137,8 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
   if (orc_bvh_build(xyz, ntri, &b) != 0) return -1;
   float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
   for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
-  qctx c = {sorted, &b};
+  orc_trirec *recs = make_recs(sorted, ntri);
+  qctx c = {sorted, &b, recs};
   if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
   pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
   udf_job j = {&c, pts, npts, out, 0, &mu};
@@ -1122,6 +1147,7 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
   for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
   free(tid);
   free(sorted);
+  free(recs);
   orc_bvh_free(&b);
   return 0;
 }

@@ -54,7 +54,18 @@ void orc_occupancy(const float *xyz, size_t ntri, const float origin[3], float h
 /* ---- distance ----------------------------------------------------------------- */
 /* 1 if the triangle takes the FP64 path (near-degenerate: sin^2 of its corner angle < 1e-4). */
 int orc_tri_is_sliver(const float *tri9);
-/* squared distance point->triangle: FP32 in query-local coordinates, FP64 for slivers */
+/* distance record of one triangle (vertices, edges, normal, reciprocals, edge normals) */
+typedef struct {
+  float p[3][3];
+  float e[3][3];
+  float n[3], inv_nn;
+  float nie[3];
+  float m[3][3];
+  int sliver;
+} orc_trirec;
+void orc_trirec_make(const float *tri9, orc_trirec *r);
+float orc_trirec_dist2(const float q[3], const orc_trirec *r);
+/* squared distance point->triangle: branch-free FP32 over the distance record, FP64 for slivers */
 float orc_tri_dist2_f32(const float q[3], const float *tri9);
 /* squared distance point->triangle, all FP64 (min over edges + interior projection) */
 double orc_tri_dist2_f64(const double q[3], const float *tri9);
