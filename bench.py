@@ -24,7 +24,7 @@ METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake late
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published)
 PEAK_HBM_GBS = 8000.0
 # FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max
-# and compares not counted): point/box distance 6 sub + 3 mul + 2 add = 11; branch-free point/
+# and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
 # triangle distance 9 sub + 3 segments x (dot 5 + scale 1 + fma point 6 + square 5) + 3 edge-normal
 # dots x 5 + plane dot 5 + square and scale 2 = 82
 FLOP_BOX = 11

@@ -77,7 +77,8 @@ typedef enum {
                             ib_stats.box_tests / tri_tests (slower; for roofline accounting) */
   IB_OPT_QUERY_VARIANT = 2, /* closest-triangle kernel: 0 = one traversal per voxel, 1 = one traversal per
                                brick shared by the wave (default); results are identical */
-  IB_OPT_BRICK_SHAPE = 3,   /* voxels of one wave: 0 = 4x4x4 (default), 1 = 8x4x2, 2 = 8x8x1, 3 = 16x4x1 */
+  IB_OPT_BRICK_SHAPE = 3,   /* voxels of one wave: 0 = 4x4x4 (default), 1 = 8x4x2, 2 = 8x8x1, 3 = 16x4x1,
+                               4 = 4x4x8 (two voxels per lane) */
   IB_OPT_BLOCKS_PER_CU = 4, /* persistent blocks per compute unit in batched bakes (default 8) */
   IB_OPT_TILE_ORDER = 5     /* 1 (default): dense per-brick queries launch their tiles costliest first
                                (estimated from the column crossings); 0: grid order.  Speed only. */

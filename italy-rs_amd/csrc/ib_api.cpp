@@ -113,10 +113,10 @@ struct ib_context {
     const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
     Buf &b = bufs[name];
     if (b.bytes < bytes) {
+      const size_t want = std::max(bytes + bytes / 4, 2 * b.bytes);  // geometric growth: few reallocations
       free_raw(b.p, b.bytes);
       b.p = nullptr;
       b.bytes = 0;
-      const size_t want = bytes + bytes / 4;
       void *p = nullptr;
       ib_status st = alloc_raw(want, &p);
       if (st != IB_OK) {
@@ -297,22 +297,44 @@ float ev_ms(ib_context *c, EventId a, EventId b) {
   return ms;
 }
 
-// Device allocations owned by one batch (or one served job) instead of the context's scratch.
+// Device allocations owned by one batch instead of the context's scratch: a bump allocator over
+// chunks of kChunk bytes (larger requests get a block of their own, rounded to 2 MiB), so a batch of
+// hundreds of meshes makes a handful of device allocations, and the chunk sizes recur from batch to
+// batch so the context's size-keyed cache serves them.  Sub-allocations are 256-byte aligned.
 struct Arena {
+  static constexpr size_t kChunk = 64ull << 20;
   ib_context *c;
   std::vector<std::pair<void *, size_t>> blocks;
+  char *cur = nullptr;
+  size_t left = 0;
   template <typename T>
   ib_status get(size_t count, T **out) {
-    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    void *p = nullptr;
-    IB_TRY(c->alloc_raw(bytes, &p));
-    blocks.push_back({p, bytes});
-    *out = static_cast<T *>(p);
+    const size_t bytes = (std::max<size_t>(count * sizeof(T), 256) + 255) & ~size_t(255);
+    if (bytes > left) {
+      if (bytes > kChunk / 4) {  // big: a block of its own, the current chunk stays open
+        const size_t bb = (bytes + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
+        void *p = nullptr;
+        IB_TRY(c->alloc_raw(bb, &p));
+        blocks.push_back({p, bb});
+        *out = static_cast<T *>(p);
+        return IB_OK;
+      }
+      void *p = nullptr;
+      IB_TRY(c->alloc_raw(kChunk, &p));
+      blocks.push_back({p, kChunk});
+      cur = static_cast<char *>(p);
+      left = kChunk;
+    }
+    *out = reinterpret_cast<T *>(cur);
+    cur += bytes;
+    left -= bytes;
     return IB_OK;
   }
   void release() {
     for (auto &b : blocks) c->free_raw(b.first, b.second);
     blocks.clear();
+    cur = nullptr;
+    left = 0;
   }
   ~Arena() { release(); }
 };
@@ -1064,7 +1086,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     IB_HIP(hipMemcpyAsync(d_args, args.data(), args.size() * sizeof(SdfArgs), hipMemcpyHostToDevice, s));
     IB_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     fill_u32(d_head, 0u, 1, s);
-    BatchArgs t{d_args, d_prefix, (uint32_t)args.size(), d_head, c->brick_shape};
+    BatchArgs t{d_args, d_prefix, (uint32_t)args.size(), d_head, c->brick_shape, nullptr};
     const uint32_t blocks = std::min<uint32_t>(prefix.back(), (uint32_t)(c->cus * c->blocks_per_cu));
     launch_sdf_batch(t, std::max(blocks, 1u), c->count_work, any_slivers, any_caps, s);
     launch_resolve_batch(d_args, (uint32_t)args.size(), s);
@@ -1134,7 +1156,7 @@ ib_status ib_autotune_batch(ib_context *c, const ib_job *jobs, uint32_t njobs, u
   static const int kBlocks[3] = {4, 8, 16};
   int best_shape = c->brick_shape, best_blocks = c->blocks_per_cu;
   double best_ms = 1e300;
-  for (int shape = 0; shape < 4; ++shape)
+  for (int shape = 0; shape < 5; ++shape)
     for (int b : kBlocks) {
       c->brick_shape = shape;
       c->blocks_per_cu = b;
@@ -1175,7 +1197,7 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
       c->query_variant = (int)value;
       return IB_OK;
     case IB_OPT_BRICK_SHAPE:
-      if (value < 0 || value > 3) return fail(IB_ERR_INVALID_ARGUMENT, "brick shape must be 0..3");
+      if (value < 0 || value > 4) return fail(IB_ERR_INVALID_ARGUMENT, "brick shape must be 0..4");
       c->brick_shape = (int)value;
       return IB_OK;
     case IB_OPT_BLOCKS_PER_CU:

@@ -114,14 +114,16 @@ __device__ __forceinline__ bool tri_box(F3 c, float bh, F3 t0, F3 t1, F3 t2) {
 __device__ __forceinline__ float dot3(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 
 // Squared distance from q to the box [lo, hi] (lo <= hi).  Per axis the gap is lo - q below the
-// box, q - hi above it and 0 inside; max(lo - q, q - hi, 0) gives exactly those values (each
-// difference is one rounded subtraction), so this branch-free form is bit-identical to the
-// three-way branch and maps to one v_max3_f32 per axis.
+// box, q - hi above it and 0 inside: max(lo - q, q - hi, 0) (one v_max3_f32).  The squares are
+// summed as fma(z, z, fma(y, y, x * x)), the same form as the triangle distance's |r|^2, so a
+// triangle vertex that is a box corner gives exactly its box's value.
+__device__ __forceinline__ float sq3(float x, float y, float z) { return fmaf(z, z, fmaf(y, y, x * x)); }
+
 __device__ __forceinline__ float box_d2(F3 q, const float *lo, const float *hi) {
   const float dx = fmaxf(fmaxf(lo[0] - q.x, q.x - hi[0]), 0.0f);
   const float dy = fmaxf(fmaxf(lo[1] - q.y, q.y - hi[1]), 0.0f);
   const float dz = fmaxf(fmaxf(lo[2] - q.z, q.z - hi[2]), 0.0f);
-  return (dx * dx + dy * dy) + dz * dz;
+  return sq3(dx, dy, dz);
 }
 
 __device__ __forceinline__ double dot3d(double ax, double ay, double az, double bx, double by, double bz) {
@@ -216,8 +218,7 @@ __device__ __forceinline__ void make_trirec(F3 p0, F3 p1, F3 p2, uint32_t flags,
 //   segment i: t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = a_i + t e_i (fma), d_i = |r|^2
 //   face:      inside = max_i dot(m_i, a_i) <= 0, d_f = (dot(n, a_0)^2) * inv_nn
 //   result     min(d_0, d_1, d_2, inside ? d_f : inf)
-// |r|^2 is summed as (x^2 + y^2) + z^2 without fusion, the order of the box distance, so a vertex
-// that is a box corner gives the same value as its box.
+// |r|^2 = fma(r_z, r_z, fma(r_y, r_y, r_x * r_x)), the form of the box distance.
 __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
   F3 a[3];
   for (int i = 0; i < 3; ++i) a[i] = {t.p[i][0] - q.x, t.p[i][1] - q.y, t.p[i][2] - q.z};
@@ -226,7 +227,7 @@ __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
     const F3 e = {t.e[i][0], t.e[i][1], t.e[i][2]};
     const float u = fminf(fmaxf(fdot(a[i], e) * t.nie[i], 0.0f), 1.0f);
     const F3 r = {fmaf(u, e.x, a[i].x), fmaf(u, e.y, a[i].y), fmaf(u, e.z, a[i].z)};
-    d = fminf(d, (r.x * r.x + r.y * r.y) + r.z * r.z);
+    d = fminf(d, sq3(r.x, r.y, r.z));
   }
   const float k0 = fdot({t.m[0][0], t.m[0][1], t.m[0][2]}, a[0]);
   const float k1 = fdot({t.m[1][0], t.m[1][1], t.m[1][2]}, a[1]);

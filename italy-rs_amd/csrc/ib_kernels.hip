@@ -332,77 +332,84 @@ __device__ __forceinline__ F3 cell_centre(const GridArgs &g, int x, int y, int z
   return {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
 }
 
-__global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ xyz, int n, GridArgs g,
-                                                   uint32_t *words) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= n) return;
-  const float *p = xyz + 9 * (size_t)t;
+// One wave per triangle: the cells of the triangle's AABB are numbered x fastest and dealt to the
+// 64 lanes (cell = lane + 64 i), so a triangle that spans thousands of cells (a coarse mesh on a fine
+// grid) is tested 64 cells at a time instead of by one thread.  fn(x, y, z, hit) runs in every lane
+// of every round, uniformly (hit = false for lanes past the last cell), so it may use wave ballots.
+template <typename Fn>
+__device__ __forceinline__ void tri_cells_wave(const float *p, const GridArgs &g, Fn &&fn) {
+  const int lane = threadIdx.x & 63;
   const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
   int lo[3], hi[3];
   tri_cells(p, g, lo, hi);
   const float half = g.h * 0.5f;
+  const uint32_t ex = (uint32_t)(hi[0] - lo[0] + 1), ey = (uint32_t)(hi[1] - lo[1] + 1);
+  const uint64_t total = (uint64_t)ex * ey * (uint64_t)(hi[2] - lo[2] + 1);
+  for (uint64_t base = 0; base < total; base += 64) {
+    const uint64_t c = base + (uint64_t)lane;
+    int x = 0, y = 0, z = 0;
+    bool hit = false;
+    if (c < total) {
+      const uint64_t row = c / ex;
+      x = lo[0] + (int)(c - row * ex);
+      y = lo[1] + (int)(row % ey);
+      z = lo[2] + (int)(row / ey);
+      hit = tri_box(cell_centre(g, x, y, z), half, t0, t1, t2);
+    }
+    fn(x, y, z, hit);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ xyz, int n, GridArgs g,
+                                                   uint32_t *words) {
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= n) return;
   const int wpr = (g.nx + 31) >> 5;
-  for (int z = lo[2]; z <= hi[2]; ++z)
-    for (int y = lo[1]; y <= hi[1]; ++y)
-      for (int x = lo[0]; x <= hi[0]; ++x)
-        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2))
-          atomicOr(&words[((size_t)z * g.ny + y) * wpr + (x >> 5)], 1u << (x & 31));
+  tri_cells_wave(xyz + 9 * (size_t)t, g, [&](int x, int y, int z, bool hit) {
+    if (hit) atomicOr(&words[((size_t)z * g.ny + y) * wpr + (x >> 5)], 1u << (x & 31));
+  });
 }
 
 // first touching triangle per cell (the order key of voxelize.cpp:88-96)
 __global__ __launch_bounds__(256) void k_first_touch(const float *__restrict__ xyz, int n, GridArgs g,
                                                      int32_t *first) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n) return;
-  const float *p = xyz + 9 * (size_t)t;
-  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
-  int lo[3], hi[3];
-  tri_cells(p, g, lo, hi);
-  const float half = g.h * 0.5f;
-  for (int z = lo[2]; z <= hi[2]; ++z)
-    for (int y = lo[1]; y <= hi[1]; ++y)
-      for (int x = lo[0]; x <= hi[0]; ++x)
-        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2))
-          atomicMin(&first[((size_t)z * g.ny + y) * g.nx + x], t);
+  tri_cells_wave(xyz + 9 * (size_t)t, g, [&](int x, int y, int z, bool hit) {
+    if (hit) atomicMin(&first[((size_t)z * g.ny + y) * g.nx + x], t);
+  });
 }
 
 // per-cell attribute splat (sdf_baker.cpp:76-128, voxelize.cpp:94-104): every (triangle, cell)
-// overlap pair in triangle order, stably sorted by cell, then summed sequentially per cell
+// overlap pair, triangles in order (pairs of one triangle are contiguous, each cell once), stably
+// sorted by cell, then summed sequentially per cell
 __global__ __launch_bounds__(256) void k_splat_count(const float *__restrict__ xyz, int n, GridArgs g,
                                                      uint32_t *counts) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n) return;
-  const float *p = xyz + 9 * (size_t)t;
-  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
-  int lo[3], hi[3];
-  tri_cells(p, g, lo, hi);
-  const float half = g.h * 0.5f;
   uint32_t c = 0;
-  for (int z = lo[2]; z <= hi[2]; ++z)
-    for (int y = lo[1]; y <= hi[1]; ++y)
-      for (int x = lo[0]; x <= hi[0]; ++x) c += tri_box(cell_centre(g, x, y, z), half, t0, t1, t2) ? 1u : 0u;
-  counts[t] = c;
+  tri_cells_wave(xyz + 9 * (size_t)t, g, [&](int, int, int, bool hit) {
+    c += (uint32_t)__popcll(__ballot(hit));
+  });
+  if ((threadIdx.x & 63) == 0) counts[t] = c;
 }
 
 __global__ __launch_bounds__(256) void k_splat_emit(const float *__restrict__ xyz, int n, GridArgs g,
                                                     const uint32_t *__restrict__ offsets, uint32_t *cell,
                                                     int32_t *tri) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (t >= n) return;
-  const float *p = xyz + 9 * (size_t)t;
-  const F3 t0 = {p[0], p[1], p[2]}, t1 = {p[3], p[4], p[5]}, t2 = {p[6], p[7], p[8]};
-  int lo[3], hi[3];
-  tri_cells(p, g, lo, hi);
-  const float half = g.h * 0.5f;
+  const uint64_t below = (1ull << (threadIdx.x & 63)) - 1ull;
   uint32_t o = offsets[t];
-  for (int z = lo[2]; z <= hi[2]; ++z)
-    for (int y = lo[1]; y <= hi[1]; ++y)
-      for (int x = lo[0]; x <= hi[0]; ++x)
-        if (tri_box(cell_centre(g, x, y, z), half, t0, t1, t2)) {
-          cell[o] = (uint32_t)(((uint64_t)z * g.ny + y) * g.nx + x);
-          tri[o] = t;
-          ++o;
-        }
+  tri_cells_wave(xyz + 9 * (size_t)t, g, [&](int x, int y, int z, bool hit) {
+    const uint64_t hits = __ballot(hit);
+    if (hit) {
+      const uint32_t at = o + (uint32_t)__popcll(hits & below);
+      cell[at] = (uint32_t)(((uint64_t)z * g.ny + y) * g.nx + x);
+      tri[at] = t;
+    }
+    o += (uint32_t)__popcll(hits);
+  });
 }
 
 // out: per cell (k sums followed by the count as float), written only for touched cells
@@ -921,43 +928,53 @@ __device__ __forceinline__ TriRec load_trirec_uniform(const TriRec *p, int i) {
   return u.r;
 }
 
-template <bool COUNT, bool SLIVERS, bool CAPS>
+// One wave = one brick of sx * sy * sz voxels with corner (bx, by, bz), V voxels per lane: lane l
+// holds voxels (l % sx, (l / sx) % sy, l / (sx * sy) + k * sz / V) for k < V.  sstack is the wave's
+// LDS stack.  Every lane's voxels are queried against every triangle the brick visits.
+template <bool COUNT, bool SLIVERS, bool CAPS, int V>
 __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int bz, int sx, int sy, int *sstack) {
   const int lane = threadIdx.x & 63;
   const GridArgs &g = a.col.g;
   if (bx >= g.nx || by >= g.ny || bz >= g.nz) return;  // whole brick outside (wave-uniform)
-  const int sz = 64 / (sx * sy);
-  const int x = bx + lane % sx, y = by + (lane / sx) % sy, z = bz + lane / (sx * sy);
-  const bool valid = x < g.nx && y < g.ny && z < g.nz;
-  // lanes past the grid edge shadow the brick's first voxel so they never widen the search
-  const int xs = valid ? x : bx, ys = valid ? y : by, zs = valid ? z : bz;
-  const F3 q = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
-  // the lane of voxel (sx/2, sy/2, sz/2) orders the children: its two box distances are read with
-  // v_readlane and compared as integers (non-negative floats order like their bit patterns)
-  const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (sz >> 1) * sx * sy);
+  const int sz = 64 * V / (sx * sy), lz = sz / V;
+  const int x = bx + lane % sx, y = by + (lane / sx) % sy;
+  int z[V];
+  bool valid[V];
+  F3 q[V];
+  float best[V];
+  for (int k = 0; k < V; ++k) {
+    z[k] = bz + lane / (sx * sy) + k * lz;
+    valid[k] = x < g.nx && y < g.ny && z[k] < g.nz;
+    // lanes past the grid edge shadow the brick's first voxel so they never widen the search
+    const int xs = valid[k] ? x : bx, ys = valid[k] ? y : by, zs = valid[k] ? z[k] : bz;
+    q[k] = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
+    best[k] = INFINITY;
+  }
+  // the lane of voxel (sx/2, sy/2, lz/2) orders the children: its two box distances (first voxel)
+  // are read with v_readlane and compared as integers (non-negative floats order like their bits)
+  const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (lz >> 1) * sx * sy);
   const Node *__restrict__ nodes = a.col.nodes;
   const TriRec *__restrict__ recs = a.col.recs;
-  float best = INFINITY;
   WorkCount wc;
   int sp = 0;
-  int cur = 0;
+  Node nd = load_node_uniform(nodes, 0);
   for (int visits = 0; visits <= a.col.nnodes; ++visits) {
-    const Node nd = load_node_uniform(nodes, cur);
-    const float d0 = box_d2(q, nd.lo0, nd.hi0);
-    const float d1 = box_d2(q, nd.lo1, nd.hi1);
-    if (COUNT) wc.box += 2;
-    const bool any0 = __ballot(d0 <= best) != 0;
-    const bool any1 = __ballot(d1 <= best) != 0;
-    if (nd.c0 < 0 && any0) {
-      best = fminf(best, tri_d2<SLIVERS>(q, load_trirec_uniform(recs, ~nd.c0)));
-      if (COUNT) wc.tri++;
+    float d0 = box_d2(q[0], nd.lo0, nd.hi0);
+    float d1 = box_d2(q[0], nd.lo1, nd.hi1);
+    bool n0 = d0 <= best[0], n1 = d1 <= best[0];
+    for (int k = 1; k < V; ++k) {
+      n0 = n0 || box_d2(q[k], nd.lo0, nd.hi0) <= best[k];
+      n1 = n1 || box_d2(q[k], nd.lo1, nd.hi1) <= best[k];
     }
-    if (nd.c1 < 0 && any1) {
-      best = fminf(best, tri_d2<SLIVERS>(q, load_trirec_uniform(recs, ~nd.c1)));
-      if (COUNT) wc.tri++;
-    }
-    const bool g0 = nd.c0 >= 0 && any0;
-    const bool g1 = nd.c1 >= 0 && any1;
+    if (COUNT) wc.box += 2 * V;
+    const bool any0 = __ballot(n0) != 0;
+    const bool any1 = __ballot(n1) != 0;
+    const bool leaf0 = nd.c0 < 0 && any0, leaf1 = nd.c1 < 0 && any1;
+    const bool g0 = nd.c0 >= 0 && any0, g1 = nd.c1 >= 0 && any1;
+    const int tri0 = ~nd.c0, tri1 = ~nd.c1;
+    // choose the next node before evaluating this node's triangles, so that its record load is in
+    // flight together with theirs
+    int next = -1;
     if (g0 && g1) {
       const bool near0 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d0), centre) <=
                          (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d1), centre);
@@ -965,46 +982,59 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
         if (lane == 0) sstack[sp] = near0 ? nd.c1 : nd.c0;
         ++sp;
       }
-      cur = near0 ? nd.c0 : nd.c1;
-      continue;
+      next = near0 ? nd.c0 : nd.c1;
+    } else if (g0) {
+      next = nd.c0;
+    } else if (g1) {
+      next = nd.c1;
+    } else if (sp > 0) {
+      --sp;
+      __builtin_amdgcn_wave_barrier();
+      next = __builtin_amdgcn_readfirstlane(sstack[sp]);
     }
-    if (g0) {
-      cur = nd.c0;
-      continue;
+    if (next >= 0) nd = load_node_uniform(nodes, next);
+    if (leaf0) {
+      const TriRec t = load_trirec_uniform(recs, tri0);
+      for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
+      if (COUNT) wc.tri += V;
     }
-    if (g1) {
-      cur = nd.c1;
-      continue;
+    if (leaf1) {
+      const TriRec t = load_trirec_uniform(recs, tri1);
+      for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
+      if (COUNT) wc.tri += V;
     }
-    if (sp == 0) break;
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
+    if (next < 0) break;
   }
-  if (COUNT && valid) {
-    atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
-    atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
+  if (COUNT) {
+    int nv = 0;
+    for (int k = 0; k < V; ++k) nv += valid[k] ? 1 : 0;
+    if (nv) {
+      atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box * nv / V);
+      atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri * nv / V);
+    }
   }
-  if (!valid) return;
-  finish_voxel<CAPS>(a, q, x, y, z, best);
+  for (int k = 0; k < V; ++k)
+    if (valid[k]) finish_voxel<CAPS>(a, q[k], x, y, z[k], best[k]);
 }
 
-// brick shapes a wave can take (sx, sy; sz = 64 / (sx * sy)); the 4 waves of a block tile 2 x 2
-__device__ __constant__ const int kShapes[4][2] = {{4, 4}, {8, 4}, {8, 8}, {16, 4}};
+// brick shapes a wave can take (sx, sy, voxels per lane; sz = 64 * V / (sx * sy)); the 4 waves of
+// a block tile 2 x 2
+constexpr int kNumShapes = 5;
+__device__ __constant__ const int kShapes[kNumShapes][3] = {{4, 4, 1}, {8, 4, 1}, {8, 8, 1}, {16, 4, 1}, {4, 4, 2}};
 
 // One block per block-tile (2 x 2 bricks); `order`, when given, lists the tiles in launch order
 // (most expensive first, from k_tile_keys), so the long-running tiles do not form the tail.
-template <bool COUNT, bool SLIVERS, bool CAPS>
-__global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a, int shape, const uint32_t *__restrict__ order,
+template <bool COUNT, bool SLIVERS, bool CAPS, int V>
+__global__ __launch_bounds__(256, 8) void k_sdf_wave(SdfArgs a, int shape, const uint32_t *__restrict__ order,
                                                   uint32_t tx_n, uint32_t ty_n) {
   __shared__ int sstack[4][kStack];
   const int wv = threadIdx.x >> 6;
-  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 / (sx * sy);
+  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 * V / (sx * sy);
   const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
   const int bx = (int)(tile % tx_n) * 2 * sx + (wv & 1) * sx;
   const int by = (int)((tile / tx_n) % ty_n) * 2 * sy + (wv >> 1) * sy;
   const int bz = (int)(tile / (tx_n * ty_n)) * sz;
-  wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, sx, sy, sstack[wv]);
+  wave_brick<COUNT, SLIVERS, CAPS, V>(a, bx, by, bz, sx, sy, sstack[wv]);
 }
 
 // Launch-order key of every block-tile: an estimate of its query cost from the column crossings
@@ -1012,12 +1042,11 @@ __global__ __launch_bounds__(256) void k_sdf_wave(SdfArgs a, int shape, const ui
 // (an upper bound of the distance there); voxels inside the surface see it in every direction and
 // cost most, so their key is 4u.  A column that misses the mesh gets half the grid height, an
 // overflowed column +inf.  keys = ~bits(key) so an ascending sort puts the costliest tile first.
-__global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint32_t tx_n, uint32_t ty_n,
-                                                   uint32_t ntiles, uint32_t *keys, int32_t *ids) {
-  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
-  if (t >= ntiles) return;
+__device__ __forceinline__ uint32_t tile_key(const ColumnArgs &a, int shape, uint32_t t) {
   const GridArgs &g = a.g;
-  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 / (sx * sy);
+  const uint32_t tx_n = (uint32_t)(g.nx + 2 * kShapes[shape][0] - 1) / (2 * kShapes[shape][0]);
+  const uint32_t ty_n = (uint32_t)(g.ny + 2 * kShapes[shape][1] - 1) / (2 * kShapes[shape][1]);
+  const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 * kShapes[shape][2] / (sx * sy);
   const int xc = min((int)(t % tx_n) * 2 * sx + sx, g.nx - 1);
   const int yc = min((int)((t / tx_n) % ty_n) * 2 * sy + sy, g.ny - 1);
   const float zc = g.oz + ((float)((int)(t / (tx_n * ty_n)) * sz) + 0.5f * (float)sz) * g.h;
@@ -1038,30 +1067,44 @@ __global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint
     }
     key = w != 0 ? 4.0f * u : u;
   }
-  keys[t] = ~__float_as_uint(key);
+  return ~__float_as_uint(key);
+}
+
+__global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint32_t ntiles, uint32_t *keys,
+                                                   int32_t *ids) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntiles) return;
+  keys[t] = tile_key(a, shape, t);
   ids[t] = (int32_t)t;
 }
 
+__device__ __forceinline__ int job_of_tile(const BatchArgs &t, uint32_t tile) {
+  int lo = 0, hi = (int)t.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (t.tile_prefix[mid] <= tile) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+
 // Persistent variant for a batch of meshes: blocks take block-tiles from one device work queue
 // (tile_prefix[j] = first tile of job j) until the queue is empty.
-template <bool COUNT, bool SLIVERS, bool CAPS>
-__global__ __launch_bounds__(256) void k_sdf_batch(BatchArgs t) {
+template <bool COUNT, bool SLIVERS, bool CAPS, int V>
+__global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
   __shared__ int sstack[4][kStack];
   __shared__ uint32_t s_tile;
   const int wv = threadIdx.x >> 6;
-  const int sx = kShapes[t.shape][0], sy = kShapes[t.shape][1], sz = 64 / (sx * sy);
+  const int sx = kShapes[t.shape][0], sy = kShapes[t.shape][1], sz = 64 * V / (sx * sy);
   const uint32_t total = t.tile_prefix[t.njobs];
   for (;;) {
     if (threadIdx.x == 0) s_tile = atomicAdd(t.head, 1u);
     __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t pos = s_tile;
     __syncthreads();
-    if (tile >= total) break;
-    int lo = 0, hi = (int)t.njobs - 1;  // job owning this tile
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (t.tile_prefix[mid] <= tile) lo = mid; else hi = mid - 1;
-    }
+    if (pos >= total) break;
+    const uint32_t tile = t.order ? t.order[pos] : pos;
+    const int lo = job_of_tile(t, tile);
     const SdfArgs &a = t.jobs[lo];
     const uint32_t local = tile - t.tile_prefix[lo];
     const GridArgs &g = a.col.g;
@@ -1072,14 +1115,14 @@ __global__ __launch_bounds__(256) void k_sdf_batch(BatchArgs t) {
       const int bx = (int)(b % nbx) * 8 + (sub & 1) * 4;
       const int by = (int)((b / nbx) % nby) * 8 + ((sub >> 1) & 1) * 4;
       const int bz = (int)(b / (nbx * nby)) * 8 + (sub >> 2) * 4;
-      wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, 4, 4, sstack[wv]);
+      wave_brick<COUNT, SLIVERS, CAPS, 1>(a, bx, by, bz, 4, 4, sstack[wv]);
       continue;
     }
     const uint32_t tx = (g.nx + 2 * sx - 1) / (2 * sx), ty = (g.ny + 2 * sy - 1) / (2 * sy);
     const int bx = (int)(local % tx) * 2 * sx + (wv & 1) * sx;
     const int by = (int)((local / tx) % ty) * 2 * sy + (wv >> 1) * sy;
     const int bz = (int)(local / (tx * ty)) * sz;
-    wave_brick<COUNT, SLIVERS, CAPS>(a, bx, by, bz, sx, sy, sstack[wv]);
+    wave_brick<COUNT, SLIVERS, CAPS, V>(a, bx, by, bz, sx, sy, sstack[wv]);
   }
 }
 
@@ -1135,10 +1178,10 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
   k_edge_collect<<<nb((uint64_t)emask + 1), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
 }
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
-  k_occupancy<<<nb(n), 256, 0, s>>>(xyz, n, g, words);
+  k_occupancy<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, words);
 }
 void launch_splat_count(const float *xyz, int n, const GridArgs &g, uint32_t *counts, hipStream_t s) {
-  k_splat_count<<<nb(n), 256, 0, s>>>(xyz, n, g, counts);
+  k_splat_count<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, counts);
 }
 hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in, uint32_t *out, uint32_t n,
                               hipStream_t s) {
@@ -1146,7 +1189,7 @@ hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in
 }
 void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_t *offsets, uint32_t *cell,
                        int32_t *tri, hipStream_t s) {
-  k_splat_emit<<<nb(n), 256, 0, s>>>(xyz, n, g, offsets, cell, tri);
+  k_splat_emit<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, offsets, cell, tri);
 }
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                           int32_t *vout, uint32_t n, int end_bit, hipStream_t s) {
@@ -1157,7 +1200,7 @@ void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npai
   if (npairs) k_splat_reduce<<<nb(npairs), 256, 0, s>>>(cell, tri, npairs, attr, k, out, ncell);
 }
 void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s) {
-  k_first_touch<<<nb(n), 256, 0, s>>>(xyz, n, g, first);
+  k_first_touch<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, first);
 }
 void launch_first_compact(const int32_t *first, uint64_t ncell, Scalars *sc, unsigned long long *keys,
                           hipStream_t s) {
@@ -1175,10 +1218,10 @@ void launch_columns(const ColumnArgs &a, hipStream_t s) {
   dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
   k_columns<<<grid, 256, 0, s>>>(a);
 }
-static const int kShapeHost[4][2] = {{4, 4}, {8, 4}, {8, 8}, {16, 4}};
+static const int kShapeHost[kNumShapes][3] = {{4, 4, 1}, {8, 4, 1}, {8, 8, 1}, {16, 4, 1}, {4, 4, 2}};
 
 void tile_dims(const GridArgs &g, int shape, uint32_t d[3]) {
-  const int sx = kShapeHost[shape][0], sy = kShapeHost[shape][1], sz = 64 / (sx * sy);
+  const int sx = kShapeHost[shape][0], sy = kShapeHost[shape][1], sz = 64 * kShapeHost[shape][2] / (sx * sy);
   d[0] = nb(g.nx, 2 * sx);
   d[1] = nb(g.ny, 2 * sy);
   d[2] = nb(g.nz, sz);
@@ -1188,8 +1231,9 @@ void launch_tile_keys(const ColumnArgs &a, int shape, uint32_t *keys, int32_t *i
   uint32_t d[3];
   tile_dims(a.g, shape, d);
   const uint32_t n = d[0] * d[1] * d[2];
-  k_tile_keys<<<nb(n), 256, 0, s>>>(a, shape, d[0], d[1], n, keys, ids);
+  k_tile_keys<<<nb(n), 256, 0, s>>>(a, shape, n, keys, ids);
 }
+
 
 template <bool COUNT, bool SLIVERS, bool CAPS>
 static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, const uint32_t *order,
@@ -1197,7 +1241,10 @@ static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, co
   uint32_t d[3];
   tile_dims(a.col.g, shape, d);
   if (variant == 1 && !deep) {
-    k_sdf_wave<COUNT, SLIVERS, CAPS><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    if (kShapeHost[shape][2] == 2)
+      k_sdf_wave<COUNT, SLIVERS, CAPS, 2><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    else
+      k_sdf_wave<COUNT, SLIVERS, CAPS, 1><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
     return;
   }
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
@@ -1226,7 +1273,8 @@ void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, 
 
 template <bool COUNT, bool SLIVERS, bool CAPS>
 static void launch_batch_t(const BatchArgs &t, uint32_t blocks, hipStream_t s) {
-  k_sdf_batch<COUNT, SLIVERS, CAPS><<<blocks, 256, 0, s>>>(t);
+  if (kShapeHost[t.shape][2] == 2) k_sdf_batch<COUNT, SLIVERS, CAPS, 2><<<blocks, 256, 0, s>>>(t);
+  else k_sdf_batch<COUNT, SLIVERS, CAPS, 1><<<blocks, 256, 0, s>>>(t);
 }
 
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s) {

@@ -88,6 +88,7 @@ struct BatchArgs {
   uint32_t njobs;
   uint32_t *head;               // work-queue counter (zeroed before the launch)
   int shape;                    // brick shape index
+  const uint32_t *order;        // queue position -> global tile (null: identity, the batch order)
 };
 
 // block-tiles of a grid for a brick shape (x, y, z)

@@ -361,7 +361,7 @@ static inline double dot3d(const double a[3], const double b[3]) { return a[0] *
  *   m_i = cross(n, e_i) (in-plane normal of edge i, pointing into the triangle);
  *   cross(a, b)_x = fmaf(a_y, b_z, -(a_z * b_y)) (cyclic), dot(a, b) = fmaf(a_z, b_z, fmaf(a_y, b_y, a_x * b_x)).
  * Query, with a_i = p_i - q: segment i t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = fmaf(t, e_i, a_i),
- * d_i = (r_x^2 + r_y^2) + r_z^2; face d_f = (s * s) * inv_nn with s = dot(n, a_0), used when
+ * d_i = fmaf(r_z, r_z, fmaf(r_y, r_y, r_x * r_x)); face d_f = (s * s) * inv_nn with s = dot(n, a_0), used when
  * max_i dot(m_i, a_i) <= 0; result min(d_0, d_1, d_2[, d_f]). */
 static inline float fdot3(const float a[3], const float b[3]) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
 static inline void fcross3(const float a[3], const float b[3], float o[3]) {
@@ -400,7 +400,7 @@ float orc_trirec_dist2(const float q[3], const orc_trirec *r) {
   for (int i = 0; i < 3; ++i) {
     const float u = fminf(fmaxf(fdot3(a[i], r->e[i]) * r->nie[i], 0.0f), 1.0f);
     const float x = fmaf(u, r->e[i][0], a[i][0]), y = fmaf(u, r->e[i][1], a[i][1]), z = fmaf(u, r->e[i][2], a[i][2]);
-    d = fminf(d, (x * x + y * y) + z * z);
+    d = fminf(d, fmaf(z, z, fmaf(y, y, x * x)));
   }
   const float k0 = fdot3(r->m[0], a[0]), k1 = fdot3(r->m[1], a[1]), k2 = fdot3(r->m[2], a[2]);
   const float s = fdot3(r->n, a[0]);
@@ -773,15 +773,11 @@ void orc_bvh_free(orc_bvh *b) {
 /* ============================================================================
  * dense CPU bake
  * ========================================================================== */
+/* squared point/box distance in the triangle distance's form fmaf(z, z, fmaf(y, y, x * x)) */
 static inline float box_dist2(const float q[3], const float *lo, const float *hi) {
-  float s = 0.0f;
-  for (int k = 0; k < 3; ++k) {
-    float d = 0.0f;
-    if (q[k] < lo[k]) d = lo[k] - q[k];
-    else if (q[k] > hi[k]) d = q[k] - hi[k];
-    s += d * d;
-  }
-  return s;
+  float d[3];
+  for (int k = 0; k < 3; ++k) d[k] = fmaxf(fmaxf(lo[k] - q[k], q[k] - hi[k]), 0.0f);
+  return fmaf(d[2], d[2], fmaf(d[1], d[1], d[0] * d[0]));
 }
 
 typedef struct {
