@@ -129,6 +129,9 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
   const int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
   nodes[i].c0 = lc;
   nodes[i].c1 = rc;
+  // 1: the node's leaf range [lo, hi] lies within one 256-leaf block, so the refit threads that
+  // meet at this node are all in one workgroup
+  nodes[i].pad = (lo >> 8) == (hi >> 8) ? 1 : 0;
   if (lc >= 0) nodes[lc].parent = i; else leaf_parent[~lc] = i;
   if (rc >= 0) nodes[rc].parent = i; else leaf_parent[~rc] = i;
   if (i == 0) nodes[0].parent = -1;
@@ -157,7 +160,36 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, c
 }
 
 // bottom-up refit: each leaf writes its box into its parent's child slot; the second thread to
-// reach a node unions the two child boxes and continues upwards.
+// reach a node unions the two child boxes and continues upwards.  Thread i handles sorted leaf i,
+// so a node whose leaf range lies within one 256-leaf block is only met by threads of one
+// workgroup: its exchange uses workgroup-scope atomics and fences; the others (about one node per
+// block boundary and level) use agent scope.
+template <int SCOPE>
+__device__ __forceinline__ bool refit_step(Node *nd, int child, float lo[3], float hi[3], uint32_t *visit) {
+  const bool left = (nd->c0 == child);
+  float *dlo = left ? nd->lo0 : nd->lo1;
+  float *dhi = left ? nd->hi0 : nd->hi1;
+  for (int k = 0; k < 3; ++k) {
+    __hip_atomic_store(&dlo[k], lo[k], __ATOMIC_RELAXED, SCOPE);
+    __hip_atomic_store(&dhi[k], hi[k], __ATOMIC_RELAXED, SCOPE);
+  }
+  if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  const uint32_t old = __hip_atomic_fetch_add(visit, 1u, __ATOMIC_RELAXED, SCOPE);
+  if (old == 0) return false;  // the sibling is still on its way; it finishes this node
+  if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  for (int k = 0; k < 3; ++k) {
+    const float l0 = __hip_atomic_load(&nd->lo0[k], __ATOMIC_RELAXED, SCOPE);
+    const float l1 = __hip_atomic_load(&nd->lo1[k], __ATOMIC_RELAXED, SCOPE);
+    const float h0 = __hip_atomic_load(&nd->hi0[k], __ATOMIC_RELAXED, SCOPE);
+    const float h1 = __hip_atomic_load(&nd->hi1[k], __ATOMIC_RELAXED, SCOPE);
+    lo[k] = smin(l0, l1);
+    hi[k] = smax(h0, h1);
+  }
+  return true;
+}
+
 __global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, int n, Node *nodes,
                                                const int32_t *__restrict__ leaf_parent, uint32_t *visits,
                                                float *root_box) {
@@ -170,25 +202,9 @@ __global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, 
   int p = leaf_parent[i];
   for (int guard = 0; p >= 0 && guard < kMaxDepth; ++guard) {
     Node *nd = nodes + p;
-    const bool left = (nd->c0 == child);
-    float *dlo = left ? nd->lo0 : nd->lo1;
-    float *dhi = left ? nd->hi0 : nd->hi1;
-    for (int k = 0; k < 3; ++k) {
-      __hip_atomic_store(&dlo[k], lo[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&dhi[k], hi[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    __threadfence();
-    const uint32_t old = atomicAdd(&visits[p], 1u);
-    if (old == 0) return;  // the sibling is still on its way; it finishes this node
-    __threadfence();
-    for (int k = 0; k < 3; ++k) {
-      const float l0 = __hip_atomic_load(&nd->lo0[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const float l1 = __hip_atomic_load(&nd->lo1[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const float h0 = __hip_atomic_load(&nd->hi0[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      const float h1 = __hip_atomic_load(&nd->hi1[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      lo[k] = smin(l0, l1);
-      hi[k] = smax(h0, h1);
-    }
+    const bool cont = nd->pad ? refit_step<__HIP_MEMORY_SCOPE_WORKGROUP>(nd, child, lo, hi, visits + p)
+                              : refit_step<__HIP_MEMORY_SCOPE_AGENT>(nd, child, lo, hi, visits + p);
+    if (!cont) return;
     child = p;
     p = nd->parent;
   }
