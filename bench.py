@@ -240,13 +240,11 @@ def main():
     roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
 
     if args.copy_calibration_mib > 0:
-        import torch
-
-        src = torch.ones(args.copy_calibration_mib << 18, dtype=torch.float32, device="cuda:0")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+        nbytes = args.copy_calibration_mib << 20
+        src, dst = ctx.alloc(nbytes), ctx.alloc(nbytes)
+        ctx.copy_device(dst, src, nbytes)
+        ctx.free(src)
+        ctx.free(dst)
 
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None

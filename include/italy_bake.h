@@ -80,8 +80,9 @@ typedef enum {
   IB_OPT_BRICK_SHAPE = 3,   /* voxels of one wave: 0 = 4x4x4 (default), 1 = 8x4x2, 2 = 8x8x1, 3 = 16x4x1,
                                4 = 4x4x8 (two voxels per lane) */
   IB_OPT_BLOCKS_PER_CU = 4, /* persistent blocks per compute unit in batched bakes (default 8) */
-  IB_OPT_TILE_ORDER = 5     /* 1 (default): dense per-brick queries launch their tiles costliest first
-                               (estimated from the column crossings); 0: grid order.  Speed only. */
+  IB_OPT_TILE_ORDER = 5     /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
+                               first (estimated from the column crossings), 2 (default) half-octave cost
+                               buckets, costliest first, Morton order inside a bucket.  Speed only. */
 } ib_option;
 
 /* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy
@@ -221,6 +222,8 @@ ib_status ib_device_alloc(ib_context *ctx, uint64_t bytes, void **d_ptr);
 ib_status ib_device_free(ib_context *ctx, void *d_ptr);
 ib_status ib_memcpy_h2d(ib_context *ctx, void *d_dst, const void *src, uint64_t bytes);
 ib_status ib_memcpy_d2h(ib_context *ctx, void *dst, const void *d_src, uint64_t bytes);
+/* device-to-device copy on the context's stream (synchronous) */
+ib_status ib_memcpy_d2d(ib_context *ctx, void *d_dst, const void *d_src, uint64_t bytes);
 ib_status ib_synchronize(ib_context *ctx);
 ib_status ib_last_stats(ib_context *ctx, ib_stats *out);
 ib_status ib_set_option(ib_context *ctx, int32_t option, int64_t value);

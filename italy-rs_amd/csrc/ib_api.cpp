@@ -62,7 +62,7 @@ struct ib_context {
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
-  bool tile_order = true;      // dense per-brick queries launch their costliest tiles first
+  int tile_order = 2;          // dense per-brick queries: 0 grid order, 1 costliest first, 2 cost buckets + Morton
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
   uint32_t batch_sc_cap = 0;
@@ -518,7 +518,7 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_TRY(c->scratch("tile_keys", nt, &keys));
     IB_TRY(c->scratch("tile_ids_in", nt, &ids_in));
     IB_TRY(c->scratch("tile_ids", nt, &ids));
-    launch_tile_keys(sa.col, c->brick_shape, keys_in, ids_in, s);
+    launch_tile_keys(sa.col, c->brick_shape, c->tile_order, keys_in, ids_in, s);
     size_t temp_bytes = 0;
     IB_HIP(sort_pairs_u32(nullptr, temp_bytes, keys_in, keys, ids_in, ids, nt, 32, s));
     char *temp;
@@ -853,6 +853,14 @@ ib_status ib_memcpy_d2h(ib_context *c, void *dst, const void *src, uint64_t byte
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
   IB_HIP(hipSetDevice(c->device));
   IB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return IB_OK;
+}
+
+ib_status ib_memcpy_d2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_HIP(hipSetDevice(c->device));
+  IB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  IB_HIP(hipStreamSynchronize(c->stream));
   return IB_OK;
 }
 
@@ -1205,7 +1213,8 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
       c->blocks_per_cu = (int)value;
       return IB_OK;
     case IB_OPT_TILE_ORDER:
-      c->tile_order = value != 0;
+      if (value < 0 || value > 2) return fail(IB_ERR_INVALID_ARGUMENT, "tile order must be 0..2");
+      c->tile_order = (int)value;
       return IB_OK;
     default:
       return fail(IB_ERR_INVALID_ARGUMENT, "unknown option " + std::to_string(option));

@@ -1057,8 +1057,9 @@ __global__ __launch_bounds__(256, 8) void k_sdf_wave(SdfArgs a, int shape, const
 // of the tile's centre column.  u = vertical distance from the tile centre to the nearest crossing
 // (an upper bound of the distance there); voxels inside the surface see it in every direction and
 // cost most, so their key is 4u.  A column that misses the mesh gets half the grid height, an
-// overflowed column +inf.  keys = ~bits(key) so an ascending sort puts the costliest tile first.
-__device__ __forceinline__ uint32_t tile_key(const ColumnArgs &a, int shape, uint32_t t) {
+// overflowed column +inf.  mode 1: sort key ~bits(key), so an ascending sort puts the costliest
+// tile first; mode 2: (31 - half-octave bucket of key / h) << 27 | Morton code of the tile.
+__device__ __forceinline__ uint32_t tile_key(const ColumnArgs &a, int shape, uint32_t t, int mode) {
   const GridArgs &g = a.g;
   const uint32_t tx_n = (uint32_t)(g.nx + 2 * kShapes[shape][0] - 1) / (2 * kShapes[shape][0]);
   const uint32_t ty_n = (uint32_t)(g.ny + 2 * kShapes[shape][1] - 1) / (2 * kShapes[shape][1]);
@@ -1083,14 +1084,21 @@ __device__ __forceinline__ uint32_t tile_key(const ColumnArgs &a, int shape, uin
     }
     key = w != 0 ? 4.0f * u : u;
   }
-  return ~__float_as_uint(key);
+  if (mode == 1) return ~__float_as_uint(key);
+  // mode 2: half-octave cost buckets (costliest first), Morton order of the tile inside a bucket so
+  // that blocks launched together work on neighbouring tiles and share cached tree records
+  const float c = key / g.h;
+  const int bucket = !(c < 1e30f) ? 31 : (c <= 1.0f ? 0 : min(31, (int)(2.0f * __log2f(c)) + 1));
+  const uint32_t tx = t % tx_n, ty = (t / tx_n) % ty_n, tz = t / (tx_n * ty_n);
+  const uint32_t m = expand10(tx & 511u) | (expand10(ty & 511u) << 1) | (expand10(tz & 511u) << 2);
+  return ((uint32_t)(31 - bucket) << 27) | m;
 }
 
-__global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint32_t ntiles, uint32_t *keys,
+__global__ __launch_bounds__(256) void k_tile_keys(ColumnArgs a, int shape, uint32_t ntiles, int mode, uint32_t *keys,
                                                    int32_t *ids) {
   const uint32_t t = blockIdx.x * 256 + threadIdx.x;
   if (t >= ntiles) return;
-  keys[t] = tile_key(a, shape, t);
+  keys[t] = tile_key(a, shape, t, mode);
   ids[t] = (int32_t)t;
 }
 
@@ -1243,11 +1251,11 @@ void tile_dims(const GridArgs &g, int shape, uint32_t d[3]) {
   d[2] = nb(g.nz, sz);
 }
 
-void launch_tile_keys(const ColumnArgs &a, int shape, uint32_t *keys, int32_t *ids, hipStream_t s) {
+void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, int32_t *ids, hipStream_t s) {
   uint32_t d[3];
   tile_dims(a.g, shape, d);
   const uint32_t n = d[0] * d[1] * d[2];
-  k_tile_keys<<<nb(n), 256, 0, s>>>(a, shape, n, keys, ids);
+  k_tile_keys<<<nb(n), 256, 0, s>>>(a, shape, n, mode, keys, ids);
 }
 
 

@@ -98,7 +98,7 @@ void tile_dims(const GridArgs &g, int shape, uint32_t d[3]);
 void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, const uint32_t *order,
                 hipStream_t s);
 // launch-order keys (ascending = costliest first) and tile ids of the per-brick query's block-tiles
-void launch_tile_keys(const ColumnArgs &a, int shape, uint32_t *keys, int32_t *ids, hipStream_t s);
+void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, int32_t *ids, hipStream_t s);
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s);
 // narrow band: bricks whose centre distance (from the coarse pass) is <= thresh

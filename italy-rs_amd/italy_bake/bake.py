@@ -167,6 +167,9 @@ class Context:
         check(self.lib.ib_memcpy_d2h(self._h, _ptr(arr), ctypes.c_void_p(d_ptr), arr.nbytes))
         return arr
 
+    def copy_device(self, d_dst, d_src, nbytes):
+        check(self.lib.ib_memcpy_d2d(self._h, ctypes.c_void_p(d_dst), ctypes.c_void_p(d_src), int(nbytes)))
+
     def synchronize(self):
         check(self.lib.ib_synchronize(self._h))
 
