@@ -238,6 +238,14 @@ def main():
                         box_tests=counters["box_tests"], tri_tests=counters["tri_tests"])
     alg_bytes = 4 * nvox + mesh.xyz.nbytes
     roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
+    # HBM-side bytes per launch, from the committed FETCH_SIZE / WRITE_SIZE profile of this kernel
+    # and workload (counters scaled by a known-size copy kernel); null when the profile is absent
+    prof = os.path.join(ROOT, "profiles", "k_sdf_wave_plain_327k_256_traffic.json")
+    if os.path.exists(prof) and args.variant in (None, 1):
+        with open(prof) as f:
+            tr = json.load(f)
+        roofline["traffic"] = tr["traffic_bytes"]
+        roofline["traffic_source"] = "profiles/" + tr["source"]
 
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
