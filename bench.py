@@ -268,6 +268,7 @@ def main():
                                       threads=cores, zstep=args.cpu_zstep)
         cpu = {"value": cst["voxels"] / cst["seconds"], "unit": "voxel-queries/s", "cores": cst["threads"],
                "kind": "port",
+               "gpu_multiple": value / (cst["voxels"] / cst["seconds"]),
                "sample": f"every {args.cpu_zstep}th z-slab of the same 256^3 bake ({cst['voxels']} voxels, exact "
                          f"closest triangle + crossing sign, LBVH build included), {cst['seconds']:.2f} s"}
 

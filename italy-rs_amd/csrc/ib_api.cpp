@@ -154,15 +154,26 @@ GridArgs to_args(const ib_grid &g) { return GridArgs{g.origin[0], g.origin[1], g
 
 // Hole caps: the unmatched directed edges form closed chains (every vertex has as many outgoing as
 // incoming unmatched edges); each chain is closed by a fan from the centroid of its vertices.  Then
-// (mesh + reversed caps) is a closed surface, and w_mesh = w_closed + w_caps.
-void build_caps(const std::vector<int2> &edges, const std::vector<float> &soup, std::vector<float> &caps) {
+// (mesh + reversed caps) is a closed surface, and w_mesh = w_closed + w_caps.  `edges` is sorted by
+// (from, to); pts holds the two endpoint positions of each edge (6 floats).  At a chain vertex the
+// walk takes the last unused edge leaving it.
+void build_caps(const std::vector<int2> &edges, const std::vector<float> &pts, std::vector<float> &caps) {
   caps.clear();
-  std::unordered_map<int, std::vector<int>> out;  // vertex -> indices of edges starting there
-  out.reserve(edges.size() * 2);
-  for (size_t i = 0; i < edges.size(); ++i) out[edges[i].x].push_back((int)i);
-  std::vector<char> used(edges.size(), 0);
+  const size_t ne = edges.size();
+  std::vector<int> cursor(ne, -1);  // at the first edge of each 'from' run: one past its last unused edge
+  for (size_t i = 0; i < ne; ++i)
+    if (i == 0 || edges[i].x != edges[i - 1].x) {
+      size_t j = i;
+      while (j < ne && edges[j].x == edges[i].x) ++j;
+      cursor[i] = (int)j;
+    }
+  auto run_start = [&](int v) -> int {
+    const auto it = std::lower_bound(edges.begin(), edges.end(), v, [](const int2 &e, int x) { return e.x < x; });
+    return (it != edges.end() && it->x == v) ? (int)(it - edges.begin()) : -1;
+  };
+  std::vector<char> used(ne, 0);
   std::vector<int> loop;
-  for (size_t s = 0; s < edges.size(); ++s) {
+  for (size_t s = 0; s < ne; ++s) {
     if (used[s]) continue;
     loop.clear();
     int e = (int)s;
@@ -172,14 +183,16 @@ void build_caps(const std::vector<int2> &edges, const std::vector<float> &soup, 
       loop.push_back(e);
       const int v = edges[e].y;
       if (v == start) break;
-      auto &lst = out[v];
+      const int lo = run_start(v);
       int next = -1;
-      while (!lst.empty()) {
-        const int c = lst.back();
-        lst.pop_back();
-        if (!used[c]) {
-          next = c;
-          break;
+      if (lo >= 0) {
+        int &cur = cursor[lo];
+        while (cur > lo) {
+          const int c = --cur;
+          if (!used[c]) {
+            next = c;
+            break;
+          }
         }
       }
       if (next < 0) break;  // cannot happen for a boundary cycle; close the fan anyway
@@ -187,52 +200,49 @@ void build_caps(const std::vector<int2> &edges, const std::vector<float> &soup, 
     }
     double cx = 0, cy = 0, cz = 0;
     for (int id : loop) {
-      const int v = edges[id].x;
-      cx += soup[3 * (size_t)v];
-      cy += soup[3 * (size_t)v + 1];
-      cz += soup[3 * (size_t)v + 2];
+      cx += pts[6 * (size_t)id];
+      cy += pts[6 * (size_t)id + 1];
+      cz += pts[6 * (size_t)id + 2];
     }
     const float apex[3] = {(float)(cx / loop.size()), (float)(cy / loop.size()), (float)(cz / loop.size())};
     for (int id : loop) {
-      const int a = edges[id].x, b = edges[id].y;
-      caps.insert(caps.end(), {apex[0], apex[1], apex[2], soup[3 * (size_t)a], soup[3 * (size_t)a + 1],
-                               soup[3 * (size_t)a + 2], soup[3 * (size_t)b], soup[3 * (size_t)b + 1],
-                               soup[3 * (size_t)b + 2]});
+      const float *p = &pts[6 * (size_t)id];
+      caps.insert(caps.end(), {apex[0], apex[1], apex[2], p[0], p[1], p[2], p[3], p[4], p[5]});
     }
   }
 }
 
 // Hierarchy over the cap triangles: median splits on the longest centroid axis, leaves of <= 4
 // triangles.  Node bounds are computed in double and rounded outwards so that the device's FP32
-// bound area / dmin^2 stays an upper bound.
-int cap_node(std::vector<float> &tris, std::vector<int> &ids, int lo, int hi, std::vector<CapNode> &nodes,
-             std::vector<float> &out) {
+// bound area / dmin^2 stays an upper bound.  Per-triangle sums, areas and area vectors are
+// computed once (CapTri) and aggregated per node.
+struct CapTri {
+  double sum[3];  // x + y + z of the three vertices, per axis (centroid * 3)
+  double lo[3], hi[3];
+  double n[3];    // area vector
+  double area;
+};
+
+int cap_node(const std::vector<float> &tris, const std::vector<CapTri> &ct, std::vector<int> &ids, int lo, int hi,
+             std::vector<CapNode> &nodes, std::vector<float> &out) {
   const int me = (int)nodes.size();
   nodes.emplace_back();
   double cmin[3] = {1e300, 1e300, 1e300}, cmax[3] = {-1e300, -1e300, -1e300};
   double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
   double area = 0, dip[3] = {0, 0, 0}, ac[3] = {0, 0, 0};
   for (int i = lo; i < hi; ++i) {
-    const float *t = &tris[9 * (size_t)ids[i]];
-    double e1[3], e2[3];
+    const CapTri &t = ct[ids[i]];
     for (int k = 0; k < 3; ++k) {
-      const double cen = ((double)t[k] + t[3 + k] + t[6 + k]) / 3.0;
+      const double cen = t.sum[k] / 3.0;
       cmin[k] = std::min(cmin[k], cen);
       cmax[k] = std::max(cmax[k], cen);
-      for (int v = 0; v < 3; ++v) {
-        bl[k] = std::min(bl[k], (double)t[3 * v + k]);
-        bh[k] = std::max(bh[k], (double)t[3 * v + k]);
-      }
-      e1[k] = (double)t[3 + k] - t[k];
-      e2[k] = (double)t[6 + k] - t[k];
+      bl[k] = std::min(bl[k], t.lo[k]);
+      bh[k] = std::max(bh[k], t.hi[k]);
     }
-    const double n[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
-                         0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
-    const double a = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-    area += a;
+    area += t.area;
     for (int k = 0; k < 3; ++k) {
-      dip[k] += n[k];
-      ac[k] += a * ((double)t[k] + t[3 + k] + t[6 + k]) / 3.0;
+      dip[k] += t.n[k];
+      ac[k] += t.area * t.sum[k] / 3.0;
     }
   }
   double c[3];
@@ -242,9 +252,10 @@ int cap_node(std::vector<float> &tris, std::vector<int> &ids, int lo, int hi, st
     const float *t = &tris[9 * (size_t)ids[i]];
     for (int v = 0; v < 3; ++v) {
       const double dx = t[3 * v] - c[0], dy = t[3 * v + 1] - c[1], dz = t[3 * v + 2] - c[2];
-      r = std::max(r, std::sqrt(dx * dx + dy * dy + dz * dz));
+      r = std::max(r, dx * dx + dy * dy + dz * dz);
     }
   }
+  r = std::sqrt(r);
   CapNode nd{};
   for (int k = 0; k < 3; ++k) {
     nd.c[k] = (float)c[k];
@@ -266,28 +277,43 @@ int cap_node(std::vector<float> &tris, std::vector<int> &ids, int lo, int hi, st
   for (int k = 1; k < 3; ++k)
     if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
   const int mid = (lo + hi) / 2;
-  auto key = [&](int id) {
-    const float *t = &tris[9 * (size_t)id];
-    return ((double)t[axis] + t[3 + axis] + t[6 + axis]);
-  };
-  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi,
-                   [&](int a, int b) { return key(a) != key(b) ? key(a) < key(b) : a < b; });
-  const int l = cap_node(tris, ids, lo, mid, nodes, out);
-  const int rr = cap_node(tris, ids, mid, hi, nodes, out);
+  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int a, int b) {
+    const double ka = ct[a].sum[axis], kb = ct[b].sum[axis];
+    return ka != kb ? ka < kb : a < b;
+  });
+  const int l = cap_node(tris, ct, ids, lo, mid, nodes, out);
+  const int rr = cap_node(tris, ct, ids, mid, hi, nodes, out);
   nd.left = l;
   nd.right = rr;
   nodes[me] = nd;
   return me;
 }
 
-void build_cap_hierarchy(std::vector<float> &tris, std::vector<CapNode> &nodes, std::vector<float> &ordered) {
+void build_cap_hierarchy(const std::vector<float> &tris, std::vector<CapNode> &nodes, std::vector<float> &ordered) {
   nodes.clear();
   ordered.clear();
   const int n = (int)(tris.size() / 9);
   if (n == 0) return;
+  std::vector<CapTri> ct(n);
+  for (int i = 0; i < n; ++i) {
+    const float *t = &tris[9 * (size_t)i];
+    CapTri &c = ct[i];
+    double e1[3], e2[3];
+    for (int k = 0; k < 3; ++k) {
+      c.sum[k] = (double)t[k] + t[3 + k] + t[6 + k];
+      c.lo[k] = std::min((double)t[k], std::min((double)t[3 + k], (double)t[6 + k]));
+      c.hi[k] = std::max((double)t[k], std::max((double)t[3 + k], (double)t[6 + k]));
+      e1[k] = (double)t[3 + k] - t[k];
+      e2[k] = (double)t[6 + k] - t[k];
+    }
+    c.n[0] = 0.5 * (e1[1] * e2[2] - e1[2] * e2[1]);
+    c.n[1] = 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]);
+    c.n[2] = 0.5 * (e1[0] * e2[1] - e1[1] * e2[0]);
+    c.area = std::sqrt(c.n[0] * c.n[0] + c.n[1] * c.n[1] + c.n[2] * c.n[2]);
+  }
   std::vector<int> ids(n);
   for (int i = 0; i < n; ++i) ids[i] = i;
-  cap_node(tris, ids, 0, n, nodes, ordered);
+  cap_node(tris, ct, ids, 0, n, nodes, ordered);
 }
 
 float ev_ms(ib_context *c, EventId a, EventId b) {
@@ -444,14 +470,29 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   } else {
     IB_TRY(c->scratch("e_bnd", 3ull * n, &bnd));
   }
-  std::vector<int2> edges(std::min<uint32_t>(h.boundary, 3u * n));
-  IB_HIP(hipMemcpyAsync(edges.data(), bnd, edges.size() * sizeof(int2), hipMemcpyDeviceToHost, s));
-  std::vector<float> soup(9ull * n);
-  IB_HIP(hipMemcpyAsync(soup.data(), d_xyz, soup.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+  // only the boundary edges and their endpoint positions come to the host
+  const uint32_t ne = std::min<uint32_t>(h.boundary, 3u * n);
+  float *d_pts;
+  IB_TRY(c->scratch("e_pts", 6ull * ne, &d_pts));
+  launch_edge_points(bnd, ne, d_xyz, d_pts, s);
+  std::vector<int2> raw(ne);
+  std::vector<float> raw_pts(6ull * ne);
+  IB_HIP(hipMemcpyAsync(raw.data(), bnd, raw.size() * sizeof(int2), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipMemcpyAsync(raw_pts.data(), d_pts, raw_pts.size() * sizeof(float), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
-  std::sort(edges.begin(), edges.end(), [](int2 a, int2 b) { return a.x != b.x ? a.x < b.x : a.y < b.y; });
+  std::vector<int> perm(ne);
+  for (uint32_t i = 0; i < ne; ++i) perm[i] = (int)i;
+  std::sort(perm.begin(), perm.end(), [&](int a, int b) {
+    return raw[a].x != raw[b].x ? raw[a].x < raw[b].x : raw[a].y < raw[b].y;
+  });
+  std::vector<int2> edges(ne);
+  std::vector<float> pts(6ull * ne);
+  for (uint32_t i = 0; i < ne; ++i) {
+    edges[i] = raw[perm[i]];
+    std::copy(&raw_pts[6 * (size_t)perm[i]], &raw_pts[6 * (size_t)perm[i]] + 6, &pts[6 * (size_t)i]);
+  }
   std::vector<float> caps, ordered;
-  build_caps(edges, soup, caps);
+  build_caps(edges, pts, caps);
   std::vector<CapNode> nodes;
   build_cap_hierarchy(caps, nodes, ordered);
   m.ncap = (int)(ordered.size() / 9);

@@ -631,24 +631,39 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
   bound = ub * 1.0001f + 1e-5f;  // + FP32 rounding of the bound and the sum
 }
 
-__device__ __noinline__ double cap_winding_f64(F3 q, const float *caps, int ncap) {
+__device__ __forceinline__ double solid_angle_f64(F3 q, const float *t) {
+  const double ax = (double)t[0] - q.x, ay = (double)t[1] - q.y, az = (double)t[2] - q.z;
+  const double bx = (double)t[3] - q.x, by = (double)t[4] - q.y, bz = (double)t[5] - q.z;
+  const double cx = (double)t[6] - q.x, cy = (double)t[7] - q.y, cz = (double)t[8] - q.z;
+  const double la = sqrt(dot3d(ax, ay, az, ax, ay, az)), lb = sqrt(dot3d(bx, by, bz, bx, by, bz)),
+               lc = sqrt(dot3d(cx, cy, cz, cx, cy, cz));
+  const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
+  const double den = la * lb * lc + dot3d(ax, ay, az, bx, by, bz) * lc + dot3d(bx, by, bz, cx, cy, cz) * la +
+                     dot3d(cx, cy, cz, ax, ay, az) * lb;
+  return 2.0 * atan2(det, den);
+}
+
+// FP64 winding number of the caps at q, computed by a whole wave: cap i goes to lane i % 64, each
+// lane keeps a Neumaier-compensated sum, and the 64 partial sums are combined by a butterfly of
+// error-free two-sums.  Every lane returns lane 0's total, so the value is wave-uniform and the
+// same from run to run.
+__device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int ncap) {
+  const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0;
-  for (int i = 0; i < ncap; ++i) {
-    const float *t = caps + 9 * i;
-    const double ax = (double)t[0] - q.x, ay = (double)t[1] - q.y, az = (double)t[2] - q.z;
-    const double bx = (double)t[3] - q.x, by = (double)t[4] - q.y, bz = (double)t[5] - q.z;
-    const double cx = (double)t[6] - q.x, cy = (double)t[7] - q.y, cz = (double)t[8] - q.z;
-    const double la = sqrt(dot3d(ax, ay, az, ax, ay, az)), lb = sqrt(dot3d(bx, by, bz, bx, by, bz)),
-                 lc = sqrt(dot3d(cx, cy, cz, cx, cy, cz));
-    const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
-    const double den = la * lb * lc + dot3d(ax, ay, az, bx, by, bz) * lc + dot3d(bx, by, bz, cx, cy, cz) * la +
-                       dot3d(cx, cy, cz, ax, ay, az) * lb;
-    const double om = 2.0 * atan2(det, den);
+  for (int i = lane; i < ncap; i += 64) {
+    const double om = solid_angle_f64(q, caps + 9 * i);
     const double s = sum + om;
     comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
     sum = s;
   }
-  return (sum + comp) * (1.0 / (4.0 * M_PI));
+  double v = sum, e = comp;
+  for (int off = 32; off >= 1; off >>= 1) {
+    const double ov = __shfl_xor(v, off), oe = __shfl_xor(e, off);
+    const double s = v + ov, bp = s - v;
+    e = e + oe + ((v - (s - bp)) + (ov - bp));
+    v = s;
+  }
+  return __shfl(v + e, 0) * (1.0 / (4.0 * M_PI));
 }
 
 // conservative ray / box slab test for the parity tie-break (boxes inflated by a relative margin)
@@ -863,12 +878,14 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
   }
 }
 
-// FP64 recheck / parity / overflow recount for the listed voxels (grid-stride over the count the
-// query kernel left in the scalars)
-__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t block, uint32_t nblocks) {
+// FP64 recheck / parity / overflow recount for the listed voxels: one wave per voxel (grid-stride
+// over the count the query kernel left in the scalars).  The crossing count and the tie-break are
+// wave-uniform; the FP64 cap sum is split over the lanes; lane 0 writes.
+__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
   const GridArgs &g = a.col.g;
-  for (uint32_t i = block * 256 + threadIdx.x; i < n; i += nblocks * 256) {
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = wave; i < n; i += nwaves) {
     const uint32_t vox = a.amb_list[i];
     const int x = (int)(vox % (uint32_t)g.nx);
     const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
@@ -888,22 +905,26 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t block, 
     if (a.col.ncap == 0) {
       inside = wi != 0;
     } else {
-      atomicAdd(&a.col.sc->rechecked, 1u);
-      const double w64 = (double)wi + cap_winding_f64(q, a.col.caps, a.col.ncap);
+      if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
+      const double w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap);
       if (fabs(fabs(w64) - 0.5) >= kTieEps) {
         inside = fabs(w64) > 0.5;
       } else {
-        atomicAdd(&a.col.sc->ties, 1u);
+        if (lane == 0) atomicAdd(&a.col.sc->ties, 1u);
         inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q);
       }
     }
-    const size_t oi = out_index(a, x, y, z);
-    const float d = a.out[oi];
-    a.out[oi] = inside ? -d : d;
+    if (lane == 0) {
+      const size_t oi = out_index(a, x, y, z);
+      const float d = a.out[oi];
+      a.out[oi] = inside ? -d : d;
+    }
   }
 }
 
-__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) { resolve_range(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
+  resolve_range(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
 
 // --------------------------------------------------------------------------------------------
 // Wave-coherent variant: the 64 lanes of a wave (one 4x4x4 brick) walk the tree together.  A
@@ -1152,7 +1173,7 @@ __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
 
 // resolve for every job of a batch: blockIdx.y = job
 __global__ __launch_bounds__(256) void k_resolve_batch(const SdfArgs *jobs) {
-  resolve_range(jobs[blockIdx.y], blockIdx.x, gridDim.x);
+  resolve_range(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // ============================================================================================
@@ -1200,6 +1221,19 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
   k_weld_min<<<nb(nc), 256, 0, s>>>(cslot, nc, minid);
   k_edge_insert<<<nb(ntri), 256, 0, s>>>(cslot, minid, ntri, ekeys, enet, emask);
   k_edge_collect<<<nb((uint64_t)emask + 1), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
+}
+__global__ __launch_bounds__(256) void k_edge_points(const int2 *__restrict__ edges, uint32_t n,
+                                                     const float *__restrict__ xyz, float *pts) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const int2 e = edges[i];
+  for (int k = 0; k < 3; ++k) {
+    pts[6 * (size_t)i + k] = xyz[3 * (size_t)e.x + k];
+    pts[6 * (size_t)i + 3 + k] = xyz[3 * (size_t)e.y + k];
+  }
+}
+void launch_edge_points(const int2 *edges, uint32_t n, const float *xyz, float *pts, hipStream_t s) {
+  if (n) k_edge_points<<<nb(n), 256, 0, s>>>(edges, n, xyz, pts);
 }
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
   k_occupancy<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, words);
@@ -1331,7 +1365,7 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
 }
 
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s) {
-  if (njobs) k_resolve_batch<<<dim3(16, njobs), 256, 0, s>>>(jobs);
+  if (njobs) k_resolve_batch<<<dim3(128, njobs), 256, 0, s>>>(jobs);
 }
 
 }  // namespace ib

@@ -79,6 +79,8 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
                  hipStream_t s);
+// positions of both endpoints (corner indices into the soup) of each boundary edge: 6 floats
+void launch_edge_points(const int2 *edges, uint32_t n, const float *xyz, float *pts, hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
 // a batch of query jobs drained by one persistent launch

@@ -19,7 +19,7 @@ def _meshes(meshgen):
     ]
 
 
-@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("shape", [0, 1, 2, 3, 4])
 def test_batch_equals_alone(ctx, meshgen, shape):
     from italy_bake import Grid, Job, bake
 
@@ -62,13 +62,20 @@ def test_brick_shapes_agree(ctx, meshgen):
     g = bake.Grid.make((-0.61, -0.66, -0.59), 0.0137, 83, 91, 77)
     ref, _ = ctx.bake_dense(m.xyz, g, None)
     try:
-        for shape in (1, 2, 3):
+        for shape in (1, 2, 3, 4):
             ctx.set_option(bake.OPT_BRICK_SHAPE, shape)
             d, _ = ctx.bake_dense(m.xyz, g, None)
             assert np.array_equal(d.view(np.uint32), ref.view(np.uint32)), shape
+        ctx.set_option(bake.OPT_BRICK_SHAPE, 0)
+        for order in (0, 1):  # tile launch order: grid order, costliest first (2, the default, is ref)
+            ctx.set_option(bake.OPT_TILE_ORDER, order)
+            d, _ = ctx.bake_dense(m.xyz, g, None)
+            assert np.array_equal(d.view(np.uint32), ref.view(np.uint32)), order
+        ctx.set_option(bake.OPT_TILE_ORDER, 2)
         ctx.set_option(bake.OPT_QUERY_VARIANT, 0)
         d, _ = ctx.bake_dense(m.xyz, g, None)
         assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
     finally:
         ctx.set_option(bake.OPT_BRICK_SHAPE, 0)
+        ctx.set_option(bake.OPT_TILE_ORDER, 2)
         ctx.set_option(bake.OPT_QUERY_VARIANT, 1)
