@@ -507,7 +507,7 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
 
 // column crossings of one grid + the query argument block (buffers from arena or scratch)
 ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float *d_out, Scalars *sc, Arena *arena,
-                        uint32_t amb_cap, SdfArgs &sa, hipStream_t s) {
+                        uint32_t amb_cap, SdfArgs &sa, hipStream_t s, bool columns = true) {
   const uint64_t ncol = (uint64_t)g.nx * g.ny;
   ColumnArgs ca{};
   ca.nodes = m.nodes;
@@ -522,7 +522,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.sc = sc;
   IB_TRY(take(c, arena, "col_cross", ncol * kColCap, &ca.cross));
   IB_TRY(take(c, arena, "col_count", ncol, &ca.count));
-  launch_columns(ca, s);
+  if (columns) launch_columns(ca, s);
   sa = SdfArgs{};
   sa.col = ca;
   sa.out = d_out;
@@ -1100,7 +1100,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   }
   // 3. columns + per-job query blocks, tile prefix
   std::vector<SdfArgs> args;
-  std::vector<uint32_t> owner, prefix(1, 0);
+  std::vector<uint32_t> owner, prefix(1, 0), col_prefix(1, 0);
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
     if (jb.ntri == 0 || jb.sdf.nx == 0) continue;
@@ -1109,7 +1109,8 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     if (band && jb.nbricks == 0) continue;
     SdfArgs sa;
     const uint32_t amb_cap = (uint32_t)std::min<uint64_t>(nvox, nvox / 16 + 4096);
-    IB_TRY(prepare_query(c, mesh[j], jb.sdf, jb.d_distances, scs + j, &arena, amb_cap, sa, s));
+    IB_TRY(prepare_query(c, mesh[j], jb.sdf, jb.d_distances, scs + j, &arena, amb_cap, sa, s, false));
+    col_prefix.push_back(col_prefix.back() + column_tiles(sa.col.g));
     uint32_t tiles;
     if (band) {
       sa.band_index = jb.d_brick_index;
@@ -1125,7 +1126,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     prefix.push_back(prefix.back() + tiles);
     c->stats.voxels += nvox;
   }
-  c->record(EV_COLS, s);
+  if (args.empty()) c->record(EV_COLS, s);
   if (!args.empty()) {
     SdfArgs *d_args;
     uint32_t *d_prefix, *d_head;
@@ -1134,6 +1135,13 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     IB_TRY(arena.get(1, &d_head));
     IB_HIP(hipMemcpyAsync(d_args, args.data(), args.size() * sizeof(SdfArgs), hipMemcpyHostToDevice, s));
     IB_HIP(hipMemcpyAsync(d_prefix, prefix.data(), prefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    // every job's column crossings in one launch
+    uint32_t *d_col_prefix;
+    IB_TRY(arena.get(col_prefix.size(), &d_col_prefix));
+    IB_HIP(hipMemcpyAsync(d_col_prefix, col_prefix.data(), col_prefix.size() * sizeof(uint32_t),
+                          hipMemcpyHostToDevice, s));
+    launch_columns_batch(d_args, d_col_prefix, (uint32_t)args.size(), col_prefix.back(), s);
+    c->record(EV_COLS, s);
     fill_u32(d_head, 0u, 1, s);
     BatchArgs t{d_args, d_prefix, (uint32_t)args.size(), d_head, c->brick_shape, nullptr};
     const uint32_t blocks = std::min<uint32_t>(prefix.back(), (uint32_t)(c->cus * c->blocks_per_cu));

@@ -533,10 +533,7 @@ __device__ __forceinline__ void for_column_caps(const ColumnArgs &a, float X, fl
   }
 }
 
-__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= a.g.nx || y >= a.g.ny) return;
+__device__ __forceinline__ void column_crossings(const ColumnArgs &a, int x, int y) {
   const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
   const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
   const size_t col = (size_t)y * a.g.nx + x;
@@ -554,6 +551,31 @@ __global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
   });
   a.count[col] = sink.cnt;
   if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
+}
+
+__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
+  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
+  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
+  if (x >= a.g.nx || y >= a.g.ny) return;
+  column_crossings(a, x, y);
+}
+
+// the columns of every job of a batch in one launch: block b is 16 x 16 columns of the job whose
+// range of the column-tile prefix holds b
+__global__ __launch_bounds__(256) void k_columns_batch(const SdfArgs *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ prefix, uint32_t njobs) {
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = (int)njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const ColumnArgs &a = jobs[lo].col;
+  const uint32_t local = b - prefix[lo], tx = (uint32_t)(a.g.nx + 15) / 16;
+  const int x = (int)(local % tx) * 16 + (threadIdx.x & 15);
+  const int y = (int)(local / tx) * 16 + (threadIdx.x >> 4);
+  if (x >= a.g.nx || y >= a.g.ny) return;
+  column_crossings(a, x, y);
 }
 
 // winding (integer part) of the closed surface mesh + reversed caps for a voxel whose column
@@ -1271,6 +1293,11 @@ hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long 
 void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
                          hipStream_t s) {
   if (n) k_unpack_cells<<<nb(n), 256, 0, s>>>(keys, n, nx, ny, cells, tri);
+}
+uint32_t column_tiles(const GridArgs &g) { return nb(g.nx, 16) * nb(g.ny, 16); }
+void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t tiles,
+                          hipStream_t s) {
+  if (tiles) k_columns_batch<<<tiles, 256, 0, s>>>(jobs, prefix, njobs);
 }
 void launch_columns(const ColumnArgs &a, hipStream_t s) {
   dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));

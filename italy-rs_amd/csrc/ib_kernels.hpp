@@ -82,6 +82,10 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
 // positions of both endpoints (corner indices into the soup) of each boundary edge: 6 floats
 void launch_edge_points(const int2 *edges, uint32_t n, const float *xyz, float *pts, hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
+// 16 x 16 column tiles of a grid, and the columns of a whole batch in one launch (prefix: first
+// column tile of each job, njobs + 1 entries on the device)
+uint32_t column_tiles(const GridArgs &g);
+void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t tiles, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
 // a batch of query jobs drained by one persistent launch
 struct BatchArgs {
