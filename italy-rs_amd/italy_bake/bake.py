@@ -128,6 +128,21 @@ class Context:
                                          ctypes.byref(blocks), ctypes.byref(ms)))
         return shape.value, blocks.value, ms.value
 
+    def splat_materials(self, xyz, materials, grid):
+        """ib_splat_materials (bakePerCellMaterial, sdf_baker.cpp:76-128): per-triangle materials
+        (ntri, 5) = r, g, b, metallic, roughness, averaged per cell in triangle order.  Returns
+        (baseColor (ncell, 3), metallic (ncell,), roughness (ncell,))."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        ntri = xyz.size // 9
+        mat = np.ascontiguousarray(materials, np.float32).reshape(ntri, 5)
+        n = grid.nx * grid.ny * grid.nz
+        rgb = np.empty((n, 3), np.float32)
+        met = np.empty(n, np.float32)
+        rough = np.empty(n, np.float32)
+        check(self.lib.ib_splat_materials(self._h, _ptr(xyz) if ntri else None, ntri, _ptr(mat) if ntri else None,
+                                          ctypes.byref(grid), _ptr(rgb), _ptr(met), _ptr(rough)))
+        return rgb, met, rough
+
     def set_option(self, option, value):
         check(self.lib.ib_set_option(self._h, int(option), int(value)))
 
@@ -226,15 +241,22 @@ def sdf_framing(bmin, bmax, resolution):
     return g
 
 
-def bakeSdf(mesh, resolution, ctx=None):
-    """italy::bakeSdf (src/convert/sdf_baker.h:8): reference framing, exact signed distances."""
+def bakeSdf(mesh, resolution, ctx=None, materials=None):
+    """italy::bakeSdf (src/convert/sdf_baker.h:8): reference framing, exact signed distances, and the
+    per-cell materials of bakePerCellMaterial (sdf_baker.cpp:76-128).  materials: (ntri, 5) per-triangle
+    r, g, b, metallic, roughness (triangleMaterial, :61-74); None gives every triangle the default
+    material (1, 1, 1, 0, 1) of MaterialAsset (mesh_asset.h:18-21)."""
     ctx = ctx or default_context()
     g = sdf_framing(mesh.bmin, mesh.bmax, resolution)
     dist = np.empty(g.nx * g.ny * g.nz, np.float32)
     xyz = np.ascontiguousarray(mesh.xyz, np.float32)
     check(ctx.lib.ib_bake_sdf(ctx.handle, _ptr(xyz) if mesh.ntri else None, mesh.ntri, _f3(mesh.bmin),
                               _f3(mesh.bmax), int(resolution), ctypes.byref(g), _ptr(dist), dist.size))
-    return SdfGrid(g, dist)
+    grid = SdfGrid(g, dist)
+    if materials is None:
+        materials = np.tile(np.array([1, 1, 1, 0, 1], np.float32), (mesh.ntri, 1))
+    grid.baseColor, grid.metallic, grid.roughness = ctx.splat_materials(xyz, materials, g)
+    return grid
 
 
 def voxelizeMesh(mesh, resolution, ctx=None):

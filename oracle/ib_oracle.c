@@ -179,6 +179,43 @@ static void cell_center(const float origin[3], float h, int x, int y, int z, flo
 }
 
 /* ============================================================================
+ * a7: bakePerCellMaterial, sdf_baker.cpp:76-128
+ * ========================================================================== */
+/* Per triangle t in order, every cell of its clamped AABB range that triBoxOverlap accepts gets
+ * mat[5t..5t+4] (r, g, b, metallic, roughness) added (FP32, starting from 0); a touched cell ends
+ * as sum / count, an untouched one keeps (1, 1, 1, 0, 1) (:77-80, :121-127).  out: 5 planes of
+ * nx*ny*nz floats, cell (z*ny + y)*nx + x. */
+void orc_splat_materials(const float *xyz, size_t ntri, const float *mat, const float origin[3], float h, int nx,
+                         int ny, int nz, float *out) {
+  const size_t ncell = (size_t)nx * ny * nz;
+  float *sum = (float *)calloc(5 * (ncell ? ncell : 1), sizeof(float));
+  int *cnt = (int *)calloc(ncell ? ncell : 1, sizeof(int));
+  const int maxc[3] = {nx - 1, ny - 1, nz - 1};
+  const float bh[3] = {h * 0.5f, h * 0.5f, h * 0.5f};
+  for (size_t t = 0; t < ntri; ++t) {
+    const float *p = xyz + 9 * t;
+    int lo[3], hi[3];
+    cell_range(p, origin, h, maxc, lo, hi);
+    for (int z = lo[2]; z <= hi[2]; ++z)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int x = lo[0]; x <= hi[0]; ++x) {
+          float c[3];
+          cell_center(origin, h, x, y, z, c);
+          if (!orc_tri_box_overlap(c, bh, p, p + 3, p + 6)) continue;
+          const size_t i = ((size_t)z * ny + y) * nx + x;
+          for (int k = 0; k < 5; ++k) sum[(size_t)k * ncell + i] += mat[5 * t + k];
+          cnt[i] += 1;
+        }
+  }
+  const float def[5] = {1.0f, 1.0f, 1.0f, 0.0f, 1.0f};
+  for (size_t i = 0; i < ncell; ++i)
+    for (int k = 0; k < 5; ++k)
+      out[(size_t)k * ncell + i] = cnt[i] ? sum[(size_t)k * ncell + i] / (float)cnt[i] : def[k];
+  free(sum);
+  free(cnt);
+}
+
+/* ============================================================================
  * a9: voxelizeMesh (first-touch order), voxelize.cpp:52-107
  * ========================================================================== */
 typedef struct {

@@ -65,6 +65,10 @@ typedef struct {
 } orc_trirec;
 void orc_trirec_make(const float *tri9, orc_trirec *r);
 float orc_trirec_dist2(const float q[3], const orc_trirec *r);
+/* bakePerCellMaterial (sdf_baker.cpp:76-128): 5 material planes (r, g, b, metallic, roughness) of
+ * nx*ny*nz cells from per-triangle materials mat[5*ntri], summed in triangle order. */
+void orc_splat_materials(const float *xyz, size_t ntri, const float *mat, const float origin[3], float h, int nx,
+                         int ny, int nz, float *out);
 /* squared distance point->triangle: branch-free FP32 over the distance record, FP64 for slivers */
 float orc_tri_dist2_f32(const float q[3], const float *tri9);
 /* squared distance point->triangle, all FP64 (min over edges + interior projection) */

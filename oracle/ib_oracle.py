@@ -59,6 +59,9 @@ def _load():
     lib.orc_voxelize_first_touch.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, _f32p, ctypes.c_int, _i32p,
                                              _f32p, ctypes.c_size_t]
     lib.orc_voxelize_first_touch.restype = ctypes.c_size_t
+    lib.orc_splat_materials.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, _f32p]
+    lib.orc_splat_materials.restype = None
     lib.orc_occupancy.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, _u32p]
     lib.orc_tri_is_sliver.argtypes = [_f32p]
@@ -146,6 +149,16 @@ def occupancy(xyz, origin, h, nx, ny, nz):
     lib().orc_occupancy(_p(xyz, _f32p), xyz.size // 9, _p(_f32(origin), _f32p), float(h), nx, ny, nz,
                         _p(words, _u32p))
     return words
+
+
+def splat_materials(xyz, mat, origin, h, nx, ny, nz):
+    """bakePerCellMaterial (sdf_baker.cpp:76-128): (5, nz, ny, nx) planes r, g, b, metallic, roughness."""
+    xyz = _f32(xyz)
+    mat = _f32(mat)
+    out = np.zeros((5, nz, ny, nx), np.float32)
+    lib().orc_splat_materials(_p(xyz, _f32p), xyz.size // 9, _p(mat, _f32p), _p(_f32(origin), _f32p), float(h),
+                              nx, ny, nz, _p(out, _f32p))
+    return out
 
 
 def tri_dist2_f32(q, tri):

@@ -111,6 +111,33 @@ def test_occupancy_contains_every_vertex_cell(oracle, meshgen):
     assert bits[idx[ok, 2], idx[ok, 1], idx[ok, 0]].all()
 
 
+def test_splat_materials_brute_force(oracle, meshgen):
+    """bakePerCellMaterial (sdf_baker.cpp:76-128) == a replay over every (triangle, cell) pair in
+    triangle order with float32 sums; untouched cells keep (1, 1, 1, 0, 1)."""
+    m = meshgen.icosphere(1, 0.5, 0.0)
+    rng = np.random.default_rng(7)
+    mat = rng.uniform(0, 1, (m.ntri, 5)).astype(np.float32)
+    n = 7
+    origin = (m.bmin - np.float32(0.0131)).astype(np.float32)
+    h = np.float32((m.bmax.max() - m.bmin.min() + np.float32(0.04)) / n)
+    half = [h * np.float32(0.5)] * 3
+    sums = np.zeros((5, n, n, n), np.float32)
+    cnt = np.zeros((n, n, n), np.int64)
+    for t in range(m.ntri):
+        for z in range(n):
+            for y in range(n):
+                for x in range(n):
+                    c = origin + (np.array([x, y, z], np.float32) + np.float32(0.5)) * h
+                    if oracle.tri_box_overlap(c, half, *m.xyz[t]):
+                        sums[:, z, y, x] = (sums[:, z, y, x] + mat[t]).astype(np.float32)
+                        cnt[z, y, x] += 1
+    want = np.where(cnt > 0, sums / np.maximum(cnt, 1).astype(np.float32),
+                    np.array([1, 1, 1, 0, 1], np.float32)[:, None, None, None]).astype(np.float32)
+    got = oracle.splat_materials(m.xyz, mat, origin, h, n, n, n)
+    assert (cnt > 0).sum() > 20 and (cnt > 1).sum() > 5
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
 # ------------------------------------------------------------------------------------------
 # framing (sdf_baker.cpp:137-148, voxelize.cpp:54-57)
 # ------------------------------------------------------------------------------------------
