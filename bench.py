@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--variant", type=int, default=None, help="query kernel: 0 per-voxel, 1 per-brick wave")
     ap.add_argument("--batch-meshes", type=int, default=512, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--no-band", action="store_true", help="skip the narrow-band precision-stress bake")
+    ap.add_argument("--no-consumer", action="store_true", help="skip the grid consumer (sampling / tracing)")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="arrival window of the served workload on the batched mesh distribution (0: skip)")
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
@@ -95,6 +96,50 @@ def served_section(args, batched):
                          "2M-triangle displaced icospheres (512^3 narrow band +-4)")
         out["stress_mix"] = r
     return out or None
+
+
+def consumer_section(ctx, sdf, d_dist):
+    """The renderer's grid consumer on the device-resident 256^3 bake: 2^24 trilinear samples with
+    B-spline normals and nearest cells, and 2^20 sphere-traced rays from a shell of radius 0.75
+    towards random points near the centre (no host round trip of the grid)."""
+    rng = np.random.default_rng(3)
+    npts, nray = 1 << 24, 1 << 20
+    pts = rng.uniform(-0.8, 0.8, (npts, 3)).astype(np.float32)
+    u = rng.normal(size=(nray, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    org = (0.75 * u).astype(np.float32)
+    dirs = rng.uniform(-0.2, 0.2, (nray, 3)) - org
+    dirs = (dirs / np.linalg.norm(dirs, axis=1, keepdims=True)).astype(np.float32)
+    bufs = {k: ctx.alloc(n) for k, n in (("pts", pts.nbytes), ("val", 4 * npts), ("nrm", 12 * npts),
+                                          ("near", 4 * npts), ("org", org.nbytes), ("dir", dirs.nbytes),
+                                          ("t", 4 * nray))}
+    try:
+        ctx.upload(bufs["pts"], pts)
+        ctx.upload(bufs["org"], org)
+        ctx.upload(bufs["dir"], dirs)
+
+        def sample():
+            ctx.sample_grid_device(sdf, d_dist, bufs["pts"], npts, bufs["val"], bufs["nrm"], bufs["near"])
+
+        def trace():
+            ctx.trace_grid_device(sdf, d_dist, bufs["org"], bufs["dir"], nray, bufs["t"])
+
+        res = {}
+        for name, fn, n in (("sample", sample, npts), ("trace", trace, nray)):
+            fn()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                fn()
+            ctx.synchronize()
+            res[name] = n * 5 / (time.perf_counter() - t0)
+        t = ctx.download(bufs["t"], np.empty(nray, np.float32))
+    finally:
+        for b in bufs.values():
+            ctx.free(b)
+    return {"workload": "256^3 grid of the 327,680-triangle bake, device-resident",
+            "samples_per_s": res["sample"], "sample_outputs": "value + B-spline normal + nearest cell",
+            "rays_per_s": res["trace"], "rays_hit": int((t > 0).sum()), "rays": nray}
 
 
 def band_section(ctx, steps=3):
@@ -254,6 +299,8 @@ def main():
         ctx.free(src)
         ctx.free(dst)
 
+    consumer = consumer_section(ctx, sdf, d_dist) if not args.no_consumer else None
+
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None
     served = served_section(args, batched)
@@ -281,6 +328,7 @@ def main():
                    "triangles": mesh.ntri, "sdf_res": sdf.nx, "occ_res": occ.nx, "parallelism": "single GPU"},
         "device_ms_per_bake": float(np.mean(total_ms)),
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band, "served": served,
+        "grid_consumer": consumer,
         "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
         if served and served.get("batched_distribution") else None,
     }

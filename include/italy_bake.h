@@ -217,6 +217,21 @@ ib_status ib_server_device_result(ib_server *srv, uint64_t ticket, const float *
 ib_status ib_server_release(ib_server *srv, uint64_t ticket);
 ib_status ib_server_stats(ib_server *srv, ib_server_counters *out);
 
+/* ---- the grid consumer on the device ----------------------------------------------------------
+ * What the renderer does with a baked SDF grid, run on device-resident bakes without a host round
+ * trip (the renderer uploads SdfGrid to the device, src/render/optix_renderer.cpp:868-930).
+ * Grids need at least 2 cells per axis.  Enqueued on `stream` (NULL = the context's stream). */
+/* For n points (3 floats each): the trilinear sample (sampleSdf, src/render/kernels/pathtracer.cu:684-701),
+ * the normalised cubic B-spline gradient (sdfGradientNormal, :806-846) and the nearest cell index
+ * (sdfNearestCellIndex, :703-709).  Any output may be NULL. */
+ib_status ib_sample_grid_device(ib_context *ctx, const ib_grid *grid, const float *d_distances, const float *d_points,
+                                uint64_t n, float *d_values, float *d_normals, int32_t *d_nearest, void *stream);
+/* Sphere-trace n rays (origins, directions: 3 floats each) through the grid exactly as the SDF
+ * intersection program does (__intersection__sdf, pathtracer.cu:740-792, with the slab test of :587-613):
+ * d_t receives the hit distance, or -1 for a miss, over [tmin, tmax]. */
+ib_status ib_trace_grid_device(ib_context *ctx, const ib_grid *grid, const float *d_distances, const float *d_origins,
+                               const float *d_dirs, uint64_t n, float tmin, float tmax, float *d_t, void *stream);
+
 /* ---- device memory helpers (for callers without a GPU framework) --------------------------- */
 ib_status ib_device_alloc(ib_context *ctx, uint64_t bytes, void **d_ptr);
 ib_status ib_device_free(ib_context *ctx, void *d_ptr);

@@ -897,6 +897,43 @@ ib_status ib_memcpy_d2h(ib_context *c, void *dst, const void *src, uint64_t byte
   return IB_OK;
 }
 
+static ib_status grid_view(const ib_grid *grid, const float *d_distances, GridView &v) {
+  if (!grid || !d_distances) return fail(IB_ERR_INVALID_ARGUMENT, "null grid or distances");
+  if (!grid_ok(grid)) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid");
+  if (grid->nx < 2 || grid->ny < 2 || grid->nz < 2)
+    return fail(IB_ERR_INVALID_ARGUMENT, "trilinear sampling needs at least 2 cells per axis");
+  v = GridView{grid->origin[0], grid->origin[1], grid->origin[2], grid->h, grid->nx, grid->ny, grid->nz, d_distances};
+  return IB_OK;
+}
+
+ib_status ib_sample_grid_device(ib_context *c, const ib_grid *grid, const float *d_distances, const float *d_points,
+                                uint64_t n, float *d_values, float *d_normals, int32_t *d_nearest, void *stream) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  GridView v;
+  IB_TRY(grid_view(grid, d_distances, v));
+  if (n && !d_points) return fail(IB_ERR_INVALID_ARGUMENT, "null points");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  launch_sample_grid(v, d_points, n, d_values, d_normals, d_nearest, s);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
+ib_status ib_trace_grid_device(ib_context *c, const ib_grid *grid, const float *d_distances, const float *d_origins,
+                               const float *d_dirs, uint64_t n, float tmin, float tmax, float *d_t, void *stream) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  GridView v;
+  IB_TRY(grid_view(grid, d_distances, v));
+  if (n && (!d_origins || !d_dirs || !d_t)) return fail(IB_ERR_INVALID_ARGUMENT, "null ray buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  launch_trace_grid(v, d_origins, d_dirs, n, tmin, tmax, d_t, s);
+  IB_HIP(hipGetLastError());
+  return IB_OK;
+}
+
 ib_status ib_memcpy_d2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
   IB_HIP(hipSetDevice(c->device));

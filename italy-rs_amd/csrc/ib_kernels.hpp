@@ -134,4 +134,15 @@ hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, u
 void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
                          float *out, uint64_t ncell, hipStream_t s);
 
+// ---- grid consumer (ib_consumer.hip) ----
+struct GridView {
+  float ox, oy, oz, h;
+  int nx, ny, nz;
+  const float *d;  // distances, (z*ny + y)*nx + x
+};
+void launch_sample_grid(const GridView &g, const float *pts, uint64_t n, float *values, float *normals,
+                        int32_t *nearest, hipStream_t s);
+void launch_trace_grid(const GridView &g, const float *org, const float *dir, uint64_t n, float tmin, float tmax,
+                       float *t_hit, hipStream_t s);
+
 }  // namespace ib

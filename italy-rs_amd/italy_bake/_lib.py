@@ -17,7 +17,8 @@ STATUS_NAMES = {
 EXPORTS = [
     "ib_create", "ib_destroy", "ib_last_error", "ib_version", "ib_sdf_framing", "ib_vox_framing", "ib_bake_sdf",
     "ib_voxelize", "ib_splat_materials", "ib_bake_dense", "ib_bake_dense_device", "ib_device_alloc",
-    "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_memcpy_d2d", "ib_synchronize", "ib_last_stats", "ib_set_option",
+    "ib_device_free", "ib_memcpy_h2d", "ib_memcpy_d2h", "ib_memcpy_d2d", "ib_synchronize", "ib_sample_grid_device",
+    "ib_trace_grid_device", "ib_last_stats", "ib_set_option",
     "ib_bake_batch_device", "ib_autotune_batch", "ib_bake_band_device", "ib_server_create", "ib_server_destroy",
     "ib_server_last_error", "ib_server_submit", "ib_server_info", "ib_server_wait", "ib_server_drain",
     "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
@@ -104,6 +105,8 @@ def load():
     lib.ib_memcpy_h2d.argtypes = [_vp, _vp, _vp, _u64]
     lib.ib_memcpy_d2h.argtypes = [_vp, _vp, _vp, _u64]
     lib.ib_memcpy_d2d.argtypes = [_vp, _vp, _vp, _u64]
+    lib.ib_sample_grid_device.argtypes = [_vp, G, _vp, _vp, _u64, _vp, _vp, _vp, _vp]
+    lib.ib_trace_grid_device.argtypes = [_vp, G, _vp, _vp, _vp, _u64, ctypes.c_float, ctypes.c_float, _vp, _vp]
     lib.ib_synchronize.argtypes = [_vp]
     lib.ib_last_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     lib.ib_set_option.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64]

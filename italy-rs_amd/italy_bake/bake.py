@@ -143,6 +143,62 @@ class Context:
                                           ctypes.byref(grid), _ptr(rgb), _ptr(met), _ptr(rough)))
         return rgb, met, rough
 
+    def sample_grid_device(self, grid, d_dist, d_points, n, d_values=None, d_normals=None, d_nearest=None):
+        """ib_sample_grid_device: the renderer's trilinear sample, B-spline normal and nearest cell of a
+        device-resident grid at n device points (enqueued; call synchronize() before reading)."""
+        vp = lambda p: ctypes.c_void_p(p) if p else None
+        check(self.lib.ib_sample_grid_device(self._h, ctypes.byref(grid), vp(d_dist), vp(d_points), int(n),
+                                             vp(d_values), vp(d_normals), vp(d_nearest), None))
+
+    def trace_grid_device(self, grid, d_dist, d_org, d_dir, n, d_t, tmin=0.0, tmax=1e16):
+        """ib_trace_grid_device: the SDF intersection program's march for n device rays."""
+        check(self.lib.ib_trace_grid_device(self._h, ctypes.byref(grid), ctypes.c_void_p(d_dist),
+                                            ctypes.c_void_p(d_org), ctypes.c_void_p(d_dir), int(n), float(tmin),
+                                            float(tmax), ctypes.c_void_p(d_t), None))
+
+    def sample_grid(self, grid, dist, points):
+        """Host convenience: upload a grid (nz, ny, nx) and points (n, 3), sample, download
+        (values, normals, nearest)."""
+        dist = np.ascontiguousarray(dist, np.float32)
+        pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = len(pts)
+        bufs = [self.alloc(max(dist.nbytes, 4)), self.alloc(max(pts.nbytes, 4)), self.alloc(4 * max(n, 1)),
+                self.alloc(12 * max(n, 1)), self.alloc(4 * max(n, 1))]
+        try:
+            self.upload(bufs[0], dist)
+            if n:
+                self.upload(bufs[1], pts)
+            self.sample_grid_device(grid, bufs[0], bufs[1], n, bufs[2], bufs[3], bufs[4])
+            self.synchronize()
+            vals = self.download(bufs[2], np.empty(n, np.float32))
+            nrm = self.download(bufs[3], np.empty((n, 3), np.float32))
+            near = self.download(bufs[4], np.empty(n, np.int32))
+        finally:
+            for b in bufs:
+                self.free(b)
+        return vals, nrm, near
+
+    def trace_grid(self, grid, dist, org, dirs, tmin=0.0, tmax=1e16):
+        """Host convenience for trace_grid_device: hit t per ray (-1 = miss)."""
+        dist = np.ascontiguousarray(dist, np.float32)
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dirs = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        n = len(org)
+        bufs = [self.alloc(max(dist.nbytes, 4)), self.alloc(12 * max(n, 1)), self.alloc(12 * max(n, 1)),
+                self.alloc(4 * max(n, 1))]
+        try:
+            self.upload(bufs[0], dist)
+            if n:
+                self.upload(bufs[1], org)
+                self.upload(bufs[2], dirs)
+            self.trace_grid_device(grid, bufs[0], bufs[1], bufs[2], n, bufs[3], tmin, tmax)
+            self.synchronize()
+            t = self.download(bufs[3], np.empty(n, np.float32))
+        finally:
+            for b in bufs:
+                self.free(b)
+        return t
+
     def set_option(self, option, value):
         check(self.lib.ib_set_option(self._h, int(option), int(value)))
 

@@ -1184,3 +1184,166 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
   orc_bvh_free(&b);
   return 0;
 }
+
+/* ============================================================================
+ * the grid consumer: pathtracer.cu:684-709 (sampleSdf, sdfNearestCellIndex), :798-846 (cubic
+ * B-spline gradient normal), :587-613 + :740-792 (slab test + SDF march)
+ * ========================================================================== */
+typedef struct {
+  float o[3], h;
+  int n[3];
+  const float *d;
+} gview;
+
+static void local_coords(const gview *g, const float p[3], float l[3]) {
+  for (int k = 0; k < 3; ++k) l[k] = (p[k] - g->o[k]) / g->h - 0.5f; /* :685 */
+}
+static inline int imin(int a, int b) { return a < b ? a : b; }
+static inline int imax(int a, int b) { return a > b ? a : b; }
+static inline float gat(const gview *g, int x, int y, int z) { return g->d[((size_t)z * g->n[1] + y) * g->n[0] + x]; }
+
+static float sample_sdf(const gview *g, const float p[3]) { /* :684-701 */
+  float l[3];
+  local_coords(g, p, l);
+  const int x0 = imax(0, imin(g->n[0] - 2, (int)floorf(l[0])));
+  const int y0 = imax(0, imin(g->n[1] - 2, (int)floorf(l[1])));
+  const int z0 = imax(0, imin(g->n[2] - 2, (int)floorf(l[2])));
+  const float fx = fminf(fmaxf(l[0] - (float)x0, 0.0f), 1.0f);
+  const float fy = fminf(fmaxf(l[1] - (float)y0, 0.0f), 1.0f);
+  const float fz = fminf(fmaxf(l[2] - (float)z0, 0.0f), 1.0f);
+  const float c00 = gat(g, x0, y0, z0) * (1.0f - fx) + gat(g, x0 + 1, y0, z0) * fx;
+  const float c10 = gat(g, x0, y0 + 1, z0) * (1.0f - fx) + gat(g, x0 + 1, y0 + 1, z0) * fx;
+  const float c01 = gat(g, x0, y0, z0 + 1) * (1.0f - fx) + gat(g, x0 + 1, y0, z0 + 1) * fx;
+  const float c11 = gat(g, x0, y0 + 1, z0 + 1) * (1.0f - fx) + gat(g, x0 + 1, y0 + 1, z0 + 1) * fx;
+  const float c0 = c00 * (1.0f - fy) + c10 * fy;
+  const float c1 = c01 * (1.0f - fy) + c11 * fy;
+  return c0 * (1.0f - fz) + c1 * fz;
+}
+
+static int nearest_cell(const gview *g, const float p[3]) { /* :703-709 */
+  float l[3];
+  local_coords(g, p, l);
+  const int x = imax(0, imin(g->n[0] - 1, (int)lroundf(l[0])));
+  const int y = imax(0, imin(g->n[1] - 1, (int)lroundf(l[1])));
+  const int z = imax(0, imin(g->n[2] - 1, (int)lroundf(l[2])));
+  return (z * g->n[1] + y) * g->n[0] + x;
+}
+
+static void bspline(float t, float w[4], float dw[4]) { /* :794-804 */
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = (1.0f - 3.0f * t + 3.0f * t2 - t3) * (1.0f / 6.0f);
+  w[1] = (4.0f - 6.0f * t2 + 3.0f * t3) * (1.0f / 6.0f);
+  w[2] = (1.0f + 3.0f * t + 3.0f * t2 - 3.0f * t3) * (1.0f / 6.0f);
+  w[3] = t3 * (1.0f / 6.0f);
+  dw[0] = (-1.0f + 2.0f * t - t2) * 0.5f;
+  dw[1] = (-4.0f * t + 3.0f * t2) * 0.5f;
+  dw[2] = (1.0f + 2.0f * t - 3.0f * t2) * 0.5f;
+  dw[3] = t2 * 0.5f;
+}
+
+static void gradient_normal(const gview *g, const float p[3], float out[3]) { /* :806-846 */
+  float l[3];
+  local_coords(g, p, l);
+  const int bx = (int)floorf(l[0]), by = (int)floorf(l[1]), bz = (int)floorf(l[2]);
+  float wx[4], dwx[4], wy[4], dwy[4], wz[4], dwz[4];
+  bspline(l[0] - (float)bx, wx, dwx);
+  bspline(l[1] - (float)by, wy, dwy);
+  bspline(l[2] - (float)bz, wz, dwz);
+  int ix[4];
+  for (int i = 0; i < 4; ++i) ix[i] = imax(0, imin(g->n[0] - 1, bx - 1 + i));
+  float gr[3] = {0.0f, 0.0f, 0.0f};
+  for (int k = 0; k < 4; ++k) {
+    const int zrow = imax(0, imin(g->n[2] - 1, bz - 1 + k)) * g->n[1];
+    float pl[3] = {0.0f, 0.0f, 0.0f};
+    for (int j = 0; j < 4; ++j) {
+      const size_t row = (size_t)(zrow + imax(0, imin(g->n[1] - 1, by - 1 + j))) * g->n[0];
+      float v = 0.0f, dv = 0.0f;
+      for (int i = 0; i < 4; ++i) {
+        const float f = g->d[row + ix[i]];
+        v += wx[i] * f;
+        dv += dwx[i] * f;
+      }
+      pl[0] += dv * wy[j];
+      pl[1] += v * dwy[j];
+      pl[2] += v * wy[j];
+    }
+    gr[0] += pl[0] * wz[k];
+    gr[1] += pl[1] * wz[k];
+    gr[2] += pl[2] * dwz[k];
+  }
+  const float inv = 1.0f / sqrtf((gr[0] * gr[0] + gr[1] * gr[1]) + gr[2] * gr[2]);
+  for (int k = 0; k < 3; ++k) out[k] = gr[k] * inv;
+}
+
+static float trace_grid(const gview *g, const float o[3], const float d[3], float t0, float t1) {
+  float lo[3], hi[3];
+  for (int k = 0; k < 3; ++k) {
+    lo[k] = g->o[k];
+    hi[k] = g->o[k] + (float)g->n[k] * g->h; /* boundsMax = origin + n * voxelSize, :742 */
+  }
+  for (int a = 0; a < 3; ++a) { /* slabTest, :587-613 */
+    const float inv = 1.0f / d[a];
+    float tn = (lo[a] - o[a]) * inv, tf = (hi[a] - o[a]) * inv;
+    if (tn > tf) {
+      const float tmp = tn;
+      tn = tf;
+      tf = tmp;
+    }
+    if (tn > t0) t0 = tn;
+    if (tf < t1) t1 = tf;
+    if (t0 > t1) return -1.0f;
+  }
+  const float eps = g->h * 0.02f, min_step = g->h * 0.05f, min_adv = g->h * 0.3f; /* :757-759 */
+  float t = t0, prev_t = t0, p[3];
+  for (int k = 0; k < 3; ++k) p[k] = o[k] + t0 * d[k];
+  float prev = sample_sdf(g, p);
+  t += fmaxf(fabsf(prev), min_step);
+  for (int i = 1; i < 512 && t <= t1; ++i) {
+    for (int k = 0; k < 3; ++k) p[k] = o[k] + t * d[k];
+    const float dist = sample_sdf(g, p);
+    const int crossed = (prev > 0.0f) != (dist > 0.0f);
+    if ((crossed || fabsf(dist) < eps) && (t - t0) > min_adv) {
+      float th = t;
+      if (crossed) { /* 8 bisections, then a secant step (:769-784) */
+        float a = prev_t, b = t, fa = prev, fb = dist;
+        for (int k = 0; k < 8; ++k) {
+          const float m = 0.5f * (a + b);
+          float pm[3];
+          for (int c = 0; c < 3; ++c) pm[c] = o[c] + m * d[c];
+          const float fm = sample_sdf(g, pm);
+          if ((fa > 0.0f) != (fm > 0.0f)) {
+            b = m;
+            fb = fm;
+          } else {
+            a = m;
+            fa = fm;
+          }
+        }
+        const float den = fa - fb;
+        th = fabsf(den) > 1e-9f ? a + (b - a) * (fa / den) : 0.5f * (a + b);
+      }
+      return fmaxf(th, t0);
+    }
+    prev_t = t;
+    prev = dist;
+    t += fmaxf(fabsf(dist) * 0.85f, min_step);
+  }
+  return -1.0f;
+}
+
+void orc_sample_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *pts,
+                     size_t n, float *values, float *normals, int32_t *nearest) {
+  const gview g = {{origin[0], origin[1], origin[2]}, h, {nx, ny, nz}, dist};
+  for (size_t i = 0; i < n; ++i) {
+    const float *p = pts + 3 * i;
+    if (values) values[i] = sample_sdf(&g, p);
+    if (normals) gradient_normal(&g, p, normals + 3 * i);
+    if (nearest) nearest[i] = nearest_cell(&g, p);
+  }
+}
+
+void orc_trace_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *org,
+                    const float *dir, size_t n, float tmin, float tmax, float *t_hit) {
+  const gview g = {{origin[0], origin[1], origin[2]}, h, {nx, ny, nz}, dist};
+  for (size_t i = 0; i < n; ++i) t_hit[i] = trace_grid(&g, org + 3 * i, dir + 3 * i, tmin, tmax);
+}

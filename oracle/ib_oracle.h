@@ -139,6 +139,14 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
 /* number of directed edges left unmatched after exact-position welding (0 => closed) */
 int64_t orc_boundary_edge_count(const float *xyz, size_t ntri);
 
+/* The renderer's grid consumer (src/render/kernels/pathtracer.cu): trilinear sampleSdf (:684-701),
+ * normalised cubic B-spline gradient (:806-846) and nearest cell (:703-709) at n points; outputs may
+ * be NULL.  orc_trace_grid: the SDF intersection program's march (:587-613, :740-792), -1 = miss. */
+void orc_sample_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *pts,
+                     size_t n, float *values, float *normals, int32_t *nearest);
+void orc_trace_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *org,
+                    const float *dir, size_t n, float tmin, float tmax, float *t_hit);
+
 #ifdef __cplusplus
 }
 #endif

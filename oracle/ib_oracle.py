@@ -59,6 +59,12 @@ def _load():
     lib.orc_voxelize_first_touch.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, _f32p, ctypes.c_int, _i32p,
                                              _f32p, ctypes.c_size_t]
     lib.orc_voxelize_first_touch.restype = ctypes.c_size_t
+    lib.orc_sample_grid.argtypes = [_f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p,
+                                    ctypes.c_size_t, _f32p, _f32p, _i32p]
+    lib.orc_sample_grid.restype = None
+    lib.orc_trace_grid.argtypes = [_f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _f32p,
+                                   _f32p, ctypes.c_size_t, ctypes.c_float, ctypes.c_float, _f32p]
+    lib.orc_trace_grid.restype = None
     lib.orc_splat_materials.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.c_float, ctypes.c_int,
                                         ctypes.c_int, ctypes.c_int, _f32p]
     lib.orc_splat_materials.restype = None
@@ -159,6 +165,33 @@ def splat_materials(xyz, mat, origin, h, nx, ny, nz):
     lib().orc_splat_materials(_p(xyz, _f32p), xyz.size // 9, _p(mat, _f32p), _p(_f32(origin), _f32p), float(h),
                               nx, ny, nz, _p(out, _f32p))
     return out
+
+
+def sample_grid(origin, h, dist, pts):
+    """The renderer's grid sampling (pathtracer.cu:684-709, :794-846): (values, unit normals, nearest
+    cell) at pts (n, 3); dist is (nz, ny, nx)."""
+    dist = _f32(dist)
+    nz, ny, nx = dist.shape
+    pts = _f32(pts).reshape(-1, 3)
+    n = len(pts)
+    vals = np.zeros(n, np.float32)
+    nrm = np.zeros((n, 3), np.float32)
+    near = np.zeros(n, np.int32)
+    lib().orc_sample_grid(_p(_f32(origin), _f32p), float(h), nx, ny, nz, _p(dist, _f32p), _p(pts, _f32p), n,
+                          _p(vals, _f32p), _p(nrm, _f32p), _p(near, _i32p))
+    return vals, nrm, near
+
+
+def trace_grid(origin, h, dist, org, dirs, tmin=0.0, tmax=1e16):
+    """The SDF intersection program's march (pathtracer.cu:587-613, :740-792): hit t per ray, -1 = miss."""
+    dist = _f32(dist)
+    nz, ny, nx = dist.shape
+    org = _f32(org).reshape(-1, 3)
+    dirs = _f32(dirs).reshape(-1, 3)
+    t = np.zeros(len(org), np.float32)
+    lib().orc_trace_grid(_p(_f32(origin), _f32p), float(h), nx, ny, nz, _p(dist, _f32p), _p(org, _f32p),
+                         _p(dirs, _f32p), len(org), float(tmin), float(tmax), _p(t, _f32p))
+    return t
 
 
 def tri_dist2_f32(q, tri):

@@ -316,3 +316,58 @@ def test_open_mesh_rejected_by_closed_bake(oracle, meshgen):
     assert oracle.boundary_edge_count(u.xyz) == 64  # seam crack + south-pole ring (float trig)
     d, st = oracle.bake_dense(u.xyz, [-1.2] * 3, 0.1, 4, 4, 4)
     assert d is None
+
+
+# ------------------------------------------------------------------------------------------
+# the renderer's grid consumer (pathtracer.cu:587-613, :684-709, :740-846)
+# ------------------------------------------------------------------------------------------
+def _affine_grid(n=12, h=0.1):
+    origin = np.array([-0.6, -0.55, -0.5], np.float32)
+    a = np.array([0.3, -0.7, 0.64], np.float64)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    c = origin.astype(np.float64) + (np.stack([x, y, z], -1) + 0.5) * np.float64(np.float32(h))
+    return origin, np.float32(h), (c @ a + 0.25).astype(np.float32), a
+
+
+def test_sample_grid_affine_field(oracle):
+    """Trilinear interpolation reproduces an affine field, and the B-spline gradient of an affine
+    field is its constant gradient (the weights sum to 1 and their derivatives to 0)."""
+    origin, h, dist, a = _affine_grid()
+    n = dist.shape[0]
+    rng = np.random.default_rng(5)
+    # inside the region where no stencil index is clamped: local coordinates in [1, n - 3]
+    loc = rng.uniform(1.0, n - 3.0, (500, 3))
+    pts = (origin + (loc + 0.5) * h).astype(np.float32)
+    vals, nrm, near = oracle.sample_grid(origin, h, dist, pts)
+    want = pts.astype(np.float64) @ a + 0.25
+    assert np.abs(vals - want).max() < 2e-6
+    assert np.abs(nrm - a / np.linalg.norm(a)).max() < 1e-5
+    l = (pts - origin) / h - np.float32(0.5)
+    r = np.clip(np.sign(l) * np.floor(np.abs(l) + 0.5), 0, n - 1).astype(int)  # lroundf: half away from zero
+    assert np.array_equal(near, (r[:, 2] * n + r[:, 1]) * n + r[:, 0])
+
+
+def test_sample_grid_clamps_outside(oracle):
+    """Outside the grid sampleSdf clamps to the border cell pair and the nearest cell to the border."""
+    origin, h, dist, _ = _affine_grid(6)
+    vals, _, near = oracle.sample_grid(origin, h, dist, [[-5.0, -5.0, -5.0], [5.0, 5.0, 5.0]])
+    assert vals[0] == dist[0, 0, 0] and vals[1] == dist[-1, -1, -1]
+    assert near.tolist() == [0, dist.size - 1]
+
+
+def test_trace_grid_sphere(oracle):
+    """Sphere tracing the sampled SDF of a sphere of radius 0.5 hits at |o| - 0.5 (within a tenth of
+    a voxel) for rays aimed at the centre, and misses for rays that leave the grid first."""
+    n, h = 48, np.float32(1.6 / 48)
+    origin = np.full(3, -0.8, np.float32)
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    c = origin + (np.stack([x, y, z], -1).astype(np.float32) + np.float32(0.5)) * h
+    dist = (np.linalg.norm(c.astype(np.float64), axis=-1) - 0.5).astype(np.float32)
+    rng = np.random.default_rng(9)
+    u = rng.normal(size=(200, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    org = (0.75 * u).astype(np.float32)
+    t = oracle.trace_grid(origin, h, dist, org, -u)
+    assert np.abs(t - 0.25).max() < 0.1 * h
+    away = oracle.trace_grid(origin, h, dist, org, u)
+    assert (away == -1).all()
