@@ -214,6 +214,26 @@ __global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, 
   }
 }
 
+// Collapse: a child slot whose node has two plain leaf children (Karras: triangles g and g + 1)
+// becomes the pair code ~(g | kPairBit), so the traversals evaluate both triangles from the parent
+// without visiting that node (its box is already in the parent's slot).  Runs after the refit and
+// the depth pass.  A slot another thread is rewriting was an internal index, and a pair code is
+// never taken for a plain leaf, so no node is collapsed on a half-written child.
+__global__ __launch_bounds__(256) void k_collapse_pairs(Node *nodes, int ninternal) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ninternal) return;
+  for (int k = 0; k < 2; ++k) {
+    const int c = k ? nodes[i].c1 : nodes[i].c0;
+    if (c < 0) continue;
+    const int a = __hip_atomic_load(&nodes[c].c0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int b = __hip_atomic_load(&nodes[c].c1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (a >= 0 || b >= 0 || ((~a) & kPairBit) || ((~b) & kPairBit) || ~b != ~a + 1) continue;
+    const int code = ~((~a) | kPairBit);
+    if (k) __hip_atomic_store(&nodes[i].c1, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else __hip_atomic_store(&nodes[i].c0, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
 __global__ __launch_bounds__(256) void k_depth(const Node *__restrict__ nodes, const int32_t *__restrict__ leaf_parent,
                                                int n, Scalars *sc) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -484,6 +504,15 @@ struct ColumnSink {
   }
 };
 
+// A leaf child code ~L names triangle L, or with kPairBit set the two consecutive triangles
+// L & ~kPairBit and the next one (a collapsed node whose children were both leaves).
+template <typename F>
+__device__ __forceinline__ void for_leaf_code(int code, F &&leaf) {
+  const int t = code & ~kPairBit;
+  leaf(t);
+  if (code & kPairBit) leaf(t + 1);
+}
+
 // visits every leaf whose box contains (X,Y) in xy
 template <typename F>
 __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes, int nnodes, float X, float Y,
@@ -497,10 +526,10 @@ __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes
     const bool in1 = X >= nd.lo1[0] && X <= nd.hi1[0] && Y >= nd.lo1[1] && Y <= nd.hi1[1];
     int next = -1;
     if (in0) {
-      if (nd.c0 < 0) leaf(~nd.c0); else next = nd.c0;
+      if (nd.c0 < 0) for_leaf_code(~nd.c0, leaf); else next = nd.c0;
     }
     if (in1) {
-      if (nd.c1 < 0) leaf(~nd.c1);
+      if (nd.c1 < 0) for_leaf_code(~nd.c1, leaf);
       else if (next < 0) next = nd.c1;
       else if (sp < kStack) stack[sp++] = nd.c1;
     }
@@ -725,8 +754,10 @@ __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int n
       int next = -1;
       if (ray_box(q, d, nd.lo0, nd.hi0)) {
         if (nd.c0 < 0) {
-          const float4 *t = tris + 3 * (~nd.c0);
-          hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+          for_leaf_code(~nd.c0, [&](int l) {
+            const float4 *t = tris + 3 * l;
+            hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+          });
         } else {
           next = nd.c0;
         }
@@ -734,8 +765,10 @@ __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int n
       if (ray_box(q, d, nd.lo1, nd.hi1)) {
         if (nd.c1 < 0) {
           if (nd.c1 != nd.c0) {
-            const float4 *t = tris + 3 * (~nd.c1);
-            hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+            for_leaf_code(~nd.c1, [&](int l) {
+              const float4 *t = tris + 3 * l;
+              hits += ray_hit(q, sh, f3(t[0]), f3(t[1]), f3(t[2])) ? 1 : 0;
+            });
           }
         } else if (next < 0) {
           next = nd.c1;
@@ -778,14 +811,16 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const float d0 = box_d2(q, nd.lo0, nd.hi0);
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
     if (COUNT) wc.box += 2;
-    if (nd.c0 < 0 && d0 <= best) {
-      best = fminf(best, tri_d2<SLIVERS>(q, recs[~nd.c0]));
-      if (COUNT) wc.tri++;
-    }
-    if (nd.c1 < 0 && d1 <= best) {
-      best = fminf(best, tri_d2<SLIVERS>(q, recs[~nd.c1]));
-      if (COUNT) wc.tri++;
-    }
+    if (nd.c0 < 0 && d0 <= best)
+      for_leaf_code(~nd.c0, [&](int l) {
+        best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
+        if (COUNT) wc.tri++;
+      });
+    if (nd.c1 < 0 && d1 <= best)
+      for_leaf_code(~nd.c1, [&](int l) {
+        best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
+        if (COUNT) wc.tri++;
+      });
     const bool g0 = nd.c0 >= 0 && d0 <= best;
     const bool g1 = nd.c1 >= 0 && d1 <= best;
     if (g0 && g1) {
@@ -1052,16 +1087,13 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
       next = __builtin_amdgcn_readfirstlane(sstack[sp]);
     }
     if (next >= 0) nd = load_node_uniform(nodes, next);
-    if (leaf0) {
-      const TriRec t = load_trirec_uniform(recs, tri0);
+    auto eval = [&](int l) {
+      const TriRec t = load_trirec_uniform(recs, l);
       for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
       if (COUNT) wc.tri += V;
-    }
-    if (leaf1) {
-      const TriRec t = load_trirec_uniform(recs, tri1);
-      for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
-      if (COUNT) wc.tri += V;
-    }
+    };
+    if (leaf0) for_leaf_code(tri0, eval);
+    if (leaf1) for_leaf_code(tri1, eval);
     if (next < 0) break;
   }
   if (COUNT) {
@@ -1234,6 +1266,7 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   }
   k_refit<<<nb(n), 256, 0, s>>>(tris, n, nodes, leaf_parent, visits, root_box);
   k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
+  k_collapse_pairs<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1);
 }
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
