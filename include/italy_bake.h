@@ -80,9 +80,11 @@ typedef enum {
   IB_OPT_BRICK_SHAPE = 3,   /* voxels of one wave: 0 = 4x4x4 (default), 1 = 8x4x2, 2 = 8x8x1, 3 = 16x4x1,
                                4 = 4x4x8 (two voxels per lane) */
   IB_OPT_BLOCKS_PER_CU = 4, /* persistent blocks per compute unit in batched bakes (default 8) */
-  IB_OPT_TILE_ORDER = 5     /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
+  IB_OPT_TILE_ORDER = 5,    /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
                                first (estimated from the column crossings), 2 (default) half-octave cost
                                buckets, costliest first, Morton order inside a bucket.  Speed only. */
+  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..8, default 4) are evaluated
+                               as one block from their parent.  Speed only; applies to later builds. */
 } ib_option;
 
 /* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy

@@ -62,6 +62,7 @@ struct ib_context {
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
+  int leaf_block = 4;          // LBVH subtrees of at most this many triangles are evaluated as one leaf block
   int tile_order = 2;          // dense per-brick queries: 0 grid order, 1 costliest first, 2 cost buckets + Morton
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
@@ -433,10 +434,12 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   char *temp;
   IB_TRY(c->scratch("sort_temp", temp_bytes, &temp));
   IB_HIP(sort_pairs(temp, temp_bytes, codes_in, codes, idx, order, n, s));
-  launch_karras(codes, n, m.nodes, leaf_parent, s);
+  int2 *ranges;
+  IB_TRY(c->scratch("node_ranges", std::max(n - 1, 1), &ranges));
+  launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits, 0u, n, s);
-  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, s);
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, c->leaf_block, s);
   m.n = n;
   if (!arena) c->record(EV_BVH, s);
   return IB_OK;
@@ -1297,6 +1300,10 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
     case IB_OPT_BLOCKS_PER_CU:
       if (value < 1 || value > 64) return fail(IB_ERR_INVALID_ARGUMENT, "blocks per CU must be 1..64");
       c->blocks_per_cu = (int)value;
+      return IB_OK;
+    case IB_OPT_LEAF_BLOCK:
+      if (value < 1 || value > kMaxLeafBlock) return fail(IB_ERR_INVALID_ARGUMENT, "leaf block must be 1..8");
+      c->leaf_block = (int)value;
       return IB_OK;
     case IB_OPT_TILE_ORDER:
       if (value < 0 || value > 2) return fail(IB_ERR_INVALID_ARGUMENT, "tile order must be 0..2");

@@ -10,7 +10,10 @@ namespace ib {
 constexpr int kStack = 64;        // traversal stack entries (>= LBVH depth, checked at build)
 constexpr int kColCap = 32;       // crossings stored per +z column before the per-voxel fallback
 constexpr uint32_t kSliver = 1u;  // triangle flag: distance evaluated in FP64
-constexpr int kPairBit = 1 << 30;  // leaf code flag: two consecutive triangles (meshes < 2^30 triangles)
+// leaf code ~L: first triangle L & kBlockMask, count (L >> kBlockShift) + 1 (meshes < 2^28 triangles)
+constexpr int kBlockShift = 28;
+constexpr int kBlockMask = (1 << kBlockShift) - 1;
+constexpr int kMaxLeafBlock = 8;
 
 // Internal LBVH node: both children's boxes inline so one 64-byte record decides both children.
 // child >= 0: internal node index, child < 0: ~leaf (sorted triangle position).

@@ -106,7 +106,7 @@ __device__ __forceinline__ int kdelta(const uint32_t *codes, int n, int i, int j
 }
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ codes, int n, Node *nodes,
-                                                int32_t *leaf_parent) {
+                                                int32_t *leaf_parent, int2 *ranges) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n - 1) return;
   const int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
@@ -132,6 +132,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
   // 1: the node's leaf range [lo, hi] lies within one 256-leaf block, so the refit threads that
   // meet at this node are all in one workgroup
   nodes[i].pad = (lo >> 8) == (hi >> 8) ? 1 : 0;
+  ranges[i] = make_int2(lo, hi);
   if (lc >= 0) nodes[lc].parent = i; else leaf_parent[~lc] = i;
   if (rc >= 0) nodes[rc].parent = i; else leaf_parent[~rc] = i;
   if (i == 0) nodes[0].parent = -1;
@@ -214,23 +215,24 @@ __global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, 
   }
 }
 
-// Collapse: a child slot whose node has two plain leaf children (Karras: triangles g and g + 1)
-// becomes the pair code ~(g | kPairBit), so the traversals evaluate both triangles from the parent
-// without visiting that node (its box is already in the parent's slot).  Runs after the refit and
-// the depth pass.  A slot another thread is rewriting was an internal index, and a pair code is
-// never taken for a plain leaf, so no node is collapsed on a half-written child.
-__global__ __launch_bounds__(256) void k_collapse_pairs(Node *nodes, int ninternal) {
+// Collapse: a child slot whose node covers at most `block` leaves (a contiguous range [lo, hi] of
+// sorted triangles, from the Karras ranges) becomes the leaf-block code ~(lo | (count - 1) << 28),
+// so the traversals evaluate those triangles from the parent without visiting the nodes below
+// (their union box is already in the parent's slot).  Only the immutable ranges are read, so the
+// order in which threads rewrite slots does not matter.  Runs after the refit and the depth pass.
+__global__ __launch_bounds__(256) void k_collapse_blocks(Node *nodes, const int2 *__restrict__ ranges, int ninternal,
+                                                         int block) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= ninternal) return;
   for (int k = 0; k < 2; ++k) {
     const int c = k ? nodes[i].c1 : nodes[i].c0;
     if (c < 0) continue;
-    const int a = __hip_atomic_load(&nodes[c].c0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const int b = __hip_atomic_load(&nodes[c].c1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a >= 0 || b >= 0 || ((~a) & kPairBit) || ((~b) & kPairBit) || ~b != ~a + 1) continue;
-    const int code = ~((~a) | kPairBit);
-    if (k) __hip_atomic_store(&nodes[i].c1, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else __hip_atomic_store(&nodes[i].c0, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int2 r = ranges[c];
+    const int cnt = r.y - r.x + 1;
+    if (cnt > block) continue;
+    const int code = ~(r.x | ((cnt - 1) << kBlockShift));
+    if (k) nodes[i].c1 = code;
+    else nodes[i].c0 = code;
   }
 }
 
@@ -504,13 +506,12 @@ struct ColumnSink {
   }
 };
 
-// A leaf child code ~L names triangle L, or with kPairBit set the two consecutive triangles
-// L & ~kPairBit and the next one (a collapsed node whose children were both leaves).
+// A leaf child code ~L names the triangles L & kBlockMask .. + (L >> kBlockShift): one triangle, or
+// a collapsed block of up to 8 consecutive ones.
 template <typename F>
 __device__ __forceinline__ void for_leaf_code(int code, F &&leaf) {
-  const int t = code & ~kPairBit;
-  leaf(t);
-  if (code & kPairBit) leaf(t + 1);
+  const int t = code & kBlockMask, last = t + (code >> kBlockShift);
+  for (int l = t; l <= last; ++l) leaf(l);
 }
 
 // visits every leaf whose box contains (X,Y) in xy
@@ -1251,22 +1252,22 @@ hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint3
   return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, (unsigned)n, 0u, 30u, s);
 }
 
-void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s) {
-  if (n > 1) k_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, nodes, leaf_parent);
+void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, int2 *ranges, hipStream_t s) {
+  if (n > 1) k_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, nodes, leaf_parent, ranges);
 }
 void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, TriRec *recs, Scalars *sc,
                    hipStream_t s) {
   k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, recs, sc);
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
-                  float *root_box, Scalars *sc, hipStream_t s) {
+                  float *root_box, Scalars *sc, const int2 *ranges, int block, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
     return;
   }
   k_refit<<<nb(n), 256, 0, s>>>(tris, n, nodes, leaf_parent, visits, root_box);
   k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
-  k_collapse_pairs<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1);
+  if (block > 1) k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
 }
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,

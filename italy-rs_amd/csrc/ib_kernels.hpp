@@ -71,11 +71,12 @@ void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s);
 void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, int32_t *idx, hipStream_t s);
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                       int32_t *vout, int n, hipStream_t s);
-void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, hipStream_t s);
+void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, int2 *ranges, hipStream_t s);
 void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, TriRec *recs, Scalars *sc,
                    hipStream_t s);
+// refit, depth, then collapse of subtrees of at most `block` leaves into leaf-block codes
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
-                  float *root_box, Scalars *sc, hipStream_t s);
+                  float *root_box, Scalars *sc, const int2 *ranges, int block, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
                  hipStream_t s);

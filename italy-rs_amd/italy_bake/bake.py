@@ -18,6 +18,7 @@ OPT_QUERY_VARIANT = 2
 OPT_BRICK_SHAPE = 3
 OPT_BLOCKS_PER_CU = 4
 OPT_TILE_ORDER = 5
+OPT_LEAF_BLOCK = 6
 
 
 def _f3(v):
