@@ -371,6 +371,7 @@ struct MeshDev {
   Node *nodes = nullptr;
   float4 *tris = nullptr;
   TriRec *recs = nullptr;
+  Node4 *nodes4 = nullptr;
   float *caps = nullptr;
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
@@ -427,6 +428,7 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(take(c, arena, "nodes", std::max(n - 1, 1), &m.nodes));
   IB_TRY(take(c, arena, "tris", 3ull * n, &m.tris));
   IB_TRY(take(c, arena, "trirecs", (size_t)n, &m.recs));
+  IB_TRY(take(c, arena, "nodes4", std::max(n - 1, 1), &m.nodes4));
   launch_prep(d_xyz, n, sc, s);
   launch_morton(d_xyz, n, sc, codes_in, idx, s);
   size_t temp_bytes = 0;
@@ -439,7 +441,7 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits, 0u, n, s);
-  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, c->leaf_block, s);
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, c->leaf_block, m.nodes4, s);
   m.n = n;
   if (!arena) c->record(EV_BVH, s);
   return IB_OK;
@@ -517,6 +519,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.nnodes = std::max(m.n - 1, 1);
   ca.tris = m.tris;
   ca.recs = m.recs;
+  ca.nodes4 = m.nodes4;
   ca.caps = m.caps;
   ca.ncap = m.ncap;
   ca.cap_nodes = m.cap_nodes;
@@ -1087,6 +1090,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       sa.col.nnodes = std::max(mesh[j].n - 1, 1);
       sa.col.tris = mesh[j].tris;
       sa.col.recs = mesh[j].recs;
+      sa.col.nodes4 = mesh[j].nodes4;
       sa.col.g = to_args(cg);
       sa.col.sc = scs + j;
       sa.out = cdist[j];

@@ -8,6 +8,7 @@
 namespace ib {
 
 constexpr int kStack = 64;        // traversal stack entries (>= LBVH depth, checked at build)
+constexpr int kStack4 = 128;      // brick query stack over the 4-wide view: <= 3 pushes per level
 constexpr int kColCap = 32;       // crossings stored per +z column before the per-voxel fallback
 constexpr uint32_t kSliver = 1u;  // triangle flag: distance evaluated in FP64
 // leaf code ~L: first triangle L & kBlockMask, count (L >> kBlockShift) + 1 (meshes < 2^28 triangles)
@@ -23,6 +24,18 @@ struct alignas(16) Node {
   int32_t c0, c1, parent, pad;
 };
 static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
+
+// 4-wide view of the LBVH for the brick query: the record of internal node x holds x's
+// grandchildren (an internal child is replaced by its two children; a leaf or leaf block stays one
+// entry): up to 4 boxes and child codes (internal node index, whose own Node4 follows, or a leaf
+// code < 0).  Entries past `count` are unused.
+struct alignas(16) Node4 {
+  float lo[4][3];
+  float hi[4][3];
+  int32_t child[4];
+  int32_t count, pad[3];
+};
+static_assert(sizeof(Node4) == 128, "4-wide record is two 64-byte lines");
 
 // Node of the hole-cap hierarchy (built on the host).  left >= 0: children left/right;
 // left < 0: leaf holding triangles [-1 - left, -1 - left + right) of the reordered cap soup.

@@ -236,6 +236,41 @@ __global__ __launch_bounds__(256) void k_collapse_blocks(Node *nodes, const int2
   }
 }
 
+__global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, int ninternal, Node4 *out) {
+  const int x = blockIdx.x * 256 + threadIdx.x;
+  if (x >= ninternal) return;
+  Node4 r;
+  int n = 0;
+  auto put = [&](int code, const float *lo, const float *hi) {
+    for (int k = 0; k < 3; ++k) {
+      r.lo[n][k] = lo[k];
+      r.hi[n][k] = hi[k];
+    }
+    r.child[n++] = code;
+  };
+  const Node &nd = nodes[x];
+  for (int k = 0; k < 2; ++k) {
+    const int c = k ? nd.c1 : nd.c0;
+    if (c >= 0) {
+      const Node &ch = nodes[c];
+      put(ch.c0, ch.lo0, ch.hi0);
+      put(ch.c1, ch.lo1, ch.hi1);
+    } else {
+      put(c, k ? nd.lo1 : nd.lo0, k ? nd.hi1 : nd.hi0);
+    }
+  }
+  for (int e = n; e < 4; ++e) {
+    for (int k = 0; k < 3; ++k) {
+      r.lo[e][k] = INFINITY;
+      r.hi[e][k] = -INFINITY;
+    }
+    r.child[e] = -1;
+  }
+  r.count = n;
+  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  out[x] = r;
+}
+
 __global__ __launch_bounds__(256) void k_depth(const Node *__restrict__ nodes, const int32_t *__restrict__ leaf_parent,
                                                int n, Scalars *sc) {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -1010,6 +1045,19 @@ __device__ __forceinline__ Node load_node_uniform(const Node *p, int i) {
   return n;
 }
 
+union Node4Bits {
+  V4 v[8];
+  Node4 r;
+};
+
+__device__ __forceinline__ Node4 load_node4_uniform(const Node4 *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  Node4Bits u;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) u.v[k] = r[k];
+  return u.r;
+}
+
 union TriRecBits {
   V4 v[9];
   TriRec r;
@@ -1048,54 +1096,68 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   // the lane of voxel (sx/2, sy/2, lz/2) orders the children: its two box distances (first voxel)
   // are read with v_readlane and compared as integers (non-negative floats order like their bits)
   const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (lz >> 1) * sx * sy);
-  const Node *__restrict__ nodes = a.col.nodes;
+  const Node4 *__restrict__ nodes4 = a.col.nodes4;
   const TriRec *__restrict__ recs = a.col.recs;
   WorkCount wc;
   int sp = 0;
-  Node nd = load_node_uniform(nodes, 0);
-  for (int visits = 0; visits <= a.col.nnodes; ++visits) {
-    float d0 = box_d2(q[0], nd.lo0, nd.hi0);
-    float d1 = box_d2(q[0], nd.lo1, nd.hi1);
-    bool n0 = d0 <= best[0], n1 = d1 <= best[0];
-    for (int k = 1; k < V; ++k) {
-      n0 = n0 || box_d2(q[k], nd.lo0, nd.hi0) <= best[k];
-      n1 = n1 || box_d2(q[k], nd.lo1, nd.hi1) <= best[k];
+  int cur = 0;
+  auto eval = [&](int l) {
+    const TriRec t = load_trirec_uniform(recs, l);
+    for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
+    if (COUNT) wc.tri += V;
+  };
+  // compare-exchange of (key, child) pairs, ascending by key (wave-uniform values)
+  auto cx = [](uint32_t &ka, int &ca, uint32_t &kb, int &cb) {
+    if (kb < ka) {
+      const uint32_t tk = ka;
+      ka = kb;
+      kb = tk;
+      const int tc = ca;
+      ca = cb;
+      cb = tc;
     }
-    if (COUNT) wc.box += 2 * V;
-    const bool any0 = __ballot(n0) != 0;
-    const bool any1 = __ballot(n1) != 0;
-    const bool leaf0 = nd.c0 < 0 && any0, leaf1 = nd.c1 < 0 && any1;
-    const bool g0 = nd.c0 >= 0 && any0, g1 = nd.c1 >= 0 && any1;
-    const int tri0 = ~nd.c0, tri1 = ~nd.c1;
-    // choose the next node before evaluating this node's triangles, so that its record load is in
-    // flight together with theirs
-    int next = -1;
-    if (g0 && g1) {
-      const bool near0 = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d0), centre) <=
-                         (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d1), centre);
-      if (sp < kStack) {
-        if (lane == 0) sstack[sp] = near0 ? nd.c1 : nd.c0;
+  };
+  for (int visits = 0; visits <= a.col.nnodes; ++visits) {
+    const Node4 nd = load_node4_uniform(nodes4, cur);
+    // per entry: does any lane reach its box, and the centre lane's distance to it (ordering key)
+    uint32_t key[4];
+    int ch[4];
+    bool leaf[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float d = box_d2(q[0], nd.lo[e], nd.hi[e]);
+      bool need = d <= best[0];
+      for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= best[k];
+      const bool any = e < nd.count && __ballot(need) != 0;
+      const int c = nd.child[e];
+      leaf[e] = any && c < 0;
+      ch[e] = (any && c >= 0) ? c : -1;
+      key[e] = ch[e] >= 0 ? (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d), centre) : 0xffffffffu;
+    }
+    if (COUNT) wc.box += 4 * V;
+    // nearest internal entry next, the others pushed farthest first (sorting network on 4)
+    cx(key[0], ch[0], key[1], ch[1]);
+    cx(key[2], ch[2], key[3], ch[3]);
+    cx(key[0], ch[0], key[2], ch[2]);
+    cx(key[1], ch[1], key[3], ch[3]);
+    cx(key[1], ch[1], key[2], ch[2]);
+#pragma unroll
+    for (int e = 3; e >= 1; --e)
+      if (ch[e] >= 0 && sp < kStack4) {
+        if (lane == 0) sstack[sp] = ch[e];
         ++sp;
       }
-      next = near0 ? nd.c0 : nd.c1;
-    } else if (g0) {
-      next = nd.c0;
-    } else if (g1) {
-      next = nd.c1;
-    } else if (sp > 0) {
-      --sp;
-      __builtin_amdgcn_wave_barrier();
-      next = __builtin_amdgcn_readfirstlane(sstack[sp]);
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (leaf[e]) for_leaf_code(~nd.child[e], eval);
+    if (ch[0] >= 0) {
+      cur = ch[0];
+      continue;
     }
-    if (next >= 0) nd = load_node_uniform(nodes, next);
-    auto eval = [&](int l) {
-      const TriRec t = load_trirec_uniform(recs, l);
-      for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
-      if (COUNT) wc.tri += V;
-    };
-    if (leaf0) for_leaf_code(tri0, eval);
-    if (leaf1) for_leaf_code(tri1, eval);
-    if (next < 0) break;
+    if (sp == 0) break;
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
   }
   if (COUNT) {
     int nv = 0;
@@ -1119,7 +1181,7 @@ __device__ __constant__ const int kShapes[kNumShapes][3] = {{4, 4, 1}, {8, 4, 1}
 template <bool COUNT, bool SLIVERS, bool CAPS, int V>
 __global__ __launch_bounds__(256, 8) void k_sdf_wave(SdfArgs a, int shape, const uint32_t *__restrict__ order,
                                                   uint32_t tx_n, uint32_t ty_n) {
-  __shared__ int sstack[4][kStack];
+  __shared__ int sstack[4][kStack4];
   const int wv = threadIdx.x >> 6;
   const int sx = kShapes[shape][0], sy = kShapes[shape][1], sz = 64 * V / (sx * sy);
   const uint32_t tile = order ? order[blockIdx.x] : blockIdx.x;
@@ -1192,7 +1254,7 @@ __device__ __forceinline__ int job_of_tile(const BatchArgs &t, uint32_t tile) {
 // (tile_prefix[j] = first tile of job j) until the queue is empty.
 template <bool COUNT, bool SLIVERS, bool CAPS, int V>
 __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
-  __shared__ int sstack[4][kStack];
+  __shared__ int sstack[4][kStack4];
   __shared__ uint32_t s_tile;
   const int wv = threadIdx.x >> 6;
   const int sx = kShapes[t.shape][0], sy = kShapes[t.shape][1], sz = 64 * V / (sx * sy);
@@ -1260,14 +1322,16 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
   k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, recs, sc);
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
-                  float *root_box, Scalars *sc, const int2 *ranges, int block, hipStream_t s) {
+                  float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
+    k_build4<<<1, 256, 0, s>>>(nodes, 1, nodes4);
     return;
   }
   k_refit<<<nb(n), 256, 0, s>>>(tris, n, nodes, leaf_parent, visits, root_box);
   k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
   if (block > 1) k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
+  k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1, nodes4);
 }
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
