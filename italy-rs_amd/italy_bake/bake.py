@@ -207,13 +207,17 @@ class Context:
         """Run one counting bake and return the traversal counters of the query kernel."""
         d_out = self.alloc(4 * sdf_grid.nx * sdf_grid.ny * sdf_grid.nz)
         try:
+            # the algorithm's own work: one traversal per voxel over the binary LBVH with single-
+            # triangle leaves (no wave union, no leaf blocks)
             self.set_option(OPT_COUNT_WORK, 1)
-            self.set_option(OPT_QUERY_VARIANT, 0)  # per-voxel traversal: the algorithm's own work
+            self.set_option(OPT_QUERY_VARIANT, 0)
+            self.set_option(OPT_LEAF_BLOCK, 1)
             self.bake_dense_device(d_xyz, ntri, sdf_grid, d_out)
             st = self.stats()
         finally:
             self.set_option(OPT_COUNT_WORK, 0)
             self.set_option(OPT_QUERY_VARIANT, 1)
+            self.set_option(OPT_LEAF_BLOCK, 4)
             self.free(d_out)
         return {"box_tests": st["box_tests"], "tri_tests": st["tri_tests"]}
 
