@@ -49,6 +49,9 @@ enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_COUN
 struct ib_context {
   int device = 0;
   hipStream_t stream = nullptr;
+  static constexpr int kSide = 4;   // streams a batch builds its meshes on, joined before the query
+  hipStream_t side[kSide] = {};
+  hipEvent_t fork = nullptr, join[kSide] = {};
   uint64_t budget = 0;
   uint64_t allocated = 0;
   std::unordered_map<std::string, Buf> bufs;
@@ -387,16 +390,17 @@ ib_status take(ib_context *c, Arena *arena, const char *name, size_t count, T **
 
 // weld by exact position and count unmatched directed edges into sc; boundary edges land in the
 // "e_bnd" scratch (valid until the next mesh is welded)
-ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, int2 **bnd, hipStream_t s) {
+ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, int2 **bnd, hipStream_t s,
+                       Arena *arena = nullptr) {
   const uint32_t vcap = pow2_at_least(2ull * 3 * n), ecap = pow2_at_least(2ull * 3 * n);
   int32_t *slots, *minid, *cslot, *enet;
   unsigned long long *ekeys;
-  IB_TRY(c->scratch("w_slots", vcap, &slots));
-  IB_TRY(c->scratch("w_minid", vcap, &minid));
-  IB_TRY(c->scratch("w_cslot", 3ull * n, &cslot));
-  IB_TRY(c->scratch("e_keys", ecap, &ekeys));
-  IB_TRY(c->scratch("e_net", ecap, &enet));
-  IB_TRY(c->scratch("e_bnd", 3ull * n, bnd));
+  IB_TRY(take(c, arena, "w_slots", vcap, &slots));
+  IB_TRY(take(c, arena, "w_minid", vcap, &minid));
+  IB_TRY(take(c, arena, "w_cslot", 3ull * n, &cslot));
+  IB_TRY(take(c, arena, "e_keys", ecap, &ekeys));
+  IB_TRY(take(c, arena, "e_net", ecap, &enet));
+  IB_TRY(take(c, arena, "e_bnd", 3ull * n, bnd));
   fill_u32(reinterpret_cast<uint32_t *>(slots), 0xffffffffu, vcap, s);
   fill_u32(reinterpret_cast<uint32_t *>(minid), 0x7fffffffu, vcap, s);
   fill_u32(reinterpret_cast<uint32_t *>(ekeys), 0xffffffffu, 2ull * ecap, s);
@@ -405,26 +409,26 @@ ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, in
   return IB_OK;
 }
 
-// ingest + LBVH of one mesh whose soup is on the device.  Resident outputs (nodes, triangle
-// records) come from `arena` when given, else from the context scratch; temporaries always from
-// scratch (reuse is ordered by the stream).  Nothing is read back here.
+// ingest + LBVH of one mesh whose soup is on the device.  Outputs and temporaries come from `arena`
+// when given (a batch builds its meshes on several streams at once, so each needs its own), else
+// from the context scratch (reuse is ordered by the stream).  Nothing is read back here.
 ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
                         hipStream_t s) {
   fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
   fill_u32(sc->cmin, 0xffffffffu, 3, s);
   int2 *bnd;
-  IB_TRY(enqueue_weld(c, d_xyz, n, sc, &bnd, s));
+  IB_TRY(enqueue_weld(c, d_xyz, n, sc, &bnd, s, arena));
   if (!arena) c->record(EV_INGEST, s);
   uint32_t *codes_in, *codes, *visits;
   int32_t *idx, *order, *leaf_parent;
   float *root_box;
-  IB_TRY(c->scratch("codes_in", n, &codes_in));
-  IB_TRY(c->scratch("codes", n, &codes));
-  IB_TRY(c->scratch("idx", n, &idx));
-  IB_TRY(c->scratch("order", n, &order));
-  IB_TRY(c->scratch("leaf_parent", n, &leaf_parent));
-  IB_TRY(c->scratch("visits", n, &visits));
-  IB_TRY(c->scratch("root_box", 8, &root_box));
+  IB_TRY(take(c, arena, "codes_in", n, &codes_in));
+  IB_TRY(take(c, arena, "codes", n, &codes));
+  IB_TRY(take(c, arena, "idx", n, &idx));
+  IB_TRY(take(c, arena, "order", n, &order));
+  IB_TRY(take(c, arena, "leaf_parent", n, &leaf_parent));
+  IB_TRY(take(c, arena, "visits", n, &visits));
+  IB_TRY(take(c, arena, "root_box", 8, &root_box));
   IB_TRY(take(c, arena, "nodes", std::max(n - 1, 1), &m.nodes));
   IB_TRY(take(c, arena, "tris", 3ull * n, &m.tris));
   IB_TRY(take(c, arena, "trirecs", (size_t)n, &m.recs));
@@ -434,10 +438,10 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   size_t temp_bytes = 0;
   IB_HIP(sort_pairs(nullptr, temp_bytes, codes_in, codes, idx, order, n, s));
   char *temp;
-  IB_TRY(c->scratch("sort_temp", temp_bytes, &temp));
+  IB_TRY(take(c, arena, "sort_temp", temp_bytes, &temp));
   IB_HIP(sort_pairs(temp, temp_bytes, codes_in, codes, idx, order, n, s));
   int2 *ranges;
-  IB_TRY(c->scratch("node_ranges", std::max(n - 1, 1), &ranges));
+  IB_TRY(take(c, arena, "node_ranges", std::max(n - 1, 1), &ranges));
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits, 0u, n, s);
@@ -655,6 +659,17 @@ ib_status ib_create(int device, uint64_t budget, ib_context **out) {
     return fail(IB_ERR_DEVICE, "stream / pinned allocation failed");
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  for (int i = 0; i < ib_context::kSide; ++i) {
+    if (hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming) != hipSuccess) {
+      ib_destroy(c);
+      return fail(IB_ERR_DEVICE, "stream creation failed");
+    }
+  }
+  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) {
+    ib_destroy(c);
+    return fail(IB_ERR_DEVICE, "event creation failed");
+  }
   hipDeviceProp_t prop;
   c->cus = hipGetDeviceProperties(&prop, device) == hipSuccess ? prop.multiProcessorCount : 256;
   *out = c;
@@ -671,6 +686,12 @@ void ib_destroy(ib_context *c) {
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   (void)hipHostFree(c->host_sc);
   if (c->host_batch_sc) (void)hipHostFree(c->host_batch_sc);
+  for (int i = 0; i < ib_context::kSide; ++i) {
+    if (c->side[i]) (void)hipStreamSynchronize(c->side[i]);
+    if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
+    if (c->join[i]) (void)hipEventDestroy(c->join[i]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1030,15 +1051,23 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   IB_TRY(arena.get(std::max(njobs, 1u), &scs));
   std::vector<MeshDev> mesh(njobs);
   c->record(EV_START, s);
-  // 1. ingest + LBVH of every mesh, occupancy, then one readback of all scalars
+  // 1. ingest + LBVH of every mesh and occupancy, spread over the side streams (small meshes leave
+  //    most of the GPU idle one at a time), joined into s; then one readback of all scalars
+  IB_HIP(hipEventRecord(c->fork, s));
+  for (int i = 0; i < ib_context::kSide; ++i) IB_HIP(hipStreamWaitEvent(c->side[i], c->fork, 0));
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
+    hipStream_t sj = c->side[j % ib_context::kSide];
     if (jb.ntri == 0) {
-      fill_u32(reinterpret_cast<uint32_t *>(scs + j), 0u, sizeof(Scalars) / 4, s);
+      fill_u32(reinterpret_cast<uint32_t *>(scs + j), 0u, sizeof(Scalars) / 4, sj);
       continue;
     }
-    IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, s));
-    if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, s);
+    IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj));
+    if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
+  }
+  for (int i = 0; i < ib_context::kSide; ++i) {
+    IB_HIP(hipEventRecord(c->join[i], c->side[i]));
+    IB_HIP(hipStreamWaitEvent(s, c->join[i], 0));
   }
   c->record(EV_BVH, s);
   IB_HIP(hipMemcpyAsync(c->host_batch_sc, scs, sizeof(Scalars) * njobs, hipMemcpyDeviceToHost, s));
