@@ -29,6 +29,10 @@ PEAK_HBM_GBS = 8000.0
 # dots x 5 + plane dot 5 + square and scale 2 = 82
 FLOP_BOX = 11
 FLOP_TRI = 82
+# FLOPs per cell of the glass cup's distance lambda (sdf_procedural.cpp:123-130; sqrt and divide = 1):
+# cell centre 3 x 3 = 9, three spheres 3 x (3 sub + 3 mul + 2 add + sqrt + sub) = 30, two plane
+# offsets p.y - h = 2, smooth intersect (b - a, * 0.5, / k, + 0.5, two 1 - h, three mul, add, mul, sub) = 12
+FLOP_CUP_CELL = 53
 
 
 def parse():
@@ -42,6 +46,7 @@ def parse():
     ap.add_argument("--batch-meshes", type=int, default=512, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--no-band", action="store_true", help="skip the narrow-band precision-stress bake")
     ap.add_argument("--no-consumer", action="store_true", help="skip the grid consumer (sampling / tracing)")
+    ap.add_argument("--no-analytic", action="store_true", help="skip the analytic CSG bake (glass cup at 320)")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="arrival window of the served workload on the batched mesh distribution (0: skip)")
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
@@ -140,6 +145,63 @@ def consumer_section(ctx, sdf, d_dist):
     return {"workload": "256^3 grid of the 327,680-triangle bake, device-resident",
             "samples_per_s": res["sample"], "sample_outputs": "value + B-spline normal + nearest cell",
             "rays_per_s": res["trace"], "rays_hit": int((t > 0).sum()), "rays": nray}
+
+
+def analytic_section(ctx, cpu=True, steps=20):
+    """The viewer's glass cup (sdf_procedural.cpp:108-140 at kGlassCupResolution 320, main.cpp:116):
+    324 x 264 x 324 cells of an analytic CSG program (two spheres, subtract, rim plane, smooth
+    intersect, two-branch select) into device-resident distances + branch bytes.  The reference
+    evaluates it on one host core; its notes put the bake at about 2 s of wall time
+    (humans.md:420-423).  The GPU grid is compared with the oracle's per-cell restatement."""
+    from italy_bake import procedural as pr
+
+    import torch
+
+    prog, bmin, bmax = pr.cup_program(-1.0, (0.0, 0.0), glass=True)
+    g = pr.analytic_framing(bmin, bmax, 320)
+    n = g.nx * g.ny * g.nz
+    d_dist, d_branch = ctx.alloc(4 * n), ctx.alloc(n)
+    stream = torch.cuda.Stream()  # default priority; the launches go to it so its events bracket them
+    try:
+        pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch, stream=stream.cuda_stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch, stream=stream.cuda_stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        wall_s = (time.perf_counter() - t0) / steps
+        dev_s = e0.elapsed_time(e1) * 1e-3 / steps
+        dist = ctx.download(d_dist, np.empty(n, np.float32))
+        branch = ctx.download(d_branch, np.empty(n, np.uint8))
+    finally:
+        ctx.free(d_dist)
+        ctx.free(d_branch)
+    t0 = time.perf_counter()
+    hd, hb = pr.bake_analytic(ctx, prog, g)
+    host_s = time.perf_counter() - t0
+    out = {"workload": f"glass cup, analytic CSG at res 320: {g.nx}x{g.ny}x{g.nz} = {n} cells, f32 distance + "
+                       "branch byte", "cells": n, "ms_per_bake": dev_s * 1e3, "cells_per_s": n / dev_s,
+           "ms_per_bake_wall": wall_s * 1e3, "ms_per_bake_with_host_copy": host_s * 1e3,
+           "gflop_per_bake": FLOP_CUP_CELL * n / 1e9,
+           "roofline": {"bound": "valu", "achieved": FLOP_CUP_CELL * n / dev_s / 1e12, "peak": PEAK_FP32_TFLOPS,
+                        "unit": "TFLOP/s", "frac": FLOP_CUP_CELL * n / dev_s / 1e12 / PEAK_FP32_TFLOPS},
+           "algorithmic_bytes_per_bake": 5 * n, "hbm_GBs_algorithmic": 5 * n / dev_s / 1e9}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        t0 = time.perf_counter()
+        _, _, cd, cb = ib_oracle.cup_cells(True, 320, -1.0, (0.0, 0.0))
+        cpu_s = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": n / cpu_s, "unit": "cells/s", "cores": 1, "kind": "port",
+                               "seconds": cpu_s, "gpu_multiple": cpu_s / dev_s,
+                               "sample": "the whole grid, the reference's lambda per cell on one core"}
+        out["matches_oracle"] = bool(np.array_equal(dist, cd.reshape(-1)) and np.array_equal(branch, cb.reshape(-1))
+                                     and np.array_equal(hd, dist) and np.array_equal(hb, branch))
+    return out
 
 
 def band_section(ctx, steps=3):
@@ -303,6 +365,7 @@ def main():
 
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None
+    analytic = analytic_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_analytic else None
     served = served_section(args, batched)
 
     cpu = None
@@ -328,7 +391,7 @@ def main():
                    "triangles": mesh.ntri, "sdf_res": sdf.nx, "occ_res": occ.nx, "parallelism": "single GPU"},
         "device_ms_per_bake": float(np.mean(total_ms)),
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band, "served": served,
-        "grid_consumer": consumer,
+        "grid_consumer": consumer, "analytic_bake": analytic,
         "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
         if served and served.get("batched_distribution") else None,
     }

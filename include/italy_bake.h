@@ -219,6 +219,44 @@ ib_status ib_server_device_result(ib_server *srv, uint64_t ticket, const float *
 ib_status ib_server_release(ib_server *srv, uint64_t ticket);
 ib_status ib_server_stats(ib_server *srv, ib_server_counters *out);
 
+/* ---- analytic SDF bake (src/convert/sdf_procedural.cpp) -------------------------------------------
+ * The reference bakes procedural content from a std::function (bakeAnalyticSdf / bakeAnalyticSdfPalette,
+ * sdf_procedural.cpp:46-79); here the function is a postfix CSG program over a stack of
+ * (distance, branch) values, evaluated on the device once per cell with the primitives and operators of
+ * src/convert/sdf_primitives.h. */
+typedef enum {
+  IB_CSG_SPHERE = 1,            /* push sdSphere(p - c, r):            c = a[0..2], r = a[3]            */
+  IB_CSG_BOX = 2,               /* push sdBox(p - c, b):               c = a[0..2], b = a[3..5]         */
+  IB_CSG_CYLINDER = 3,          /* push sdCylinder(p - c, r, h):       c = a[0..2], r = a[3], h = a[4]  */
+  IB_CSG_TORUS = 4,             /* push sdTorus(p - c, major, minor):  c = a[0..2], a[3], a[4]          */
+  IB_CSG_PLANE = 5,             /* push dot(n, p) - d:                 n = a[3..5], d = a[6]            */
+  IB_CSG_UNION = 10,            /* pop b, a; push std::min(a, b) (the operand chosen keeps its branch)  */
+  IB_CSG_SUBTRACT = 11,         /* std::max(a, -b), branch of a                                          */
+  IB_CSG_INTERSECT = 12,        /* std::max(a, b) (the operand chosen keeps its branch)                  */
+  IB_CSG_SMOOTH_UNION = 13,     /* opSmoothUnion(a, b, k = a[0]), branch of a                           */
+  IB_CSG_SMOOTH_INTERSECT = 14, /* opSmoothIntersect(a, b, k = a[0]), branch of a                       */
+  IB_CSG_SELECT = 15            /* a <= b ? a : b with its branch (the two-branch pick of the glass cup) */
+} ib_csg_opcode;
+
+typedef struct {
+  int32_t op;      /* ib_csg_opcode */
+  int32_t branch;  /* branch id of a primitive (0..255) */
+  float a[8];
+} ib_csg_op;
+
+/* Grid framing of the analytic bake (sdf_procedural.cpp:12-26): h = longest extent / max(res, 4),
+ * origin = bmin - 2h, n = ceil(extent / h) + 4 per axis. */
+ib_status ib_analytic_framing(const float bmin[3], const float bmax[3], int32_t res, ib_grid *out);
+/* Evaluate a CSG program (at most 256 instructions, stack depth <= 16, exactly one value left) at
+ * every cell centre of `grid`: d_distances (FP32) and, if not NULL, d_branch (one byte per cell).
+ * Enqueued on `stream` (NULL: the context's stream); programs of up to 64 instructions travel in the
+ * kernel arguments and the call returns without waiting, longer ones wait for their upload. */
+ib_status ib_bake_analytic_device(ib_context *ctx, const ib_csg_op *program, uint32_t nops, const ib_grid *grid,
+                                  float *d_distances, uint8_t *d_branch, void *stream);
+/* Host-buffer form. */
+ib_status ib_bake_analytic(ib_context *ctx, const ib_csg_op *program, uint32_t nops, const ib_grid *grid,
+                           float *distances, uint8_t *branch);
+
 /* ---- the grid consumer on the device ----------------------------------------------------------
  * What the renderer does with a baked SDF grid, run on device-resident bakes without a host round
  * trip (the renderer uploads SdfGrid to the device, src/render/optix_renderer.cpp:868-930).

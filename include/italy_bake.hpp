@@ -2,6 +2,8 @@
 //
 //   italy_bake::bakeSdf(mesh, resolution)       replaces italy::bakeSdf      (src/convert/sdf_baker.h:8)
 //   italy_bake::voxelizeMesh(mesh, resolution)  replaces italy::voxelizeMesh (src/convert/voxelize.h:8)
+//   italy_bake::makeCupSdf / makeGlassCupSdf / makeMaterialProbeSdfs, bakeAnalyticSdf(Palette)
+//                                               replace src/convert/sdf_procedural.h:28-40
 //
 // `Mesh` is any type shaped like italy::MeshAsset (src/io/mesh_asset.h:33-54): a contiguous
 // `positions` container of 3-float vectors (triangle soup, 3 per triangle) and `boundsMin` /
@@ -16,6 +18,7 @@
 
 #include <array>
 #include <cstdint>
+#include <initializer_list>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -40,6 +43,23 @@ inline ib_context *context() {
   return h.c;
 }
 
+enum class SdfMaterialKind : uint8_t { Opaque = 0, Dielectric = 1 };
+
+struct SdfPaletteMaterial {  // src/convert/sdf_grid.h:12-19
+  SdfMaterialKind kind = SdfMaterialKind::Opaque;
+  std::array<float, 3> baseColor{1.0f, 1.0f, 1.0f};
+  float metallic = 0.0f;
+  float roughness = 1.0f;
+  float ior = 1.0f;
+  std::array<float, 3> extinction{0.0f, 0.0f, 0.0f};
+};
+
+struct SdfSurfaceMaterial {  // src/convert/sdf_procedural.h:17-21
+  std::array<float, 3> baseColor{1.0f, 1.0f, 1.0f};
+  float metallic = 0.0f;
+  float roughness = 1.0f;
+};
+
 struct SdfGrid {  // src/convert/sdf_grid.h:21-36
   std::array<float, 3> origin{0.0f, 0.0f, 0.0f};
   float voxelSize = 1.0f;
@@ -48,6 +68,8 @@ struct SdfGrid {  // src/convert/sdf_grid.h:21-36
   std::vector<std::array<float, 3>> baseColor;
   std::vector<float> metallic;
   std::vector<float> roughness;
+  std::vector<uint8_t> branch;
+  std::vector<SdfPaletteMaterial> palette;
   size_t index(int x, int y, int z) const { return (static_cast<size_t>(z) * ny + y) * nx + x; }
   std::array<float, 3> boundsMax() const {
     return {origin[0] + nx * voxelSize, origin[1] + ny * voxelSize, origin[2] + nz * voxelSize};
@@ -158,6 +180,177 @@ VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution, ColorFn &&color) {
 template <class Mesh>
 VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution) {
   return voxelizeMesh(mesh, resolution, [](uint64_t, float *c) { c[0] = c[1] = c[2] = 1.0f; });
+}
+
+// ---- analytic SDF bake (src/convert/sdf_procedural.cpp) ------------------------------------------
+// The reference's SdfDistanceFn is a host std::function; here the distance is a CSG program
+// (ib_csg_op, include/italy_bake.h) evaluated on the device.  Csg builds one in postfix order.
+struct Csg {
+  std::vector<ib_csg_op> ops;
+  Csg &push(int op, int branch, std::initializer_list<float> a) {
+    ib_csg_op o{};
+    o.op = op;
+    o.branch = branch;
+    int i = 0;
+    for (float v : a) o.a[i++] = v;
+    ops.push_back(o);
+    return *this;
+  }
+  Csg &sphere(float cx, float cy, float cz, float r, int branch = 0) { return push(IB_CSG_SPHERE, branch, {cx, cy, cz, r}); }
+  Csg &box(float cx, float cy, float cz, float bx, float by, float bz, int branch = 0) {
+    return push(IB_CSG_BOX, branch, {cx, cy, cz, bx, by, bz});
+  }
+  Csg &cylinder(float cx, float cy, float cz, float r, float h, int branch = 0) {
+    return push(IB_CSG_CYLINDER, branch, {cx, cy, cz, r, h});
+  }
+  Csg &torus(float cx, float cy, float cz, float major, float minor, int branch = 0) {
+    return push(IB_CSG_TORUS, branch, {cx, cy, cz, major, minor});
+  }
+  Csg &plane(float nx, float ny, float nz, float d, int branch = 0) {
+    return push(IB_CSG_PLANE, branch, {0.0f, 0.0f, 0.0f, nx, ny, nz, d});
+  }
+  Csg &opUnion() { return push(IB_CSG_UNION, 0, {}); }
+  Csg &opSubtract() { return push(IB_CSG_SUBTRACT, 0, {}); }
+  Csg &opIntersect() { return push(IB_CSG_INTERSECT, 0, {}); }
+  Csg &opSmoothUnion(float k) { return push(IB_CSG_SMOOTH_UNION, 0, {k}); }
+  Csg &opSmoothIntersect(float k) { return push(IB_CSG_SMOOTH_INTERSECT, 0, {k}); }
+  Csg &select() { return push(IB_CSG_SELECT, 0, {}); }
+};
+
+inline SdfGrid analyticGrid(const float bmin[3], const float bmax[3], int resolution, const Csg &distance,
+                            bool withBranch) {
+  ib_grid g{};
+  check(ib_analytic_framing(bmin, bmax, resolution, &g), "ib_analytic_framing");
+  SdfGrid grid;
+  grid.origin = {g.origin[0], g.origin[1], g.origin[2]};
+  grid.voxelSize = g.h;
+  grid.nx = g.nx;
+  grid.ny = g.ny;
+  grid.nz = g.nz;
+  const size_t n = static_cast<size_t>(g.nx) * g.ny * g.nz;
+  grid.distances.resize(n);
+  grid.branch.resize(n);
+  check(ib_bake_analytic(context(), distance.ops.data(), static_cast<uint32_t>(distance.ops.size()), &g,
+                         grid.distances.data(), grid.branch.data()),
+        "ib_bake_analytic");
+  if (!withBranch) grid.branch.clear();
+  return grid;
+}
+
+// MaterialFn(branch) -> SdfSurfaceMaterial: per branch (sdf_procedural.cpp:46-64 passes the cell
+// position too; none of the reference's scenes reads it).
+template <class V, class MaterialFn>
+SdfGrid bakeAnalyticSdf(const V &boundsMin, const V &boundsMax, int resolution, const Csg &distance,
+                        MaterialFn &&material) {
+  float bmin[3], bmax[3];
+  bounds(boundsMin, boundsMax, bmin, bmax);
+  SdfGrid grid = analyticGrid(bmin, bmax, resolution, distance, true);
+  const size_t n = grid.distances.size();
+  grid.baseColor.resize(n);
+  grid.metallic.resize(n);
+  grid.roughness.resize(n);
+  SdfSurfaceMaterial table[256];
+  bool have[256] = {};
+  for (size_t i = 0; i < n; ++i) {
+    const uint8_t b = grid.branch[i];
+    if (!have[b]) {
+      table[b] = material(static_cast<int>(b));
+      have[b] = true;
+    }
+    grid.baseColor[i] = table[b].baseColor;
+    grid.metallic[i] = table[b].metallic;
+    grid.roughness[i] = table[b].roughness;
+  }
+  grid.branch.clear();
+  return grid;
+}
+
+template <class V, class PaletteFn>
+SdfGrid bakeAnalyticSdfPalette(const V &boundsMin, const V &boundsMax, int resolution, const Csg &distance,
+                               PaletteFn &&palette, int maxBranches) {  // sdf_procedural.cpp:66-79
+  float bmin[3], bmax[3];
+  bounds(boundsMin, boundsMax, bmin, bmax);
+  SdfGrid grid = analyticGrid(bmin, bmax, resolution, distance, true);
+  grid.palette.resize(static_cast<size_t>(maxBranches));
+  for (int b = 0; b < maxBranches; ++b) grid.palette[b] = palette(b);
+  return grid;
+}
+
+namespace detail {
+struct V3 {
+  float x, y, z;
+};
+struct CupShape {  // the constants of sdf_procedural.cpp:82-93 / :109-121, in the same float order
+  float R = 0.55f, wall = 0.09f, ri = R - wall, lift = 0.16f, rimFraction = 0.62f;
+  V3 oc, ic, bmin, bmax;
+  float rimY, fillY;
+  CupShape(float originY, float cx, float cz) {
+    oc = {cx, originY + R, cz};
+    ic = {oc.x + 0.0f, oc.y + lift, oc.z + 0.0f};
+    rimY = oc.y + R * rimFraction;
+    fillY = ic.y - ri + wall * 0.5f + 0.42f * (2.0f * ri);
+    bmin = {oc.x - R, oc.y - R, oc.z - R};
+    bmax = {oc.x + R, rimY, oc.z + R};
+  }
+  Csg shell() const {  // opIntersect(opSubtract(outer, inner), p.y - rimY)
+    return Csg().sphere(oc.x, oc.y, oc.z, R).sphere(ic.x, ic.y, ic.z, ri).opSubtract().plane(0, 1, 0, rimY).opIntersect();
+  }
+};
+}  // namespace detail
+
+inline SdfGrid makeCupSdf(int resolution, float originY, float centerX = 0.0f, float centerZ = 0.0f) {
+  const detail::CupShape c(originY, centerX, centerZ);  // sdf_procedural.cpp:81-106
+  return bakeAnalyticSdf(c.bmin, c.bmax, resolution, c.shell(),
+                         [](int) { return SdfSurfaceMaterial{{0.85f, 0.83f, 0.78f}, 0.0f, 0.35f}; });
+}
+template <class V2>
+SdfGrid makeCupSdf(int resolution, float originY, const V2 &centerXZ) {
+  return makeCupSdf(resolution, originY, centerXZ.x, centerXZ.y);
+}
+
+inline SdfGrid makeGlassCupSdf(int resolution, float originY, float centerX = 0.0f, float centerZ = 0.0f) {
+  const detail::CupShape c(originY, centerX, centerZ);  // sdf_procedural.cpp:108-140
+  Csg d = c.shell();
+  d.sphere(c.ic.x, c.ic.y, c.ic.z, c.ri, 1).plane(0, 1, 0, c.fillY, 1).opSmoothIntersect(0.012f).select();
+  return bakeAnalyticSdfPalette(c.bmin, c.bmax, resolution, d, [](int branch) {
+    if (branch == 0)
+      return SdfPaletteMaterial{SdfMaterialKind::Dielectric, {1.0f, 1.0f, 1.0f}, 0.0f, 0.05f, 1.47f, {0.02f, 0.02f, 0.018f}};
+    return SdfPaletteMaterial{SdfMaterialKind::Dielectric, {1.0f, 1.0f, 1.0f}, 0.0f, 0.02f, 1.333f, {0.9f, 0.35f, 0.12f}};
+  }, 2);
+}
+template <class V2>
+SdfGrid makeGlassCupSdf(int resolution, float originY, const V2 &centerXZ) {
+  return makeGlassCupSdf(resolution, originY, centerXZ.x, centerXZ.y);
+}
+
+inline std::vector<SdfGrid> makeMaterialProbeSdfs(int resolution) {  // sdf_procedural.cpp:142-193
+  const float radius = 0.4f;
+  auto sphereGrid = [&](detail::V3 c, const SdfPaletteMaterial &mat) {
+    const detail::V3 lo{c.x - radius, c.y - radius, c.z - radius}, hi{c.x + radius, c.y + radius, c.z + radius};
+    return bakeAnalyticSdfPalette(lo, hi, resolution, Csg().sphere(c.x, c.y, c.z, radius),
+                                  [&](int) { return mat; }, 1);
+  };
+  std::vector<SdfGrid> grids;
+  grids.reserve(9);
+  const float row1X[5] = {-2.2f, -1.1f, 0.0f, 1.1f, 2.2f};
+  const float row1Roughness[5] = {0.05f, 0.3f, 0.55f, 0.8f, 1.0f};
+  for (int i = 0; i < 5; ++i) {
+    SdfPaletteMaterial m;
+    m.baseColor = {0.7f, 0.7f, 0.72f};
+    m.roughness = row1Roughness[i];
+    grids.push_back(sphereGrid({row1X[i], radius, -0.6f}, m));
+  }
+  const float row2X[4] = {-1.65f, -0.55f, 0.55f, 1.65f};
+  const float row2Ior[4] = {1.1f, 1.35f, 1.6f, 2.0f};
+  for (int i = 0; i < 4; ++i) {
+    SdfPaletteMaterial m;
+    m.kind = SdfMaterialKind::Dielectric;
+    m.roughness = 0.02f;
+    m.ior = row2Ior[i];
+    m.extinction = {0.05f, 0.05f, 0.05f};
+    grids.push_back(sphereGrid({row2X[i], radius, 0.6f}, m));
+  }
+  return grids;
 }
 
 }  // namespace italy_bake

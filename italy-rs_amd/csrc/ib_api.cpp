@@ -924,6 +924,96 @@ ib_status ib_memcpy_d2h(ib_context *c, void *dst, const void *src, uint64_t byte
   return IB_OK;
 }
 
+static_assert(sizeof(ib_csg_op) == sizeof(CsgOp), "CSG instruction layout");
+
+ib_status ib_analytic_framing(const float bmin[3], const float bmax[3], int32_t res, ib_grid *g) {
+  if (!bmin || !bmax || !g) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  // sdf_procedural.cpp:12-26
+  float ext[3];
+  for (int k = 0; k < 3; ++k) ext[k] = bmax[k] - bmin[k];
+  const float longest = std::max({ext[0], ext[1], ext[2]});
+  const float vs = longest / static_cast<float>(std::max(res, 4));
+  if (!(vs > 0.0f) || !std::isfinite(vs)) return fail(IB_ERR_INVALID_ARGUMENT, "empty or non-finite bounds");
+  for (int k = 0; k < 3; ++k) g->origin[k] = bmin[k] - 2.0f * vs;
+  g->nx = static_cast<int>(std::ceil(ext[0] / vs)) + 4;
+  g->ny = static_cast<int>(std::ceil(ext[1] / vs)) + 4;
+  g->nz = static_cast<int>(std::ceil(ext[2] / vs)) + 4;
+  g->h = vs;
+  g->reserved = 0;
+  return IB_OK;
+}
+
+static ib_status check_program(const ib_csg_op *prog, uint32_t nops) {
+  if (!prog || nops == 0) return fail(IB_ERR_INVALID_ARGUMENT, "empty CSG program");
+  if (nops > 256) return fail(IB_ERR_UNSUPPORTED, "CSG program longer than 256 instructions");
+  int depth = 0;
+  for (uint32_t i = 0; i < nops; ++i) {
+    const int op = prog[i].op;
+    if (op >= IB_CSG_SPHERE && op <= IB_CSG_PLANE) {
+      if (prog[i].branch < 0 || prog[i].branch > 255) return fail(IB_ERR_INVALID_ARGUMENT, "branch outside 0..255");
+      if (++depth > kCsgStack) return fail(IB_ERR_UNSUPPORTED, "CSG stack deeper than 16");
+    } else if (op >= IB_CSG_UNION && op <= IB_CSG_SELECT) {
+      if (depth < 2) return fail(IB_ERR_INVALID_ARGUMENT, "CSG operator without two operands");
+      --depth;
+    } else {
+      return fail(IB_ERR_INVALID_ARGUMENT, "unknown CSG opcode " + std::to_string(op));
+    }
+    for (float v : prog[i].a)
+      if (!std::isfinite(v)) return fail(IB_ERR_INVALID_ARGUMENT, "non-finite CSG parameter");
+  }
+  if (depth != 1) return fail(IB_ERR_INVALID_ARGUMENT, "CSG program must leave exactly one value");
+  return IB_OK;
+}
+
+// Programs of up to kCsgInline instructions go to the device inside the kernel arguments (the
+// launch copies them), so the call is fully asynchronous; longer ones are uploaded to a scratch
+// buffer and the call waits for that copy, since `prog` is pageable host memory.
+static ib_status bake_analytic(ib_context *c, const ib_csg_op *prog, uint32_t nops, const ib_grid *grid, float *d_dist,
+                               uint8_t *d_branch, hipStream_t s) {
+  const CsgOp *host = reinterpret_cast<const CsgOp *>(prog);
+  CsgOp *d_prog = nullptr;
+  if (nops > (uint32_t)kCsgInline) {
+    IB_TRY(c->scratch("csg_prog", nops, &d_prog));
+    IB_HIP(hipMemcpyAsync(d_prog, host, nops * sizeof(CsgOp), hipMemcpyHostToDevice, s));
+  }
+  launch_bake_csg(host, d_prog, (int)nops, to_args(*grid), d_dist, d_branch, s);
+  IB_HIP(hipGetLastError());
+  if (d_prog) IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
+ib_status ib_bake_analytic_device(ib_context *c, const ib_csg_op *prog, uint32_t nops, const ib_grid *grid,
+                                  float *d_dist, uint8_t *d_branch, void *stream) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_TRY(check_program(prog, nops));
+  if (!grid || !grid_ok(grid) || !d_dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  IB_TRY(bake_analytic(c, prog, nops, grid, d_dist, d_branch, s));
+  return IB_OK;
+}
+
+ib_status ib_bake_analytic(ib_context *c, const ib_csg_op *prog, uint32_t nops, const ib_grid *grid, float *dist,
+                           uint8_t *branch) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  IB_TRY(check_program(prog, nops));
+  if (!grid || !grid_ok(grid) || !dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const uint64_t n = (uint64_t)grid->nx * grid->ny * grid->nz;
+  float *d_dist;
+  uint8_t *d_branch = nullptr;
+  IB_TRY(c->scratch("an_dist", n, &d_dist));
+  if (branch) IB_TRY(c->scratch("an_branch", n, &d_branch));
+  IB_TRY(bake_analytic(c, prog, nops, grid, d_dist, d_branch, s));
+  IB_HIP(hipMemcpyAsync(dist, d_dist, n * sizeof(float), hipMemcpyDeviceToHost, s));
+  if (branch) IB_HIP(hipMemcpyAsync(branch, d_branch, n, hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
 static ib_status grid_view(const ib_grid *grid, const float *d_distances, GridView &v) {
   if (!grid || !d_distances) return fail(IB_ERR_INVALID_ARGUMENT, "null grid or distances");
   if (!grid_ok(grid)) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid");

@@ -147,4 +147,23 @@ void launch_sample_grid(const GridView &g, const float *pts, uint64_t n, float *
 void launch_trace_grid(const GridView &g, const float *org, const float *dir, uint64_t n, float tmin, float tmax,
                        float *t_hit, hipStream_t s);
 
+// ---- analytic bake (ib_csg.hip) ----
+// one instruction of a postfix CSG program (same layout as ib_csg_op of the C ABI)
+struct CsgOp {
+  int32_t op, branch;
+  float a[8];
+};
+constexpr int kCsgSphere = 1, kCsgBox = 2, kCsgCylinder = 3, kCsgTorus = 4, kCsgPlane = 5;
+constexpr int kCsgUnion = 10, kCsgSubtract = 11, kCsgIntersect = 12, kCsgSmoothUnion = 13,
+              kCsgSmoothIntersect = 14, kCsgSelect = 15;
+constexpr int kCsgStack = 16;
+constexpr int kCsgInline = 64;  // programs up to this long travel in the kernel arguments
+struct CsgInline {
+  CsgOp ops[kCsgInline];
+};
+// `host_prog` is read at launch when nops <= kCsgInline (copied into the kernel arguments);
+// longer programs are read from `dev_prog`, which must then hold a device copy.
+void launch_bake_csg(const CsgOp *host_prog, const CsgOp *dev_prog, int nops, const GridArgs &g, float *dist,
+                     uint8_t *branch, hipStream_t s);
+
 }  // namespace ib

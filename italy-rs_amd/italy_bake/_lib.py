@@ -22,6 +22,7 @@ EXPORTS = [
     "ib_bake_batch_device", "ib_autotune_batch", "ib_bake_band_device", "ib_server_create", "ib_server_destroy",
     "ib_server_last_error", "ib_server_submit", "ib_server_info", "ib_server_wait", "ib_server_drain",
     "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
+    "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
 ]
 
 
@@ -107,6 +108,9 @@ def load():
     lib.ib_memcpy_d2d.argtypes = [_vp, _vp, _vp, _u64]
     lib.ib_sample_grid_device.argtypes = [_vp, G, _vp, _vp, _u64, _vp, _vp, _vp, _vp]
     lib.ib_trace_grid_device.argtypes = [_vp, G, _vp, _vp, _vp, _u64, ctypes.c_float, ctypes.c_float, _vp, _vp]
+    lib.ib_analytic_framing.argtypes = [_f3, _f3, _i32, G]
+    lib.ib_bake_analytic_device.argtypes = [_vp, _vp, ctypes.c_uint32, G, _vp, _vp, _vp]
+    lib.ib_bake_analytic.argtypes = [_vp, _vp, ctypes.c_uint32, G, _vp, _vp]
     lib.ib_synchronize.argtypes = [_vp]
     lib.ib_last_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     lib.ib_set_option.argtypes = [_vp, ctypes.c_int32, ctypes.c_int64]

@@ -262,6 +262,8 @@ class SdfGrid:
         self.baseColor = np.empty((0, 3), np.float32)
         self.metallic = np.empty(0, np.float32)
         self.roughness = np.empty(0, np.float32)
+        self.branch = np.empty(0, np.uint8)
+        self.palette = []
 
     def index(self, x, y, z):
         return (z * self.ny + y) * self.nx + x

@@ -1347,3 +1347,176 @@ void orc_trace_grid(const float origin[3], float h, int nx, int ny, int nz, cons
   const gview g = {{origin[0], origin[1], origin[2]}, h, {nx, ny, nz}, dist};
   for (size_t i = 0; i < n; ++i) t_hit[i] = trace_grid(&g, org + 3 * i, dir + 3 * i, tmin, tmax);
 }
+
+/* ---- analytic SDF bake (src/convert/sdf_procedural.cpp, sdf_primitives.h) ------------------------ */
+
+static inline float glen3(float x, float y, float z) { return sqrtf((x * x + y * y) + z * z); }
+static inline float glen2(float x, float y) { return sqrtf(x * x + y * y); }
+
+/* sdf_primitives.h:10 */
+static float sd_sphere(float x, float y, float z, float r) { return glen3(x, y, z) - r; }
+/* :12-15; std::max({qx, qy, qz}) keeps the first largest */
+static float sd_box(float x, float y, float z, float bx, float by, float bz) {
+  const float qx = fabsf(x) - bx, qy = fabsf(y) - by, qz = fabsf(z) - bz;
+  float m = qx;
+  if (m < qy) m = qy;
+  if (m < qz) m = qz;
+  return glen3(smax(qx, 0.0f), smax(qy, 0.0f), smax(qz, 0.0f)) + smin(m, 0.0f);
+}
+/* :17-20 */
+static float sd_cylinder(float x, float y, float z, float r, float h) {
+  const float dx = fabsf(glen2(x, z)) - r, dy = fabsf(y) - h;
+  return smin(smax(dx, dy), 0.0f) + glen2(smax(dx, 0.0f), smax(dy, 0.0f));
+}
+/* :22-25 */
+static float sd_torus(float x, float y, float z, float major, float minor) {
+  return glen2(glen2(x, z) - major, y) - minor;
+}
+/* :31-34: glm::clamp = min(max(x, lo), hi); glm::mix(b, a, h) = b * (1 - h) + a * h */
+static float op_smooth_union(float a, float b, float k) {
+  const float h = smin(smax(0.5f + 0.5f * (b - a) / k, 0.0f), 1.0f);
+  return (b * (1.0f - h) + a * h) - k * h * (1.0f - h);
+}
+/* :36 */
+static float op_smooth_intersect(float a, float b, float k) { return -op_smooth_union(-a, -b, k); }
+
+float orc_csg_eval(const orc_csg_op *prog, int nops, const float p[3], int *branch) {
+  float sd[16];
+  int sb[16];
+  int sp = 0;
+  for (int i = 0; i < nops; ++i) {
+    const orc_csg_op *o = &prog[i];
+    const float *a = o->a;
+    const float x = p[0] - a[0], y = p[1] - a[1], z = p[2] - a[2];
+    float d;
+    switch (o->op) {
+      case 1: d = sd_sphere(x, y, z, a[3]); break;
+      case 2: d = sd_box(x, y, z, a[3], a[4], a[5]); break;
+      case 3: d = sd_cylinder(x, y, z, a[3], a[4]); break;
+      case 4: d = sd_torus(x, y, z, a[3], a[4]); break;
+      case 5: d = ((a[3] * p[0] + a[4] * p[1]) + a[5] * p[2]) - a[6]; break;
+      default: {
+        if (sp < 2) return NAN;
+        const float db = sd[--sp];
+        const int bb = sb[sp];
+        const float da = sd[sp - 1];
+        const int ba = sb[sp - 1];
+        float r = da;
+        int rb = ba;
+        switch (o->op) {
+          case 10: if (db < da) { r = db; rb = bb; } break;         /* std::min (:27) */
+          case 11: r = smax(da, -db); break;                          /* std::max(a, -b) (:29) */
+          case 12: if (da < db) { r = db; rb = bb; } break;          /* std::max (:31) */
+          case 13: r = op_smooth_union(da, db, a[0]); break;
+          case 14: r = op_smooth_intersect(da, db, a[0]); break;
+          case 15: if (!(da <= db)) { r = db; rb = bb; } break;      /* sdf_procedural.cpp:129 */
+          default: return NAN;
+        }
+        sd[sp - 1] = r;
+        sb[sp - 1] = rb;
+        continue;
+      }
+    }
+    if (sp >= 16) return NAN;
+    sd[sp] = d;
+    sb[sp] = o->branch;
+    ++sp;
+  }
+  if (sp != 1) return NAN;
+  if (branch) *branch = sb[0];
+  return sd[0];
+}
+
+/* sdf_procedural.cpp:12-26 */
+void orc_analytic_framing(const float bmin[3], const float bmax[3], int res, float origin[3], float *h, int32_t n[3]) {
+  const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
+  float longest = ex;
+  if (longest < ey) longest = ey;
+  if (longest < ez) longest = ez;
+  const float vs = longest / (float)(res > 4 ? res : 4);
+  *h = vs;
+  origin[0] = bmin[0] - 2.0f * vs;
+  origin[1] = bmin[1] - 2.0f * vs;
+  origin[2] = bmin[2] - 2.0f * vs;
+  n[0] = (int32_t)ceilf(ex / vs) + 4;
+  n[1] = (int32_t)ceilf(ey / vs) + 4;
+  n[2] = (int32_t)ceilf(ez / vs) + 4;
+}
+
+typedef struct {
+  const orc_csg_op *prog;
+  int nops;
+  const float *origin;
+  float h;
+  int nx, ny, nz, z0, z1;
+  float *dist;
+  uint8_t *branch;
+} csg_slab;
+
+static void *csg_slab_run(void *arg) {
+  const csg_slab *s = (const csg_slab *)arg;
+  for (int z = s->z0; z < s->z1; ++z)
+    for (int y = 0; y < s->ny; ++y)
+      for (int x = 0; x < s->nx; ++x) {
+        /* origin + (vec3(x, y, z) + 0.5) * voxelSize (:32-35) */
+        const float p[3] = {s->origin[0] + ((float)x + 0.5f) * s->h, s->origin[1] + ((float)y + 0.5f) * s->h,
+                            s->origin[2] + ((float)z + 0.5f) * s->h};
+        int b = 0;
+        const size_t i = ((size_t)z * s->ny + y) * s->nx + x;
+        s->dist[i] = orc_csg_eval(s->prog, s->nops, p, &b);
+        if (s->branch) s->branch[i] = (uint8_t)b;
+      }
+  return NULL;
+}
+
+void orc_bake_csg(const orc_csg_op *prog, int nops, const float origin[3], float h, int nx, int ny, int nz,
+                  int threads, float *dist, uint8_t *branch) {
+  if (threads < 1) threads = 1;
+  if (threads > 64) threads = 64;
+  pthread_t tid[64];
+  csg_slab sl[64];
+  for (int t = 0; t < threads; ++t) {
+    sl[t] = (csg_slab){prog, nops, origin, h, nx, ny, nz, (int)((int64_t)nz * t / threads),
+                       (int)((int64_t)nz * (t + 1) / threads), dist, branch};
+    if (threads > 1) pthread_create(&tid[t], NULL, csg_slab_run, &sl[t]);
+    else csg_slab_run(&sl[t]);
+  }
+  if (threads > 1)
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
+/* makeCupSdf / makeGlassCupSdf (sdf_procedural.cpp:81-140), the lambdas written out per cell */
+void orc_cup_cells(int glass, int res, float origin_y, float cx, float cz, float origin[3], float *h, int32_t n[3],
+                   float *dist, uint8_t *branch) {
+  const float R = 0.55f, wall = 0.09f, ri = R - wall, lift = 0.16f, rim_frac = 0.62f, k = 0.012f;
+  const float oc[3] = {cx, origin_y + R, cz};
+  const float ic[3] = {oc[0] + 0.0f, oc[1] + lift, oc[2] + 0.0f};
+  const float rim_y = oc[1] + R * rim_frac;
+  const float fill_y = ic[1] - ri + wall * 0.5f + 0.42f * (2.0f * ri);
+  const float bmin[3] = {oc[0] - R, oc[1] - R, oc[2] - R};
+  const float bmax[3] = {oc[0] + R, rim_y, oc[2] + R};
+  orc_analytic_framing(bmin, bmax, res, origin, h, n);
+  if (!dist) return;
+  const float vs = *h;
+  for (int z = 0; z < n[2]; ++z)
+    for (int y = 0; y < n[1]; ++y)
+      for (int x = 0; x < n[0]; ++x) {
+        const float p[3] = {origin[0] + ((float)x + 0.5f) * vs, origin[1] + ((float)y + 0.5f) * vs,
+                            origin[2] + ((float)z + 0.5f) * vs};
+        const float d_outer = sd_sphere(p[0] - oc[0], p[1] - oc[1], p[2] - oc[2], R);
+        const float d_inner = sd_sphere(p[0] - ic[0], p[1] - ic[1], p[2] - ic[2], ri);
+        const float d_shell = smax(smax(d_outer, -d_inner), p[1] - rim_y);
+        float d = d_shell;
+        int b = 0;
+        if (glass) {
+          const float d_liquid = op_smooth_intersect(d_inner, p[1] - fill_y, k);
+          if (!(d_shell <= d_liquid)) {
+            d = d_liquid;
+            b = 1;
+          }
+        }
+        const size_t i = ((size_t)z * n[1] + y) * n[0] + x;
+        dist[i] = d;
+        if (branch) branch[i] = (uint8_t)b;
+      }
+}

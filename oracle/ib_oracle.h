@@ -147,6 +147,23 @@ void orc_sample_grid(const float origin[3], float h, int nx, int ny, int nz, con
 void orc_trace_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *org,
                     const float *dir, size_t n, float tmin, float tmax, float *t_hit);
 
+/* Analytic SDF bake (src/convert/sdf_procedural.cpp).  A CSG instruction is {int32 op, int32 branch,
+ * float a[8]} with the opcodes of include/italy_bake.h (ib_csg_opcode); orc_csg_eval evaluates a
+ * postfix program at one point with the primitives of sdf_primitives.h:10-36, orc_bake_csg every
+ * cell centre of a grid framed by orc_analytic_framing (sdf_procedural.cpp:12-26).
+ * orc_cup_cells / orc_glass_cup_cells restate makeCupSdf / makeGlassCupSdf (:81-140) as written,
+ * without a program: (grid, distances, branch), branch may be NULL. */
+typedef struct {
+  int32_t op, branch;
+  float a[8];
+} orc_csg_op;
+float orc_csg_eval(const orc_csg_op *prog, int nops, const float p[3], int *branch);
+void orc_analytic_framing(const float bmin[3], const float bmax[3], int res, float origin[3], float *h, int32_t n[3]);
+void orc_bake_csg(const orc_csg_op *prog, int nops, const float origin[3], float h, int nx, int ny, int nz,
+                  int threads, float *dist, uint8_t *branch);
+void orc_cup_cells(int glass, int res, float origin_y, float cx, float cz, float origin[3], float *h, int32_t n[3],
+                   float *dist, uint8_t *branch);
+
 #ifdef __cplusplus
 }
 #endif

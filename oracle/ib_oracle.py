@@ -93,6 +93,16 @@ def _load():
     lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
     lib.orc_udf_points_f32.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
     lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
+    lib.orc_csg_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _i32p]
+    lib.orc_csg_eval.restype = ctypes.c_float
+    lib.orc_analytic_framing.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, _f32p, _i32p]
+    lib.orc_analytic_framing.restype = None
+    lib.orc_bake_csg.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, _f32p, ctypes.c_void_p]
+    lib.orc_bake_csg.restype = None
+    lib.orc_cup_cells.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, _f32p,
+                                  _f32p, _i32p, _f32p, ctypes.c_void_p]
+    lib.orc_cup_cells.restype = None
     lib.orc_boundary_edge_count.restype = ctypes.c_int64
     return lib
 
@@ -192,6 +202,57 @@ def trace_grid(origin, h, dist, org, dirs, tmin=0.0, tmax=1e16):
     lib().orc_trace_grid(_p(_f32(origin), _f32p), float(h), nx, ny, nz, _p(dist, _f32p), _p(org, _f32p),
                          _p(dirs, _f32p), len(org), float(tmin), float(tmax), _p(t, _f32p))
     return t
+
+
+CSG_DTYPE = np.dtype([("op", "<i4"), ("branch", "<i4"), ("a", "<f4", (8,))])
+
+
+def _csg(prog):
+    return np.ascontiguousarray(prog, CSG_DTYPE)
+
+
+def csg_eval(prog, p):
+    """One point of a CSG program (sdf_primitives.h:10-36): (distance, branch); NaN if malformed."""
+    prog = _csg(prog)
+    b = ctypes.c_int32(-1)
+    d = lib().orc_csg_eval(prog.ctypes.data_as(ctypes.c_void_p), len(prog), _p(_f32(p), _f32p), ctypes.byref(b))
+    return np.float32(d), b.value
+
+
+def analytic_framing(bmin, bmax, res):
+    """bakeSdfGridShape (sdf_procedural.cpp:12-26): (origin, h, (nx, ny, nz))."""
+    o = np.zeros(3, np.float32)
+    h = np.zeros(1, np.float32)
+    n = np.zeros(3, np.int32)
+    lib().orc_analytic_framing(_p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p), int(res), _p(o, _f32p), _p(h, _f32p),
+                               _p(n, _i32p))
+    return o, np.float32(h[0]), tuple(int(v) for v in n)
+
+
+def bake_csg(prog, origin, h, nx, ny, nz, threads=1):
+    """Every cell centre of the grid (sdf_procedural.cpp:28-44): (distances (nz, ny, nx), branch)."""
+    prog = _csg(prog)
+    dist = np.empty((nz, ny, nx), np.float32)
+    branch = np.empty((nz, ny, nx), np.uint8)
+    lib().orc_bake_csg(prog.ctypes.data_as(ctypes.c_void_p), len(prog), _p(_f32(origin), _f32p), float(h), nx, ny,
+                       nz, int(threads), _p(dist, _f32p), branch.ctypes.data_as(ctypes.c_void_p))
+    return dist, branch
+
+
+def cup_cells(glass, res, origin_y=-1.0, center_xz=(0.0, 0.0)):
+    """makeCupSdf / makeGlassCupSdf (sdf_procedural.cpp:81-140) written out per cell, without a program:
+    (origin, h, distances (nz, ny, nx), branch)."""
+    o = np.zeros(3, np.float32)
+    h = np.zeros(1, np.float32)
+    n = np.zeros(3, np.int32)
+    lib().orc_cup_cells(int(glass), int(res), float(origin_y), float(center_xz[0]), float(center_xz[1]),
+                        _p(o, _f32p), _p(h, _f32p), _p(n, _i32p), None, None)
+    dist = np.empty((n[2], n[1], n[0]), np.float32)
+    branch = np.empty((n[2], n[1], n[0]), np.uint8)
+    lib().orc_cup_cells(int(glass), int(res), float(origin_y), float(center_xz[0]), float(center_xz[1]),
+                        _p(o, _f32p), _p(h, _f32p), _p(n, _i32p), _p(dist, _f32p),
+                        branch.ctypes.data_as(ctypes.c_void_p))
+    return o, np.float32(h[0]), dist, branch
 
 
 def tri_dist2_f32(q, tri):

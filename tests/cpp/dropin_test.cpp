@@ -1,11 +1,15 @@
 // The reference's two hot-path tests, run through the C++ drop-ins of include/italy_bake.hpp:
 //   tests/sdf_sign_test.cpp:46-82  (UV sphere r=1, 32x16, bakeSdf res 24, sign + magnitude)
 //   tests/voxelize_test.cpp:37-58  (unit cube shell, voxelizeMesh res 8, membership)
+// and the procedural scenes of src/convert/sdf_procedural.cpp:81-193 through the analytic drop-ins,
+// checked bit for bit against the oracle's per-cell restatement of the same lambdas.
 // Checks print a line and set a non-zero exit status instead of relying on assert.
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <vector>
 
+#include "ib_oracle.h"
 #include "italy_bake.hpp"
 
 namespace {
@@ -117,10 +121,50 @@ void voxelizeTest() {
 
 }  // namespace
 
+void cupTest(bool glass, float cx, float cz) {
+  const int res = 96;
+  const italy_bake::SdfGrid g =
+      glass ? italy_bake::makeGlassCupSdf(res, -1.0f, cx, cz) : italy_bake::makeCupSdf(res, -1.0f, cx, cz);
+  float o[3], h;
+  int32_t n[3];
+  orc_cup_cells(glass, res, -1.0f, cx, cz, o, &h, n, nullptr, nullptr);
+  const size_t cells = static_cast<size_t>(n[0]) * n[1] * n[2];
+  std::vector<float> d(cells);
+  std::vector<uint8_t> b(cells);
+  orc_cup_cells(glass, res, -1.0f, cx, cz, o, &h, n, d.data(), b.data());
+  expect(g.nx == n[0] && g.ny == n[1] && g.nz == n[2] && g.voxelSize == h, "cup framing");
+  expect(g.origin[0] == o[0] && g.origin[1] == o[1] && g.origin[2] == o[2], "cup origin");
+  expect(g.distances.size() == cells && std::memcmp(g.distances.data(), d.data(), cells * 4) == 0,
+         "cup distances bit-exact");
+  if (glass) {
+    expect(g.branch.size() == cells && std::memcmp(g.branch.data(), b.data(), cells) == 0, "glass cup branches");
+    expect(g.palette.size() == 2 && g.palette[0].ior == 1.47f && g.palette[1].ior == 1.333f, "glass cup palette");
+  } else {
+    expect(g.roughness.size() == cells && g.roughness[0] == 0.35f && g.baseColor[cells / 2][0] == 0.85f,
+           "cup material");
+  }
+}
+
+void probeTest() {
+  const std::vector<italy_bake::SdfGrid> grids = italy_bake::makeMaterialProbeSdfs(48);
+  expect(grids.size() == 9, "nine probes");
+  for (const auto &g : grids) {
+    // a sphere of radius 0.4 framed at 48: the cell nearest the centre is inside by about the radius
+    float lo = 1e30f;
+    for (float v : g.distances) lo = std::min(lo, v);
+    expect(g.nx == 52 && g.ny == 52 && g.nz == 52, "probe framing 48 + 4");
+    expect(lo < -0.38f && lo > -0.4f, "probe centre depth");
+  }
+  expect(grids[4].palette[0].roughness == 1.0f && grids[8].palette[0].ior == 2.0f, "probe palettes");
+}
+
 int main() {
   try {
     sdfSignTest();
     voxelizeTest();
+    cupTest(false, -1.6f, 1.6f);
+    cupTest(true, 0.0f, 0.0f);
+    probeTest();
   } catch (const std::exception &e) {
     std::printf("FAILED: exception %s\n", e.what());
     return 2;
