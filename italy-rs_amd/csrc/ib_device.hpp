@@ -188,22 +188,26 @@ __device__ __noinline__ double tri_d2_f64(F3 q, F3 p0, F3 p1, F3 p2) {
   return best;
 }
 
-// Distance record of one triangle (144 bytes), built once per mesh from the three vertices:
-//   e_i = p_{i+1} - p_i                       edge vectors (p_3 = p_0)
-//   n = cross(e_0, p_2 - p_0)                 plane normal (unnormalised), inv_nn = 1 / |n|^2
-//   nie_i = -1 / |e_i|^2                      segment parameter scale
-//   m_i = cross(n, e_i)                       in-plane normal of edge i, pointing into the triangle
-// with cross(a, b)_x = fma(a_y, b_z, -(a_z * b_y)) (and cyclic) and dot(a, b) = fma(a_z, b_z,
-// fma(a_y, b_y, a_x * b_x)).  The record is read at wave-uniform addresses (scalar loads).
+// Distance record of one triangle (96 bytes), built once per mesh from the three vertices: an
+// orthonormal frame at p0 in which the triangle is planar, with its corners p1 = (len, 0) and
+// p2 = (c0, c1), c1 > 0:
+//   e0 = p1 - p0, ac = p2 - p0, n = cross(e0, ac), len = sqrt(dot(e0, e0)), inv_len = 1 / len
+//   X = e0 * inv_len, Z = n * (1 / sqrt(dot(n, n))), Y = cross(Z, X)
+//   c = (dot(X, ac), dot(Y, ac)), e1 = (c0 - len, c1) = p2 - p1
+//   inv_e1 = 1 / |e1|^2, nie2 = -1 / |c|^2
+// with cross(a, b)_x = fma(a_y, b_z, -(a_z * b_y)) (and cyclic), dot(a, b) = fma(a_z, b_z,
+// fma(a_y, b_y, a_x * b_x)) and |v|^2 = fma(v_1, v_1, v_0 * v_0) in 2D.  A sliver (flag kSliver)
+// stores its three vertices in o, x, y instead, for the FP64 path.  The record is read at
+// wave-uniform addresses (scalar loads).
 struct alignas(16) TriRec {
-  float p[3][3];
-  float e[3][3];
-  float n[3], inv_nn;
-  float nie[3];
-  float m[3][3];
+  float o[3], x[3], y[3], z[3];
+  float len, inv_len;
+  float e1[2], inv_e1;
+  float c[2], nie2;
   uint32_t flags;
+  float pad[3];
 };
-static_assert(sizeof(TriRec) == 144, "distance record is 9 x 16 bytes");
+static_assert(sizeof(TriRec) == 96, "distance record is 6 x 16 bytes");
 
 __device__ __forceinline__ float fdot(F3 a, F3 b) { return fmaf(a.z, b.z, fmaf(a.y, b.y, a.x * b.x)); }
 __device__ __forceinline__ F3 fcross(F3 a, F3 b) {
@@ -211,47 +215,90 @@ __device__ __forceinline__ F3 fcross(F3 a, F3 b) {
 }
 
 __device__ __forceinline__ void make_trirec(F3 p0, F3 p1, F3 p2, uint32_t flags, TriRec &r) {
-  const F3 P[3] = {p0, p1, p2};
-  F3 E[3];
-  for (int i = 0; i < 3; ++i) {
-    const F3 a = P[i], b = P[i == 2 ? 0 : i + 1];
-    E[i] = {b.x - a.x, b.y - a.y, b.z - a.z};
-  }
-  const F3 ac = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
-  const F3 n = fcross(E[0], ac);
-  for (int i = 0; i < 3; ++i) {
-    r.p[i][0] = P[i].x; r.p[i][1] = P[i].y; r.p[i][2] = P[i].z;
-    r.e[i][0] = E[i].x; r.e[i][1] = E[i].y; r.e[i][2] = E[i].z;
-    r.nie[i] = -1.0f / fdot(E[i], E[i]);
-    const F3 m = fcross(n, E[i]);
-    r.m[i][0] = m.x; r.m[i][1] = m.y; r.m[i][2] = m.z;
-  }
-  r.n[0] = n.x; r.n[1] = n.y; r.n[2] = n.z;
-  r.inv_nn = 1.0f / fdot(n, n);
+  r = TriRec{};
   r.flags = flags;
+  if (flags & kSliver) {
+    const F3 P[3] = {p0, p1, p2};
+    float *dst[3] = {r.o, r.x, r.y};
+    for (int i = 0; i < 3; ++i) dst[i][0] = P[i].x, dst[i][1] = P[i].y, dst[i][2] = P[i].z;
+    return;
+  }
+  const F3 e0 = {p1.x - p0.x, p1.y - p0.y, p1.z - p0.z};
+  const F3 ac = {p2.x - p0.x, p2.y - p0.y, p2.z - p0.z};
+  const F3 n = fcross(e0, ac);
+  r.len = sqrtf(fdot(e0, e0));
+  r.inv_len = 1.0f / r.len;
+  const float inv_n = 1.0f / sqrtf(fdot(n, n));
+  const F3 X = {e0.x * r.inv_len, e0.y * r.inv_len, e0.z * r.inv_len};
+  const F3 Z = {n.x * inv_n, n.y * inv_n, n.z * inv_n};
+  const F3 Y = fcross(Z, X);
+  r.o[0] = p0.x, r.o[1] = p0.y, r.o[2] = p0.z;
+  r.x[0] = X.x, r.x[1] = X.y, r.x[2] = X.z;
+  r.y[0] = Y.x, r.y[1] = Y.y, r.y[2] = Y.z;
+  r.z[0] = Z.x, r.z[1] = Z.y, r.z[2] = Z.z;
+  r.c[0] = fdot(X, ac);
+  r.c[1] = fdot(Y, ac);
+  r.e1[0] = r.c[0] - r.len;
+  r.e1[1] = r.c[1];
+  r.inv_e1 = 1.0f / fmaf(r.e1[1], r.e1[1], r.e1[0] * r.e1[0]);
+  r.nie2 = -1.0f / fmaf(r.c[1], r.c[1], r.c[0] * r.c[0]);
 }
 
-// FP32 squared distance from q to a non-sliver triangle, branch free.  With a_i = p_i - q:
-//   segment i: t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = a_i + t e_i (fma), d_i = |r|^2
-//   face:      inside = max_i dot(m_i, a_i) <= 0, d_f = (dot(n, a_0)^2) * inv_nn
-//   result     min(d_0, d_1, d_2, inside ? d_f : inf)
-// |r|^2 = fma(r_z, r_z, fma(r_y, r_y, r_x * r_x)), the form of the box distance.
+__device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
+
+// FP32 squared distance from q to a non-sliver triangle, branch free.  With d = q - o and
+// (u, v, w) = (dot(X, d), dot(Y, d), dot(Z, d)):
+//   segment 0 (p0 p1): t = clamp(u * inv_len), r = fma(-t, len, u), d0 = fma(v, v, r * r)
+//   segment 1 (p1 p2): a = (u - len, v), t = clamp(dot(a, e1) * inv_e1), r = fma(-t, e1, a), d1 = |r|^2
+//   segment 2 (p2 p0): a = (u - c0, v - c1), t = clamp(dot(a, c) * nie2), r = fma(t, c, a), d2 = |r|^2
+//   inside = max(-v, -k1, k2) <= 0, k1 = fma(e1_0, v, -(e1_1 * a1_0)), k2 = fma(c0, a2_1, -(c1 * a2_0))
+//   result (inside ? 0 : min(d0, d1, d2)) + w * w
+// (2D dots as fma(a_1, b_1, a_0 * b_0)).
 __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
-  F3 a[3];
-  for (int i = 0; i < 3; ++i) a[i] = {t.p[i][0] - q.x, t.p[i][1] - q.y, t.p[i][2] - q.z};
-  float d = INFINITY;
-  for (int i = 0; i < 3; ++i) {
-    const F3 e = {t.e[i][0], t.e[i][1], t.e[i][2]};
-    const float u = fminf(fmaxf(fdot(a[i], e) * t.nie[i], 0.0f), 1.0f);
-    const F3 r = {fmaf(u, e.x, a[i].x), fmaf(u, e.y, a[i].y), fmaf(u, e.z, a[i].z)};
-    d = fminf(d, sq3(r.x, r.y, r.z));
-  }
-  const float k0 = fdot({t.m[0][0], t.m[0][1], t.m[0][2]}, a[0]);
-  const float k1 = fdot({t.m[1][0], t.m[1][1], t.m[1][2]}, a[1]);
-  const float k2 = fdot({t.m[2][0], t.m[2][1], t.m[2][2]}, a[2]);
-  const float s = fdot({t.n[0], t.n[1], t.n[2]}, a[0]);
-  const float df = (s * s) * t.inv_nn;
-  return fmaxf(fmaxf(k0, k1), k2) <= 0.0f ? fminf(d, df) : d;
+  const F3 d = {q.x - t.o[0], q.y - t.o[1], q.z - t.o[2]};
+  const float u = fdot({t.x[0], t.x[1], t.x[2]}, d);
+  const float v = fdot({t.y[0], t.y[1], t.y[2]}, d);
+  const float w = fdot({t.z[0], t.z[1], t.z[2]}, d);
+  const float t0 = clamp01(u * t.inv_len);
+  const float r0 = fmaf(-t0, t.len, u);
+  const float d0 = fmaf(v, v, r0 * r0);
+  const float a1 = u - t.len;
+  const float t1 = clamp01(fmaf(v, t.e1[1], a1 * t.e1[0]) * t.inv_e1);
+  const float s1x = fmaf(-t1, t.e1[0], a1), s1y = fmaf(-t1, t.e1[1], v);
+  const float d1 = fmaf(s1y, s1y, s1x * s1x);
+  const float a2x = u - t.c[0], a2y = v - t.c[1];
+  const float t2 = clamp01(fmaf(a2y, t.c[1], a2x * t.c[0]) * t.nie2);
+  const float s2x = fmaf(t2, t.c[0], a2x), s2y = fmaf(t2, t.c[1], a2y);
+  const float d2 = fmaf(s2y, s2y, s2x * s2x);
+  const float k1 = fmaf(t.e1[0], v, -(t.e1[1] * a1));
+  const float k2 = fmaf(t.c[0], a2y, -(t.c[1] * a2x));
+  const bool inside = fmaxf(fmaxf(-v, -k1), k2) <= 0.0f;
+  const float dmin = fminf(fminf(d0, d1), d2);
+  return (inside ? 0.0f : dmin) + w * w;
+}
+
+// Pruning with FP32 distances.  |sqrt(tri_d2_f32(q, t)) - dist(q, t)| stays below
+// 2^-17 (Mq + 3 Mp) when every coordinate of the query is at most Mq and every vertex coordinate
+// at most Mp in magnitude (the worst case measured over 4e6 triangles and queries across six
+// decades of scale and size is 4 u (Mq + 3 Mp), u = 2^-24).  A traversal enters a box when its
+// squared distance is at most prune_bound(best, slack): every triangle whose FP32 distance could
+// still beat `best` is then reached whatever the visiting order, so each traversal returns the
+// minimum over all triangles and the query kernels, the brick shapes and the oracle agree exactly.
+__device__ __forceinline__ float prune_bound(float best, float slack) {
+  const float r = __builtin_amdgcn_sqrtf(best) + slack;  // hardware root, within 1 ulp
+  return (r * r) * 1.0000077f;  // covers that root, these roundings and the box distance's
+}
+__device__ __forceinline__ float finite_abs(float x) { return fabsf(x) < INFINITY ? fabsf(x) : 0.0f; }
+// slack for the queries of grid g over a mesh whose boxes `lo`/`hi` (n of them, empty boxes
+// allowed) cover every vertex
+__device__ __forceinline__ float dist_slack(float ox, float oy, float oz, float h, int nx, int ny, int nz,
+                                            const float (*lo)[3], const float (*hi)[3], int n) {
+  const float mq = fmaxf(fmaxf(fmaxf(fabsf(ox), fabsf(ox + (float)nx * h)), fmaxf(fabsf(oy), fabsf(oy + (float)ny * h))),
+                         fmaxf(fabsf(oz), fabsf(oz + (float)nz * h)));
+  float mp = 0.0f;
+  for (int e = 0; e < n; ++e)
+    for (int k = 0; k < 3; ++k) mp = fmaxf(mp, fmaxf(finite_abs(lo[e][k]), finite_abs(hi[e][k])));
+  return (mq + 3.0f * mp) * 7.62939453125e-6f;  // 2^-17
 }
 
 // SLIVERS = false is only instantiated for meshes without sliver triangles (the host counts them
@@ -259,8 +306,7 @@ __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
 template <bool SLIVERS>
 __device__ __forceinline__ float tri_d2(F3 q, const TriRec &t) {
   if (SLIVERS && (t.flags & kSliver))
-    return (float)tri_d2_f64(q, {t.p[0][0], t.p[0][1], t.p[0][2]}, {t.p[1][0], t.p[1][1], t.p[1][2]},
-                             {t.p[2][0], t.p[2][1], t.p[2][2]});
+    return (float)tri_d2_f64(q, {t.o[0], t.o[1], t.o[2]}, {t.x[0], t.x[1], t.x[2]}, {t.y[0], t.y[1], t.y[2]});
   return tri_d2_f32(q, t);
 }
 

@@ -834,8 +834,9 @@ struct WorkCount {
 
 template <bool DEEP, bool COUNT, bool SLIVERS>
 __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const TriRec *__restrict__ recs,
-                                            F3 q, int *gstack, float *gstackd, int cap, WorkCount &wc) {
-  float best = INFINITY;
+                                            F3 q, float slack, int *gstack, float *gstackd, int cap,
+                                            WorkCount &wc) {
+  float best = INFINITY, bound = INFINITY;
   int lstack[DEEP ? 1 : kStack];
   float lstackd[DEEP ? 1 : kStack];
   int *st = DEEP ? gstack : lstack;
@@ -847,18 +848,22 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const float d0 = box_d2(q, nd.lo0, nd.hi0);
     const float d1 = box_d2(q, nd.lo1, nd.hi1);
     if (COUNT) wc.box += 2;
-    if (nd.c0 < 0 && d0 <= best)
+    if (nd.c0 < 0 && d0 <= bound) {
       for_leaf_code(~nd.c0, [&](int l) {
         best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
         if (COUNT) wc.tri++;
       });
-    if (nd.c1 < 0 && d1 <= best)
+      bound = prune_bound(best, slack);
+    }
+    if (nd.c1 < 0 && d1 <= bound) {
       for_leaf_code(~nd.c1, [&](int l) {
         best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
         if (COUNT) wc.tri++;
       });
-    const bool g0 = nd.c0 >= 0 && d0 <= best;
-    const bool g1 = nd.c1 >= 0 && d1 <= best;
+      bound = prune_bound(best, slack);
+    }
+    const bool g0 = nd.c0 >= 0 && d0 <= bound;
+    const bool g1 = nd.c1 >= 0 && d1 <= bound;
     if (g0 && g1) {
       const bool first0 = d0 <= d1;
       if (sp < cap) {
@@ -880,7 +885,7 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     cur = -1;
     while (sp > 0) {
       --sp;
-      if (sd[sp] <= best) {
+      if (sd[sp] <= bound) {
         cur = st[sp];
         break;
       }
@@ -892,6 +897,14 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
 
 template <bool CAPS>
 __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best);
+
+// distance slack of a query grid over a mesh (dist_slack), from the root's two child boxes
+__device__ __forceinline__ float query_slack(const ColumnArgs &c) {
+  const Node root = c.nodes[0];
+  const float lo[2][3] = {{root.lo0[0], root.lo0[1], root.lo0[2]}, {root.lo1[0], root.lo1[1], root.lo1[2]}};
+  const float hi[2][3] = {{root.hi0[0], root.hi0[1], root.hi0[2]}, {root.hi1[0], root.hi1[1], root.hi1[2]}};
+  return dist_slack(c.g.ox, c.g.oy, c.g.oz, c.g.h, c.g.nx, c.g.ny, c.g.nz, lo, hi, 2);
+}
 
 template <bool DEEP, bool COUNT, bool SLIVERS, bool CAPS>
 __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
@@ -910,7 +923,7 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
     gsd = a.gstackd + tid * a.gstack_depth;
   }
   WorkCount wc;
-  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.recs, q, gs, gsd,
+  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.recs, q, query_slack(a.col), gs, gsd,
                                                      DEEP ? a.gstack_depth : kStack, wc);
   if (COUNT) {
     atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
@@ -1059,7 +1072,7 @@ __device__ __forceinline__ Node4 load_node4_uniform(const Node4 *p, int i) {
 }
 
 union TriRecBits {
-  V4 v[9];
+  V4 v[6];
   TriRec r;
 };
 
@@ -1067,7 +1080,7 @@ __device__ __forceinline__ TriRec load_trirec_uniform(const TriRec *p, int i) {
   const CV4 *r = (const CV4 *)(p + i);
   TriRecBits u;
 #pragma unroll
-  for (int k = 0; k < 9; ++k) u.v[k] = r[k];
+  for (int k = 0; k < 6; ++k) u.v[k] = r[k];
   return u.r;
 }
 
@@ -1084,7 +1097,7 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   int z[V];
   bool valid[V];
   F3 q[V];
-  float best[V];
+  float best[V], bound[V];
   for (int k = 0; k < V; ++k) {
     z[k] = bz + lane / (sx * sy) + k * lz;
     valid[k] = x < g.nx && y < g.ny && z[k] < g.nz;
@@ -1092,7 +1105,9 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
     const int xs = valid[k] ? x : bx, ys = valid[k] ? y : by, zs = valid[k] ? z[k] : bz;
     q[k] = {g.ox + ((float)xs + 0.5f) * g.h, g.oy + ((float)ys + 0.5f) * g.h, g.oz + ((float)zs + 0.5f) * g.h};
     best[k] = INFINITY;
+    bound[k] = INFINITY;
   }
+  const float slack = query_slack(a.col);
   // the lane of voxel (sx/2, sy/2, lz/2) orders the children: its two box distances (first voxel)
   // are read with v_readlane and compared as integers (non-negative floats order like their bits)
   const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (lz >> 1) * sx * sy);
@@ -1126,8 +1141,8 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float d = box_d2(q[0], nd.lo[e], nd.hi[e]);
-      bool need = d <= best[0];
-      for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= best[k];
+      bool need = d <= bound[0];
+      for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= bound[k];
       const bool any = e < nd.count && __ballot(need) != 0;
       const int c = nd.child[e];
       leaf[e] = any && c < 0;
@@ -1147,9 +1162,15 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
         if (lane == 0) sstack[sp] = ch[e];
         ++sp;
       }
+    bool evaluated = false;
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (leaf[e]) for_leaf_code(~nd.child[e], eval);
+      if (leaf[e]) {
+        for_leaf_code(~nd.child[e], eval);
+        evaluated = true;
+      }
+    if (evaluated)
+      for (int k = 0; k < V; ++k) bound[k] = prune_bound(best[k], slack);
     if (ch[0] >= 0) {
       cur = ch[0];
       continue;

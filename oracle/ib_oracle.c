@@ -394,55 +394,77 @@ static inline double dot3d(const double a[3], const double b[3]) { return a[0] *
  * closest-point query behind the bake's distances (the reference finds the closest triangle with
  * OptiX, src/convert/sdf_baker.cpp:150-305, and its shaders are not part of the tree); the result is
  * pinned against the FP64 distance (orc_tri_dist2_f64) and closed forms in the oracle tests.
- *   e_i = p_{i+1} - p_i, n = cross(e_0, p_2 - p_0), inv_nn = 1/|n|^2, nie_i = -1/|e_i|^2,
- *   m_i = cross(n, e_i) (in-plane normal of edge i, pointing into the triangle);
+ * Record: e0 = p1 - p0, ac = p2 - p0, n = cross(e0, ac), len = sqrt(dot(e0, e0)), inv_len = 1 / len,
+ *   X = e0 * inv_len, Z = n * (1 / sqrt(dot(n, n))), Y = cross(Z, X), c = (dot(X, ac), dot(Y, ac)),
+ *   e1 = (c0 - len, c1), inv_e1 = 1 / (e1_1^2 + e1_0^2 as fmaf(e1_1, e1_1, e1_0 * e1_0)),
+ *   nie2 = -1 / fmaf(c1, c1, c0 * c0);
  *   cross(a, b)_x = fmaf(a_y, b_z, -(a_z * b_y)) (cyclic), dot(a, b) = fmaf(a_z, b_z, fmaf(a_y, b_y, a_x * b_x)).
- * Query, with a_i = p_i - q: segment i t = clamp(dot(a_i, e_i) * nie_i, 0, 1), r = fmaf(t, e_i, a_i),
- * d_i = fmaf(r_z, r_z, fmaf(r_y, r_y, r_x * r_x)); face d_f = (s * s) * inv_nn with s = dot(n, a_0), used when
- * max_i dot(m_i, a_i) <= 0; result min(d_0, d_1, d_2[, d_f]). */
+ * Query: d = q - o, (u, v, w) = (dot(X, d), dot(Y, d), dot(Z, d)) in the triangle's plane frame;
+ *   segment 0: t = clamp(u * inv_len), r = fmaf(-t, len, u), d0 = fmaf(v, v, r * r);
+ *   segment 1: a = (u - len, v), t = clamp(fmaf(v, e1_1, a_0 * e1_0) * inv_e1), r = fmaf(-t, e1, a), d1 = |r|^2;
+ *   segment 2: a = (u - c0, v - c1), t = clamp(fmaf(a_1, c1, a_0 * c0) * nie2), r = fmaf(t, c, a), d2 = |r|^2;
+ *   inside = max(-v, -k1, k2) <= 0 with k1 = fmaf(e1_0, v, -(e1_1 * a1_0)), k2 = fmaf(c0, a2_1, -(c1 * a2_0));
+ *   result (inside ? 0 : min(d0, d1, d2)) + w * w.  |r|^2 in 2D is fmaf(r_1, r_1, r_0 * r_0). */
 static inline float fdot3(const float a[3], const float b[3]) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
 static inline void fcross3(const float a[3], const float b[3], float o[3]) {
   o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
   o[1] = fmaf(a[2], b[0], -(a[0] * b[2]));
   o[2] = fmaf(a[0], b[1], -(a[1] * b[0]));
 }
+static inline float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
 void orc_trirec_make(const float *t, orc_trirec *r) {
-  for (int i = 0; i < 3; ++i) {
-    const float *a = t + 3 * i, *b = t + 3 * ((i + 1) % 3);
-    for (int k = 0; k < 3; ++k) {
-      r->p[i][k] = a[k];
-      r->e[i][k] = b[k] - a[k];
-    }
-  }
-  const float ac[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
-  fcross3(r->e[0], ac, r->n);
-  for (int i = 0; i < 3; ++i) {
-    r->nie[i] = -1.0f / fdot3(r->e[i], r->e[i]);
-    fcross3(r->n, r->e[i], r->m[i]);
-  }
-  r->inv_nn = 1.0f / fdot3(r->n, r->n);
+  memset(r, 0, sizeof(*r));
+  memcpy(r->v, t, 9 * sizeof(float));
   r->sliver = orc_tri_is_sliver(t);
+  if (r->sliver) return;
+  const float e0[3] = {t[3] - t[0], t[4] - t[1], t[5] - t[2]};
+  const float ac[3] = {t[6] - t[0], t[7] - t[1], t[8] - t[2]};
+  float n[3];
+  fcross3(e0, ac, n);
+  r->len = sqrtf(fdot3(e0, e0));
+  r->inv_len = 1.0f / r->len;
+  const float inv_n = 1.0f / sqrtf(fdot3(n, n));
+  for (int k = 0; k < 3; ++k) {
+    r->o[k] = t[k];
+    r->x[k] = e0[k] * r->inv_len;
+    r->z[k] = n[k] * inv_n;
+  }
+  fcross3(r->z, r->x, r->y);
+  r->c[0] = fdot3(r->x, ac);
+  r->c[1] = fdot3(r->y, ac);
+  r->e1[0] = r->c[0] - r->len;
+  r->e1[1] = r->c[1];
+  r->inv_e1 = 1.0f / fmaf(r->e1[1], r->e1[1], r->e1[0] * r->e1[0]);
+  r->nie2 = -1.0f / fmaf(r->c[1], r->c[1], r->c[0] * r->c[0]);
 }
 
 float orc_trirec_dist2(const float q[3], const orc_trirec *r) {
   if (r->sliver) {
     const double qd[3] = {q[0], q[1], q[2]};
-    return (float)orc_tri_dist2_f64(qd, &r->p[0][0]);
+    return (float)orc_tri_dist2_f64(qd, r->v);
   }
-  float a[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 3; ++k) a[i][k] = r->p[i][k] - q[k];
-  float d = INFINITY;
-  for (int i = 0; i < 3; ++i) {
-    const float u = fminf(fmaxf(fdot3(a[i], r->e[i]) * r->nie[i], 0.0f), 1.0f);
-    const float x = fmaf(u, r->e[i][0], a[i][0]), y = fmaf(u, r->e[i][1], a[i][1]), z = fmaf(u, r->e[i][2], a[i][2]);
-    d = fminf(d, fmaf(z, z, fmaf(y, y, x * x)));
-  }
-  const float k0 = fdot3(r->m[0], a[0]), k1 = fdot3(r->m[1], a[1]), k2 = fdot3(r->m[2], a[2]);
-  const float s = fdot3(r->n, a[0]);
-  const float df = (s * s) * r->inv_nn;
-  return fmaxf(fmaxf(k0, k1), k2) <= 0.0f ? fminf(d, df) : d;
+  const float d[3] = {q[0] - r->o[0], q[1] - r->o[1], q[2] - r->o[2]};
+  const float u = fdot3(r->x, d), v = fdot3(r->y, d), w = fdot3(r->z, d);
+  /* segment 0: p0 -> p1 along the X axis */
+  const float t0 = clamp01(u * r->inv_len);
+  const float r0 = fmaf(-t0, r->len, u);
+  const float d0 = fmaf(v, v, r0 * r0);
+  /* segment 1: p1 -> p2 */
+  const float a1 = u - r->len;
+  const float t1 = clamp01(fmaf(v, r->e1[1], a1 * r->e1[0]) * r->inv_e1);
+  const float s1x = fmaf(-t1, r->e1[0], a1), s1y = fmaf(-t1, r->e1[1], v);
+  const float d1 = fmaf(s1y, s1y, s1x * s1x);
+  /* segment 2: p2 -> p0 */
+  const float a2x = u - r->c[0], a2y = v - r->c[1];
+  const float t2 = clamp01(fmaf(a2y, r->c[1], a2x * r->c[0]) * r->nie2);
+  const float s2x = fmaf(t2, r->c[0], a2x), s2y = fmaf(t2, r->c[1], a2y);
+  const float d2 = fmaf(s2y, s2y, s2x * s2x);
+  const float k1 = fmaf(r->e1[0], v, -(r->e1[1] * a1));
+  const float k2 = fmaf(r->c[0], a2y, -(r->c[1] * a2x));
+  const int inside = fmaxf(fmaxf(-v, -k1), k2) <= 0.0f;
+  const float dmin = fminf(fminf(d0, d1), d2);
+  return (inside ? 0.0f : dmin) + w * w;
 }
 
 float orc_tri_dist2_f32(const float q[3], const float *t) {
@@ -821,12 +843,29 @@ typedef struct {
   const float *tris;       /* sorted soup, 9 floats */
   const orc_bvh *b;
   const orc_trirec *recs;  /* distance records of the sorted soup */
+  float mp;                /* largest |vertex coordinate| */
 } qctx;
 
 static orc_trirec *make_recs(const float *sorted, size_t ntri) {
   orc_trirec *r = (orc_trirec *)malloc(sizeof(orc_trirec) * (ntri ? ntri : 1));
   for (size_t i = 0; i < ntri; ++i) orc_trirec_make(sorted + 9 * i, r + i);
   return r;
+}
+
+static float max_abs_coord(const float *xyz, size_t ntri) {
+  float m = 0.0f;
+  for (size_t i = 0; i < 9 * ntri; ++i) m = fmaxf(m, fabsf(xyz[i]));
+  return m;
+}
+
+/* Pruning with FP32 distances: |sqrt(orc_trirec_dist2(q, t)) - dist(q, t)| < 2^-17 (Mq + 3 Mp) for
+ * |q_k| <= Mq and |vertex_k| <= Mp (tests/test_oracle.py measures the worst case, about 4 u (Mq + 3 Mp)).
+ * A box is entered when its squared distance is at most this bound on what can still beat `best`,
+ * so the traversal's result is the minimum over all triangles, whatever the order. */
+static inline float prune_bound(float best, const float q[3], float mp) {
+  const float mq = fmaxf(fmaxf(fabsf(q[0]), fabsf(q[1])), fabsf(q[2]));
+  const float r = sqrtf(best) + (mq + 3.0f * mp) * 7.62939453125e-6f;
+  return (r * r) * 1.0000077f;
 }
 
 static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t *tests) {
@@ -850,9 +889,10 @@ static float closest2(const qctx *c, const float q[3], uint64_t *nodes, uint64_t
       (*nodes)++;
     }
     const int first = dd[1] < dd[0] ? 1 : 0;
+    const float bound = prune_bound(best, q, c->mp);
     for (int o = 1; o >= 0; --o) { /* push far first so near is popped first */
       const int k = o ? 1 - first : first;
-      if (!(dd[k] <= best)) continue;
+      if (!(dd[k] <= bound)) continue;
       if (ch[k] < 0) {
         (*tests)++;
         const float d2 = orc_trirec_dist2(q, c->recs + (size_t)(~ch[k]));
@@ -982,7 +1022,7 @@ int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3],
   float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
   for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
   orc_trirec *recs = make_recs(sorted, ntri);
-  qctx c = {sorted, &b, recs};
+  qctx c = {sorted, &b, recs, max_abs_coord(sorted, ntri)};
   if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
   if (z0 < 0) z0 = 0;
   if (z1 > nz || z1 < 0) z1 = nz;
@@ -1171,7 +1211,7 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
   float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
   for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
   orc_trirec *recs = make_recs(sorted, ntri);
-  qctx c = {sorted, &b, recs};
+  qctx c = {sorted, &b, recs, max_abs_coord(sorted, ntri)};
   if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
   pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
   udf_job j = {&c, pts, npts, out, 0, &mu};
@@ -1519,4 +1559,40 @@ void orc_cup_cells(int glass, int res, float origin_y, float cx, float cz, float
         dist[i] = d;
         if (branch) branch[i] = (uint8_t)b;
       }
+}
+
+/* Largest (exact - FP32) and (FP32 - exact) distance of orc_trirec_dist2 over n seeded random
+ * non-sliver triangles and queries (coordinates across six decades of scale, triangle sizes across
+ * four, queries far, near and very near), in units of u (Mq + 3 Mp), u = 2^-24: the constant behind
+ * the traversal's pruning slack. */
+void orc_f32_error_ratio(long n, uint64_t seed, double out[2]) {
+  uint64_t s = seed;
+  const double u = ldexp(1.0, -24);
+  out[0] = out[1] = 0.0;
+#define RND() ((s = s * 6364136223846793005ULL + 1442695040888963407ULL), (double)(s >> 11) * (1.0 / 9007199254740992.0))
+  for (long it = 0; it < n; ++it) {
+    const double S = pow(10.0, -3 + 6 * RND());
+    const double size = S * pow(10.0, -4 + 4 * RND());
+    double c[3];
+    float t[9];
+    for (int k = 0; k < 3; ++k) c[k] = S * (2 * RND() - 1);
+    for (int v = 0; v < 3; ++v)
+      for (int k = 0; k < 3; ++k) t[3 * v + k] = (float)(c[k] + size * (2 * RND() - 1));
+    if (orc_tri_is_sliver(t)) continue;
+    orc_trirec r;
+    orc_trirec_make(t, &r);
+    float q[3];
+    const int mode = (int)(it % 3);
+    for (int k = 0; k < 3; ++k)
+      q[k] = (float)(mode == 0 ? S * (2 * RND() - 1) : c[k] + (mode == 1 ? size : size * 1e-3) * (2 * RND() - 1));
+    const double qd[3] = {q[0], q[1], q[2]};
+    const double d64 = sqrt(orc_tri_dist2_f64(qd, t));
+    const double d32 = sqrt((double)orc_trirec_dist2(q, &r));
+    double mq = fmax(fabs(q[0]), fmax(fabs(q[1]), fabs(q[2]))), mp = 0.0;
+    for (int k = 0; k < 9; ++k) mp = fmax(mp, fabs(t[k]));
+    const double scale = u * (mq + 3.0 * mp);
+    out[0] = fmax(out[0], (d64 - d32) / scale);
+    out[1] = fmax(out[1], (d32 - d64) / scale);
+  }
+#undef RND
 }

@@ -54,14 +54,16 @@ void orc_occupancy(const float *xyz, size_t ntri, const float origin[3], float h
 /* ---- distance ----------------------------------------------------------------- */
 /* 1 if the triangle takes the FP64 path (near-degenerate: sin^2 of its corner angle < 1e-4). */
 int orc_tri_is_sliver(const float *tri9);
-/* distance record of one triangle (vertices, edges, normal, reciprocals, edge normals) */
+/* distance record of one triangle: an orthonormal frame at p0 (X along p1 - p0, Z the unit normal,
+ * Y = Z x X) with the triangle's corners in it, p1 = (len, 0), p2 = (c0, c1), and the reciprocals
+ * the query uses; a sliver keeps its three vertices for the FP64 path instead. */
 typedef struct {
-  float p[3][3];
-  float e[3][3];
-  float n[3], inv_nn;
-  float nie[3];
-  float m[3][3];
+  float o[3], x[3], y[3], z[3];
+  float len, inv_len;
+  float e1[2], inv_e1;
+  float c[2], nie2;
   int sliver;
+  float v[9]; /* the vertices (FP64 path) */
 } orc_trirec;
 void orc_trirec_make(const float *tri9, orc_trirec *r);
 float orc_trirec_dist2(const float q[3], const orc_trirec *r);
@@ -146,6 +148,9 @@ void orc_sample_grid(const float origin[3], float h, int nx, int ny, int nz, con
                      size_t n, float *values, float *normals, int32_t *nearest);
 void orc_trace_grid(const float origin[3], float h, int nx, int ny, int nz, const float *dist, const float *org,
                     const float *dir, size_t n, float tmin, float tmax, float *t_hit);
+
+/* worst (exact - FP32), (FP32 - exact) distance of orc_trirec_dist2 in units of 2^-24 (Mq + 3 Mp) */
+void orc_f32_error_ratio(long n, uint64_t seed, double out[2]);
 
 /* Analytic SDF bake (src/convert/sdf_procedural.cpp).  A CSG instruction is {int32 op, int32 branch,
  * float a[8]} with the opcodes of include/italy_bake.h (ib_csg_opcode); orc_csg_eval evaluates a

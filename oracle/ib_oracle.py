@@ -93,6 +93,8 @@ def _load():
     lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
     lib.orc_udf_points_f32.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
     lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
+    lib.orc_f32_error_ratio.argtypes = [ctypes.c_long, ctypes.c_uint64, _f64p]
+    lib.orc_f32_error_ratio.restype = None
     lib.orc_csg_eval.argtypes = [ctypes.c_void_p, ctypes.c_int, _f32p, _i32p]
     lib.orc_csg_eval.restype = ctypes.c_float
     lib.orc_analytic_framing.argtypes = [_f32p, _f32p, ctypes.c_int, _f32p, _f32p, _i32p]
@@ -202,6 +204,13 @@ def trace_grid(origin, h, dist, org, dirs, tmin=0.0, tmax=1e16):
     lib().orc_trace_grid(_p(_f32(origin), _f32p), float(h), nx, ny, nz, _p(dist, _f32p), _p(org, _f32p),
                          _p(dirs, _f32p), len(org), float(tmin), float(tmax), _p(t, _f32p))
     return t
+
+
+def f32_error_ratio(n=200000, seed=1):
+    """Worst (exact - FP32) and (FP32 - exact) triangle distance in units of 2^-24 (Mq + 3 Mp)."""
+    out = np.zeros(2, np.float64)
+    lib().orc_f32_error_ratio(int(n), int(seed), _p(out, _f64p))
+    return float(out[0]), float(out[1])
 
 
 CSG_DTYPE = np.dtype([("op", "<i4"), ("branch", "<i4"), ("a", "<f4", (8,))])

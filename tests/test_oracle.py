@@ -194,6 +194,26 @@ def test_f32_distance_matches_f64(oracle):
     assert worst < 2e-6
 
 
+def test_f32_distance_error_within_pruning_slack(oracle):
+    """The traversals enter a box when it could hold a triangle within sqrt(best) + 2^-17 (Mq + 3 Mp);
+    that slack must exceed the FP32 distance's error by a wide margin (128 units here)."""
+    below, above = oracle.f32_error_ratio(400000, seed=7)
+    assert 0.0 < below < 16.0 and above < 16.0
+
+
+def test_lbvh_distance_equals_brute_force_on_boxes(oracle, meshgen):
+    """Axis-aligned box faces lie on their LBVH boxes' faces, where a box bound and the FP32 triangle
+    distance can disagree in the last bits: the slack keeps the traversal equal to brute force."""
+    m = meshgen.stress(seed=11, ncomp=4, remove_frac=0.0)
+    rng = np.random.default_rng(5)
+    g = np.linspace(-1.0, 1.0, 11, dtype=np.float32)
+    pts = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    pts = np.concatenate([pts, rng.uniform(-1, 1, (500, 3)).astype(np.float32)])
+    ud = oracle.udf_points(m.xyz, pts)
+    brute = np.sqrt(oracle.udf2_f32_brute(m.xyz, pts))
+    assert np.array_equal(ud, brute)
+
+
 def test_box_mesh_distance_is_sdbox(oracle, meshgen):
     """The cube shell's exact SDF is sdBox (src/convert/sdf_primitives.h:12-15)."""
     cube = meshgen.cube_shell()
