@@ -23,12 +23,13 @@ sys.path.insert(0, os.path.join(ROOT, "italy-rs_amd"))
 METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake latency"
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published)
 PEAK_HBM_GBS = 8000.0
-# FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max
-# and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
-# triangle distance 9 sub + 3 segments x (dot 5 + scale 1 + fma point 6 + square 5) + 3 edge-normal
-# dots x 5 + plane dot 5 + square and scale 2 = 82
+# FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max,
+# clamps and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
+# triangle distance in the triangle's plane frame: offset 3 + three frame dots 15 + edge 0 (scale 1,
+# fma 2, square 3) 6 + edge 1 (offset 1, dot 3, scale 1, two fma 4, square 3) 12 + edge 2 (offsets 2,
+# dot 3, scale 1, two fma 4, square 3) 13 + two edge functions 6 + w^2 and add 2 = 57
 FLOP_BOX = 11
-FLOP_TRI = 82
+FLOP_TRI = 57
 # FLOPs per cell of the glass cup's distance lambda (sdf_procedural.cpp:123-130; sqrt and divide = 1):
 # cell centre 3 x 3 = 9, three spheres 3 x (3 sub + 3 mul + 2 add + sqrt + sub) = 30, two plane
 # offsets p.y - h = 2, smooth intersect (b - a, * 0.5, / k, + 0.5, two 1 - h, three mul, add, mul, sub) = 12
@@ -42,6 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-zstep", type=int, default=4, help="the CPU bakes every zstep-th z-slab of the grid")
+    ap.add_argument("--cpu-zstep1", type=int, default=64, help="z-slab stride of the single-thread CPU sample")
+    ap.add_argument("--no-voxelize", action="store_true", help="skip the voxelizeMesh drop-in comparison")
     ap.add_argument("--variant", type=int, default=None, help="query kernel: 0 per-voxel, 1 per-brick wave")
     ap.add_argument("--batch-meshes", type=int, default=512, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--no-band", action="store_true", help="skip the narrow-band precision-stress bake")
@@ -155,25 +158,23 @@ def analytic_section(ctx, cpu=True, steps=20):
     (humans.md:420-423).  The GPU grid is compared with the oracle's per-cell restatement."""
     from italy_bake import procedural as pr
 
-    import torch
-
     prog, bmin, bmax = pr.cup_program(-1.0, (0.0, 0.0), glass=True)
     g = pr.analytic_framing(bmin, bmax, 320)
     n = g.nx * g.ny * g.nz
     d_dist, d_branch = ctx.alloc(4 * n), ctx.alloc(n)
-    stream = torch.cuda.Stream()  # default priority; the launches go to it so its events bracket them
     try:
-        pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch, stream=stream.cuda_stream)
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch)
+        ctx.synchronize()
         t0 = time.perf_counter()
-        e0.record(stream)
         for _ in range(steps):
-            pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch, stream=stream.cuda_stream)
-        e1.record(stream)
-        torch.cuda.synchronize()
+            pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch)
+        ctx.synchronize()
         wall_s = (time.perf_counter() - t0) / steps
-        dev_s = e0.elapsed_time(e1) * 1e-3 / steps
+        ev = []
+        for _ in range(steps):  # the launch alone, HIP events on its stream
+            pr.bake_analytic_device(ctx, prog, g, d_dist, d_branch)
+            ev.append(ctx.stats()["ms_total"])
+        dev_s = float(np.mean(ev)) * 1e-3
         dist = ctx.download(d_dist, np.empty(n, np.float32))
         branch = ctx.download(d_branch, np.empty(n, np.uint8))
     finally:
@@ -201,6 +202,31 @@ def analytic_section(ctx, cpu=True, steps=20):
                                "sample": "the whole grid, the reference's lambda per cell on one core"}
         out["matches_oracle"] = bool(np.array_equal(dist, cd.reshape(-1)) and np.array_equal(branch, cb.reshape(-1))
                                      and np.array_equal(hd, dist) and np.array_equal(hb, branch))
+    return out
+
+
+def voxelize_section(ctx, mesh, res=256, cpu=True, steps=3):
+    """italy::voxelizeMesh (voxelize.cpp:52-107) as the reference calls it: mesh in host memory, cells in
+    first-touch order back in host memory, on the 327,680-triangle mesh at res 256.  Beside it the
+    oracle's single-threaded restatement (the reference runs it on one thread)."""
+    from italy_bake import bake
+
+    grid = bake.voxelizeMesh(mesh, res, ctx)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        grid = bake.voxelizeMesh(mesh, res, ctx)
+    gpu_s = (time.perf_counter() - t0) / steps
+    out = {"workload": f"voxelizeMesh(327,680-triangle mesh, {res}): {len(grid.cells)} cells in first-touch order, "
+                       "host mesh in, host cells out", "cells": int(len(grid.cells)), "gpu_ms": gpu_s * 1e3}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        t0 = time.perf_counter()
+        cells, _ = ib_oracle.voxelize_first_touch(mesh.xyz, mesh.bmin, mesh.bmax, res)
+        cpu_s = time.perf_counter() - t0
+        out.update(cpu_ms=cpu_s * 1e3, cpu_cores=1, gpu_multiple=cpu_s / gpu_s,
+                   matches_oracle=bool(np.array_equal(cells, grid.cells)))
     return out
 
 
@@ -366,6 +392,7 @@ def main():
     batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None
     analytic = analytic_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_analytic else None
+    voxelize = voxelize_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_voxelize else None
     served = served_section(args, batched)
 
     cpu = None
@@ -381,6 +408,13 @@ def main():
                "gpu_multiple": value / (cst["voxels"] / cst["seconds"]),
                "sample": f"every {args.cpu_zstep}th z-slab of the same 256^3 bake ({cst['voxels']} voxels, exact "
                          f"closest triangle + crossing sign, LBVH build included), {cst['seconds']:.2f} s"}
+        # one thread, as the reference's CPU code runs
+        _, c1 = ib_oracle.bake_dense(mesh.xyz, list(sdf.origin), sdf.h, sdf.nx, sdf.ny, sdf.nz, 0, sdf.nz,
+                                     threads=1, zstep=args.cpu_zstep1)
+        cpu["single_thread"] = {"value": c1["voxels"] / c1["seconds"], "cores": 1,
+                                "gpu_multiple": value / (c1["voxels"] / c1["seconds"]),
+                                "sample": f"every {args.cpu_zstep1}th z-slab ({c1['voxels']} voxels), "
+                                          f"{c1['seconds']:.2f} s"}
 
     out = {
         "metric": METRIC, "value": value, "unit": "voxel-queries/s", "n_gpus": 1, "steps": args.steps,
@@ -391,7 +425,7 @@ def main():
                    "triangles": mesh.ntri, "sdf_res": sdf.nx, "occ_res": occ.nx, "parallelism": "single GPU"},
         "device_ms_per_bake": float(np.mean(total_ms)),
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band, "served": served,
-        "grid_consumer": consumer, "analytic_bake": analytic,
+        "grid_consumer": consumer, "analytic_bake": analytic, "voxelize_mesh": voxelize,
         "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
         if served and served.get("batched_distribution") else None,
     }

@@ -250,7 +250,9 @@ ib_status ib_analytic_framing(const float bmin[3], const float bmax[3], int32_t 
 /* Evaluate a CSG program (at most 256 instructions, stack depth <= 16, exactly one value left) at
  * every cell centre of `grid`: d_distances (FP32) and, if not NULL, d_branch (one byte per cell).
  * Enqueued on `stream` (NULL: the context's stream); programs of up to 64 instructions travel in the
- * kernel arguments and the call returns without waiting, longer ones wait for their upload. */
+ * kernel arguments and the call returns without waiting, longer ones wait for their upload.
+ * ib_last_stats then reports the launch's device time as ms_total (HIP events on `stream`).
+ * Grids are limited to ny <= 262140 and nz <= 65535. */
 ib_status ib_bake_analytic_device(ib_context *ctx, const ib_csg_op *program, uint32_t nops, const ib_grid *grid,
                                   float *d_distances, uint8_t *d_branch, void *stream);
 /* Host-buffer form. */

@@ -943,6 +943,9 @@ ib_status ib_analytic_framing(const float bmin[3], const float bmax[3], int32_t 
   return IB_OK;
 }
 
+// the analytic launch is a (x / 64, y / 4, z) grid of blocks
+static bool analytic_grid_ok(const ib_grid *g) { return grid_ok(g) && g->ny <= 4 * 65535 && g->nz <= 65535; }
+
 static ib_status check_program(const ib_csg_op *prog, uint32_t nops) {
   if (!prog || nops == 0) return fail(IB_ERR_INVALID_ARGUMENT, "empty CSG program");
   if (nops > 256) return fail(IB_ERR_UNSUPPORTED, "CSG program longer than 256 instructions");
@@ -986,11 +989,15 @@ ib_status ib_bake_analytic_device(ib_context *c, const ib_csg_op *prog, uint32_t
                                   float *d_dist, uint8_t *d_branch, void *stream) {
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
   IB_TRY(check_program(prog, nops));
-  if (!grid || !grid_ok(grid) || !d_dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
+  if (!grid || !analytic_grid_ok(grid) || !d_dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  reset_stats(c);
+  c->stats.voxels = (uint64_t)grid->nx * grid->ny * grid->nz;
+  c->record(EV_START, s);
   IB_TRY(bake_analytic(c, prog, nops, grid, d_dist, d_branch, s));
+  c->record(EV_SDF, s);
   return IB_OK;
 }
 
@@ -998,7 +1005,7 @@ ib_status ib_bake_analytic(ib_context *c, const ib_csg_op *prog, uint32_t nops, 
                            uint8_t *branch) {
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
   IB_TRY(check_program(prog, nops));
-  if (!grid || !grid_ok(grid) || !dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
+  if (!grid || !analytic_grid_ok(grid) || !dist) return fail(IB_ERR_INVALID_ARGUMENT, "invalid grid or output");
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;

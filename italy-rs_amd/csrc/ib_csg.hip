@@ -62,12 +62,11 @@ template <class Prog>
 __global__ __launch_bounds__(256) void k_bake_csg(Prog prog, int nops, GridArgs g, float *dist, uint8_t *branch) {
   __shared__ float s_d[kCsgStack - 1][256];
   __shared__ uint8_t s_b[kCsgStack - 1][256];
+  // a block is 64 x 4 cells of one z slice (a wave per row of 64 x), no index division
   const int tid = threadIdx.x;
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + tid;
-  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
-  if (i >= n) return;
-  const int x = (int)(i % (uint64_t)g.nx), y = (int)((i / (uint64_t)g.nx) % (uint64_t)g.ny);
-  const int z = (int)(i / ((uint64_t)g.nx * g.ny));
+  const int x = blockIdx.x * 64 + (tid & 63), y = blockIdx.y * 4 + (tid >> 6), z = blockIdx.z;
+  if (x >= g.nx || y >= g.ny) return;
+  const uint64_t i = ((uint64_t)z * g.ny + y) * g.nx + x;
   // origin + (vec3(x, y, z) + 0.5) * voxelSize (sdf_procedural.cpp:32-35)
   const float px = g.ox + ((float)x + 0.5f) * g.h, py = g.oy + ((float)y + 0.5f) * g.h,
               pz = g.oz + ((float)z + 0.5f) * g.h;
@@ -122,9 +121,8 @@ __global__ __launch_bounds__(256) void k_bake_csg(Prog prog, int nops, GridArgs 
 
 void launch_bake_csg(const CsgOp *host_prog, const CsgOp *dev_prog, int nops, const GridArgs &g, float *dist,
                      uint8_t *branch, hipStream_t s) {
-  const uint64_t n = (uint64_t)g.nx * g.ny * g.nz;
-  if (!n) return;
-  const unsigned blocks = (unsigned)((n + 255) / 256);
+  if (g.nx <= 0 || g.ny <= 0 || g.nz <= 0) return;
+  const dim3 blocks((unsigned)((g.nx + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nz);
   if (nops <= kCsgInline) {
     ProgInline p{};
     for (int k = 0; k < nops; ++k) p.p.ops[k] = host_prog[k];

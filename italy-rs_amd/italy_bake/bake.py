@@ -112,7 +112,7 @@ class Context:
                 check(self.lib.ib_bake_band_device(self._h, ctypes.c_void_p(d_xyz), ntri, ctypes.byref(grid),
                                                    int(band_voxels), ctypes.c_void_p(d_idx),
                                                    ctypes.c_void_p(d_pool), cap, ctypes.byref(n), None))
-                pool = self.download(d_pool, np.empty((cap, 8, 8, 8), np.float32))[: n.value]
+                pool = self.download(d_pool, np.empty((n.value, 8, 8, 8), np.float32))
             finally:
                 self.free(d_pool)
             index = self.download(d_idx, np.empty((grid.nz // 8, grid.ny // 8, grid.nx // 8), np.int32))
