@@ -141,6 +141,35 @@ ib_status ib_voxelize(ib_context *ctx, const float *xyz, uint64_t ntri, const fl
 ib_status ib_splat_materials(ib_context *ctx, const float *xyz, uint64_t ntri, const float *tri_material,
                              const ib_grid *grid, float *base_color, float *metallic, float *roughness);
 
+/* ---- mesh ingestion gate (src/io/mesh_weld.cpp, src/io/mesh_validate.cpp) ----------------------
+ * The checks the reference runs on a loaded mesh before it bakes (src/app/main.cpp:227-241). */
+/* Replaces italy::weldPositions(positions, boundsMin, boundsMax, &uniquePositions)
+ * (src/io/mesh_weld.h:10-12, body mesh_weld.cpp:26-48): positions (3 floats each) that round to the same
+ * point of the lattice of pitch max(|bmax - bmin| * 1e-5, 1e-6) share an index; indices count distinct
+ * lattice points in order of first occurrence.  welded: npos indices; unique (may be NULL; room for
+ * 3 * npos floats): the first position of each index; *nunique = weldedIndexCount(welded)
+ * (mesh_weld.cpp:50-54).  A position whose quotient by the pitch is not finite or reaches 2^63 (where
+ * std::lround is unspecified) is rejected with IB_ERR_INVALID_MESH.  Host buffers. */
+ib_status ib_weld_positions(ib_context *ctx, const float *positions, uint64_t npos, const float bmin[3],
+                            const float bmax[3], uint32_t *welded, float *unique, uint64_t *nunique);
+/* Same with device buffers, on `stream` (NULL = the context's stream); returns when *nunique is known. */
+ib_status ib_weld_positions_device(ib_context *ctx, const float *d_positions, uint64_t npos, const float bmin[3],
+                                   const float bmax[3], uint32_t *d_welded, float *d_unique, uint64_t *nunique,
+                                   void *stream);
+/* Replaces italy::isWatertight(const MeshAsset&, std::string &err) (src/io/mesh_validate.h:9, body
+ * mesh_validate.cpp:21-59) for a triangle soup with its bounds: welds the corners as above, skips
+ * triangles with a repeated index, and counts per undirected edge how many triangles use it.
+ * *watertight = 1 when ntri > 0 and every edge is used exactly twice.  boundary_edges (used once) and
+ * nonmanifold_edges (used more than twice) may be NULL.  message (may be NULL) receives the reference's
+ * err text, "" when watertight, truncated to message_cap - 1 bytes. */
+ib_status ib_watertight(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                        int32_t *watertight, uint64_t *boundary_edges, uint64_t *nonmanifold_edges, char *message,
+                        uint64_t message_cap);
+/* Same on a device soup, on `stream` (NULL = the context's stream); returns when the verdict is known. */
+ib_status ib_watertight_device(ib_context *ctx, const float *d_xyz, uint64_t ntri, const float bmin[3],
+                               const float bmax[3], int32_t *watertight, uint64_t *boundary_edges,
+                               uint64_t *nonmanifold_edges, char *message, uint64_t message_cap, void *stream);
+
 /* ---- explicit grids (the workload entry points) -------------------------------------------- */
 /* Dense bake of one mesh: sdf grid (may be NULL) and occupancy grid (may be NULL).  Host buffers. */
 ib_status ib_bake_dense(ib_context *ctx, const float *xyz, uint64_t ntri, const ib_grid *sdf, float *distances,

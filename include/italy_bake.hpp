@@ -4,6 +4,8 @@
 //   italy_bake::voxelizeMesh(mesh, resolution)  replaces italy::voxelizeMesh (src/convert/voxelize.h:8)
 //   italy_bake::makeCupSdf / makeGlassCupSdf / makeMaterialProbeSdfs, bakeAnalyticSdf(Palette)
 //                                               replace src/convert/sdf_procedural.h:28-40
+//   italy_bake::weldPositions / weldedIndexCount replace src/io/mesh_weld.h:10-14
+//   italy_bake::isWatertight(mesh, err)         replaces italy::isWatertight (src/io/mesh_validate.h:9)
 //
 // `Mesh` is any type shaped like italy::MeshAsset (src/io/mesh_asset.h:33-54): a contiguous
 // `positions` container of 3-float vectors (triangle soup, 3 per triangle) and `boundsMin` /
@@ -180,6 +182,56 @@ VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution, ColorFn &&color) {
 template <class Mesh>
 VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution) {
   return voxelizeMesh(mesh, resolution, [](uint64_t, float *c) { c[0] = c[1] = c[2] = 1.0f; });
+}
+
+// ---- mesh ingestion gate (src/io/mesh_weld.h, src/io/mesh_validate.h) ------------------------------
+// weldPositions(positions, boundsMin, boundsMax, &unique): the reference's signature with any container
+// of packed 3-float vectors (std::vector<glm::vec3> qualifies).
+template <class Positions, class V, class Unique = Positions>
+std::vector<uint32_t> weldPositions(const Positions &positions, const V &boundsMin, const V &boundsMax,
+                                    Unique *uniquePositions = nullptr) {
+  static_assert(sizeof(positions[0]) == 3 * sizeof(float), "positions must be packed 3-float vectors");
+  float bmin[3], bmax[3];
+  bounds(boundsMin, boundsMax, bmin, bmax);
+  const uint64_t n = positions.size();
+  std::vector<uint32_t> welded(n);
+  std::vector<float> unique(uniquePositions ? 3 * n : 0);
+  uint64_t count = 0;
+  check(ib_weld_positions(context(), n ? reinterpret_cast<const float *>(positions.data()) : nullptr, n, bmin, bmax,
+                          welded.data(), uniquePositions ? unique.data() : nullptr, &count),
+        "ib_weld_positions");
+  if (uniquePositions) {
+    uniquePositions->clear();
+    uniquePositions->resize(count);
+    for (uint64_t i = 0; i < count; ++i) {
+      auto &u = (*uniquePositions)[i];
+      u.x = unique[3 * i];
+      u.y = unique[3 * i + 1];
+      u.z = unique[3 * i + 2];
+    }
+  }
+  return welded;
+}
+
+// src/io/mesh_weld.cpp:50-54
+inline size_t weldedIndexCount(const std::vector<uint32_t> &welded) {
+  size_t m = 0;
+  for (uint32_t w : welded) m = w + 1 > m ? w + 1 : m;
+  return welded.empty() ? 0 : m;
+}
+
+// isWatertight(mesh, err): false with the reference's message in err, true with err untouched
+template <class Mesh>
+bool isWatertight(const Mesh &mesh, std::string &err) {
+  float bmin[3], bmax[3];
+  bounds(mesh.boundsMin, mesh.boundsMax, bmin, bmax);
+  int32_t ok = 0;
+  char msg[512];
+  check(ib_watertight(context(), soup(mesh), mesh.positions.size() / 3, bmin, bmax, &ok, nullptr, nullptr, msg,
+                      sizeof(msg)),
+        "ib_watertight");
+  if (!ok) err = msg;
+  return ok != 0;
 }
 
 // ---- analytic SDF bake (src/convert/sdf_procedural.cpp) ------------------------------------------

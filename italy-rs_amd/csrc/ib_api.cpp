@@ -891,6 +891,187 @@ ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, con
   return IB_OK;
 }
 
+// ---- mesh ingestion gate (src/io/mesh_weld.cpp:26-48, src/io/mesh_validate.cpp:21-59) ---------------
+// pitch = std::max(glm::length(bmax - bmin) * 1e-5f, 1e-6f), glm::length(v) = sqrt((x*x + y*y) + z*z)
+// (mesh_weld.cpp:29-31); this file is compiled without FP contraction, so each product is rounded
+static float weld_pitch(const float bmin[3], const float bmax[3]) {
+  const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
+  const float diag = std::sqrt((ex * ex + ey * ey) + ez * ez);
+  return std::max(diag * 1e-5f, 1e-6f);
+}
+
+// Welds npos device positions into d_welded (device, npos entries) and, when d_unique is not null, the
+// first position of each index (device, 3 * npos floats).  Waits for the result; *nunique = indices.
+static ib_status weld_device(ib_context *c, const float *d_pos, uint64_t npos, const float bmin[3],
+                             const float bmax[3], uint32_t *d_welded, float *d_unique, uint64_t *nunique,
+                             hipStream_t s) {
+  *nunique = 0;
+  if (npos == 0) return IB_OK;
+  const uint32_t n = (uint32_t)npos;
+  const uint32_t cap = pow2_at_least(2ull * n);
+  long long *keys;
+  int32_t *slots;
+  uint32_t *vslot, *first, *flag, *rank, *bad;
+  IB_TRY(c->scratch("qw_keys", 3ull * n, &keys));
+  IB_TRY(c->scratch("qw_slots", cap, &slots));
+  IB_TRY(c->scratch("qw_vslot", n, &vslot));
+  IB_TRY(c->scratch("qw_first", n, &first));
+  IB_TRY(c->scratch("qw_flag", (uint64_t)n + 1, &flag));
+  IB_TRY(c->scratch("qw_rank", (uint64_t)n + 1, &rank));
+  IB_TRY(c->scratch("qw_bad", 2, &bad));
+  fill_u32(reinterpret_cast<uint32_t *>(slots), 0xffffffffu, cap, s);
+  fill_u32(bad, 0u, 1, s);
+  launch_qweld(d_pos, n, weld_pitch(bmin, bmax), keys, slots, cap - 1, vslot, first, flag, bad, s);
+  size_t tb = 0;
+  IB_HIP(exclusive_scan_u32(nullptr, tb, flag, rank, n + 1, s));
+  char *temp;
+  IB_TRY(c->scratch("qw_temp", tb, &temp));
+  IB_HIP(exclusive_scan_u32(temp, tb, flag, rank, n + 1, s));
+  launch_qweld_out(d_pos, first, flag, rank, n, d_welded, d_unique, s);
+  uint32_t h[2] = {0, 0};
+  IB_HIP(hipMemcpyAsync(&h[0], bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipMemcpyAsync(&h[1], rank + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  if (h[0])
+    return fail(IB_ERR_INVALID_MESH, "a position divided by the weld pitch is not finite or exceeds 2^63 "
+                                     "(std::lround is unspecified there)");
+  *nunique = h[1];
+  return IB_OK;
+}
+
+static ib_status weld_args_ok(uint64_t npos, const float bmin[3], const float bmax[3]) {
+  if (!bmin || !bmax) return fail(IB_ERR_INVALID_ARGUMENT, "null bounds");
+  if (npos >= (uint64_t)INT_MAX) return fail(IB_ERR_INVALID_ARGUMENT, "too many positions (indices are 32-bit)");
+  return IB_OK;
+}
+
+ib_status ib_weld_positions_device(ib_context *c, const float *d_pos, uint64_t npos, const float bmin[3],
+                                   const float bmax[3], uint32_t *d_welded, float *d_unique, uint64_t *nunique,
+                                   void *stream) {
+  if (!c || !nunique) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  IB_TRY(weld_args_ok(npos, bmin, bmax));
+  if (npos && (!d_pos || !d_welded)) return fail(IB_ERR_INVALID_ARGUMENT, "null position or index buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  reset_stats(c);
+  c->record(EV_START, s);
+  IB_TRY(weld_device(c, d_pos, npos, bmin, bmax, d_welded, d_unique, nunique, s));
+  c->record(EV_INGEST, s);
+  return IB_OK;
+}
+
+ib_status ib_weld_positions(ib_context *c, const float *pos, uint64_t npos, const float bmin[3], const float bmax[3],
+                            uint32_t *welded, float *unique, uint64_t *nunique) {
+  if (!c || !nunique) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  IB_TRY(weld_args_ok(npos, bmin, bmax));
+  *nunique = 0;
+  if (npos == 0) return IB_OK;
+  if (!pos || !welded) return fail(IB_ERR_INVALID_ARGUMENT, "null position or index buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  reset_stats(c);
+  float *d_pos, *d_unique = nullptr;
+  uint32_t *d_welded;
+  IB_TRY(c->scratch("in_xyz", 3 * npos, &d_pos));
+  IB_TRY(c->scratch("qw_welded", npos, &d_welded));
+  if (unique) IB_TRY(c->scratch("qw_unique", 3 * npos, &d_unique));
+  IB_HIP(hipMemcpyAsync(d_pos, pos, 3 * npos * sizeof(float), hipMemcpyHostToDevice, s));
+  c->record(EV_START, s);
+  IB_TRY(weld_device(c, d_pos, npos, bmin, bmax, d_welded, d_unique, nunique, s));
+  c->record(EV_INGEST, s);
+  IB_HIP(hipMemcpyAsync(welded, d_welded, npos * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (unique) IB_HIP(hipMemcpyAsync(unique, d_unique, 3 * *nunique * sizeof(float), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
+// isWatertight's verdict and message (mesh_validate.cpp:22-58) from the edge counts
+static void watertight_message(uint64_t ntri, uint64_t boundary, uint64_t nonmanifold, char *msg, uint64_t cap) {
+  if (!msg || cap == 0) return;
+  std::string e;
+  if (ntri == 0) {
+    e = "mesh has no triangles";
+  } else if (boundary || nonmanifold) {
+    e = "mesh is not watertight (" + std::to_string(boundary) + " boundary edge" + (boundary == 1 ? "" : "s");
+    if (nonmanifold)
+      e += ", " + std::to_string(nonmanifold) + " non-manifold edge" + (nonmanifold == 1 ? "" : "s");
+    e += ") \u2014 fill holes and merge duplicate vertices in your DCC tool and re-export";
+  }
+  const size_t k = std::min<size_t>(e.size(), cap - 1);
+  std::memcpy(msg, e.data(), k);
+  msg[k] = '\0';
+}
+
+static ib_status watertight_device(ib_context *c, const float *d_xyz, uint64_t ntri, const float bmin[3],
+                                   const float bmax[3], uint64_t *boundary, uint64_t *nonmanifold, hipStream_t s) {
+  uint32_t *d_welded;
+  IB_TRY(c->scratch("qw_welded", 3 * ntri, &d_welded));
+  uint64_t nunique = 0;
+  IB_TRY(weld_device(c, d_xyz, 3 * ntri, bmin, bmax, d_welded, nullptr, &nunique, s));
+  const uint32_t cap = pow2_at_least(2ull * 3 * ntri);
+  unsigned long long *ekeys, *out;
+  uint32_t *ecount;
+  IB_TRY(c->scratch("wt_keys", cap, &ekeys));
+  IB_TRY(c->scratch("wt_count", cap, &ecount));
+  IB_TRY(c->scratch("wt_out", 2, &out));
+  fill_u32(reinterpret_cast<uint32_t *>(ekeys), 0xffffffffu, 2ull * cap, s);
+  fill_u32(ecount, 0u, cap, s);
+  fill_u32(reinterpret_cast<uint32_t *>(out), 0u, 4, s);
+  launch_watertight(d_welded, (uint32_t)ntri, ekeys, ecount, cap - 1, out, s);
+  c->record(EV_INGEST, s);
+  unsigned long long h[2] = {0, 0};
+  IB_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  *boundary = h[0];
+  *nonmanifold = h[1];
+  return IB_OK;
+}
+
+static ib_status watertight_entry(ib_context *c, const float *xyz, bool on_device, uint64_t ntri, const float bmin[3],
+                                  const float bmax[3], int32_t *watertight, uint64_t *boundary, uint64_t *nonmanifold,
+                                  char *msg, uint64_t msg_cap, void *stream) {
+  if (!c || !watertight) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  if (ntri > (uint64_t)(INT_MAX / 9)) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  IB_TRY(weld_args_ok(3 * ntri, bmin, bmax));
+  if (ntri && !xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+  uint64_t b = 0, m = 0;
+  if (ntri) {
+    std::lock_guard<std::mutex> lk(c->mu);
+    IB_HIP(hipSetDevice(c->device));
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    reset_stats(c);
+    c->stats.triangles = ntri;
+    const float *d_xyz = xyz;
+    if (!on_device) {
+      float *d_in;
+      IB_TRY(c->scratch("in_xyz", 9 * ntri, &d_in));
+      IB_HIP(hipMemcpyAsync(d_in, xyz, 9 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
+      d_xyz = d_in;
+    }
+    c->record(EV_START, s);
+    IB_TRY(watertight_device(c, d_xyz, ntri, bmin, bmax, &b, &m, s));
+    c->stats.boundary_edges = (int64_t)b;
+  }
+  *watertight = (ntri > 0 && b == 0 && m == 0) ? 1 : 0;
+  if (boundary) *boundary = b;
+  if (nonmanifold) *nonmanifold = m;
+  watertight_message(ntri, b, m, msg, msg_cap);
+  return IB_OK;
+}
+
+ib_status ib_watertight(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                        int32_t *watertight, uint64_t *boundary, uint64_t *nonmanifold, char *msg, uint64_t msg_cap) {
+  return watertight_entry(c, xyz, false, ntri, bmin, bmax, watertight, boundary, nonmanifold, msg, msg_cap, nullptr);
+}
+
+ib_status ib_watertight_device(ib_context *c, const float *d_xyz, uint64_t ntri, const float bmin[3],
+                               const float bmax[3], int32_t *watertight, uint64_t *boundary, uint64_t *nonmanifold,
+                               char *msg, uint64_t msg_cap, void *stream) {
+  return watertight_entry(c, d_xyz, true, ntri, bmin, bmax, watertight, boundary, nonmanifold, msg, msg_cap, stream);
+}
+
 ib_status ib_device_alloc(ib_context *c, uint64_t bytes, void **p) {
   if (!c || !p) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -1088,7 +1269,9 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   st.ms_occupancy = ev_ms(c, EV_BVH, EV_OCC);
   st.ms_columns = ev_ms(c, EV_OCC, EV_COLS);
   st.ms_sdf = ev_ms(c, EV_COLS, EV_SDF);
-  st.ms_total = c->ev_valid[EV_SDF] ? ev_ms(c, EV_START, EV_SDF) : ev_ms(c, EV_START, EV_OCC);
+  st.ms_total = c->ev_valid[EV_SDF]   ? ev_ms(c, EV_START, EV_SDF)
+                : c->ev_valid[EV_OCC] ? ev_ms(c, EV_START, EV_OCC)
+                                      : ev_ms(c, EV_START, EV_INGEST);
   // device-side counters of the last bake
   Scalars *sc = nullptr;
   auto it = c->bufs.find("sc");

@@ -147,6 +147,19 @@ void launch_sample_grid(const GridView &g, const float *pts, uint64_t n, float *
 void launch_trace_grid(const GridView &g, const float *org, const float *dir, uint64_t n, float tmin, float tmax,
                        float *t_hit, hipStream_t s);
 
+// ---- mesh ingestion gate (ib_ingest.hip) ----
+// weldPositions: lattice keys, first-occurrence slots (slots: mask + 1 entries filled with -1), first
+// position of every index and the first-occurrence flags (n + 1 entries; flag[n] = 0)
+void launch_qweld(const float *pos, uint32_t n, float pitch, long long *keys, int32_t *slots, uint32_t mask,
+                  uint32_t *vslot, uint32_t *first, uint32_t *flag, uint32_t *bad, hipStream_t s);
+// welded indices from the exclusive scan `rank` of the flags; unique (may be null) gets the first positions
+void launch_qweld_out(const float *pos, const uint32_t *first, const uint32_t *flag, const uint32_t *rank, uint32_t n,
+                      uint32_t *welded, float *unique, hipStream_t s);
+// isWatertight's edge counts: ekeys (mask + 1 entries, all bits set), ecount (zeroed); out[0] += edges
+// used once, out[1] += edges used more than twice
+void launch_watertight(const uint32_t *welded, uint32_t ntri, unsigned long long *ekeys, uint32_t *ecount,
+                       uint32_t mask, unsigned long long *out, hipStream_t s);
+
 // ---- analytic bake (ib_csg.hip) ----
 // one instruction of a postfix CSG program (same layout as ib_csg_op of the C ABI)
 struct CsgOp {

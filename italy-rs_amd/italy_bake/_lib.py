@@ -23,6 +23,7 @@ EXPORTS = [
     "ib_server_last_error", "ib_server_submit", "ib_server_info", "ib_server_wait", "ib_server_drain",
     "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
     "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
+    "ib_weld_positions", "ib_weld_positions_device", "ib_watertight", "ib_watertight_device",
 ]
 
 
@@ -119,6 +120,12 @@ def load():
     lib.ib_autotune_batch.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_double)]
+    lib.ib_weld_positions.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64)]
+    lib.ib_weld_positions_device.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64), _vp]
+    lib.ib_watertight.argtypes = [_vp, _vp, _u64, _f3, _f3, ctypes.POINTER(_i32), ctypes.POINTER(_u64),
+                                  ctypes.POINTER(_u64), ctypes.c_char_p, _u64]
+    lib.ib_watertight_device.argtypes = [_vp, _vp, _u64, _f3, _f3, ctypes.POINTER(_i32), ctypes.POINTER(_u64),
+                                         ctypes.POINTER(_u64), ctypes.c_char_p, _u64, _vp]
     _lib = lib
     return lib
 

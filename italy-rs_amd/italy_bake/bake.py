@@ -129,6 +129,37 @@ class Context:
                                          ctypes.byref(blocks), ctypes.byref(ms)))
         return shape.value, blocks.value, ms.value
 
+    # ---- mesh ingestion gate ------------------------------------------------------------------
+    def weld_positions(self, positions, bmin, bmax):
+        """ib_weld_positions (weldPositions, src/io/mesh_weld.cpp:26-48): (welded index per position
+        (uint32), first position of each index (nunique, 3) float32)."""
+        pos = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        welded = np.zeros(max(len(pos), 1), np.uint32)
+        unique = np.zeros((max(len(pos), 1), 3), np.float32)
+        n = ctypes.c_uint64()
+        check(self.lib.ib_weld_positions(self._h, _ptr(pos) if len(pos) else None, len(pos), _f3(bmin), _f3(bmax),
+                                         _ptr(welded), _ptr(unique), ctypes.byref(n)))
+        return welded[:len(pos)], unique[:n.value].copy()
+
+    def watertight(self, xyz, bmin, bmax):
+        """ib_watertight (isWatertight, src/io/mesh_validate.cpp:21-59): (watertight, boundary edges,
+        non-manifold edges, err text)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        ntri = xyz.size // 9
+        ok, b, m = ctypes.c_int32(), ctypes.c_uint64(), ctypes.c_uint64()
+        msg = ctypes.create_string_buffer(512)
+        check(self.lib.ib_watertight(self._h, _ptr(xyz) if ntri else None, ntri, _f3(bmin), _f3(bmax),
+                                     ctypes.byref(ok), ctypes.byref(b), ctypes.byref(m), msg, len(msg)))
+        return bool(ok.value), b.value, m.value, msg.value.decode("utf-8")
+
+    def watertight_device(self, d_xyz, ntri, bmin, bmax, stream=None):
+        """ib_watertight_device on a device soup: (watertight, boundary edges, non-manifold edges)."""
+        ok, b, m = ctypes.c_int32(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(self.lib.ib_watertight_device(self._h, ctypes.c_void_p(d_xyz), int(ntri), _f3(bmin), _f3(bmax),
+                                            ctypes.byref(ok), ctypes.byref(b), ctypes.byref(m), None, 0,
+                                            ctypes.c_void_p(stream) if stream else None))
+        return bool(ok.value), b.value, m.value
+
     def splat_materials(self, xyz, materials, grid):
         """ib_splat_materials (bakePerCellMaterial, sdf_baker.cpp:76-128): per-triangle materials
         (ntri, 5) = r, g, b, metallic, roughness, averaged per cell in triangle order.  Returns
@@ -342,6 +373,24 @@ def voxelizeMesh(mesh, resolution, ctx=None):
     return VoxelGrid(list(origin), vs.value, cells[:n], tri[:n])
 
 
+def weldPositions(positions, boundsMin, boundsMax, ctx=None):
+    """italy::weldPositions (src/io/mesh_weld.h:10-12): (welded indices, unique positions)."""
+    return (ctx or default_context()).weld_positions(positions, boundsMin, boundsMax)
+
+
+def weldedIndexCount(welded):
+    """italy::weldedIndexCount (src/io/mesh_weld.cpp:50-54): largest index + 1, 0 when empty."""
+    welded = np.asarray(welded)
+    return int(welded.max()) + 1 if welded.size else 0
+
+
+def isWatertight(mesh, ctx=None):
+    """italy::isWatertight (src/io/mesh_validate.h:9): (bool, err).  err is the reference's text
+    ("" when the mesh is watertight)."""
+    ok, _, _, err = (ctx or default_context()).watertight(mesh.xyz, mesh.bmin, mesh.bmax)
+    return ok, err
+
+
 def dense_grid(lo, hi, res):
     """Cell-centred res^3 grid over [lo, hi]^3."""
     h = np.float32((np.float32(hi) - np.float32(lo)) / np.float32(res))
@@ -349,4 +398,4 @@ def dense_grid(lo, hi, res):
 
 
 __all__ = ["Context", "SdfGrid", "VoxelGrid", "bakeSdf", "voxelizeMesh", "sdf_framing", "dense_grid", "BakeError",
-           "Grid", "Job"]
+           "Grid", "Job", "weldPositions", "weldedIndexCount", "isWatertight"]

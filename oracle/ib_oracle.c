@@ -1596,3 +1596,190 @@ void orc_f32_error_ratio(long n, uint64_t seed, double out[2]) {
   }
 #undef RND
 }
+
+/* ---- mesh ingestion gate --------------------------------------------------------------------------
+ * weldPositions (src/io/mesh_weld.cpp:26-48): pitch cell = std::max(glm::length(bmax - bmin) * 1e-5f, 1e-6f)
+ * with glm::length = sqrt((x*x + y*y) + z*z); key = std::lround(p / cell) per axis; the map assigns
+ * weldMap.size() at the first insertion of a key, so indices count distinct keys in order of first
+ * occurrence. */
+static float weld_cell(const float bmin[3], const float bmax[3]) {
+  const float ex = bmax[0] - bmin[0], ey = bmax[1] - bmin[1], ez = bmax[2] - bmin[2];
+  const float diag = sqrtf((ex * ex + ey * ey) + ez * ez);
+  const float a = diag * 1e-5f, b = 1e-6f;
+  return a < b ? b : a; /* std::max(a, b) */
+}
+
+typedef struct {
+  int64_t k[3];
+} weld_key;
+
+int64_t orc_weld_positions(const float *pos, size_t npos, const float bmin[3], const float bmax[3], uint32_t *welded,
+                           float *unique) {
+  const float cell = weld_cell(bmin, bmax);
+  weld_key *keys = (weld_key *)malloc(sizeof(weld_key) * (npos ? npos : 1));
+  for (size_t i = 0; i < npos; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const float q = pos[3 * i + a] / cell;
+      if (!(fabsf(q) < 0x1p63f)) { /* lround's result is unspecified here */
+        free(keys);
+        return -1;
+      }
+      keys[i].k[a] = (int64_t)lroundf(q);
+    }
+  size_t cap = 16;
+  while (cap < 2 * npos) cap <<= 1;
+  int64_t *slot = (int64_t *)malloc(sizeof(int64_t) * cap); /* index of the first position with the key */
+  uint32_t *sid = (uint32_t *)malloc(sizeof(uint32_t) * cap);
+  for (size_t s = 0; s < cap; ++s) slot[s] = -1;
+  uint32_t next = 0;
+  for (size_t i = 0; i < npos; ++i) {
+    const weld_key *k = &keys[i];
+    size_t s = mix64((uint64_t)k->k[0] ^ mix64((uint64_t)k->k[1] ^ mix64((uint64_t)k->k[2]))) & (cap - 1);
+    for (;;) {
+      if (slot[s] < 0) {
+        slot[s] = (int64_t)i;
+        sid[s] = next++;
+        if (unique) memcpy(unique + 3 * (size_t)sid[s], pos + 3 * i, 3 * sizeof(float));
+        break;
+      }
+      const weld_key *o = &keys[slot[s]];
+      if (o->k[0] == k->k[0] && o->k[1] == k->k[1] && o->k[2] == k->k[2]) break;
+      s = (s + 1) & (cap - 1);
+    }
+    welded[i] = sid[s];
+  }
+  free(slot);
+  free(sid);
+  free(keys);
+  return (int64_t)next;
+}
+
+static int cmp_u64(const void *a, const void *b) {
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : x > y;
+}
+
+/* isWatertight (src/io/mesh_validate.cpp:21-59): undirected edges of the welded triangles, skipping
+ * triangles with a repeated index; boundary = edges used once, nonmanifold = edges used more than twice. */
+int orc_watertight(const float *xyz, size_t ntri, const float bmin[3], const float bmax[3], int64_t *boundary,
+                   int64_t *nonmanifold) {
+  *boundary = 0;
+  *nonmanifold = 0;
+  if (ntri == 0) return 0;
+  uint32_t *w = (uint32_t *)malloc(sizeof(uint32_t) * 3 * ntri);
+  if (orc_weld_positions(xyz, 3 * ntri, bmin, bmax, w, NULL) < 0) {
+    free(w);
+    return -1;
+  }
+  uint64_t *e = (uint64_t *)malloc(sizeof(uint64_t) * 3 * ntri);
+  size_t ne = 0;
+  for (size_t t = 0; t < ntri; ++t) {
+    const uint32_t v[3] = {w[3 * t], w[3 * t + 1], w[3 * t + 2]};
+    if (v[0] == v[1] || v[1] == v[2] || v[2] == v[0]) continue;
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t a = v[k], b = v[(k + 1) % 3];
+      e[ne++] = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+    }
+  }
+  qsort(e, ne, sizeof(uint64_t), cmp_u64);
+  for (size_t i = 0; i < ne;) {
+    size_t j = i;
+    while (j < ne && e[j] == e[i]) ++j;
+    if (j - i == 1) ++*boundary;
+    else if (j - i > 2) ++*nonmanifold;
+    i = j;
+  }
+  free(e);
+  free(w);
+  return (*boundary == 0 && *nonmanifold == 0) ? 1 : 0;
+}
+
+/* ---- FP64 unsigned distance through the LBVH (the FP64 oracle at sizes brute force cannot reach) ----
+ * Same minimum as the brute force over every triangle: a subtree is skipped only when its box is farther
+ * than the best FP64 distance so far by a relative margin (1e-9) far above the rounding of the FP64 box
+ * distance, so no triangle that could hold the minimum is skipped. */
+typedef struct {
+  const float *sorted;
+  const orc_bvh *b;
+  const double *pts;
+  size_t npts;
+  double *out;
+  size_t next;
+  pthread_mutex_t *mu;
+} f64bvh_job;
+
+static double box_d2_f64(const double q[3], const float *lo, const float *hi) {
+  double s = 0.0;
+  for (int k = 0; k < 3; ++k) {
+    double g = 0.0;
+    if (q[k] < (double)lo[k]) g = (double)lo[k] - q[k];
+    else if (q[k] > (double)hi[k]) g = q[k] - (double)hi[k];
+    s += g * g;
+  }
+  return s;
+}
+
+static double closest2_f64(const float *sorted, const orc_bvh *b, const double q[3]) {
+  const int n = b->n;
+  if (n == 1) return orc_tri_dist2_f64(q, sorted);
+  double best = INFINITY;
+  int stack[256];
+  int sp = 0;
+  stack[sp++] = 0; /* internal node ids >= 0, leaves ~i */
+  while (sp) {
+    const int node = stack[--sp];
+    if (node < 0) {
+      const int leaf = ~node;
+      const double d = orc_tri_dist2_f64(q, sorted + 9 * (size_t)leaf);
+      if (d < best) best = d;
+      continue;
+    }
+    const int ch[2] = {b->left[node], b->right[node]};
+    double cd[2];
+    for (int k = 0; k < 2; ++k) {
+      const int box = ch[k] >= 0 ? ch[k] : (n - 1) + ~ch[k];
+      cd[k] = box_d2_f64(q, b->lo + 3 * (size_t)box, b->hi + 3 * (size_t)box);
+    }
+    const int first = cd[1] < cd[0];
+    for (int k = 1; k >= 0; --k) { /* push the farther child first */
+      const int c = k ? first ^ 1 : first;
+      if (cd[c] <= best * (1.0 + 1e-9) && sp < 256) stack[sp++] = ch[c];
+    }
+  }
+  return best;
+}
+
+static void *f64bvh_fn(void *arg) {
+  f64bvh_job *j = (f64bvh_job *)arg;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 256;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 256 < j->npts ? i0 + 256 : j->npts;
+    for (size_t i = i0; i < i1; ++i) j->out[i] = sqrt(closest2_f64(j->sorted, j->b, j->pts + 3 * i));
+  }
+  return NULL;
+}
+
+int orc_udf_points_f64(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out) {
+  orc_bvh b;
+  if (ntri == 0 || orc_bvh_build(xyz, ntri, &b) != 0) return -1;
+  if (b.depth >= 250) {
+    orc_bvh_free(&b);
+    return -2;
+  }
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  f64bvh_job j = {sorted, &b, pts, npts, out, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, f64bvh_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+  free(sorted);
+  orc_bvh_free(&b);
+  return 0;
+}

@@ -15,10 +15,11 @@
  *   - closest-point distance (FP32 in query-local coordinates, FP64 for slivers; FP64 reference)
  *   - generalized winding number (FP64 brute-force solid angles, Van Oosterom-Strackee 1983)
  *   - signed watertight crossing count along +z columns (Woop et al. 2013 edge functions)
- *   - LBVH (Karras 2012) over 30-bit Morton codes, same integer topology the device builds.
+ *   - LBVH (Karras 2012) over 30-bit Morton codes, same integer topology the device builds;
+ *   - weldPositions / isWatertight  src/io/mesh_weld.cpp:26-48, src/io/mesh_validate.cpp:21-59.
  *
- * How each function is pinned is listed in oracle/PINNING.md; a function without a
- * pin there is "parity unpinned".
+ * tests/test_oracle.py and tests/test_oracle_csg.py pin every function against known answers,
+ * closed forms and independent brute force; a function without such a check is "parity unpinned".
  */
 #ifndef IB_ORACLE_H
 #define IB_ORACLE_H
@@ -135,6 +136,10 @@ void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t
 /* FP32 brute force: out[i] = min over triangles of orc_tri_dist2_f32(pts[i], tri) */
 void orc_udf2_f32_brute(const float *xyz, size_t ntri, const float *pts, size_t npts, float *out);
 
+/* FP64 unsigned distance at points through the LBVH: the same minimum of orc_tri_dist2_f64 over every
+ * triangle as the brute force (subtrees are skipped only beyond a 1e-9 relative margin); -1 for an empty
+ * mesh, -2 for a tree deeper than 249 levels */
+int orc_udf_points_f64(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out);
 /* FP32 unsigned distance at points through the LBVH (equals the FP32 brute force) */
 int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out);
 
@@ -168,6 +173,18 @@ void orc_bake_csg(const orc_csg_op *prog, int nops, const float origin[3], float
                   int threads, float *dist, uint8_t *branch);
 void orc_cup_cells(int glass, int res, float origin_y, float cx, float cz, float origin[3], float *h, int32_t n[3],
                    float *dist, uint8_t *branch);
+
+/* ---- mesh ingestion gate (src/io/mesh_weld.cpp:26-48, src/io/mesh_validate.cpp:21-59) ---------------
+ * orc_weld_positions: positions rounding to the same point of a lattice of pitch
+ * max(|bmax - bmin| * 1e-5, 1e-6) share an index; indices in order of first occurrence; unique (may be
+ * NULL) gets the first position of each index.  Returns the number of indices, or -1 when p / pitch is
+ * non-finite or beyond 2^63 (where lround is unspecified).
+ * orc_watertight: 1 if ntri > 0 and every undirected edge of the welded, non-degenerate triangles is
+ * used exactly twice, 0 if not, -1 as above; boundary = edges used once, nonmanifold = more than twice. */
+int64_t orc_weld_positions(const float *pos, size_t npos, const float bmin[3], const float bmax[3], uint32_t *welded,
+                           float *unique);
+int orc_watertight(const float *xyz, size_t ntri, const float bmin[3], const float bmax[3], int64_t *boundary,
+                   int64_t *nonmanifold);
 
 #ifdef __cplusplus
 }

@@ -106,6 +106,13 @@ def _load():
                                   _f32p, _i32p, _f32p, ctypes.c_void_p]
     lib.orc_cup_cells.restype = None
     lib.orc_boundary_edge_count.restype = ctypes.c_int64
+    lib.orc_udf_points_f64.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p]
+    lib.orc_udf_points_f64.restype = ctypes.c_int
+    lib.orc_weld_positions.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.orc_weld_positions.restype = ctypes.c_int64
+    lib.orc_watertight.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int64)]
+    lib.orc_watertight.restype = ctypes.c_int
     return lib
 
 
@@ -353,6 +360,40 @@ def sdf_f64_points(xyz, pts, threads=0):
 def boundary_edge_count(xyz):
     xyz = _f32(xyz)
     return int(lib().orc_boundary_edge_count(_p(xyz, _f32p), xyz.size // 9))
+
+
+def udf_points_f64(xyz, pts, threads=0):
+    """FP64 unsigned distance at points through the LBVH (equal to the brute-force FP64 minimum)."""
+    xyz = _f32(xyz)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    out = np.zeros(len(pts), np.float64)
+    rc = lib().orc_udf_points_f64(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f64p), len(pts), threads, _p(out, _f64p))
+    if rc != 0:
+        raise ValueError(f"orc_udf_points_f64 failed ({rc})")
+    return out
+
+
+def weld_positions(positions, bmin, bmax):
+    """weldPositions (src/io/mesh_weld.cpp:26-48): (welded index per position, first position of each index)."""
+    pos = _f32(positions).reshape(-1, 3)
+    welded = np.zeros(len(pos), np.uint32)
+    unique = np.zeros((max(len(pos), 1), 3), np.float32)
+    n = lib().orc_weld_positions(_p(pos, _f32p), len(pos), _p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p),
+                                 welded.ctypes.data, unique.ctypes.data)
+    if n < 0:
+        raise ValueError("position not representable on the weld lattice")
+    return welded, unique[:n].copy()
+
+
+def watertight(xyz, bmin, bmax):
+    """isWatertight (src/io/mesh_validate.cpp:21-59) as counts: (watertight, boundary edges, non-manifold edges)."""
+    xyz = _f32(xyz)
+    b, m = ctypes.c_int64(), ctypes.c_int64()
+    r = lib().orc_watertight(_p(xyz, _f32p), xyz.size // 9, _p(_f32(bmin), _f32p), _p(_f32(bmax), _f32p),
+                             ctypes.byref(b), ctypes.byref(m))
+    if r < 0:
+        raise ValueError("position not representable on the weld lattice")
+    return bool(r), int(b.value), int(m.value)
 
 
 def udf2_f32_brute(xyz, pts):

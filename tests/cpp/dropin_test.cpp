@@ -1,12 +1,14 @@
 // The reference's two hot-path tests, run through the C++ drop-ins of include/italy_bake.hpp:
 //   tests/sdf_sign_test.cpp:46-82  (UV sphere r=1, 32x16, bakeSdf res 24, sign + magnitude)
 //   tests/voxelize_test.cpp:37-58  (unit cube shell, voxelizeMesh res 8, membership)
+// the reference's watertight gate test (tests/mesh_validate_test.cpp:37-62) with its messages,
 // and the procedural scenes of src/convert/sdf_procedural.cpp:81-193 through the analytic drop-ins,
 // checked bit for bit against the oracle's per-cell restatement of the same lambdas.
 // Checks print a line and set a non-zero exit status instead of relying on assert.
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "ib_oracle.h"
@@ -119,6 +121,50 @@ void voxelizeTest() {
   std::printf("voxelize_test: %zu cells\n", grid.cells.size());
 }
 
+// tests/mesh_validate_test.cpp:37-62 through the drop-ins (its makeUnitCube has the quads of the cube
+// shell above), with the reference's message text; weldPositions against the oracle's restatement
+void meshValidateTest() {
+  {
+    const Mesh cube = makeUnitCubeShell();
+    std::string err;
+    expect(italy_bake::isWatertight(cube, err), "unit cube is watertight");
+    expect(err.empty(), "no message for a watertight cube");
+    std::vector<vec3> unique;
+    const std::vector<uint32_t> w = italy_bake::weldPositions(cube.positions, cube.boundsMin, cube.boundsMax, &unique);
+    expect(italy_bake::weldedIndexCount(w) == 8 && unique.size() == 8, "unit cube welds to 8 corners");
+    std::vector<uint32_t> ow(cube.positions.size());
+    std::vector<float> ou(3 * cube.positions.size());
+    const float bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
+    const int64_t on = orc_weld_positions(&cube.positions[0].x, cube.positions.size(), bmin, bmax, ow.data(), ou.data());
+    expect(on == 8 && ow == w, "weld indices equal the oracle's");
+    expect(std::memcmp(ou.data(), unique.data(), 8 * sizeof(vec3)) == 0, "unique positions equal the oracle's");
+  }
+  {
+    Mesh holey = makeUnitCubeShell();
+    holey.positions.resize(holey.positions.size() - 6);
+    std::string err;
+    expect(!italy_bake::isWatertight(holey, err), "holey cube is not watertight");
+    expect(err == "mesh is not watertight (4 boundary edges) \u2014 fill holes and merge duplicate vertices in "
+                  "your DCC tool and re-export",
+           "holey cube message");
+  }
+  {
+    Mesh dup = makeUnitCubeShell();
+    for (int k = 0; k < 3; ++k) dup.positions.push_back(dup.positions[k]);
+    std::string err;
+    expect(!italy_bake::isWatertight(dup, err), "duplicated face is not watertight");
+    expect(err == "mesh is not watertight (0 boundary edges, 3 non-manifold edges) \u2014 fill holes and merge "
+                  "duplicate vertices in your DCC tool and re-export",
+           "non-manifold message");
+  }
+  {
+    Mesh empty;
+    std::string err;
+    expect(!italy_bake::isWatertight(empty, err), "empty mesh is not watertight");
+    expect(err == "mesh has no triangles", "empty mesh message");
+  }
+}
+
 }  // namespace
 
 void cupTest(bool glass, float cx, float cz) {
@@ -162,6 +208,7 @@ int main() {
   try {
     sdfSignTest();
     voxelizeTest();
+    meshValidateTest();
     cupTest(false, -1.6f, 1.6f);
     cupTest(true, 0.0f, 0.0f);
     probeTest();

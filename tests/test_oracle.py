@@ -391,3 +391,19 @@ def test_trace_grid_sphere(oracle):
     assert np.abs(t - 0.25).max() < 0.1 * h
     away = oracle.trace_grid(origin, h, dist, org, u)
     assert (away == -1).all()
+
+
+@pytest.mark.parametrize("which", ["icosphere", "displaced", "stress", "cube"])
+def test_udf_points_f64_equals_brute_force(oracle, meshgen, which):
+    """The FP64 LBVH query returns the brute-force FP64 minimum bit for bit (seeded points in and around
+    the mesh, including points on its vertices)."""
+    m = {"icosphere": lambda: meshgen.icosphere(4, 0.5, 1e-3), "displaced": lambda: meshgen.displaced(5),
+         "stress": lambda: meshgen.stress(ncomp=8), "cube": meshgen.cube_shell}[which]()
+    rng = np.random.default_rng(17)
+    lo, hi = m.bmin - 0.3, m.bmax + 0.3
+    pts = rng.uniform(lo, hi, (3000, 3))
+    pts[:200] = m.xyz.reshape(-1, 3)[rng.choice(m.ntri * 3, 200)]
+    d_bvh = oracle.udf_points_f64(m.xyz, pts)
+    d_brute, _ = oracle.sdf_f64_points(m.xyz, pts)
+    assert np.array_equal(d_bvh, np.abs(d_brute))
+    assert (d_bvh[:200] == 0).all()
