@@ -27,7 +27,7 @@ PEAK_HBM_GBS = 8000.0
 # clamps and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
 # triangle distance in the triangle's plane frame: offset 3 + three frame dots 15 + edge 0 (scale 1,
 # fma 2, square 3) 6 + edge 1 (offset 1, dot 3, scale 1, two fma 4, square 3) 12 + edge 2 (offsets 2,
-# dot 3, scale 1, two fma 4, square 3) 13 + two edge functions 6 + w^2 and add 2 = 57
+# dot 3, scale 1, two fma 4, square 3) 13 + two edge functions 6 + w^2 added to the in-plane term (one fma) 2 = 57
 FLOP_BOX = 11
 FLOP_TRI = 57
 # FLOPs per cell of the glass cup's distance lambda (sdf_procedural.cpp:123-130; sqrt and divide = 1):
@@ -51,6 +51,7 @@ def parse():
     ap.add_argument("--no-consumer", action="store_true", help="skip the grid consumer (sampling / tracing)")
     ap.add_argument("--no-analytic", action="store_true", help="skip the analytic CSG bake (glass cup at 320)")
     ap.add_argument("--no-ingest", action="store_true", help="skip the weld / watertight gate")
+    ap.add_argument("--no-material", action="store_true", help="skip the per-cell material splat")
     ap.add_argument("--serve-seconds", type=float, default=8.0,
                     help="arrival window of the served workload on the batched mesh distribution (0: skip)")
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
@@ -241,6 +242,41 @@ def ingest_section(ctx, mesh, d_xyz, cpu=True, steps=10):
     return out
 
 
+def material_section(ctx, mesh, res=256, cpu=True, steps=3):
+    """bakePerCellMaterial (sdf_baker.cpp:76-128), the per-cell material half of bakeSdf(mesh, 256): every
+    cell a triangle's box test touches (the bakeSdf framing of the 327,680-triangle mesh) averages the
+    triangle's (r, g, b, metallic, roughness) in triangle order.  Host buffers in and out, as bakeSdf
+    returns them; beside it the oracle's single-threaded restatement (the reference's loop is
+    single-threaded)."""
+    from italy_bake import bake
+
+    g = bake.sdf_framing(mesh.bmin, mesh.bmax, res)
+    mat = np.random.default_rng(5).uniform(0, 1, (mesh.ntri, 5)).astype(np.float32)
+    ctx.splat_materials(mesh.xyz, mat, g)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        rgb, met, rough = ctx.splat_materials(mesh.xyz, mat, g)
+    gpu_s = (time.perf_counter() - t0) / steps
+    ncell = g.nx * g.ny * g.nz
+    out = {"workload": f"bakePerCellMaterial of the {mesh.ntri}-triangle mesh on bakeSdf's {g.nx}x{g.ny}x{g.nz} grid "
+                       "(res 256), host buffers in and out (PCIe included)",
+           "cells": ncell, "ms_per_splat": gpu_s * 1e3, "triangles_per_s": mesh.ntri / gpu_s}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        t0 = time.perf_counter()
+        ref = ib_oracle.splat_materials(mesh.xyz, mat, list(g.origin), g.h, g.nx, g.ny, g.nz)
+        cpu_s = time.perf_counter() - t0
+        ref = ref.reshape(5, -1)
+        out["matches_oracle"] = bool(np.array_equal(rgb.T, ref[:3]) and np.array_equal(met, ref[3])
+                                     and np.array_equal(rough, ref[4]))
+        out["cpu_baseline"] = {"value": mesh.ntri / cpu_s, "unit": "triangles/s", "cores": 1, "kind": "port",
+                               "seconds": cpu_s, "gpu_multiple": cpu_s / gpu_s,
+                               "sample": "the whole mesh on one core, in triangle order"}
+    return out
+
+
 def voxelize_section(ctx, mesh, res=256, cpu=True, steps=3):
     """italy::voxelizeMesh (voxelize.cpp:52-107) as the reference calls it: mesh in host memory, cells in
     first-touch order back in host memory, on the 327,680-triangle mesh at res 256.  Beside it the
@@ -343,8 +379,8 @@ def batched_section(ctx, n):
     st = ctx.stats()
     for p in ptrs:
         ctx.free(p)
-    ctx.set_option(bake.OPT_BRICK_SHAPE, 0)
-    ctx.set_option(bake.OPT_BLOCKS_PER_CU, 8)
+    for opt in (bake.OPT_BRICK_SHAPE, bake.OPT_BLOCKS_PER_CU):  # the autotuner's choice applies to batches only
+        ctx.set_option(opt, bake.DEFAULT_OPTIONS[opt])
     return {
         "workload": f"{n} displaced icospheres, log-uniform 500..200k triangles ({tris} total), SDF + occupancy "
                     f"at 32/64/128/256^3 over [-0.8,0.8]^3 ({voxels} voxels)",
@@ -360,8 +396,9 @@ def batched_section(ctx, n):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the bake shards by mesh or "
-              "brick across GPUs without communication, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (batched and served bakes "
+              "shard by whole meshes and one large bake by brick ranges, with no communication between GPUs; "
+              "only the one-GPU share is measured here)", file=sys.stderr)
         sys.exit(2)
     from italy_bake import bake, meshgen
 
@@ -430,6 +467,7 @@ def main():
     analytic = analytic_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_analytic else None
     voxelize = voxelize_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_voxelize else None
     ingest = ingest_section(ctx, mesh, d_xyz, cpu=not args.no_cpu_baseline) if not args.no_ingest else None
+    material = material_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_material else None
     served = served_section(args, batched)
 
     cpu = None
@@ -463,7 +501,7 @@ def main():
         "device_ms_per_bake": float(np.mean(total_ms)),
         "roofline": roofline, "cpu_baseline": cpu, "batched": batched, "narrow_band": band, "served": served,
         "grid_consumer": consumer, "analytic_bake": analytic, "voxelize_mesh": voxelize,
-        "ingest_gate": ingest,
+        "ingest_gate": ingest, "material_splat": material,
         "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
         if served and served.get("batched_distribution") else None,
     }

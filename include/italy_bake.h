@@ -83,7 +83,7 @@ typedef enum {
   IB_OPT_TILE_ORDER = 5,    /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
                                first (estimated from the column crossings), 2 (default) half-octave cost
                                buckets, costliest first, Morton order inside a bucket.  Speed only. */
-  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..8, default 4) are evaluated
+  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..16, default 8) are evaluated
                                as one block from their parent.  Speed only; applies to later builds. */
 } ib_option;
 
@@ -191,7 +191,8 @@ ib_status ib_bake_band_device(ib_context *ctx, const float *d_xyz, uint64_t ntri
                               int32_t band_voxels, int32_t *d_brick_index, float *d_pool, uint64_t pool_bricks,
                               uint64_t *nbricks, void *stream);
 
-/* Autotuner for batched bakes: bakes every sample_stride-th job of `jobs` with each brick shape
+/* Autotuner for batched bakes: bakes every sample_stride-th job of `jobs` (in order of voxel count, so that
+ * the sample spans the batch's grid sizes) with each brick shape
  * (IB_OPT_BRICK_SHAPE) and persistent-block count (IB_OPT_BLOCKS_PER_CU), times the persistent
  * query launch with HIP events, and leaves the fastest setting on the context.  The choice changes
  * speed only, never results.  Outputs (may be NULL): the chosen shape, blocks per CU and time. */
@@ -245,6 +246,7 @@ ib_status ib_server_fetch(ib_server *srv, uint64_t ticket, float *distances, uin
 /* device pointers of a finished result (valid until release) */
 ib_status ib_server_device_result(ib_server *srv, uint64_t ticket, const float **d_distances,
                                   const int32_t **d_brick_index, const uint32_t **d_occ_words);
+/* frees a finished result; IB_ERR_NOT_READY while the ticket runs or another thread fetches it */
 ib_status ib_server_release(ib_server *srv, uint64_t ticket);
 ib_status ib_server_stats(ib_server *srv, ib_server_counters *out);
 

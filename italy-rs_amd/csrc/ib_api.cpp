@@ -65,7 +65,7 @@ struct ib_context {
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
-  int leaf_block = 4;          // LBVH subtrees of at most this many triangles are evaluated as one leaf block
+  int leaf_block = 8;          // LBVH subtrees of at most this many triangles are evaluated as one leaf block
   int tile_order = 2;          // dense per-brick queries: 0 grid order, 1 costliest first, 2 cost buckets + Morton
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
@@ -445,7 +445,8 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits, 0u, n, s);
-  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, c->leaf_block, m.nodes4, s);
+  const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, block, m.nodes4, s);
   m.n = n;
   if (!arena) c->record(EV_BVH, s);
   return IB_OK;
@@ -836,7 +837,7 @@ ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, con
   constexpr int K = 5;
   float *d_out;
   IB_TRY(c->scratch("sp_out", (K + 0) * ncell, &d_out));
-  // defaults of sdf_baker.cpp:77-80: colour 1, metallic 0, roughness 1
+  // defaults of sdf_baker.cpp:77-80: colour 1 (3 interleaved floats per cell), metallic 0, roughness 1
   fill_u32(reinterpret_cast<uint32_t *>(d_out), 0x3f800000u, 3 * ncell, s);
   fill_u32(reinterpret_cast<uint32_t *>(d_out) + 3 * ncell, 0u, ncell, s);
   fill_u32(reinterpret_cast<uint32_t *>(d_out) + 4 * ncell, 0x3f800000u, ncell, s);
@@ -877,17 +878,12 @@ ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, con
       launch_splat_reduce(cell_s, tri_s, npairs, d_attr, K, d_out, ncell, s);
     }
   }
-  // planes on the device; the host layout interleaves rgb per cell (std::vector<glm::vec3>)
-  std::vector<float> planes(K * ncell);
-  IB_HIP(hipMemcpyAsync(planes.data(), d_out, K * ncell * sizeof(float), hipMemcpyDeviceToHost, s));
+  // the device buffer already has the host layout: rgb interleaved per cell (std::vector<glm::vec3>),
+  // then the metallic and roughness planes
+  IB_HIP(hipMemcpyAsync(base_color, d_out, 3 * ncell * sizeof(float), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipMemcpyAsync(metallic, d_out + 3 * ncell, ncell * sizeof(float), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipMemcpyAsync(roughness, d_out + 4 * ncell, ncell * sizeof(float), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
-  for (uint64_t i = 0; i < ncell; ++i) {
-    base_color[3 * i] = planes[i];
-    base_color[3 * i + 1] = planes[ncell + i];
-    base_color[3 * i + 2] = planes[2 * ncell + i];
-    metallic[i] = planes[3 * ncell + i];
-    roughness[i] = planes[4 * ncell + i];
-  }
   return IB_OK;
 }
 
@@ -1561,8 +1557,18 @@ ib_status ib_autotune_batch(ib_context *c, const ib_job *jobs, uint32_t njobs, u
   // the band pass uses fixed 4x4x4 sub-bricks; the shape and block count tune the dense tiles
   if (!c || (njobs && !jobs)) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
   if (sample_stride == 0) sample_stride = 1;
+  // stratified sample: every sample_stride-th job in order of its voxel count, so that the sample
+  // holds large and small grids in the batch's proportions
+  std::vector<uint32_t> order(njobs);
+  for (uint32_t j = 0; j < njobs; ++j) order[j] = j;
+  const auto cells = [&](uint32_t j) {
+    const ib_grid &g = jobs[j].sdf;
+    return jobs[j].band_voxels > 0 ? 0ull : (uint64_t)g.nx * g.ny * g.nz;
+  };
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cells(a) > cells(b); });
   std::vector<ib_job> sample;
-  for (uint32_t j = 0; j < njobs; j += sample_stride) sample.push_back(jobs[j]);
+  for (uint32_t k = sample_stride / 2; k < njobs; k += sample_stride) sample.push_back(jobs[order[k]]);
+  if (sample.empty() && njobs) sample.push_back(jobs[order[0]]);
   static const int kBlocks[3] = {4, 8, 16};
   int best_shape = c->brick_shape, best_blocks = c->blocks_per_cu;
   double best_ms = 1e300;
@@ -1615,7 +1621,8 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
       c->blocks_per_cu = (int)value;
       return IB_OK;
     case IB_OPT_LEAF_BLOCK:
-      if (value < 1 || value > kMaxLeafBlock) return fail(IB_ERR_INVALID_ARGUMENT, "leaf block must be 1..8");
+      if (value < 1 || value > kMaxLeafBlock)
+        return fail(IB_ERR_INVALID_ARGUMENT, "leaf block must be 1.." + std::to_string(kMaxLeafBlock));
       c->leaf_block = (int)value;
       return IB_OK;
     case IB_OPT_TILE_ORDER:

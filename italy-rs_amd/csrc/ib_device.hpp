@@ -11,10 +11,11 @@ constexpr int kStack = 64;        // traversal stack entries (>= LBVH depth, che
 constexpr int kStack4 = 128;      // brick query stack over the 4-wide view: <= 3 pushes per level
 constexpr int kColCap = 32;       // crossings stored per +z column before the per-voxel fallback
 constexpr uint32_t kSliver = 1u;  // triangle flag: distance evaluated in FP64
-// leaf code ~L: first triangle L & kBlockMask, count (L >> kBlockShift) + 1 (meshes < 2^28 triangles)
-constexpr int kBlockShift = 28;
+// leaf code ~L: first triangle L & kBlockMask, count (L >> kBlockShift) + 1; blocks of more than one
+// triangle are built only for meshes of at most 2^27 triangles, so that L stays non-negative
+constexpr int kBlockShift = 27;
 constexpr int kBlockMask = (1 << kBlockShift) - 1;
-constexpr int kMaxLeafBlock = 8;
+constexpr int kMaxLeafBlock = 16;
 
 // Internal LBVH node: both children's boxes inline so one 64-byte record decides both children.
 // child >= 0: internal node index, child < 0: ~leaf (sorted triangle position).
@@ -246,13 +247,20 @@ __device__ __forceinline__ void make_trirec(F3 p0, F3 p1, F3 p2, uint32_t flags,
 
 __device__ __forceinline__ float clamp01(float x) { return fminf(fmaxf(x, 0.0f), 1.0f); }
 
+// Minimum of two squared distances (non-negative floats or +inf) through their bit patterns, which
+// order like unsigned integers: the same value as fminf, without the canonicalisation fminf needs
+// for an operand carried around a loop (one instruction per triangle evaluated).
+__device__ __forceinline__ float min_nonneg(float a, float b) {
+  return __uint_as_float(min(__float_as_uint(a), __float_as_uint(b)));
+}
+
 // FP32 squared distance from q to a non-sliver triangle, branch free.  With d = q - o and
 // (u, v, w) = (dot(X, d), dot(Y, d), dot(Z, d)):
 //   segment 0 (p0 p1): t = clamp(u * inv_len), r = fma(-t, len, u), d0 = fma(v, v, r * r)
 //   segment 1 (p1 p2): a = (u - len, v), t = clamp(dot(a, e1) * inv_e1), r = fma(-t, e1, a), d1 = |r|^2
 //   segment 2 (p2 p0): a = (u - c0, v - c1), t = clamp(dot(a, c) * nie2), r = fma(t, c, a), d2 = |r|^2
 //   inside = max(-v, -k1, k2) <= 0, k1 = fma(e1_0, v, -(e1_1 * a1_0)), k2 = fma(c0, a2_1, -(c1 * a2_0))
-//   result (inside ? 0 : min(d0, d1, d2)) + w * w
+//   result fma(w, w, inside ? 0 : min(d0, d1, d2))
 // (2D dots as fma(a_1, b_1, a_0 * b_0)).
 __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
   const F3 d = {q.x - t.o[0], q.y - t.o[1], q.z - t.o[2]};
@@ -274,7 +282,7 @@ __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
   const float k2 = fmaf(t.c[0], a2y, -(t.c[1] * a2x));
   const bool inside = fmaxf(fmaxf(-v, -k1), k2) <= 0.0f;
   const float dmin = fminf(fminf(d0, d1), d2);
-  return (inside ? 0.0f : dmin) + w * w;
+  return fmaf(w, w, inside ? 0.0f : dmin);
 }
 
 // Pruning with FP32 distances.  |sqrt(tri_d2_f32(q, t)) - dist(q, t)| stays below

@@ -502,7 +502,9 @@ __global__ __launch_bounds__(256) void k_splat_reduce(const uint32_t *__restrict
     ++cnt;
   }
   const float fc = (float)cnt;
-  for (int j = 0; j < k && j < 8; ++j) out[(size_t)j * ncell + c] = sum[j] / fc;
+  // attributes 0..2 (the colour) interleaved per cell, as std::vector<glm::vec3> holds them; the others
+  // in planes of ncell after them
+  for (int j = 0; j < k && j < 8; ++j) out[j < 3 ? 3 * (size_t)c + j : (size_t)j * ncell + c] = sum[j] / fc;
 }
 
 __global__ __launch_bounds__(256) void k_first_compact(const int32_t *__restrict__ first, uint64_t ncell,
@@ -542,7 +544,7 @@ struct ColumnSink {
 };
 
 // A leaf child code ~L names the triangles L & kBlockMask .. + (L >> kBlockShift): one triangle, or
-// a collapsed block of up to 8 consecutive ones.
+// a collapsed block of up to kMaxLeafBlock consecutive ones.
 template <typename F>
 __device__ __forceinline__ void for_leaf_code(int code, F &&leaf) {
   const int t = code & kBlockMask, last = t + (code >> kBlockShift);
@@ -850,14 +852,14 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     if (COUNT) wc.box += 2;
     if (nd.c0 < 0 && d0 <= bound) {
       for_leaf_code(~nd.c0, [&](int l) {
-        best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
+        best = min_nonneg(best, tri_d2<SLIVERS>(q, recs[l]));
         if (COUNT) wc.tri++;
       });
       bound = prune_bound(best, slack);
     }
     if (nd.c1 < 0 && d1 <= bound) {
       for_leaf_code(~nd.c1, [&](int l) {
-        best = fminf(best, tri_d2<SLIVERS>(q, recs[l]));
+        best = min_nonneg(best, tri_d2<SLIVERS>(q, recs[l]));
         if (COUNT) wc.tri++;
       });
       bound = prune_bound(best, slack);
@@ -1118,7 +1120,7 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   int cur = 0;
   auto eval = [&](int l) {
     const TriRec t = load_trirec_uniform(recs, l);
-    for (int k = 0; k < V; ++k) best[k] = fminf(best[k], tri_d2<SLIVERS>(q[k], t));
+    for (int k = 0; k < V; ++k) best[k] = min_nonneg(best[k], tri_d2<SLIVERS>(q[k], t));
     if (COUNT) wc.tri += V;
   };
   // compare-exchange of (key, child) pairs, ascending by key (wave-uniform values)

@@ -133,6 +133,8 @@ void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_
                        int32_t *tri, hipStream_t s);
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                           int32_t *vout, uint32_t n, int end_bit, hipStream_t s);
+// per-cell means of the k attributes (k <= 8) of the (cell, triangle) pairs sorted by cell: attributes 0..2
+// interleaved per cell at out[3 * cell + j], attribute j >= 3 at out[j * ncell + cell]
 void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
                          float *out, uint64_t ncell, hipStream_t s);
 

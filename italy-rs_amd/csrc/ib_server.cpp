@@ -95,6 +95,7 @@ struct Ticket {
   uint32_t *d_occ = nullptr;
   uint64_t out_bytes = 0, index_bytes = 0, occ_bytes = 0, pool_bricks = 0, nbricks = 0;
   int worker = -1;
+  int fetching = 0;        // fetches copying the result right now; release waits for none
 };
 
 uint64_t voxels(const ib_grid &g) { return (uint64_t)g.nx * g.ny * g.nz; }
@@ -401,20 +402,31 @@ ib_status ib_server_fetch(ib_server *s, uint64_t ticket, float *distances, uint6
     if (it->second.state == ST_FAILED) return sfail(it->second.status, it->second.err);
     if (it->second.state != ST_DONE) return sfail(IB_ERR_NOT_READY, "ticket still running");
     t = it->second;
+    ++it->second.fetching;  // the device buffers stay owned by this ticket until the copies end
   }
-  (void)hipSetDevice(s->device);
-  const uint64_t floats = t.req.band_voxels > 0 ? t.nbricks * 512 : (t.req.sdf.nx ? voxels(t.req.sdf) : 0);
-  if (distances && floats) {
-    if (capacity < floats) return sfail(IB_ERR_CAPACITY, "distance buffer holds " + std::to_string(capacity) +
-                                                             " floats, the result has " + std::to_string(floats));
-    if (hipMemcpy(distances, t.d_out, floats * 4, hipMemcpyDeviceToHost) != hipSuccess)
-      return sfail(IB_ERR_DEVICE, "copying the distances failed");
+  const auto copies = [&]() -> ib_status {
+    (void)hipSetDevice(s->device);
+    const uint64_t floats = t.req.band_voxels > 0 ? t.nbricks * 512 : (t.req.sdf.nx ? voxels(t.req.sdf) : 0);
+    if (distances && floats) {
+      if (capacity < floats) return sfail(IB_ERR_CAPACITY, "distance buffer holds " + std::to_string(capacity) +
+                                                               " floats, the result has " + std::to_string(floats));
+      if (hipMemcpy(distances, t.d_out, floats * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return sfail(IB_ERR_DEVICE, "copying the distances failed");
+    }
+    if (brick_index && t.d_index &&
+        hipMemcpy(brick_index, t.d_index, t.index_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      return sfail(IB_ERR_DEVICE, "copying the brick index failed");
+    if (occ_words && t.d_occ && hipMemcpy(occ_words, t.d_occ, t.occ_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+      return sfail(IB_ERR_DEVICE, "copying the occupancy failed");
+    return IB_OK;
+  };
+  const ib_status st = copies();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    --s->tickets[ticket].fetching;
   }
-  if (brick_index && t.d_index && hipMemcpy(brick_index, t.d_index, t.index_bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    return sfail(IB_ERR_DEVICE, "copying the brick index failed");
-  if (occ_words && t.d_occ && hipMemcpy(occ_words, t.d_occ, t.occ_bytes, hipMemcpyDeviceToHost) != hipSuccess)
-    return sfail(IB_ERR_DEVICE, "copying the occupancy failed");
-  return IB_OK;
+  s->cv_done.notify_all();
+  return st;
 }
 
 ib_status ib_server_device_result(ib_server *s, uint64_t ticket, const float **d_distances, const int32_t **d_index,
@@ -437,6 +449,7 @@ ib_status ib_server_release(ib_server *s, uint64_t ticket) {
   if (it == s->tickets.end()) return sfail(IB_ERR_INVALID_ARGUMENT, "unknown ticket");
   Ticket &t = it->second;
   if (t.state == ST_QUEUED || t.state == ST_RUNNING) return sfail(IB_ERR_NOT_READY, "ticket still running");
+  if (t.fetching) return sfail(IB_ERR_NOT_READY, "ticket is being fetched");
   s->cache.put(t.d_out, t.out_bytes);
   s->cache.put(t.d_index, t.index_bytes);
   s->cache.put(t.d_occ, t.occ_bytes);

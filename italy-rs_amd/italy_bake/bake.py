@@ -19,6 +19,9 @@ OPT_BRICK_SHAPE = 3
 OPT_BLOCKS_PER_CU = 4
 OPT_TILE_ORDER = 5
 OPT_LEAF_BLOCK = 6
+# the library's defaults (ib_create), per option
+DEFAULT_OPTIONS = {OPT_COUNT_WORK: 0, OPT_QUERY_VARIANT: 1, OPT_BRICK_SHAPE: 0, OPT_BLOCKS_PER_CU: 8,
+                   OPT_TILE_ORDER: 2, OPT_LEAF_BLOCK: 8}
 
 
 def _f3(v):
@@ -41,6 +44,7 @@ class Context:
         check(lib.ib_create(int(device), int(hbm_budget_bytes), ctypes.byref(h)))
         self._h = h
         self.lib = lib
+        self._options = dict(DEFAULT_OPTIONS)  # the values set on this context
 
     def close(self):
         if self._h:
@@ -233,10 +237,16 @@ class Context:
 
     def set_option(self, option, value):
         check(self.lib.ib_set_option(self._h, int(option), int(value)))
+        self._options[int(option)] = int(value)
+
+    def option(self, option):
+        """the value last set for `option` on this context (the library default if never set)"""
+        return self._options[int(option)]
 
     def query_counters(self, d_xyz, ntri, sdf_grid):
         """Run one counting bake and return the traversal counters of the query kernel."""
         d_out = self.alloc(4 * sdf_grid.nx * sdf_grid.ny * sdf_grid.nz)
+        saved = {k: self.option(k) for k in (OPT_COUNT_WORK, OPT_QUERY_VARIANT, OPT_LEAF_BLOCK)}
         try:
             # the algorithm's own work: one traversal per voxel over the binary LBVH with single-
             # triangle leaves (no wave union, no leaf blocks)
@@ -246,9 +256,8 @@ class Context:
             self.bake_dense_device(d_xyz, ntri, sdf_grid, d_out)
             st = self.stats()
         finally:
-            self.set_option(OPT_COUNT_WORK, 0)
-            self.set_option(OPT_QUERY_VARIANT, 1)
-            self.set_option(OPT_LEAF_BLOCK, 4)
+            for k, v in saved.items():
+                self.set_option(k, v)
             self.free(d_out)
         return {"box_tests": st["box_tests"], "tri_tests": st["tri_tests"]}
 

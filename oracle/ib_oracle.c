@@ -404,7 +404,7 @@ static inline double dot3d(const double a[3], const double b[3]) { return a[0] *
  *   segment 1: a = (u - len, v), t = clamp(fmaf(v, e1_1, a_0 * e1_0) * inv_e1), r = fmaf(-t, e1, a), d1 = |r|^2;
  *   segment 2: a = (u - c0, v - c1), t = clamp(fmaf(a_1, c1, a_0 * c0) * nie2), r = fmaf(t, c, a), d2 = |r|^2;
  *   inside = max(-v, -k1, k2) <= 0 with k1 = fmaf(e1_0, v, -(e1_1 * a1_0)), k2 = fmaf(c0, a2_1, -(c1 * a2_0));
- *   result (inside ? 0 : min(d0, d1, d2)) + w * w.  |r|^2 in 2D is fmaf(r_1, r_1, r_0 * r_0). */
+ *   result fmaf(w, w, inside ? 0 : min(d0, d1, d2)).  |r|^2 in 2D is fmaf(r_1, r_1, r_0 * r_0). */
 static inline float fdot3(const float a[3], const float b[3]) { return fmaf(a[2], b[2], fmaf(a[1], b[1], a[0] * b[0])); }
 static inline void fcross3(const float a[3], const float b[3], float o[3]) {
   o[0] = fmaf(a[1], b[2], -(a[2] * b[1]));
@@ -464,7 +464,7 @@ float orc_trirec_dist2(const float q[3], const orc_trirec *r) {
   const float k2 = fmaf(r->c[0], a2y, -(r->c[1] * a2x));
   const int inside = fmaxf(fmaxf(-v, -k1), k2) <= 0.0f;
   const float dmin = fminf(fminf(d0, d1), d2);
-  return (inside ? 0.0f : dmin) + w * w;
+  return fmaf(w, w, inside ? 0.0f : dmin);
 }
 
 float orc_tri_dist2_f32(const float q[3], const float *t) {

@@ -1,8 +1,10 @@
-"""Error of the GPU bake against the FP64 oracle (brute-force distance + winding number): every voxel
-of the 5,120-triangle 64^3 bake, seeded samples of the larger workloads.  Max and RMS |d - d64|
-(bound 1e-5 * extent) and sign agreement on every checked voxel with |d64| > 1e-4 * extent whose
-|w64| is not within 1e-6 of 0.5.  The numbers are written to gpurun_out/accuracy.json when that
-folder exists."""
+"""Error of the GPU bake against the FP64 oracle: every voxel of the 5,120-triangle 64^3 bake (brute-force
+FP64 distance and winding number), and 1,048,576 seeded voxels of the 256^3 and 384^3 narrow-band bakes
+(FP64 distance through the oracle's LBVH, equal to the brute force bit for bit; half of each sample within
+3 voxels of the surface), whose sign is checked on a 4,096-voxel subsample against the brute-force FP64
+winding number.  Max and RMS |d - d64| (bound 1e-5 * extent) and sign agreement on every checked voxel
+with |d64| > 1e-4 * extent whose |w64| is not within 1e-6 of 0.5.  The numbers are written to
+gpurun_out/accuracy.json when that folder exists."""
 import json
 import os
 
@@ -15,18 +17,32 @@ pytestmark = pytest.mark.gpu
 REPORT = {}
 
 
-def _record(name, dist, pts, xyz, extent, oracle):
-    d64, w = oracle.sdf_f64_points(xyz, pts.astype(np.float64))
-    err = dist.astype(np.float64) - d64
-    decided = (np.abs(d64) > 1e-4 * extent) & (np.abs(np.abs(w) - 0.5) > 1e-6)
-    sign_bad = int(((dist < 0) != (np.abs(w) > 0.5))[decided].sum())
-    REPORT[name] = {"samples": int(len(pts)), "max_abs_err": float(np.abs(err).max()),
-                    "rms_err": float(np.sqrt(np.mean(err ** 2))), "max_err_over_extent": float(np.abs(err).max() / extent),
-                    "sign_mismatches": sign_bad, "sign_checked": int(decided.sum())}
+def _write():
     out = os.path.join(ROOT, "gpurun_out")
     if os.path.isdir(out):
         with open(os.path.join(out, "accuracy.json"), "w") as f:
             json.dump(REPORT, f, indent=1)
+
+
+def _record(name, dist, pts, xyz, extent, oracle, sign_sample=None):
+    """dist: GPU signed distances at pts.  sign_sample: indices whose sign is checked against the FP64
+    winding number (None: every point, with brute-force FP64 distances too)."""
+    pts = pts.astype(np.float64)
+    if sign_sample is None:
+        d64s, w = oracle.sdf_f64_points(xyz, pts)
+        d64 = np.abs(d64s)
+        sign_sample = np.arange(len(pts))
+    else:
+        d64 = oracle.udf_points_f64(xyz, pts)
+        _, w = oracle.sdf_f64_points(xyz, pts[sign_sample])
+    err = np.abs(dist.astype(np.float64)) - d64
+    ds = d64[sign_sample]
+    decided = (ds > 1e-4 * extent) & (np.abs(np.abs(w) - 0.5) > 1e-6)
+    sign_bad = int(((dist[sign_sample] < 0) != (np.abs(w) > 0.5))[decided].sum())
+    REPORT[name] = {"samples": int(len(pts)), "max_abs_err": float(np.abs(err).max()),
+                    "rms_err": float(np.sqrt(np.mean(err ** 2))), "max_err_over_extent": float(np.abs(err).max() / extent),
+                    "sign_mismatches": sign_bad, "sign_checked": int(decided.sum())}
+    _write()
     assert np.abs(err).max() <= 1e-5 * extent
     assert sign_bad == 0
 
@@ -46,18 +62,25 @@ def test_accuracy_plain_small(ctx, oracle, meshgen):
     _record("plain_small_64_all_voxels", dist.reshape(-1), pts, m.xyz, cfg["hi"] - cfg["lo"], oracle)
 
 
+def _near_half(zyx, near, seed):
+    """replace the first half of a sample by voxels drawn from `near`"""
+    half = len(zyx) // 2
+    zyx[:half] = near[np.random.default_rng(seed).choice(len(near), half, replace=len(near) < half)]
+    return zyx
+
+
 def test_accuracy_plain_large(ctx, oracle, meshgen):
     from italy_bake import bake
 
     m, cfg = meshgen.workload_plain_large()
     g = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["sdf_res"])
     dist, _ = ctx.bake_dense(m.xyz, g, None)
-    zyx = _sample(g, 1024, 2)
-    # half the sample within 3 voxels of the surface, where sign and distance are hardest
-    near = np.argwhere(np.abs(dist) < 3 * g.h)
-    zyx[:512] = near[np.random.default_rng(3).choice(len(near), 512, replace=False)]
+    zyx = _sample(g, 1 << 20, 2)
+    zyx = _near_half(zyx, np.argwhere(np.abs(dist) < 3 * g.h), 3)
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    _record("plain_large_256", dist[zyx[:, 0], zyx[:, 1], zyx[:, 2]], pts, m.xyz, cfg["hi"] - cfg["lo"], oracle)
+    signs = np.concatenate([np.arange(2048), (1 << 19) + np.arange(2048)])  # near the surface, and anywhere
+    _record("plain_large_256", dist[zyx[:, 0], zyx[:, 1], zyx[:, 2]], pts, m.xyz, cfg["hi"] - cfg["lo"], oracle,
+            sign_sample=signs)
 
 
 def test_accuracy_stress_band(ctx, oracle, meshgen):
@@ -69,15 +92,22 @@ def test_accuracy_stress_band(ctx, oracle, meshgen):
     st = ctx.stats()
     slots = np.argwhere(index >= 0)
     rng = np.random.default_rng(4)
-    pick = rng.choice(len(slots), 1024)
-    loc = rng.integers(0, 8, size=(1024, 3))
-    zyx = slots[pick] * 8 + loc
+    n = 1 << 20
+    pick = rng.choice(len(slots), n)
+    loc = rng.integers(0, 8, size=(n, 3))
     vals = pool[index[tuple(slots[pick].T)], loc[:, 0], loc[:, 1], loc[:, 2]]
+    # half the sample within 3 voxels of the surface
+    near = np.argwhere(np.abs(pool) < 3 * g.h)
+    sel = near[rng.choice(len(near), n // 2)]
+    pick[: n // 2] = -1
+    zyx = slots[np.maximum(pick, 0)] * 8 + loc
+    brick_of_slot = np.full(len(pool), -1, np.int64)
+    brick_of_slot[index[tuple(slots.T)]] = np.arange(len(slots))
+    zyx[: n // 2] = slots[brick_of_slot[sel[:, 0]]] * 8 + sel[:, 1:]
+    vals[: n // 2] = pool[sel[:, 0], sel[:, 1], sel[:, 2], sel[:, 3]]
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    _record("stress_band_384", vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle)
+    signs = np.concatenate([np.arange(2048), (n // 2) + np.arange(2048)])
+    _record("stress_band_384", vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle, sign_sample=signs)
     REPORT["stress_band_384"].update(bricks=int((index >= 0).sum()), fp64_rechecked=int(st["rechecked"]),
                                      parity_ties=int(st["ties"]), boundary_edges=int(st["boundary_edges"]))
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
-        with open(os.path.join(out, "accuracy.json"), "w") as f:
-            json.dump(REPORT, f, indent=1)
+    _write()

@@ -72,16 +72,14 @@ def test_brick_shapes_agree(ctx, meshgen):
             d, _ = ctx.bake_dense(m.xyz, g, None)
             assert np.array_equal(d.view(np.uint32), ref.view(np.uint32)), order
         ctx.set_option(bake.OPT_TILE_ORDER, 2)
-        for block in (1, 2, 8):  # LBVH leaf blocks (4 is the default, ref)
+        for block in (1, 2, 4, 8, 16):  # LBVH leaf blocks (ref: the default)
             ctx.set_option(bake.OPT_LEAF_BLOCK, block)
             d, _ = ctx.bake_dense(m.xyz, g, None)
             assert np.array_equal(d.view(np.uint32), ref.view(np.uint32)), block
-        ctx.set_option(bake.OPT_LEAF_BLOCK, 4)
+        ctx.set_option(bake.OPT_LEAF_BLOCK, bake.DEFAULT_OPTIONS[bake.OPT_LEAF_BLOCK])
         ctx.set_option(bake.OPT_QUERY_VARIANT, 0)
         d, _ = ctx.bake_dense(m.xyz, g, None)
         assert np.array_equal(d.view(np.uint32), ref.view(np.uint32))
     finally:
-        ctx.set_option(bake.OPT_BRICK_SHAPE, 0)
-        ctx.set_option(bake.OPT_TILE_ORDER, 2)
-        ctx.set_option(bake.OPT_LEAF_BLOCK, 4)
-        ctx.set_option(bake.OPT_QUERY_VARIANT, 1)
+        for opt in (bake.OPT_BRICK_SHAPE, bake.OPT_TILE_ORDER, bake.OPT_LEAF_BLOCK, bake.OPT_QUERY_VARIANT):
+            ctx.set_option(opt, bake.DEFAULT_OPTIONS[opt])
