@@ -1142,16 +1142,22 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
     bool leaf[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
+      key[e] = 0xffffffffu;
+      ch[e] = -1;
+      leaf[e] = false;
+      if (e >= nd.count) continue;  // wave-uniform: unused entries cost no box test
       const float d = box_d2(q[0], nd.lo[e], nd.hi[e]);
       bool need = d <= bound[0];
       for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= bound[k];
-      const bool any = e < nd.count && __ballot(need) != 0;
+      if (__ballot(need) == 0) continue;
       const int c = nd.child[e];
-      leaf[e] = any && c < 0;
-      ch[e] = (any && c >= 0) ? c : -1;
-      key[e] = ch[e] >= 0 ? (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d), centre) : 0xffffffffu;
+      leaf[e] = c < 0;
+      if (c >= 0) {
+        ch[e] = c;
+        key[e] = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d), centre);
+      }
     }
-    if (COUNT) wc.box += 4 * V;
+    if (COUNT) wc.box += nd.count * V;
     // nearest internal entry next, the others pushed farthest first (sorting network on 4)
     cx(key[0], ch[0], key[1], ch[1]);
     cx(key[2], ch[2], key[3], ch[3]);
