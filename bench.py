@@ -57,6 +57,9 @@ def parse():
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
                     help="arrival window of the served precision-stress + large-job mix (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--batch-profile", type=str, default=None, metavar="SHAPE,BLOCKS",
+                    help="only run the batched workload twice at this brick shape and blocks per CU (no autotuner, no "
+                         "plain comparison) and print its timings: the command a kernel trace of the batch runs")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (a known-size kernel that "
                          "scales the FETCH_SIZE / WRITE_SIZE counters of a profiling run)")
@@ -393,6 +396,31 @@ def batched_section(ctx, n):
     }
 
 
+def batch_profile(shape, blocks, n=512):
+    """The batched workload alone, baked twice at a fixed brick shape and block count (for profilers)."""
+    from italy_bake import Job, bake, meshgen
+
+    ctx = bake.Context(0)
+    meshes, res = meshgen.workload_batched(n)
+    jobs = []
+    for m, r in zip(meshes, res):
+        g = bake.dense_grid(-0.8, 0.8, r)
+        d_xyz = ctx.alloc(m.xyz.nbytes)
+        ctx.upload(d_xyz, m.xyz)
+        jobs.append(Job(d_xyz, m.ntri, g, ctx.alloc(4 * r ** 3), g, ctx.alloc(4 * ((r + 31) // 32) * r * r)))
+    ctx.set_option(bake.OPT_BRICK_SHAPE, shape)
+    ctx.set_option(bake.OPT_BLOCKS_PER_CU, blocks)
+    out = []
+    for _ in range(2):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.bake_batch_device(jobs)
+        ctx.synchronize()
+        st = ctx.stats()
+        out.append({"wall_ms": (time.perf_counter() - t0) * 1e3, "query_ms": st["ms_sdf"], "total_ms": st["ms_total"]})
+    print(json.dumps({"batch_profile": {"meshes": n, "brick_shape": shape, "blocks_per_cu": blocks, "runs": out}}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -400,6 +428,10 @@ def main():
               "shard by whole meshes and one large bake by brick ranges, with no communication between GPUs; "
               "only the one-GPU share is measured here)", file=sys.stderr)
         sys.exit(2)
+    if args.batch_profile:
+        shape, blocks = (int(v) for v in args.batch_profile.split(","))
+        batch_profile(shape, blocks, args.batch_meshes or 512)
+        return
     from italy_bake import bake, meshgen
 
     mesh, cfg = meshgen.workload_plain_large()
