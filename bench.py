@@ -70,8 +70,9 @@ def served_section(args, batched):
     """Open-loop Poisson arrivals into the bake server (3 worker streams, HBM budget 200 GB).
 
     batched_distribution: requests drawn from the batched workload's distribution (dense SDF +
-      occupancy, 500..200k triangles, 32..256^3): the plain per-mesh bake of the same meshes is the
-      comparison point.
+      occupancy, 500..200k triangles, 32..256^3), keyed by mesh so that each LBVH stays resident after
+      its first request; batched_distribution_rebuilt: the same arrivals without keys (every request
+      uploaded and built).  The plain per-mesh bake of the same meshes is the comparison point.
     stress_mix: 95% precision-stress unions (384^3 narrow band) + 5% large jobs (2M-triangle displaced
       icosphere, 512^3 narrow band), the other reading of the served configuration; far beyond one
       GPU's rate at 200 req/s, so its latency is queueing."""
@@ -82,23 +83,30 @@ def served_section(args, batched):
     budget = 200 << 30
     if args.serve_seconds > 0:
         meshes, res = meshgen.workload_batched(128, seed=meshgen.BASE_SEED + 9000)
-        reqs = [(m.xyz, bake.dense_grid(-0.8, 0.8, r), 0, bake.dense_grid(-0.8, 0.8, r)) for m, r in zip(meshes, res)]
-        with Server(hbm_budget_bytes=budget, workers=3, max_batch=256) as srv:
-            open_loop(srv, reqs[:8], rate_hz=50, duration_s=0.5, seed=7)  # warm the worker contexts
-            r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_seconds, seed=1)
-        r.pop("_latencies")
-        r["workload"] = ("requests drawn from 128 meshes of the batched distribution (dense SDF + occupancy at "
-                         "32/64/128/256^3)")
-        if batched:
-            r["plain_per_mesh_meshes_per_s"] = batched["plain_meshes_per_s"]
-        out["batched_distribution"] = r
+        grids = [bake.dense_grid(-0.8, 0.8, r) for r in res]
+        # keyed: each mesh's LBVH stays resident after its first request; unkeyed: every request is built
+        for name, keyed in (("batched_distribution", True), ("batched_distribution_rebuilt", False)):
+            reqs = [(m.xyz, g, 0, g, (i + 1) if keyed else 0) for i, (m, g) in enumerate(zip(meshes, grids))]
+            with Server(hbm_budget_bytes=budget, workers=3, max_batch=256) as srv:
+                open_loop(srv, [q[:4] for q in reqs[:8]], rate_hz=50, duration_s=0.5, seed=7)  # warm the contexts
+                r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_seconds, seed=1)
+                c = srv.counters()
+            r.pop("_latencies")
+            r["workload"] = ("requests drawn from 128 meshes of the batched distribution (dense SDF + occupancy at "
+                             "32/64/128/256^3)" + (", each mesh's LBVH kept resident under its key" if keyed else
+                                                   ", every request uploaded and built"))
+            r["mesh_builds"], r["mesh_hits"] = c["mesh_builds"], c["mesh_hits"]
+            if batched:
+                r["plain_per_mesh_meshes_per_s"] = batched["plain_meshes_per_s"]
+            out[name] = r
     if args.serve_stress_seconds > 0:
         stress = [meshgen.stress(meshgen.BASE_SEED + 100 + i, 64, 0.02) for i in range(6)]
         big = meshgen.decimated(2_000_000, meshgen.BASE_SEED + 200)
         g384 = bake.dense_grid(-1.0, 1.0, 384)
         g512 = bake.dense_grid(-0.8, 0.8, 512)
         # 95% / 5%: 19 stress entries per large entry in the uniform draw
-        reqs = [(m.xyz, g384, 4, None) for m in stress] * 19 + [(big.xyz, g512, 4, None)] * 6
+        # keyed by mesh: a mesh's LBVH stays resident after its first request
+        reqs = [(m.xyz, g384, 4, None, 1 + i) for i, m in enumerate(stress)] * 19 + [(big.xyz, g512, 4, None, 99)] * 6
         with Server(hbm_budget_bytes=budget, workers=3, max_batch=64) as srv:
             r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_stress_seconds, seed=2)
         lat = r.pop("_latencies")

@@ -40,6 +40,7 @@ typedef enum {
 } ib_status;
 
 typedef struct ib_context ib_context;
+typedef struct ib_mesh ib_mesh;
 
 /* Cell-centred grid: cell (x,y,z) spans origin + [i, i+1) * h. */
 typedef struct {
@@ -88,7 +89,9 @@ typedef enum {
 } ib_option;
 
 /* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy
- * grid (occ.nx == 0: none), device outputs laid out as for ib_bake_dense_device.
+ * grid (occ.nx == 0: none), device outputs laid out as for ib_bake_dense_device.  A job may name a
+ * resident mesh (ib_mesh_create) instead of a soup: its LBVH is used as built, and d_xyz / ntri are
+ * written back from the mesh.
  * Narrow band (band_voxels > 0, grid dims divisible by 8): only the 8^3 bricks that can hold a voxel
  * with |d| <= band_voxels * h are baked.  d_brick_index gets, per brick (x fastest), its slot in the
  * pool or -1; d_distances is the pool: slot * 512 + (z%8 * 8 + y%8) * 8 + x%8; nbricks is written
@@ -105,6 +108,7 @@ typedef struct {
   int32_t *d_brick_index;
   uint64_t pool_bricks;
   uint64_t nbricks;
+  const ib_mesh *mesh;   /* NULL: build from d_xyz / ntri */
 } ib_job;
 
 /* ---- context ------------------------------------------------------------------------------ */
@@ -113,6 +117,19 @@ ib_status ib_create(int device, uint64_t hbm_budget_bytes, ib_context **out);
 void ib_destroy(ib_context *ctx);
 const char *ib_last_error(void);
 const char *ib_version(void);
+
+/* ---- resident meshes ----------------------------------------------------------------------------
+ * A mesh built once and kept on the device: a copy of its soup, its LBVH, distance records and hole caps.
+ * Batched bakes take it in place of a soup (ib_job.mesh) and skip the ingest and LBVH build.  Built from
+ * a device soup (copied, so the caller's buffer may be reused when the call returns) on `stream` (NULL =
+ * the context's stream); returns when the mesh is complete.  Its memory counts against ctx's budget.
+ * Destroy a mesh after the bakes that use it have returned and before its context.  No counterpart in
+ * the reference, which builds its acceleration structure on every bakeSdf call
+ * (src/convert/sdf_baker.cpp:170-193). */
+ib_status ib_mesh_create(ib_context *ctx, const float *d_xyz, uint64_t ntri, void *stream, ib_mesh **out);
+void ib_mesh_destroy(ib_mesh *mesh);
+uint64_t ib_mesh_bytes(const ib_mesh *mesh);   /* device bytes the mesh holds */
+uint64_t ib_mesh_triangles(const ib_mesh *mesh);
 
 /* ---- framing (pure host arithmetic, identical to the reference) ----------------------------- */
 /* grid of italy::bakeSdf(mesh, resolution): src/convert/sdf_baker.cpp:139-148 */
@@ -203,9 +220,12 @@ ib_status ib_autotune_batch(ib_context *ctx, const ib_job *jobs, uint32_t njobs,
  * A server admits requests first-come-first-served while their device footprint fits its HBM budget,
  * bakes whatever is admitted in batches on `workers` contexts at once (each on its own stream; a
  * request that arrives while batches run joins the next free worker's batch), and keeps every
- * result resident on the device until ib_server_release.  The soup passed to submit is read when
- * the request starts and must stay valid until the ticket is done.  No counterpart in the
- * reference, whose bake is a synchronous call on the UI thread (src/app/main.cpp:250-259). */
+ * result resident on the device until ib_server_release.  Requests with a mesh_key also keep their
+ * LBVH resident (a cache inside the same HBM budget; least recently used meshes are evicted when an
+ * admission needs the memory), so a mesh seen before is not uploaded or built again.  The soup passed
+ * to submit is read when the request starts and must stay valid until the ticket is done.  No
+ * counterpart in the reference, whose bake is a synchronous call on the UI thread
+ * (src/app/main.cpp:250-259). */
 typedef struct ib_server ib_server;
 
 typedef struct {
@@ -215,6 +235,8 @@ typedef struct {
   int32_t band_voxels;   /* 0: dense SDF; > 0: narrow band in 8^3 bricks (grid dims divisible by 8) */
   int32_t reserved;
   ib_grid occ;           /* occ.nx == 0: no occupancy */
+  uint64_t mesh_key;     /* != 0: the soup's identity; the server keeps the built LBVH resident under this key
+                            and later requests with the same key (and the same soup) reuse it */
 } ib_request;
 
 typedef struct {
@@ -229,6 +251,7 @@ typedef struct {
 
 typedef struct {
   uint64_t budget, in_use, cached, pending, ready, running_batches, batches, completed;
+  uint64_t mesh_hits, mesh_builds, meshes_resident, mesh_bytes;  /* resident-mesh cache (ib_request.mesh_key) */
 } ib_server_counters;
 
 ib_status ib_server_create(int device, uint64_t hbm_budget_bytes, int32_t workers, int32_t max_batch,

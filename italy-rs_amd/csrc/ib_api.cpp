@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -413,7 +414,8 @@ ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, in
 // when given (a batch builds its meshes on several streams at once, so each needs its own), else
 // from the context scratch (reuse is ordered by the stream).  Nothing is read back here.
 ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
-                        hipStream_t s) {
+                        hipStream_t s, Arena *keep = nullptr) {
+  Arena *out = keep ? keep : arena;  // the tree and records outlive the temporaries when `keep` is given
   fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
   fill_u32(sc->cmin, 0xffffffffu, 3, s);
   int2 *bnd;
@@ -429,10 +431,10 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(take(c, arena, "leaf_parent", n, &leaf_parent));
   IB_TRY(take(c, arena, "visits", n, &visits));
   IB_TRY(take(c, arena, "root_box", 8, &root_box));
-  IB_TRY(take(c, arena, "nodes", std::max(n - 1, 1), &m.nodes));
-  IB_TRY(take(c, arena, "tris", 3ull * n, &m.tris));
-  IB_TRY(take(c, arena, "trirecs", (size_t)n, &m.recs));
-  IB_TRY(take(c, arena, "nodes4", std::max(n - 1, 1), &m.nodes4));
+  IB_TRY(take(c, out, "nodes", std::max(n - 1, 1), &m.nodes));
+  IB_TRY(take(c, out, "tris", 3ull * n, &m.tris));
+  IB_TRY(take(c, out, "trirecs", (size_t)n, &m.recs));
+  IB_TRY(take(c, out, "nodes4", std::max(n - 1, 1), &m.nodes4));
   launch_prep(d_xyz, n, sc, s);
   launch_morton(d_xyz, n, sc, codes_in, idx, s);
   size_t temp_bytes = 0;
@@ -631,10 +633,62 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
 
 }  // namespace
 
+// A resident mesh: its soup copy, tree, records and caps live in `keep`, which belongs to context c.
+struct ib_mesh {
+  ib_context *c;
+  Arena keep;
+  MeshDev m;
+  float *d_xyz = nullptr;
+  uint64_t ntri = 0;
+  uint32_t boundary = 0;
+  uint64_t bytes = 0;
+  explicit ib_mesh(ib_context *ctx) : c(ctx), keep{ctx} {}
+};
+
 // ============================================================================================
 // C ABI
 // ============================================================================================
 extern "C" {
+
+ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void *stream, ib_mesh **out) {
+  if (!c || !out) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (ntri == 0 || !d_xyz) return fail(IB_ERR_INVALID_ARGUMENT, "a resident mesh needs a device soup of triangles");
+  if (ntri > (uint64_t)(INT_MAX / 9)) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  std::unique_ptr<ib_mesh> mh(new ib_mesh(c));
+  Arena temp{c};  // weld tables, Morton codes, sort space: released when the build is done
+  Scalars *sc;
+  IB_TRY(temp.get(1, &sc));
+  IB_TRY(mh->keep.get(9 * ntri, &mh->d_xyz));
+  IB_HIP(hipMemcpyAsync(mh->d_xyz, d_xyz, 9 * ntri * sizeof(float), hipMemcpyDeviceToDevice, s));
+  const int n = (int)ntri;
+  IB_TRY(enqueue_build(c, mh->d_xyz, n, sc, mh->m, &temp, s, &mh->keep));
+  IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  const Scalars h = *c->host_sc;
+  IB_TRY(finish_build(c, mh->d_xyz, n, h, mh->m, &mh->keep, false, s));
+  IB_HIP(hipStreamSynchronize(s));
+  mh->ntri = ntri;
+  mh->boundary = h.boundary;
+  for (const auto &b : mh->keep.blocks) mh->bytes += b.second;
+  *out = mh.release();
+  return IB_OK;
+}
+
+void ib_mesh_destroy(ib_mesh *mh) {
+  if (!mh) return;
+  {
+    std::lock_guard<std::mutex> lk(mh->c->mu);  // the blocks go back to the context's cache
+    mh->keep.release();
+  }
+  delete mh;
+}
+
+uint64_t ib_mesh_bytes(const ib_mesh *mh) { return mh ? mh->bytes : 0; }
+uint64_t ib_mesh_triangles(const ib_mesh *mh) { return mh ? mh->ntri : 0; }
 
 const char *ib_last_error(void) { return g_err.c_str(); }
 const char *ib_version(void) { return "italy-bake 0.1 gfx950"; }
@@ -1298,6 +1352,10 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   for (uint32_t j = 0; j < njobs; ++j) {
     ib_job &jb = jobs[j];
     const std::string tag = "job " + std::to_string(j) + ": ";
+    if (jb.mesh) {  // a resident mesh brings its own soup
+      jb.d_xyz = jb.mesh->d_xyz;
+      jb.ntri = jb.mesh->ntri;
+    }
     const bool has_sdf = jb.sdf.nx != 0, has_occ = jb.occ.nx != 0;
     // a narrow-band job may pass no pool with pool_bricks = 0 to learn the pool size (IB_ERR_CAPACITY)
     const bool sizing = jb.band_voxels > 0 && jb.pool_bricks == 0;
@@ -1334,8 +1392,12 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
     hipStream_t sj = c->side[j % ib_context::kSide];
-    if (jb.ntri == 0) {
+    if (jb.ntri == 0 || jb.mesh) {  // nothing to build: the query counters still start at zero
       fill_u32(reinterpret_cast<uint32_t *>(scs + j), 0u, sizeof(Scalars) / 4, sj);
+      if (jb.mesh) {
+        mesh[j] = jb.mesh->m;
+        if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
+      }
       continue;
     }
     IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj));
@@ -1363,12 +1425,14 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       }
       continue;
     }
-    const ib_status st = finish_build(c, jb.d_xyz, (int)jb.ntri, c->host_batch_sc[j], mesh[j], &arena, false, s);
-    if (st != IB_OK) return fail(st, "job " + std::to_string(j) + ": " + g_err);
+    if (!jb.mesh) {
+      const ib_status st = finish_build(c, jb.d_xyz, (int)jb.ntri, c->host_batch_sc[j], mesh[j], &arena, false, s);
+      if (st != IB_OK) return fail(st, "job " + std::to_string(j) + ": " + g_err);
+    }
     any_slivers |= mesh[j].slivers;
     any_caps |= mesh[j].ncap > 0;
     c->stats.triangles += jb.ntri;
-    c->stats.boundary_edges += c->host_batch_sc[j].boundary;
+    c->stats.boundary_edges += jb.mesh ? jb.mesh->boundary : c->host_batch_sc[j].boundary;
     c->stats.bvh_depth = std::max(c->stats.bvh_depth, mesh[j].depth);
   }
   c->record(EV_OCC, s);

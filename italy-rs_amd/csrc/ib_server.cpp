@@ -96,6 +96,15 @@ struct Ticket {
   uint64_t out_bytes = 0, index_bytes = 0, occ_bytes = 0, pool_bricks = 0, nbricks = 0;
   int worker = -1;
   int fetching = 0;        // fetches copying the result right now; release waits for none
+  const ib_mesh *mesh = nullptr;  // resident mesh used by the running bake (cache entry pinned)
+};
+
+// A resident mesh kept under the request's mesh_key; `users` pins it while bakes use it.
+struct CachedMesh {
+  ib_mesh *mesh = nullptr;
+  uint64_t bytes = 0;
+  uint64_t last_use = 0;
+  int users = 0;
 };
 
 uint64_t voxels(const ib_grid &g) { return (uint64_t)g.nx * g.ny * g.nz; }
@@ -118,6 +127,23 @@ struct ib_server {
   std::vector<ib_context *> ctxs;
   DevCache cache;
   uint64_t batches = 0, completed = 0;
+  std::unordered_map<uint64_t, CachedMesh> meshes;  // mesh_key -> resident mesh
+  uint64_t mesh_clock = 0, mesh_hits = 0, mesh_builds = 0, mesh_bytes = 0;
+
+  // least recently used resident meshes that no bake is using leave until `need` more bytes fit
+  void evict_locked(uint64_t need) {
+    while (in_use + need > budget) {
+      auto victim = meshes.end();
+      for (auto it = meshes.begin(); it != meshes.end(); ++it)
+        if (it->second.users == 0 && (victim == meshes.end() || it->second.last_use < victim->second.last_use))
+          victim = it;
+      if (victim == meshes.end()) return;
+      in_use -= victim->second.bytes;
+      mesh_bytes -= victim->second.bytes;
+      ib_mesh_destroy(victim->second.mesh);
+      meshes.erase(victim);
+    }
+  }
 
   // device bytes a request needs while it is baked, and what stays resident afterwards
   static void estimate(const ib_request &r, uint64_t &est, uint64_t &resident) {
@@ -141,6 +167,7 @@ struct ib_server {
   void try_admit_locked() {
     while (!pending.empty()) {
       Ticket &t = tickets[pending.front()];
+      if (in_use + t.est > budget) evict_locked(t.est);
       if (in_use + t.est > budget) {
         if (t.est > budget) {  // can never fit
           t.state = ST_FAILED;
@@ -194,7 +221,18 @@ struct ib_server {
         ib_job &j = jobs[k];
         j.ntri = r.ntri;
         bool ok = true;
-        if (r.ntri) {
+        if (r.mesh_key && r.ntri) {  // a resident mesh under this key: no upload, no build
+          std::lock_guard<std::mutex> lk(mu);
+          auto it = meshes.find(r.mesh_key);
+          if (it != meshes.end() && ib_mesh_triangles(it->second.mesh) == r.ntri) {
+            ++it->second.users;
+            it->second.last_use = ++mesh_clock;
+            ++mesh_hits;
+            t->mesh = it->second.mesh;
+            j.mesh = t->mesh;
+          }
+        }
+        if (r.ntri && !j.mesh) {
           t->d_xyz = cache.get(r.ntri * 36);
           ok &= t->d_xyz && hipMemcpy(t->d_xyz, r.xyz, r.ntri * 36, hipMemcpyHostToDevice) == hipSuccess;
           j.d_xyz = static_cast<const float *>(t->d_xyz);
@@ -230,6 +268,53 @@ struct ib_server {
           t->err = "staging the request on the device failed";
         }
       }
+      // keyed requests that missed: build their mesh once, keep it resident under the key
+      for (size_t k = 0; k < batch.size(); ++k) {
+        Ticket *t;
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          t = &tickets[batch[k]];
+        }
+        if (!t->req.mesh_key || t->mesh || !t->d_xyz || t->status != IB_OK) continue;
+        // use the entry if it appeared since staging (an earlier job of this batch, another worker); a key
+        // whose entry holds a different triangle count is baked from its soup and not cached
+        const auto take_existing = [&]() -> bool {
+          auto it = meshes.find(t->req.mesh_key);
+          if (it == meshes.end()) return false;
+          if (ib_mesh_triangles(it->second.mesh) == t->req.ntri) {
+            ++it->second.users;
+            it->second.last_use = ++mesh_clock;
+            ++mesh_hits;
+            t->mesh = it->second.mesh;
+            jobs[k].mesh = t->mesh;
+          }
+          return true;
+        };
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          if (take_existing()) continue;
+        }
+        ib_mesh *mh = nullptr;
+        if (ib_mesh_create(ctx, static_cast<const float *>(t->d_xyz), t->req.ntri, nullptr, &mh) != IB_OK) continue;
+        std::lock_guard<std::mutex> lk(mu);
+        if (take_existing()) {
+          ib_mesh_destroy(mh);
+          continue;
+        }
+        {
+          CachedMesh cm;
+          cm.mesh = mh;
+          cm.bytes = ib_mesh_bytes(mh);
+          cm.last_use = ++mesh_clock;
+          cm.users = 1;
+          meshes.emplace(t->req.mesh_key, cm);
+          in_use += cm.bytes;
+          mesh_bytes += cm.bytes;
+          ++mesh_builds;
+          t->mesh = mh;
+        }
+        jobs[k].mesh = t->mesh;
+      }
       // drop requests that could not be staged, bake the rest as one batch
       std::vector<ib_job> run;
       std::vector<uint64_t> ids;
@@ -258,6 +343,11 @@ struct ib_server {
           t.t_done = t1;
           cache.put(t.d_xyz, t.req.ntri * 36);  // the soup is no longer needed on the device
           t.d_xyz = nullptr;
+          if (t.mesh) {
+            auto it = meshes.find(t.req.mesh_key);
+            if (it != meshes.end() && it->second.mesh == t.mesh) --it->second.users;
+            t.mesh = nullptr;
+          }
           t.resident = t.out_bytes + t.index_bytes + t.occ_bytes;
           in_use -= t.est;
           in_use += t.resident;
@@ -329,6 +419,8 @@ void ib_server_destroy(ib_server *s) {
     s->cache.put(t.d_occ, t.occ_bytes);
     s->cache.put(t.d_xyz, t.req.ntri * 36);
   }
+  for (auto &kv : s->meshes) ib_mesh_destroy(kv.second.mesh);  // before the contexts that own their memory
+  s->meshes.clear();
   s->cache.clear();
   for (auto *c : s->ctxs) ib_destroy(c);
   delete s;
@@ -470,6 +562,10 @@ ib_status ib_server_stats(ib_server *s, ib_server_counters *out) {
   out->running_batches = (uint64_t)s->running;
   out->batches = s->batches;
   out->completed = s->completed;
+  out->mesh_hits = s->mesh_hits;
+  out->mesh_builds = s->mesh_builds;
+  out->meshes_resident = s->meshes.size();
+  out->mesh_bytes = s->mesh_bytes;
   return IB_OK;
 }
 

@@ -24,6 +24,7 @@ EXPORTS = [
     "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
     "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
     "ib_weld_positions", "ib_weld_positions_device", "ib_watertight", "ib_watertight_device",
+    "ib_mesh_create", "ib_mesh_destroy", "ib_mesh_bytes", "ib_mesh_triangles",
 ]
 
 
@@ -71,7 +72,7 @@ class Job(ctypes.Structure):
     _fields_ = [("d_xyz", ctypes.c_void_p), ("ntri", ctypes.c_uint64), ("sdf", Grid), ("d_distances", ctypes.c_void_p),
                 ("occ", Grid), ("d_occ_words", ctypes.c_void_p), ("band_voxels", ctypes.c_int32),
                 ("reserved", ctypes.c_int32), ("d_brick_index", ctypes.c_void_p), ("pool_bricks", ctypes.c_uint64),
-                ("nbricks", ctypes.c_uint64)]
+                ("nbricks", ctypes.c_uint64), ("mesh", ctypes.c_void_p)]
 
 
 _lib = None
@@ -120,6 +121,13 @@ def load():
     lib.ib_autotune_batch.argtypes = [_vp, ctypes.POINTER(Job), ctypes.c_uint32, ctypes.c_uint32,
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_double)]
+    lib.ib_mesh_create.argtypes = [_vp, _vp, _u64, _vp, ctypes.POINTER(_vp)]
+    lib.ib_mesh_destroy.argtypes = [_vp]
+    lib.ib_mesh_destroy.restype = None
+    lib.ib_mesh_bytes.argtypes = [_vp]
+    lib.ib_mesh_bytes.restype = _u64
+    lib.ib_mesh_triangles.argtypes = [_vp]
+    lib.ib_mesh_triangles.restype = _u64
     lib.ib_weld_positions.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64)]
     lib.ib_weld_positions_device.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64), _vp]
     lib.ib_watertight.argtypes = [_vp, _vp, _u64, _f3, _f3, ctypes.POINTER(_i32), ctypes.POINTER(_u64),

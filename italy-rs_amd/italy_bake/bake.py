@@ -91,11 +91,14 @@ class Context:
 
     def bake_batch_device(self, jobs, stream=None):
         """One batched bake (ib_bake_batch_device) of a list of Job structures; synchronous.
-        Narrow-band jobs get their nbricks written back into the list."""
+        Narrow-band jobs get their nbricks written back into the list, jobs naming a resident mesh its
+        soup pointer and triangle count."""
         arr = (Job * len(jobs))(*jobs)
         st = self.lib.ib_bake_batch_device(self._h, arr, len(jobs), ctypes.c_void_p(stream) if stream else None)
-        for i in range(len(jobs)):
+        for i in range(len(jobs)):  # written back by the library: band sizes, a resident mesh's soup
             jobs[i].nbricks = arr[i].nbricks
+            jobs[i].d_xyz = arr[i].d_xyz
+            jobs[i].ntri = arr[i].ntri
         check(st)
 
     def bake_band(self, xyz, grid, band_voxels):
@@ -132,6 +135,21 @@ class Context:
         check(self.lib.ib_autotune_batch(self._h, arr, len(jobs), int(sample_stride), ctypes.byref(shape),
                                          ctypes.byref(blocks), ctypes.byref(ms)))
         return shape.value, blocks.value, ms.value
+
+    # ---- resident meshes -----------------------------------------------------------------------
+    def mesh_create(self, d_xyz, ntri, stream=None):
+        """ib_mesh_create: build a device soup's LBVH once and keep it resident; returns the handle
+        (a Job may name it in place of its soup).  Free it with mesh_destroy before the context closes."""
+        h = ctypes.c_void_p()
+        check(self.lib.ib_mesh_create(self._h, ctypes.c_void_p(d_xyz), int(ntri),
+                                      ctypes.c_void_p(stream) if stream else None, ctypes.byref(h)))
+        return h.value
+
+    def mesh_destroy(self, mesh):
+        self.lib.ib_mesh_destroy(ctypes.c_void_p(mesh))
+
+    def mesh_bytes(self, mesh):
+        return int(self.lib.ib_mesh_bytes(ctypes.c_void_p(mesh)))
 
     # ---- mesh ingestion gate ------------------------------------------------------------------
     def weld_positions(self, positions, bmin, bmax):

@@ -13,7 +13,7 @@ STATE_NAMES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 class Request(ctypes.Structure):
     _fields_ = [("xyz", ctypes.c_void_p), ("ntri", ctypes.c_uint64), ("sdf", Grid), ("band_voxels", ctypes.c_int32),
-                ("reserved", ctypes.c_int32), ("occ", Grid)]
+                ("reserved", ctypes.c_int32), ("occ", Grid), ("mesh_key", ctypes.c_uint64)]
 
 
 class TicketInfo(ctypes.Structure):
@@ -25,7 +25,8 @@ class TicketInfo(ctypes.Structure):
 
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
-                ("budget", "in_use", "cached", "pending", "ready", "running_batches", "batches", "completed")]
+                ("budget", "in_use", "cached", "pending", "ready", "running_batches", "batches", "completed",
+                 "mesh_hits", "mesh_builds", "meshes_resident", "mesh_bytes")]
 
 
 def _bind(lib):
@@ -72,11 +73,12 @@ class Server:
     def __exit__(self, *a):
         self.close()
 
-    def submit(self, xyz, sdf_grid=None, band_voxels=0, occ_grid=None):
+    def submit(self, xyz, sdf_grid=None, band_voxels=0, occ_grid=None, mesh_key=0):
+        """mesh_key != 0 names the soup: its LBVH stays resident and later requests with the key reuse it."""
         xyz = np.ascontiguousarray(xyz, np.float32)
         r = Request(ctypes.c_void_p(xyz.ctypes.data) if xyz.size else None, xyz.size // 9,
                     sdf_grid if sdf_grid is not None else Grid(), int(band_voxels), 0,
-                    occ_grid if occ_grid is not None else Grid())
+                    occ_grid if occ_grid is not None else Grid(), int(mesh_key))
         t = ctypes.c_uint64()
         self._check(self.lib.ib_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
         self._keep[t.value] = (xyz, sdf_grid, band_voxels, occ_grid)
@@ -123,7 +125,7 @@ class Server:
 
 
 def open_loop(server, requests, rate_hz, duration_s, seed=0):
-    """Submit `requests` (list of (xyz, sdf_grid, band_voxels, occ_grid)) drawn uniformly at random
+    """Submit `requests` (list of (xyz, sdf_grid, band_voxels, occ_grid[, mesh_key])) drawn uniformly at random
     at Poisson arrival times of `rate_hz` for `duration_s`.  Like a client that fetches as soon as
     a result is ready, finished tickets are released while later ones are still arriving.
     Returns throughput and end-to-end latency percentiles (submit -> result resident on the device)."""
@@ -162,8 +164,9 @@ def open_loop(server, requests, rate_hz, duration_s, seed=0):
         delay = t0 + at - time.perf_counter()
         if delay > 0:
             time.sleep(delay)
-        xyz, g, band, og = requests[k]
-        outstanding.append((server.submit(xyz, g, band, og), k))
+        xyz, g, band, og = requests[k][:4]
+        key = requests[k][4] if len(requests[k]) > 4 else 0
+        outstanding.append((server.submit(xyz, g, band, og, key), k))
         if n % 16 == 15:
             reap(False)
     reap(True)

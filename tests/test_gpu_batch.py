@@ -83,3 +83,69 @@ def test_brick_shapes_agree(ctx, meshgen):
     finally:
         for opt in (bake.OPT_BRICK_SHAPE, bake.OPT_TILE_ORDER, bake.OPT_LEAF_BLOCK, bake.OPT_QUERY_VARIANT):
             ctx.set_option(opt, bake.DEFAULT_OPTIONS[opt])
+
+
+def test_resident_meshes_equal_soups(ctx, meshgen):
+    """Jobs that name resident meshes (ib_mesh_create) give the same bits as jobs that bring soups, for
+    closed and open meshes, slivers, a narrow band and occupancy, twice over the same meshes."""
+    from italy_bake import Job, bake
+
+    items = [it for it in _meshes(meshgen) if it[0].ntri]
+    band_mesh = meshgen.stress(seed=31, ncomp=3, remove_frac=0.02)
+    bufs, meshes = [], []
+    try:
+        soup_jobs, mesh_jobs, outs = [], [], []
+        for m, res in items:
+            g = bake.dense_grid(-0.8, 0.8, res)
+            og = bake.dense_grid(-0.8, 0.8, res + 5)
+            d_xyz = ctx.alloc(m.xyz.nbytes)
+            ctx.upload(d_xyz, m.xyz)
+            h = ctx.mesh_create(d_xyz, m.ntri)
+            meshes.append(h)
+            assert ctx.mesh_bytes(h) > 0
+            pair = []
+            for _ in range(2):
+                d_dist, d_occ = ctx.alloc(4 * res ** 3), ctx.alloc(4 * ((og.nx + 31) // 32) * og.ny * og.nz)
+                bufs += [d_dist, d_occ]
+                pair.append((d_dist, d_occ))
+            soup_jobs.append(Job(d_xyz, m.ntri, g, pair[0][0], og, pair[0][1]))
+            mesh_jobs.append(Job(None, 0, g, pair[1][0], og, pair[1][1], mesh=h))
+            outs.append((g, og, pair))
+            bufs.append(d_xyz)
+        # a narrow band over an open mesh
+        g = bake.dense_grid(-1.0, 1.0, 64)
+        nb = (g.nx // 8) ** 3
+        d_xyz = ctx.alloc(band_mesh.xyz.nbytes)
+        ctx.upload(d_xyz, band_mesh.xyz)
+        hb = ctx.mesh_create(d_xyz, band_mesh.ntri)
+        meshes.append(hb)
+        band = []
+        for _ in range(2):
+            d_idx, d_pool = ctx.alloc(4 * nb), ctx.alloc(4 * 512 * nb)
+            bufs += [d_idx, d_pool]
+            band.append((d_idx, d_pool))
+        bufs.append(d_xyz)
+        soup_jobs.append(Job(d_xyz, band_mesh.ntri, g, band[0][1], bake.Grid(), None, 4, 0, band[0][0], nb))
+        mesh_jobs.append(Job(None, 0, g, band[1][1], bake.Grid(), None, 4, 0, band[1][0], nb, mesh=hb))
+        ctx.bake_batch_device(soup_jobs)
+        for rep in range(2):
+            ctx.bake_batch_device(mesh_jobs)
+            assert mesh_jobs[0].ntri == items[0][0].ntri  # written back from the mesh
+            for g, og, pair in outs:
+                a = ctx.download(pair[0][0], np.empty(g.shape(), np.float32))
+                b = ctx.download(pair[1][0], np.empty(g.shape(), np.float32))
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), rep
+                nw = ((og.nx + 31) // 32) * og.ny * og.nz
+                assert np.array_equal(ctx.download(pair[0][1], np.empty(nw, np.uint32)),
+                                      ctx.download(pair[1][1], np.empty(nw, np.uint32)))
+            assert mesh_jobs[-1].nbricks == soup_jobs[-1].nbricks > 0
+            n = soup_jobs[-1].nbricks
+            assert np.array_equal(ctx.download(band[0][0], np.empty(nb, np.int32)),
+                                  ctx.download(band[1][0], np.empty(nb, np.int32)))
+            assert np.array_equal(ctx.download(band[0][1], np.empty(512 * n, np.uint32)),
+                                  ctx.download(band[1][1], np.empty(512 * n, np.uint32)))
+    finally:
+        for h in meshes:
+            ctx.mesh_destroy(h)
+        for p in bufs:
+            ctx.free(p)

@@ -72,3 +72,39 @@ def test_open_loop_latency(meshgen):
         r = open_loop(srv, reqs, rate_hz=100, duration_s=2.0, seed=1)
     assert r["failed"] == 0 and r["requests"] > 100
     assert r["latency_s"]["p99"] >= r["latency_s"]["p50"] > 0
+
+
+def test_resident_meshes_in_the_server(ctx, meshgen):
+    """Requests with mesh keys reuse the resident LBVH: every result equals the bake alone, the cache
+    counts one build per key, and a key whose soup has another triangle count is baked without it."""
+    from italy_bake import bake
+    from italy_bake.serve import Server
+
+    reqs = _requests(meshgen, bake)
+    with Server(workers=3, max_batch=4) as srv:
+        tickets = []
+        for rnd in range(3):
+            for i, r in enumerate(reqs):
+                tickets.append((srv.submit(*r, mesh_key=100 + i), r))
+            srv.drain()
+        other = meshgen.decimated(1200, 5)  # reuses key 100 with a different triangle count
+        og = bake.dense_grid(-0.8, 0.8, 30)
+        tickets.append((srv.submit(other.xyz, og, 0, og, mesh_key=100), (other.xyz, og, 0, og)))
+        srv.drain()
+        for t, (xyz, g, band, occ_g) in tickets:
+            d, idx, occ = srv.fetch(t)
+            if band:
+                ai, ap = ctx.bake_band(xyz, g, band)
+                assert np.array_equal(idx, ai)
+                assert np.array_equal(d.view(np.uint32), ap.view(np.uint32))
+            else:
+                ad, aocc = ctx.bake_dense(xyz, g, occ_g)
+                assert np.array_equal(d.view(np.uint32), ad.view(np.uint32))
+                if occ_g is not None:
+                    assert np.array_equal(occ, aocc)
+            srv.release(t)
+        c = srv.counters()
+        assert c["mesh_builds"] == len(reqs)
+        assert c["mesh_hits"] >= 2 * len(reqs)
+        assert c["meshes_resident"] == len(reqs) and c["mesh_bytes"] > 0
+        assert c["in_use"] == c["mesh_bytes"]  # every result released: only the resident meshes remain
