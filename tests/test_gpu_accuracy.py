@@ -1,26 +1,24 @@
 """Error of the GPU bake against the FP64 oracle: every voxel of the 5,120-triangle 64^3 bake (brute-force
-FP64 distance and winding number), and 1,048,576 seeded voxels of the 256^3 and 384^3 narrow-band bakes
+FP64 distance and winding number), and 1,048,576 seeded voxels of the 256^3 dense bake and the 384^3 narrow band
 (FP64 distance through the oracle's LBVH, equal to the brute force bit for bit; half of each sample within
 3 voxels of the surface), whose sign is checked on a 4,096-voxel subsample against the brute-force FP64
 winding number.  Max and RMS |d - d64| (bound 1e-5 * extent) and sign agreement on every checked voxel
-with |d64| > 1e-4 * extent whose |w64| is not within 1e-6 of 0.5.  The numbers are written to
-gpurun_out/accuracy.json when that folder exists."""
+with |d64| > 1e-4 * extent whose |w64| is not within 1e-6 of 0.5.  The numbers are written as JSON
+to the file the environment variable IB_ACCURACY_JSON names, when it is set."""
 import json
 import os
 
 import numpy as np
 import pytest
 
-from conftest import ROOT
-
 pytestmark = pytest.mark.gpu
 REPORT = {}
 
 
 def _write():
-    out = os.path.join(ROOT, "gpurun_out")
-    if os.path.isdir(out):
-        with open(os.path.join(out, "accuracy.json"), "w") as f:
+    path = os.environ.get("IB_ACCURACY_JSON")
+    if path:
+        with open(path, "w") as f:
             json.dump(REPORT, f, indent=1)
 
 
