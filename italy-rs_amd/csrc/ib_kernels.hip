@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, 
 }
 
 // Collapse: a child slot whose node covers at most `block` leaves (a contiguous range [lo, hi] of
-// sorted triangles, from the Karras ranges) becomes the leaf-block code ~(lo | (count - 1) << 28),
+// sorted triangles, from the Karras ranges) becomes the leaf-block code ~(lo | (count - 1) << kBlockShift),
 // so the traversals evaluate those triangles from the parent without visiting the nodes below
 // (their union box is already in the parent's slot).  Only the immutable ranges are read, so the
 // order in which threads rewrite slots does not matter.  Runs after the refit and the depth pass.
