@@ -352,7 +352,7 @@ def band_section(ctx, steps=3):
     }
 
 
-def batched_section(ctx, n):
+def batched_section(ctx, n, cpu=True):
     """n meshes (log-uniform 500..200k triangles, SDF + occupancy at 32/64/128/256^3): the plain
     per-mesh bake loop against one batched bake with an autotuned persistent query launch."""
     from italy_bake import Job, bake, meshgen
@@ -392,7 +392,34 @@ def batched_section(ctx, n):
         ctx.free(p)
     for opt in (bake.OPT_BRICK_SHAPE, bake.OPT_BLOCKS_PER_CU):  # the autotuner's choice applies to batches only
         ctx.set_option(opt, bake.DEFAULT_OPTIONS[opt])
+    cpu_out = None
+    if cpu:
+        # the oracle's CPU bake (same exact math, all cores) on every 32nd mesh over about 8 z-slabs of its
+        # grid; each mesh's full bake time is its measured voxel rate applied to its whole grid
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        cores = min(16, os.cpu_count() or 1)
+        t_cpu = time.perf_counter()
+        est_s, sampled = 0.0, 0
+        for i in range(0, n, 32):
+            m, r = meshes[i], res[i]
+            g = bake.dense_grid(-0.8, 0.8, r)
+            _, cst = ib_oracle.bake_dense(m.xyz, list(g.origin), g.h, r, r, r, 0, r, threads=cores,
+                                          zstep=max(1, r // 8))
+            if cst is None:  # open mesh: the oracle's dense bake signs closed meshes only
+                continue
+            est_s += cst["seconds"] * r ** 3 / cst["voxels"]
+            sampled += 1
+        if sampled:
+            cpu_s = est_s * n / sampled
+            cpu_out = {"value": n / cpu_s, "unit": "meshes/s", "cores": cores, "kind": "port",
+                       "gpu_multiple": cpu_s / batch_s,
+                       "sample": f"every 32nd mesh ({sampled} meshes), about 8 z-slabs of each grid, each mesh's "
+                                 f"rate applied to its whole grid and the mean to all {n} meshes (LBVH build "
+                                 f"included per sample); {time.perf_counter() - t_cpu:.1f} s"}
     return {
+        "cpu_baseline": cpu_out,
         "workload": f"{n} displaced icospheres, log-uniform 500..200k triangles ({tris} total), SDF + occupancy "
                     f"at 32/64/128/256^3 over [-0.8,0.8]^3 ({voxels} voxels)",
         "meshes": n, "triangles": tris, "voxels": voxels,
@@ -502,7 +529,7 @@ def main():
 
     consumer = consumer_section(ctx, sdf, d_dist) if not args.no_consumer else None
 
-    batched = batched_section(ctx, args.batch_meshes) if args.batch_meshes > 0 else None
+    batched = batched_section(ctx, args.batch_meshes, cpu=not args.no_cpu_baseline) if args.batch_meshes > 0 else None
     band = band_section(ctx) if not args.no_band else None
     analytic = analytic_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_analytic else None
     voxelize = voxelize_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_voxelize else None
