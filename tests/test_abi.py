@@ -77,3 +77,21 @@ def test_null_arguments_are_rejected():
     assert lib.ib_sdf_framing(None, None, 4, None) == 1
     assert lib.ib_bake_dense(None, None, 0, None, None, None, None) == 1
     assert b"null" in lib.ib_last_error()
+
+
+def test_new_entry_points_reject_null_arguments():
+    """The ingestion gate and resident-mesh entry points validate their arguments before any device work."""
+    import ctypes
+
+    from italy_bake import _lib
+
+    lib = _lib.load()
+    ok = ctypes.c_int32()
+    n = ctypes.c_uint64()
+    h = ctypes.c_void_p()
+    b3 = (ctypes.c_float * 3)(0, 0, 0)
+    assert lib.ib_watertight(None, None, 0, b3, b3, ctypes.byref(ok), None, None, None, 0) == 1
+    assert lib.ib_weld_positions(None, None, 0, b3, b3, None, None, ctypes.byref(n)) == 1
+    assert lib.ib_mesh_create(None, None, 0, None, ctypes.byref(h)) == 1
+    assert lib.ib_mesh_bytes(None) == 0 and lib.ib_mesh_triangles(None) == 0
+    lib.ib_mesh_destroy(None)  # no-op
