@@ -152,6 +152,29 @@ ib_status ib_bake_sdf(ib_context *ctx, const float *xyz, uint64_t ntri, const fl
 ib_status ib_voxelize(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
                       int32_t resolution, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count);
 
+/* Per-triangle material as the bake and the voxelizer compute it: triangleMaterial
+ * (src/convert/sdf_baker.cpp:61-74) and triangleColor (src/convert/voxelize.cpp:26-38), with
+ * sampleColorNearest and srgbToLinear (sdf_baker.cpp:42-53).  Per triangle t: the material
+ * tri_material[t] (0 past the table or out of range; MaterialAsset's defaults for an empty table,
+ * src/io/mesh_asset.h:18-52); its base colour times the nearest texel at the centroid of the triangle's
+ * three UVs (wrapped by x - floor(x)) when it names a texture with texels and the triangle has UVs, sRGB
+ * textures converted to linear.  uvs: 2 floats per corner (ncorner_uv of them, 3 per triangle).  out: 5 floats
+ * per triangle (r, g, b, metallic, roughness), the input of ib_splat_materials.  Host buffers. */
+typedef struct {
+  float base_color[3];          /* baseColorFactor */
+  float metallic, roughness;
+  int32_t base_color_texture;   /* index into the texture table, -1: none */
+} ib_material;
+typedef struct {
+  const uint8_t *rgba;          /* width * height texels of 4 bytes, rows of width texels */
+  int32_t width, height;
+  int32_t srgb;                 /* != 0: colour bytes are sRGB encoded */
+  int32_t reserved;
+} ib_texture;
+ib_status ib_triangle_materials(ib_context *ctx, uint64_t ntri, const float *uvs, uint64_t ncorner_uv,
+                                const uint32_t *tri_material, uint64_t ntri_material, const ib_material *materials,
+                                uint32_t nmaterials, const ib_texture *textures, uint32_t ntextures, float *out);
+
 /* Per-cell material splat of bakeSdf (src/convert/sdf_baker.cpp:76-128): per-triangle material
  * (r,g,b,metallic,roughness: 5 floats each) averaged over every grid cell the triangle overlaps,
  * accumulated in triangle order; cells without triangles get (1,1,1,0,1). */

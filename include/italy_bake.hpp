@@ -24,6 +24,8 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "italy_bake.h"
@@ -101,6 +103,53 @@ inline const float *soup(const Mesh &mesh) {
   return mesh.positions.empty() ? nullptr : reinterpret_cast<const float *>(mesh.positions.data());
 }
 
+namespace detail {
+// a MeshAsset-shaped type: materials, textures, per-corner uvs and per-triangle material indices
+template <class M, class = void>
+struct has_materials : std::false_type {};
+template <class M>
+struct has_materials<M, std::void_t<decltype(std::declval<const M &>().materials[0].baseColorFactor.x),
+                                    decltype(std::declval<const M &>().materials[0].baseColorTexture),
+                                    decltype(std::declval<const M &>().textures[0].pixelsRGBA.data()),
+                                    decltype(std::declval<const M &>().uvs.data()),
+                                    decltype(std::declval<const M &>().triangleMaterial.data())>> : std::true_type {};
+
+// (r, g, b, metallic, roughness) per triangle: triangleMaterial (src/convert/sdf_baker.cpp:61-74) on the device
+// for a MeshAsset, the default material for a type without materials
+template <class Mesh>
+std::vector<float> triangle_materials(const Mesh &mesh) {
+  const uint64_t ntri = mesh.positions.size() / 3;
+  std::vector<float> out(5 * ntri);
+  if constexpr (has_materials<Mesh>::value) {
+    std::vector<ib_material> mats(mesh.materials.size());
+    for (size_t i = 0; i < mats.size(); ++i) {
+      const auto &m = mesh.materials[i];
+      mats[i] = ib_material{{m.baseColorFactor.x, m.baseColorFactor.y, m.baseColorFactor.z}, m.metallic, m.roughness,
+                            m.baseColorTexture};
+    }
+    std::vector<ib_texture> texs(mesh.textures.size());
+    for (size_t i = 0; i < texs.size(); ++i) {
+      const auto &t = mesh.textures[i];
+      if (t.width > 0 && t.height > 0 && t.pixelsRGBA.size() < (size_t)t.width * (size_t)t.height * 4)
+        throw std::runtime_error("texture " + std::to_string(i) + " holds fewer texels than width * height");
+      texs[i] = ib_texture{t.pixelsRGBA.data(), t.width, t.height, t.srgb ? 1 : 0, 0};
+    }
+    static_assert(sizeof(mesh.uvs[0]) == 2 * sizeof(float), "uvs must be packed 2-float vectors");
+    check(ib_triangle_materials(context(), ntri, mesh.uvs.empty() ? nullptr : &mesh.uvs[0].x, mesh.uvs.size(),
+                                mesh.triangleMaterial.data(), mesh.triangleMaterial.size(), mats.data(),
+                                (uint32_t)mats.size(), texs.data(), (uint32_t)texs.size(), out.data()),
+          "ib_triangle_materials");
+  } else {
+    for (uint64_t t = 0; t < ntri; ++t) {
+      out[5 * t] = out[5 * t + 1] = out[5 * t + 2] = 1.0f;
+      out[5 * t + 3] = 0.0f;
+      out[5 * t + 4] = 1.0f;
+    }
+  }
+  return out;
+}
+}  // namespace detail
+
 // Per-triangle material: 5 floats (r, g, b, metallic, roughness) per triangle; MaterialFn(t, out5).
 template <class Mesh, class MaterialFn>
 SdfGrid bakeSdf(const Mesh &mesh, int resolution, MaterialFn &&material) {
@@ -130,14 +179,13 @@ SdfGrid bakeSdf(const Mesh &mesh, int resolution, MaterialFn &&material) {
   return grid;
 }
 
-// Without a material callback every triangle carries the default material (colour 1, metallic 0,
-// roughness 1), as a MeshAsset with one default MaterialAsset does.
+// Without a material callback: a MeshAsset's own materials and textures (triangleMaterial, computed on the
+// device), or the default material (colour 1, metallic 0, roughness 1) for a type without materials.
 template <class Mesh>
 SdfGrid bakeSdf(const Mesh &mesh, int resolution) {
-  return bakeSdf(mesh, resolution, [](uint64_t, float *m) {
-    m[0] = m[1] = m[2] = 1.0f;
-    m[3] = 0.0f;
-    m[4] = 1.0f;
+  const std::vector<float> mat = detail::triangle_materials(mesh);
+  return bakeSdf(mesh, resolution, [&](uint64_t t, float *m) {
+    for (int k = 0; k < 5; ++k) m[k] = mat[5 * t + k];
   });
 }
 
@@ -179,9 +227,14 @@ VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution, ColorFn &&color) {
   return grid;
 }
 
+// Without a colour callback: triangleColor (src/convert/voxelize.cpp:26-38) of a MeshAsset, computed on the
+// device, or white for a type without materials.
 template <class Mesh>
 VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution) {
-  return voxelizeMesh(mesh, resolution, [](uint64_t, float *c) { c[0] = c[1] = c[2] = 1.0f; });
+  const std::vector<float> mat = detail::triangle_materials(mesh);
+  return voxelizeMesh(mesh, resolution, [&](uint64_t t, float *c) {
+    for (int k = 0; k < 3; ++k) c[k] = mat[5 * t + k];
+  });
 }
 
 // ---- mesh ingestion gate (src/io/mesh_weld.h, src/io/mesh_validate.h) ------------------------------

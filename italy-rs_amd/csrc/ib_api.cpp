@@ -877,6 +877,70 @@ ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const floa
   return IB_OK;
 }
 
+static_assert(sizeof(ib_material) == sizeof(MatDev), "material layout");
+
+// srgbToLinear (src/convert/sdf_baker.cpp:42) of every byte value b / 255, evaluated on the host with the
+// reference's expression (float division, std::pow) so that the device's table lookup returns its values
+static const float *srgb_table() {
+  static float lut[256];
+  static std::once_flag once;
+  std::call_once(once, [] {
+    for (int b = 0; b < 256; ++b) {
+      const float c = (float)b / 255.0f;
+      lut[b] = c <= 0.04045f ? c / 12.92f : std::pow((c + 0.055f) / 1.055f, 2.4f);
+    }
+  });
+  return lut;
+}
+
+ib_status ib_triangle_materials(ib_context *c, uint64_t ntri, const float *uvs, uint64_t nuv,
+                                const uint32_t *tri_material, uint64_t ntm, const ib_material *materials,
+                                uint32_t nmat, const ib_texture *textures, uint32_t ntex, float *out) {
+  if (!c || (ntri && !out)) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  if ((nuv && !uvs) || (ntm && !tri_material) || (nmat && !materials) || (ntex && !textures))
+    return fail(IB_ERR_INVALID_ARGUMENT, "null table with a non-zero size");
+  if (ntri >= (1ull << 32) / 5) return fail(IB_ERR_INVALID_ARGUMENT, "too many triangles");
+  if (ntri == 0) return IB_OK;
+  std::vector<TexDev> tex(std::max<uint32_t>(ntex, 1));
+  uint64_t bytes = 0;
+  for (uint32_t i = 0; i < ntex; ++i) {
+    const ib_texture &t = textures[i];
+    tex[i] = TexDev{bytes, t.width, t.height, t.srgb ? 1 : 0, 0};
+    if (t.width > 0 && t.height > 0) {
+      if (!t.rgba) return fail(IB_ERR_INVALID_ARGUMENT, "texture " + std::to_string(i) + " has no texels");
+      bytes += (uint64_t)t.width * (uint64_t)t.height * 4;
+    }
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  float *d_uv = nullptr, *d_lut, *d_out;
+  uint32_t *d_tm = nullptr;
+  MatDev *d_mat = nullptr;
+  TexDev *d_tex;
+  uint8_t *d_px;
+  if (nuv) IB_TRY(c->scratch("tm_uv", 2 * nuv, &d_uv));
+  if (ntm) IB_TRY(c->scratch("tm_tri", ntm, &d_tm));
+  if (nmat) IB_TRY(c->scratch("tm_mat", nmat, &d_mat));
+  IB_TRY(c->scratch("tm_tex", tex.size(), &d_tex));
+  IB_TRY(c->scratch("tm_px", std::max<uint64_t>(bytes, 4), &d_px));
+  IB_TRY(c->scratch("tm_lut", 256, &d_lut));
+  IB_TRY(c->scratch("tm_out", 5 * ntri, &d_out));
+  if (nuv) IB_HIP(hipMemcpyAsync(d_uv, uvs, 2 * nuv * sizeof(float), hipMemcpyHostToDevice, s));
+  if (ntm) IB_HIP(hipMemcpyAsync(d_tm, tri_material, ntm * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  if (nmat) IB_HIP(hipMemcpyAsync(d_mat, materials, nmat * sizeof(MatDev), hipMemcpyHostToDevice, s));
+  IB_HIP(hipMemcpyAsync(d_tex, tex.data(), tex.size() * sizeof(TexDev), hipMemcpyHostToDevice, s));
+  for (uint32_t i = 0; i < ntex; ++i)
+    if (textures[i].width > 0 && textures[i].height > 0)
+      IB_HIP(hipMemcpyAsync(d_px + tex[i].offset, textures[i].rgba,
+                            (uint64_t)textures[i].width * textures[i].height * 4, hipMemcpyHostToDevice, s));
+  IB_HIP(hipMemcpyAsync(d_lut, srgb_table(), 256 * sizeof(float), hipMemcpyHostToDevice, s));
+  launch_tri_materials((uint32_t)ntri, d_uv, nuv, d_tm, ntm, d_mat, nmat, d_tex, ntex, d_px, d_lut, d_out, s);
+  IB_HIP(hipMemcpyAsync(out, d_out, 5 * ntri * sizeof(float), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  return IB_OK;
+}
+
 ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, const float *tri_material,
                              const ib_grid *grid, float *base_color, float *metallic, float *roughness) {
   if (!c || !grid || !base_color || !metallic || !roughness) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");

@@ -507,6 +507,46 @@ __global__ __launch_bounds__(256) void k_splat_reduce(const uint32_t *__restrict
   for (int j = 0; j < k && j < 8; ++j) out[j < 3 ? 3 * (size_t)c + j : (size_t)j * ncell + c] = sum[j] / fc;
 }
 
+// triangleMaterial (src/convert/sdf_baker.cpp:61-74) with sampleColorNearest (:44-53); the sRGB curve
+// (:42) comes in as a table over the 256 byte values, made on the host with the reference's expression
+__global__ __launch_bounds__(256) void k_tri_materials(uint32_t ntri, const float *__restrict__ uvs, uint64_t nuv,
+                                                       const uint32_t *__restrict__ tri_mat, uint64_t ntm,
+                                                       const MatDev *__restrict__ mats, uint32_t nmat,
+                                                       const TexDev *__restrict__ texs, uint32_t ntex,
+                                                       const uint8_t *__restrict__ pixels,
+                                                       const float *__restrict__ lut, float *out) {
+  const uint32_t t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= ntri) return;
+  MatDev m = {{1.0f, 1.0f, 1.0f}, 0.0f, 1.0f, -1};  // MaterialAsset's defaults for an empty table
+  if (nmat) {
+    const uint32_t mi = t < ntm ? tri_mat[t] : 0u;
+    m = mats[mi < nmat ? mi : 0u];
+  }
+  float col[3] = {m.base[0], m.base[1], m.base[2]};
+  const uint64_t base = 3ull * t;
+  if (m.tex >= 0 && (uint32_t)m.tex < ntex && base + 2 < nuv) {
+    const TexDev tx = texs[m.tex];
+    if (tx.w > 0 && tx.h > 0) {
+      const float u = ((uvs[2 * base] + uvs[2 * base + 2]) + uvs[2 * base + 4]) / 3.0f;
+      const float v = ((uvs[2 * base + 1] + uvs[2 * base + 3]) + uvs[2 * base + 5]) / 3.0f;
+      const float wu = u - floorf(u), wv = v - floorf(v);
+      const int px = min(max((int)(wu * (float)tx.w), 0), tx.w - 1);
+      const int py = min(max((int)(wv * (float)tx.h), 0), tx.h - 1);
+      const uint8_t *tp = pixels + tx.offset + ((uint64_t)py * (uint64_t)tx.w + (uint64_t)px) * 4;
+      for (int k = 0; k < 3; ++k) {
+        const float raw = tx.srgb ? lut[tp[k]] : (float)tp[k] / 255.0f;
+        col[k] = raw * m.base[k];
+      }
+    }
+  }
+  float *o = out + 5ull * t;
+  o[0] = col[0];
+  o[1] = col[1];
+  o[2] = col[2];
+  o[3] = m.metallic;
+  o[4] = m.roughness;
+}
+
 __global__ __launch_bounds__(256) void k_first_compact(const int32_t *__restrict__ first, uint64_t ncell,
                                                        Scalars *sc, unsigned long long *keys) {
   const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
@@ -1405,6 +1445,11 @@ hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, u
 void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
                          float *out, uint64_t ncell, hipStream_t s) {
   if (npairs) k_splat_reduce<<<nb(npairs), 256, 0, s>>>(cell, tri, npairs, attr, k, out, ncell);
+}
+void launch_tri_materials(uint32_t ntri, const float *uvs, uint64_t nuv, const uint32_t *tri_mat, uint64_t ntm,
+                          const MatDev *mats, uint32_t nmat, const TexDev *texs, uint32_t ntex, const uint8_t *pixels,
+                          const float *lut, float *out, hipStream_t s) {
+  if (ntri) k_tri_materials<<<nb(ntri), 256, 0, s>>>(ntri, uvs, nuv, tri_mat, ntm, mats, nmat, texs, ntex, pixels, lut, out);
 }
 void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s) {
   k_first_touch<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, first);

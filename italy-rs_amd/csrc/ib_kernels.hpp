@@ -138,6 +138,21 @@ hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, u
 void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,
                          float *out, uint64_t ncell, hipStream_t s);
 
+// ---- per-triangle materials ----
+struct MatDev {  // the layout of ib_material
+  float base[3];
+  float metallic, roughness;
+  int32_t tex;
+};
+struct TexDev {
+  uint64_t offset;  // first byte in the packed texel buffer
+  int32_t w, h, srgb, pad;
+};
+// out: 5 floats per triangle; lut: srgbToLinear of byte / 255 for the 256 byte values
+void launch_tri_materials(uint32_t ntri, const float *uvs, uint64_t nuv, const uint32_t *tri_mat, uint64_t ntm,
+                          const MatDev *mats, uint32_t nmat, const TexDev *texs, uint32_t ntex, const uint8_t *pixels,
+                          const float *lut, float *out, hipStream_t s);
+
 // ---- grid consumer (ib_consumer.hip) ----
 struct GridView {
   float ox, oy, oz, h;

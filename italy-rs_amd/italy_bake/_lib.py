@@ -24,7 +24,7 @@ EXPORTS = [
     "ib_server_fetch", "ib_server_device_result", "ib_server_release", "ib_server_stats",
     "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
     "ib_weld_positions", "ib_weld_positions_device", "ib_watertight", "ib_watertight_device",
-    "ib_mesh_create", "ib_mesh_destroy", "ib_mesh_bytes", "ib_mesh_triangles",
+    "ib_mesh_create", "ib_mesh_destroy", "ib_mesh_bytes", "ib_mesh_triangles", "ib_triangle_materials",
 ]
 
 
@@ -65,6 +65,18 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Material(ctypes.Structure):
+    """ib_material: baseColorFactor, metallic, roughness, baseColorTexture (-1: none)."""
+    _fields_ = [("base_color", ctypes.c_float * 3), ("metallic", ctypes.c_float), ("roughness", ctypes.c_float),
+                ("base_color_texture", ctypes.c_int32)]
+
+
+class Texture(ctypes.Structure):
+    """ib_texture: RGBA8 texels, width, height, sRGB flag."""
+    _fields_ = [("rgba", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("srgb", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
 class Job(ctypes.Structure):
@@ -128,6 +140,8 @@ def load():
     lib.ib_mesh_bytes.restype = _u64
     lib.ib_mesh_triangles.argtypes = [_vp]
     lib.ib_mesh_triangles.restype = _u64
+    lib.ib_triangle_materials.argtypes = [_vp, _u64, _vp, _u64, _vp, _u64, _vp, ctypes.c_uint32, _vp, ctypes.c_uint32,
+                                          _vp]
     lib.ib_weld_positions.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64)]
     lib.ib_weld_positions_device.argtypes = [_vp, _vp, _u64, _f3, _f3, _vp, _vp, ctypes.POINTER(_u64), _vp]
     lib.ib_watertight.argtypes = [_vp, _vp, _u64, _f3, _f3, ctypes.POINTER(_i32), ctypes.POINTER(_u64),

@@ -182,6 +182,28 @@ class Context:
                                             ctypes.c_void_p(stream) if stream else None))
         return bool(ok.value), b.value, m.value
 
+    def triangle_materials(self, ntri, uvs, tri_material, materials, textures):
+        """ib_triangle_materials (triangleMaterial, sdf_baker.cpp:42-74): (ntri, 5) r, g, b, metallic,
+        roughness.  uvs (ncorner, 2); tri_material (m,) uint32; materials: list of (base rgb, metallic,
+        roughness, texture index); textures: list of (rgba uint8 (h, w, 4), srgb)."""
+        from ._lib import Material, Texture
+
+        uvs = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        tm = np.ascontiguousarray(tri_material, np.uint32)
+        mats = (Material * max(len(materials), 1))()
+        for i, (base, met, rough, tex) in enumerate(materials):
+            mats[i] = Material((ctypes.c_float * 3)(*base), met, rough, tex)
+        keep = [np.ascontiguousarray(px, np.uint8) for px, _ in textures]
+        texs = (Texture * max(len(textures), 1))()
+        for i, (px, srgb) in enumerate(textures):
+            texs[i] = Texture(keep[i].ctypes.data if keep[i].size else None, keep[i].shape[1], keep[i].shape[0],
+                              int(srgb), 0)
+        out = np.zeros((max(ntri, 1), 5), np.float32)
+        check(self.lib.ib_triangle_materials(self._h, int(ntri), _ptr(uvs) if len(uvs) else None, len(uvs),
+                                             _ptr(tm) if len(tm) else None, len(tm), ctypes.addressof(mats),
+                                             len(materials), ctypes.addressof(texs), len(textures), _ptr(out)))
+        return out[:ntri]
+
     def splat_materials(self, xyz, materials, grid):
         """ib_splat_materials (bakePerCellMaterial, sdf_baker.cpp:76-128): per-triangle materials
         (ntri, 5) = r, g, b, metallic, roughness, averaged per cell in triangle order.  Returns

@@ -1783,3 +1783,42 @@ int orc_udf_points_f64(const float *xyz, size_t ntri, const double *pts, size_t 
   orc_bvh_free(&b);
   return 0;
 }
+
+/* ---- per-triangle material (src/convert/sdf_baker.cpp:42-74; the same as triangleColor,
+ * src/convert/voxelize.cpp:12-38, without metallic and roughness) ---------------------------------- */
+static float srgb_to_linear(float c) { return c <= 0.04045f ? c / 12.92f : powf((c + 0.055f) / 1.055f, 2.4f); }
+
+void orc_triangle_materials(size_t ntri, const float *uvs, size_t nuv, const uint32_t *tri_material,
+                            size_t ntri_material, const orc_material *materials, size_t nmat,
+                            const orc_texture *textures, size_t ntex, float *out) {
+  const orc_material def = {{1.0f, 1.0f, 1.0f}, 0.0f, 1.0f, -1};
+  for (size_t t = 0; t < ntri; ++t) {
+    /* materialForTriangle: mesh_asset.h:49-52 */
+    const uint32_t mi = t < ntri_material ? tri_material[t] : 0u;
+    const orc_material *mat = nmat ? &materials[mi < nmat ? mi : 0] : &def;
+    float col[3] = {mat->base[0], mat->base[1], mat->base[2]};
+    const size_t base = 3 * t;
+    if (mat->tex >= 0 && (size_t)mat->tex < ntex && base + 2 < nuv) {
+      const orc_texture *tx = &textures[mat->tex];
+      if (tx->width > 0 && tx->height > 0) {
+        /* (uv0 + uv1 + uv2) / 3.0f, component-wise */
+        const float u = ((uvs[2 * base] + uvs[2 * base + 2]) + uvs[2 * base + 4]) / 3.0f;
+        const float v = ((uvs[2 * base + 1] + uvs[2 * base + 3]) + uvs[2 * base + 5]) / 3.0f;
+        const float wu = u - floorf(u), wv = v - floorf(v);
+        int px = (int)(wu * (float)tx->width), py = (int)(wv * (float)tx->height);
+        px = px < 0 ? 0 : (px > tx->width - 1 ? tx->width - 1 : px); /* std::clamp(x, 0, w - 1) */
+        py = py < 0 ? 0 : (py > tx->height - 1 ? tx->height - 1 : py);
+        const size_t idx = ((size_t)py * (size_t)tx->width + (size_t)px) * 4;
+        for (int k = 0; k < 3; ++k) {
+          const float raw = (float)tx->rgba[idx + k] / 255.0f;
+          col[k] = (tx->srgb ? srgb_to_linear(raw) : raw) * mat->base[k];
+        }
+      }
+    }
+    out[5 * t] = col[0];
+    out[5 * t + 1] = col[1];
+    out[5 * t + 2] = col[2];
+    out[5 * t + 3] = mat->metallic;
+    out[5 * t + 4] = mat->roughness;
+  }
+}

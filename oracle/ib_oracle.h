@@ -186,6 +186,24 @@ int64_t orc_weld_positions(const float *pos, size_t npos, const float bmin[3], c
 int orc_watertight(const float *xyz, size_t ntri, const float bmin[3], const float bmax[3], int64_t *boundary,
                    int64_t *nonmanifold);
 
+/* ---- per-triangle material (src/convert/sdf_baker.cpp:42-74, voxelize.cpp:12-38) --------------------
+ * (r, g, b, metallic, roughness) per triangle: the material of triangleMaterial / materialForTriangle
+ * (mesh_asset.h:49-52; an empty table means the default material), its base colour times the nearest texel
+ * at the centroid UV (sampleColorNearest, wrapped by x - floor(x)) when it has a texture and the triangle has
+ * UVs, sRGB bytes through srgbToLinear.  uvs: 2 floats per corner (nuv corners). */
+typedef struct {
+  float base[3];
+  float metallic, roughness;
+  int32_t tex;
+} orc_material;
+typedef struct {
+  const uint8_t *rgba;
+  int32_t width, height, srgb;
+} orc_texture;
+void orc_triangle_materials(size_t ntri, const float *uvs, size_t nuv, const uint32_t *tri_material,
+                            size_t ntri_material, const orc_material *materials, size_t nmat,
+                            const orc_texture *textures, size_t ntex, float *out);
+
 #ifdef __cplusplus
 }
 #endif

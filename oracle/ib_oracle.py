@@ -108,6 +108,9 @@ def _load():
     lib.orc_boundary_edge_count.restype = ctypes.c_int64
     lib.orc_udf_points_f64.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p]
     lib.orc_udf_points_f64.restype = ctypes.c_int
+    lib.orc_triangle_materials.argtypes = [ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                           ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t, _f32p]
+    lib.orc_triangle_materials.restype = None
     lib.orc_weld_positions.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.c_void_p, ctypes.c_void_p]
     lib.orc_weld_positions.restype = ctypes.c_int64
     lib.orc_watertight.argtypes = [_f32p, ctypes.c_size_t, _f32p, _f32p, ctypes.POINTER(ctypes.c_int64),
@@ -370,6 +373,35 @@ def udf_points_f64(xyz, pts, threads=0):
     rc = lib().orc_udf_points_f64(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f64p), len(pts), threads, _p(out, _f64p))
     if rc != 0:
         raise ValueError(f"orc_udf_points_f64 failed ({rc})")
+    return out
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("base", ctypes.c_float * 3), ("metallic", ctypes.c_float), ("roughness", ctypes.c_float),
+                ("tex", ctypes.c_int32)]
+
+
+class Texture(ctypes.Structure):
+    _fields_ = [("rgba", ctypes.c_void_p), ("width", ctypes.c_int32), ("height", ctypes.c_int32),
+                ("srgb", ctypes.c_int32)]
+
+
+def triangle_materials(ntri, uvs, tri_material, materials, textures):
+    """triangleMaterial (sdf_baker.cpp:42-74) per triangle: (ntri, 5) = r, g, b, metallic, roughness.
+    uvs (ncorner, 2); tri_material (m,) uint32; materials: list of (base rgb, metallic, roughness, texture);
+    textures: list of (rgba uint8 (h, w, 4), srgb)."""
+    uvs = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+    tm = np.ascontiguousarray(tri_material, np.uint32)
+    mats = (Material * max(len(materials), 1))()
+    for i, (base, met, rough, tex) in enumerate(materials):
+        mats[i] = Material((ctypes.c_float * 3)(*base), met, rough, tex)
+    keep = [np.ascontiguousarray(px, np.uint8) for px, _ in textures]
+    texs = (Texture * max(len(textures), 1))()
+    for i, (px, srgb) in enumerate(textures):
+        texs[i] = Texture(keep[i].ctypes.data, keep[i].shape[1], keep[i].shape[0], int(srgb))
+    out = np.zeros((ntri, 5), np.float32)
+    lib().orc_triangle_materials(ntri, _p(uvs, _f32p), len(uvs), tm.ctypes.data, len(tm), ctypes.addressof(mats),
+                                 len(materials), ctypes.addressof(texs), len(textures), _p(out, _f32p))
     return out
 
 

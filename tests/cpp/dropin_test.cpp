@@ -2,6 +2,7 @@
 //   tests/sdf_sign_test.cpp:46-82  (UV sphere r=1, 32x16, bakeSdf res 24, sign + magnitude)
 //   tests/voxelize_test.cpp:37-58  (unit cube shell, voxelizeMesh res 8, membership)
 // the reference's watertight gate test (tests/mesh_validate_test.cpp:37-62) with its messages,
+// a textured MeshAsset's per-triangle materials (src/convert/sdf_baker.cpp:42-74) against the oracle,
 // and the procedural scenes of src/convert/sdf_procedural.cpp:81-193 through the analytic drop-ins,
 // checked bit for bit against the oracle's per-cell restatement of the same lambdas.
 // Checks print a line and set a non-zero exit status instead of relying on assert.
@@ -23,6 +24,26 @@ struct vec3 {  // three packed floats, like glm::vec3
 struct Mesh {  // the fields of italy::MeshAsset the bake reads
   std::vector<vec3> positions;
   vec3 boundsMin{0, 0, 0}, boundsMax{0, 0, 0};
+};
+
+struct vec2 {
+  float x, y;
+};
+struct TextureAsset {  // italy::TextureAsset
+  std::vector<uint8_t> pixelsRGBA;
+  int width = 0, height = 0;
+  bool srgb = false;
+};
+struct MaterialAsset {  // the fields of italy::MaterialAsset the bake reads
+  vec3 baseColorFactor{1, 1, 1};
+  float metallic = 0.0f, roughness = 1.0f;
+  int baseColorTexture = -1;
+};
+struct TexturedMesh : Mesh {  // italy::MeshAsset with materials
+  std::vector<vec2> uvs;
+  std::vector<uint32_t> triangleMaterial;
+  std::vector<MaterialAsset> materials;
+  std::vector<TextureAsset> textures;
 };
 
 int failures = 0;
@@ -191,6 +212,77 @@ void cupTest(bool glass, float cx, float cz) {
   }
 }
 
+// every triangle of the cube shell gets a material; the device table equals the oracle's bit for bit, and
+// voxelizeMesh / bakeSdf carry a uniform textured colour into every cell
+void texturedTest() {
+  TexturedMesh mesh;
+  static_cast<Mesh &>(mesh) = makeUnitCubeShell();
+  const size_t ntri = mesh.positions.size() / 3;
+  TextureAsset srgbTex, linTex;
+  srgbTex.width = 3, srgbTex.height = 2, srgbTex.srgb = true;
+  linTex.width = 2, linTex.height = 2;
+  for (int i = 0; i < 3 * 2 * 4; ++i) srgbTex.pixelsRGBA.push_back(static_cast<uint8_t>(37 * i + 11));
+  for (int i = 0; i < 2 * 2 * 4; ++i) linTex.pixelsRGBA.push_back(static_cast<uint8_t>(53 * i + 5));
+  mesh.textures = {srgbTex, linTex};
+  mesh.materials.resize(4);
+  mesh.materials[0].baseColorFactor = {0.25f, 0.5f, 0.75f};
+  mesh.materials[1].baseColorTexture = 0, mesh.materials[1].metallic = 0.5f;
+  mesh.materials[2].baseColorTexture = 1, mesh.materials[2].roughness = 0.25f;
+  mesh.materials[2].baseColorFactor = {0.9f, 0.8f, 0.7f};
+  mesh.materials[3].baseColorTexture = 7;  // no such texture: the factor
+  for (size_t t = 0; t + 2 < ntri; ++t) mesh.triangleMaterial.push_back(static_cast<uint32_t>(t % 5));
+  for (size_t c = 0; c < 3 * ntri; ++c)
+    mesh.uvs.push_back({0.37f * static_cast<float>(c) - 2.1f, 1.3f - 0.29f * static_cast<float>(c)});
+  const std::vector<float> got = italy_bake::detail::triangle_materials(mesh);
+
+  std::vector<orc_material> om;
+  for (const auto &m : mesh.materials)
+    om.push_back(orc_material{{m.baseColorFactor.x, m.baseColorFactor.y, m.baseColorFactor.z}, m.metallic,
+                              m.roughness, m.baseColorTexture});
+  std::vector<orc_texture> ot;
+  for (const auto &t : mesh.textures) ot.push_back(orc_texture{t.pixelsRGBA.data(), t.width, t.height, t.srgb});
+  std::vector<float> want(5 * ntri);
+  orc_triangle_materials(ntri, &mesh.uvs[0].x, mesh.uvs.size(), mesh.triangleMaterial.data(),
+                         mesh.triangleMaterial.size(), om.data(), om.size(), ot.data(), ot.size(), want.data());
+  expect(std::memcmp(got.data(), want.data(), want.size() * sizeof(float)) == 0,
+         "device triangle materials equal the oracle's");
+
+  // one textured material on every triangle, all UVs on one texel: every cell carries its colour
+  for (auto &m : mesh.triangleMaterial) m = 2;
+  mesh.triangleMaterial.resize(ntri, 2);
+  for (auto &uv : mesh.uvs) uv = {0.75f, 0.25f};  // texel (1, 0)
+  const float r = 0.9f * (static_cast<float>(linTex.pixelsRGBA[4]) / 255.0f);
+  const auto vg = italy_bake::voxelizeMesh(mesh, 8);
+  // cells average the colours of the triangles they touch: equal colours average to r within rounding
+  bool colOk = !vg.colors.empty();
+  for (const auto &c : vg.colors) colOk &= std::fabs(c[0] - r) <= 1e-6f;
+  expect(colOk, "voxelizeMesh carries the texel colour");
+  // bakeSdf's per-cell planes equal the oracle's splat of the oracle's per-triangle materials.  At 13 the
+  // cube's faces cut through cells; at 12 (and 24) they fall on cell faces, where triBoxOverlap's rounding
+  // leaves both neighbours untouched and every cell keeps the default material, on the device as in the oracle
+  const auto sg = italy_bake::bakeSdf(mesh, 13);
+  std::vector<float> tm(5 * ntri);
+  orc_triangle_materials(ntri, &mesh.uvs[0].x, mesh.uvs.size(), mesh.triangleMaterial.data(),
+                         mesh.triangleMaterial.size(), om.data(), om.size(), ot.data(), ot.size(), tm.data());
+  const size_t ncell = sg.distances.size();
+  std::vector<float> planes(5 * ncell);
+  orc_splat_materials(&mesh.positions[0].x, ntri, tm.data(), sg.origin.data(), sg.voxelSize, sg.nx, sg.ny, sg.nz,
+                      planes.data());
+  size_t splatted = 0, bad = 0;
+  for (size_t i = 0; i < ncell; ++i) {
+    const float want5[5] = {planes[i], planes[ncell + i], planes[2 * ncell + i], planes[3 * ncell + i],
+                            planes[4 * ncell + i]};
+    const float got5[5] = {sg.baseColor[i][0], sg.baseColor[i][1], sg.baseColor[i][2], sg.metallic[i],
+                           sg.roughness[i]};
+    splatted += want5[4] != 1.0f;
+    if (std::memcmp(want5, got5, sizeof(want5)) != 0 && bad++ == 0)
+      std::printf("cell %zu: got %.9g %.9g %.9g %.9g %.9g, oracle %.9g %.9g %.9g %.9g %.9g\n", i, got5[0], got5[1],
+                  got5[2], got5[3], got5[4], want5[0], want5[1], want5[2], want5[3], want5[4]);
+  }
+  expect(bad == 0 && splatted > 0, "bakeSdf carries the textured material (oracle splat)");
+  std::printf("textured_test: %zu triangles, %zu voxel cells\n", ntri, vg.colors.size());
+}
+
 void probeTest() {
   const std::vector<italy_bake::SdfGrid> grids = italy_bake::makeMaterialProbeSdfs(48);
   expect(grids.size() == 9, "nine probes");
@@ -209,6 +301,7 @@ int main() {
     sdfSignTest();
     voxelizeTest();
     meshValidateTest();
+    texturedTest();
     cupTest(false, -1.6f, 1.6f);
     cupTest(true, 0.0f, 0.0f);
     probeTest();
