@@ -285,6 +285,38 @@ def material_section(ctx, mesh, res=256, cpu=True, steps=3):
         out["cpu_baseline"] = {"value": mesh.ntri / cpu_s, "unit": "triangles/s", "cores": 1, "kind": "port",
                                "seconds": cpu_s, "gpu_multiple": cpu_s / gpu_s,
                                "sample": "the whole mesh on one core, in triangle order"}
+    out["triangle_materials"] = triangle_materials_timing(ctx, mesh.ntri, cpu, steps)
+    return out
+
+
+def triangle_materials_timing(ctx, ntri, cpu=True, steps=3):
+    """triangleMaterial (sdf_baker.cpp:42-74), the per-triangle input of the splat: 8 materials, 6 of them
+    textured by four 1024x1024 RGBA8 textures (two sRGB), random UVs over the mesh's triangles.  Host
+    buffers in and out (the textures are uploaded by every call)."""
+    rng = np.random.default_rng(9)
+    texs = [(rng.integers(0, 256, (1024, 1024, 4), dtype=np.uint8), i % 2 == 0) for i in range(4)]
+    mats = [(tuple(rng.uniform(0, 1, 3)), float(rng.uniform()), float(rng.uniform()), int(t))
+            for t in (-1, 0, 1, 2, 3, 0, 1, -1)]
+    uvs = rng.uniform(-2, 3, (3 * ntri, 2)).astype(np.float32)
+    tm = rng.integers(0, len(mats), ntri).astype(np.uint32)
+    got = ctx.triangle_materials(ntri, uvs, tm, mats, texs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        got = ctx.triangle_materials(ntri, uvs, tm, mats, texs)
+    gpu_s = (time.perf_counter() - t0) / steps
+    out = {"workload": f"{ntri} triangles, 8 materials, four 1024x1024 RGBA8 textures (host buffers, PCIe "
+                       "included)", "ms": gpu_s * 1e3, "triangles_per_s": ntri / gpu_s}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import ib_oracle
+
+        t0 = time.perf_counter()
+        ref = ib_oracle.triangle_materials(ntri, uvs, tm, mats, texs)
+        cpu_s = time.perf_counter() - t0
+        out["matches_oracle"] = bool(np.array_equal(got.view(np.uint32), ref.view(np.uint32)))
+        out["cpu_baseline"] = {"value": ntri / cpu_s, "unit": "triangles/s", "cores": 1, "kind": "port",
+                               "seconds": cpu_s, "gpu_multiple": cpu_s / gpu_s,
+                               "sample": "every triangle on one core"}
     return out
 
 
