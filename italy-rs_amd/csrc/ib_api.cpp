@@ -429,7 +429,10 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(take(c, arena, "idx", n, &idx));
   IB_TRY(take(c, arena, "order", n, &order));
   IB_TRY(take(c, arena, "leaf_parent", n, &leaf_parent));
-  IB_TRY(take(c, arena, "visits", n, &visits));
+  float *leafbox, *blockbox;
+  IB_TRY(take(c, arena, "refit_span", (size_t)n + 1, &visits));
+  IB_TRY(take(c, arena, "leafbox", 6ull * n, &leafbox));
+  IB_TRY(take(c, arena, "blockbox", 6ull * ((n + 255) / 256), &blockbox));
   IB_TRY(take(c, arena, "root_box", 8, &root_box));
   IB_TRY(take(c, out, "nodes", std::max(n - 1, 1), &m.nodes));
   IB_TRY(take(c, out, "tris", 3ull * n, &m.tris));
@@ -446,9 +449,9 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   IB_TRY(take(c, arena, "node_ranges", std::max(n - 1, 1), &ranges));
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
-  fill_u32(visits, 0u, n, s);
+  fill_u32(visits + n, 0u, 1, s);  // the count of nodes that span 256-leaf blocks
   const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
-  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, root_box, sc, ranges, block, m.nodes4, s);
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4, s);
   m.n = n;
   if (!arena) c->record(EV_BVH, s);
   return IB_OK;

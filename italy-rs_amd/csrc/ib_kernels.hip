@@ -129,9 +129,7 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
   const int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
   nodes[i].c0 = lc;
   nodes[i].c1 = rc;
-  // 1: the node's leaf range [lo, hi] lies within one 256-leaf block, so the refit threads that
-  // meet at this node are all in one workgroup
-  nodes[i].pad = (lo >> 8) == (hi >> 8) ? 1 : 0;
+  nodes[i].pad = 0;
   ranges[i] = make_int2(lo, hi);
   if (lc >= 0) nodes[lc].parent = i; else leaf_parent[~lc] = i;
   if (rc >= 0) nodes[rc].parent = i; else leaf_parent[~rc] = i;
@@ -160,58 +158,113 @@ __global__ __launch_bounds__(256) void k_gather(const float *__restrict__ xyz, c
   recs[i] = r;
 }
 
-// bottom-up refit: each leaf writes its box into its parent's child slot; the second thread to
-// reach a node unions the two child boxes and continues upwards.  Thread i handles sorted leaf i,
-// so a node whose leaf range lies within one 256-leaf block is only met by threads of one
-// workgroup: its exchange uses workgroup-scope atomics and fences; the others (about one node per
-// block boundary and level) use agent scope.
-template <int SCOPE>
-__device__ __forceinline__ bool refit_step(Node *nd, int child, float lo[3], float hi[3], uint32_t *visit) {
-  const bool left = (nd->c0 == child);
-  float *dlo = left ? nd->lo0 : nd->lo1;
-  float *dhi = left ? nd->hi0 : nd->hi1;
+// Refit from the Karras ranges instead of bottom-up atomics: every internal node covers a contiguous
+// range of sorted leaves, so each child slot's box is the union of the leaf boxes of the child's range.
+// Min and max are exact, so any order of the unions gives the same box (up to the sign of a zero bound,
+// which no distance or box test can see).  Pass 1 (one workgroup per 256-leaf block) builds a sparse
+// table of the block's leaf boxes in LDS and answers the nodes whose range lies inside the block with
+// two table reads per slot; it writes the leaf and block boxes and lists the other nodes.  Pass 2 gives
+// each listed node one wave, which unions the partial blocks at the ends of each child's range from the
+// leaf boxes and the whole blocks in between from the block boxes.  No workgroup waits for another.
+constexpr int kRefitLevels = 9;  // windows of 1 .. 256 leaves
+
+__device__ __forceinline__ void box_union(float lo[3], float hi[3], const float *b) {
   for (int k = 0; k < 3; ++k) {
-    __hip_atomic_store(&dlo[k], lo[k], __ATOMIC_RELAXED, SCOPE);
-    __hip_atomic_store(&dhi[k], hi[k], __ATOMIC_RELAXED, SCOPE);
+    lo[k] = smin(lo[k], b[k]);
+    hi[k] = smax(hi[k], b[3 + k]);
   }
-  if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-  else __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-  const uint32_t old = __hip_atomic_fetch_add(visit, 1u, __ATOMIC_RELAXED, SCOPE);
-  if (old == 0) return false;  // the sibling is still on its way; it finishes this node
-  if (SCOPE == __HIP_MEMORY_SCOPE_WORKGROUP) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  else __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-  for (int k = 0; k < 3; ++k) {
-    const float l0 = __hip_atomic_load(&nd->lo0[k], __ATOMIC_RELAXED, SCOPE);
-    const float l1 = __hip_atomic_load(&nd->lo1[k], __ATOMIC_RELAXED, SCOPE);
-    const float h0 = __hip_atomic_load(&nd->hi0[k], __ATOMIC_RELAXED, SCOPE);
-    const float h1 = __hip_atomic_load(&nd->hi1[k], __ATOMIC_RELAXED, SCOPE);
-    lo[k] = smin(l0, l1);
-    hi[k] = smax(h0, h1);
-  }
-  return true;
 }
 
-__global__ __launch_bounds__(256) void k_refit(const float4 *__restrict__ tris, int n, Node *nodes,
-                                               const int32_t *__restrict__ leaf_parent, uint32_t *visits,
-                                               float *root_box) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
-  float lo[3] = {smin(smin(a.x, b.x), c.x), smin(smin(a.y, b.y), c.y), smin(smin(a.z, b.z), c.z)};
-  float hi[3] = {smax(smax(a.x, b.x), c.x), smax(smax(a.y, b.y), c.y), smax(smax(a.z, b.z), c.z)};
-  int child = ~i;
-  int p = leaf_parent[i];
-  for (int guard = 0; p >= 0 && guard < kMaxDepth; ++guard) {
-    Node *nd = nodes + p;
-    const bool cont = nd->pad ? refit_step<__HIP_MEMORY_SCOPE_WORKGROUP>(nd, child, lo, hi, visits + p)
-                              : refit_step<__HIP_MEMORY_SCOPE_AGENT>(nd, child, lo, hi, visits + p);
-    if (!cont) return;
-    child = p;
-    p = nd->parent;
+__global__ __launch_bounds__(256) void k_refit_blocks(const float4 *__restrict__ tris, int n, Node *nodes,
+                                                      const int2 *__restrict__ ranges, float *leafbox,
+                                                      float *blockbox, float *root_box, uint32_t *span) {
+  __shared__ float st[kRefitLevels][6][256];
+  const int t = threadIdx.x, base = blockIdx.x * 256, i = base + t;
+  float b[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};  // neutral past the last leaf
+  if (i < n) {
+    const float4 a = tris[3 * i], v = tris[3 * i + 1], c = tris[3 * i + 2];
+    b[0] = smin(smin(a.x, v.x), c.x), b[1] = smin(smin(a.y, v.y), c.y), b[2] = smin(smin(a.z, v.z), c.z);
+    b[3] = smax(smax(a.x, v.x), c.x), b[4] = smax(smax(a.y, v.y), c.y), b[5] = smax(smax(a.z, v.z), c.z);
+    for (int k = 0; k < 6; ++k) leafbox[6 * (size_t)i + k] = b[k];
   }
-  for (int k = 0; k < 3; ++k) {
-    root_box[k] = lo[k];
-    root_box[3 + k] = hi[k];
+  for (int k = 0; k < 6; ++k) st[0][k][t] = b[k];
+  // st[l][.][t] = union of leaves t .. t + 2^l - 1 of the block, valid where t + 2^l <= 256
+  for (int l = 1; l < kRefitLevels; ++l) {
+    __syncthreads();
+    const int h = 1 << (l - 1);
+    if (t + 2 * h <= 256) {
+      for (int k = 0; k < 3; ++k) st[l][k][t] = smin(st[l - 1][k][t], st[l - 1][k][t + h]);
+      for (int k = 3; k < 6; ++k) st[l][k][t] = smax(st[l - 1][k][t], st[l - 1][k][t + h]);
+    }
+  }
+  __syncthreads();
+  if (t < 6) blockbox[6 * (size_t)blockIdx.x + t] = st[kRefitLevels - 1][t][0];
+  if (i >= n - 1) return;
+  const int2 r = ranges[i];
+  if ((r.x >> 8) != (r.y >> 8)) {  // spans blocks: pass 2 (the count sits past the n-entry list)
+    span[atomicAdd(&span[n], 1u)] = (uint32_t)i;
+    return;
+  }
+  Node *nd = nodes + i;
+  float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int slot = 0; slot < 2; ++slot) {
+    const int c = slot ? nd->c1 : nd->c0;
+    const int2 cr = c < 0 ? make_int2(~c, ~c) : ranges[c];
+    const int x = cr.x - base, y = cr.y - base;  // inside [0, 256): the child's range is inside the node's
+    const int l = 31 - __clz(y - x + 1);
+    float lo[3], hi[3];
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = smin(st[l][k][x], st[l][k][y - (1 << l) + 1]);
+      hi[k] = smax(st[l][3 + k][x], st[l][3 + k][y - (1 << l) + 1]);
+    }
+    for (int k = 0; k < 3; ++k) {
+      (slot ? nd->lo1 : nd->lo0)[k] = lo[k];
+      (slot ? nd->hi1 : nd->hi0)[k] = hi[k];
+      rlo[k] = smin(rlo[k], lo[k]);
+      rhi[k] = smax(rhi[k], hi[k]);
+    }
+  }
+  if (i == 0)
+    for (int k = 0; k < 3; ++k) root_box[k] = rlo[k], root_box[3 + k] = rhi[k];
+}
+
+__global__ __launch_bounds__(256) void k_refit_span(int n, Node *nodes, const int2 *__restrict__ ranges,
+                                                    const float *__restrict__ leafbox,
+                                                    const float *__restrict__ blockbox, float *root_box,
+                                                    const uint32_t *__restrict__ span) {
+  const int lane = threadIdx.x & 63;
+  const uint32_t count = span[n];
+  const uint32_t nwaves = gridDim.x * 4;
+  for (uint32_t w = blockIdx.x * 4 + (threadIdx.x >> 6); w < count; w += nwaves) {
+    const int i = (int)span[w];
+    Node *nd = nodes + i;
+    float rlo[3] = {INFINITY, INFINITY, INFINITY}, rhi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int slot = 0; slot < 2; ++slot) {
+      const int c = slot ? nd->c1 : nd->c0;
+      const int2 cr = c < 0 ? make_int2(~c, ~c) : ranges[c];
+      const int bx = cr.x >> 8, by = cr.y >> 8;
+      float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+      if (bx == by) {
+        for (int j = cr.x + lane; j <= cr.y; j += 64) box_union(lo, hi, leafbox + 6 * (size_t)j);
+      } else {
+        for (int j = cr.x + lane; j < (bx + 1) * 256; j += 64) box_union(lo, hi, leafbox + 6 * (size_t)j);
+        for (int j = bx + 1 + lane; j < by; j += 64) box_union(lo, hi, blockbox + 6 * (size_t)j);
+        for (int j = by * 256 + lane; j <= cr.y; j += 64) box_union(lo, hi, leafbox + 6 * (size_t)j);
+      }
+      for (int off = 32; off >= 1; off >>= 1)
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = smin(lo[k], __shfl_xor(lo[k], off));
+          hi[k] = smax(hi[k], __shfl_xor(hi[k], off));
+        }
+      if (lane == 0)
+        for (int k = 0; k < 3; ++k) {
+          (slot ? nd->lo1 : nd->lo0)[k] = lo[k];
+          (slot ? nd->hi1 : nd->hi0)[k] = hi[k];
+        }
+      for (int k = 0; k < 3; ++k) rlo[k] = smin(rlo[k], lo[k]), rhi[k] = smax(rhi[k], hi[k]);
+    }
+    if (i == 0 && lane == 0)
+      for (int k = 0; k < 3; ++k) root_box[k] = rlo[k], root_box[3 + k] = rhi[k];
   }
 }
 
@@ -1390,14 +1443,16 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
                    hipStream_t s) {
   k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, recs, sc);
 }
-void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
-                  float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s) {
+void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
+                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
     k_build4<<<1, 256, 0, s>>>(nodes, 1, nodes4);
     return;
   }
-  k_refit<<<nb(n), 256, 0, s>>>(tris, n, nodes, leaf_parent, visits, root_box);
+  const int nblk = (n + 255) / 256;
+  k_refit_blocks<<<nblk, 256, 0, s>>>(tris, n, nodes, ranges, leafbox, blockbox, root_box, span);
+  k_refit_span<<<std::min(2 * nblk, 1024), 256, 0, s>>>(n, nodes, ranges, leafbox, blockbox, root_box, span);
   k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
   if (block > 1) k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
   k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1, nodes4);

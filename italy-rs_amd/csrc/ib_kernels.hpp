@@ -75,9 +75,10 @@ hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint3
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, int2 *ranges, hipStream_t s);
 void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, TriRec *recs, Scalars *sc,
                    hipStream_t s);
-// refit, depth, collapse of subtrees of at most `block` leaves into leaf-block codes, 4-wide view
-void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *visits,
-                  float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s);
+// refit, depth, collapse of subtrees of at most `block` leaves into leaf-block codes, 4-wide view.
+// Scratch: span n + 1 words (entry n zeroed by the caller), leafbox 6n floats, blockbox 6 per 256 leaves.
+void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
+                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
                  hipStream_t s);
