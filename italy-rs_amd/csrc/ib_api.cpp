@@ -45,7 +45,7 @@ struct Buf {
 
 }  // namespace
 
-enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_COUNT };
+enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_OCC_SIDE, EV_COUNT };
 
 struct ib_context {
   int device = 0;
@@ -622,16 +622,29 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
   IB_TRY(c->scratch("sc", 1, &sc));
   c->stats_in_sc = true;
   c->record(EV_START, s);
-  IB_TRY(enqueue_build(c, d_xyz, (int)ntri, sc, m, nullptr, s));
-  if (occ && d_occ) enqueue_occupancy(d_xyz, (int)ntri, *occ, d_occ, s);
-  c->record(EV_OCC, s);
-  IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipStreamSynchronize(s));
-  IB_TRY(finish_build(c, d_xyz, (int)ntri, *c->host_sc, m, nullptr, true, s));
-  c->stats.bvh_depth = m.depth;
-  c->stats.boundary_edges = c->host_sc->boundary;
-  if (sdf) IB_TRY(query_dense(c, m, *sdf, d_dist, s));
-  return IB_OK;
+  // occupancy needs only the soup: it runs on a side stream beside the build, the columns and the
+  // query, and the caller's stream waits for it before anything after this call
+  const bool side_occ = occ && d_occ;
+  if (side_occ) {
+    IB_HIP(hipEventRecord(c->fork, s));
+    IB_HIP(hipStreamWaitEvent(c->side[0], c->fork, 0));
+    enqueue_occupancy(d_xyz, (int)ntri, *occ, d_occ, c->side[0]);
+    c->record(EV_OCC_SIDE, c->side[0]);
+    IB_HIP(hipEventRecord(c->join[0], c->side[0]));
+  }
+  const ib_status st = [&]() -> ib_status {
+    IB_TRY(enqueue_build(c, d_xyz, (int)ntri, sc, m, nullptr, s));
+    c->record(EV_OCC, s);
+    IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+    IB_HIP(hipStreamSynchronize(s));
+    IB_TRY(finish_build(c, d_xyz, (int)ntri, *c->host_sc, m, nullptr, true, s));
+    c->stats.bvh_depth = m.depth;
+    c->stats.boundary_edges = c->host_sc->boundary;
+    if (sdf) IB_TRY(query_dense(c, m, *sdf, d_dist, s));
+    return IB_OK;
+  }();
+  if (side_occ) IB_HIP(hipStreamWaitEvent(s, c->join[0], 0));
+  return st;
 }
 
 }  // namespace
@@ -1375,20 +1388,18 @@ ib_status ib_synchronize(ib_context *c) {
 ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   if (!c || !out) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
   std::lock_guard<std::mutex> lk(c->mu);
-  for (int e = EV_COUNT - 1; e >= 0; --e)
-    if (c->ev_valid[e]) {
-      IB_HIP(hipEventSynchronize(c->ev[e]));
-      break;
-    }
+  for (int e = 0; e < EV_COUNT; ++e)  // the events may sit on two streams
+    if (c->ev_valid[e]) IB_HIP(hipEventSynchronize(c->ev[e]));
   ib_stats st = c->stats;
   st.ms_ingest = ev_ms(c, EV_START, EV_INGEST);
   st.ms_bvh = ev_ms(c, EV_INGEST, EV_BVH);
-  st.ms_occupancy = ev_ms(c, EV_BVH, EV_OCC);
+  st.ms_occupancy = c->ev_valid[EV_OCC_SIDE] ? ev_ms(c, EV_START, EV_OCC_SIDE) : ev_ms(c, EV_BVH, EV_OCC);
   st.ms_columns = ev_ms(c, EV_OCC, EV_COLS);
   st.ms_sdf = ev_ms(c, EV_COLS, EV_SDF);
   st.ms_total = c->ev_valid[EV_SDF]   ? ev_ms(c, EV_START, EV_SDF)
                 : c->ev_valid[EV_OCC] ? ev_ms(c, EV_START, EV_OCC)
                                       : ev_ms(c, EV_START, EV_INGEST);
+  if (c->ev_valid[EV_OCC_SIDE]) st.ms_total = std::max(st.ms_total, (double)ev_ms(c, EV_START, EV_OCC_SIDE));
   // device-side counters of the last bake
   Scalars *sc = nullptr;
   auto it = c->bufs.find("sc");
