@@ -410,6 +410,9 @@ ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, in
   return IB_OK;
 }
 
+ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
+                       hipStream_t s, Arena *out);
+
 // ingest + LBVH of one mesh whose soup is on the device.  Outputs and temporaries come from `arena`
 // when given (a batch builds its meshes on several streams at once, so each needs its own), else
 // from the context scratch (reuse is ordered by the stream).  Nothing is read back here.
@@ -419,8 +422,28 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
   fill_u32(sc->cmin, 0xffffffffu, 3, s);
   int2 *bnd;
-  IB_TRY(enqueue_weld(c, d_xyz, n, sc, &bnd, s, arena));
-  if (!arena) c->record(EV_INGEST, s);
+  // a plain bake welds on a side stream beside the tree build (they share only the zeroed scalars);
+  // the build's stream waits for the weld before it returns, on every path
+  hipStream_t ws = s;
+  if (!arena) {
+    IB_HIP(hipEventRecord(c->fork, s));
+    IB_HIP(hipStreamWaitEvent(c->side[1], c->fork, 0));
+    ws = c->side[1];
+  }
+  ib_status st = enqueue_weld(c, d_xyz, n, sc, &bnd, ws, arena);
+  if (!arena) {
+    c->record(EV_INGEST, ws);
+    IB_HIP(hipEventRecord(c->join[1], ws));
+  }
+  if (st == IB_OK) st = enqueue_tree(c, d_xyz, n, sc, m, arena, s, out);
+  if (!arena) IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
+  if (st == IB_OK && !arena) c->record(EV_BVH, s);
+  return st;
+}
+
+// Morton codes, sort, Karras tree, records, refit, 4-wide view
+ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
+                       hipStream_t s, Arena *out) {
   uint32_t *codes_in, *codes, *visits;
   int32_t *idx, *order, *leaf_parent;
   float *root_box;
@@ -453,7 +476,6 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
   launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4, s);
   m.n = n;
-  if (!arena) c->record(EV_BVH, s);
   return IB_OK;
 }
 

@@ -42,19 +42,19 @@ __device__ __forceinline__ uint32_t hash64(uint64_t x) {
 // ============================================================================================
 __global__ __launch_bounds__(256) void k_prep(const float *__restrict__ xyz, int n, Scalars *sc) {
   __shared__ uint32_t red[6][256];
-  const int i = blockIdx.x * 256 + threadIdx.x;
   uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
-  if (i < n) {
+  bool finite = true;
+  // a capped grid strides over the triangles, so that few blocks meet at the six global atomics
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
     const float *t = xyz + 9 * (size_t)i;
-    bool finite = true;
     for (int k = 0; k < 9; ++k) finite &= isfinite(t[k]);
-    if (!finite) atomicOr(&sc->flags, kFlagNonFinite);
     for (int k = 0; k < 3; ++k) {
-      const float c = ((t[k] + t[3 + k]) + t[6 + k]) / 3.0f;
-      mn[k] = f2o(c);
-      mx[k] = f2o(c);
+      const uint32_t c = f2o(((t[k] + t[3 + k]) + t[6 + k]) / 3.0f);
+      mn[k] = min(mn[k], c);
+      mx[k] = max(mx[k], c);
     }
   }
+  if (!finite) atomicOr(&sc->flags, kFlagNonFinite);
   for (int k = 0; k < 3; ++k) {
     red[k][threadIdx.x] = mn[k];
     red[3 + k][threadIdx.x] = mx[k];
@@ -1426,7 +1426,9 @@ void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
   k_fill_u32<<<blocks, 256, 0, s>>>(p, v, n);
 }
 
-void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s) { k_prep<<<nb(n), 256, 0, s>>>(xyz, n, sc); }
+void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s) {
+  k_prep<<<std::min(nb(n), 512u), 256, 0, s>>>(xyz, n, sc);
+}
 void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, int32_t *idx, hipStream_t s) {
   k_morton<<<nb(n), 256, 0, s>>>(xyz, n, sc, codes, idx);
 }
