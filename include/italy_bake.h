@@ -53,9 +53,9 @@ typedef struct {
 /* Per-bake counters and device timings (milliseconds, HIP events on the bake stream). */
 typedef struct {
   double ms_total;       /* first kernel start .. last kernel end */
-  double ms_ingest;      /* weld + closedness */
-  double ms_bvh;         /* LBVH build */
-  double ms_occupancy;
+  double ms_ingest;      /* weld + closedness (a plain bake runs them on a side stream beside the LBVH build) */
+  double ms_bvh;         /* LBVH build, from the end of the weld */
+  double ms_occupancy;   /* a plain bake runs occupancy on a side stream beside the build, columns and query */
   double ms_columns;     /* +z column crossings */
   double ms_sdf;         /* closest-triangle + sign per voxel */
   uint64_t triangles;
