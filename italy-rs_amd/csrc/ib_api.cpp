@@ -471,10 +471,20 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   int2 *ranges;
   IB_TRY(take(c, arena, "node_ranges", std::max(n - 1, 1), &ranges));
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
+  // a plain bake walks the parent chains for the depth on a side stream beside the records and the refit
+  const bool side_depth = !arena;
+  if (side_depth) {
+    IB_HIP(hipEventRecord(c->fork, s));
+    IB_HIP(hipStreamWaitEvent(c->side[2], c->fork, 0));
+    launch_depth(m.nodes, leaf_parent, n, sc, c->side[2]);
+    IB_HIP(hipEventRecord(c->join[2], c->side[2]));
+  }
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits + n, 0u, 1, s);  // the count of nodes that span 256-leaf blocks
   const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
-  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4, s);
+  launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4,
+               !side_depth, s);
+  if (side_depth) IB_HIP(hipStreamWaitEvent(s, c->join[2], 0));
   m.n = n;
   return IB_OK;
 }

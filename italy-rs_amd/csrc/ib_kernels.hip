@@ -1446,7 +1446,8 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
   k_gather<<<nb(n), 256, 0, s>>>(xyz, order, n, tris, recs, sc);
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
-                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s) {
+                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4,
+                  bool depth, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
     k_build4<<<1, 256, 0, s>>>(nodes, 1, nodes4);
@@ -1455,9 +1456,12 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   const int nblk = (n + 255) / 256;
   k_refit_blocks<<<nblk, 256, 0, s>>>(tris, n, nodes, ranges, leafbox, blockbox, root_box, span);
   k_refit_span<<<std::min(2 * nblk, 1024), 256, 0, s>>>(n, nodes, ranges, leafbox, blockbox, root_box, span);
-  k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
+  if (depth) k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
   if (block > 1) k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
   k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1, nodes4);
+}
+void launch_depth(const Node *nodes, const int32_t *leaf_parent, int n, Scalars *sc, hipStream_t s) {
+  if (n > 1) k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
 }
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,

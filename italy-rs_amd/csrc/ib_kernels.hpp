@@ -78,7 +78,10 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
 // refit, depth, collapse of subtrees of at most `block` leaves into leaf-block codes, 4-wide view.
 // Scratch: span n + 1 words (entry n zeroed by the caller), leafbox 6n floats, blockbox 6 per 256 leaves.
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
-                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4, hipStream_t s);
+                  float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4,
+                  bool depth, hipStream_t s);
+// the tree depth into sc->depth (parent chains of the Karras tree; launch_refit runs it when `depth`)
+void launch_depth(const Node *nodes, const int32_t *leaf_parent, int n, Scalars *sc, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
                  hipStream_t s);
