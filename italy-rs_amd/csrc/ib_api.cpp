@@ -391,22 +391,40 @@ ib_status take(ib_context *c, Arena *arena, const char *name, size_t count, T **
 
 // weld by exact position and count unmatched directed edges into sc; boundary edges land in the
 // "e_bnd" scratch (valid until the next mesh is welded)
+struct WeldBufs {
+  uint32_t vcap = 0, ecap = 0;
+  int32_t *slots = nullptr, *minid = nullptr, *cslot = nullptr, *enet = nullptr;
+  unsigned long long *ekeys = nullptr;
+  int2 *bnd = nullptr;
+};
+
+// the weld's tables, and the fills that clear them appended to `l`
+ib_status take_weld(ib_context *c, int n, Arena *arena, WeldBufs &w, FillList &l) {
+  w.vcap = pow2_at_least(2ull * 3 * n);
+  w.ecap = pow2_at_least(2ull * 3 * n);
+  IB_TRY(take(c, arena, "w_slots", w.vcap, &w.slots));
+  IB_TRY(take(c, arena, "w_minid", w.vcap, &w.minid));
+  IB_TRY(take(c, arena, "w_cslot", 3ull * n, &w.cslot));
+  IB_TRY(take(c, arena, "e_keys", w.ecap, &w.ekeys));
+  IB_TRY(take(c, arena, "e_net", w.ecap, &w.enet));
+  IB_TRY(take(c, arena, "e_bnd", 3ull * n, &w.bnd));
+  l.add(reinterpret_cast<uint32_t *>(w.slots), 0xffffffffu, w.vcap);
+  l.add(reinterpret_cast<uint32_t *>(w.minid), 0x7fffffffu, w.vcap);
+  l.add(reinterpret_cast<uint32_t *>(w.ekeys), 0xffffffffu, 2ull * w.ecap);
+  l.add(reinterpret_cast<uint32_t *>(w.enet), 0u, w.ecap);
+  return IB_OK;
+}
+
+// weld by exact position and count unmatched directed edges into sc; boundary edges land in the
+// "e_bnd" scratch (valid until the next mesh is welded)
 ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, int2 **bnd, hipStream_t s,
                        Arena *arena = nullptr) {
-  const uint32_t vcap = pow2_at_least(2ull * 3 * n), ecap = pow2_at_least(2ull * 3 * n);
-  int32_t *slots, *minid, *cslot, *enet;
-  unsigned long long *ekeys;
-  IB_TRY(take(c, arena, "w_slots", vcap, &slots));
-  IB_TRY(take(c, arena, "w_minid", vcap, &minid));
-  IB_TRY(take(c, arena, "w_cslot", 3ull * n, &cslot));
-  IB_TRY(take(c, arena, "e_keys", ecap, &ekeys));
-  IB_TRY(take(c, arena, "e_net", ecap, &enet));
-  IB_TRY(take(c, arena, "e_bnd", 3ull * n, bnd));
-  fill_u32(reinterpret_cast<uint32_t *>(slots), 0xffffffffu, vcap, s);
-  fill_u32(reinterpret_cast<uint32_t *>(minid), 0x7fffffffu, vcap, s);
-  fill_u32(reinterpret_cast<uint32_t *>(ekeys), 0xffffffffu, 2ull * ecap, s);
-  fill_u32(reinterpret_cast<uint32_t *>(enet), 0u, ecap, s);
-  launch_weld(d_xyz, n, slots, minid, vcap - 1, cslot, ekeys, enet, ecap - 1, sc, *bnd, 3u * n, s);
+  WeldBufs w;
+  FillList l;
+  IB_TRY(take_weld(c, n, arena, w, l));
+  fill_multi(l, s);
+  launch_weld(d_xyz, n, w.slots, w.minid, w.vcap - 1, w.cslot, w.ekeys, w.enet, w.ecap - 1, sc, w.bnd, 3u * n, s);
+  *bnd = w.bnd;
   return IB_OK;
 }
 
@@ -419,10 +437,15 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
 ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
                         hipStream_t s, Arena *keep = nullptr) {
   Arena *out = keep ? keep : arena;  // the tree and records outlive the temporaries when `keep` is given
-  fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
-  fill_u32(sc->cmin, 0xffffffffu, 3, s);
-  int2 *bnd;
-  // a plain bake welds on a side stream beside the tree build (they share only the zeroed scalars);
+  // the scalars (cmin first, all ones; the rest zero) and the weld tables, cleared in one launch
+  static_assert(offsetof(Scalars, cmin) == 0, "cmin leads the scalars");
+  WeldBufs w;
+  FillList l;
+  l.add(sc->cmin, 0xffffffffu, 3);
+  l.add(reinterpret_cast<uint32_t *>(sc) + 3, 0u, sizeof(Scalars) / 4 - 3);
+  IB_TRY(take_weld(c, n, arena, w, l));
+  fill_multi(l, s);
+  // a plain bake welds on a side stream beside the tree build (they share only the cleared scalars);
   // the build's stream waits for the weld before it returns, on every path
   hipStream_t ws = s;
   if (!arena) {
@@ -430,7 +453,8 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
     IB_HIP(hipStreamWaitEvent(c->side[1], c->fork, 0));
     ws = c->side[1];
   }
-  ib_status st = enqueue_weld(c, d_xyz, n, sc, &bnd, ws, arena);
+  launch_weld(d_xyz, n, w.slots, w.minid, w.vcap - 1, w.cslot, w.ekeys, w.enet, w.ecap - 1, sc, w.bnd, 3u * n, ws);
+  ib_status st = IB_OK;
   if (!arena) {
     c->record(EV_INGEST, ws);
     IB_HIP(hipEventRecord(c->join[1], ws));

@@ -16,6 +16,15 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
     p[i] = v;
 }
 
+// several fills in one launch (the ranges must not overlap)
+__global__ void k_fill_multi(FillList l) {
+  for (int k = 0; k < l.count; ++k) {
+    const FillRange f = l.r[k];
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < f.n; i += (uint64_t)gridDim.x * blockDim.x)
+      f.p[i] = f.v;
+  }
+}
+
 __device__ __forceinline__ uint32_t canon(float f) { return f == 0.0f ? 0u : __float_as_uint(f); }
 
 __device__ __forceinline__ uint32_t hash3(uint32_t a, uint32_t b, uint32_t c) {
@@ -1433,6 +1442,11 @@ void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, 
   k_morton<<<nb(n), 256, 0, s>>>(xyz, n, sc, codes, idx);
 }
 
+void fill_multi(const FillList &l, hipStream_t s) {
+  uint64_t mx = 0;
+  for (int k = 0; k < l.count; ++k) mx = std::max(mx, l.r[k].n);
+  if (mx) k_fill_multi<<<std::min(nb(mx), 1024u), 256, 0, s>>>(l);
+}
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                       int32_t *vout, int n, hipStream_t s) {
   return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, (unsigned)n, 0u, 30u, s);

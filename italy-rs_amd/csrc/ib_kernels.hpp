@@ -68,6 +68,19 @@ struct SdfArgs {
 };
 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
+struct FillRange {
+  uint32_t *p;
+  uint32_t v;
+  uint64_t n;
+};
+constexpr int kMaxFill = 8;
+struct FillList {
+  FillRange r[kMaxFill];
+  int count = 0;
+  void add(uint32_t *p, uint32_t v, uint64_t n) { r[count++] = FillRange{p, v, n}; }
+};
+// the fills of `l` in one launch (ranges must not overlap)
+void fill_multi(const FillList &l, hipStream_t s);
 void launch_prep(const float *xyz, int n, Scalars *sc, hipStream_t s);
 void launch_morton(const float *xyz, int n, const Scalars *sc, uint32_t *codes, int32_t *idx, hipStream_t s);
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
