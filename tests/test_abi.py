@@ -95,3 +95,5 @@ def test_new_entry_points_reject_null_arguments():
     assert lib.ib_mesh_create(None, None, 0, None, ctypes.byref(h)) == 1
     assert lib.ib_mesh_bytes(None) == 0 and lib.ib_mesh_triangles(None) == 0
     lib.ib_mesh_destroy(None)  # no-op
+    out = (ctypes.c_float * 5)()
+    assert lib.ib_triangle_materials(None, 1, None, 0, None, 0, None, 0, None, 0, out) == 1
