@@ -61,15 +61,17 @@ typedef struct {
   uint64_t triangles;
   uint64_t voxels;       /* SDF voxel queries */
   uint64_t occ_cells;    /* occupancy cells covered by the grid */
-  int32_t bvh_depth;
-  int32_t deep_stack;    /* 1 if the global-memory traversal stack was used */
+  int32_t bvh_depth;     /* internal nodes on the longest root-to-leaf path (at most 30 + ceil(log2 ntri)) */
+  int32_t pad0;
   int64_t boundary_edges;
   uint64_t column_overflows;
   uint64_t rechecked;    /* voxels whose winding number was re-evaluated in FP64 */
   uint64_t ties;         /* voxels decided by the ray-parity tie-break */
   uint64_t box_tests;    /* point/box distance tests of the closest-triangle traversal (counting mode) */
   uint64_t tri_tests;    /* point/triangle distance tests (counting mode) */
-  uint64_t reserved[4];
+  uint64_t stack_overflows; /* traversal pushes that found the stack full: 0 unless the tree is malformed */
+  double ms_caps;        /* host time of the hole caps of open meshes (boundary read-back, chains, cap tree) */
+  uint64_t reserved[2];
 } ib_stats;
 
 /* Options for ib_set_option. */
@@ -273,8 +275,14 @@ typedef struct {
 } ib_ticket_info;
 
 typedef struct {
-  uint64_t budget, in_use, cached, pending, ready, running_batches, batches, completed;
+  uint64_t budget;
+  uint64_t in_use;       /* reserved by admitted requests, kept by finished results and resident meshes */
+  uint64_t cached;       /* staging blocks kept for reuse; counted against the budget at admission */
+  uint64_t pending, ready, running_batches, batches, completed;
   uint64_t mesh_hits, mesh_builds, meshes_resident, mesh_bytes;  /* resident-mesh cache (ib_request.mesh_key) */
+  uint64_t evictions;      /* resident meshes evicted to admit requests */
+  uint64_t failed;         /* requests that ended in an error */
+  uint64_t alloc_failures; /* staging allocations that failed even after the cache was freed */
 } ib_server_counters;
 
 ib_status ib_server_create(int device, uint64_t hbm_budget_bytes, int32_t workers, int32_t max_batch,

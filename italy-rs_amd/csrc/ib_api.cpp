@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -20,9 +21,11 @@ using namespace ib;
 namespace {
 
 thread_local std::string g_err;
+thread_local uint64_t g_failures = 0;  // fail() calls on this thread: a call failed if it moved
 
 ib_status fail(ib_status s, const std::string &msg) {
   g_err = msg;
+  ++g_failures;
   return s;
 }
 
@@ -53,6 +56,10 @@ struct ib_context {
   static constexpr int kSide = 4;   // streams a batch builds its meshes on, joined before the query
   hipStream_t side[kSide] = {};
   hipEvent_t fork = nullptr, join[kSide] = {};
+  // end of the last call's device work (on whatever stream it ran): every call's stream waits for it
+  // first, so calls on different streams never overlap on the context's shared scratch
+  hipEvent_t tail = nullptr;
+  bool tail_valid = false;
   uint64_t budget = 0;
   uint64_t allocated = 0;
   std::unordered_map<std::string, Buf> bufs;
@@ -62,7 +69,8 @@ struct ib_context {
   bool ev_valid[EV_COUNT] = {};
   ib_stats stats{};
   bool count_work = false;
-  bool stats_in_sc = false;    // last bake was a plain bake: its device counters are in the "sc" scratch
+  bool stats_in_sc = false;    // last bake was a plain bake: its device counters are at last_sc
+  Scalars *last_sc = nullptr;
   int query_variant = 1;
   int brick_shape = 0;         // wave brick shape (autotuned for batches)
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
@@ -143,6 +151,30 @@ struct ib_context {
 };
 
 namespace {
+
+// Orders one entry point's device work after every earlier call on the context and records where it
+// ends.  Built under the context lock once the call's stream is known.  On destruction (every return
+// path) the side streams are joined into the call's stream, so work that a failed call left queued on
+// them is ordered too, and the tail event is recorded there.  A call that failed (fail() ran on this
+// thread meanwhile) also waits for its queued work before it returns, because its caller may free or
+// reuse what that work reads and writes, and a batch's arena blocks go back to the context's free list.
+struct Ordered {
+  ib_context *c;
+  hipStream_t s;
+  uint64_t failures;
+  Ordered(ib_context *ctx, hipStream_t st) : c(ctx), s(st), failures(g_failures) {
+    if (c->tail_valid) (void)hipStreamWaitEvent(s, c->tail, 0);
+  }
+  ~Ordered() {
+    for (int i = 0; i < ib_context::kSide; ++i) {
+      (void)hipEventRecord(c->join[i], c->side[i]);
+      (void)hipStreamWaitEvent(s, c->join[i], 0);
+    }
+    (void)hipEventRecord(c->tail, s);
+    c->tail_valid = true;
+    if (g_failures != failures) (void)hipStreamSynchronize(s);
+  }
+};
 
 uint32_t pow2_at_least(uint64_t v) {
   uint64_t c = 1024;
@@ -329,20 +361,25 @@ float ev_ms(ib_context *c, EventId a, EventId b) {
 }
 
 // Device allocations owned by one batch instead of the context's scratch: a bump allocator over
-// chunks of kChunk bytes (larger requests get a block of their own, rounded to 2 MiB), so a batch of
-// hundreds of meshes makes a handful of device allocations, and the chunk sizes recur from batch to
-// batch so the context's size-keyed cache serves them.  Sub-allocations are 256-byte aligned.
+// chunks of 64 MiB, or 1/16 of a smaller budget (at least 4 MiB), so a batch of hundreds of meshes makes
+// a handful of device allocations, the chunk sizes recur from batch to batch so the context's size-keyed
+// cache serves them, and a small budget is not spent on one chunk.  Larger requests get a block of their
+// own, rounded to 2 MiB.  Sub-allocations are 256-byte aligned.
 struct Arena {
-  static constexpr size_t kChunk = 64ull << 20;
   ib_context *c;
   std::vector<std::pair<void *, size_t>> blocks;
   char *cur = nullptr;
   size_t left = 0;
+  size_t chunk() const {
+    const size_t k = std::min<size_t>(64ull << 20, std::max<size_t>(4ull << 20, c->budget / 16));
+    return k & ~size_t((2u << 20) - 1);
+  }
   template <typename T>
   ib_status get(size_t count, T **out) {
     const size_t bytes = (std::max<size_t>(count * sizeof(T), 256) + 255) & ~size_t(255);
     if (bytes > left) {
-      if (bytes > kChunk / 4) {  // big: a block of its own, the current chunk stays open
+      const size_t kc = chunk();
+      if (bytes > kc / 4) {  // big: a block of its own, the current chunk stays open
         const size_t bb = (bytes + (2u << 20) - 1) & ~size_t((2u << 20) - 1);
         void *p = nullptr;
         IB_TRY(c->alloc_raw(bb, &p));
@@ -351,10 +388,10 @@ struct Arena {
         return IB_OK;
       }
       void *p = nullptr;
-      IB_TRY(c->alloc_raw(kChunk, &p));
-      blocks.push_back({p, kChunk});
+      IB_TRY(c->alloc_raw(kc, &p));
+      blocks.push_back({p, kc});
       cur = static_cast<char *>(p);
-      left = kChunk;
+      left = kc;
     }
     *out = reinterpret_cast<T *>(cur);
     cur += bytes;
@@ -527,11 +564,13 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   m.depth = (int)h.depth;
   m.slivers = h.slivers > 0;
   if (m.depth > kMaxDepth) return fail(IB_ERR_INTERNAL, "BVH parent chain does not reach the root");
-  if (m.depth + 2 > kStack)
-    return fail(IB_ERR_UNSUPPORTED, "BVH depth " + std::to_string(m.depth) + " exceeds the traversal stack");
+  if (m.depth > kMaxTreeDepth)
+    return fail(IB_ERR_INTERNAL, "BVH depth " + std::to_string(m.depth) + " exceeds the bound of the key length (" +
+                                     std::to_string(kMaxTreeDepth) + ")");
   m.ncap = 0;
   m.caps = nullptr;
   if (h.boundary == 0) return IB_OK;
+  const auto t_caps = std::chrono::steady_clock::now();
   int2 *bnd;
   if (!bnd_valid) {
     Scalars *sc2;
@@ -573,6 +612,7 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   IB_HIP(hipMemcpyAsync(m.caps, ordered.data(), ordered.size() * sizeof(float), hipMemcpyHostToDevice, s));
   IB_HIP(hipMemcpyAsync(m.cap_nodes, nodes.data(), nodes.size() * sizeof(CapNode), hipMemcpyHostToDevice, s));
   IB_HIP(hipStreamSynchronize(s));
+  c->stats.ms_caps += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_caps).count();
   return IB_OK;
 }
 
@@ -611,16 +651,8 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
   SdfArgs sa;
   IB_TRY(prepare_query(c, m, g, d_out, sc, nullptr, (uint32_t)nvox, sa, s));
   c->record(EV_COLS, s);
-  const bool deep = m.depth + 2 > kStack;
-  c->stats.deep_stack = deep ? 1 : 0;
-  if (deep) {
-    const uint64_t threads = (uint64_t)((g.nx + 7) / 8) * ((g.ny + 7) / 8) * ((g.nz + 3) / 4) * 256;
-    sa.gstack_depth = m.depth + 2;
-    IB_TRY(c->scratch("gstack", threads * sa.gstack_depth, &sa.gstack));
-    IB_TRY(c->scratch("gstackd", threads * sa.gstack_depth, &sa.gstackd));
-  }
   const uint32_t *order = nullptr;
-  if (c->query_variant == 1 && !deep && c->tile_order) {
+  if (c->query_variant == 1 && c->tile_order) {
     // costliest block-tiles first: sort the tiles by the column-crossing cost estimate
     uint32_t d[3];
     tile_dims(sa.col.g, c->brick_shape, d);
@@ -639,7 +671,7 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_HIP(sort_pairs_u32(temp, temp_bytes, keys_in, keys, ids_in, ids, nt, 32, s));
     order = reinterpret_cast<const uint32_t *>(ids);
   }
-  launch_sdf(sa, deep, c->count_work, c->query_variant, m.slivers, c->brick_shape, order, s);
+  launch_sdf(sa, c->count_work, c->query_variant, m.slivers, c->brick_shape, order, s);
   launch_resolve(sa, s);
   c->record(EV_SDF, s);
   return IB_OK;
@@ -677,6 +709,7 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
   Scalars *sc;
   IB_TRY(c->scratch("sc", 1, &sc));
   c->stats_in_sc = true;
+  c->last_sc = sc;
   c->record(EV_START, s);
   // occupancy needs only the soup: it runs on a side stream beside the build, the columns and the
   // query, and the caller's stream waits for it before anything after this call
@@ -732,6 +765,7 @@ ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void 
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
   std::unique_ptr<ib_mesh> mh(new ib_mesh(c));
   Arena temp{c};  // weld tables, Morton codes, sort space: released when the build is done
+  Ordered ord(c, s);  // after the arenas: a failed build waits for its kernels before the blocks are released
   Scalars *sc;
   IB_TRY(temp.get(1, &sc));
   IB_TRY(mh->keep.get(9 * ntri, &mh->d_xyz));
@@ -793,7 +827,8 @@ ib_status ib_create(int device, uint64_t budget, ib_context **out) {
       return fail(IB_ERR_DEVICE, "stream creation failed");
     }
   }
-  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) {
+  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->tail, hipEventDisableTiming) != hipSuccess) {
     ib_destroy(c);
     return fail(IB_ERR_DEVICE, "event creation failed");
   }
@@ -819,6 +854,7 @@ void ib_destroy(ib_context *c) {
     if (c->join[i]) (void)hipEventDestroy(c->join[i]);
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->tail) (void)hipEventDestroy(c->tail);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -858,6 +894,7 @@ ib_status ib_bake_dense_device(ib_context *c, const float *d_xyz, uint64_t ntri,
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
   return bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s);
 }
 
@@ -869,6 +906,7 @@ ib_status ib_bake_dense(ib_context *c, const float *xyz, uint64_t ntri, const ib
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   float *d_xyz = nullptr, *d_dist = nullptr;
   uint32_t *d_occ = nullptr;
   const uint64_t nvox = sdf ? (uint64_t)sdf->nx * sdf->ny * sdf->nz : 0;
@@ -880,7 +918,14 @@ ib_status ib_bake_dense(ib_context *c, const float *xyz, uint64_t ntri, const ib
   IB_TRY(bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s));
   if (sdf) IB_HIP(hipMemcpyAsync(dist, d_dist, nvox * sizeof(float), hipMemcpyDeviceToHost, s));
   if (occ) IB_HIP(hipMemcpyAsync(occ_words, d_occ, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  if (ntri && sdf) {
+    Scalars *sc;
+    IB_TRY(c->scratch("sc", 1, &sc));
+    IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
+  }
   IB_HIP(hipStreamSynchronize(s));
+  if (ntri && sdf && c->host_sc->stack_overflow)
+    return fail(IB_ERR_INTERNAL, "a traversal stack overflowed (the tree exceeds its depth bound)");
   return IB_OK;
 }
 
@@ -908,6 +953,7 @@ ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const floa
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   ib_grid g{};
   IB_TRY(ib_vox_framing(bmin, bmax, res, g.origin, &g.h));
   g.nx = g.ny = g.nz = res;
@@ -986,6 +1032,7 @@ ib_status ib_triangle_materials(ib_context *c, uint64_t ntri, const float *uvs, 
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   float *d_uv = nullptr, *d_lut, *d_out;
   uint32_t *d_tm = nullptr;
   MatDev *d_mat = nullptr;
@@ -1024,6 +1071,7 @@ ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, con
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   constexpr int K = 5;
   float *d_out;
   IB_TRY(c->scratch("sp_out", (K + 0) * ncell, &d_out));
@@ -1140,6 +1188,7 @@ ib_status ib_weld_positions_device(ib_context *c, const float *d_pos, uint64_t n
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
   reset_stats(c);
   c->record(EV_START, s);
   IB_TRY(weld_device(c, d_pos, npos, bmin, bmax, d_welded, d_unique, nunique, s));
@@ -1157,6 +1206,7 @@ ib_status ib_weld_positions(ib_context *c, const float *pos, uint64_t npos, cons
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   reset_stats(c);
   float *d_pos, *d_unique = nullptr;
   uint32_t *d_welded;
@@ -1227,6 +1277,7 @@ static ib_status watertight_entry(ib_context *c, const float *xyz, bool on_devic
     std::lock_guard<std::mutex> lk(c->mu);
     IB_HIP(hipSetDevice(c->device));
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+    Ordered ord(c, s);
     reset_stats(c);
     c->stats.triangles = ntri;
     const float *d_xyz = xyz;
@@ -1277,18 +1328,24 @@ ib_status ib_device_free(ib_context *c, void *p) {
   return IB_OK;
 }
 
-ib_status ib_memcpy_h2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
+// copies on the context's stream after every earlier call's work; they return when the copy is done
+static ib_status ordered_copy(ib_context *c, void *dst, const void *src, uint64_t bytes, hipMemcpyKind kind) {
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
-  IB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  hipStream_t s = c->stream;
+  Ordered ord(c, s);
+  IB_HIP(hipMemcpyAsync(dst, src, bytes, kind, s));
+  IB_HIP(hipStreamSynchronize(s));
   return IB_OK;
 }
 
+ib_status ib_memcpy_h2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
+  return ordered_copy(c, dst, src, bytes, hipMemcpyHostToDevice);
+}
+
 ib_status ib_memcpy_d2h(ib_context *c, void *dst, const void *src, uint64_t bytes) {
-  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
-  IB_HIP(hipSetDevice(c->device));
-  IB_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
-  return IB_OK;
+  return ordered_copy(c, dst, src, bytes, hipMemcpyDeviceToHost);
 }
 
 static_assert(sizeof(ib_csg_op) == sizeof(CsgOp), "CSG instruction layout");
@@ -1360,6 +1417,7 @@ ib_status ib_bake_analytic_device(ib_context *c, const ib_csg_op *prog, uint32_t
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
   reset_stats(c);
   c->stats.voxels = (uint64_t)grid->nx * grid->ny * grid->nz;
   c->record(EV_START, s);
@@ -1376,6 +1434,7 @@ ib_status ib_bake_analytic(ib_context *c, const ib_csg_op *prog, uint32_t nops, 
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = c->stream;
+  Ordered ord(c, s);
   const uint64_t n = (uint64_t)grid->nx * grid->ny * grid->nz;
   float *d_dist;
   uint8_t *d_branch = nullptr;
@@ -1406,6 +1465,7 @@ ib_status ib_sample_grid_device(ib_context *c, const ib_grid *grid, const float 
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
   launch_sample_grid(v, d_points, n, d_values, d_normals, d_nearest, s);
   IB_HIP(hipGetLastError());
   return IB_OK;
@@ -1420,24 +1480,22 @@ ib_status ib_trace_grid_device(ib_context *c, const ib_grid *grid, const float *
   std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
   launch_trace_grid(v, d_origins, d_dirs, n, tmin, tmax, d_t, s);
   IB_HIP(hipGetLastError());
   return IB_OK;
 }
 
 ib_status ib_memcpy_d2d(ib_context *c, void *dst, const void *src, uint64_t bytes) {
-  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
-  IB_HIP(hipSetDevice(c->device));
-  IB_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
-  IB_HIP(hipStreamSynchronize(c->stream));
-  return IB_OK;
+  return ordered_copy(c, dst, src, bytes, hipMemcpyDeviceToDevice);
 }
 
 ib_status ib_synchronize(ib_context *c) {
   if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  std::lock_guard<std::mutex> lk(c->mu);
   IB_HIP(hipSetDevice(c->device));
+  if (c->tail_valid) IB_HIP(hipEventSynchronize(c->tail));
   IB_HIP(hipStreamSynchronize(c->stream));
-  IB_HIP(hipDeviceSynchronize());
   return IB_OK;
 }
 
@@ -1457,10 +1515,8 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
                                       : ev_ms(c, EV_START, EV_INGEST);
   if (c->ev_valid[EV_OCC_SIDE]) st.ms_total = std::max(st.ms_total, (double)ev_ms(c, EV_START, EV_OCC_SIDE));
   // device-side counters of the last bake
-  Scalars *sc = nullptr;
-  auto it = c->bufs.find("sc");
-  if (c->stats_in_sc && it != c->bufs.end()) {  // batched bakes accumulate their counters on the host
-    sc = static_cast<Scalars *>(it->second.p);
+  if (c->stats_in_sc && c->last_sc) {  // batched bakes accumulate their counters on the host
+    const Scalars *sc = c->last_sc;
     Scalars h;
     IB_HIP(hipMemcpy(&h, sc, sizeof(Scalars), hipMemcpyDeviceToHost));
     st.column_overflows = h.col_overflow;
@@ -1468,6 +1524,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
     st.ties = h.ties;
     st.box_tests = h.box_tests;
     st.tri_tests = h.tri_tests;
+    st.stack_overflows = h.stack_overflow;
   }
   *out = st;
   return IB_OK;
@@ -1515,6 +1572,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->batch_sc_cap = njobs;
   }
   Arena arena{c};
+  Ordered ord(c, s);  // after the arena: a failed batch waits for its kernels before the blocks are released
   Scalars *scs;
   IB_TRY(arena.get(std::max(njobs, 1u), &scs));
   std::vector<MeshDev> mesh(njobs);
@@ -1707,6 +1765,9 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.column_overflows += h.col_overflow;
     c->stats.box_tests += h.box_tests;
     c->stats.tri_tests += h.tri_tests;
+    c->stats.stack_overflows += h.stack_overflow;
+    if (h.stack_overflow)
+      return fail(IB_ERR_INTERNAL, "job " + std::to_string(j) + ": a traversal stack overflowed");
     if (h.ambiguous > args[k].amb_cap) {
       SdfArgs sa = args[k];
       const ib_job &jb = jobs[j];

@@ -653,10 +653,14 @@ __device__ __forceinline__ void for_leaf_code(int code, F &&leaf) {
   for (int l = t; l <= last; ++l) leaf(l);
 }
 
+// a traversal push that found the stack full: impossible within kMaxTreeDepth (the host refuses deeper
+// trees), counted so that the host reports IB_ERR_INTERNAL instead of a silently pruned search
+__device__ __noinline__ void stack_overflow(Scalars *sc) { atomicAdd(&sc->stack_overflow, 1u); }
+
 // visits every leaf whose box contains (X,Y) in xy
 template <typename F>
 __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes, int nnodes, float X, float Y,
-                                                  F &&leaf) {
+                                                  Scalars *sc, F &&leaf) {
   int stack[kStack + 2];
   int sp = 0;
   int cur = 0;
@@ -672,6 +676,7 @@ __device__ __forceinline__ void for_column_leaves(const Node *__restrict__ nodes
       if (nd.c1 < 0) for_leaf_code(~nd.c1, leaf);
       else if (next < 0) next = nd.c1;
       else if (sp < kStack) stack[sp++] = nd.c1;
+      else stack_overflow(sc);
     }
     if (next >= 0) {
       cur = next;
@@ -707,7 +712,7 @@ __device__ __forceinline__ void column_crossings(const ColumnArgs &a, int x, int
   const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
   const size_t col = (size_t)y * a.g.nx + x;
   ColumnSink sink{a.cross + col * kColCap, 0};
-  for_column_leaves(a.nodes, a.nnodes, X, Y, [&](int leaf) {
+  for_column_leaves(a.nodes, a.nnodes, X, Y, a.sc, [&](int leaf) {
     const float4 *t = a.tris + 3 * leaf;
     float z;
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
@@ -751,7 +756,7 @@ __global__ __launch_bounds__(256) void k_columns_batch(const SdfArgs *__restrict
 // overflowed the stored crossings: recount crossings above q.z
 __device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y, float qz) {
   int w = 0;
-  for_column_leaves(a.nodes, a.nnodes, X, Y, [&](int leaf) {
+  for_column_leaves(a.nodes, a.nnodes, X, Y, a.sc, [&](int leaf) {
     const float4 *t = a.tris + 3 * leaf;
     float z;
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
@@ -881,7 +886,7 @@ __device__ __forceinline__ bool ray_box(F3 o, F3 d, const float *lo, const float
 }
 
 __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int nnodes, const float4 *__restrict__ tris,
-                                           F3 q) {
+                                           F3 q, Scalars *sc) {
   int votes = 0;
   for (int v = 0; v < 5; ++v) {
     const F3 d = {kParityDirs[3 * v], kParityDirs[3 * v + 1], kParityDirs[3 * v + 2]};
@@ -914,6 +919,8 @@ __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int n
           next = nd.c1;
         } else if (sp < kStack) {
           stack[sp++] = nd.c1;
+        } else {
+          stack_overflow(sc);
         }
       }
       if (next >= 0) {
@@ -936,15 +943,13 @@ struct WorkCount {
   uint32_t box = 0, tri = 0;
 };
 
-template <bool DEEP, bool COUNT, bool SLIVERS>
+
+template <bool COUNT, bool SLIVERS>
 __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int nnodes, const TriRec *__restrict__ recs,
-                                            F3 q, float slack, int *gstack, float *gstackd, int cap,
-                                            WorkCount &wc) {
+                                            F3 q, float slack, Scalars *sc, WorkCount &wc) {
   float best = INFINITY, bound = INFINITY;
-  int lstack[DEEP ? 1 : kStack];
-  float lstackd[DEEP ? 1 : kStack];
-  int *st = DEEP ? gstack : lstack;
-  float *sd = DEEP ? gstackd : lstackd;
+  int st[kStack];
+  float sd[kStack];
   int sp = 0;
   int cur = 0;
   for (int visits = 0; visits <= nnodes; ++visits) {
@@ -970,10 +975,12 @@ __device__ __forceinline__ float closest_d2(const Node *__restrict__ nodes, int 
     const bool g1 = nd.c1 >= 0 && d1 <= bound;
     if (g0 && g1) {
       const bool first0 = d0 <= d1;
-      if (sp < cap) {
+      if (sp < kStack) {
         st[sp] = first0 ? nd.c1 : nd.c0;
         sd[sp] = first0 ? d1 : d0;
         ++sp;
+      } else {
+        stack_overflow(sc);
       }
       cur = first0 ? nd.c0 : nd.c1;
       continue;
@@ -1010,7 +1017,7 @@ __device__ __forceinline__ float query_slack(const ColumnArgs &c) {
   return dist_slack(c.g.ox, c.g.oy, c.g.oz, c.g.h, c.g.nx, c.g.ny, c.g.nz, lo, hi, 2);
 }
 
-template <bool DEEP, bool COUNT, bool SLIVERS, bool CAPS>
+template <bool COUNT, bool SLIVERS, bool CAPS>
 __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int x = blockIdx.x * 8 + (wv & 1) * 4 + (lane & 3);
@@ -1019,16 +1026,9 @@ __global__ __launch_bounds__(256) void k_sdf(SdfArgs a) {
   const GridArgs &g = a.col.g;
   if (x >= g.nx || y >= g.ny || z >= g.nz) return;
   const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
-  int *gs = nullptr;
-  float *gsd = nullptr;
-  if (DEEP) {
-    const size_t tid = (((size_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x) * 256 + threadIdx.x;
-    gs = a.gstack + tid * a.gstack_depth;
-    gsd = a.gstackd + tid * a.gstack_depth;
-  }
   WorkCount wc;
-  const float best = closest_d2<DEEP, COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.recs, q, query_slack(a.col), gs, gsd,
-                                                     DEEP ? a.gstack_depth : kStack, wc);
+  const float best =
+      closest_d2<COUNT, SLIVERS>(a.col.nodes, a.col.nnodes, a.col.recs, q, query_slack(a.col), a.col.sc, wc);
   if (COUNT) {
     atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box);
     atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri);
@@ -1121,7 +1121,7 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, u
         inside = fabs(w64) > 0.5;
       } else {
         if (lane == 0) atomicAdd(&a.col.sc->ties, 1u);
-        inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q);
+        inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q, a.col.sc);
       }
     }
     if (lane == 0) {
@@ -1268,9 +1268,13 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
     cx(key[1], ch[1], key[2], ch[2]);
 #pragma unroll
     for (int e = 3; e >= 1; --e)
-      if (ch[e] >= 0 && sp < kStack4) {
-        if (lane == 0) sstack[sp] = ch[e];
-        ++sp;
+      if (ch[e] >= 0) {
+        if (sp < kStack4) {
+          if (lane == 0) sstack[sp] = ch[e];
+          ++sp;
+        } else {
+          stack_overflow(a.col.sc);
+        }
       }
     bool evaluated = false;
 #pragma unroll
@@ -1568,11 +1572,10 @@ void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, 
 
 
 template <bool COUNT, bool SLIVERS, bool CAPS>
-static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, const uint32_t *order,
-                         hipStream_t s) {
+static void launch_sdf_t(const SdfArgs &a, int variant, int shape, const uint32_t *order, hipStream_t s) {
   uint32_t d[3];
   tile_dims(a.col.g, shape, d);
-  if (variant == 1 && !deep) {
+  if (variant == 1) {
     if (kShapeHost[shape][2] == 2)
       k_sdf_wave<COUNT, SLIVERS, CAPS, 2><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
     else
@@ -1580,8 +1583,7 @@ static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, co
     return;
   }
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
-  if (deep) k_sdf<true, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
-  else k_sdf<false, COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
+  k_sdf<COUNT, SLIVERS, CAPS><<<grid, 256, 0, s>>>(a);
 }
 
 #define IB_DISPATCH(fn, count, slivers, caps, ...)                              \
@@ -1596,9 +1598,9 @@ static void launch_sdf_t(const SdfArgs &a, bool deep, int variant, int shape, co
     default: fn<true, true, true>(__VA_ARGS__); break;                          \
   }
 
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, const uint32_t *order,
+void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int shape, const uint32_t *order,
                 hipStream_t s) {
-  IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, deep, variant, shape, order, s);
+  IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, variant, shape, order, s);
 }
 
 void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, s>>>(a); }

@@ -15,6 +15,14 @@ constexpr double kTieEps = 1e-6;
 constexpr uint32_t kFlagNonFinite = 1u;
 // parent-chain walks stop here; a deeper chain means a malformed tree and is reported as depth
 constexpr int kMaxDepth = 4096;
+// Depth bound of the Karras tree: keys are (30-bit Morton code << 32 | sorted position), an internal
+// node's split is at the first bit where its range's keys differ, and that prefix length grows strictly
+// from a node to its internal children.  Prefix lengths lie in [2, 31] inside the code and in
+// [64 - b, 63] inside a position of b = ceil(log2 n) bits, so a root-to-leaf path holds at most 30 + b
+// internal nodes: 58 for the 2^28 triangles the entry points accept.  The binary traversals need one
+// stack entry per level (kStack = 64); the 4-wide brick walk descends two levels per visit and pushes at
+// most 3 entries per visit, at most 3 * 29 = 87 < kStack4.  A deeper tree is a construction bug.
+constexpr int kMaxTreeDepth = 58;
 
 // device-side scalars of one bake (reset before every bake)
 struct Scalars {
@@ -28,7 +36,8 @@ struct Scalars {
   uint32_t occ_set;       // occupied cells (first-touch voxelizer)
   uint32_t slivers;       // triangles taking the FP64 distance path
   uint32_t ambiguous;     // voxels handed to the resolve kernel (column overflow or |w| near 0.5)
-  uint32_t pad[3];
+  uint32_t stack_overflow;  // traversal pushes that found the stack full (0 by the depth bound; checked by the host)
+  uint32_t pad[2];
   unsigned long long box_tests;  // counting mode only
   unsigned long long tri_tests;
 };
@@ -62,9 +71,6 @@ struct SdfArgs {
   int unsigned_only;            // 1: store |d| only (brick selection pass; no columns needed)
   uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
   uint32_t amb_cap;    // capacity of amb_list
-  int *gstack;
-  float *gstackd;
-  int gstack_depth;
 };
 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
@@ -120,7 +126,7 @@ struct BatchArgs {
 void tile_dims(const GridArgs &g, int shape, uint32_t d[3]);
 // variant 0: per-lane traversal, 1: wave-coherent brick traversal; slivers: the mesh has triangles
 // on the FP64 distance path; shape: brick shape of the wave (0: 4x4x4, 1: 8x4x2, 2: 8x8x1, 3: 16x4x1)
-void launch_sdf(const SdfArgs &a, bool deep, bool count, int variant, bool slivers, int shape, const uint32_t *order,
+void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int shape, const uint32_t *order,
                 hipStream_t s);
 // launch-order keys (ascending = costliest first) and tile ids of the per-brick query's block-tiles
 void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, int32_t *ids, hipStream_t s);

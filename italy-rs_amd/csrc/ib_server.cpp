@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -37,7 +38,7 @@ double now_s() {
 struct DevCache {
   std::mutex mu;
   std::map<size_t, std::vector<void *>> free_blocks;
-  size_t cached = 0;
+  std::atomic<size_t> cached{0};  // read by the server's admission without this cache's lock
   static size_t klass(size_t n) {
     if (n <= (64ull << 20)) {
       size_t c = 4096;
@@ -47,6 +48,7 @@ struct DevCache {
     const size_t step = 16ull << 20;
     return (n + step - 1) / step * step;
   }
+  uint64_t failures = 0;  // hipMalloc calls that failed even after the cache was returned
   void *get(size_t n) {
     const size_t k = klass(std::max<size_t>(n, 1));
     {
@@ -60,8 +62,25 @@ struct DevCache {
       }
     }
     void *p = nullptr;
-    if (hipMalloc(&p, k) != hipSuccess) return nullptr;
-    return p;
+    if (hipMalloc(&p, k) == hipSuccess) return p;
+    clear();  // out of device memory: give the cached blocks back and try once more
+    if (hipMalloc(&p, k) == hipSuccess) return p;
+    std::lock_guard<std::mutex> lk(mu);
+    ++failures;
+    return nullptr;
+  }
+  // frees cached blocks, largest classes first, until at most `keep` bytes stay cached
+  void trim(size_t keep) {
+    std::lock_guard<std::mutex> lk(mu);
+    while (cached > keep && !free_blocks.empty()) {
+      auto it = std::prev(free_blocks.end());
+      while (!it->second.empty() && cached > keep) {
+        (void)hipFree(it->second.back());
+        it->second.pop_back();
+        cached -= it->first;
+      }
+      if (it->second.empty()) free_blocks.erase(it);
+    }
   }
   void put(void *p, size_t n) {
     if (!p) return;
@@ -105,6 +124,7 @@ struct CachedMesh {
   uint64_t bytes = 0;
   uint64_t last_use = 0;
   int users = 0;
+  int owner = 0;  // worker whose context holds the mesh's memory
 };
 
 uint64_t voxels(const ib_grid &g) { return (uint64_t)g.nx * g.ny * g.nz; }
@@ -128,11 +148,24 @@ struct ib_server {
   DevCache cache;
   uint64_t batches = 0, completed = 0;
   std::unordered_map<uint64_t, CachedMesh> meshes;  // mesh_key -> resident mesh
-  uint64_t mesh_clock = 0, mesh_hits = 0, mesh_builds = 0, mesh_bytes = 0;
+  uint64_t mesh_clock = 0, mesh_hits = 0, mesh_builds = 0, mesh_bytes = 0, evictions = 0, failed = 0;
+  // evicted meshes waiting for their owning worker to destroy them between its batches (destroying one
+  // takes the owner's context lock, which the owner holds for a whole batch)
+  std::vector<std::vector<ib_mesh *>> doomed;
+  uint64_t doomed_bytes = 0;
 
-  // least recently used resident meshes that no bake is using leave until `need` more bytes fit
+  // device bytes in use or held: admitted requests and results, resident meshes, meshes awaiting their
+  // destruction and the staging cache
+  uint64_t held_locked() const { return in_use + doomed_bytes + cache.cached; }
+
+  // makes room for `need` more bytes: trims the staging cache, then evicts the least recently used
+  // resident meshes that no bake is using (their owners destroy them)
   void evict_locked(uint64_t need) {
-    while (in_use + need > budget) {
+    if (held_locked() + need > budget && cache.cached) {
+      const uint64_t over = held_locked() + need - budget;
+      cache.trim(cache.cached > over ? cache.cached - over : 0);
+    }
+    while (in_use + doomed_bytes + need > budget) {
       auto victim = meshes.end();
       for (auto it = meshes.begin(); it != meshes.end(); ++it)
         if (it->second.users == 0 && (victim == meshes.end() || it->second.last_use < victim->second.last_use))
@@ -140,9 +173,30 @@ struct ib_server {
       if (victim == meshes.end()) return;
       in_use -= victim->second.bytes;
       mesh_bytes -= victim->second.bytes;
-      ib_mesh_destroy(victim->second.mesh);
+      doomed_bytes += victim->second.bytes;
+      doomed[victim->second.owner].push_back(victim->second.mesh);
+      ++evictions;
       meshes.erase(victim);
+      cv_work.notify_all();
     }
+  }
+
+  // destroys the evicted meshes worker w owns; called by w between batches, without the server lock
+  void reap(int w) {
+    std::vector<ib_mesh *> mine;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      mine.swap(doomed[w]);
+    }
+    if (mine.empty()) return;
+    uint64_t freed = 0;
+    for (ib_mesh *m : mine) {
+      freed += ib_mesh_bytes(m);
+      ib_mesh_destroy(m);
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    doomed_bytes -= freed;
+    try_admit_locked();
   }
 
   // device bytes a request needs while it is baked, and what stays resident afterwards
@@ -168,13 +222,14 @@ struct ib_server {
   void try_admit_locked() {
     while (!pending.empty()) {
       Ticket &t = tickets[pending.front()];
-      if (in_use + t.est > budget) evict_locked(t.est);
-      if (in_use + t.est > budget) {
+      if (held_locked() + t.est > budget) evict_locked(t.est);
+      if (held_locked() + t.est > budget) {
         if (t.est > budget) {  // can never fit
           t.state = ST_FAILED;
           t.status = IB_ERR_OUT_OF_BUDGET;
           t.err = "request needs " + std::to_string(t.est) + " bytes, the budget is " + std::to_string(budget);
           t.t_done = now_s();
+          ++failed;
           pending.pop_front();
           cv_done.notify_all();
           continue;
@@ -196,9 +251,11 @@ struct ib_server {
     std::vector<ib_job> jobs;
     for (;;) {
       batch.clear();
+      reap(w);
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv_work.wait(lk, [&] { return stop || !ready.empty(); });
+        cv_work.wait(lk, [&] { return stop || !ready.empty() || !doomed[w].empty(); });
+        if (!doomed[w].empty() && !stop) continue;  // destroy evicted meshes first, outside the lock
         if (stop && ready.empty()) return;
         while (!ready.empty() && (int)batch.size() < max_batch) {
           batch.push_back(ready.front());
@@ -266,7 +323,7 @@ struct ib_server {
         if (!ok) {
           std::lock_guard<std::mutex> lk(mu);
           t->status = IB_ERR_DEVICE;
-          t->err = "staging the request on the device failed";
+          t->err = "staging the request on the device failed (hipMalloc or the soup upload)";
         }
       }
       // keyed requests that missed: build their mesh once, keep it resident under the key
@@ -305,6 +362,7 @@ struct ib_server {
         {
           CachedMesh cm;
           cm.mesh = mh;
+          cm.owner = w;
           cm.bytes = ib_mesh_bytes(mh);
           cm.last_use = ++mesh_clock;
           cm.users = 1;
@@ -353,6 +411,7 @@ struct ib_server {
           in_use -= t.est;
           in_use += t.resident;
           if (t.state == ST_FAILED) {  // failed requests keep nothing
+            ++failed;
             cache.put(t.d_out, t.out_bytes);
             cache.put(t.d_index, t.index_bytes);
             cache.put(t.d_occ, t.occ_bytes);
@@ -390,9 +449,12 @@ ib_status ib_server_create(int device, uint64_t hbm_budget, int32_t workers, int
   size_t freeb = 0, total = 0;
   (void)hipMemGetInfo(&freeb, &total);
   s->budget = hbm_budget ? hbm_budget : (uint64_t)(0.8 * (double)freeb);
+  s->doomed.resize(workers);
   for (int w = 0; w < workers; ++w) {
+    // every worker context gets the whole budget (not 90% of free memory each): the server's admission
+    // is what divides the budget between the workers' batches
     ib_context *c = nullptr;
-    const ib_status st = ib_create(device, 0, &c);
+    const ib_status st = ib_create(device, s->budget, &c);
     if (st != IB_OK) {
       for (auto *x : s->ctxs) ib_destroy(x);
       delete s;
@@ -422,6 +484,9 @@ void ib_server_destroy(ib_server *s) {
   }
   for (auto &kv : s->meshes) ib_mesh_destroy(kv.second.mesh);  // before the contexts that own their memory
   s->meshes.clear();
+  for (auto &v : s->doomed)
+    for (ib_mesh *m : v) ib_mesh_destroy(m);
+  s->doomed.clear();
   s->cache.clear();
   for (auto *c : s->ctxs) ib_destroy(c);
   delete s;
@@ -567,6 +632,9 @@ ib_status ib_server_stats(ib_server *s, ib_server_counters *out) {
   out->mesh_builds = s->mesh_builds;
   out->meshes_resident = s->meshes.size();
   out->mesh_bytes = s->mesh_bytes;
+  out->evictions = s->evictions;
+  out->failed = s->failed;
+  out->alloc_failures = s->cache.failures;
   return IB_OK;
 }
 

@@ -7,6 +7,10 @@ from ._paths import LIB_DIR
 LIB_PATH = os.path.join(LIB_DIR, "libitaly_bake.so")
 
 IB_OK = 0
+IB_ERR_INVALID_ARGUMENT = 1
+IB_ERR_OUT_OF_BUDGET = 3
+IB_ERR_CAPACITY = 7
+IB_ERR_INTERNAL = 9
 STATUS_NAMES = {
     0: "IB_OK", 1: "IB_ERR_INVALID_ARGUMENT", 2: "IB_ERR_INVALID_MESH", 3: "IB_ERR_OUT_OF_BUDGET",
     4: "IB_ERR_DEVICE", 5: "IB_ERR_NO_DEVICE", 6: "IB_ERR_UNSUPPORTED", 7: "IB_ERR_CAPACITY",
@@ -58,13 +62,14 @@ class Stats(ctypes.Structure):
         ("ms_total", ctypes.c_double), ("ms_ingest", ctypes.c_double), ("ms_bvh", ctypes.c_double),
         ("ms_occupancy", ctypes.c_double), ("ms_columns", ctypes.c_double), ("ms_sdf", ctypes.c_double),
         ("triangles", ctypes.c_uint64), ("voxels", ctypes.c_uint64), ("occ_cells", ctypes.c_uint64),
-        ("bvh_depth", ctypes.c_int32), ("deep_stack", ctypes.c_int32), ("boundary_edges", ctypes.c_int64),
+        ("bvh_depth", ctypes.c_int32), ("pad0", ctypes.c_int32), ("boundary_edges", ctypes.c_int64),
         ("column_overflows", ctypes.c_uint64), ("rechecked", ctypes.c_uint64), ("ties", ctypes.c_uint64),
-        ("box_tests", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4),
+        ("box_tests", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64), ("stack_overflows", ctypes.c_uint64),
+        ("ms_caps", ctypes.c_double), ("reserved", ctypes.c_uint64 * 2),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if k not in ("reserved", "pad0")}
 
 
 class Material(ctypes.Structure):

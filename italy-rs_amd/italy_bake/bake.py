@@ -11,7 +11,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import BakeError, Grid, Job, Stats, check  # noqa: F401
+from ._lib import (IB_ERR_CAPACITY, IB_ERR_INTERNAL, IB_ERR_INVALID_ARGUMENT, IB_ERR_OUT_OF_BUDGET, IB_OK,  # noqa: F401
+                   BakeError, Grid, Job, Stats, check)
 
 OPT_COUNT_WORK = 1
 OPT_QUERY_VARIANT = 2

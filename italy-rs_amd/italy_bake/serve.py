@@ -26,7 +26,8 @@ class TicketInfo(ctypes.Structure):
 class Counters(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in
                 ("budget", "in_use", "cached", "pending", "ready", "running_batches", "batches", "completed",
-                 "mesh_hits", "mesh_builds", "meshes_resident", "mesh_bytes")]
+                 "mesh_hits", "mesh_builds", "meshes_resident", "mesh_bytes", "evictions", "failed",
+                 "alloc_failures")]
 
 
 def _bind(lib):
