@@ -222,6 +222,23 @@ ib_status ib_bake_dense(ib_context *ctx, const float *xyz, uint64_t ntri, const 
 ib_status ib_bake_dense_device(ib_context *ctx, const float *d_xyz, uint64_t ntri, const ib_grid *sdf,
                                float *d_distances, const ib_grid *occ, uint32_t *d_occ_words, void *stream);
 
+/* Scratch-free form of ib_bake_dense_device, for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference, whose bakeSdf allocates per call (src/convert/sdf_baker.cpp:165-305).  Same arguments and
+ * results (bit for bit) as ib_bake_dense_device; it calls no hipMalloc, hipFree or other allocating
+ * device routine: every table of the bake lives in one fixed 96 MiB region per device, which the library
+ * holds from the first ib_create on that device until the process ends.  Limits (IB_ERR_CAPACITY before
+ * any work beyond them): at most IB_NOALLOC_MAX_TRIANGLES triangles, SDF grids of at most
+ * IB_NOALLOC_MAX_COLUMNS columns (nx * ny) and IB_NOALLOC_MAX_VOXELS voxels; the occupancy grid is
+ * output only and has no limit.  Every call on one device shares the region, whatever its context, and
+ * a call keeps it until the call's device work ends: calls that overlap on one device (two contexts, two
+ * threads, or a later call before the earlier one's work has finished on another context) race. */
+#define IB_NOALLOC_MAX_TRIANGLES 65536ull
+#define IB_NOALLOC_MAX_COLUMNS 16384ull
+#define IB_NOALLOC_MAX_VOXELS 2097152ull
+ib_status ib_bake_dense_device_noalloc(ib_context *ctx, const float *d_xyz, uint64_t ntri, const ib_grid *sdf,
+                                       float *d_distances, const ib_grid *occ, uint32_t *d_occ_words, void *stream);
+
 /* Batched bake: every job's LBVH is built, then ONE persistent launch drains the brick tiles of
  * all jobs from a device work queue.  Synchronous: returns when every output is written.  Results
  * are identical to baking each job alone with ib_bake_dense_device (no counterpart in the

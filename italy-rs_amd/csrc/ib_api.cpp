@@ -120,10 +120,40 @@ struct ib_context {
     free_blocks.clear();
     cached = 0;
   }
+  // Scratch-free calls: while fixed_base is set, named scratch is carved from that fixed device region
+  // (bump allocation, 256-byte aligned; a name asked for again gets its earlier range when it fits)
+  // instead of the allocating scratch below.  fixed_base is set for one call and cleared after it.
+  char *fixed_base = nullptr;
+  size_t fixed_cap = 0, fixed_used = 0;
+  void *noalloc_base = nullptr;  // this device's region of ib_bake_dense_device_noalloc (set by ib_create)
+  std::unordered_map<std::string, std::pair<size_t, size_t>> fixed_map;  // name -> (offset, bytes)
+  void begin_fixed(void *base, size_t cap) {
+    fixed_base = static_cast<char *>(base);
+    fixed_cap = cap;
+    fixed_used = 0;
+    fixed_map.clear();
+  }
+  void end_fixed() { fixed_base = nullptr; }
+
   // named scratch that only grows; contents are not preserved across growth
   template <typename T>
   ib_status scratch(const char *name, size_t count, T **out) {
     const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (fixed_base) {
+      auto it = fixed_map.find(name);
+      if (it == fixed_map.end() || it->second.second < bytes) {
+        const size_t off = (fixed_used + 255) & ~size_t(255);
+        if (off + bytes > fixed_cap)
+          return fail(IB_ERR_CAPACITY, std::string("the fixed scratch region (") + std::to_string(fixed_cap) +
+                                           " bytes) cannot hold '" + name + "' (" + std::to_string(bytes) +
+                                           " bytes after " + std::to_string(off) + ")");
+        fixed_map[name] = {off, bytes};
+        fixed_used = off + bytes;
+        it = fixed_map.find(name);
+      }
+      *out = reinterpret_cast<T *>(fixed_base + it->second.first);
+      return IB_OK;
+    }
     Buf &b = bufs[name];
     if (b.bytes < bytes) {
       const size_t want = std::max(bytes + bytes / 4, 2 * b.bytes);  // geometric growth: few reallocations
@@ -738,6 +768,31 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
 
 }  // namespace
 
+// Scratch of the scratch-free bake (ib_bake_dense_device_noalloc): one fixed region per device, part of
+// the library's code object, so every process that loads the library holds kNoAllocScratch bytes on each
+// device it uses until it exits.  Host code never names the array except to look up its device address
+// (noalloc_scratch_ptr); kernels get that address.  All calls on one device share the region.
+constexpr size_t kNoAllocScratch = 96ull << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
+constexpr int kMaxDevices = 64;
+static void *g_noalloc_ptr[kMaxDevices] = {};
+static std::mutex g_noalloc_mu;
+
+// the region's device address on `device`, looked up once per device with that device current
+static ib_status noalloc_scratch_ptr(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(IB_ERR_UNSUPPORTED, "device ordinal above 63");
+  std::lock_guard<std::mutex> lk(g_noalloc_mu);
+  if (!g_noalloc_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(IB_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[device] = p;
+  }
+  *out = g_noalloc_ptr[device];
+  return IB_OK;
+}
+
 // A resident mesh: its soup copy, tree, records and caps live in `keep`, which belongs to context c.
 struct ib_mesh {
   ib_context *c;
@@ -834,6 +889,11 @@ ib_status ib_create(int device, uint64_t budget, ib_context **out) {
   }
   hipDeviceProp_t prop;
   c->cus = hipGetDeviceProperties(&prop, device) == hipSuccess ? prop.multiProcessorCount : 256;
+  if (noalloc_scratch_ptr(device, &c->noalloc_base) != IB_OK) {
+    const std::string e = g_err;
+    ib_destroy(c);
+    return fail(IB_ERR_DEVICE, e);
+  }
   *out = c;
   return IB_OK;
 }
@@ -896,6 +956,28 @@ ib_status ib_bake_dense_device(ib_context *c, const float *d_xyz, uint64_t ntri,
   hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
   Ordered ord(c, s);
   return bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s);
+}
+
+ib_status ib_bake_dense_device_noalloc(ib_context *c, const float *d_xyz, uint64_t ntri, const ib_grid *sdf,
+                                       float *d_dist, const ib_grid *occ, uint32_t *d_occ, void *stream) {
+  if (!c) return fail(IB_ERR_INVALID_ARGUMENT, "null context");
+  if (ntri > IB_NOALLOC_MAX_TRIANGLES)
+    return fail(IB_ERR_CAPACITY, "the scratch-free bake takes at most " + std::to_string(IB_NOALLOC_MAX_TRIANGLES) +
+                                     " triangles (got " + std::to_string(ntri) + ")");
+  if (sdf && ((uint64_t)sdf->nx * (uint64_t)sdf->ny > IB_NOALLOC_MAX_COLUMNS ||
+              (uint64_t)sdf->nx * sdf->ny * sdf->nz > IB_NOALLOC_MAX_VOXELS))
+    return fail(IB_ERR_CAPACITY, "the scratch-free bake takes SDF grids of at most " +
+                                     std::to_string(IB_NOALLOC_MAX_COLUMNS) + " columns (nx * ny) and " +
+                                     std::to_string(IB_NOALLOC_MAX_VOXELS) + " voxels");
+  std::lock_guard<std::mutex> lk(c->mu);
+  IB_HIP(hipSetDevice(c->device));
+  hipStream_t s = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  Ordered ord(c, s);
+  // the region keeps the previous call's contents: the bake clears every table it reads before use
+  c->begin_fixed(c->noalloc_base, kNoAllocScratch);
+  const ib_status st = bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s);
+  c->end_fixed();
+  return st;
 }
 
 ib_status ib_bake_dense(ib_context *c, const float *xyz, uint64_t ntri, const ib_grid *sdf, float *dist,

@@ -11,6 +11,10 @@ IB_ERR_INVALID_ARGUMENT = 1
 IB_ERR_OUT_OF_BUDGET = 3
 IB_ERR_CAPACITY = 7
 IB_ERR_INTERNAL = 9
+# limits of ib_bake_dense_device_noalloc (italy_bake.h)
+NOALLOC_MAX_TRIANGLES = 65536
+NOALLOC_MAX_COLUMNS = 16384
+NOALLOC_MAX_VOXELS = 2097152
 STATUS_NAMES = {
     0: "IB_OK", 1: "IB_ERR_INVALID_ARGUMENT", 2: "IB_ERR_INVALID_MESH", 3: "IB_ERR_OUT_OF_BUDGET",
     4: "IB_ERR_DEVICE", 5: "IB_ERR_NO_DEVICE", 6: "IB_ERR_UNSUPPORTED", 7: "IB_ERR_CAPACITY",
@@ -29,6 +33,7 @@ EXPORTS = [
     "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
     "ib_weld_positions", "ib_weld_positions_device", "ib_watertight", "ib_watertight_device",
     "ib_mesh_create", "ib_mesh_destroy", "ib_mesh_bytes", "ib_mesh_triangles", "ib_triangle_materials",
+    "ib_bake_dense_device_noalloc",
 ]
 
 
@@ -120,6 +125,7 @@ def load():
     lib.ib_splat_materials.argtypes = [_vp, _vp, _u64, _vp, G, _vp, _vp, _vp]
     lib.ib_bake_dense.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp]
     lib.ib_bake_dense_device.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp, _vp]
+    lib.ib_bake_dense_device_noalloc.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp, _vp]
     lib.ib_device_alloc.argtypes = [_vp, _u64, ctypes.POINTER(_vp)]
     lib.ib_device_free.argtypes = [_vp, _vp]
     lib.ib_memcpy_h2d.argtypes = [_vp, _vp, _vp, _u64]

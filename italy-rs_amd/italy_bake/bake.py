@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (IB_ERR_CAPACITY, IB_ERR_INTERNAL, IB_ERR_INVALID_ARGUMENT, IB_ERR_OUT_OF_BUDGET, IB_OK,  # noqa: F401
-                   BakeError, Grid, Job, Stats, check)
+                   NOALLOC_MAX_COLUMNS, NOALLOC_MAX_TRIANGLES, NOALLOC_MAX_VOXELS, BakeError, Grid, Job, Stats, check)
 
 OPT_COUNT_WORK = 1
 OPT_QUERY_VARIANT = 2
@@ -89,6 +89,15 @@ class Context:
                                             ctypes.byref(occ_grid) if occ_grid is not None else None,
                                             ctypes.c_void_p(d_occ) if d_occ else None,
                                             ctypes.c_void_p(stream) if stream else None))
+
+    def bake_dense_device_noalloc(self, d_xyz, ntri, sdf_grid, d_dist, occ_grid=None, d_occ=None, stream=None):
+        """ib_bake_dense_device_noalloc: bake_dense_device without any device allocation (fixed scratch)."""
+        check(self.lib.ib_bake_dense_device_noalloc(self._h, ctypes.c_void_p(d_xyz), int(ntri),
+                                                    ctypes.byref(sdf_grid) if sdf_grid is not None else None,
+                                                    ctypes.c_void_p(d_dist) if d_dist else None,
+                                                    ctypes.byref(occ_grid) if occ_grid is not None else None,
+                                                    ctypes.c_void_p(d_occ) if d_occ else None,
+                                                    ctypes.c_void_p(stream) if stream else None))
 
     def bake_batch_device(self, jobs, stream=None):
         """One batched bake (ib_bake_batch_device) of a list of Job structures; synchronous.
