@@ -653,6 +653,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ColumnArgs ca{};
   ca.nodes = m.nodes;
   ca.nnodes = std::max(m.n - 1, 1);
+  ca.ntri = m.n;
   ca.tris = m.tris;
   ca.recs = m.recs;
   ca.nodes4 = m.nodes4;
@@ -1731,6 +1732,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       SdfArgs sa{};
       sa.col.nodes = mesh[j].nodes;
       sa.col.nnodes = std::max(mesh[j].n - 1, 1);
+      sa.col.ntri = mesh[j].n;
       sa.col.tris = mesh[j].tris;
       sa.col.recs = mesh[j].recs;
       sa.col.nodes4 = mesh[j].nodes4;

@@ -773,14 +773,27 @@ __device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y,
 // ============================================================================================
 // hole caps: solid angles (Van Oosterom & Strackee 1983)
 // ============================================================================================
-__device__ __forceinline__ float solid_angle_f32(F3 q, const float *t) {
+// Omega = 2 atan2(det, den).  `singular` is set when q lies on or next to the triangle itself, where the
+// solid angle jumps by 4 pi: den within 1e-3 la lb lc of zero or below (q's projection inside the
+// triangle or at its edge) while q is within 1e-3 la lb lc of its plane by det, or within a few FP32 ulps
+// of it along z (|det| / |n_z|, n = (b - a) x (c - a)), the resolution of the column's crossing z.
+// There the FP32 sign, and the consistency of this solid angle with the column crossing of the same
+// cap, are not reliable, and the voxel goes to the FP64 recheck.
+__device__ __forceinline__ float solid_angle_f32(F3 q, const float *t, bool &singular) {
   const F3 a = {t[0] - q.x, t[1] - q.y, t[2] - q.z};
   const F3 b = {t[3] - q.x, t[4] - q.y, t[5] - q.z};
   const F3 c = {t[6] - q.x, t[7] - q.y, t[8] - q.z};
   const float la = sqrtf(dot3(a, a)), lb = sqrtf(dot3(b, b)), lc = sqrtf(dot3(c, c));
   const F3 bc = {b.y * c.z - b.z * c.y, b.z * c.x - b.x * c.z, b.x * c.y - b.y * c.x};
   const float det = dot3(a, bc);
-  const float den = la * lb * lc + dot3(a, b) * lc + dot3(b, c) * la + dot3(c, a) * lb;
+  const float l3 = la * lb * lc;
+  const float den = l3 + dot3(a, b) * lc + dot3(b, c) * la + dot3(c, a) * lb;
+  const float tol = 1e-3f * l3;
+  if (den <= tol) {
+    const float nz = (b.x - a.x) * (c.y - a.y) - (b.y - a.y) * (c.x - a.x);
+    const float zeps = 4.8e-7f * (fabsf(q.z) + fabsf(t[2]) + fabsf(t[5]) + fabsf(t[8]));  // ~4 ulps of z
+    singular = singular || fabsf(det) <= tol || fabsf(det) <= zeps * fabsf(nz);
+  }
   return 2.0f * atan2f(det, den);
 }
 
@@ -794,6 +807,7 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
   if (a.ncap == 0 || a.ncapnode == 0 || !a.cap_nodes) return;
   const float inv4pi = (float)(1.0 / (4.0 * M_PI));
   float sum = 0.0f, cmp = 0.0f, ub = 0.0f;
+  bool singular = false;
   int stack[kCapStack];
   int sp = 0;
   stack[sp++] = 0;
@@ -811,7 +825,7 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
     if (nd.left < 0) {
       const int first = -1 - nd.left;
       for (int k = 0; k < nd.right; ++k) {
-        const float om = solid_angle_f32(q, a.caps + 9 * (first + k));
+        const float om = solid_angle_f32(q, a.caps + 9 * (first + k), singular);
         const float t = sum + om;
         cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
         sum = t;
@@ -824,34 +838,32 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
     }
   }
   w = (sum + cmp) * inv4pi;
-  bound = ub * 1.0001f + 1e-5f;  // + FP32 rounding of the bound and the sum
+  bound = singular ? 1.0f : ub * 1.0001f + 1e-5f;  // + FP32 rounding of the bound and the sum
 }
 
-__device__ __forceinline__ double solid_angle_f64(F3 q, const float *t) {
+// FP64 solid angle of triangle t (9 floats) at q; `singular` as for the FP32 form, with 1e-9 in place of
+// 1e-3 and the same z resolution (the column crossings are stored in FP32)
+__device__ __forceinline__ double solid_angle_f64(F3 q, const float *t, bool &singular) {
   const double ax = (double)t[0] - q.x, ay = (double)t[1] - q.y, az = (double)t[2] - q.z;
   const double bx = (double)t[3] - q.x, by = (double)t[4] - q.y, bz = (double)t[5] - q.z;
   const double cx = (double)t[6] - q.x, cy = (double)t[7] - q.y, cz = (double)t[8] - q.z;
   const double la = sqrt(dot3d(ax, ay, az, ax, ay, az)), lb = sqrt(dot3d(bx, by, bz, bx, by, bz)),
                lc = sqrt(dot3d(cx, cy, cz, cx, cy, cz));
   const double det = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx);
-  const double den = la * lb * lc + dot3d(ax, ay, az, bx, by, bz) * lc + dot3d(bx, by, bz, cx, cy, cz) * la +
+  const double l3 = la * lb * lc;
+  const double den = l3 + dot3d(ax, ay, az, bx, by, bz) * lc + dot3d(bx, by, bz, cx, cy, cz) * la +
                      dot3d(cx, cy, cz, ax, ay, az) * lb;
+  if (den <= 1e-9 * l3) {
+    const double nz = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+    const double zeps = 4.8e-7 * (fabs((double)q.z) + fabs((double)t[2]) + fabs((double)t[5]) + fabs((double)t[8]));
+    singular = singular || fabs(det) <= 1e-9 * l3 || fabs(det) <= zeps * fabs(nz);
+  }
   return 2.0 * atan2(det, den);
 }
 
-// FP64 winding number of the caps at q, computed by a whole wave: cap i goes to lane i % 64, each
-// lane keeps a Neumaier-compensated sum, and the 64 partial sums are combined by a butterfly of
-// error-free two-sums.  Every lane returns lane 0's total, so the value is wave-uniform and the
-// same from run to run.
-__device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int ncap) {
-  const int lane = threadIdx.x & 63;
-  double sum = 0.0, comp = 0.0;
-  for (int i = lane; i < ncap; i += 64) {
-    const double om = solid_angle_f64(q, caps + 9 * i);
-    const double s = sum + om;
-    comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
-    sum = s;
-  }
+// Compensated sum of the 64 lanes' (sum, compensation) pairs by a butterfly of error-free two-sums;
+// every lane returns the total, so the value is wave-uniform and the same from run to run.
+__device__ __forceinline__ double wave_sum_f64(double sum, double comp) {
   double v = sum, e = comp;
   for (int off = 32; off >= 1; off >>= 1) {
     const double ov = __shfl_xor(v, off), oe = __shfl_xor(e, off);
@@ -859,7 +871,41 @@ __device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int
     e = e + oe + ((v - (s - bp)) + (ov - bp));
     v = s;
   }
-  return __shfl(v + e, 0) * (1.0 / (4.0 * M_PI));
+  return __shfl(v + e, 0);
+}
+
+// FP64 winding number of the caps at q, computed by a whole wave: cap i goes to lane i % 64, each
+// lane keeps a Neumaier-compensated sum, and the 64 partial sums are combined by wave_sum_f64.
+// `singular` (wave-uniform) is set when q lies on or next to a cap triangle.
+__device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int ncap, bool &singular) {
+  const int lane = threadIdx.x & 63;
+  double sum = 0.0, comp = 0.0;
+  bool sing = false;
+  for (int i = lane; i < ncap; i += 64) {
+    const double om = solid_angle_f64(q, caps + 9 * i, sing);
+    const double s = sum + om;
+    comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
+    sum = s;
+  }
+  singular = __ballot(sing) != 0;
+  return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+}
+
+// FP64 winding number of the mesh itself at q (no caps), by the whole wave over its ntri triangles: the
+// direct form of the closure identity's sum, for a q that lies on a cap (where the identity does not hold)
+__device__ __noinline__ double mesh_winding_f64_wave(F3 q, const float4 *__restrict__ tris, int ntri) {
+  const int lane = threadIdx.x & 63;
+  double sum = 0.0, comp = 0.0;
+  bool unused = false;
+  for (int i = lane; i < ntri; i += 64) {
+    const float4 *t = tris + 3 * i;
+    const float v[9] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z, t[2].x, t[2].y, t[2].z};
+    const double om = solid_angle_f64(q, v, unused);
+    const double s = sum + om;
+    comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
+    sum = s;
+  }
+  return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
 }
 
 // conservative ray / box slab test for the parity tie-break (boxes inflated by a relative margin)
@@ -1116,7 +1162,9 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, u
       inside = wi != 0;
     } else {
       if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
-      const double w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap);
+      bool on_cap = false;
+      double w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
+      if (on_cap) w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
       if (fabs(fabs(w64) - 0.5) >= kTieEps) {
         inside = fabs(w64) > 0.5;
       } else {

@@ -50,6 +50,7 @@ struct GridArgs {
 struct ColumnArgs {
   const Node *nodes;
   int nnodes;         // internal nodes (bounds every traversal loop)
+  int ntri;           // triangles (tris / recs entries)
   const float4 *tris;
   const TriRec *recs;  // distance records, same order as tris
   const Node4 *nodes4; // 4-wide view of the nodes (brick query)

@@ -637,21 +637,34 @@ void orc_parity_dirs(float dirs[15]) {
   memcpy(dirs, tab, sizeof(tab));
 }
 
-int orc_parity_inside(const float q[3], const float *xyz, size_t ntri) {
+void orc_parity_hits(const float q[3], const float *xyz, size_t ntri, int hits[5]) {
   float dirs[15];
   orc_parity_dirs(dirs);
-  int votes = 0;
   for (int v = 0; v < 5; ++v) {
     ray_shear r;
     make_shear(dirs + 3 * v, &r);
-    int hits = 0;
+    hits[v] = 0;
     for (size_t t = 0; t < ntri; ++t) {
       double th;
-      if (ray_tri(q, &r, xyz + 9 * t, &th) != 0 && th > 0.0) hits++;
+      if (ray_tri(q, &r, xyz + 9 * t, &th) != 0 && th > 0.0) hits[v]++; /* every hit, sdf_bake.cu:58-65 */
     }
-    if (hits & 1) votes++; /* sdf_bake.cu:66-67 */
   }
-  return votes * 2 > 5; /* sdf_bake.cu:69 */
+}
+
+int orc_parity_inside(const float q[3], const float *xyz, size_t ntri) {
+  int hits[5], votes = 0;
+  orc_parity_hits(q, xyz, ntri, hits);
+  for (int v = 0; v < 5; ++v)
+    if (hits[v] & 1) votes++; /* sdf_bake.cu:66-67 */
+  return votes * 2 > 5;       /* sdf_bake.cu:69 */
+}
+
+int orc_gwn_sign(const float q[3], const float *xyz, size_t ntri, int *tie) {
+  const double qd[3] = {q[0], q[1], q[2]};
+  const double w = orc_gwn_f64(qd, xyz, ntri);
+  const int t = fabs(fabs(w) - 0.5) < 1e-6;
+  if (tie) *tie = t;
+  return t ? orc_parity_inside(q, xyz, ntri) : fabs(w) > 0.5;
 }
 
 /* ============================================================================
@@ -1217,6 +1230,61 @@ int orc_udf_points_f32(const float *xyz, size_t ntri, const float *pts, size_t n
   udf_job j = {&c, pts, npts, out, 0, &mu};
   pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
   for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, udf_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+  free(sorted);
+  free(recs);
+  orc_bvh_free(&b);
+  return 0;
+}
+
+typedef struct {
+  const qctx *c;
+  const float *xyz;
+  size_t ntri;
+  const float *pts;
+  size_t npts;
+  float *out;
+  uint8_t *tie;
+  size_t next;
+  pthread_mutex_t *mu;
+} signed_job;
+
+static void *signed_fn(void *arg) {
+  signed_job *j = (signed_job *)arg;
+  uint64_t nodes = 0, tests = 0;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 16;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 16 < j->npts ? i0 + 16 : j->npts;
+    for (size_t i = i0; i < i1; ++i) {
+      const float *q = j->pts + 3 * i;
+      const float d = sqrtf(closest2(j->c, q, &nodes, &tests));
+      int tie = 0;
+      const int inside = orc_gwn_sign(q, j->xyz, j->ntri, &tie);
+      j->out[i] = inside ? -d : d;
+      if (j->tie) j->tie[i] = (uint8_t)tie;
+    }
+  }
+  return NULL;
+}
+
+int orc_signed_points_gwn(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out,
+                          uint8_t *tie) {
+  orc_bvh b;
+  if (ntri == 0 || orc_bvh_build(xyz, ntri, &b) != 0) return -1;
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  orc_trirec *recs = make_recs(sorted, ntri);
+  qctx c = {sorted, &b, recs, max_abs_coord(sorted, ntri)};
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  signed_job j = {&c, xyz, ntri, pts, npts, out, tie, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, signed_fn, &j);
   for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
   free(tid);
   free(sorted);

@@ -89,6 +89,17 @@ int orc_crossings_up(const float q[3], const float *xyz, size_t ntri);
 int orc_parity_inside(const float q[3], const float *xyz, size_t ntri);
 /* the 5 tie-break directions used by both the oracle and the device code */
 void orc_parity_dirs(float dirs[15]);
+/* hits (t > 0, every crossing counted once) of each of the 5 parity rays from q */
+void orc_parity_hits(const float q[3], const float *xyz, size_t ntri, int hits[5]);
+/* The bake's sign rule for any mesh, open or closed: inside iff |w| > 0.5 for the FP64 brute-force
+ * winding number w at the FP32 point q, except that |w| within 1e-6 of 0.5 is a tie, decided by
+ * orc_parity_inside (*tie = 1 then; tie may be NULL). */
+int orc_gwn_sign(const float q[3], const float *xyz, size_t ntri, int *tie);
+/* Signed distances at FP32 points of any mesh: the FP32 closest-triangle distance through the LBVH (equal
+ * to the FP32 brute force) with the sign of orc_gwn_sign (negative inside); tie (may be NULL) gets 1 per
+ * point decided by parity.  O(ntri) per point for the sign: for small meshes and samples.  -1: empty mesh. */
+int orc_signed_points_gwn(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out,
+                          uint8_t *tie);
 
 /* ---- LBVH (Karras 2012) --------------------------------------------------------- */
 typedef struct {

@@ -80,6 +80,10 @@ def _load():
     lib.orc_crossings_up.argtypes = [_f32p, _f32p, ctypes.c_size_t]
     lib.orc_parity_inside.argtypes = [_f32p, _f32p, ctypes.c_size_t]
     lib.orc_parity_dirs.argtypes = [_f32p]
+    lib.orc_parity_hits.argtypes = [_f32p, _f32p, ctypes.c_size_t, _i32p]
+    lib.orc_gwn_sign.argtypes = [_f32p, _f32p, ctypes.c_size_t, _i32p]
+    lib.orc_signed_points_gwn.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p,
+                                          ctypes.c_void_p]
     lib.orc_bvh_build.argtypes = [_f32p, ctypes.c_size_t, ctypes.POINTER(Bvh)]
     lib.orc_bvh_free.argtypes = [ctypes.POINTER(Bvh)]
     lib.orc_morton_codes.argtypes = [_f32p, ctypes.c_size_t, _u32p]
@@ -299,6 +303,35 @@ def crossings_up(q, xyz):
 def parity_inside(q, xyz):
     xyz = _f32(xyz)
     return bool(lib().orc_parity_inside(_p(_f32(q), _f32p), _p(xyz, _f32p), xyz.size // 9))
+
+
+def parity_hits(q, xyz):
+    """Hits of each of the 5 parity rays from q (t > 0)."""
+    xyz = _f32(xyz)
+    h = np.zeros(5, np.int32)
+    lib().orc_parity_hits(_p(_f32(q), _f32p), _p(xyz, _f32p), xyz.size // 9, _p(h, _i32p))
+    return h
+
+
+def gwn_sign(q, xyz):
+    """(inside, tie): the bake's sign rule with the FP64 brute-force winding number, parity on ties."""
+    xyz = _f32(xyz)
+    t = np.zeros(1, np.int32)
+    r = lib().orc_gwn_sign(_p(_f32(q), _f32p), _p(xyz, _f32p), xyz.size // 9, _p(t, _i32p))
+    return bool(r), bool(t[0])
+
+
+def signed_points_gwn(xyz, pts, threads=0):
+    """(signed FP32 distances, tie flags) at FP32 points of any mesh (FP64 brute-force winding sign)."""
+    xyz = _f32(xyz)
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.empty(len(pts), np.float32)
+    tie = np.zeros(len(pts), np.uint8)
+    r = lib().orc_signed_points_gwn(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f32p), len(pts), int(threads),
+                                    _p(out, _f32p), tie.ctypes.data)
+    if r != 0:
+        raise ValueError("empty mesh")
+    return out, tie.astype(bool)
 
 
 def parity_dirs():

@@ -407,3 +407,62 @@ def test_udf_points_f64_equals_brute_force(oracle, meshgen, which):
     d_brute, _ = oracle.sdf_f64_points(m.xyz, pts)
     assert np.array_equal(d_bvh, np.abs(d_brute))
     assert (d_bvh[:200] == 0).all()
+
+
+def _open_box_hits(q, dirs):
+    """Parity-ray hits of the cube [-0.5, 0.5]^3 without its +z face, from a point q inside it or on the open
+    face's plane, by slab arithmetic: a ray leaves a convex box through exactly one face, and crosses the
+    surface once unless that face is the missing one."""
+    hits = []
+    for d in dirs.reshape(5, 3).astype(np.float64):
+        t_exit, face = np.inf, None
+        for k in range(3):
+            if d[k] != 0.0:
+                t = ((0.5 if d[k] > 0 else -0.5) - q[k]) / d[k]
+                if t < t_exit:
+                    t_exit, face = t, (k, d[k] > 0)
+        hits.append(0 if face == (2, True) else 1)
+    return np.array(hits)
+
+
+def test_parity_hits_and_tie_on_an_open_face(oracle):
+    """On the plane of a missing cube face, inside the hole, the winding number of the open box is exactly 1/2
+    (the five faces subtend half the sphere from either side), so the sign is a tie and the 5-ray parity vote
+    decides (sdf_bake.cu:53-69): rays leaving through the hole see no surface, the others one face."""
+    import shapes
+
+    box = shapes.box_faces(0.5, 4, skip=("+z",))
+    dirs = oracle.parity_dirs()
+    assert oracle.boundary_edge_count(box) == 16
+    for q in ([0.1, 0.2, 0.5], [-0.3, 0.05, 0.5], [0.45, -0.45, 0.5]):
+        hits = oracle.parity_hits(q, box)
+        assert np.array_equal(hits, _open_box_hits(np.array(q, np.float64), dirs)), (q, hits)
+        assert np.array_equal(hits, [0, 1, 0, 1, 0])  # +z and the two upward rays leave through the hole
+        assert abs(oracle.gwn_f64(q, box) - 0.5) < 1e-12
+        assert oracle.gwn_sign(q, box) == (False, True)  # 2 of 5 odd: outside, by parity
+    rng = np.random.default_rng(3)
+    for q in rng.uniform(-0.45, 0.45, (20, 3)):  # strictly inside: the vote agrees with |w| > 1/2
+        hits = oracle.parity_hits(q, box)
+        assert np.array_equal(hits, _open_box_hits(q.astype(np.float32).astype(np.float64), dirs))
+        inside, tie = oracle.gwn_sign(q, box)
+        assert not tie and inside == (abs(oracle.gwn_f64(q, box)) > 0.5)
+    # a planted mistake: counting the +z ray as a hit turns the vote at the tie
+    q = [0.1, 0.2, 0.5]
+    h = oracle.parity_hits(q, box)
+    h[0] += 1
+    assert (np.count_nonzero(h & 1) * 2 > 5) != oracle.parity_inside(q, box)
+
+
+def test_signed_points_gwn_matches_brute_force(oracle):
+    """signed_points_gwn: FP32 brute-force distance, sign from the FP64 winding number, on an open mesh."""
+    import shapes
+
+    box = shapes.box_faces(0.5, 3, skip=("+z", "-x"))
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(-0.8, 0.8, (400, 3)).astype(np.float32)
+    d, tie = oracle.signed_points_gwn(box, pts, threads=4)
+    ref = np.sqrt(oracle.udf2_f32_brute(box, pts))
+    assert np.array_equal(np.abs(d), ref)
+    w = np.array([oracle.gwn_f64(p, box) for p in pts.astype(np.float64)])
+    assert not tie.any()
+    assert np.array_equal(d < 0, np.abs(w) > 0.5)
