@@ -1272,6 +1272,63 @@ static void *signed_fn(void *arg) {
   return NULL;
 }
 
+typedef struct {
+  const qctx *c;
+  const float *pts;
+  size_t npts;
+  float *out;
+  size_t next;
+  pthread_mutex_t *mu;
+} closed_job;
+
+static void *closed_fn(void *arg) {
+  closed_job *j = (closed_job *)arg;
+  uint64_t nodes = 0, tests = 0;
+  crossing *cr = NULL;
+  int cap = 0;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 64;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 64 < j->npts ? i0 + 64 : j->npts;
+    for (size_t i = i0; i < i1; ++i) {
+      const float *q = j->pts + 3 * i;
+      const float d = sqrtf(closest2(j->c, q, &nodes, &tests));
+      const int nc = column_crossings(j->c, q[0], q[1], &cr, &cap, &tests);
+      int w = 0;
+      for (int k = 0; k < nc; ++k)
+        if (cr[k].z > q[2]) w += cr[k].s;
+      j->out[i] = w != 0 ? -d : d;
+    }
+  }
+  free(cr);
+  return NULL;
+}
+
+int orc_signed_points_closed(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out) {
+  if (ntri == 0) return -2;
+  if (orc_boundary_edge_count(xyz, ntri) != 0) return -1;
+  orc_bvh b;
+  if (orc_bvh_build(xyz, ntri, &b) != 0) return -2;
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  orc_trirec *recs = make_recs(sorted, ntri);
+  qctx c = {sorted, &b, recs, max_abs_coord(sorted, ntri)};
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  closed_job j = {&c, pts, npts, out, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, closed_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+  free(sorted);
+  free(recs);
+  orc_bvh_free(&b);
+  return 0;
+}
+
 int orc_signed_points_gwn(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out,
                           uint8_t *tie) {
   orc_bvh b;

@@ -100,6 +100,10 @@ int orc_gwn_sign(const float q[3], const float *xyz, size_t ntri, int *tie);
  * point decided by parity.  O(ntri) per point for the sign: for small meshes and samples.  -1: empty mesh. */
 int orc_signed_points_gwn(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out,
                           uint8_t *tie);
+/* The dense bake at arbitrary FP32 points of a closed mesh: orc_bake_dense's distance and crossing sign
+ * (the signed count of +z crossings above the point, through the LBVH), per point instead of per slab.
+ * -1: the mesh has boundary edges, -2: empty. */
+int orc_signed_points_closed(const float *xyz, size_t ntri, const float *pts, size_t npts, int threads, float *out);
 
 /* ---- LBVH (Karras 2012) --------------------------------------------------------- */
 typedef struct {

@@ -84,6 +84,7 @@ def _load():
     lib.orc_gwn_sign.argtypes = [_f32p, _f32p, ctypes.c_size_t, _i32p]
     lib.orc_signed_points_gwn.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p,
                                           ctypes.c_void_p]
+    lib.orc_signed_points_closed.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
     lib.orc_bvh_build.argtypes = [_f32p, ctypes.c_size_t, ctypes.POINTER(Bvh)]
     lib.orc_bvh_free.argtypes = [ctypes.POINTER(Bvh)]
     lib.orc_morton_codes.argtypes = [_f32p, ctypes.c_size_t, _u32p]
@@ -332,6 +333,30 @@ def signed_points_gwn(xyz, pts, threads=0):
     if r != 0:
         raise ValueError("empty mesh")
     return out, tie.astype(bool)
+
+
+def signed_points_closed(xyz, pts, threads=0):
+    """The dense CPU bake's signed distances at FP32 points of a closed mesh (None if the mesh is open)."""
+    xyz = _f32(xyz)
+    pts = _f32(pts).reshape(-1, 3)
+    out = np.empty(len(pts), np.float32)
+    r = lib().orc_signed_points_closed(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f32p), len(pts), int(threads),
+                                       _p(out, _f32p))
+    if r == -1:
+        return None
+    if r != 0:
+        raise ValueError("empty mesh")
+    return out
+
+
+def cell_centres_at(origin, h, idx, nx, ny):
+    """FP32 centres, as the bake computes them, of the cells with linear indices idx ((z*ny + y)*nx + x)."""
+    idx = np.asarray(idx, np.int64)
+    x, y, z = idx % nx, (idx // nx) % ny, idx // (nx * ny)
+    h = np.float32(h)
+    o = [np.float32(v) for v in origin]
+    c = [o[k] + (v.astype(np.float32) + np.float32(0.5)) * h for k, v in enumerate((x, y, z))]
+    return np.stack(c, -1).astype(np.float32)
 
 
 def parity_dirs():

@@ -466,3 +466,17 @@ def test_signed_points_gwn_matches_brute_force(oracle):
     w = np.array([oracle.gwn_f64(p, box) for p in pts.astype(np.float64)])
     assert not tie.any()
     assert np.array_equal(d < 0, np.abs(w) > 0.5)
+
+
+def test_signed_points_closed_equals_the_dense_bake(oracle, meshgen):
+    """signed_points_closed is the dense CPU bake evaluated per point: equal bits at every cell centre."""
+    m = meshgen.displaced(4, 12)
+    h = np.float32(1.6 / 24)
+    origin = [-0.8, -0.8, -0.8]
+    dense, _ = oracle.bake_dense(m.xyz, origin, h, 24, 24, 24)
+    pts = oracle.cell_centres_at(origin, h, np.arange(24 ** 3), 24, 24)
+    got = oracle.signed_points_closed(m.xyz, pts, threads=4)
+    assert np.array_equal(got.view(np.uint32), dense.reshape(-1).view(np.uint32))
+    import shapes
+
+    assert oracle.signed_points_closed(shapes.box_faces(0.5, 2, skip=("+z",)), pts[:10]) is None
