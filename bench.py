@@ -21,7 +21,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "italy-rs_amd"))
 
 METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake latency"
-PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published, with packed FP32)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6  # without packed FP32, which this build disables
 PEAK_HBM_GBS = 8000.0
 # FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max,
 # clamps and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
@@ -57,6 +58,8 @@ def parse():
     ap.add_argument("--serve-stress-seconds", type=float, default=2.0,
                     help="arrival window of the served precision-stress + large-job mix (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--no-serve-capacity", dest="serve_capacity", action="store_false",
+                    help="skip the served capacity measurements (closed-loop saturation and rate sweep)")
     ap.add_argument("--batch-profile", type=str, default=None, metavar="SHAPE,BLOCKS",
                     help="only run the batched workload twice at this brick shape and blocks per CU (no autotuner, no "
                          "plain comparison) and print its timings: the command a kernel trace of the batch runs")
@@ -66,18 +69,57 @@ def parse():
     return ap.parse_args()
 
 
-def served_section(args, batched):
-    """Open-loop Poisson arrivals into the bake server (3 worker streams, HBM budget 200 GB).
+def host_cores():
+    """CPUs this process may run on: its affinity set, limited by a cgroup CPU quota when one is set."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            quota, period = f.read().split()[:2]
+        if quota != "max":
+            n = min(n, max(1, int(int(quota) // int(period))))
+    except (OSError, ValueError):
+        pass
+    return n
 
-    batched_distribution: requests drawn from the batched workload's distribution (dense SDF +
-      occupancy, 500..200k triangles, 32..256^3), keyed by mesh so that each LBVH stays resident after
-      its first request; batched_distribution_rebuilt: the same arrivals without keys (every request
-      uploaded and built).  The plain per-mesh bake of the same meshes is the comparison point.
+
+def plain_band_rate(ctx, meshes, grids, weights, reps=2):
+    """Plain per-mesh narrow-band bakes (one device call per mesh, soup resident): meshes/s of a request
+    mix drawn with `weights`, from each mesh's best device-call time."""
+    from italy_bake import Job
+
+    times = []
+    for m, g in zip(meshes, grids):
+        nb = (g.nx // 8) * (g.ny // 8) * (g.nz // 8)
+        d_xyz, d_idx, d_pool = ctx.alloc(m.xyz.nbytes), ctx.alloc(4 * nb), ctx.alloc(2048 * nb)
+        ctx.upload(d_xyz, m.xyz)
+        best = np.inf
+        for _ in range(reps + 1):
+            job = Job(d_xyz, m.ntri, g, d_pool, type(g)(), None)
+            job.band_voxels, job.d_brick_index, job.pool_bricks = 4, d_idx, nb
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.bake_batch_device([job])
+            best = min(best, time.perf_counter() - t0)
+        times.append(best)
+        for p in (d_xyz, d_idx, d_pool):
+            ctx.free(p)
+    w = np.asarray(weights, np.float64) / np.sum(weights)
+    return 1.0 / float(np.dot(w, times)), times
+
+
+def served_section(args, batched, ctx):
+    """The bake server (3 worker streams, HBM budget 200 GB) under two readings of the served configuration.
+
+    batched_distribution: requests drawn from the batched workload's distribution (dense SDF + occupancy,
+      500..200k triangles, 32..256^3), keyed by mesh so that each LBVH stays resident after its first
+      request; batched_distribution_rebuilt: the same requests without keys (every request uploaded and
+      built).  The plain per-mesh bake of the batched workload is the comparison point.
     stress_mix: 95% precision-stress unions (384^3 narrow band) + 5% large jobs (2M-triangle displaced
-      icosphere, 512^3 narrow band), the other reading of the served configuration; far beyond one
-      GPU's rate at 200 req/s, so its latency is queueing."""
+      icosphere, 512^3 narrow band), compared with the plain per-mesh band bake of the same mix.
+    Each reading reports the open-loop Poisson run at the configured rate and the server's capacity: the
+    closed-loop saturation throughput and an open-loop sweep below it (serve.capacity)."""
     from italy_bake import bake, meshgen
-    from italy_bake.serve import Server, open_loop
+    from italy_bake.serve import Server, capacity, open_loop
 
     out = {}
     budget = 200 << 30
@@ -90,6 +132,9 @@ def served_section(args, batched):
             with Server(hbm_budget_bytes=budget, workers=3, max_batch=256) as srv:
                 open_loop(srv, [q[:4] for q in reqs[:8]], rate_hz=50, duration_s=0.5, seed=7)  # warm the contexts
                 r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_seconds, seed=1)
+                if args.serve_capacity:
+                    r["capacity"] = capacity(srv, reqs, n=1536, concurrency=256, rates=(0.5, 0.8, 0.95),
+                                             duration_s=2.0, seed=3)
                 c = srv.counters()
             r.pop("_latencies")
             r["workload"] = ("requests drawn from 128 meshes of the batched distribution (dense SDF + occupancy at "
@@ -98,6 +143,8 @@ def served_section(args, batched):
             r["mesh_builds"], r["mesh_hits"] = c["mesh_builds"], c["mesh_hits"]
             if batched:
                 r["plain_per_mesh_meshes_per_s"] = batched["plain_meshes_per_s"]
+                if "capacity" in r:
+                    r["capacity_vs_plain"] = r["capacity"]["capacity_meshes_per_s"] / batched["plain_meshes_per_s"]
             out[name] = r
     if args.serve_stress_seconds > 0:
         stress = [meshgen.stress(meshgen.BASE_SEED + 100 + i, 64, 0.02) for i in range(6)]
@@ -109,12 +156,21 @@ def served_section(args, batched):
         reqs = [(m.xyz, g384, 4, None, 1 + i) for i, m in enumerate(stress)] * 19 + [(big.xyz, g512, 4, None, 99)] * 6
         with Server(hbm_budget_bytes=budget, workers=3, max_batch=64) as srv:
             r = open_loop(srv, reqs, rate_hz=args.serve_rate, duration_s=args.serve_stress_seconds, seed=2)
+            if args.serve_capacity:
+                r["capacity"] = capacity(srv, reqs, n=160, concurrency=24, rates=(0.5, 0.8, 0.95), duration_s=3.0,
+                                         seed=4)
         lat = r.pop("_latencies")
         big_lat = [l for k, l in lat if reqs[k][1] is g512]
         r["large_jobs"] = len(big_lat)
         r["large_job_latency_s_max"] = max(big_lat) if big_lat else None
         r["workload"] = ("95% precision-stress unions (64 tori/boxes, 2% removed, 384^3 narrow band +-4) + 5% "
                          "2M-triangle displaced icospheres (512^3 narrow band +-4)")
+        plain, times = plain_band_rate(ctx, stress + [big], [g384] * 6 + [g512], [0.95 / 6] * 6 + [0.05])
+        r["plain_per_mesh_meshes_per_s"] = plain
+        r["plain_ms"] = {"precision_stress": 1e3 * float(np.mean(times[:6])), "large_2m_512": 1e3 * times[6]}
+        if "capacity" in r:
+            r["capacity_vs_plain"] = r["capacity"]["capacity_meshes_per_s"] / plain
+            r["offered_vs_capacity"] = args.serve_rate / r["capacity"]["capacity_meshes_per_s"]
         out["stress_mix"] = r
     return out or None
 
@@ -431,7 +487,7 @@ def batched_section(ctx, n, cpu=True):
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import ib_oracle
 
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_cores()
         t_cpu = time.perf_counter()
         est_s, sampled = 0.0, 0
         for i in range(0, n, 32):
@@ -524,7 +580,7 @@ def main():
     for _ in range(args.steps):
         step()
         st = ctx.stats()  # event timings of this bake (waits for its last event)
-        sdf_ms.append(st["ms_sdf"])
+        sdf_ms.append(st["ms_query"])  # HIP events around the query kernel's launch, on its stream
         total_ms.append(st["ms_total"])
     ctx.synchronize()
     t1 = time.perf_counter()
@@ -535,12 +591,13 @@ def main():
     counters = ctx.query_counters(d_xyz, mesh.ntri, sdf) if hasattr(ctx, "query_counters") else None
     sdf_avg_ms = float(np.mean(sdf_ms))
     roofline = {"bound": "valu", "achieved": None, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": None,
-                "traffic": None, "kernel": "k_sdf", "kernel_ms": sdf_avg_ms}
+                "traffic": None, "kernel": "k_sdf_wave", "kernel_ms": sdf_avg_ms}
     if counters:
         flops = counters["box_tests"] * FLOP_BOX + counters["tri_tests"] * FLOP_TRI
         achieved = flops / (sdf_avg_ms * 1e-3) / 1e12
         roofline.update(achieved=achieved, frac=achieved / PEAK_FP32_TFLOPS, flops_per_launch=flops,
-                        box_tests=counters["box_tests"], tri_tests=counters["tri_tests"])
+                        box_tests=counters["box_tests"], tri_tests=counters["tri_tests"],
+                        frac_of_non_packed_peak=achieved / PEAK_FP32_NONPACKED_TFLOPS)
     alg_bytes = 4 * nvox + mesh.xyz.nbytes
     roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
     # HBM-side bytes per launch, from the committed FETCH_SIZE / WRITE_SIZE profile of this kernel
@@ -567,14 +624,14 @@ def main():
     voxelize = voxelize_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_voxelize else None
     ingest = ingest_section(ctx, mesh, d_xyz, cpu=not args.no_cpu_baseline) if not args.no_ingest else None
     material = material_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_material else None
-    served = served_section(args, batched)
+    served = served_section(args, batched, ctx)
 
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import ib_oracle
 
-        cores = min(16, os.cpu_count() or 1)
+        cores = host_cores()
         _, cst = ib_oracle.bake_dense(mesh.xyz, list(sdf.origin), sdf.h, sdf.nx, sdf.ny, sdf.nz, 0, sdf.nz,
                                       threads=cores, zstep=args.cpu_zstep)
         cpu = {"value": cst["voxels"] / cst["seconds"], "unit": "voxel-queries/s", "cores": cst["threads"],
@@ -603,6 +660,10 @@ def main():
         "ingest_gate": ingest, "material_splat": material,
         "p99_bake_latency_s": served["batched_distribution"]["latency_s"]["p99"]
         if served and served.get("batched_distribution") else None,
+        "p99_bake_latency_s_stress_mix": served["stress_mix"]["latency_s"]["p99"]
+        if served and served.get("stress_mix") else None,
+        "host": {"cpus_usable": host_cores(), "cpus_affinity": len(os.sched_getaffinity(0)),
+                 "cpu_count": os.cpu_count()},
     }
     print(json.dumps(out))
 

@@ -71,7 +71,8 @@ typedef struct {
   uint64_t tri_tests;    /* point/triangle distance tests (counting mode) */
   uint64_t stack_overflows; /* traversal pushes that found the stack full: 0 unless the tree is malformed */
   double ms_caps;        /* host time of the hole caps of open meshes (boundary read-back, chains, cap tree) */
-  uint64_t reserved[2];
+  double ms_query;       /* the closest-triangle query kernel alone (dense bakes; HIP events around its launch) */
+  uint64_t reserved[1];
 } ib_stats;
 
 /* Options for ib_set_option. */

@@ -48,7 +48,7 @@ struct Buf {
 
 }  // namespace
 
-enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_OCC_SIDE, EV_COUNT };
+enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_OCC_SIDE, EV_Q0, EV_Q1, EV_COUNT };
 
 struct ib_context {
   int device = 0;
@@ -702,7 +702,9 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
     IB_HIP(sort_pairs_u32(temp, temp_bytes, keys_in, keys, ids_in, ids, nt, 32, s));
     order = reinterpret_cast<const uint32_t *>(ids);
   }
+  c->record(EV_Q0, s);
   launch_sdf(sa, c->count_work, c->query_variant, m.slivers, c->brick_shape, order, s);
+  c->record(EV_Q1, s);
   launch_resolve(sa, s);
   c->record(EV_SDF, s);
   return IB_OK;
@@ -1593,6 +1595,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   st.ms_occupancy = c->ev_valid[EV_OCC_SIDE] ? ev_ms(c, EV_START, EV_OCC_SIDE) : ev_ms(c, EV_BVH, EV_OCC);
   st.ms_columns = ev_ms(c, EV_OCC, EV_COLS);
   st.ms_sdf = ev_ms(c, EV_COLS, EV_SDF);
+  st.ms_query = ev_ms(c, EV_Q0, EV_Q1);
   st.ms_total = c->ev_valid[EV_SDF]   ? ev_ms(c, EV_START, EV_SDF)
                 : c->ev_valid[EV_OCC] ? ev_ms(c, EV_START, EV_OCC)
                                       : ev_ms(c, EV_START, EV_INGEST);

@@ -183,3 +183,64 @@ def open_loop(server, requests, rate_hz, duration_s, seed=0):
     }
     out["_latencies"] = list(zip(kinds, lat))
     return out
+
+
+def closed_loop(server, requests, n, concurrency, seed=0):
+    """Saturation throughput: keep `concurrency` requests (drawn uniformly at random from `requests`, as in
+    open_loop) in flight, submitting the next one as each finishes, until n have finished.  Returns
+    meshes/s over the run and the latency percentiles (submit -> result resident on the device)."""
+    rng = np.random.default_rng(seed)
+    draws = rng.integers(len(requests), size=n)
+    outstanding, lat = [], []
+    failed = submitted = 0
+
+    def submit():
+        nonlocal submitted
+        xyz, g, band, og = requests[draws[submitted]][:4]
+        key = requests[draws[submitted]][4] if len(requests[draws[submitted]]) > 4 else 0
+        outstanding.append(server.submit(xyz, g, band, og, key))
+        submitted += 1
+
+    t0 = time.perf_counter()
+    while submitted < min(concurrency, n):
+        submit()
+    while outstanding:
+        keep = []
+        for tk in outstanding:
+            i = server.info(tk)
+            if i["state"] < 2:
+                keep.append(tk)
+                continue
+            if i["state"] == 2:
+                lat.append(i["t_done"] - i["t_submit"])
+            else:
+                failed += 1
+            server.release(tk)
+        outstanding[:] = keep
+        while submitted < n and len(outstanding) < concurrency:
+            submit()
+        if len(outstanding) == len(keep) and outstanding:
+            time.sleep(0.0002)
+    t1 = time.perf_counter()
+    lat_a = np.array(lat) if lat else np.zeros(1)
+    return {"requests": n, "concurrency": concurrency, "failed": failed, "wall_s": t1 - t0,
+            "meshes_per_s": (n - failed) / (t1 - t0),
+            "latency_s": {"p50": float(np.percentile(lat_a, 50)), "p95": float(np.percentile(lat_a, 95)),
+                          "p99": float(np.percentile(lat_a, 99)), "max": float(lat_a.max())}}
+
+
+def capacity(server, requests, n, concurrency, rates, duration_s, seed=0):
+    """Served capacity: the closed-loop saturation throughput, then open-loop Poisson runs at the given
+    fractions of it.  A rate is sustained when the run drains within 10% of its arrival window (the queue
+    does not grow); `sustained_meshes_per_s` is the highest sustained rate of the sweep."""
+    sat = closed_loop(server, requests, n, concurrency, seed)
+    sweep = []
+    for f in rates:
+        r = open_loop(server, requests, rate_hz=f * sat["meshes_per_s"], duration_s=duration_s, seed=seed + 1)
+        r.pop("_latencies", None)
+        r["fraction_of_saturation"] = f
+        r["sustained"] = r["failed"] == 0 and r["wall_s"] <= 1.1 * duration_s
+        sweep.append(r)
+    ok = [r["rate_hz"] for r in sweep if r["sustained"]]
+    return {"saturation": sat, "sweep": sweep, "capacity_meshes_per_s": sat["meshes_per_s"],
+            "sustained_meshes_per_s": max(ok) if ok else None}
