@@ -401,43 +401,76 @@ def voxelize_section(ctx, mesh, res=256, cpu=True, steps=3):
     return out
 
 
-def band_section(ctx, steps=3):
-    """The precision-stress union (64 tori/boxes, 2% of triangles removed) as a 384^3 narrow band."""
+def band_section(ctx, steps=3, cpu=True):
+    """The precision-stress union (64 tori/boxes, 2% of triangles removed) as a 384^3 narrow band, at the
+    configuration's explicit parameters (513,803 triangles) and at its stated ~1M triangles (64x64-segment tori,
+    46x46-quad box faces).  cpu_baseline: the oracle's band bake (same distance, FP64 winding number by the
+    closure identity) on every 8th band brick, on all usable host cores."""
     import ctypes
-    from italy_bake import Grid, bake, meshgen
+    from italy_bake import bake, meshgen
+
+    def one(mesh, cfg, label):
+        g = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["res"])
+        nbrick = (g.nx // 8) * (g.ny // 8) * (g.nz // 8)
+        d_xyz = ctx.alloc(mesh.xyz.nbytes)
+        ctx.upload(d_xyz, mesh.xyz)
+        d_idx = ctx.alloc(4 * nbrick)
+        d_pool = ctx.alloc(4 * 512 * nbrick)
+        n = ctypes.c_uint64()
+
+        def run():
+            bake.check(ctx.lib.ib_bake_band_device(ctx.handle, ctypes.c_void_p(d_xyz), mesh.ntri, ctypes.byref(g),
+                                                   cfg["band"], ctypes.c_void_p(d_idx), ctypes.c_void_p(d_pool),
+                                                   nbrick, ctypes.byref(n), None))
+
+        run()
+        ctx.synchronize()
+        times = []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            run()
+            times.append(time.perf_counter() - t0)
+        st = ctx.stats()
+        ctx.set_option(bake.OPT_COUNT_WORK, 1)
+        run()
+        cst = ctx.stats()
+        ctx.set_option(bake.OPT_COUNT_WORK, 0)
+        for p in (d_xyz, d_idx, d_pool):
+            ctx.free(p)
+        ms = 1e3 * float(np.median(times))
+        exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI
+        rate = int(n.value) * 512 / (ms * 1e-3)
+        out = {
+            "workload": f"{label} ({mesh.ntri} triangles after 2% removed), {cfg['res']}^3 narrow band "
+                        f"+-{cfg['band']} voxels in 8^3 FP32 bricks over [-1,1]^3",
+            "triangles": mesh.ntri, "bricks": int(n.value), "band_voxels_computed": int(n.value) * 512,
+            "ms_per_bake": ms, "ms_spread": [1e3 * min(times), 1e3 * max(times)], "voxel_queries_per_s": rate,
+            "boundary_edges": st["boundary_edges"], "fp64_rechecked": st["rechecked"], "parity_ties": st["ties"],
+            "ms_hole_caps_host": st["ms_caps"], "ms_query_launch": st["ms_sdf"],
+            "query_executed_work": {"flops": exec_flops, "TFLOP/s": exec_flops / (st["ms_sdf"] * 1e-3) / 1e12,
+                                    "note": "executed FP32 work of the band's persistent query launch (counting "
+                                            "build, per lane), over its event-timed duration, coarse pass excluded"},
+        }
+        if cpu:
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import ib_oracle
+
+            _, _, ost = ib_oracle.bake_band(mesh.xyz, list(g.origin), g.h, g.nx, g.ny, g.nz, cfg["band"], stride=8,
+                                            threads=host_cores())
+            crate = ost["voxels"] / ost["seconds"]
+            out["cpu_baseline"] = {
+                "value": crate, "unit": "band voxel-queries/s", "cores": ost["threads"], "kind": "port",
+                "gpu_multiple": rate / crate,
+                "sample": f"every 8th of the {int(n.value)} band bricks ({ost['voxels']} voxels: FP32 closest triangle, "
+                          f"FP64 winding number by the closure identity over {ost['cap_triangles']} cap triangles; "
+                          f"brick selection and LBVH build included), {ost['seconds']:.2f} s"}
+        return out
 
     mesh, cfg = meshgen.workload_stress()
-    g = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["res"])
-    nbrick = (g.nx // 8) * (g.ny // 8) * (g.nz // 8)
-    d_xyz = ctx.alloc(mesh.xyz.nbytes)
-    ctx.upload(d_xyz, mesh.xyz)
-    d_idx = ctx.alloc(4 * nbrick)
-    d_pool = ctx.alloc(4 * 512 * nbrick)
-    n = ctypes.c_uint64()
-
-    def run():
-        bake.check(ctx.lib.ib_bake_band_device(ctx.handle, ctypes.c_void_p(d_xyz), mesh.ntri, ctypes.byref(g),
-                                               cfg["band"], ctypes.c_void_p(d_idx), ctypes.c_void_p(d_pool), nbrick,
-                                               ctypes.byref(n), None))
-
-    run()
-    ctx.synchronize()
-    times = []
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        run()
-        times.append(time.perf_counter() - t0)
-    st = ctx.stats()
-    for p in (d_xyz, d_idx, d_pool):
-        ctx.free(p)
-    ms = 1e3 * float(np.median(times))
-    return {
-        "workload": f"64-component stress union ({mesh.ntri} triangles after 2% removed), {cfg['res']}^3 narrow band "
-                    f"+-{cfg['band']} voxels in 8^3 FP32 bricks over [-1,1]^3",
-        "triangles": mesh.ntri, "bricks": int(n.value), "band_voxels_computed": int(n.value) * 512,
-        "ms_per_bake": ms, "voxel_queries_per_s": int(n.value) * 512 / (ms * 1e-3),
-        "boundary_edges": st["boundary_edges"], "fp64_rechecked": st["rechecked"], "parity_ties": st["ties"],
-    }
+    res = one(mesh, cfg, "64-component stress union")
+    mesh1m, cfg1m = meshgen.workload_stress_1m()
+    res["stress_1m"] = one(mesh1m, cfg1m, "64-component stress union at ~1M triangles")
+    return res
 
 
 def batched_section(ctx, n, cpu=True):
@@ -459,23 +492,41 @@ def batched_section(ctx, n, cpu=True):
         ptrs += [d_xyz, d_dist, d_occ]
     voxels = sum(r ** 3 for r in res)
     tris = sum(m.ntri for m in meshes)
-    # plain: one ib_bake_dense_device per mesh, as a caller without the batch API would
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    for j in jobs:
-        ctx.bake_dense_device(j.d_xyz, j.ntri, j.sdf, j.d_distances, j.occ, j.d_occ_words)
-    ctx.synchronize()
-    plain_s = time.perf_counter() - t0
-    # batched: autotune on a sample, then one batched bake of all meshes
+    # plain: one ib_bake_dense_device per mesh, as a caller without the batch API would (3 runs)
+    plain_runs = []
+    for _ in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for j in jobs:
+            ctx.bake_dense_device(j.d_xyz, j.ntri, j.sdf, j.d_distances, j.occ, j.d_occ_words)
+        ctx.synchronize()
+        plain_runs.append(time.perf_counter() - t0)
+    plain_s = float(np.median(plain_runs))
+    # batched: autotune on a sample, then one batched bake of all meshes (3 runs)
     t0 = time.perf_counter()
     shape, blocks, _ = ctx.autotune_batch(jobs, sample_stride=32)
     tune_s = time.perf_counter() - t0
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.bake_batch_device(jobs)
-    ctx.synchronize()
-    batch_s = time.perf_counter() - t0
+    batch_runs, query_ms = [], []
+    for _ in range(3):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.bake_batch_device(jobs)
+        ctx.synchronize()
+        batch_runs.append(time.perf_counter() - t0)
+        query_ms.append(ctx.stats()["ms_sdf"])
+    batch_s = float(np.median(batch_runs))
     st = ctx.stats()
+    # work the persistent query executes (wave level: every lane's tests over its brick's candidates)
+    ctx.set_option(bake.OPT_COUNT_WORK, 1)
+    ctx.bake_batch_device(jobs)
+    cst = ctx.stats()
+    ctx.set_option(bake.OPT_COUNT_WORK, 0)
+    exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI
+    qms = float(np.median(query_ms))
+    executed = {"flops": exec_flops, "TFLOP/s": exec_flops / (qms * 1e-3) / 1e12,
+                "frac_of_157_3": exec_flops / (qms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,
+                "note": "executed FP32 work of the persistent query (counting build, per lane: the brick's union of "
+                        "candidates), over its median event-timed duration, columns and resolve included"}
     for p in ptrs:
         ctx.free(p)
     for opt in (bake.OPT_BRICK_SHAPE, bake.OPT_BLOCKS_PER_CU):  # the autotuner's choice applies to batches only
@@ -515,7 +566,8 @@ def batched_section(ctx, n, cpu=True):
         "batched_s": batch_s, "batched_meshes_per_s": n / batch_s, "batched_voxel_queries_per_s": voxels / batch_s,
         "speedup_vs_plain": plain_s / batch_s, "autotuned": {"brick_shape": shape, "blocks_per_cu": blocks,
                                                               "tune_s": tune_s},
-        "query_ms": st["ms_sdf"], "host_generation_s": gen_s,
+        "plain_runs_s": plain_runs, "batched_runs_s": batch_runs,
+        "query_ms": qms, "query_executed_work": executed, "host_generation_s": gen_s,
     }
 
 
@@ -619,7 +671,7 @@ def main():
     consumer = consumer_section(ctx, sdf, d_dist) if not args.no_consumer else None
 
     batched = batched_section(ctx, args.batch_meshes, cpu=not args.no_cpu_baseline) if args.batch_meshes > 0 else None
-    band = band_section(ctx) if not args.no_band else None
+    band = band_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_band else None
     analytic = analytic_section(ctx, cpu=not args.no_cpu_baseline) if not args.no_analytic else None
     voxelize = voxelize_section(ctx, mesh, cpu=not args.no_cpu_baseline) if not args.no_voxelize else None
     ingest = ingest_section(ctx, mesh, d_xyz, cpu=not args.no_cpu_baseline) if not args.no_ingest else None

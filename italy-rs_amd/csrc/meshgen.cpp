@@ -440,18 +440,23 @@ int ib_gen_decimated(int64_t target, uint64_t seed, ib_soup *out) {
 }
 
 int ib_gen_stress(uint64_t seed, int ncomp, double remove_frac, ib_soup *out) {
+  return ib_gen_stress_res(seed, ncomp, remove_frac, 64, 32, 32, out);
+}
+
+int ib_gen_stress_res(uint64_t seed, int ncomp, double remove_frac, int torus_nu, int torus_nv, int box_n,
+                      ib_soup *out) {
   std::memset(out, 0, sizeof(*out));
-  if (ncomp <= 0) return -1;
+  if (ncomp <= 0 || torus_nu < 3 || torus_nv < 3 || box_n < 1) return -1;
   SplitMix64 r(seed);
   IndexedMesh all;
   for (int c = 0; c < ncomp; ++c) {
     IndexedMesh part;
     if ((c & 1) == 0) {
       const double R = r.uniform(0.05, 0.2), rr = r.uniform(0.01, 0.05);
-      part = torus(R, rr, 64, 32);
+      part = torus(R, rr, torus_nu, torus_nv);
     } else {
       const double s[3] = {r.uniform(0.05, 0.3), r.uniform(0.05, 0.3), r.uniform(0.05, 0.3)};
-      part = subdivided_box(s, 32);
+      part = subdivided_box(s, box_n);
     }
     double Rm[9];
     random_rotation(r, Rm);

@@ -30,6 +30,10 @@ int ib_gen_displaced(int level, uint64_t seed, ib_soup *out);
 int ib_gen_decimated(int64_t target_tris, uint64_t seed, ib_soup *out);
 /* ncomp components (even index torus, odd index box), remove_frac of all triangles removed */
 int ib_gen_stress(uint64_t seed, int ncomp, double remove_frac, ib_soup *out);
+/* same with the tori's segment counts (nu around the ring, nv around the tube) and the boxes' quads per face edge
+ * given (ib_gen_stress uses 64, 32 and 32) */
+int ib_gen_stress_res(uint64_t seed, int ncomp, double remove_frac, int torus_nu, int torus_nv, int box_n,
+                      ib_soup *out);
 /* n decimated meshes generated on `threads` host threads (<= 0: all cores) */
 int ib_gen_decimated_batch(int n, const int64_t *targets, const uint64_t *seeds, int threads, ib_soup *outs);
 void ib_soup_free(ib_soup *s);

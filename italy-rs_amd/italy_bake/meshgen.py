@@ -48,6 +48,8 @@ def _load():
         lib.ib_gen_displaced.argtypes = [ctypes.c_int, ctypes.c_uint64, S]
         lib.ib_gen_decimated.argtypes = [ctypes.c_int64, ctypes.c_uint64, S]
         lib.ib_gen_stress.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, S]
+        lib.ib_gen_stress_res.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, S]
         lib.ib_soup_free.argtypes = [S]
         lib.ib_gen_decimated_batch.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
                                                ctypes.POINTER(ctypes.c_uint64), ctypes.c_int, S]
@@ -89,8 +91,9 @@ def decimated(target, seed):
     return _take(_load().ib_gen_decimated, int(target), int(seed))
 
 
-def stress(seed=0x17A1E5, ncomp=64, remove_frac=0.02):
-    return _take(_load().ib_gen_stress, int(seed), int(ncomp), float(remove_frac))
+def stress(seed=0x17A1E5, ncomp=64, remove_frac=0.02, torus_segments=(64, 32), box_quads=32):
+    return _take(_load().ib_gen_stress_res, int(seed), int(ncomp), float(remove_frac), int(torus_segments[0]),
+                 int(torus_segments[1]), int(box_quads))
 
 
 # ---- the workloads -----------------------------------------------------------------------
@@ -146,4 +149,12 @@ def workload_stress(seed=BASE_SEED):
     with random rotations and positions in [-0.7, 0.7]^3, overlaps allowed, 2% of triangles removed
     at random; baked as a 384^3 narrow band of +-4 voxels in 8^3 bricks over [-1, 1]^3."""
     m = stress(seed, 64, 0.02)
+    return m, dict(lo=-1.0, hi=1.0, res=384, band=4)
+
+
+def workload_stress_1m(seed=BASE_SEED):
+    """The precision-stress union at the configuration's stated size (~1M triangles): 32 tori of 64x64
+    segments and 32 boxes of 46x46 quads per face (1,074,688 triangles, 1,053,195 after 2% are removed),
+    otherwise as workload_stress."""
+    m = stress(seed, 64, 0.02, torus_segments=(64, 64), box_quads=46)
     return m, dict(lo=-1.0, hi=1.0, res=384, band=4)

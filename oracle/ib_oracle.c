@@ -1947,3 +1947,460 @@ void orc_triangle_materials(size_t ntri, const float *uvs, size_t nuv, const uin
     out[5 * t + 4] = mat->roughness;
   }
 }
+
+/* ============================================================================
+ * Narrow-band bake of any mesh (the precision-stress configuration's CPU baseline and checker).
+ * Distance: the FP32 closest triangle through the LBVH (as orc_bake_dense).  Sign: the winding
+ * number w = (signed +z crossings above q of the closed surface S + reversed cap) + w_cap(q), with
+ * one cap for all holes: the cone from the mean of the boundary vertices P over every unmatched
+ * directed edge u->v (triangle (P, u, v)), whose boundary is the mesh's (the spokes cancel along each
+ * closed boundary loop).  w_cap in FP64: cap triangles near q summed exactly, farther LBVH nodes only
+ * bounded (|Omega| <= area / dmin^2); when the bound leaves |w| within reach of 1/2 the whole cap is
+ * summed; when q lies on the cap (where the identity does not hold) w is the mesh's own FP64 winding
+ * number; |w| within 1e-6 of 1/2 is a tie, decided by orc_parity_inside.
+ * ========================================================================== */
+typedef struct {
+  size_t n;          /* cap triangles */
+  float *sorted;     /* 9 floats each, in the cap LBVH's leaf order */
+  orc_bvh b;
+  double *area;      /* per LBVH node (internal, then leaves): sum of triangle areas below */
+} capset;
+
+/* Unmatched directed edges (with multiplicity) after the exact-position weld, as (from, to) vertex ids
+ * (2 per edge in *edges) and the position of each id (3 floats per id in *pos); returns the edge count. */
+static size_t boundary_edges_ids(const float *xyz, size_t ntri, int64_t **edges, float **pos) {
+  const size_t nv = 3 * ntri;
+  size_t cap = 16;
+  while (cap < nv * 2) cap <<= 1;
+  uint32_t *kx = (uint32_t *)malloc(cap * 4), *ky = (uint32_t *)malloc(cap * 4), *kz = (uint32_t *)malloc(cap * 4);
+  int64_t *kid = (int64_t *)malloc(cap * 8);
+  for (size_t i = 0; i < cap; ++i) kid[i] = -1;
+  int64_t *vid = (int64_t *)malloc((nv ? nv : 1) * 8);
+  float *vp = (float *)malloc(sizeof(float) * 3 * (nv ? nv : 1));
+  int64_t nextid = 0;
+  for (size_t i = 0; i < nv; ++i) {
+    const uint32_t a = canon_bits(xyz[3 * i]), b = canon_bits(xyz[3 * i + 1]), c = canon_bits(xyz[3 * i + 2]);
+    size_t s = mix64(((uint64_t)a << 32 | b) ^ mix64(c)) & (cap - 1);
+    for (;;) {
+      if (kid[s] == -1) {
+        kx[s] = a;
+        ky[s] = b;
+        kz[s] = c;
+        memcpy(vp + 3 * nextid, xyz + 3 * i, 3 * sizeof(float));
+        kid[s] = nextid++;
+        break;
+      }
+      if (kx[s] == a && ky[s] == b && kz[s] == c) break;
+      s = (s + 1) & (cap - 1);
+    }
+    vid[i] = kid[s];
+  }
+  free(kx);
+  free(ky);
+  free(kz);
+  free(kid);
+  i64map m;
+  map_init(&m, nv);
+  size_t count = 0;
+  for (size_t t = 0; t < ntri; ++t)
+    for (int k = 0; k < 3; ++k) {
+      const int64_t a = vid[3 * t + k], b = vid[3 * t + (k + 1) % 3];
+      if (a == b) continue;
+      const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+      int ins;
+      int64_t *v = map_get(&m, (lo << 31) ^ hi, &ins, &count);
+      if (ins) *v = 0;
+      *v += (a < b) ? 1 : -1;
+    }
+  size_t ne = 0;
+  for (size_t i = 0; i < m.cap; ++i)
+    if (m.keys[i] != -1) ne += (size_t)(m.vals[i] < 0 ? -m.vals[i] : m.vals[i]);
+  int64_t *e = (int64_t *)malloc(sizeof(int64_t) * 2 * (ne ? ne : 1));
+  size_t k = 0;
+  for (size_t i = 0; i < m.cap; ++i) {
+    if (m.keys[i] == -1 || m.vals[i] == 0) continue;
+    const int64_t key = m.keys[i], lo = key >> 31, hi = key & ((1ll << 31) - 1);
+    const int64_t from = m.vals[i] > 0 ? lo : hi, to = m.vals[i] > 0 ? hi : lo;
+    const int64_t mult = m.vals[i] < 0 ? -m.vals[i] : m.vals[i];
+    for (int64_t r = 0; r < mult; ++r, ++k) {
+      e[2 * k] = from;
+      e[2 * k + 1] = to;
+    }
+  }
+  map_free(&m);
+  free(vid);
+  *edges = e;
+  *pos = vp;
+  return ne;
+}
+
+static int cmp_edge(const void *pa, const void *pb) {
+  const int64_t *a = (const int64_t *)pa, *b = (const int64_t *)pb;
+  return a[0] != b[0] ? (a[0] < b[0] ? -1 : 1) : (a[1] < b[1] ? -1 : (a[1] > b[1] ? 1 : 0));
+}
+
+static double tri_area_f64(const float *t) {
+  const double e1[3] = {(double)t[3] - t[0], (double)t[4] - t[1], (double)t[5] - t[2]};
+  const double e2[3] = {(double)t[6] - t[0], (double)t[7] - t[1], (double)t[8] - t[2]};
+  const double c[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+  return 0.5 * sqrt(dot3d(c, c));
+}
+
+/* The cap: the unmatched edges decompose into closed loops (every vertex has as many unmatched edges
+ * leaving as entering it); each loop, walked from an unused edge by any unused edge leaving the current
+ * vertex until it returns to its start, is closed by a fan from the mean of its vertices: triangles
+ * (P, u, v) for its edges u -> v, whose boundary is the loop. */
+static int capset_build(const float *xyz, size_t ntri, capset *cs) {
+  memset(cs, 0, sizeof(*cs));
+  int64_t *e;
+  float *vp;
+  const size_t ne = boundary_edges_ids(xyz, ntri, &e, &vp);
+  if (ne == 0) {
+    free(e);
+    free(vp);
+    return 0;
+  }
+  qsort(e, ne, 2 * sizeof(int64_t), cmp_edge);
+  char *used = (char *)calloc(ne, 1);
+  size_t *loop = (size_t *)malloc(sizeof(size_t) * ne);
+  float *caps = (float *)malloc(sizeof(float) * 9 * ne);
+  size_t nc = 0;
+  size_t *cursor = (size_t *)malloc(sizeof(size_t) * ne); /* next edge to try in each vertex's run */
+  for (size_t i = 0; i < ne; ++i) cursor[i] = i;
+  for (size_t s0 = 0; s0 < ne; ++s0) {
+    if (used[s0]) continue;
+    size_t len = 0, cur = s0;
+    const int64_t start = e[2 * s0];
+    for (;;) {
+      used[cur] = 1;
+      loop[len++] = cur;
+      const int64_t v = e[2 * cur + 1];
+      if (v == start) break;
+      /* first edge of v's run, then the first unused one in it */
+      size_t lo = 0, hi = ne;
+      while (lo < hi) {
+        const size_t mid = (lo + hi) / 2;
+        if (e[2 * mid] < v) lo = mid + 1;
+        else hi = mid;
+      }
+      size_t nxt = ne;
+      for (size_t q = cursor[lo < ne ? lo : 0]; lo < ne && q < ne && e[2 * q] == v; ++q)
+        if (!used[q]) {
+          nxt = q;
+          cursor[lo] = q + 1;
+          break;
+        }
+      if (nxt == ne) break; /* cannot happen for a boundary of closed loops; the fan closes it anyway */
+      cur = nxt;
+    }
+    double c[3] = {0, 0, 0};
+    for (size_t q = 0; q < len; ++q)
+      for (int k = 0; k < 3; ++k) c[k] += vp[3 * e[2 * loop[q]] + k];
+    const float apex[3] = {(float)(c[0] / len), (float)(c[1] / len), (float)(c[2] / len)};
+    for (size_t q = 0; q < len; ++q, ++nc) {
+      memcpy(caps + 9 * nc, apex, 3 * sizeof(float));
+      memcpy(caps + 9 * nc + 3, vp + 3 * e[2 * loop[q]], 3 * sizeof(float));
+      memcpy(caps + 9 * nc + 6, vp + 3 * e[2 * loop[q] + 1], 3 * sizeof(float));
+    }
+  }
+  free(cursor);
+  free(used);
+  free(loop);
+  free(e);
+  free(vp);
+  if (orc_bvh_build(caps, nc, &cs->b) != 0) {
+    free(caps);
+    return -1;
+  }
+  cs->n = nc;
+  cs->sorted = (float *)malloc(sizeof(float) * 9 * nc);
+  for (size_t i = 0; i < nc; ++i) memcpy(cs->sorted + 9 * i, caps + 9 * (size_t)cs->b.order[i], 9 * sizeof(float));
+  free(caps);
+  const int ni = (int)nc - 1;
+  cs->area = (double *)calloc(2 * nc - 1, sizeof(double));
+  for (size_t i = 0; i < nc; ++i) {
+    const double a = tri_area_f64(cs->sorted + 9 * i);
+    for (int x = ni + (int)i; x >= 0; x = cs->b.parent[x]) cs->area[x] += a;
+  }
+  return 0;
+}
+
+static void capset_free(capset *cs) {
+  if (cs->n) orc_bvh_free(&cs->b);
+  free(cs->sorted);
+  free(cs->area);
+}
+
+/* FP64 solid angle of t at q, and whether q lies on or next to t (as the device's singular test) */
+static double solid_angle_chk(const double q[3], const float *t, int *singular) {
+  const double a[3] = {(double)t[0] - q[0], (double)t[1] - q[1], (double)t[2] - q[2]};
+  const double b[3] = {(double)t[3] - q[0], (double)t[4] - q[1], (double)t[5] - q[2]};
+  const double c[3] = {(double)t[6] - q[0], (double)t[7] - q[1], (double)t[8] - q[2]};
+  const double la = sqrt(dot3d(a, a)), lb = sqrt(dot3d(b, b)), lc = sqrt(dot3d(c, c));
+  const double bc[3] = {b[1] * c[2] - b[2] * c[1], b[2] * c[0] - b[0] * c[2], b[0] * c[1] - b[1] * c[0]};
+  const double det = dot3d(a, bc);
+  const double l3 = la * lb * lc;
+  const double den = l3 + dot3d(a, b) * lc + dot3d(b, c) * la + dot3d(c, a) * lb;
+  if (den <= 1e-9 * l3) {
+    const double nz = (b[0] - a[0]) * (c[1] - a[1]) - (b[1] - a[1]) * (c[0] - a[0]);
+    const double zeps = 4.8e-7 * (fabs(q[2]) + fabs((double)t[2]) + fabs((double)t[5]) + fabs((double)t[8]));
+    if (fabs(det) <= 1e-9 * l3 || fabs(det) <= zeps * fabs(nz)) *singular = 1;
+  }
+  return 2.0 * atan2(det, den);
+}
+
+static inline void neumaier(double *sum, double *comp, double v) {
+  const double s = *sum + v;
+  *comp += (fabs(*sum) >= fabs(v)) ? (*sum - s) + v : (v - s) + *sum;
+  *sum = s;
+}
+
+/* cap winding at q: exact near q, bounded far away (*bound, winding units); *singular if q is on the cap */
+static double cap_winding_bounded_f64(const capset *cs, const double q[3], double *bound, int *singular) {
+  const orc_bvh *b = &cs->b;
+  const int ni = (int)cs->n - 1;
+  double sum = 0.0, comp = 0.0, ub = 0.0;
+  int stack[512];
+  int sp = 0;
+  stack[sp++] = ni > 0 ? 0 : ~0;
+  while (sp) {
+    const int x = stack[--sp];
+    const int id = x >= 0 ? x : ni + ~x;
+    const float *lo = b->lo + 3 * id, *hi = b->hi + 3 * id;
+    double d2 = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double g = fmax(fmax((double)lo[k] - q[k], q[k] - (double)hi[k]), 0.0);
+      d2 += g * g;
+    }
+    if (d2 > 0.0) {
+      const double bnd = cs->area[id] / (4.0 * M_PI * d2);
+      if (bnd <= 1e-3) {
+        ub += bnd;
+        continue;
+      }
+    }
+    if (x < 0) {
+      neumaier(&sum, &comp, solid_angle_chk(q, cs->sorted + 9 * (size_t)(~x), singular));
+    } else if (sp + 2 <= 512) {
+      stack[sp++] = b->left[x];
+      stack[sp++] = b->right[x];
+    } else {
+      ub += 1.0;
+    }
+  }
+  *bound = ub * (1.0 + 1e-9);
+  return (sum + comp) / (4.0 * M_PI);
+}
+
+static double cap_winding_full_f64(const capset *cs, const double q[3], int *singular) {
+  double sum = 0.0, comp = 0.0;
+  for (size_t i = 0; i < cs->n; ++i) neumaier(&sum, &comp, solid_angle_chk(q, cs->sorted + 9 * i, singular));
+  return (sum + comp) / (4.0 * M_PI);
+}
+
+/* +z crossings of the reversed cap on the column through (x, y): appended to *out with negated signs */
+static int cap_column_crossings(const capset *cs, float x, float y, crossing **out, int *capn, int cnt) {
+  const orc_bvh *b = &cs->b;
+  const int ni = (int)cs->n - 1;
+  const float o[3] = {x, y, 0.0f};
+  const float d[3] = {0.0f, 0.0f, 1.0f};
+  ray_shear r;
+  make_shear(d, &r);
+  int stack[512];
+  int sp = 0;
+  stack[sp++] = ni > 0 ? 0 : ~0;
+  while (sp) {
+    const int x0 = stack[--sp];
+    const int id = x0 >= 0 ? x0 : ni + ~x0;
+    const float *lo = b->lo + 3 * id, *hi = b->hi + 3 * id;
+    if (!(x >= lo[0] && x <= hi[0] && y >= lo[1] && y <= hi[1])) continue;
+    if (x0 >= 0) {
+      stack[sp++] = b->left[x0];
+      stack[sp++] = b->right[x0];
+      continue;
+    }
+    double th;
+    const int s = ray_tri(o, &r, cs->sorted + 9 * (size_t)(~x0), &th);
+    if (s == 0) continue;
+    if (cnt == *capn) {
+      *capn = *capn ? *capn * 2 : 16;
+      *out = (crossing *)realloc(*out, sizeof(crossing) * (*capn));
+    }
+    (*out)[cnt].z = (float)th;
+    (*out)[cnt].s = s; /* reversed: -(-s) */
+    cnt++;
+  }
+  return cnt;
+}
+
+typedef struct {
+  const qctx *c;
+  const capset *cs;
+  const float *xyz;
+  size_t ntri;
+  const float *origin;
+  float h;
+  int nbx, nby;
+  const uint32_t *bricks; /* linear brick index per task */
+  const uint32_t *slots;  /* pool slot per task */
+  size_t ntask;
+  float *pool;
+  size_t next;
+  pthread_mutex_t *mu;
+  uint64_t rechecked, full, ties, singular;
+} band_job;
+
+static void *band_fn(void *arg) {
+  band_job *j = (band_job *)arg;
+  crossing *cr = NULL;
+  int capn = 0;
+  uint64_t nodes = 0, tests = 0, rechecked = 0, full = 0, ties = 0, singular = 0;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t t = j->next++;
+    pthread_mutex_unlock(j->mu);
+    if (t >= j->ntask) break;
+    const uint32_t bl = j->bricks[t];
+    const int bx = (int)(bl % (uint32_t)j->nbx), by = (int)((bl / (uint32_t)j->nbx) % (uint32_t)j->nby),
+              bz = (int)(bl / ((uint32_t)j->nbx * (uint32_t)j->nby));
+    float *dst = j->pool + 512 * (size_t)j->slots[t];
+    for (int ly = 0; ly < 8; ++ly)
+      for (int lx = 0; lx < 8; ++lx) {
+        const int x = bx * 8 + lx, y = by * 8 + ly;
+        const float px = j->origin[0] + ((float)x + 0.5f) * j->h;
+        const float py = j->origin[1] + ((float)y + 0.5f) * j->h;
+        int nc = column_crossings(j->c, px, py, &cr, &capn, &tests);
+        if (j->cs->n) nc = cap_column_crossings(j->cs, px, py, &cr, &capn, nc);
+        for (int lz = 0; lz < 8; ++lz) {
+          const int z = bz * 8 + lz;
+          const float q[3] = {px, py, j->origin[2] + ((float)z + 0.5f) * j->h};
+          const float d = sqrtf(closest2(j->c, q, &nodes, &tests));
+          int wi = 0;
+          for (int k = 0; k < nc; ++k)
+            if (cr[k].z > q[2]) wi += cr[k].s;
+          int inside;
+          if (!j->cs->n) {
+            inside = wi != 0;
+          } else {
+            const double qd[3] = {q[0], q[1], q[2]};
+            double bound;
+            int sing = 0;
+            double w = wi + cap_winding_bounded_f64(j->cs, qd, &bound, &sing);
+            if (sing || fabs(fabs(w) - 0.5) < bound + 1e-9) {
+              ++rechecked;
+              if (!sing) {
+                ++full;
+                w = wi + cap_winding_full_f64(j->cs, qd, &sing);
+              }
+              if (sing) {
+                ++singular;
+                w = orc_gwn_f64(qd, j->xyz, j->ntri);
+              }
+            }
+            if (fabs(fabs(w) - 0.5) < 1e-6) {
+              ++ties;
+              inside = orc_parity_inside(q, j->xyz, j->ntri);
+            } else {
+              inside = fabs(w) > 0.5;
+            }
+          }
+          dst[(lz * 8 + ly) * 8 + lx] = inside ? -d : d;
+        }
+      }
+  }
+  free(cr);
+  pthread_mutex_lock(j->mu);
+  j->rechecked += rechecked;
+  j->full += full;
+  j->ties += ties;
+  j->singular += singular;
+  pthread_mutex_unlock(j->mu);
+  return NULL;
+}
+
+int orc_bake_band(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz, int band,
+                  int brick_stride, int threads, int32_t *brick_index, float *pool, size_t pool_bricks,
+                  size_t *nbricks, orc_band_stats *st) {
+  if (ntri == 0 || (nx | ny | nz) & 7 || band <= 0) return -2;
+  if (brick_stride < 1) brick_stride = 1;
+  const double t0 = now_s();
+  orc_bvh b;
+  if (orc_bvh_build(xyz, ntri, &b) != 0) return -2;
+  float *sorted = (float *)malloc(sizeof(float) * 9 * ntri);
+  for (size_t i = 0; i < ntri; ++i) memcpy(sorted + 9 * i, xyz + 9 * (size_t)b.order[i], 9 * sizeof(float));
+  orc_trirec *recs = make_recs(sorted, ntri);
+  qctx c = {sorted, &b, recs, max_abs_coord(sorted, ntri)};
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  /* bricks: coarse unsigned distance at the brick centres (cell centres of spacing 8h) against the
+   * band threshold h * (band + 3.5 sqrt 3), as the bake computes them */
+  const int nbx = nx / 8, nby = ny / 8, nbz = nz / 8;
+  const size_t nb = (size_t)nbx * nby * nbz;
+  float *cpts = (float *)malloc(sizeof(float) * 3 * nb);
+  const float h8 = h * 8.0f;
+  for (size_t i = 0; i < nb; ++i) {
+    const int x = (int)(i % nbx), y = (int)((i / nbx) % nby), z = (int)(i / ((size_t)nbx * nby));
+    cpts[3 * i] = origin[0] + ((float)x + 0.5f) * h8;
+    cpts[3 * i + 1] = origin[1] + ((float)y + 0.5f) * h8;
+    cpts[3 * i + 2] = origin[2] + ((float)z + 0.5f) * h8;
+  }
+  float *cd = (float *)malloc(sizeof(float) * nb);
+  {
+    pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    udf_job uj = {&c, cpts, nb, cd, 0, &mu};
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, udf_fn, &uj);
+    for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+    free(tid);
+  }
+  const float thresh = h * ((float)band + 6.0621778f);
+  size_t kept = 0;
+  for (size_t i = 0; i < nb; ++i) {
+    const int in = cd[i] <= thresh;
+    brick_index[i] = in ? (int32_t)kept : -1;
+    kept += (size_t)in;
+  }
+  *nbricks = kept;
+  free(cpts);
+  free(cd);
+  int rc = 0;
+  if (kept > pool_bricks) {
+    rc = -1;
+  } else {
+    uint32_t *bricks = (uint32_t *)malloc(sizeof(uint32_t) * (kept ? kept : 1));
+    uint32_t *slots = (uint32_t *)malloc(sizeof(uint32_t) * (kept ? kept : 1));
+    size_t ntask = 0;
+    for (size_t i = 0; i < nb; ++i)
+      if (brick_index[i] >= 0 && brick_index[i] % brick_stride == 0) {
+        bricks[ntask] = (uint32_t)i;
+        slots[ntask++] = (uint32_t)brick_index[i];
+      }
+    capset cs;
+    capset_build(xyz, ntri, &cs);
+    pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+    band_job j = {&c, &cs, xyz, ntri, origin, h, nbx, nby, bricks, slots, ntask, pool, 0, &mu, 0, 0, 0, 0};
+    pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+    for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, band_fn, &j);
+    for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+    free(tid);
+    if (st) {
+      st->voxels = 512 * (uint64_t)ntask;
+      st->bricks_baked = ntask;
+      st->cap_triangles = cs.n;
+      st->rechecked = j.rechecked;
+      st->full_cap_sums = j.full;
+      st->on_cap = j.singular;
+      st->ties = j.ties;
+    }
+    capset_free(&cs);
+    free(bricks);
+    free(slots);
+  }
+  if (st) {
+    st->seconds = now_s() - t0;
+    st->threads = threads;
+  }
+  free(sorted);
+  free(recs);
+  orc_bvh_free(&b);
+  return rc;
+}

@@ -144,6 +144,24 @@ int orc_bake_dense(const float *xyz, size_t ntri, const float origin[3], float h
 int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz,
                            int z_begin, int z_end, int zstep, int threads, float *dist, orc_bake_stats *stats);
 
+/* Narrow band of +-band voxels in 8^3 bricks (the bake's band output) for any mesh, open or closed.  Bricks
+ * are kept by the bake's rule (FP32 unsigned distance at the brick centre, on the grid of spacing 8h,
+ * <= h (band + 3.5 sqrt 3)); brick_index (nx/8 * ny/8 * nz/8, x fastest) gets each brick's slot in order of
+ * its linear index, or -1, and *nbricks the count.  Every brick_stride-th slot is baked into pool
+ * (slot * 512 + ((z%8)*8 + y%8)*8 + x%8; the other slots are left as they are): FP32 closest-triangle
+ * distance, sign by the closure identity with a single cone cap over all holes and an FP64 cap winding
+ * number (bounded far field, full sum near 1/2, the mesh's own winding number on the cap, parity on ties).
+ * Returns 0, -1 if pool_bricks < *nbricks (nothing baked), -2 for an empty mesh or grid dims not
+ * divisible by 8. */
+typedef struct {
+  uint64_t voxels, bricks_baked, cap_triangles, rechecked, full_cap_sums, on_cap, ties;
+  double seconds;
+  int threads;
+} orc_band_stats;
+int orc_bake_band(const float *xyz, size_t ntri, const float origin[3], float h, int nx, int ny, int nz, int band,
+                  int brick_stride, int threads, int32_t *brick_index, float *pool, size_t pool_bricks,
+                  size_t *nbricks, orc_band_stats *st);
+
 /* FP64 brute-force signed distance at arbitrary points (inside where |GWN| > 0.5, FP64) */
 void orc_sdf_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_dist,
                         double *out_w);

@@ -85,6 +85,9 @@ def _load():
     lib.orc_signed_points_gwn.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p,
                                           ctypes.c_void_p]
     lib.orc_signed_points_closed.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
+    lib.orc_bake_band.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_float, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _f32p,
+                                  ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(BandStats)]
     lib.orc_bvh_build.argtypes = [_f32p, ctypes.c_size_t, ctypes.POINTER(Bvh)]
     lib.orc_bvh_free.argtypes = [ctypes.POINTER(Bvh)]
     lib.orc_morton_codes.argtypes = [_f32p, ctypes.c_size_t, _u32p]
@@ -333,6 +336,34 @@ def signed_points_gwn(xyz, pts, threads=0):
     if r != 0:
         raise ValueError("empty mesh")
     return out, tie.astype(bool)
+
+
+class BandStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_uint64) for k in ("voxels", "bricks_baked", "cap_triangles", "rechecked",
+                                               "full_cap_sums", "on_cap", "ties")] + \
+               [("seconds", ctypes.c_double), ("threads", ctypes.c_int)]
+
+
+def bake_band(xyz, origin, h, nx, ny, nz, band, stride=1, threads=0):
+    """Narrow band of any mesh: (brick index (nz/8, ny/8, nx/8) int32, pool (nbricks, 8, 8, 8) float32 with
+    NaN in the slots not baked (every stride-th slot is), stats dict).  See orc_bake_band."""
+    xyz = _f32(xyz)
+    nb = (nx // 8) * (ny // 8) * (nz // 8)
+    idx = np.empty(nb, np.int32)
+    n = ctypes.c_size_t()
+    o = _f32(origin)
+    st = BandStats()
+    args = (_p(xyz, _f32p), xyz.size // 9, _p(o, _f32p), float(h), nx, ny, nz, int(band), int(stride), int(threads),
+            _p(idx, _i32p))
+    dummy = np.zeros(1, np.float32)
+    rc = lib().orc_bake_band(*args, _p(dummy, _f32p), 0, ctypes.byref(n), ctypes.byref(st))
+    if rc == -2:
+        raise ValueError("empty mesh or grid dims not divisible by 8")
+    pool = np.full((max(n.value, 1), 8, 8, 8), np.nan, np.float32)
+    rc = lib().orc_bake_band(*args, _p(pool, _f32p), n.value, ctypes.byref(n), ctypes.byref(st))
+    assert rc == 0
+    stats = {k: getattr(st, k) for k, _ in BandStats._fields_}
+    return idx.reshape(nz // 8, ny // 8, nx // 8), pool[: n.value], stats
 
 
 def signed_points_closed(xyz, pts, threads=0):

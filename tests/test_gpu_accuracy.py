@@ -1,10 +1,17 @@
-"""Error of the GPU bake against the FP64 oracle: every voxel of the 5,120-triangle 64^3 bake (brute-force
-FP64 distance and winding number), and 1,048,576 seeded voxels of the 256^3 dense bake and the 384^3 narrow band
-(FP64 distance through the oracle's LBVH, equal to the brute force bit for bit; half of each sample within
-3 voxels of the surface), whose sign is checked on a 4,096-voxel subsample against the brute-force FP64
-winding number.  Max and RMS |d - d64| (bound 1e-5 * extent) and sign agreement on every checked voxel
-with |d64| > 1e-4 * extent whose |w64| is not within 1e-6 of 0.5.  The numbers are written as JSON
-to the file the environment variable IB_ACCURACY_JSON names, when it is set."""
+"""Error of the GPU bake against the FP64 oracle, and every voxel of the large bakes against the CPU oracle.
+
+- 5,120-triangle 64^3 bake: every voxel against the brute-force FP64 distance and winding number.
+- 327,680-triangle 256^3 bake: all 16,777,216 voxels bit for bit against the oracle's dense bake (whose sign is
+  the exact winding number of the closed mesh, its signed +z crossing count); 1,048,576 seeded voxels against
+  the FP64 distance through the oracle's LBVH (equal to the brute force bit for bit); 16,384 of them (half
+  within 3 voxels of the surface) against the brute-force FP64 winding number.
+- Precision-stress narrow bands (513,803 and 1,053,195 triangles, 384^3, +-4 voxels): every band voxel (every
+  4th brick for the larger) bit for bit against the oracle's band bake (FP64 winding number by the closure
+  identity), 1,048,576 voxels against the FP64 distance, 8,192 (4,096) against the brute-force FP64 winding
+  number.
+Max and RMS |d - d64| (bound 1e-5 * extent) and sign agreement on every checked voxel with |d64| > 1e-4 * extent
+whose |w64| is not within 1e-6 of 0.5.  The numbers are written as JSON to the file the environment variable
+IB_ACCURACY_JSON names, when it is set."""
 import json
 import os
 
@@ -76,15 +83,31 @@ def test_accuracy_plain_large(ctx, oracle, meshgen):
     zyx = _sample(g, 1 << 20, 2)
     zyx = _near_half(zyx, np.argwhere(np.abs(dist) < 3 * g.h), 3)
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    signs = np.concatenate([np.arange(2048), (1 << 19) + np.arange(2048)])  # near the surface, and anywhere
+    signs = np.concatenate([np.arange(8192), (1 << 19) + np.arange(8192)])  # near the surface, and anywhere
     _record("plain_large_256", dist[zyx[:, 0], zyx[:, 1], zyx[:, 2]], pts, m.xyz, cfg["hi"] - cfg["lo"], oracle,
             sign_sample=signs)
 
 
-def test_accuracy_stress_band(ctx, oracle, meshgen):
+def test_plain_large_every_voxel_equals_oracle(ctx, oracle, meshgen):
+    """All 16.8M voxels of the 327,680-triangle 256^3 bake, bit for bit, against the oracle's dense bake."""
     from italy_bake import bake
 
-    m, cfg = meshgen.workload_stress()
+    m, cfg = meshgen.workload_plain_large()
+    g = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["sdf_res"])
+    dist, _ = ctx.bake_dense(m.xyz, g, None)
+    ref, st = oracle.bake_dense(m.xyz, list(g.origin), g.h, g.nx, g.ny, g.nz)
+    bad = int(np.count_nonzero(dist.view(np.uint32) != ref.view(np.uint32)))
+    near = np.abs(ref) < 3 * g.h
+    REPORT["plain_large_256_every_voxel"] = {"voxels": int(ref.size), "bit_mismatches": bad,
+                                             "near_surface_voxels": int(near.sum()), "inside": int((ref < 0).sum()),
+                                             "oracle_seconds": st["seconds"], "oracle_threads": st["threads"]}
+    _write()
+    assert bad == 0
+
+
+def _band_check(ctx, oracle, name, m, cfg, sign_n, stride):
+    from italy_bake import bake
+
     g = bake.dense_grid(cfg["lo"], cfg["hi"], cfg["res"])
     index, pool = ctx.bake_band(m.xyz, g, cfg["band"])
     st = ctx.stats()
@@ -104,8 +127,27 @@ def test_accuracy_stress_band(ctx, oracle, meshgen):
     zyx[: n // 2] = slots[brick_of_slot[sel[:, 0]]] * 8 + sel[:, 1:]
     vals[: n // 2] = pool[sel[:, 0], sel[:, 1], sel[:, 2], sel[:, 3]]
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    signs = np.concatenate([np.arange(2048), (n // 2) + np.arange(2048)])
-    _record("stress_band_384", vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle, sign_sample=signs)
-    REPORT["stress_band_384"].update(bricks=int((index >= 0).sum()), fp64_rechecked=int(st["rechecked"]),
-                                     parity_ties=int(st["ties"]), boundary_edges=int(st["boundary_edges"]))
+    signs = np.concatenate([np.arange(sign_n // 2), (n // 2) + np.arange(sign_n // 2)])
+    _record(name, vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle, sign_sample=signs)
+    # every (stride-th) band brick against the oracle's band bake
+    oi, op, ost = oracle.bake_band(m.xyz, list(g.origin), g.h, g.nx, g.ny, g.nz, cfg["band"], stride=stride)
+    assert np.array_equal(oi, index)
+    baked = np.arange(0, len(pool), stride)
+    bad = int(np.count_nonzero(pool[baked].view(np.uint32) != op[baked].view(np.uint32)))
+    REPORT[name].update(bricks=int((index >= 0).sum()), fp64_rechecked=int(st["rechecked"]),
+                        parity_ties=int(st["ties"]), boundary_edges=int(st["boundary_edges"]),
+                        triangles=int(m.ntri), oracle_band_voxels=int(ost["voxels"]), oracle_band_mismatches=bad,
+                        oracle_band_seconds=ost["seconds"], oracle_band_threads=ost["threads"])
     _write()
+    assert bad == 0
+
+
+def test_accuracy_stress_band(ctx, oracle, meshgen):
+    m, cfg = meshgen.workload_stress()
+    _band_check(ctx, oracle, "stress_band_384", m, cfg, 8192, 1)
+
+
+def test_accuracy_stress_band_1m(ctx, oracle, meshgen):
+    """The precision-stress union at its stated ~1M triangles (64x64-segment tori, 46x46-quad box faces)."""
+    m, cfg = meshgen.workload_stress_1m()
+    _band_check(ctx, oracle, "stress_band_384_1m", m, cfg, 4096, 4)

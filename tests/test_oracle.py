@@ -480,3 +480,64 @@ def test_signed_points_closed_equals_the_dense_bake(oracle, meshgen):
     import shapes
 
     assert oracle.signed_points_closed(shapes.box_faces(0.5, 2, skip=("+z",)), pts[:10]) is None
+
+
+def test_bake_band_against_brute_force(oracle, meshgen):
+    """orc_bake_band: the bake's brick rule (equal to band_bricks), FP32 distances equal to the LBVH point
+    query, signs equal to the FP64 brute-force winding number (or its parity tie) on an open union; on a
+    closed mesh, every baked voxel equal to the dense CPU bake."""
+    m = meshgen.stress(seed=8, ncomp=4, remove_frac=0.02)
+    h = np.float32(2.0 / 48)
+    origin = [-1.0, -1.0, -1.0]
+    idx, pool, st = oracle.bake_band(m.xyz, origin, h, 48, 48, 48, 4, stride=1, threads=4)
+    assert np.array_equal(idx.reshape(-1) >= 0, oracle.band_bricks(m.xyz, origin, h, 48, 48, 48, 4))
+    assert st["cap_triangles"] == oracle.boundary_edge_count(m.xyz) > 0
+    kept = np.nonzero(idx.reshape(-1) >= 0)[0]
+    slots = idx.reshape(-1)[kept]
+    rng = np.random.default_rng(5)
+    pick = rng.choice(len(kept), 24, replace=False)
+    lin, vals = [], []
+    for i in pick:
+        b = kept[i]
+        bx, by, bz = b % 6, (b // 6) % 6, b // 36
+        off = np.arange(0, 512, 5)
+        lin.append(((bz * 8 + off // 64) * 48 + by * 8 + (off // 8) % 8) * 48 + bx * 8 + off % 8)
+        vals.append(pool[slots[i]].reshape(-1)[off])
+    lin, vals = np.concatenate(lin), np.concatenate(vals)
+    pts = oracle.cell_centres_at(origin, h, lin, 48, 48)
+    ref, _ = oracle.signed_points_gwn(m.xyz, pts, threads=4)
+    assert np.array_equal(np.abs(vals).view(np.uint32), np.abs(ref).view(np.uint32))
+    assert np.array_equal(vals < 0, ref < 0)
+    closed = meshgen.displaced(4, 2)
+    h2 = np.float32(1.6 / 32)
+    idx, pool, st = oracle.bake_band(closed.xyz, [-0.8] * 3, h2, 32, 32, 32, 4, stride=1, threads=4)
+    assert st["cap_triangles"] == 0
+    dense, _ = oracle.bake_dense(closed.xyz, [-0.8] * 3, h2, 32, 32, 32)
+    for b in np.nonzero(idx.reshape(-1) >= 0)[0]:
+        bx, by, bz = b % 4, (b // 4) % 4, b // 16
+        want = dense[bz * 8:bz * 8 + 8, by * 8:by * 8 + 8, bx * 8:bx * 8 + 8]
+        assert np.array_equal(pool[idx.reshape(-1)[b]].view(np.uint32), want.view(np.uint32))
+
+
+def test_bake_band_ties_on_an_open_face(oracle):
+    """The band of the open cube on the tie grid: the hole-plane voxels of the kept bricks are parity ties,
+    signed as orc_gwn_sign signs them."""
+    import shapes
+
+    box = shapes.box_faces(0.5, 4, skip=("+z",))
+    origin, h = [-0.78125] * 3, np.float32(0.0625)
+    idx, pool, st = oracle.bake_band(box, origin, h, 24, 24, 24, 4, stride=1, threads=4)
+    assert st["ties"] > 0 and st["on_cap"] > 0
+    kept = np.nonzero(idx.reshape(-1) >= 0)[0]
+    lin, vals = [], []
+    for b in kept:
+        bx, by, bz = b % 3, (b // 3) % 3, b // 9
+        off = np.arange(512)
+        lin.append(((bz * 8 + off // 64) * 24 + by * 8 + (off // 8) % 8) * 24 + bx * 8 + off % 8)
+        vals.append(pool[idx.reshape(-1)[b]].reshape(-1))
+    lin, vals = np.concatenate(lin), np.concatenate(vals)
+    ref, tie = oracle.signed_points_gwn(box, oracle.cell_centres_at(origin, h, lin, 24, 24), threads=4)
+    far = np.abs(ref) > 1.5e-4
+    assert tie.sum() > 0
+    assert np.array_equal(np.abs(vals), np.abs(ref))
+    assert np.array_equal((vals < 0)[far], (ref < 0)[far])
