@@ -31,6 +31,9 @@ PEAK_HBM_GBS = 8000.0
 # dot 3, scale 1, two fma 4, square 3) 13 + two edge functions 6 + w^2 added to the in-plane term (one fma) 2 = 57
 FLOP_BOX = 11
 FLOP_TRI = 57
+# point/oriented-box distance of a leaf block: offset 3 + three frame dots 15 + three |u| - e 3 + sum of
+# squares 5 = 26
+FLOP_OBB = 26
 # FLOPs per cell of the glass cup's distance lambda (sdf_procedural.cpp:123-130; sqrt and divide = 1):
 # cell centre 3 x 3 = 9, three spheres 3 x (3 sub + 3 mul + 2 add + sqrt + sub) = 30, two plane
 # offsets p.y - h = 2, smooth intersect (b - a, * 0.5, / k, + 0.5, two 1 - h, three mul, add, mul, sub) = 12
@@ -438,7 +441,7 @@ def band_section(ctx, steps=3, cpu=True):
         for p in (d_xyz, d_idx, d_pool):
             ctx.free(p)
         ms = 1e3 * float(np.median(times))
-        exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI
+        exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI + cst["obb_tests"] * FLOP_OBB
         rate = int(n.value) * 512 / (ms * 1e-3)
         out = {
             "workload": f"{label} ({mesh.ntri} triangles after 2% removed), {cfg['res']}^3 narrow band "
@@ -521,7 +524,7 @@ def batched_section(ctx, n, cpu=True):
     ctx.bake_batch_device(jobs)
     cst = ctx.stats()
     ctx.set_option(bake.OPT_COUNT_WORK, 0)
-    exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI
+    exec_flops = cst["box_tests"] * FLOP_BOX + cst["tri_tests"] * FLOP_TRI + cst["obb_tests"] * FLOP_OBB
     qms = float(np.median(query_ms))
     executed = {"flops": exec_flops, "TFLOP/s": exec_flops / (qms * 1e-3) / 1e12,
                 "frac_of_157_3": exec_flops / (qms * 1e-3) / 1e12 / PEAK_FP32_TFLOPS,

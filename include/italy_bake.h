@@ -72,7 +72,7 @@ typedef struct {
   uint64_t stack_overflows; /* traversal pushes that found the stack full: 0 unless the tree is malformed */
   double ms_caps;        /* host time of the hole caps of open meshes (boundary read-back, chains, cap tree) */
   double ms_query;       /* the closest-triangle query kernel alone (dense bakes; HIP events around its launch) */
-  uint64_t reserved[1];
+  uint64_t obb_tests;    /* point/oriented-box tests of leaf blocks in the per-brick query (counting mode) */
 } ib_stats;
 
 /* Options for ib_set_option. */

@@ -443,6 +443,7 @@ struct MeshDev {
   float4 *tris = nullptr;
   TriRec *recs = nullptr;
   Node4 *nodes4 = nullptr;
+  Obb *obbs = nullptr;
   float *caps = nullptr;
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
@@ -552,6 +553,7 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   IB_TRY(take(c, out, "tris", 3ull * n, &m.tris));
   IB_TRY(take(c, out, "trirecs", (size_t)n, &m.recs));
   IB_TRY(take(c, out, "nodes4", std::max(n - 1, 1), &m.nodes4));
+  IB_TRY(take(c, out, "obbs", (size_t)n, &m.obbs));
   launch_prep(d_xyz, n, sc, s);
   launch_morton(d_xyz, n, sc, codes_in, idx, s);
   size_t temp_bytes = 0;
@@ -574,7 +576,7 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   fill_u32(visits + n, 0u, 1, s);  // the count of nodes that span 256-leaf blocks
   const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
   launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4,
-               !side_depth, s);
+               m.obbs, !side_depth, s);
   if (side_depth) IB_HIP(hipStreamWaitEvent(s, c->join[2], 0));
   m.n = n;
   return IB_OK;
@@ -657,6 +659,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.tris = m.tris;
   ca.recs = m.recs;
   ca.nodes4 = m.nodes4;
+  ca.obbs = m.obbs;
   ca.caps = m.caps;
   ca.ncap = m.ncap;
   ca.cap_nodes = m.cap_nodes;
@@ -1611,6 +1614,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
     st.box_tests = h.box_tests;
     st.tri_tests = h.tri_tests;
     st.stack_overflows = h.stack_overflow;
+    st.obb_tests = h.obb_tests;
   }
   *out = st;
   return IB_OK;
@@ -1739,6 +1743,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       sa.col.tris = mesh[j].tris;
       sa.col.recs = mesh[j].recs;
       sa.col.nodes4 = mesh[j].nodes4;
+      sa.col.obbs = mesh[j].obbs;
       sa.col.g = to_args(cg);
       sa.col.sc = scs + j;
       sa.out = cdist[j];
@@ -1852,6 +1857,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.column_overflows += h.col_overflow;
     c->stats.box_tests += h.box_tests;
     c->stats.tri_tests += h.tri_tests;
+    c->stats.obb_tests += h.obb_tests;
     c->stats.stack_overflows += h.stack_overflow;
     if (h.stack_overflow)
       return fail(IB_ERR_INTERNAL, "job " + std::to_string(j) + ": a traversal stack overflowed");

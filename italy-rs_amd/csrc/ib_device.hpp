@@ -38,6 +38,19 @@ struct alignas(16) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "4-wide record is two 64-byte lines");
 
+// Oriented box of a leaf block of two or more triangles: centre c, axes x, y, z (the FP32 roundings of an
+// orthonormal FP64 frame whose z is the block's area-weighted normal) and half extents e (FP64 maxima of
+// |axis . (v - c)| over the block's vertices, rounded up): every triangle of the block lies in
+// { p : |axis_k . (p - c)| <= e_k }.  On a smooth surface a block is nearly planar, so the box is as thin
+// as the patch's curvature, where its axis-aligned box is as thick as the patch is wide.
+struct alignas(16) Obb {
+  float c[3], ex;
+  float x[3], ey;
+  float y[3], ez;
+  float z[3], pad;
+};
+static_assert(sizeof(Obb) == 64, "oriented box record is one 64-byte line");
+
 // Node of the hole-cap hierarchy (built on the host).  left >= 0: children left/right;
 // left < 0: leaf holding triangles [-1 - left, -1 - left + right) of the reordered cap soup.
 // (c, r) bounds every vertex below; area = sum of |triangle area|; dip = sum of area vectors.
@@ -285,6 +298,12 @@ __device__ __forceinline__ float tri_d2_f32(F3 q, const TriRec &t) {
   return fmaf(w, w, inside ? 0.0f : dmin);
 }
 
+// Squared distance from q to the oriented box b, evaluated in FP32 in the box's frame.  Its error, from the
+// rounding of q - c and of the dots and from the frame's departure from orthonormality, is below
+// 2^-19 (Mq + Mp) for coordinates bounded by Mq (query) and Mp (vertices), a quarter of the pruning slack;
+// the traversals therefore compare it against prune_bound(best, 1.25 * slack).
+__device__ __forceinline__ float obb_d2(F3 q, const Obb &b);
+
 // Pruning with FP32 distances.  |sqrt(tri_d2_f32(q, t)) - dist(q, t)| stays below
 // 2^-17 (Mq + 3 Mp) when every coordinate of the query is at most Mq and every vertex coordinate
 // at most Mp in magnitude (the worst case measured over 4e6 triangles and queries across six
@@ -307,6 +326,14 @@ __device__ __forceinline__ float dist_slack(float ox, float oy, float oz, float 
   for (int e = 0; e < n; ++e)
     for (int k = 0; k < 3; ++k) mp = fmaxf(mp, fmaxf(finite_abs(lo[e][k]), finite_abs(hi[e][k])));
   return (mq + 3.0f * mp) * 7.62939453125e-6f;  // 2^-17
+}
+
+__device__ __forceinline__ float obb_d2(F3 q, const Obb &b) {
+  const F3 d = {q.x - b.c[0], q.y - b.c[1], q.z - b.c[2]};
+  const float u = fdot({b.x[0], b.x[1], b.x[2]}, d);
+  const float v = fdot({b.y[0], b.y[1], b.y[2]}, d);
+  const float w = fdot({b.z[0], b.z[1], b.z[2]}, d);
+  return sq3(fmaxf(fabsf(u) - b.ex, 0.0f), fmaxf(fabsf(v) - b.ey, 0.0f), fmaxf(fabsf(w) - b.ez, 0.0f));
 }
 
 // SLIVERS = false is only instantiated for meshes without sliver triangles (the host counts them

@@ -298,6 +298,111 @@ __global__ __launch_bounds__(256) void k_collapse_blocks(Node *nodes, const int2
   }
 }
 
+// Oriented box of every leaf block of two or more triangles (the collapsed child slots of the live binary
+// nodes, each block in exactly one slot), written at the block's first triangle.  Frame: z the normalised
+// sum of the triangles' area vectors (the world z if that sum vanishes), x the first triangle's longest
+// edge with its z part removed (a world axis if that vanishes), y = z x x, all in FP64 and rounded to FP32.
+// The centre halves the vertices' span along the rounded axes; the half extents are the FP64 maxima of
+// |axis . (v - c)| with the rounded axes and centre, rounded up to FP32.
+__device__ void block_obb(const float4 *__restrict__ tris, int first, int count, Obb &b) {
+  double n[3] = {0.0, 0.0, 0.0};
+  for (int t = first; t < first + count; ++t) {
+    const float4 a = tris[3 * t], p = tris[3 * t + 1], q = tris[3 * t + 2];
+    const double e1[3] = {(double)p.x - a.x, (double)p.y - a.y, (double)p.z - a.z};
+    const double e2[3] = {(double)q.x - a.x, (double)q.y - a.y, (double)q.z - a.z};
+    n[0] += e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] += e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] += e1[0] * e2[1] - e1[1] * e2[0];
+  }
+  double z[3] = {0.0, 0.0, 1.0};
+  const double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (nl > 0.0 && nl < INFINITY) z[0] = n[0] / nl, z[1] = n[1] / nl, z[2] = n[2] / nl;
+  double x[3];
+  {
+    const float4 a = tris[3 * first], p = tris[3 * first + 1], q = tris[3 * first + 2];
+    const double e[3][3] = {{(double)p.x - a.x, (double)p.y - a.y, (double)p.z - a.z},
+                            {(double)q.x - p.x, (double)q.y - p.y, (double)q.z - p.z},
+                            {(double)a.x - q.x, (double)a.y - q.y, (double)a.z - q.z}};
+    int k = 0;
+    double best = -1.0;
+    for (int i = 0; i < 3; ++i) {
+      const double l = e[i][0] * e[i][0] + e[i][1] * e[i][1] + e[i][2] * e[i][2];
+      if (l > best) best = l, k = i;
+    }
+    for (int i = 0; i < 3; ++i) x[i] = e[k][i];
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const double dz = x[0] * z[0] + x[1] * z[1] + x[2] * z[2];
+    for (int i = 0; i < 3; ++i) x[i] -= dz * z[i];
+    const double xl = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (xl > 1e-12 && xl < INFINITY) {
+      for (int i = 0; i < 3; ++i) x[i] /= xl;
+      break;
+    }
+    x[0] = fabs(z[0]) < 0.5 ? 1.0 : 0.0;  // a world axis not along z
+    x[1] = fabs(z[0]) < 0.5 ? 0.0 : 1.0;
+    x[2] = 0.0;
+  }
+  const double y[3] = {z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]};
+  const float ax[3][3] = {{(float)x[0], (float)x[1], (float)x[2]},
+                          {(float)y[0], (float)y[1], (float)y[2]},
+                          {(float)z[0], (float)z[1], (float)z[2]}};
+  const float4 o = tris[3 * first];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int t = first; t < first + count; ++t)
+    for (int v = 0; v < 3; ++v) {
+      const float4 p = tris[3 * t + v];
+      const double d[3] = {(double)p.x - o.x, (double)p.y - o.y, (double)p.z - o.z};
+      for (int k = 0; k < 3; ++k) {
+        const double s = ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2];
+        lo[k] = fmin(lo[k], s);
+        hi[k] = fmax(hi[k], s);
+      }
+    }
+  double c[3] = {o.x, o.y, o.z};
+  for (int k = 0; k < 3; ++k)
+    for (int i = 0; i < 3; ++i) c[i] += 0.5 * (lo[k] + hi[k]) * ax[k][i];
+  const float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
+  double e[3] = {0.0, 0.0, 0.0};
+  for (int t = first; t < first + count; ++t)
+    for (int v = 0; v < 3; ++v) {
+      const float4 p = tris[3 * t + v];
+      const double d[3] = {(double)p.x - cf[0], (double)p.y - cf[1], (double)p.z - cf[2]};
+      for (int k = 0; k < 3; ++k) e[k] = fmax(e[k], fabs(ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2]));
+    }
+  float ef[3];
+  for (int k = 0; k < 3; ++k) {
+    ef[k] = (float)e[k];
+    if ((double)ef[k] < e[k]) ef[k] = nextafterf(ef[k], INFINITY);
+  }
+  b.c[0] = cf[0], b.c[1] = cf[1], b.c[2] = cf[2];
+  b.x[0] = ax[0][0], b.x[1] = ax[0][1], b.x[2] = ax[0][2];
+  b.y[0] = ax[1][0], b.y[1] = ax[1][1], b.y[2] = ax[1][2];
+  b.z[0] = ax[2][0], b.z[1] = ax[2][1], b.z[2] = ax[2][2];
+  b.ex = ef[0], b.ey = ef[1], b.ez = ef[2];
+  b.pad = 0.0f;
+}
+
+// Only live nodes write: the root, and nodes covering more than `block` leaves.  A node covering fewer sits
+// inside a collapsed subtree (its parent's slot became a leaf block), and its own smaller blocks may start
+// at the same triangle as the live block around them.
+__global__ __launch_bounds__(256) void k_block_obbs(const Node *__restrict__ nodes, const int2 *__restrict__ ranges,
+                                                    const float4 *__restrict__ tris, int ninternal, int block,
+                                                    Obb *obbs) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ninternal) return;
+  if (i != 0 && ranges[i].y - ranges[i].x + 1 <= block) return;
+  for (int k = 0; k < 2; ++k) {
+    const int c = k ? nodes[i].c1 : nodes[i].c0;
+    if (c >= 0) continue;
+    const int code = ~c, first = code & kBlockMask, count = (code >> kBlockShift) + 1;
+    if (count < 2) continue;
+    Obb b;
+    block_obb(tris, first, count, b);
+    obbs[first] = b;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, int ninternal, Node4 *out) {
   const int x = blockIdx.x * 256 + threadIdx.x;
   if (x >= ninternal) return;
@@ -986,7 +1091,7 @@ __device__ __noinline__ bool parity_inside(const Node *__restrict__ nodes, int n
 // covers an 8x8x4 region.
 // ============================================================================================
 struct WorkCount {
-  uint32_t box = 0, tri = 0;
+  uint32_t box = 0, tri = 0, obb = 0;
 };
 
 
@@ -1223,6 +1328,19 @@ __device__ __forceinline__ Node4 load_node4_uniform(const Node4 *p, int i) {
   return u.r;
 }
 
+union ObbBits {
+  V4 v[4];
+  Obb r;
+};
+
+__device__ __forceinline__ Obb load_obb_uniform(const Obb *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  ObbBits u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u.v[k] = r[k];
+  return u.r;
+}
+
 union TriRecBits {
   V4 v[6];
   TriRec r;
@@ -1259,7 +1377,10 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
     best[k] = INFINITY;
     bound[k] = INFINITY;
   }
-  const float slack = query_slack(a.col);
+  const float slack = query_slack(a.col), slack_obb = 1.25f * slack;
+  float bound_obb[V];
+  for (int k = 0; k < V; ++k) bound_obb[k] = INFINITY;
+  const Obb *__restrict__ obbs = a.col.obbs;
   // the lane of voxel (sx/2, sy/2, lz/2) orders the children: its two box distances (first voxel)
   // are read with v_readlane and compared as integers (non-negative floats order like their bits)
   const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (lz >> 1) * sx * sy);
@@ -1328,8 +1449,17 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
 #pragma unroll
     for (int e = 0; e < 4; ++e)
       if (leaf[e]) {
-        for_leaf_code(~nd.child[e], eval);
+        const int code = ~nd.child[e];
+        if (code >> kBlockShift) {  // a block of 2+ triangles: its oriented box must reach some lane too
+          const Obb b = load_obb_uniform(obbs, code & kBlockMask);
+          bool need = false;
+          for (int k = 0; k < V; ++k) need = need || obb_d2(q[k], b) <= bound_obb[k];
+          if (COUNT) wc.obb += V;
+          if (__ballot(need) == 0) continue;
+        }
+        for_leaf_code(code, eval);
         evaluated = true;
+        for (int k = 0; k < V; ++k) bound_obb[k] = prune_bound(best[k], slack_obb);
       }
     if (evaluated)
       for (int k = 0; k < V; ++k) bound[k] = prune_bound(best[k], slack);
@@ -1347,6 +1477,7 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
     for (int k = 0; k < V; ++k) nv += valid[k] ? 1 : 0;
     if (nv) {
       atomicAdd(&a.col.sc->box_tests, (unsigned long long)wc.box * nv / V);
+      atomicAdd(&a.col.sc->obb_tests, (unsigned long long)wc.obb * nv / V);
       atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri * nv / V);
     }
   }
@@ -1513,7 +1644,7 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
                   float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4,
-                  bool depth, hipStream_t s) {
+                  Obb *obbs, bool depth, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
     k_build4<<<1, 256, 0, s>>>(nodes, 1, nodes4);
@@ -1523,7 +1654,10 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   k_refit_blocks<<<nblk, 256, 0, s>>>(tris, n, nodes, ranges, leafbox, blockbox, root_box, span);
   k_refit_span<<<std::min(2 * nblk, 1024), 256, 0, s>>>(n, nodes, ranges, leafbox, blockbox, root_box, span);
   if (depth) k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);
-  if (block > 1) k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
+  if (block > 1) {
+    k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
+    k_block_obbs<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, tris, n - 1, block, obbs);
+  }
   k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1, nodes4);
 }
 void launch_depth(const Node *nodes, const int32_t *leaf_parent, int n, Scalars *sc, hipStream_t s) {

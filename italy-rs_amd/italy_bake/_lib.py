@@ -70,7 +70,7 @@ class Stats(ctypes.Structure):
         ("bvh_depth", ctypes.c_int32), ("pad0", ctypes.c_int32), ("boundary_edges", ctypes.c_int64),
         ("column_overflows", ctypes.c_uint64), ("rechecked", ctypes.c_uint64), ("ties", ctypes.c_uint64),
         ("box_tests", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64), ("stack_overflows", ctypes.c_uint64),
-        ("ms_caps", ctypes.c_double), ("ms_query", ctypes.c_double), ("reserved", ctypes.c_uint64 * 1),
+        ("ms_caps", ctypes.c_double), ("ms_query", ctypes.c_double), ("obb_tests", ctypes.c_uint64),
     ]
 
     def as_dict(self):
