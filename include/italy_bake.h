@@ -87,7 +87,7 @@ typedef enum {
   IB_OPT_TILE_ORDER = 5,    /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
                                first (estimated from the column crossings), 2 (default) half-octave cost
                                buckets, costliest first, Morton order inside a bucket.  Speed only. */
-  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..16, default 8) are evaluated
+  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..32, default 8) are evaluated
                                as one block from their parent.  Speed only; applies to later builds. */
 } ib_option;
 

@@ -444,6 +444,7 @@ struct MeshDev {
   TriRec *recs = nullptr;
   Node4 *nodes4 = nullptr;
   Obb *obbs = nullptr;
+  Obb *node_obbs = nullptr;
   float *caps = nullptr;
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
@@ -554,6 +555,7 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   IB_TRY(take(c, out, "trirecs", (size_t)n, &m.recs));
   IB_TRY(take(c, out, "nodes4", std::max(n - 1, 1), &m.nodes4));
   IB_TRY(take(c, out, "obbs", (size_t)n, &m.obbs));
+  IB_TRY(take(c, out, "node_obbs", (size_t)std::max(n - 1, 1), &m.node_obbs));
   launch_prep(d_xyz, n, sc, s);
   launch_morton(d_xyz, n, sc, codes_in, idx, s);
   size_t temp_bytes = 0;
@@ -574,9 +576,9 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   }
   launch_gather(d_xyz, order, n, m.tris, m.recs, sc, s);
   fill_u32(visits + n, 0u, 1, s);  // the count of nodes that span 256-leaf blocks
-  const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^27 triangles
+  const int block = (uint64_t)n <= (uint64_t)kBlockMask + 1 ? c->leaf_block : 1;  // leaf codes index 2^26 triangles
   launch_refit(m.tris, n, m.nodes, leaf_parent, visits, leafbox, blockbox, root_box, sc, ranges, block, m.nodes4,
-               m.obbs, !side_depth, s);
+               m.obbs, m.node_obbs, !side_depth, s);
   if (side_depth) IB_HIP(hipStreamWaitEvent(s, c->join[2], 0));
   m.n = n;
   return IB_OK;
@@ -660,6 +662,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.recs = m.recs;
   ca.nodes4 = m.nodes4;
   ca.obbs = m.obbs;
+  ca.node_obbs = m.node_obbs;
   ca.caps = m.caps;
   ca.ncap = m.ncap;
   ca.cap_nodes = m.cap_nodes;
@@ -1744,6 +1747,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       sa.col.recs = mesh[j].recs;
       sa.col.nodes4 = mesh[j].nodes4;
       sa.col.obbs = mesh[j].obbs;
+      sa.col.node_obbs = mesh[j].node_obbs;
       sa.col.g = to_args(cg);
       sa.col.sc = scs + j;
       sa.out = cdist[j];

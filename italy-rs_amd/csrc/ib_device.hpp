@@ -12,10 +12,10 @@ constexpr int kStack4 = 128;      // brick query stack over the 4-wide view: <= 
 constexpr int kColCap = 32;       // crossings stored per +z column before the per-voxel fallback
 constexpr uint32_t kSliver = 1u;  // triangle flag: distance evaluated in FP64
 // leaf code ~L: first triangle L & kBlockMask, count (L >> kBlockShift) + 1; blocks of more than one
-// triangle are built only for meshes of at most 2^27 triangles, so that L stays non-negative
-constexpr int kBlockShift = 27;
+// triangle are built only for meshes of at most 2^26 triangles, so that L stays non-negative
+constexpr int kBlockShift = 26;
 constexpr int kBlockMask = (1 << kBlockShift) - 1;
-constexpr int kMaxLeafBlock = 16;
+constexpr int kMaxLeafBlock = 32;
 
 // Internal LBVH node: both children's boxes inline so one 64-byte record decides both children.
 // child >= 0: internal node index, child < 0: ~leaf (sorted triangle position).
@@ -29,13 +29,16 @@ static_assert(sizeof(Node) == 64, "node record is one 64-byte line");
 // 4-wide view of the LBVH for the brick query: the record of internal node x holds x's
 // grandchildren (an internal child is replaced by its two children; a leaf or leaf block stays one
 // entry): up to 4 boxes and child codes (internal node index, whose own Node4 follows, or a leaf
-// code < 0).  Entries past `count` are unused.
+// code < 0).  Entries past `count` are unused.  Bit e of obb_mask: entry e is an internal node of at most
+// kNodeObbMax triangles, whose oriented box is in the node-box array at its index.
 struct alignas(16) Node4 {
   float lo[4][3];
   float hi[4][3];
   int32_t child[4];
-  int32_t count, pad[3];
+  int32_t count, obb_mask, pad[2];
 };
+// internal nodes of at most this many triangles (and more than a leaf block) get an oriented box too
+constexpr int kNodeObbMax = 1024;
 static_assert(sizeof(Node4) == 128, "4-wide record is two 64-byte lines");
 
 // Oriented box of a leaf block of two or more triangles: centre c, axes x, y, z (the FP32 roundings of an

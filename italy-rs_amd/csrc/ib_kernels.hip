@@ -403,7 +403,113 @@ __global__ __launch_bounds__(256) void k_block_obbs(const Node *__restrict__ nod
   }
 }
 
-__global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, int ninternal, Node4 *out) {
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
+  return v;
+}
+
+// Oriented box of every live internal node covering more than `block` and at most kNodeObbMax triangles,
+// at the node's index (those are the internal entries k_build4 marks): block_obb's construction with the
+// node's triangles dealt to the 64 lanes of a wave and the sums, minima and maxima combined across the wave.
+// Any frame gives a box that holds the triangles; the sums' order only moves the frame.
+__global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ ranges, const float4 *__restrict__ tris,
+                                                   int ninternal, int block, Obb *node_obbs) {
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (i >= ninternal) return;  // wave-uniform
+  const int first = ranges[i].x, size = ranges[i].y - ranges[i].x + 1;
+  if (size <= block || size > kNodeObbMax) return;
+  double n[3] = {0.0, 0.0, 0.0};
+  for (int t = first + lane; t < first + size; t += 64) {
+    const float4 a = tris[3 * t], p = tris[3 * t + 1], q = tris[3 * t + 2];
+    const double e1[3] = {(double)p.x - a.x, (double)p.y - a.y, (double)p.z - a.z};
+    const double e2[3] = {(double)q.x - a.x, (double)q.y - a.y, (double)q.z - a.z};
+    n[0] += e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] += e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] += e1[0] * e2[1] - e1[1] * e2[0];
+  }
+  for (int k = 0; k < 3; ++k) n[k] = wave_sum(n[k]);
+  double z[3] = {0.0, 0.0, 1.0};
+  const double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+  if (nl > 0.0 && nl < INFINITY) z[0] = n[0] / nl, z[1] = n[1] / nl, z[2] = n[2] / nl;
+  double x[3];
+  {
+    const float4 a = tris[3 * first], p = tris[3 * first + 1], q = tris[3 * first + 2];
+    const double e[3][3] = {{(double)p.x - a.x, (double)p.y - a.y, (double)p.z - a.z},
+                            {(double)q.x - p.x, (double)q.y - p.y, (double)q.z - p.z},
+                            {(double)a.x - q.x, (double)a.y - q.y, (double)a.z - q.z}};
+    int k = 0;
+    double best = -1.0;
+    for (int j = 0; j < 3; ++j) {
+      const double l = e[j][0] * e[j][0] + e[j][1] * e[j][1] + e[j][2] * e[j][2];
+      if (l > best) best = l, k = j;
+    }
+    for (int j = 0; j < 3; ++j) x[j] = e[k][j];
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    const double dz = x[0] * z[0] + x[1] * z[1] + x[2] * z[2];
+    for (int j = 0; j < 3; ++j) x[j] -= dz * z[j];
+    const double xl = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+    if (xl > 1e-12 && xl < INFINITY) {
+      for (int j = 0; j < 3; ++j) x[j] /= xl;
+      break;
+    }
+    x[0] = fabs(z[0]) < 0.5 ? 1.0 : 0.0;
+    x[1] = fabs(z[0]) < 0.5 ? 0.0 : 1.0;
+    x[2] = 0.0;
+  }
+  const double y[3] = {z[1] * x[2] - z[2] * x[1], z[2] * x[0] - z[0] * x[2], z[0] * x[1] - z[1] * x[0]};
+  const float ax[3][3] = {{(float)x[0], (float)x[1], (float)x[2]},
+                          {(float)y[0], (float)y[1], (float)y[2]},
+                          {(float)z[0], (float)z[1], (float)z[2]}};
+  const float4 o = tris[3 * first];
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = lane; c < 3 * size; c += 64) {
+    const float4 p = tris[3 * first + c];
+    const double d[3] = {(double)p.x - o.x, (double)p.y - o.y, (double)p.z - o.z};
+    for (int k = 0; k < 3; ++k) {
+      const double s = ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2];
+      lo[k] = fmin(lo[k], s);
+      hi[k] = fmax(hi[k], s);
+    }
+  }
+  for (int k = 0; k < 3; ++k) lo[k] = wave_min(lo[k]), hi[k] = wave_max(hi[k]);
+  double cc[3] = {o.x, o.y, o.z};
+  for (int k = 0; k < 3; ++k)
+    for (int j = 0; j < 3; ++j) cc[j] += 0.5 * (lo[k] + hi[k]) * ax[k][j];
+  const float cf[3] = {(float)cc[0], (float)cc[1], (float)cc[2]};
+  double e[3] = {0.0, 0.0, 0.0};
+  for (int c = lane; c < 3 * size; c += 64) {
+    const float4 p = tris[3 * first + c];
+    const double d[3] = {(double)p.x - cf[0], (double)p.y - cf[1], (double)p.z - cf[2]};
+    for (int k = 0; k < 3; ++k) e[k] = fmax(e[k], fabs(ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2]));
+  }
+  for (int k = 0; k < 3; ++k) e[k] = wave_max(e[k]);
+  if (lane != 0) return;
+  Obb b;
+  float ef[3];
+  for (int k = 0; k < 3; ++k) {
+    ef[k] = (float)e[k];
+    if ((double)ef[k] < e[k]) ef[k] = nextafterf(ef[k], INFINITY);
+  }
+  b.c[0] = cf[0], b.c[1] = cf[1], b.c[2] = cf[2];
+  b.x[0] = ax[0][0], b.x[1] = ax[0][1], b.x[2] = ax[0][2];
+  b.y[0] = ax[1][0], b.y[1] = ax[1][1], b.y[2] = ax[1][2];
+  b.z[0] = ax[2][0], b.z[1] = ax[2][1], b.z[2] = ax[2][2];
+  b.ex = ef[0], b.ey = ef[1], b.ez = ef[2];
+  b.pad = 0.0f;
+  node_obbs[i] = b;
+}
+
+__global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, const int2 *__restrict__ ranges,
+                                                int ninternal, int block, Node4 *out) {
   const int x = blockIdx.x * 256 + threadIdx.x;
   if (x >= ninternal) return;
   Node4 r;
@@ -434,7 +540,14 @@ __global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, 
     r.child[e] = -1;
   }
   r.count = n;
-  r.pad[0] = r.pad[1] = r.pad[2] = 0;
+  r.obb_mask = 0;
+  if (block > 1)
+    for (int e = 0; e < n; ++e)
+      if (r.child[e] >= 0) {
+        const int size = ranges[r.child[e]].y - ranges[r.child[e]].x + 1;
+        if (size <= kNodeObbMax) r.obb_mask |= 1 << e;
+      }
+  r.pad[0] = r.pad[1] = 0;
   out[x] = r;
 }
 
@@ -812,6 +925,13 @@ __device__ __forceinline__ void for_column_caps(const ColumnArgs &a, float X, fl
   }
 }
 
+// (X, Y) inside the closed xy bounding box of the triangle: a +z line can cross the triangle only then (the
+// crossing test's half-open rule only decides between triangles that all contain the point in that box)
+__device__ __forceinline__ bool column_in_xy_box(float X, float Y, const float4 *t) {
+  return X >= fminf(fminf(t[0].x, t[1].x), t[2].x) && X <= fmaxf(fmaxf(t[0].x, t[1].x), t[2].x) &&
+         Y >= fminf(fminf(t[0].y, t[1].y), t[2].y) && Y <= fmaxf(fmaxf(t[0].y, t[1].y), t[2].y);
+}
+
 __device__ __forceinline__ void column_crossings(const ColumnArgs &a, int x, int y) {
   const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
   const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
@@ -819,6 +939,7 @@ __device__ __forceinline__ void column_crossings(const ColumnArgs &a, int x, int
   ColumnSink sink{a.cross + col * kColCap, 0};
   for_column_leaves(a.nodes, a.nnodes, X, Y, a.sc, [&](int leaf) {
     const float4 *t = a.tris + 3 * leaf;
+    if (!column_in_xy_box(X, Y, t)) return;
     float z;
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
     if (s) sink.add(z, s);
@@ -863,6 +984,7 @@ __device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y,
   int w = 0;
   for_column_leaves(a.nodes, a.nnodes, X, Y, a.sc, [&](int leaf) {
     const float4 *t = a.tris + 3 * leaf;
+    if (!column_in_xy_box(X, Y, t)) return;
     float z;
     const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
     if (s && z > qz) w += s;
@@ -1422,6 +1544,13 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
       for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= bound[k];
       if (__ballot(need) == 0) continue;
       const int c = nd.child[e];
+      if ((nd.obb_mask >> e) & 1) {  // a small subtree: its oriented box must reach some lane too
+        const Obb b = load_obb_uniform(a.col.node_obbs, c);
+        bool near = false;
+        for (int k = 0; k < V; ++k) near = near || obb_d2(q[k], b) <= bound_obb[k];
+        if (COUNT) wc.obb += V;
+        if (__ballot(near) == 0) continue;
+      }
       leaf[e] = c < 0;
       if (c >= 0) {
         ch[e] = c;
@@ -1644,10 +1773,10 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
 }
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
                   float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4,
-                  Obb *obbs, bool depth, hipStream_t s) {
+                  Obb *obbs, Obb *node_obbs, bool depth, hipStream_t s) {
   if (n == 1) {
     k_single<<<1, 1, 0, s>>>(tris, nodes, root_box);
-    k_build4<<<1, 256, 0, s>>>(nodes, 1, nodes4);
+    k_build4<<<1, 256, 0, s>>>(nodes, nullptr, 1, 1, nodes4);
     return;
   }
   const int nblk = (n + 255) / 256;
@@ -1657,8 +1786,9 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   if (block > 1) {
     k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
     k_block_obbs<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, tris, n - 1, block, obbs);
+    k_node_obbs<<<nb(n - 1, 4), 256, 0, s>>>(ranges, tris, n - 1, block, node_obbs);
   }
-  k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, n - 1, nodes4);
+  k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block, nodes4);
 }
 void launch_depth(const Node *nodes, const int32_t *leaf_parent, int n, Scalars *sc, hipStream_t s) {
   if (n > 1) k_depth<<<nb(n), 256, 0, s>>>(nodes, leaf_parent, n, sc);

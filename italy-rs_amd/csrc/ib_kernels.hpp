@@ -56,6 +56,7 @@ struct ColumnArgs {
   const TriRec *recs;  // distance records, same order as tris
   const Node4 *nodes4; // 4-wide view of the nodes (brick query)
   const Obb *obbs;     // oriented box of every leaf block of 2+ triangles, at the block's first triangle
+  const Obb *node_obbs;  // oriented box of every internal node of at most kNodeObbMax triangles, at its index
   const float *caps;  // 9 floats per cap triangle, in cap-hierarchy leaf order
   int ncap;
   const CapNode *cap_nodes;
@@ -101,7 +102,7 @@ void launch_gather(const float *xyz, const int32_t *order, int n, float4 *tris, 
 // Scratch: span n + 1 words (entry n zeroed by the caller), leafbox 6n floats, blockbox 6 per 256 leaves.
 void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_parent, uint32_t *span,
                   float *leafbox, float *blockbox, float *root_box, Scalars *sc, const int2 *ranges, int block, Node4 *nodes4,
-                  Obb *obbs, bool depth, hipStream_t s);
+                  Obb *obbs, Obb *node_obbs, bool depth, hipStream_t s);
 // the tree depth into sc->depth (parent chains of the Karras tree; launch_refit runs it when `depth`)
 void launch_depth(const Node *nodes, const int32_t *leaf_parent, int n, Scalars *sc, hipStream_t s);
 void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uint32_t vmask, int32_t *cslot,

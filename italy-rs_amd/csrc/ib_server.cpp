@@ -215,7 +215,7 @@ struct ib_server {
     const uint64_t cols = r.sdf.nx ? (uint64_t)r.sdf.nx * r.sdf.ny * (32 * 8 + 4) : 0;
     // soup + triangle and distance records + nodes (binary and 4-wide) + oriented leaf boxes + weld/edge tables
     // + sort temporaries + refit leaf boxes + columns + lists
-    est = n * (36 + 48 + 96 + 64 + 128 + 64 + 320 + 24) + cols + out / 8 + out + (8u << 20);
+    est = n * (36 + 48 + 96 + 64 + 128 + 128 + 320 + 24) + cols + out / 8 + out + (8u << 20);
     resident = out;
   }
 
