@@ -155,6 +155,16 @@ ib_status ib_bake_sdf(ib_context *ctx, const float *xyz, uint64_t ntri, const fl
 ib_status ib_voxelize(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
                       int32_t resolution, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count);
 
+/* The whole of italy::voxelizeMesh (voxelize.cpp:52-107) in one call: the occupied cells in first-touch
+ * order (as ib_voxelize) and VoxelGrid.colors, per cell the mean of its triangles' colours (tri_colors: 3
+ * floats per triangle, the reference's triangleColor; NULL: white), summed in triangle order and divided by
+ * the count per component.  Memory and work scale with the (cell, triangle) overlap pairs, not with the grid.
+ * cells / colors (3 per cell each) may be NULL to only count; on IB_ERR_CAPACITY *count holds the number
+ * of cells needed.  Host buffers. */
+ib_status ib_voxelize_colors(ib_context *ctx, const float *xyz, uint64_t ntri, const float bmin[3],
+                             const float bmax[3], int32_t resolution, const float *tri_colors, int32_t *cells,
+                             float *colors, uint64_t capacity, uint64_t *count);
+
 /* Per-triangle material as the bake and the voxelizer compute it: triangleMaterial
  * (src/convert/sdf_baker.cpp:61-74) and triangleColor (src/convert/voxelize.cpp:26-38), with
  * sampleColorNearest and srgbToLinear (sdf_baker.cpp:42-53).  Per triangle t: the material

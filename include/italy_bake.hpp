@@ -197,33 +197,29 @@ VoxelGrid voxelizeMesh(const Mesh &mesh, int resolution, ColorFn &&color) {
   bounds(mesh.boundsMin, mesh.boundsMax, bmin, bmax);
   check(ib_vox_framing(bmin, bmax, resolution, grid.origin.data(), &grid.voxelSize), "ib_vox_framing");
   const uint64_t ntri = mesh.positions.size() / 3;
-  uint64_t count = 0;
-  check(ib_voxelize(context(), soup(mesh), ntri, bmin, bmax, resolution, nullptr, nullptr, 0, &count),
-        "ib_voxelize");
-  grid.cells.resize(count);
-  if (count)
-    check(ib_voxelize(context(), soup(mesh), ntri, bmin, bmax, resolution, grid.cells[0].data(), nullptr, count,
-                      &count),
-          "ib_voxelize");
-  // colours: average over every touching triangle in triangle order, on the voxel grid
-  const int r = resolution < 1 ? 1 : resolution;
-  ib_grid vg{{grid.origin[0], grid.origin[1], grid.origin[2]}, grid.voxelSize, r, r, r, 0};
-  std::vector<float> mat(5 * ntri);
+  std::vector<float> rgb(3 * ntri);
   for (uint64_t t = 0; t < ntri; ++t) {
-    color(t, &mat[5 * t]);
-    mat[5 * t + 3] = 0.0f;
-    mat[5 * t + 4] = 0.0f;
+    float m[5] = {1.0f, 1.0f, 1.0f, 0.0f, 0.0f};
+    color(t, m);
+    for (int k = 0; k < 3; ++k) rgb[3 * t + k] = m[k];
   }
-  const uint64_t ncell = static_cast<uint64_t>(r) * r * r;
-  std::vector<float> rgb(3 * ncell), m(ncell), rough(ncell);
-  if (count) check(ib_splat_materials(context(), soup(mesh), ntri, mat.data(), &vg, rgb.data(), m.data(), rough.data()),
-                   "ib_splat_materials");
+  // one call for the common case: room for a few cells per triangle, a second call with the exact count
+  // when the triangles are large against the voxels
+  uint64_t count = 0, cap = std::max<uint64_t>(1024, 4 * ntri);
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    grid.cells.resize(cap);
+    grid.colors.resize(cap);
+    const ib_status st = ib_voxelize_colors(context(), soup(mesh), ntri, bmin, bmax, resolution, rgb.data(),
+                                            grid.cells[0].data(), grid.colors[0].data(), cap, &count);
+    if (st == IB_ERR_CAPACITY && attempt == 0) {
+      cap = count;
+      continue;
+    }
+    check(st, "ib_voxelize_colors");
+    break;
+  }
+  grid.cells.resize(count);
   grid.colors.resize(count);
-  for (uint64_t i = 0; i < count; ++i) {
-    const auto &c = grid.cells[i];
-    const uint64_t idx = (static_cast<uint64_t>(c[2]) * r + c[1]) * r + c[0];
-    grid.colors[i] = {rgb[3 * idx], rgb[3 * idx + 1], rgb[3 * idx + 2]};
-  }
   return grid;
 }
 

@@ -1031,12 +1031,18 @@ ib_status ib_bake_sdf(ib_context *c, const float *xyz, uint64_t ntri, const floa
   return ib_bake_dense(c, xyz, ntri, grid, dist, nullptr, nullptr);
 }
 
-ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
-                      int32_t res, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count) {
-  if (!c || !count) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
-  if (ntri && !xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
+// voxelizeMesh (src/convert/voxelize.cpp:52-107) without a dense grid: every (cell, triangle) overlap pair
+// of the SAT test over each triangle's clamped cell range, in triangle order; a stable sort by cell puts each
+// cell's triangles in order, so the first is the cell's first touch and the colour sum runs in triangle order
+// (one thread per cell); the cells are then sorted by (first triangle, z, y, x), the reference's insertion
+// order.  Memory and work scale with the pairs.  Two read-backs: the pair count, then the cell count.
+static ib_status voxelize_sparse(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3],
+                                 const float bmax[3], int32_t res, const float *tri_colors, int32_t *cells,
+                                 int32_t *cell_tri, float *colors, bool want_colors, uint64_t capacity,
+                                 uint64_t *count) {
   *count = 0;
   if (ntri == 0) return IB_OK;
+  if (!xyz) return fail(IB_ERR_INVALID_ARGUMENT, "null triangle buffer");
   if (res <= 0)
     return fail(IB_ERR_INVALID_ARGUMENT, "resolution must be >= 1 (the reference clamps cells to resolution-1)");
   if ((uint64_t)res * res * res >= (1ull << 32)) return fail(IB_ERR_UNSUPPORTED, "resolution too large");
@@ -1049,41 +1055,93 @@ ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const floa
   IB_TRY(ib_vox_framing(bmin, bmax, res, g.origin, &g.h));
   g.nx = g.ny = g.nz = res;
   const uint64_t ncell = (uint64_t)res * res * res;
-  float *d_xyz;
-  int32_t *first;
-  Scalars *sc;
+  const int n = (int)ntri;
+  float *d_xyz, *d_rgb_in = nullptr;
+  uint32_t *counts, *offsets;
   IB_TRY(c->scratch("in_xyz", 9 * ntri, &d_xyz));
-  IB_TRY(c->scratch("vx_first", ncell, &first));
-  IB_TRY(c->scratch("sc", 1, &sc));
+  IB_TRY(c->scratch("sp_counts", ntri + 1, &counts));
+  IB_TRY(c->scratch("sp_offsets", ntri + 1, &offsets));
   IB_HIP(hipMemcpyAsync(d_xyz, xyz, 9 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
-  fill_u32(reinterpret_cast<uint32_t *>(sc), 0u, sizeof(Scalars) / 4, s);
-  fill_u32(reinterpret_cast<uint32_t *>(first), 0x7fffffffu, ncell, s);
-  launch_first_touch(d_xyz, (int)ntri, to_args(g), first, s);
-  // compact the occupied cells into (first triangle, linear cell) keys and read the count back
-  unsigned long long *keys, *sorted;
-  IB_TRY(c->scratch("vx_keys", ncell, &keys));
-  launch_first_compact(first, ncell, sc, keys, s);
-  IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipStreamSynchronize(s));
-  const uint32_t n = c->host_sc->occ_set;
-  *count = n;
-  if (!cells) return IB_OK;
-  if (capacity < n) return fail(IB_ERR_CAPACITY, "cell buffer holds " + std::to_string(capacity) + " cells, " +
-                                                     std::to_string(n) + " are occupied");
-  IB_TRY(c->scratch("vx_sorted", std::max<uint32_t>(n, 1), &sorted));
+  if (want_colors && tri_colors) {
+    IB_TRY(c->scratch("vx_rgb_in", 3 * ntri, &d_rgb_in));
+    IB_HIP(hipMemcpyAsync(d_rgb_in, tri_colors, 3 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
+  }
+  launch_splat_count(d_xyz, n, to_args(g), counts, s);
+  fill_u32(counts + n, 0u, 1, s);
   size_t tb = 0;
-  IB_HIP(sort_keys64(nullptr, tb, keys, sorted, n, s));
+  IB_HIP(exclusive_scan_u32(nullptr, tb, counts, offsets, (uint32_t)n + 1, s));
   char *temp;
+  IB_TRY(c->scratch("sp_temp", tb, &temp));
+  IB_HIP(exclusive_scan_u32(temp, tb, counts, offsets, (uint32_t)n + 1, s));
+  uint32_t npairs = 0;
+  IB_HIP(hipMemcpyAsync(&npairs, offsets + n, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  if (npairs == 0) return IB_OK;
+  uint32_t *cell, *cell_s, *head, *run_of;
+  int32_t *tri, *tri_s;
+  IB_TRY(c->scratch("sp_cell", npairs, &cell));
+  IB_TRY(c->scratch("sp_cell_s", npairs, &cell_s));
+  IB_TRY(c->scratch("sp_tri", npairs, &tri));
+  IB_TRY(c->scratch("sp_tri_s", npairs, &tri_s));
+  IB_TRY(c->scratch("vx_head", (uint64_t)npairs + 1, &head));
+  IB_TRY(c->scratch("vx_run", (uint64_t)npairs + 1, &run_of));
+  launch_splat_emit(d_xyz, n, to_args(g), offsets, cell, tri, s);
+  int bits = 1;
+  while (bits < 32 && (1ull << bits) < ncell) ++bits;
+  tb = 0;
+  IB_HIP(sort_pairs_u32(nullptr, tb, cell, cell_s, tri, tri_s, npairs, bits, s));
+  IB_TRY(c->scratch("sp_temp", tb, &temp));
+  IB_HIP(sort_pairs_u32(temp, tb, cell, cell_s, tri, tri_s, npairs, bits, s));
+  launch_vox_heads(cell_s, npairs, head, s);
+  tb = 0;
+  IB_HIP(exclusive_scan_u32(nullptr, tb, head, run_of, npairs + 1, s));
+  IB_TRY(c->scratch("sp_temp", tb, &temp));
+  IB_HIP(exclusive_scan_u32(temp, tb, head, run_of, npairs + 1, s));
+  uint32_t nruns = 0;
+  IB_HIP(hipMemcpyAsync(&nruns, run_of + npairs, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(hipStreamSynchronize(s));
+  *count = nruns;
+  if (!cells && !cell_tri && !colors) return IB_OK;
+  if (capacity < nruns) return fail(IB_ERR_CAPACITY, "cell buffer holds " + std::to_string(capacity) + " cells, " +
+                                                         std::to_string(nruns) + " are occupied");
+  unsigned long long *keys, *keys_s;
+  uint32_t *vals, *perm;
+  float *rgb = nullptr;
+  IB_TRY(c->scratch("vx_keys", nruns, &keys));
+  IB_TRY(c->scratch("vx_keys_s", nruns, &keys_s));
+  IB_TRY(c->scratch("vx_vals", nruns, &vals));
+  IB_TRY(c->scratch("vx_perm", nruns, &perm));
+  if (want_colors) IB_TRY(c->scratch("vx_rgb", 3ull * nruns, &rgb));
+  launch_vox_runs(cell_s, tri_s, npairs, run_of, d_rgb_in, keys, vals, rgb, s);
+  tb = 0;
+  IB_HIP(sort_pairs_u64(nullptr, tb, keys, keys_s, vals, perm, nruns, s));
   IB_TRY(c->scratch("vx_temp", tb, &temp));
-  IB_HIP(sort_keys64(temp, tb, keys, sorted, n, s));
-  int32_t *d_cells, *d_tri = nullptr;
-  IB_TRY(c->scratch("vx_cells", 3ull * std::max<uint32_t>(n, 1), &d_cells));
-  if (cell_tri) IB_TRY(c->scratch("vx_tri", std::max<uint32_t>(n, 1), &d_tri));
-  launch_unpack_cells(sorted, n, res, res, d_cells, d_tri, s);
-  IB_HIP(hipMemcpyAsync(cells, d_cells, 3ull * n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-  if (cell_tri) IB_HIP(hipMemcpyAsync(cell_tri, d_tri, (uint64_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  IB_HIP(sort_pairs_u64(temp, tb, keys, keys_s, vals, perm, nruns, s));
+  int32_t *d_cells = nullptr, *d_tri = nullptr;
+  float *d_colors = nullptr;
+  if (cells) IB_TRY(c->scratch("vx_cells", 3ull * nruns, &d_cells));
+  if (cell_tri) IB_TRY(c->scratch("vx_tri", nruns, &d_tri));
+  if (colors) IB_TRY(c->scratch("vx_colors", 3ull * nruns, &d_colors));
+  launch_vox_out(keys_s, perm, nruns, res, res, rgb, d_cells, d_tri, d_colors, s);
+  if (cells) IB_HIP(hipMemcpyAsync(cells, d_cells, 3ull * nruns * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (cell_tri) IB_HIP(hipMemcpyAsync(cell_tri, d_tri, (uint64_t)nruns * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  if (colors) IB_HIP(hipMemcpyAsync(colors, d_colors, 3ull * nruns * sizeof(float), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
   return IB_OK;
+}
+
+ib_status ib_voxelize(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                      int32_t res, int32_t *cells, int32_t *cell_tri, uint64_t capacity, uint64_t *count) {
+  if (!c || !count) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  return voxelize_sparse(c, xyz, ntri, bmin, bmax, res, nullptr, cells, cell_tri, nullptr, false, capacity, count);
+}
+
+ib_status ib_voxelize_colors(ib_context *c, const float *xyz, uint64_t ntri, const float bmin[3], const float bmax[3],
+                             int32_t res, const float *tri_colors, int32_t *cells, float *colors, uint64_t capacity,
+                             uint64_t *count) {
+  if (!c || !count) return fail(IB_ERR_INVALID_ARGUMENT, "null argument");
+  return voxelize_sparse(c, xyz, ntri, bmin, bmax, res, tri_colors, cells, nullptr, colors, colors != nullptr,
+                         capacity, count);
 }
 
 static_assert(sizeof(ib_material) == sizeof(MatDev), "material layout");

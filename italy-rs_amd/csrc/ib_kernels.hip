@@ -723,15 +723,6 @@ __global__ __launch_bounds__(256) void k_occupancy(const float *__restrict__ xyz
   });
 }
 
-// first touching triangle per cell (the order key of voxelize.cpp:88-96)
-__global__ __launch_bounds__(256) void k_first_touch(const float *__restrict__ xyz, int n, GridArgs g,
-                                                     int32_t *first) {
-  const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (t >= n) return;
-  tri_cells_wave(xyz + 9 * (size_t)t, g, [&](int x, int y, int z, bool hit) {
-    if (hit) atomicMin(&first[((size_t)z * g.ny + y) * g.nx + x], t);
-  });
-}
 
 // per-cell attribute splat (sdf_baker.cpp:76-128, voxelize.cpp:94-104): every (triangle, cell)
 // overlap pair, triangles in order (pairs of one triangle are contiguous, each cell once), stably
@@ -787,6 +778,66 @@ __global__ __launch_bounds__(256) void k_splat_reduce(const uint32_t *__restrict
   for (int j = 0; j < k && j < 8; ++j) out[j < 3 ? 3 * (size_t)c + j : (size_t)j * ncell + c] = sum[j] / fc;
 }
 
+// voxelizeMesh's cells from the (cell, triangle) pairs sorted by cell (triangles in order inside a cell):
+// head[i] = 1 where a cell's run starts
+__global__ __launch_bounds__(256) void k_vox_heads(const uint32_t *__restrict__ cell, uint32_t npairs, uint32_t *head) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < npairs) head[i] = (i == 0 || cell[i - 1] != cell[i]) ? 1u : 0u;
+  if (i == npairs) head[i] = 0u;
+}
+
+// one thread per run (run index from the exclusive scan of the heads): the first-touch key
+// (first triangle << 32 | cell), its position as the sort value, and the colour mean of voxelize.cpp:94-104
+// (the sum over the cell's triangles in triangle order, divided by their count, per component)
+__global__ __launch_bounds__(256) void k_vox_runs(const uint32_t *__restrict__ cell, const int32_t *__restrict__ tri,
+                                                  uint32_t npairs, const uint32_t *__restrict__ run_of,
+                                                  const float *__restrict__ tri_rgb, unsigned long long *keys,
+                                                  uint32_t *vals, float *rgb) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= npairs || (i > 0 && cell[i - 1] == cell[i])) return;
+  const uint32_t r = run_of[i], c = cell[i];
+  keys[r] = ((unsigned long long)(uint32_t)tri[i] << 32) | c;
+  vals[r] = r;
+  if (!rgb) return;
+  float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+  uint32_t n = 0;
+  for (uint32_t p = i; p < npairs && cell[p] == c; ++p, ++n) {
+    if (tri_rgb) {
+      const float *a = tri_rgb + 3 * (size_t)tri[p];
+      s0 += a[0], s1 += a[1], s2 += a[2];
+    } else {
+      s0 += 1.0f, s1 += 1.0f, s2 += 1.0f;
+    }
+  }
+  const float fn = (float)n;
+  rgb[3 * (size_t)r] = s0 / fn;
+  rgb[3 * (size_t)r + 1] = s1 / fn;
+  rgb[3 * (size_t)r + 2] = s2 / fn;
+}
+
+// the sorted runs as cells (x, y, z), their first triangles and their colours (gathered through the sort)
+__global__ __launch_bounds__(256) void k_vox_out(const unsigned long long *__restrict__ keys,
+                                                 const uint32_t *__restrict__ perm, uint32_t n, int nx, int ny,
+                                                 const float *__restrict__ rgb, int32_t *cells, int32_t *first,
+                                                 float *colors) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= n) return;
+  const unsigned long long key = keys[k];
+  const uint32_t i = (uint32_t)(key & 0xffffffffu);
+  if (cells) {
+    cells[3 * k + 0] = (int32_t)(i % (uint32_t)nx);
+    cells[3 * k + 1] = (int32_t)((i / (uint32_t)nx) % (uint32_t)ny);
+    cells[3 * k + 2] = (int32_t)(i / ((uint32_t)nx * (uint32_t)ny));
+  }
+  if (first) first[k] = (int32_t)(key >> 32);
+  if (colors) {
+    const uint32_t r = perm[k];
+    colors[3 * k] = rgb[3 * (size_t)r];
+    colors[3 * k + 1] = rgb[3 * (size_t)r + 1];
+    colors[3 * k + 2] = rgb[3 * (size_t)r + 2];
+  }
+}
+
 // triangleMaterial (src/convert/sdf_baker.cpp:61-74) with sampleColorNearest (:44-53); the sRGB curve
 // (:42) comes in as a table over the 256 byte values, made on the host with the reference's expression
 __global__ __launch_bounds__(256) void k_tri_materials(uint32_t ntri, const float *__restrict__ uvs, uint64_t nuv,
@@ -827,27 +878,7 @@ __global__ __launch_bounds__(256) void k_tri_materials(uint32_t ntri, const floa
   o[4] = m.roughness;
 }
 
-__global__ __launch_bounds__(256) void k_first_compact(const int32_t *__restrict__ first, uint64_t ncell,
-                                                       Scalars *sc, unsigned long long *keys) {
-  const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
-  if (i >= ncell) return;
-  const int32_t t = first[i];
-  if (t == 0x7fffffff) return;
-  const uint32_t pos = atomicAdd(&sc->occ_set, 1u);
-  keys[pos] = ((unsigned long long)(uint32_t)t << 32) | (uint32_t)i;
-}
 
-__global__ __launch_bounds__(256) void k_unpack_cells(const unsigned long long *__restrict__ keys, uint32_t n,
-                                                      int nx, int ny, int32_t *cells, int32_t *tri) {
-  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
-  if (k >= n) return;
-  const unsigned long long key = keys[k];
-  const uint32_t i = (uint32_t)(key & 0xffffffffu);
-  cells[3 * k + 0] = (int32_t)(i % (uint32_t)nx);
-  cells[3 * k + 1] = (int32_t)((i / (uint32_t)nx) % (uint32_t)ny);
-  cells[3 * k + 2] = (int32_t)(i / ((uint32_t)nx * (uint32_t)ny));
-  if (tri) tri[k] = (int32_t)(key >> 32);
-}
 
 // ============================================================================================
 // +z column crossings through the mesh and (reversed) hole caps
@@ -1842,20 +1873,20 @@ void launch_tri_materials(uint32_t ntri, const float *uvs, uint64_t nuv, const u
                           const float *lut, float *out, hipStream_t s) {
   if (ntri) k_tri_materials<<<nb(ntri), 256, 0, s>>>(ntri, uvs, nuv, tri_mat, ntm, mats, nmat, texs, ntex, pixels, lut, out);
 }
-void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s) {
-  k_first_touch<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, first);
+void launch_vox_heads(const uint32_t *cell, uint32_t npairs, uint32_t *head, hipStream_t s) {
+  k_vox_heads<<<nb((uint64_t)npairs + 1), 256, 0, s>>>(cell, npairs, head);
 }
-void launch_first_compact(const int32_t *first, uint64_t ncell, Scalars *sc, unsigned long long *keys,
-                          hipStream_t s) {
-  k_first_compact<<<nb(ncell), 256, 0, s>>>(first, ncell, sc, keys);
+void launch_vox_runs(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const uint32_t *run_of,
+                     const float *tri_rgb, unsigned long long *keys, uint32_t *vals, float *rgb, hipStream_t s) {
+  if (npairs) k_vox_runs<<<nb(npairs), 256, 0, s>>>(cell, tri, npairs, run_of, tri_rgb, keys, vals, rgb);
 }
-hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
-                       uint32_t n, hipStream_t s) {
-  return rocprim::radix_sort_keys(temp, temp_bytes, kin, kout, n, 0u, 64u, s);
+hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                          const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s) {
+  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, 64u, s);
 }
-void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
-                         hipStream_t s) {
-  if (n) k_unpack_cells<<<nb(n), 256, 0, s>>>(keys, n, nx, ny, cells, tri);
+void launch_vox_out(const unsigned long long *keys, const uint32_t *perm, uint32_t n, int nx, int ny, const float *rgb,
+                    int32_t *cells, int32_t *first, float *colors, hipStream_t s) {
+  if (n) k_vox_out<<<nb(n), 256, 0, s>>>(keys, perm, n, nx, ny, rgb, cells, first, colors);
 }
 uint32_t column_tiles(const GridArgs &g) { return nb(g.nx, 16) * nb(g.ny, 16); }
 void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t tiles,

@@ -33,7 +33,7 @@ struct Scalars {
   uint32_t col_overflow;  // columns with more than kColCap crossings
   uint32_t rechecked;     // voxels re-evaluated in FP64
   uint32_t ties;          // voxels decided by ray parity
-  uint32_t occ_set;       // occupied cells (first-touch voxelizer)
+  uint32_t unused0;
   uint32_t slivers;       // triangles taking the FP64 distance path
   uint32_t ambiguous;     // voxels handed to the resolve kernel (column overflow or |w| near 0.5)
   uint32_t stack_overflow;  // traversal pushes that found the stack full (0 by the depth bound; checked by the host)
@@ -143,14 +143,15 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
 // FP64 winding recheck, parity tie-break and column-overflow recount for the listed voxels
 void launch_resolve(const SdfArgs &a, hipStream_t s);
 
-// first-touch voxelizer (drop-in voxelizeMesh order)
-void launch_first_touch(const float *xyz, int n, const GridArgs &g, int32_t *first, hipStream_t s);
-void launch_first_compact(const int32_t *first, uint64_t ncell, Scalars *sc, unsigned long long *keys,
-                          hipStream_t s);
-hipError_t sort_keys64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
-                       uint32_t n, hipStream_t s);
-void launch_unpack_cells(const unsigned long long *keys, uint32_t n, int nx, int ny, int32_t *cells, int32_t *tri,
-                         hipStream_t s);
+// sparse voxelizeMesh over the (cell, triangle) pairs sorted by cell: run heads (npairs + 1 flags), per run the
+// first-touch key, its index and (rgb != null) the colour mean; then the runs sorted by key unpacked
+void launch_vox_heads(const uint32_t *cell, uint32_t npairs, uint32_t *head, hipStream_t s);
+void launch_vox_runs(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const uint32_t *run_of,
+                     const float *tri_rgb, unsigned long long *keys, uint32_t *vals, float *rgb, hipStream_t s);
+hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                          const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s);
+void launch_vox_out(const unsigned long long *keys, const uint32_t *perm, uint32_t n, int nx, int ny, const float *rgb,
+                    int32_t *cells, int32_t *first, float *colors, hipStream_t s);
 
 // per-cell attribute splat
 void launch_splat_count(const float *xyz, int n, const GridArgs &g, uint32_t *counts, hipStream_t s);

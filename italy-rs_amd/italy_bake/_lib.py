@@ -33,7 +33,7 @@ EXPORTS = [
     "ib_analytic_framing", "ib_bake_analytic_device", "ib_bake_analytic",
     "ib_weld_positions", "ib_weld_positions_device", "ib_watertight", "ib_watertight_device",
     "ib_mesh_create", "ib_mesh_destroy", "ib_mesh_bytes", "ib_mesh_triangles", "ib_triangle_materials",
-    "ib_bake_dense_device_noalloc",
+    "ib_bake_dense_device_noalloc", "ib_voxelize_colors",
 ]
 
 
@@ -122,6 +122,7 @@ def load():
     lib.ib_vox_framing.argtypes = [_f3, _f3, _i32, _f3, _f3]
     lib.ib_bake_sdf.argtypes = [_vp, _vp, _u64, _f3, _f3, _i32, G, _vp, _u64]
     lib.ib_voxelize.argtypes = [_vp, _vp, _u64, _f3, _f3, _i32, _vp, _vp, _u64, ctypes.POINTER(_u64)]
+    lib.ib_voxelize_colors.argtypes = [_vp, _vp, _u64, _f3, _f3, _i32, _vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]
     lib.ib_splat_materials.argtypes = [_vp, _vp, _u64, _vp, G, _vp, _vp, _vp]
     lib.ib_bake_dense.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp]
     lib.ib_bake_dense_device.argtypes = [_vp, _vp, _u64, G, _vp, G, _vp, _vp]

@@ -412,24 +412,41 @@ def bakeSdf(mesh, resolution, ctx=None, materials=None):
     return grid
 
 
-def voxelizeMesh(mesh, resolution, ctx=None):
-    """italy::voxelizeMesh (src/convert/voxelize.h:8): occupied cells in first-touch order."""
+def voxelizeMesh(mesh, resolution, ctx=None, tri_colors=None):
+    """italy::voxelizeMesh (src/convert/voxelize.h:8): occupied cells in first-touch order, each cell's first
+    triangle, and VoxelGrid.colors (the mean of the touching triangles' colours; tri_colors: (ntri, 3) float32,
+    None: white).  Room for a few cells per triangle first, the exact count on a second call if needed."""
     ctx = ctx or default_context()
     lib = ctx.lib
     xyz = np.ascontiguousarray(mesh.xyz, np.float32)
     origin = (ctypes.c_float * 3)()
     vs = ctypes.c_float()
     check(lib.ib_vox_framing(_f3(mesh.bmin), _f3(mesh.bmax), int(resolution), origin, ctypes.byref(vs)))
+    rgb = None if tri_colors is None else np.ascontiguousarray(tri_colors, np.float32).reshape(-1, 3)
+    if rgb is not None and len(rgb) != mesh.ntri:
+        raise ValueError(f"tri_colors holds {len(rgb)} colours for {mesh.ntri} triangles")
     count = ctypes.c_uint64()
-    check(lib.ib_voxelize(ctx.handle, _ptr(xyz) if mesh.ntri else None, mesh.ntri, _f3(mesh.bmin), _f3(mesh.bmax),
-                          int(resolution), None, None, 0, ctypes.byref(count)))
+    cap = max(1024, 4 * mesh.ntri)
+    for attempt in range(2):
+        cells = np.zeros((cap, 3), np.int32)
+        tri = np.zeros(cap, np.int32)
+        st = lib.ib_voxelize(ctx.handle, _ptr(xyz) if mesh.ntri else None, mesh.ntri, _f3(mesh.bmin),
+                             _f3(mesh.bmax), int(resolution), _ptr(cells), _ptr(tri), cap, ctypes.byref(count))
+        if st == IB_ERR_CAPACITY and attempt == 0:
+            cap = count.value
+            continue
+        check(st)
+        break
     n = count.value
-    cells = np.zeros((max(n, 1), 3), np.int32)
-    tri = np.zeros(max(n, 1), np.int32)
-    if n:
-        check(lib.ib_voxelize(ctx.handle, _ptr(xyz), mesh.ntri, _f3(mesh.bmin), _f3(mesh.bmax), int(resolution),
-                              _ptr(cells), _ptr(tri), n, ctypes.byref(count)))
-    return VoxelGrid(list(origin), vs.value, cells[:n], tri[:n])
+    # white triangles average to exactly 1.0 (n / n): only given colours need the second pass
+    colors = np.ones((max(n, 1), 3), np.float32)
+    if n and rgb is not None:
+        check(lib.ib_voxelize_colors(ctx.handle, _ptr(xyz), mesh.ntri, _f3(mesh.bmin), _f3(mesh.bmax),
+                                     int(resolution), _ptr(rgb), None, _ptr(colors), n,
+                                     ctypes.byref(count)))
+    g = VoxelGrid(list(origin), vs.value, cells[:n], tri[:n])
+    g.colors = colors[:n]
+    return g
 
 
 def weldPositions(positions, boundsMin, boundsMax, ctx=None):

@@ -247,6 +247,24 @@ void texturedTest() {
   expect(std::memcmp(got.data(), want.data(), want.size() * sizeof(float)) == 0,
          "device triangle materials equal the oracle's");
 
+  // five different materials over the faces: voxelizeMesh's cells and colour means equal the oracle's
+  {
+    std::vector<float> rgb(3 * ntri);
+    for (size_t t = 0; t < ntri; ++t)
+      for (int k = 0; k < 3; ++k) rgb[3 * t + k] = want[5 * t + k];
+    const float bmin[3] = {mesh.boundsMin.x, mesh.boundsMin.y, mesh.boundsMin.z};
+    const float bmax[3] = {mesh.boundsMax.x, mesh.boundsMax.y, mesh.boundsMax.z};
+    const size_t n = orc_voxelize_first_touch(&mesh.positions[0].x, ntri, rgb.data(), bmin, bmax, 13, nullptr,
+                                              nullptr, 0);
+    std::vector<int32_t> rc(3 * n);
+    std::vector<float> rcol(3 * n);
+    orc_voxelize_first_touch(&mesh.positions[0].x, ntri, rgb.data(), bmin, bmax, 13, rc.data(), rcol.data(), n);
+    const auto vg = italy_bake::voxelizeMesh(mesh, 13);
+    expect(n > 0 && vg.cells.size() == n && std::memcmp(vg.cells.data(), rc.data(), rc.size() * 4) == 0 &&
+               std::memcmp(vg.colors.data(), rcol.data(), rcol.size() * 4) == 0,
+           "voxelizeMesh cells and per-cell colours equal the oracle's");
+  }
+
   // one textured material on every triangle, all UVs on one texel: every cell carries its colour
   for (auto &m : mesh.triangleMaterial) m = 2;
   mesh.triangleMaterial.resize(ntri, 2);
