@@ -22,7 +22,6 @@ sys.path.insert(0, os.path.join(ROOT, "italy-rs_amd"))
 
 METRIC = "SDF voxel-queries/s (327k tris, 256³) batched vs plain; p99 bake latency"
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (published, with packed FP32)
-PEAK_FP32_NONPACKED_TFLOPS = 78.6  # without packed FP32, which this build disables
 PEAK_HBM_GBS = 8000.0
 # FLOPs per test, counted from the kernel source (ib_device.hpp; add/sub/mul = 1, fma = 2, min/max,
 # clamps and compares not counted): point/box distance 6 sub + 1 mul + 2 fma = 11; branch-free point/
@@ -652,7 +651,10 @@ def main():
         achieved = flops / (sdf_avg_ms * 1e-3) / 1e12
         roofline.update(achieved=achieved, frac=achieved / PEAK_FP32_TFLOPS, flops_per_launch=flops,
                         box_tests=counters["box_tests"], tri_tests=counters["tri_tests"],
-                        frac_of_non_packed_peak=achieved / PEAK_FP32_NONPACKED_TFLOPS)
+                        work_note="algorithmic FLOPs = box and triangle tests of a per-voxel closest-triangle "
+                                  "search over a binary LBVH with one triangle per leaf (counted by a counting run), "
+                                  "x FLOP_BOX and x FLOP_TRI FLOPs; the wave kernel shares candidates across a brick and culls "
+                                  "by oriented boxes, so it executes fewer tests than this count")
     alg_bytes = 4 * nvox + mesh.xyz.nbytes
     roofline["hbm_algorithmic_GBs"] = alg_bytes / (sdf_avg_ms * 1e-3) / 1e9
     # HBM-side bytes per launch, from the committed FETCH_SIZE / WRITE_SIZE profile of this kernel
@@ -663,6 +665,8 @@ def main():
             tr = json.load(f)
         roofline["traffic"] = tr["traffic_bytes"]
         roofline["traffic_source"] = "profiles/" + tr["source"]
+        if "valu_busy" in tr:  # executed work from the same profile's SQ counters
+            roofline["executed"] = {k: tr[k] for k in ("valu_insts", "valu_insts_per_brick", "valu_busy")}
 
     if args.copy_calibration_mib > 0:
         nbytes = args.copy_calibration_mib << 20
