@@ -70,7 +70,7 @@ typedef struct {
   uint64_t box_tests;    /* point/box distance tests of the closest-triangle traversal (counting mode) */
   uint64_t tri_tests;    /* point/triangle distance tests (counting mode) */
   uint64_t stack_overflows; /* traversal pushes that found the stack full: 0 unless the tree is malformed */
-  double ms_caps;        /* host time of the hole caps of open meshes (boundary read-back, chains, cap tree) */
+  double ms_caps;        /* device time of the hole caps of open meshes (loop labels, fans, cap tree), events */
   double ms_query;       /* the closest-triangle query kernel alone (dense bakes; HIP events around its launch) */
   uint64_t obb_tests;    /* point/oriented-box tests of leaf blocks in the per-brick query (counting mode) */
 } ib_stats;
@@ -237,7 +237,7 @@ ib_status ib_bake_dense_device(ib_context *ctx, const float *d_xyz, uint64_t ntr
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  No counterpart in the
  * reference, whose bakeSdf allocates per call (src/convert/sdf_baker.cpp:165-305).  Same arguments and
  * results (bit for bit) as ib_bake_dense_device; it calls no hipMalloc, hipFree or other allocating
- * device routine: every table of the bake lives in one fixed 96 MiB region per device, which the library
+ * device routine: every table of the bake lives in one fixed 128 MiB region per device, which the library
  * holds from the first ib_create on that device until the process ends.  Limits (IB_ERR_CAPACITY before
  * any work beyond them): at most IB_NOALLOC_MAX_TRIANGLES triangles, SDF grids of at most
  * IB_NOALLOC_MAX_COLUMNS columns (nx * ny) and IB_NOALLOC_MAX_VOXELS voxels; the occupancy grid is

@@ -48,7 +48,7 @@ struct Buf {
 
 }  // namespace
 
-enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_OCC_SIDE, EV_Q0, EV_Q1, EV_COUNT };
+enum EventId { EV_START = 0, EV_INGEST, EV_BVH, EV_OCC, EV_COLS, EV_SDF, EV_OCC_SIDE, EV_Q0, EV_Q1, EV_CAP0, EV_CAP1, EV_COUNT };
 
 struct ib_context {
   int device = 0;
@@ -218,170 +218,6 @@ bool grid_ok(const ib_grid *g) {
 }
 
 GridArgs to_args(const ib_grid &g) { return GridArgs{g.origin[0], g.origin[1], g.origin[2], g.h, g.nx, g.ny, g.nz}; }
-
-// Hole caps: the unmatched directed edges form closed chains (every vertex has as many outgoing as
-// incoming unmatched edges); each chain is closed by a fan from the centroid of its vertices.  Then
-// (mesh + reversed caps) is a closed surface, and w_mesh = w_closed + w_caps.  `edges` is sorted by
-// (from, to); pts holds the two endpoint positions of each edge (6 floats).  At a chain vertex the
-// walk takes the last unused edge leaving it.
-void build_caps(const std::vector<int2> &edges, const std::vector<float> &pts, std::vector<float> &caps) {
-  caps.clear();
-  const size_t ne = edges.size();
-  std::vector<int> cursor(ne, -1);  // at the first edge of each 'from' run: one past its last unused edge
-  for (size_t i = 0; i < ne; ++i)
-    if (i == 0 || edges[i].x != edges[i - 1].x) {
-      size_t j = i;
-      while (j < ne && edges[j].x == edges[i].x) ++j;
-      cursor[i] = (int)j;
-    }
-  auto run_start = [&](int v) -> int {
-    const auto it = std::lower_bound(edges.begin(), edges.end(), v, [](const int2 &e, int x) { return e.x < x; });
-    return (it != edges.end() && it->x == v) ? (int)(it - edges.begin()) : -1;
-  };
-  std::vector<char> used(ne, 0);
-  std::vector<int> loop;
-  for (size_t s = 0; s < ne; ++s) {
-    if (used[s]) continue;
-    loop.clear();
-    int e = (int)s;
-    const int start = edges[s].x;
-    for (;;) {
-      used[e] = 1;
-      loop.push_back(e);
-      const int v = edges[e].y;
-      if (v == start) break;
-      const int lo = run_start(v);
-      int next = -1;
-      if (lo >= 0) {
-        int &cur = cursor[lo];
-        while (cur > lo) {
-          const int c = --cur;
-          if (!used[c]) {
-            next = c;
-            break;
-          }
-        }
-      }
-      if (next < 0) break;  // cannot happen for a boundary cycle; close the fan anyway
-      e = next;
-    }
-    double cx = 0, cy = 0, cz = 0;
-    for (int id : loop) {
-      cx += pts[6 * (size_t)id];
-      cy += pts[6 * (size_t)id + 1];
-      cz += pts[6 * (size_t)id + 2];
-    }
-    const float apex[3] = {(float)(cx / loop.size()), (float)(cy / loop.size()), (float)(cz / loop.size())};
-    for (int id : loop) {
-      const float *p = &pts[6 * (size_t)id];
-      caps.insert(caps.end(), {apex[0], apex[1], apex[2], p[0], p[1], p[2], p[3], p[4], p[5]});
-    }
-  }
-}
-
-// Hierarchy over the cap triangles: median splits on the longest centroid axis, leaves of <= 4
-// triangles.  Node bounds are computed in double and rounded outwards so that the device's FP32
-// bound area / dmin^2 stays an upper bound.  Per-triangle sums, areas and area vectors are
-// computed once (CapTri) and aggregated per node.
-struct CapTri {
-  double sum[3];  // x + y + z of the three vertices, per axis (centroid * 3)
-  double lo[3], hi[3];
-  double n[3];    // area vector
-  double area;
-};
-
-int cap_node(const std::vector<float> &tris, const std::vector<CapTri> &ct, std::vector<int> &ids, int lo, int hi,
-             std::vector<CapNode> &nodes, std::vector<float> &out) {
-  const int me = (int)nodes.size();
-  nodes.emplace_back();
-  double cmin[3] = {1e300, 1e300, 1e300}, cmax[3] = {-1e300, -1e300, -1e300};
-  double bl[3] = {1e300, 1e300, 1e300}, bh[3] = {-1e300, -1e300, -1e300};
-  double area = 0, dip[3] = {0, 0, 0}, ac[3] = {0, 0, 0};
-  for (int i = lo; i < hi; ++i) {
-    const CapTri &t = ct[ids[i]];
-    for (int k = 0; k < 3; ++k) {
-      const double cen = t.sum[k] / 3.0;
-      cmin[k] = std::min(cmin[k], cen);
-      cmax[k] = std::max(cmax[k], cen);
-      bl[k] = std::min(bl[k], t.lo[k]);
-      bh[k] = std::max(bh[k], t.hi[k]);
-    }
-    area += t.area;
-    for (int k = 0; k < 3; ++k) {
-      dip[k] += t.n[k];
-      ac[k] += t.area * t.sum[k] / 3.0;
-    }
-  }
-  double c[3];
-  for (int k = 0; k < 3; ++k) c[k] = area > 0 ? ac[k] / area : 0.5 * (bl[k] + bh[k]);
-  double r = 0;
-  for (int i = lo; i < hi; ++i) {
-    const float *t = &tris[9 * (size_t)ids[i]];
-    for (int v = 0; v < 3; ++v) {
-      const double dx = t[3 * v] - c[0], dy = t[3 * v + 1] - c[1], dz = t[3 * v + 2] - c[2];
-      r = std::max(r, dx * dx + dy * dy + dz * dz);
-    }
-  }
-  r = std::sqrt(r);
-  CapNode nd{};
-  for (int k = 0; k < 3; ++k) {
-    nd.c[k] = (float)c[k];
-    nd.dip[k] = (float)dip[k];
-    nd.lo[k] = std::nextafter((float)bl[k], -INFINITY);
-    nd.hi[k] = std::nextafter((float)bh[k], INFINITY);
-  }
-  // the float centre moved by at most one ulp per axis: widen r by that, then by a relative margin
-  nd.r = (float)((r + 1e-6 * (std::fabs(c[0]) + std::fabs(c[1]) + std::fabs(c[2]))) * (1.0 + 1e-5));
-  nd.area = (float)(area * (1.0 + 1e-5));
-  if (hi - lo <= 4) {
-    nd.left = -1 - (int)(out.size() / 9);
-    nd.right = hi - lo;
-    for (int i = lo; i < hi; ++i) out.insert(out.end(), &tris[9 * (size_t)ids[i]], &tris[9 * (size_t)ids[i]] + 9);
-    nodes[me] = nd;
-    return me;
-  }
-  int axis = 0;
-  for (int k = 1; k < 3; ++k)
-    if (cmax[k] - cmin[k] > cmax[axis] - cmin[axis]) axis = k;
-  const int mid = (lo + hi) / 2;
-  std::nth_element(ids.begin() + lo, ids.begin() + mid, ids.begin() + hi, [&](int a, int b) {
-    const double ka = ct[a].sum[axis], kb = ct[b].sum[axis];
-    return ka != kb ? ka < kb : a < b;
-  });
-  const int l = cap_node(tris, ct, ids, lo, mid, nodes, out);
-  const int rr = cap_node(tris, ct, ids, mid, hi, nodes, out);
-  nd.left = l;
-  nd.right = rr;
-  nodes[me] = nd;
-  return me;
-}
-
-void build_cap_hierarchy(const std::vector<float> &tris, std::vector<CapNode> &nodes, std::vector<float> &ordered) {
-  nodes.clear();
-  ordered.clear();
-  const int n = (int)(tris.size() / 9);
-  if (n == 0) return;
-  std::vector<CapTri> ct(n);
-  for (int i = 0; i < n; ++i) {
-    const float *t = &tris[9 * (size_t)i];
-    CapTri &c = ct[i];
-    double e1[3], e2[3];
-    for (int k = 0; k < 3; ++k) {
-      c.sum[k] = (double)t[k] + t[3 + k] + t[6 + k];
-      c.lo[k] = std::min((double)t[k], std::min((double)t[3 + k], (double)t[6 + k]));
-      c.hi[k] = std::max((double)t[k], std::max((double)t[3 + k], (double)t[6 + k]));
-      e1[k] = (double)t[3 + k] - t[k];
-      e2[k] = (double)t[6 + k] - t[k];
-    }
-    c.n[0] = 0.5 * (e1[1] * e2[2] - e1[2] * e2[1]);
-    c.n[1] = 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]);
-    c.n[2] = 0.5 * (e1[0] * e2[1] - e1[1] * e2[0]);
-    c.area = std::sqrt(c.n[0] * c.n[0] + c.n[1] * c.n[1] + c.n[2] * c.n[2]);
-  }
-  std::vector<int> ids(n);
-  for (int i = 0; i < n; ++i) ids[i] = i;
-  cap_node(tris, ct, ids, 0, n, nodes, ordered);
-}
 
 float ev_ms(ib_context *c, EventId a, EventId b) {
   if (!c->ev_valid[a] || !c->ev_valid[b]) return 0.0f;
@@ -590,6 +426,93 @@ void enqueue_occupancy(const float *d_xyz, int n, const ib_grid &occ, uint32_t *
   launch_occupancy(d_xyz, n, to_args(occ), d_occ, s);
 }
 
+// Hole caps of the ne unmatched edges `bnd` (vertex ids = first corner of each welded position), built on
+// the stream without a host round trip: the loops are labelled on the device, each closed by a fan from its
+// mean start vertex, and the fans get their tree (the kernels' comments give the steps).  The cap soup and
+// tree come from `arena` (or the context's scratch); the temporaries from the context's scratch, one work
+// region carved per step (offsets in units of the edge count rounded up to 64), free again once the stream
+// has run the steps (every user of the scratch runs on the same stream or after it).
+ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint32_t ne, const Scalars &h, MeshDev &m,
+                       Arena *arena, hipStream_t s) {
+  if (ne == 0) return IB_OK;
+  const size_t st = ((size_t)ne + 63) & ~size_t(63);
+  char *w;
+  IB_TRY(c->scratch("cap_work", 108 * st, &w));
+  auto u64at = [&](size_t off) { return reinterpret_cast<unsigned long long *>(w + off * st); };
+  auto u32at = [&](size_t off) { return reinterpret_cast<uint32_t *>(w + off * st); };
+  unsigned long long *ek0 = u64at(0), *ek1 = u64at(8), *ik0 = u64at(16), *ik1 = u64at(24);
+  uint32_t *iv0 = u32at(32), *iv1 = u32at(36);
+  unsigned long long *acc = u64at(40);
+  float *tmp = reinterpret_cast<float *>(w + 72 * st);
+  uint32_t *bounds;
+  IB_TRY(c->scratch("cap_bounds", 8, &bounds));
+  IB_TRY(take(c, arena, "caps", 9ull * ne, &m.caps));
+  m.ncap = (int)ne;
+  m.ncapnode = ne > 1 ? (int)ne - 1 : 1;
+  IB_TRY(take(c, arena, "cap_nodes", (size_t)m.ncapnode, &m.cap_nodes));
+  // vertex ids are corner indices below 3n: keys of 32 + bits(3n - 1) bits
+  int idbits = 1;
+  while (idbits < 32 && ((uint64_t)3 * m.n - 1) >> idbits) ++idbits;
+  int labbits = 1;
+  while (labbits < 32 && ((uint64_t)ne - 1) >> labbits) ++labbits;
+  // the largest sort space of the three sorts below
+  size_t tb = 0, t1 = 0, t2 = 0, t3 = 0;
+  IB_HIP(sort_keys_u64(nullptr, t1, ek0, ek1, ne, 32 + idbits, s));
+  IB_HIP(sort_pairs_u64(nullptr, t2, ik0, ik1, iv0, iv1, ne, 32 + idbits, s));
+  IB_HIP(sort_pairs_u32(nullptr, t3, u32at(16), u32at(24), reinterpret_cast<int32_t *>(iv0),
+                        reinterpret_cast<int32_t *>(u32at(28)), ne, labbits, s));
+  tb = std::max(t1, std::max(t2, t3));
+  size_t t4 = 0;
+  IB_HIP(sort_pairs(nullptr, t4, iv0, u32at(24), reinterpret_cast<int32_t *>(iv1),
+                    reinterpret_cast<int32_t *>(u32at(28)), (int)ne, s));
+  tb = std::max(tb, t4);
+  char *temp;
+  IB_TRY(c->scratch("cap_sort", tb, &temp));
+  // 1. edges in (from, to) order; 2. the same edges in (to, from) order with their (from, to) positions
+  launch_cap_keys(bnd, ne, ek0, s);
+  size_t tbx = tb;
+  IB_HIP(sort_keys_u64(temp, tbx, ek0, ek1, ne, 32 + idbits, s));
+  launch_cap_in_keys(ek1, ne, ik0, iv0, s);
+  tbx = tb;
+  IB_HIP(sort_pairs_u64(temp, tbx, ik0, ik1, iv0, iv1, ne, 32 + idbits, s));
+  // 3. successors and loop labels (ping-pong between the free halves of ik0 and ek0)
+  uint32_t *lab[2] = {u32at(16), u32at(0)}, *jmp[2] = {u32at(20), u32at(4)};
+  launch_cap_succ(ek1, ik1, iv1, ne, lab[0], jmp[0], s);
+  int cur = 0;
+  for (uint64_t span = 2; span < ne; span = 2 * span - 1) {  // labels cover span edges of each loop
+    launch_cap_jump(lab[cur], jmp[cur], ne, lab[cur ^ 1], jmp[cur ^ 1], s);
+    cur ^= 1;
+  }
+  const uint32_t *label = lab[cur];
+  // 4. edges sorted by label (into ik1's space), the loops' fixed-point sums
+  uint32_t *slab = u32at(24), *sedge = u32at(28);
+  launch_iota(iv0, ne, s);
+  tbx = tb;
+  IB_HIP(sort_pairs_u32(temp, tbx, label, slab, reinterpret_cast<const int32_t *>(iv0),
+                        reinterpret_cast<int32_t *>(sedge), ne, labbits, s));
+  FillList fl{};
+  fl.add(bounds, 0xffffffffu, 3);
+  fl.add(bounds + 3, 0u, 3);
+  fl.add(reinterpret_cast<uint32_t *>(acc), 0u, 8ull * ne);
+  fill_multi(fl, s);
+  launch_cap_bounds(ek1, ne, d_xyz, bounds, s);
+  launch_cap_sums(slab, sedge, ne, ek1, d_xyz, bounds, acc, s);
+  // 5. one fan triangle per edge, sorted by the Morton code of its centroid, into the mesh's cap soup
+  uint32_t *codes = iv0;
+  int32_t *idx = reinterpret_cast<int32_t *>(iv1);
+  launch_cap_tris(ek1, label, ne, d_xyz, bounds, acc, h.cmin, h.cmax, tmp, codes, idx, s);
+  uint32_t *codes_s = u32at(24);
+  int32_t *order = reinterpret_cast<int32_t *>(u32at(28));
+  tbx = tb;
+  IB_HIP(sort_pairs(temp, tbx, codes, codes_s, idx, order, (int)ne, s));
+  launch_cap_gather(tmp, order, ne, m.caps, s);
+  // 6. the tree (children, parents and ranges in acc's space)
+  int2 *child = reinterpret_cast<int2 *>(w + 40 * st), *ranges = reinterpret_cast<int2 *>(w + 48 * st);
+  int32_t *parent = reinterpret_cast<int32_t *>(w + 56 * st);
+  launch_cap_tree(m.caps, codes_s, (int)ne, child, parent, ranges, m.cap_nodes, s);
+  return IB_OK;
+}
+
 // after the scalars are on the host: validity, depth, and hole caps for open meshes.  bnd_valid:
 // the "e_bnd" scratch still holds this mesh's boundary edges (false in a batch -> re-weld).
 ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &h, MeshDev &m, Arena *arena,
@@ -604,7 +527,7 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   m.ncap = 0;
   m.caps = nullptr;
   if (h.boundary == 0) return IB_OK;
-  const auto t_caps = std::chrono::steady_clock::now();
+  if (!c->ev_valid[EV_CAP0]) c->record(EV_CAP0, s);
   int2 *bnd;
   if (!bnd_valid) {
     Scalars *sc2;
@@ -614,39 +537,9 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   } else {
     IB_TRY(c->scratch("e_bnd", 3ull * n, &bnd));
   }
-  // only the boundary edges and their endpoint positions come to the host
   const uint32_t ne = std::min<uint32_t>(h.boundary, 3u * n);
-  float *d_pts;
-  IB_TRY(c->scratch("e_pts", 6ull * ne, &d_pts));
-  launch_edge_points(bnd, ne, d_xyz, d_pts, s);
-  std::vector<int2> raw(ne);
-  std::vector<float> raw_pts(6ull * ne);
-  IB_HIP(hipMemcpyAsync(raw.data(), bnd, raw.size() * sizeof(int2), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipMemcpyAsync(raw_pts.data(), d_pts, raw_pts.size() * sizeof(float), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipStreamSynchronize(s));
-  std::vector<int> perm(ne);
-  for (uint32_t i = 0; i < ne; ++i) perm[i] = (int)i;
-  std::sort(perm.begin(), perm.end(), [&](int a, int b) {
-    return raw[a].x != raw[b].x ? raw[a].x < raw[b].x : raw[a].y < raw[b].y;
-  });
-  std::vector<int2> edges(ne);
-  std::vector<float> pts(6ull * ne);
-  for (uint32_t i = 0; i < ne; ++i) {
-    edges[i] = raw[perm[i]];
-    std::copy(&raw_pts[6 * (size_t)perm[i]], &raw_pts[6 * (size_t)perm[i]] + 6, &pts[6 * (size_t)i]);
-  }
-  std::vector<float> caps, ordered;
-  build_caps(edges, pts, caps);
-  std::vector<CapNode> nodes;
-  build_cap_hierarchy(caps, nodes, ordered);
-  m.ncap = (int)(ordered.size() / 9);
-  m.ncapnode = (int)nodes.size();
-  IB_TRY(take(c, arena, "caps", ordered.size(), &m.caps));
-  IB_TRY(take(c, arena, "cap_nodes", nodes.size(), &m.cap_nodes));
-  IB_HIP(hipMemcpyAsync(m.caps, ordered.data(), ordered.size() * sizeof(float), hipMemcpyHostToDevice, s));
-  IB_HIP(hipMemcpyAsync(m.cap_nodes, nodes.data(), nodes.size() * sizeof(CapNode), hipMemcpyHostToDevice, s));
-  IB_HIP(hipStreamSynchronize(s));
-  c->stats.ms_caps += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_caps).count();
+  IB_TRY(enqueue_caps(c, d_xyz, bnd, ne, h, m, arena, s));
+  c->record(EV_CAP1, s);
   return IB_OK;
 }
 
@@ -781,7 +674,7 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
 // the library's code object, so every process that loads the library holds kNoAllocScratch bytes on each
 // device it uses until it exits.  Host code never names the array except to look up its device address
 // (noalloc_scratch_ptr); kernels get that address.  All calls on one device share the region.
-constexpr size_t kNoAllocScratch = 96ull << 20;
+constexpr size_t kNoAllocScratch = 128ull << 20;
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratch];
 constexpr int kMaxDevices = 64;
 static void *g_noalloc_ptr[kMaxDevices] = {};
@@ -1114,9 +1007,9 @@ static ib_status voxelize_sparse(ib_context *c, const float *xyz, uint64_t ntri,
   if (want_colors) IB_TRY(c->scratch("vx_rgb", 3ull * nruns, &rgb));
   launch_vox_runs(cell_s, tri_s, npairs, run_of, d_rgb_in, keys, vals, rgb, s);
   tb = 0;
-  IB_HIP(sort_pairs_u64(nullptr, tb, keys, keys_s, vals, perm, nruns, s));
+  IB_HIP(sort_pairs_u64(nullptr, tb, keys, keys_s, vals, perm, nruns, 64, s));
   IB_TRY(c->scratch("vx_temp", tb, &temp));
-  IB_HIP(sort_pairs_u64(temp, tb, keys, keys_s, vals, perm, nruns, s));
+  IB_HIP(sort_pairs_u64(temp, tb, keys, keys_s, vals, perm, nruns, 64, s));
   int32_t *d_cells = nullptr, *d_tri = nullptr;
   float *d_colors = nullptr;
   if (cells) IB_TRY(c->scratch("vx_cells", 3ull * nruns, &d_cells));
@@ -1660,6 +1553,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
   st.ms_columns = ev_ms(c, EV_OCC, EV_COLS);
   st.ms_sdf = ev_ms(c, EV_COLS, EV_SDF);
   st.ms_query = ev_ms(c, EV_Q0, EV_Q1);
+  st.ms_caps = ev_ms(c, EV_CAP0, EV_CAP1);
   st.ms_total = c->ev_valid[EV_SDF]   ? ev_ms(c, EV_START, EV_SDF)
                 : c->ev_valid[EV_OCC] ? ev_ms(c, EV_START, EV_OCC)
                                       : ev_ms(c, EV_START, EV_INGEST);

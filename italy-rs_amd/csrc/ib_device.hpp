@@ -54,9 +54,10 @@ struct alignas(16) Obb {
 };
 static_assert(sizeof(Obb) == 64, "oriented box record is one 64-byte line");
 
-// Node of the hole-cap hierarchy (built on the host).  left >= 0: children left/right;
-// left < 0: leaf holding triangles [-1 - left, -1 - left + right) of the reordered cap soup.
-// (c, r) bounds every vertex below; area = sum of |triangle area|; dip = sum of area vectors.
+// Node of the hole-cap hierarchy.  Leaf (left < 0 and right < 0): fans [-1 - left, -1 - left - right)
+// of the sorted cap soup.  Inner node: left / right are child codes, a node index (>= 0) or a single fan
+// ~index (< 0), at most one of them a fan.  (c, r) bounds every vertex below; lo / hi is the box;
+// area = sum of |fan area|; dip = sum of area vectors.
 struct alignas(16) CapNode {
   float c[3], r;
   float dip[3], area;
@@ -66,7 +67,10 @@ struct alignas(16) CapNode {
   int32_t right;
 };
 static_assert(sizeof(CapNode) == 64, "cap node record is one 64-byte line");
-constexpr int kCapStack = 48;
+// leaves hold at most kCapLeaf fans; the tree's depth is at most 30 + ceil(log2 fans) (Karras tree over
+// 30-bit codes split by sorted position), so a traversal that keeps one pending sibling per level fits 64
+constexpr int kCapLeaf = 4;
+constexpr int kCapStack = 64;
 // a cap node farther than this bound on |its solid angle| / 4pi is not evaluated, only bounded
 constexpr float kCapTau = 1e-3f;
 

@@ -84,14 +84,11 @@ __global__ __launch_bounds__(256) void k_prep(const float *__restrict__ xyz, int
     }
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ xyz, int n, const Scalars *sc,
-                                                uint32_t *codes, int32_t *idx) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const float *t = xyz + 9 * (size_t)i;
+// 30-bit Morton code of the centroid of triangle t (9 floats) over the centroid bounds (ordered-int encoded)
+__device__ __forceinline__ uint32_t centroid_code(const float *t, const uint32_t *cmin, const uint32_t *cmax) {
   uint32_t q[3];
   for (int k = 0; k < 3; ++k) {
-    const float lo = o2f(sc->cmin[k]), hi = o2f(sc->cmax[k]);
+    const float lo = o2f(cmin[k]), hi = o2f(cmax[k]);
     const float ext = hi - lo;
     const float c = ((t[k] + t[3 + k]) + t[6 + k]) / 3.0f;
     const float u = ext > 0.0f ? (c - lo) / ext : 0.5f;
@@ -99,7 +96,15 @@ __global__ __launch_bounds__(256) void k_morton(const float *__restrict__ xyz, i
     s = s < 0.0f ? 0.0f : (s > 1023.0f ? 1023.0f : s);
     q[k] = (uint32_t)s;
   }
-  codes[i] = (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+  return (expand10(q[0]) << 2) | (expand10(q[1]) << 1) | expand10(q[2]);
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float *__restrict__ xyz, int n, const Scalars *sc,
+                                                uint32_t *codes, int32_t *idx) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float *t = xyz + 9 * (size_t)i;
+  codes[i] = centroid_code(t, sc->cmin, sc->cmax);
   idx[i] = i;
 }
 
@@ -114,10 +119,9 @@ __device__ __forceinline__ int kdelta(const uint32_t *codes, int n, int i, int j
   return __clzll((long long)(a ^ b));
 }
 
-__global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ codes, int n, Node *nodes,
-                                                int32_t *leaf_parent, int2 *ranges) {
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n - 1) return;
+// children (an internal node index, or ~leaf) and covered range [lo, hi] of internal node i
+__device__ __forceinline__ void karras_node(const uint32_t *__restrict__ codes, int n, int i, int &lc, int &rc,
+                                            int &lo, int &hi) {
   const int d = (kdelta(codes, n, i, i + 1) - kdelta(codes, n, i, i - 1)) >= 0 ? 1 : -1;
   const int dmin = kdelta(codes, n, i, i - d);
   int lmax = 2;
@@ -133,9 +137,18 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ cod
     if (kdelta(codes, n, i, i + (s + t) * d) > dnode) s += t;
   } while (t > 1);
   const int gamma = i + s * d + (d < 0 ? d : 0);
-  const int lo = min(i, j), hi = max(i, j);
-  const int lc = (lo == gamma) ? ~gamma : gamma;
-  const int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  lo = min(i, j);
+  hi = max(i, j);
+  lc = (lo == gamma) ? ~gamma : gamma;
+  rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t *__restrict__ codes, int n, Node *nodes,
+                                                int32_t *leaf_parent, int2 *ranges) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n - 1) return;
+  int lc, rc, lo, hi;
+  karras_node(codes, n, i, lc, rc, lo, hi);
   nodes[i].c0 = lc;
   nodes[i].c1 = rc;
   nodes[i].pad = 0;
@@ -659,6 +672,281 @@ __global__ __launch_bounds__(256) void k_edge_collect(const unsigned long long *
 }
 
 // ============================================================================================
+// Hole caps.  The unmatched directed edges (from, to) of an open mesh form closed loops: at every
+// vertex as many of them leave as enter (each triangle enters and leaves each of its corners once, and
+// a matched pair removes one leaving and one entering edge).  Each loop is closed by a fan from the mean
+// of its start vertices; (mesh + reversed fans) is then a closed surface, and the mesh's winding number
+// is the closed surface's plus the fans'.  The fans get a hierarchy (Karras tree over their centroids'
+// Morton codes) with per-node bounding sphere, box, area and area vector.
+// ============================================================================================
+using u64 = unsigned long long;
+
+__device__ __forceinline__ u64 shfl_down_u64(u64 v, int off) {
+  const uint32_t lo = __shfl_down((uint32_t)v, off), hi = __shfl_down((uint32_t)(v >> 32), off);
+  return ((u64)hi << 32) | lo;
+}
+
+// sort keys (from << 32 | to) of the unmatched edges
+__global__ __launch_bounds__(256) void k_cap_keys(const int2 *__restrict__ bnd, uint32_t ne, u64 *keys) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ne) return;
+  const int2 e = bnd[i];
+  keys[i] = ((u64)(uint32_t)e.x << 32) | (uint32_t)e.y;
+}
+
+// for every edge, in (from, to) order: the key (to << 32 | from) and its position
+__global__ __launch_bounds__(256) void k_cap_in_keys(const u64 *__restrict__ ek, uint32_t ne, u64 *ik, uint32_t *iv) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ne) return;
+  const u64 k = ek[i];
+  ik[i] = (k << 32) | (k >> 32);
+  iv[i] = i;
+}
+
+__device__ __forceinline__ uint32_t lower_bound_u64(const u64 *__restrict__ a, uint32_t n, u64 key) {
+  uint32_t lo = 0, hi = n;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// The successor of every edge along its loop: the r-th edge entering vertex v (in (to, from) order)
+// continues with the r-th edge leaving v (in (from, to) order); any such pairing closes the loops.
+// Starts the labelling: lab = least edge index of the edge and its successor, jmp = successor.
+__global__ __launch_bounds__(256) void k_cap_succ(const u64 *__restrict__ ek, const u64 *__restrict__ ik,
+                                                  const uint32_t *__restrict__ iv, uint32_t ne, uint32_t *lab,
+                                                  uint32_t *jmp) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p >= ne) return;
+  const uint32_t e = iv[p];
+  const uint32_t v = (uint32_t)(ik[p] >> 32);
+  const u64 key = (u64)v << 32;
+  const uint32_t in0 = lower_bound_u64(ik, ne, key), out0 = lower_bound_u64(ek, ne, key);
+  const uint32_t idx = out0 + (p - in0);
+  // the balance above makes idx an edge leaving v; an edge without one would close on itself
+  const uint32_t nxt = (idx < ne && (uint32_t)(ek[idx] >> 32) == v) ? idx : e;
+  lab[e] = min(e, nxt);
+  jmp[e] = nxt;
+}
+
+// Pointer jumping: after step t every edge holds the least index among the next 2^t + 1 edges of its loop
+// (the loop's least edge index, its label, once 2^t + 1 reaches the loop's length).
+__global__ __launch_bounds__(256) void k_cap_jump(const uint32_t *__restrict__ lab0, const uint32_t *__restrict__ jmp0,
+                                                  uint32_t ne, uint32_t *lab1, uint32_t *jmp1) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= ne) return;
+  const uint32_t j = jmp0[i];
+  lab1[i] = min(lab0[i], lab0[j]);
+  jmp1[i] = jmp0[j];
+}
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t *v, uint32_t n) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+// bounds of the edges' start vertices, ordered-int encoded (bounds[0..2] min, [3..5] max; cleared to
+// 0xffffffff and 0 first): the frame of the fixed-point sums
+__global__ __launch_bounds__(256) void k_cap_bounds(const u64 *__restrict__ ek, uint32_t ne,
+                                                    const float *__restrict__ xyz, uint32_t *bounds) {
+  uint32_t mn[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, mx[3] = {0u, 0u, 0u};
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < ne; i += gridDim.x * 256) {
+    const float *p = xyz + 3 * (size_t)(uint32_t)(ek[i] >> 32);
+    for (int k = 0; k < 3; ++k) {
+      const uint32_t o = f2o(p[k]);
+      mn[k] = min(mn[k], o);
+      mx[k] = max(mx[k], o);
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1)
+    for (int k = 0; k < 3; ++k) {
+      mn[k] = min(mn[k], (uint32_t)__shfl_xor(mn[k], off));
+      mx[k] = max(mx[k], (uint32_t)__shfl_xor(mx[k], off));
+    }
+  if ((threadIdx.x & 63) == 0)
+    for (int k = 0; k < 3; ++k) {
+      atomicMin(&bounds[k], mn[k]);
+      atomicMax(&bounds[3 + k], mx[k]);
+    }
+}
+
+// fixed-point frame of axis k: x -> (x - lo) * 2^32 / (hi - lo), in [0, 2^32]
+__device__ __forceinline__ void cap_frame(const uint32_t *bounds, int k, double &lo, double &scale) {
+  lo = (double)o2f(bounds[k]);
+  const double ext = (double)o2f(bounds[3 + k]) - lo;
+  scale = ext > 0.0 ? 4294967296.0 / ext : 0.0;
+}
+
+// Per loop: the sums of its start vertices in that fixed point, and its edge count (acc[4 * label + 0..3]).
+// Integer sums do not depend on the order of the atomics.  The edges come sorted by label, so each wave
+// first adds up its runs of equal label (segmented suffix sums) and the first lane of each run adds it.
+__global__ __launch_bounds__(256) void k_cap_sums(const uint32_t *__restrict__ slab, const uint32_t *__restrict__ sedge,
+                                                  uint32_t ne, const u64 *__restrict__ ek,
+                                                  const float *__restrict__ xyz, const uint32_t *__restrict__ bounds,
+                                                  u64 *acc) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const bool valid = i < ne;
+  const uint32_t L = valid ? slab[i] : 0xffffffffu;
+  u64 v[4] = {0, 0, 0, 0};
+  if (valid) {
+    const float *p = xyz + 3 * (size_t)(uint32_t)(ek[sedge[i]] >> 32);
+    for (int k = 0; k < 3; ++k) {
+      double lo, scale;
+      cap_frame(bounds, k, lo, scale);
+      const double q = fmin(fmax(((double)p[k] - lo) * scale, 0.0), 4294967296.0);
+      v[k] = (u64)rint(q);
+    }
+    v[3] = 1;
+  }
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t Lo = __shfl_down(L, off);
+    u64 o[4];
+    for (int k = 0; k < 4; ++k) o[k] = shfl_down_u64(v[k], off);
+    if (lane + off < 64 && Lo == L)
+      for (int k = 0; k < 4; ++k) v[k] += o[k];
+  }
+  const uint32_t Lp = __shfl_up(L, 1);
+  if (valid && (lane == 0 || Lp != L))
+    for (int k = 0; k < 4; ++k) atomicAdd(&acc[4 * (size_t)L + k], v[k]);
+}
+
+struct CapCodeArgs {
+  uint32_t cmin[3], cmax[3];  // the mesh's centroid bounds (ordered-int encoded)
+};
+
+// The fan triangle of every edge: its loop's apex (the mean start vertex), the edge's start, its end;
+// and the Morton code of the triangle's centroid over the mesh's centroid bounds.
+__global__ __launch_bounds__(256) void k_cap_tris(const u64 *__restrict__ ek, const uint32_t *__restrict__ lab,
+                                                  uint32_t ne, const float *__restrict__ xyz,
+                                                  const uint32_t *__restrict__ bounds, const u64 *__restrict__ acc,
+                                                  CapCodeArgs ca, float *tris, uint32_t *codes, int32_t *idx) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= ne) return;
+  const uint32_t L = lab[e];
+  const double cnt = (double)acc[4 * (size_t)L + 3];
+  float t[9];
+  for (int k = 0; k < 3; ++k) {
+    double lo, scale;
+    cap_frame(bounds, k, lo, scale);
+    t[k] = scale > 0.0 ? (float)(lo + ((double)acc[4 * (size_t)L + k] / cnt) / scale) : (float)lo;
+  }
+  const u64 key = ek[e];
+  const float *a = xyz + 3 * (size_t)(uint32_t)(key >> 32), *b = xyz + 3 * (size_t)(uint32_t)key;
+  for (int k = 0; k < 3; ++k) {
+    t[3 + k] = a[k];
+    t[6 + k] = b[k];
+  }
+  for (int k = 0; k < 9; ++k) tris[9 * (size_t)e + k] = t[k];
+  codes[e] = centroid_code(t, ca.cmin, ca.cmax);
+  idx[e] = (int32_t)e;
+}
+
+__global__ __launch_bounds__(256) void k_cap_gather(const float *__restrict__ tmp, const int32_t *__restrict__ order,
+                                                    uint32_t ne, float *caps) {
+  const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+  if (k >= ne) return;
+  const float *src = tmp + 9 * (size_t)order[k];
+  for (int j = 0; j < 9; ++j) caps[9 * (size_t)k + j] = src[j];
+}
+
+// Karras tree of the sorted fans: children codes (internal index, or ~triangle), parent (-1 at the root),
+// covered range
+__global__ __launch_bounds__(256) void k_cap_karras(const uint32_t *__restrict__ codes, int n, int2 *child,
+                                                    int32_t *parent, int2 *ranges) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n - 1) return;
+  int lc, rc, lo, hi;
+  karras_node(codes, n, i, lc, rc, lo, hi);
+  child[i] = make_int2(lc, rc);
+  ranges[i] = make_int2(lo, hi);
+  if (lc >= 0) parent[lc] = i;
+  if (rc >= 0) parent[rc] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+// One wave per internal node the traversal reaches (the root and every node whose parent covers more
+// than kCapLeaf fans): FP64 sums over the node's fans, bounds rounded outwards so that the FP32 far-field
+// bound area / dmin^2 stays an upper bound.  A node of at most kCapLeaf fans becomes a leaf (left =
+// -1 - first, right = -count); an inner node keeps its children's codes, of which at most one is a single
+// fan (~index).  With one fan, record 0 is a leaf of that fan.
+__global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ caps, const int2 *__restrict__ child,
+                                                   const int32_t *__restrict__ parent,
+                                                   const int2 *__restrict__ ranges, int n, CapNode *nodes) {
+  const int rec = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (rec >= (n > 1 ? n - 1 : 1)) return;  // wave-uniform
+  int lo = 0, hi = 0;
+  if (n > 1) {
+    lo = ranges[rec].x;
+    hi = ranges[rec].y;
+    const int p = parent[rec];
+    if (p >= 0 && ranges[p].y - ranges[p].x + 1 <= kCapLeaf) return;
+  }
+  double area = 0.0, ac[3] = {0.0, 0.0, 0.0}, dip[3] = {0.0, 0.0, 0.0};
+  double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int t = lo + lane; t <= hi; t += 64) {
+    const float *v = caps + 9 * (size_t)t;
+    double e1[3], e2[3], sum[3];
+    for (int k = 0; k < 3; ++k) {
+      sum[k] = ((double)v[k] + v[3 + k]) + v[6 + k];
+      bl[k] = fmin(bl[k], fmin((double)v[k], fmin((double)v[3 + k], (double)v[6 + k])));
+      bh[k] = fmax(bh[k], fmax((double)v[k], fmax((double)v[3 + k], (double)v[6 + k])));
+      e1[k] = (double)v[3 + k] - v[k];
+      e2[k] = (double)v[6 + k] - v[k];
+    }
+    const double nv[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                          0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
+    const double a = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    area += a;
+    for (int k = 0; k < 3; ++k) {
+      dip[k] += nv[k];
+      ac[k] += a * sum[k] / 3.0;
+    }
+  }
+  area = wave_sum(area);
+  for (int k = 0; k < 3; ++k) {
+    ac[k] = wave_sum(ac[k]);
+    dip[k] = wave_sum(dip[k]);
+    bl[k] = wave_min(bl[k]);
+    bh[k] = wave_max(bh[k]);
+  }
+  double c[3];
+  for (int k = 0; k < 3; ++k) c[k] = area > 0.0 ? ac[k] / area : 0.5 * (bl[k] + bh[k]);
+  double r2 = 0.0;
+  for (int t = lo + lane; t <= hi; t += 64) {
+    const float *v = caps + 9 * (size_t)t;
+    for (int j = 0; j < 3; ++j) {
+      const double dx = v[3 * j] - c[0], dy = v[3 * j + 1] - c[1], dz = v[3 * j + 2] - c[2];
+      r2 = fmax(r2, dx * dx + dy * dy + dz * dz);
+    }
+  }
+  r2 = wave_max(r2);
+  if (lane != 0) return;
+  CapNode nd;
+  for (int k = 0; k < 3; ++k) {
+    nd.c[k] = (float)c[k];
+    nd.dip[k] = (float)dip[k];
+    nd.lo[k] = nextafterf((float)bl[k], -INFINITY);
+    nd.hi[k] = nextafterf((float)bh[k], INFINITY);
+  }
+  // the float centre moved by at most one ulp per axis: widen r by that, then by a relative margin
+  nd.r = (float)((sqrt(r2) + 1e-6 * (fabs(c[0]) + fabs(c[1]) + fabs(c[2]))) * (1.0 + 1e-5));
+  nd.area = (float)(area * (1.0 + 1e-5));
+  const int size = hi - lo + 1;
+  if (size <= kCapLeaf) {
+    nd.left = -1 - lo;
+    nd.right = -size;
+  } else {
+    nd.left = child[rec].x;
+    nd.right = child[rec].y;
+  }
+  nodes[rec] = nd;
+}
+
+// ============================================================================================
 // surface occupancy: voxelize.cpp:64-100 per triangle, bit-packed with atomicOr
 // ============================================================================================
 // inclusive cell range of a triangle's AABB: floor((min - origin) / h) clamped with
@@ -944,14 +1232,24 @@ __device__ __forceinline__ void for_column_caps(const ColumnArgs &a, float X, fl
   int sp = 0;
   stack[sp++] = 0;
   while (sp > 0) {
-    const CapNode nd = a.cap_nodes[stack[--sp]];
+    const int code = stack[--sp];
+    if (code < 0) {  // a single fan
+      const float *t = a.caps + 9 * (size_t)~code;
+      if (X >= fminf(fminf(t[0], t[3]), t[6]) && X <= fmaxf(fmaxf(t[0], t[3]), t[6]) &&
+          Y >= fminf(fminf(t[1], t[4]), t[7]) && Y <= fmaxf(fmaxf(t[1], t[4]), t[7]))
+        tri(t);
+      continue;
+    }
+    const CapNode nd = a.cap_nodes[code];
     if (!(X >= nd.lo[0] && X <= nd.hi[0] && Y >= nd.lo[1] && Y <= nd.hi[1])) continue;
-    if (nd.left < 0) {
+    if (nd.left < 0 && nd.right < 0) {
       const int first = -1 - nd.left;
-      for (int k = 0; k < nd.right; ++k) tri(a.caps + 9 * (first + k));
+      for (int k = 0; k < -nd.right; ++k) tri(a.caps + 9 * (size_t)(first + k));
     } else if (sp + 2 <= kCapStack) {
       stack[sp++] = nd.left;
       stack[sp++] = nd.right;
+    } else {
+      stack_overflow(a.sc);
     }
   }
 }
@@ -1070,7 +1368,15 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
   int sp = 0;
   stack[sp++] = 0;
   while (sp > 0) {
-    const CapNode nd = a.cap_nodes[stack[--sp]];
+    const int code = stack[--sp];
+    if (code < 0) {  // a single fan
+      const float om = solid_angle_f32(q, a.caps + 9 * (size_t)~code, singular);
+      const float t = sum + om;
+      cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
+      sum = t;
+      continue;
+    }
+    const CapNode nd = a.cap_nodes[code];
     const float dx = q.x - nd.c[0], dy = q.y - nd.c[1], dz = q.z - nd.c[2];
     const float dmin = sqrtf(dx * dx + dy * dy + dz * dz) - nd.r;
     if (dmin > 0.0f) {
@@ -1080,10 +1386,10 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
         continue;
       }
     }
-    if (nd.left < 0) {
+    if (nd.left < 0 && nd.right < 0) {
       const int first = -1 - nd.left;
-      for (int k = 0; k < nd.right; ++k) {
-        const float om = solid_angle_f32(q, a.caps + 9 * (first + k), singular);
+      for (int k = 0; k < -nd.right; ++k) {
+        const float om = solid_angle_f32(q, a.caps + 9 * (size_t)(first + k), singular);
         const float t = sum + om;
         cmp += (fabsf(sum) >= fabsf(om)) ? (sum - t) + om : (om - t) + sum;
         sum = t;
@@ -1833,18 +2139,50 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
   k_edge_insert<<<nb(ntri), 256, 0, s>>>(cslot, minid, ntri, ekeys, enet, emask);
   k_edge_collect<<<nb((uint64_t)emask + 1), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
 }
-__global__ __launch_bounds__(256) void k_edge_points(const int2 *__restrict__ edges, uint32_t n,
-                                                     const float *__restrict__ xyz, float *pts) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i >= n) return;
-  const int2 e = edges[i];
-  for (int k = 0; k < 3; ++k) {
-    pts[6 * (size_t)i + k] = xyz[3 * (size_t)e.x + k];
-    pts[6 * (size_t)i + 3 + k] = xyz[3 * (size_t)e.y + k];
-  }
+void launch_cap_keys(const int2 *bnd, uint32_t ne, unsigned long long *keys, hipStream_t s) {
+  if (ne) k_cap_keys<<<nb(ne), 256, 0, s>>>(bnd, ne, keys);
 }
-void launch_edge_points(const int2 *edges, uint32_t n, const float *xyz, float *pts, hipStream_t s) {
-  if (n) k_edge_points<<<nb(n), 256, 0, s>>>(edges, n, xyz, pts);
+hipError_t sort_keys_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                         uint32_t n, int end_bit, hipStream_t s) {
+  return rocprim::radix_sort_keys(temp, temp_bytes, kin, kout, n, 0u, (unsigned)end_bit, s);
+}
+void launch_cap_in_keys(const unsigned long long *ek, uint32_t ne, unsigned long long *ik, uint32_t *iv,
+                        hipStream_t s) {
+  if (ne) k_cap_in_keys<<<nb(ne), 256, 0, s>>>(ek, ne, ik, iv);
+}
+void launch_cap_succ(const unsigned long long *ek, const unsigned long long *ik, const uint32_t *iv, uint32_t ne,
+                     uint32_t *lab, uint32_t *jmp, hipStream_t s) {
+  if (ne) k_cap_succ<<<nb(ne), 256, 0, s>>>(ek, ik, iv, ne, lab, jmp);
+}
+void launch_cap_jump(const uint32_t *lab0, const uint32_t *jmp0, uint32_t ne, uint32_t *lab1, uint32_t *jmp1,
+                     hipStream_t s) {
+  if (ne) k_cap_jump<<<nb(ne), 256, 0, s>>>(lab0, jmp0, ne, lab1, jmp1);
+}
+void launch_iota(uint32_t *v, uint32_t n, hipStream_t s) {
+  if (n) k_iota<<<nb(n), 256, 0, s>>>(v, n);
+}
+void launch_cap_bounds(const unsigned long long *ek, uint32_t ne, const float *xyz, uint32_t *bounds, hipStream_t s) {
+  if (ne) k_cap_bounds<<<std::min(nb(ne), 256u), 256, 0, s>>>(ek, ne, xyz, bounds);
+}
+void launch_cap_sums(const uint32_t *slab, const uint32_t *sedge, uint32_t ne, const unsigned long long *ek,
+                     const float *xyz, const uint32_t *bounds, unsigned long long *acc, hipStream_t s) {
+  if (ne) k_cap_sums<<<nb(ne), 256, 0, s>>>(slab, sedge, ne, ek, xyz, bounds, acc);
+}
+void launch_cap_tris(const unsigned long long *ek, const uint32_t *lab, uint32_t ne, const float *xyz,
+                     const uint32_t *bounds, const unsigned long long *acc, const uint32_t cmin[3],
+                     const uint32_t cmax[3], float *tris, uint32_t *codes, int32_t *idx, hipStream_t s) {
+  CapCodeArgs ca;
+  for (int k = 0; k < 3; ++k) ca.cmin[k] = cmin[k], ca.cmax[k] = cmax[k];
+  if (ne) k_cap_tris<<<nb(ne), 256, 0, s>>>(ek, lab, ne, xyz, bounds, acc, ca, tris, codes, idx);
+}
+void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, float *caps, hipStream_t s) {
+  if (ne) k_cap_gather<<<nb(ne), 256, 0, s>>>(tmp, order, ne, caps);
+}
+void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *child, int32_t *parent, int2 *ranges,
+                     CapNode *nodes, hipStream_t s) {
+  if (n <= 0) return;
+  if (n > 1) k_cap_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, child, parent, ranges);
+  k_cap_nodes<<<nb(n > 1 ? n - 1 : 1, 4), 256, 0, s>>>(caps, child, parent, ranges, n, nodes);
 }
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
   k_occupancy<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, words);
@@ -1881,8 +2219,8 @@ void launch_vox_runs(const uint32_t *cell, const int32_t *tri, uint32_t npairs, 
   if (npairs) k_vox_runs<<<nb(npairs), 256, 0, s>>>(cell, tri, npairs, run_of, tri_rgb, keys, vals, rgb);
 }
 hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
-                          const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s) {
-  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, 64u, s);
+                          const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t s) {
+  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, s);
 }
 void launch_vox_out(const unsigned long long *keys, const uint32_t *perm, uint32_t n, int nx, int ny, const float *rgb,
                     int32_t *cells, int32_t *first, float *colors, hipStream_t s) {

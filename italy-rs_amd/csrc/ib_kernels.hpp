@@ -109,7 +109,28 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
                  unsigned long long *ekeys, int32_t *enet, uint32_t emask, Scalars *sc, int2 *bnd, uint32_t bnd_cap,
                  hipStream_t s);
 // positions of both endpoints (corner indices into the soup) of each boundary edge: 6 floats
-void launch_edge_points(const int2 *edges, uint32_t n, const float *xyz, float *pts, hipStream_t s);
+// hole caps of the ne unmatched edges, all on the device: keys (from, to) sorted; keys (to, from) sorted with
+// their positions; loop labels by pointer jumping; per-loop fixed-point vertex sums; one fan triangle per
+// edge with its centroid's Morton code; the fans sorted by code; their tree
+void launch_cap_keys(const int2 *bnd, uint32_t ne, unsigned long long *keys, hipStream_t s);
+hipError_t sort_keys_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
+                         uint32_t n, int end_bit, hipStream_t s);
+void launch_cap_in_keys(const unsigned long long *ek, uint32_t ne, unsigned long long *ik, uint32_t *iv,
+                        hipStream_t s);
+void launch_cap_succ(const unsigned long long *ek, const unsigned long long *ik, const uint32_t *iv, uint32_t ne,
+                     uint32_t *lab, uint32_t *jmp, hipStream_t s);
+void launch_cap_jump(const uint32_t *lab0, const uint32_t *jmp0, uint32_t ne, uint32_t *lab1, uint32_t *jmp1,
+                     hipStream_t s);
+void launch_iota(uint32_t *v, uint32_t n, hipStream_t s);
+void launch_cap_bounds(const unsigned long long *ek, uint32_t ne, const float *xyz, uint32_t *bounds, hipStream_t s);
+void launch_cap_sums(const uint32_t *slab, const uint32_t *sedge, uint32_t ne, const unsigned long long *ek,
+                     const float *xyz, const uint32_t *bounds, unsigned long long *acc, hipStream_t s);
+void launch_cap_tris(const unsigned long long *ek, const uint32_t *lab, uint32_t ne, const float *xyz,
+                     const uint32_t *bounds, const unsigned long long *acc, const uint32_t cmin[3],
+                     const uint32_t cmax[3], float *tris, uint32_t *codes, int32_t *idx, hipStream_t s);
+void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, float *caps, hipStream_t s);
+void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *child, int32_t *parent, int2 *ranges,
+                     CapNode *nodes, hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 // 16 x 16 column tiles of a grid, and the columns of a whole batch in one launch (prefix: first
 // column tile of each job, njobs + 1 entries on the device)
@@ -149,7 +170,7 @@ void launch_vox_heads(const uint32_t *cell, uint32_t npairs, uint32_t *head, hip
 void launch_vox_runs(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const uint32_t *run_of,
                      const float *tri_rgb, unsigned long long *keys, uint32_t *vals, float *rgb, hipStream_t s);
 hipError_t sort_pairs_u64(void *temp, size_t &temp_bytes, const unsigned long long *kin, unsigned long long *kout,
-                          const uint32_t *vin, uint32_t *vout, uint32_t n, hipStream_t s);
+                          const uint32_t *vin, uint32_t *vout, uint32_t n, int end_bit, hipStream_t s);
 void launch_vox_out(const unsigned long long *keys, const uint32_t *perm, uint32_t n, int nx, int ny, const float *rgb,
                     int32_t *cells, int32_t *first, float *colors, hipStream_t s);
 
