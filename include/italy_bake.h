@@ -73,6 +73,7 @@ typedef struct {
   double ms_caps;        /* device time of the hole caps of open meshes (loop labels, fans, cap tree), events */
   double ms_query;       /* the closest-triangle query kernel alone (dense bakes; HIP events around its launch) */
   uint64_t obb_tests;    /* point/oriented-box tests of leaf blocks in the per-brick query (counting mode) */
+  uint64_t on_cap;       /* rechecked voxels lying on a hole cap, decided by the mesh's own FP64 winding number */
 } ib_stats;
 
 /* Options for ib_set_option. */

@@ -285,6 +285,7 @@ struct MeshDev {
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
   int ncapnode = 0;
+  int2 *bnd = nullptr;  // the weld's unmatched edges, until the caps are built
   int depth = 0;
   bool slivers = false;
 };
@@ -359,6 +360,7 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
     ws = c->side[1];
   }
   launch_weld(d_xyz, n, w.slots, w.minid, w.vcap - 1, w.cslot, w.ekeys, w.enet, w.ecap - 1, sc, w.bnd, 3u * n, ws);
+  m.bnd = w.bnd;
   ib_status st = IB_OK;
   if (!arena) {
     c->record(EV_INGEST, ws);
@@ -513,10 +515,10 @@ ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint3
   return IB_OK;
 }
 
-// after the scalars are on the host: validity, depth, and hole caps for open meshes.  bnd_valid:
-// the "e_bnd" scratch still holds this mesh's boundary edges (false in a batch -> re-weld).
+// after the scalars are on the host: validity, depth, and hole caps for open meshes (from the unmatched
+// edges the build's weld left in m.bnd: the context's scratch for a plain bake, the build's arena otherwise)
 ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &h, MeshDev &m, Arena *arena,
-                       bool bnd_valid, hipStream_t s) {
+                       hipStream_t s) {
   if (h.flags & kFlagNonFinite) return fail(IB_ERR_INVALID_MESH, "mesh has non-finite vertex coordinates");
   m.depth = (int)h.depth;
   m.slivers = h.slivers > 0;
@@ -528,20 +530,17 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   m.caps = nullptr;
   if (h.boundary == 0) return IB_OK;
   if (!c->ev_valid[EV_CAP0]) c->record(EV_CAP0, s);
-  int2 *bnd;
-  if (!bnd_valid) {
-    Scalars *sc2;
-    IB_TRY(c->scratch("sc_reweld", 1, &sc2));
-    fill_u32(reinterpret_cast<uint32_t *>(sc2), 0u, sizeof(Scalars) / 4, s);
-    IB_TRY(enqueue_weld(c, d_xyz, n, sc2, &bnd, s));
-  } else {
-    IB_TRY(c->scratch("e_bnd", 3ull * n, &bnd));
-  }
+  if (!m.bnd) return fail(IB_ERR_INTERNAL, "the weld's unmatched edges are gone");
   const uint32_t ne = std::min<uint32_t>(h.boundary, 3u * n);
-  IB_TRY(enqueue_caps(c, d_xyz, bnd, ne, h, m, arena, s));
+  IB_TRY(enqueue_caps(c, d_xyz, m.bnd, ne, h, m, arena, s));
+  m.bnd = nullptr;  // temporaries: not kept with a resident mesh
   c->record(EV_CAP1, s);
   return IB_OK;
 }
+
+// voxels on a hole cap per query whose mesh winding numbers are summed in chunks across the chip (more go
+// to the wave that finds them)
+constexpr uint32_t kSingCap = 1024;
 
 // column crossings of one grid + the query argument block (buffers from arena or scratch)
 ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float *d_out, Scalars *sc, Arena *arena,
@@ -570,6 +569,12 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   sa.out = d_out;
   sa.amb_cap = amb_cap;
   IB_TRY(take(c, arena, "amb_list", amb_cap, &sa.amb_list));
+  if (m.ncap > 0) {  // only hole caps can put a voxel on a cap
+    sa.sing_cap = std::min<uint32_t>(amb_cap, kSingCap);
+    sa.sing_chunks = (uint32_t)((m.n + kSingChunk - 1) / kSingChunk);
+    IB_TRY(take(c, arena, "sing_list", sa.sing_cap, &sa.sing));
+    IB_TRY(take(c, arena, "sing_part", (size_t)sa.sing_cap * sa.sing_chunks, &sa.sing_part));
+  }
   return IB_OK;
 }
 
@@ -658,7 +663,7 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
     c->record(EV_OCC, s);
     IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
     IB_HIP(hipStreamSynchronize(s));
-    IB_TRY(finish_build(c, d_xyz, (int)ntri, *c->host_sc, m, nullptr, true, s));
+    IB_TRY(finish_build(c, d_xyz, (int)ntri, *c->host_sc, m, nullptr, s));
     c->stats.bvh_depth = m.depth;
     c->stats.boundary_edges = c->host_sc->boundary;
     if (sdf) IB_TRY(query_dense(c, m, *sdf, d_dist, s));
@@ -732,7 +737,7 @@ ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void 
   IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
   const Scalars h = *c->host_sc;
-  IB_TRY(finish_build(c, mh->d_xyz, n, h, mh->m, &mh->keep, false, s));
+  IB_TRY(finish_build(c, mh->d_xyz, n, h, mh->m, &mh->keep, s));
   IB_HIP(hipStreamSynchronize(s));
   mh->ntri = ntri;
   mh->boundary = h.boundary;
@@ -1570,6 +1575,7 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
     st.tri_tests = h.tri_tests;
     st.stack_overflows = h.stack_overflow;
     st.obb_tests = h.obb_tests;
+    st.on_cap = h.singular;
   }
   *out = st;
   return IB_OK;
@@ -1663,7 +1669,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       continue;
     }
     if (!jb.mesh) {
-      const ib_status st = finish_build(c, jb.d_xyz, (int)jb.ntri, c->host_batch_sc[j], mesh[j], &arena, false, s);
+      const ib_status st = finish_build(c, jb.d_xyz, (int)jb.ntri, c->host_batch_sc[j], mesh[j], &arena, s);
       if (st != IB_OK) return fail(st, "job " + std::to_string(j) + ": " + g_err);
     }
     any_slivers |= mesh[j].slivers;
@@ -1799,7 +1805,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     BatchArgs t{d_args, d_prefix, (uint32_t)args.size(), d_head, c->brick_shape, nullptr};
     const uint32_t blocks = std::min<uint32_t>(prefix.back(), (uint32_t)(c->cus * c->blocks_per_cu));
     launch_sdf_batch(t, std::max(blocks, 1u), c->count_work, any_slivers, any_caps, s);
-    launch_resolve_batch(d_args, (uint32_t)args.size(), s);
+    launch_resolve_batch(d_args, (uint32_t)args.size(), any_caps, s);
   }
   c->record(EV_SDF, s);
   IB_HIP(hipMemcpyAsync(c->host_batch_sc, scs, sizeof(Scalars) * njobs, hipMemcpyDeviceToHost, s));
@@ -1814,6 +1820,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.box_tests += h.box_tests;
     c->stats.tri_tests += h.tri_tests;
     c->stats.obb_tests += h.obb_tests;
+    c->stats.on_cap += h.singular;
     c->stats.stack_overflows += h.stack_overflow;
     if (h.stack_overflow)
       return fail(IB_ERR_INTERNAL, "job " + std::to_string(j) + ": a traversal stack overflowed");
@@ -1836,7 +1843,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       BatchArgs t{d_one, d_pf, 1, d_hd, jb.band_voxels > 0 ? 0 : c->brick_shape};
       launch_sdf_batch(t, std::max(1u, std::min<uint32_t>(pf[1], (uint32_t)(c->cus * c->blocks_per_cu))), false,
                        any_slivers, any_caps, s);
-      launch_resolve_batch(d_one, 1, s);
+      launch_resolve_batch(d_one, 1, any_caps, s);
       IB_HIP(hipStreamSynchronize(s));
     }
   }

@@ -654,20 +654,34 @@ __global__ __launch_bounds__(256) void k_edge_insert(const int32_t *__restrict__
 }
 
 // boundary edges: emit each unmatched directed edge (as corner indices = representative vertex
-// ids) with its multiplicity
+// ids) with its multiplicity; a wave reserves its edges' places with one atomic
 __global__ __launch_bounds__(256) void k_edge_collect(const unsigned long long *__restrict__ keys,
                                                       const int32_t *__restrict__ net, uint32_t cap, Scalars *sc,
                                                       int2 *out, uint32_t out_cap) {
   const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  if (s >= cap) return;
-  const unsigned long long key = keys[s];
-  if (key == ~0ull) return;
-  const int m = net[s];
-  if (m == 0) return;
-  const int lo = (int)(key >> 32), hi = (int)(key & 0xffffffffu);
-  const int cnt = m > 0 ? m : -m;
-  const uint32_t pos = atomicAdd(&sc->boundary, (uint32_t)cnt);
-  for (int k = 0; k < cnt; ++k)
+  const int lane = threadIdx.x & 63;
+  int m = 0, lo = 0, hi = 0;
+  if (s < cap) {
+    const unsigned long long key = keys[s];
+    if (key != ~0ull) {
+      m = net[s];
+      lo = (int)(key >> 32);
+      hi = (int)(key & 0xffffffffu);
+    }
+  }
+  const uint32_t cnt = (uint32_t)(m > 0 ? m : -m);
+  uint32_t incl = cnt;  // inclusive prefix sum over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  const uint32_t total = __shfl(incl, 63);
+  if (total == 0) return;  // wave-uniform
+  uint32_t base = 0;
+  if (lane == 63) base = atomicAdd(&sc->boundary, total);
+  base = __shfl(base, 63);
+  const uint32_t pos = base + incl - cnt;
+  for (uint32_t k = 0; k < cnt; ++k)
     if (pos + k < out_cap) out[pos + k] = m > 0 ? make_int2(lo, hi) : make_int2(hi, lo);
 }
 
@@ -1457,11 +1471,12 @@ __device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int
 
 // FP64 winding number of the mesh itself at q (no caps), by the whole wave over its ntri triangles: the
 // direct form of the closure identity's sum, for a q that lies on a cap (where the identity does not hold)
-__device__ __noinline__ double mesh_winding_f64_wave(F3 q, const float4 *__restrict__ tris, int ntri) {
+// (the sum of solid angles of triangles [t0, t1), wave-uniform)
+__device__ __noinline__ double mesh_solid_angle_f64_wave(F3 q, const float4 *__restrict__ tris, int t0, int t1) {
   const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0;
   bool unused = false;
-  for (int i = lane; i < ntri; i += 64) {
+  for (int i = t0 + lane; i < t1; i += 64) {
     const float4 *t = tris + 3 * i;
     const float v[9] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z, t[2].x, t[2].y, t[2].z};
     const double om = solid_angle_f64(q, v, unused);
@@ -1469,7 +1484,10 @@ __device__ __noinline__ double mesh_winding_f64_wave(F3 q, const float4 *__restr
     comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
     sum = s;
   }
-  return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+  return wave_sum_f64(sum, comp);
+}
+__device__ __forceinline__ double mesh_winding_f64_wave(F3 q, const float4 *__restrict__ tris, int ntri) {
+  return mesh_solid_angle_f64_wave(q, tris, 0, ntri) * (1.0 / (4.0 * M_PI));
 }
 
 // conservative ray / box slab test for the parity tie-break (boxes inflated by a relative margin)
@@ -1701,6 +1719,13 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
 // FP64 recheck / parity / overflow recount for the listed voxels: one wave per voxel (grid-stride
 // over the count the query kernel left in the scalars).  The crossing count and the tie-break are
 // wave-uniform; the FP64 cap sum is split over the lanes; lane 0 writes.
+// sign from an FP64 winding number: |w| > 1/2, or the ray-parity vote when |w| is within kTieEps of 1/2
+__device__ __forceinline__ bool decide_w64(const SdfArgs &a, double w64, F3 q) {
+  if (fabs(fabs(w64) - 0.5) >= kTieEps) return fabs(w64) > 0.5;
+  if ((threadIdx.x & 63) == 0) atomicAdd(&a.col.sc->ties, 1u);
+  return parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q, a.col.sc);
+}
+
 __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
   const GridArgs &g = a.col.g;
@@ -1728,14 +1753,67 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, u
       if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
       bool on_cap = false;
       double w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
-      if (on_cap) w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
-      if (fabs(fabs(w64) - 0.5) >= kTieEps) {
-        inside = fabs(w64) > 0.5;
-      } else {
-        if (lane == 0) atomicAdd(&a.col.sc->ties, 1u);
-        inside = parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q, a.col.sc);
+      if (on_cap) {
+        // the mesh's own winding number: summed in chunks by every wave of the next launch (k_singular),
+        // or here when the list is full
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(&a.col.sc->singular, 1u);
+        slot = __shfl(slot, 0);
+        if (slot < a.sing_cap) {
+          if (lane == 0) a.sing[slot] = vox;
+          continue;
+        }
+        w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
       }
+      inside = decide_w64(a, w64, q);
     }
+    if (lane == 0) {
+      const size_t oi = out_index(a, x, y, z);
+      const float d = a.out[oi];
+      a.out[oi] = inside ? -d : d;
+    }
+  }
+}
+
+// Voxels on a hole cap (listed by resolve_range): partial FP64 solid-angle sums of the mesh over chunks of
+// kSingChunk triangles, one (voxel, chunk) item per wave (grid-stride), then per voxel the chunks' sum in
+// chunk order and the decision.  The partials depend only on the item, not on the wave that sums it.
+__device__ __forceinline__ void singular_parts(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
+  const uint32_t ns = min(a.col.sc->singular, a.sing_cap), nc = a.sing_chunks;
+  const uint32_t items = ns * nc;
+  const GridArgs &g = a.col.g;
+  for (uint32_t it = wave; it < items; it += nwaves) {
+    const uint32_t slot = it / nc, chunk = it % nc;
+    const uint32_t vox = a.sing[slot];
+    const int x = (int)(vox % (uint32_t)g.nx);
+    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
+    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
+    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+    const int t0 = (int)(chunk * kSingChunk), t1 = min(a.col.ntri, t0 + kSingChunk);
+    const double part = mesh_solid_angle_f64_wave(q, a.col.tris, t0, t1);
+    if ((threadIdx.x & 63) == 0) a.sing_part[(size_t)slot * nc + chunk] = part;
+  }
+}
+
+__device__ __forceinline__ void singular_finish(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
+  const uint32_t ns = min(a.col.sc->singular, a.sing_cap), nc = a.sing_chunks;
+  const GridArgs &g = a.col.g;
+  const int lane = threadIdx.x & 63;
+  for (uint32_t slot = wave; slot < ns; slot += nwaves) {
+    const uint32_t vox = a.sing[slot];
+    const int x = (int)(vox % (uint32_t)g.nx);
+    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
+    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
+    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+    double sum = 0.0, comp = 0.0;
+    for (uint32_t k = lane; k < nc; k += 64) {
+      const double om = a.sing_part[(size_t)slot * nc + k];
+      const double s = sum + om;
+      comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
+      sum = s;
+    }
+    const double w64 = wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+    const bool inside = decide_w64(a, w64, q);
     if (lane == 0) {
       const size_t oi = out_index(a, x, y, z);
       const float d = a.out[oi];
@@ -1746,6 +1824,12 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, u
 
 __global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
   resolve_range(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
+__global__ __launch_bounds__(256) void k_singular_parts(SdfArgs a) {
+  singular_parts(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
+__global__ __launch_bounds__(256) void k_singular_finish(SdfArgs a) {
+  singular_finish(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 
 // --------------------------------------------------------------------------------------------
@@ -2072,6 +2156,12 @@ __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
 __global__ __launch_bounds__(256) void k_resolve_batch(const SdfArgs *jobs) {
   resolve_range(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
+__global__ __launch_bounds__(256) void k_singular_parts_batch(const SdfArgs *jobs) {
+  singular_parts(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
+__global__ __launch_bounds__(256) void k_singular_finish_batch(const SdfArgs *jobs) {
+  singular_finish(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+}
 
 // ============================================================================================
 // host-side launchers
@@ -2284,7 +2374,13 @@ void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int sha
   IB_DISPATCH(launch_sdf_t, count, slivers, a.col.ncap > 0, a, variant, shape, order, s);
 }
 
-void launch_resolve(const SdfArgs &a, hipStream_t s) { k_resolve<<<512, 256, 0, s>>>(a); }
+void launch_resolve(const SdfArgs &a, hipStream_t s) {
+  k_resolve<<<512, 256, 0, s>>>(a);
+  if (a.sing_cap) {
+    k_singular_parts<<<1024, 256, 0, s>>>(a);
+    k_singular_finish<<<64, 256, 0, s>>>(a);
+  }
+}
 
 template <bool COUNT, bool SLIVERS, bool CAPS>
 static void launch_batch_t(const BatchArgs &t, uint32_t blocks, hipStream_t s) {
@@ -2321,8 +2417,15 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
   if (nbrick) k_band_scatter<<<nb(nbrick), 256, 0, s>>>(flags, slots, nbrick, index, bricks, count);
 }
 
-void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s) {
-  if (njobs) k_resolve_batch<<<dim3(128, njobs), 256, 0, s>>>(jobs);
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s) {
+  if (!njobs) return;
+  // a few jobs (bands, served requests) get the whole chip; a large batch a share per job
+  const unsigned per_job = njobs <= 4 ? 512u : 128u;
+  k_resolve_batch<<<dim3(per_job, njobs), 256, 0, s>>>(jobs);
+  if (singular) {
+    k_singular_parts_batch<<<dim3(njobs <= 4 ? 1024u : 64u, njobs), 256, 0, s>>>(jobs);
+    k_singular_finish_batch<<<dim3(16, njobs), 256, 0, s>>>(jobs);
+  }
 }
 
 }  // namespace ib

@@ -33,7 +33,7 @@ struct Scalars {
   uint32_t col_overflow;  // columns with more than kColCap crossings
   uint32_t rechecked;     // voxels re-evaluated in FP64
   uint32_t ties;          // voxels decided by ray parity
-  uint32_t unused0;
+  uint32_t singular;      // FP64 rechecks lying on a hole cap (the mesh's own winding number decides)
   uint32_t slivers;       // triangles taking the FP64 distance path
   uint32_t ambiguous;     // voxels handed to the resolve kernel (column overflow or |w| near 0.5)
   uint32_t stack_overflow;  // traversal pushes that found the stack full (0 by the depth bound; checked by the host)
@@ -75,7 +75,12 @@ struct SdfArgs {
   int unsigned_only;            // 1: store |d| only (brick selection pass; no columns needed)
   uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
   uint32_t amb_cap;    // capacity of amb_list
+  uint32_t sing_cap;     // voxels on a hole cap summed by the chunked launches (0: by the resolving wave)
+  uint32_t sing_chunks;  // chunks of kSingChunk triangles
+  uint32_t *sing;        // those voxels (sing_cap)
+  double *sing_part;     // their chunk sums (sing_cap x sing_chunks)
 };
+constexpr int kSingChunk = 2048;
 
 void fill_u32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s);
 struct FillRange {
@@ -156,7 +161,7 @@ void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int sha
 // launch-order keys (ascending = costliest first) and tile ids of the per-brick query's block-tiles
 void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, int32_t *ids, hipStream_t s);
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
-void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, hipStream_t s);
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s);
 // narrow band: bricks whose centre distance (from the coarse pass) is <= thresh
 void launch_band_flags(const float *coarse, uint32_t nbrick, float thresh, uint32_t *flags, hipStream_t s);
 void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t nbrick, int32_t *index,
