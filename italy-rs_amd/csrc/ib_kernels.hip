@@ -1676,8 +1676,11 @@ __device__ __forceinline__ size_t out_index(const SdfArgs &a, int x, int y, int 
   return (size_t)a.band_index[b] * 512 + (size_t)((((z & 7) << 3) | (y & 7)) << 3 | (x & 7));
 }
 
+// store the voxel: its sign from the column crossings and, with caps, the caps' winding number wc (within
+// cbound); voxels that cannot be decided here are stored unsigned and listed for the resolve kernel
 template <bool CAPS>
-__device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
+__device__ __forceinline__ void store_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best, float wc,
+                                            float cbound) {
   const GridArgs &g = a.col.g;
   const size_t vox = ((size_t)z * g.ny + y) * g.nx + x;
   const size_t col = (size_t)y * g.nx + x;
@@ -1700,10 +1703,8 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
     if (!CAPS) {
       inside = wi != 0;
     } else {
-      float wc, bound;
-      cap_winding_bounded(a.col, q, wc, bound);
       const float w32 = (float)wi + wc;
-      if (fabsf(fabsf(w32) - 0.5f) >= 0.05f + bound) inside = fabsf(w32) > 0.5f;
+      if (fabsf(fabsf(w32) - 0.5f) >= 0.05f + cbound) inside = fabsf(w32) > 0.5f;
       else ambiguous = true;
     }
   }
@@ -1714,6 +1715,15 @@ __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int 
   } else {
     a.out[oi] = inside ? -d : d;
   }
+}
+
+// one voxel on its own (per-voxel kernel): its own walk of the cap tree
+template <bool CAPS>
+__device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
+  float wc = 0.0f, cb = 0.0f;
+  if (CAPS && !a.unsigned_only && a.col.count[(size_t)y * a.col.g.nx + x] <= kColCap)
+    cap_winding_bounded(a.col, q, wc, cb);
+  store_voxel<CAPS>(a, q, x, y, z, best, wc, cb);
 }
 
 // FP64 recheck / parity / overflow recount for the listed voxels: one wave per voxel (grid-stride
@@ -1884,6 +1894,93 @@ __device__ __forceinline__ Obb load_obb_uniform(const Obb *p, int i) {
   return u.r;
 }
 
+union CapNodeBits {
+  V4 v[4];
+  CapNode r;
+};
+__device__ __forceinline__ CapNode load_capnode_uniform(const CapNode *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  CapNodeBits u;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u.v[k] = r[k];
+  return u.r;
+}
+
+// The caps' winding number at every needed voxel of a wave's brick (V per lane), with cap_winding_bounded's
+// bound, by one walk of the cap tree for the whole wave: a node is bounded (not entered) when its bound
+// area / (4pi dmin^2) is at most kCapTau for every needed voxel, else entered by all; fans are read once
+// (scalar loads) and evaluated by every lane.  The stack is the wave's LDS stack, free after the
+// closest-triangle walk.
+template <int V>
+__device__ __forceinline__ void cap_winding_wave(const ColumnArgs &a, const F3 *q, const bool *need, float *w,
+                                                 float *bound, int *sstack) {
+  const float inv4pi = (float)(1.0 / (4.0 * M_PI));
+  float sum[V], cmp[V], ub[V];
+  bool singular[V];
+  bool any = false;
+  for (int k = 0; k < V; ++k) {
+    sum[k] = cmp[k] = ub[k] = 0.0f;
+    singular[k] = false;
+    w[k] = bound[k] = 0.0f;
+    any = any || need[k];
+  }
+  // a batch runs the capped kernel for every job once one job has caps: closed meshes have none
+  if (a.ncap == 0 || a.ncapnode == 0 || !a.cap_nodes || __ballot(any) == 0) return;
+  const int lane = threadIdx.x & 63;
+  typedef const __attribute__((address_space(4))) float CF;
+  auto fan = [&](int i) {
+    CF *p = (CF *)(a.caps + 9 * (size_t)i);
+    float t[9];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) t[j] = p[j];
+    for (int k = 0; k < V; ++k) {
+      const float om = solid_angle_f32(q[k], t, singular[k]);
+      const float s = sum[k] + om;
+      cmp[k] += (fabsf(sum[k]) >= fabsf(om)) ? (sum[k] - s) + om : (om - s) + sum[k];
+      sum[k] = s;
+    }
+  };
+  int sp = 0, cur = 0;
+  bool overflow = false;
+  for (;;) {
+    if (cur < 0) {
+      fan(~cur);
+    } else {
+      const CapNode nd = load_capnode_uniform(a.cap_nodes, cur);
+      float b[V];
+      bool open = false;
+      for (int k = 0; k < V; ++k) {
+        const float dx = q[k].x - nd.c[0], dy = q[k].y - nd.c[1], dz = q[k].z - nd.c[2];
+        const float dmin = sqrtf(dx * dx + dy * dy + dz * dz) - nd.r;
+        b[k] = dmin > 0.0f ? nd.area * inv4pi / (dmin * dmin) : INFINITY;
+        open = open || (need[k] && !(b[k] <= kCapTau));
+      }
+      if (__ballot(open) == 0) {
+        for (int k = 0; k < V; ++k) ub[k] += b[k];
+      } else if (nd.left < 0 && nd.right < 0) {
+        const int first = -1 - nd.left;
+        for (int i = 0; i < -nd.right; ++i) fan(first + i);
+      } else if (sp < kCapStack) {
+        if (lane == 0) sstack[sp] = nd.right;
+        ++sp;
+        cur = nd.left;
+        continue;
+      } else {
+        overflow = true;  // cannot happen within the depth bound; the voxels go to the FP64 recheck
+      }
+    }
+    if (sp == 0) break;
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
+  }
+  if (overflow && lane == 0) stack_overflow(a.sc);
+  for (int k = 0; k < V; ++k) {
+    w[k] = (sum[k] + cmp[k]) * inv4pi;
+    bound[k] = (singular[k] || overflow) ? INFINITY : ub[k] * 1.0001f + 1e-5f;
+  }
+}
+
 union TriRecBits {
   V4 v[6];
   TriRec r;
@@ -2031,8 +2128,15 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
       atomicAdd(&a.col.sc->tri_tests, (unsigned long long)wc.tri * nv / V);
     }
   }
+  float capw[V], capb[V];
+  for (int k = 0; k < V; ++k) capw[k] = capb[k] = 0.0f;
+  if (CAPS && !a.unsigned_only) {
+    bool need[V];
+    for (int k = 0; k < V; ++k) need[k] = valid[k] && a.col.count[(size_t)y * g.nx + x] <= kColCap;
+    cap_winding_wave<V>(a.col, q, need, capw, capb, sstack);
+  }
   for (int k = 0; k < V; ++k)
-    if (valid[k]) finish_voxel<CAPS>(a, q[k], x, y, z[k], best[k]);
+    if (valid[k]) store_voxel<CAPS>(a, q[k], x, y, z[k], best[k], capw[k], capb[k]);
 }
 
 // brick shapes a wave can take (sx, sy, voxels per lane; sz = 64 * V / (sx * sy)); the 4 waves of
