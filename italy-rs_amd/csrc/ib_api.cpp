@@ -538,6 +538,10 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   return IB_OK;
 }
 
+// brick selection: a brick is in the band when its centre is within band * h + 3.5 * sqrt(3) * h of the
+// surface (1-Lipschitz distance: then every voxel within band * h of the surface lies in a kept brick)
+float band_thresh(const ib_job &jb) { return jb.sdf.h * ((float)jb.band_voxels + 6.0621778f); }
+
 // voxels on a hole cap per query whose mesh winding numbers are summed in chunks across the chip (more go
 // to the wave that finds them)
 constexpr uint32_t kSingCap = 1024;
@@ -1679,66 +1683,32 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.bvh_depth = std::max(c->stats.bvh_depth, mesh[j].depth);
   }
   c->record(EV_OCC, s);
-  // 2. narrow band: unsigned distance at every brick centre (a cell-centred grid of spacing 8h),
-  //    bricks within band*h + 3.5*sqrt(3)*h of the surface (1-Lipschitz: every voxel of the band)
+  // 2. narrow band: the bricks whose centre (a cell-centred grid of spacing 8h) lies within
+  //    band*h + 3.5*sqrt(3)*h of the surface (1-Lipschitz: every voxel of the band), flagged by the triangles
   std::vector<uint32_t> band_jobs;
   std::vector<uint32_t *> band_bricks(njobs, nullptr), band_count(njobs, nullptr);
   for (uint32_t j = 0; j < njobs; ++j)
     if (jobs[j].band_voxels > 0 && jobs[j].ntri) band_jobs.push_back(j);
   if (!band_jobs.empty()) {
-    std::vector<SdfArgs> coarse;
-    std::vector<uint32_t> cprefix(1, 0);
-    std::vector<float *> cdist(njobs, nullptr);
-    for (uint32_t j : band_jobs) {
-      const ib_grid &g = jobs[j].sdf;
-      ib_grid cg = g;
-      cg.h = g.h * 8.0f;
-      cg.nx = g.nx / 8;
-      cg.ny = g.ny / 8;
-      cg.nz = g.nz / 8;
-      IB_TRY(arena.get((uint64_t)cg.nx * cg.ny * cg.nz, &cdist[j]));
-      SdfArgs sa{};
-      sa.col.nodes = mesh[j].nodes;
-      sa.col.nnodes = std::max(mesh[j].n - 1, 1);
-      sa.col.ntri = mesh[j].n;
-      sa.col.tris = mesh[j].tris;
-      sa.col.recs = mesh[j].recs;
-      sa.col.nodes4 = mesh[j].nodes4;
-      sa.col.obbs = mesh[j].obbs;
-      sa.col.node_obbs = mesh[j].node_obbs;
-      sa.col.g = to_args(cg);
-      sa.col.sc = scs + j;
-      sa.out = cdist[j];
-      sa.unsigned_only = 1;
-      uint32_t d[3];
-      tile_dims(sa.col.g, 0, d);
-      coarse.push_back(sa);
-      cprefix.push_back(cprefix.back() + d[0] * d[1] * d[2]);
-    }
-    SdfArgs *d_c;
-    uint32_t *d_cp, *d_head;
-    IB_TRY(arena.get(coarse.size(), &d_c));
-    IB_TRY(arena.get(cprefix.size(), &d_cp));
-    IB_TRY(arena.get(1, &d_head));
-    IB_HIP(hipMemcpyAsync(d_c, coarse.data(), coarse.size() * sizeof(SdfArgs), hipMemcpyHostToDevice, s));
-    IB_HIP(hipMemcpyAsync(d_cp, cprefix.data(), cprefix.size() * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-    fill_u32(d_head, 0u, 1, s);
-    BatchArgs t{d_c, d_cp, (uint32_t)coarse.size(), d_head, 0};
-    launch_sdf_batch(t, std::max(1u, std::min<uint32_t>(cprefix.back(), (uint32_t)(c->cus * 8))), false, any_slivers,
-                     false, s);
     std::vector<uint32_t> host_counts(band_jobs.size());
     uint32_t *d_counts;
     IB_TRY(arena.get(band_jobs.size(), &d_counts));
     for (size_t k = 0; k < band_jobs.size(); ++k) {
       const uint32_t j = band_jobs[k];
       const ib_grid &g = jobs[j].sdf;
+      ib_grid cg = g;
+      cg.h = g.h * 8.0f;
+      cg.nx = g.nx / 8;
+      cg.ny = g.ny / 8;
+      cg.nz = g.nz / 8;
       const uint32_t nbrick = (uint32_t)((g.nx / 8) * (g.ny / 8) * (g.nz / 8));
-      const float thresh = g.h * ((float)jobs[j].band_voxels + 6.0621778f);  // 3.5 * sqrt(3)
       uint32_t *flags, *slots;
       IB_TRY(arena.get(nbrick + 1, &flags));
       IB_TRY(arena.get(nbrick + 1, &slots));
       IB_TRY(arena.get(nbrick, &band_bricks[j]));
-      launch_band_flags(cdist[j], nbrick, thresh, flags, s);
+      fill_u32(flags, 0u, nbrick, s);
+      launch_band_splat(mesh[j].tris, mesh[j].recs, mesh[j].n, to_args(cg), band_thresh(jobs[j]), mesh[j].slivers,
+                        flags, s);
       size_t tb = 0;
       IB_HIP(exclusive_scan_u32(nullptr, tb, flags, slots, nbrick, s));
       char *temp;

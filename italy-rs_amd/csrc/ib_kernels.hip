@@ -654,35 +654,58 @@ __global__ __launch_bounds__(256) void k_edge_insert(const int32_t *__restrict__
 }
 
 // boundary edges: emit each unmatched directed edge (as corner indices = representative vertex
-// ids) with its multiplicity; a wave reserves its edges' places with one atomic
+// ids) with its multiplicity.  Each block owns one contiguous range of slots: it counts its edges,
+// reserves their places with one atomic, then writes them in slot order (block-wide prefix sums).
+__device__ __forceinline__ uint32_t edge_mult(const unsigned long long *keys, const int32_t *net, uint32_t s,
+                                              uint32_t end, int &m, int &lo, int &hi) {
+  m = 0;
+  if (s < end && keys[s] != ~0ull) {
+    const unsigned long long key = keys[s];
+    m = net[s];
+    lo = (int)(key >> 32);
+    hi = (int)(key & 0xffffffffu);
+  }
+  return (uint32_t)(m > 0 ? m : -m);
+}
+
 __global__ __launch_bounds__(256) void k_edge_collect(const unsigned long long *__restrict__ keys,
                                                       const int32_t *__restrict__ net, uint32_t cap, Scalars *sc,
                                                       int2 *out, uint32_t out_cap) {
-  const uint32_t s = blockIdx.x * 256 + threadIdx.x;
-  const int lane = threadIdx.x & 63;
-  int m = 0, lo = 0, hi = 0;
-  if (s < cap) {
-    const unsigned long long key = keys[s];
-    if (key != ~0ull) {
-      m = net[s];
-      lo = (int)(key >> 32);
-      hi = (int)(key & 0xffffffffu);
+  __shared__ uint32_t wsum[4], base;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const uint32_t per = ((cap + gridDim.x - 1) / gridDim.x + 255) & ~255u;
+  const uint32_t start = min(cap, blockIdx.x * per), end = min(cap, start + per);
+  int m, lo, hi;
+  uint32_t own = 0;
+  for (uint32_t s = start + threadIdx.x; s < end; s += 256) own += edge_mult(keys, net, s, end, m, lo, hi);
+  for (int off = 32; off >= 1; off >>= 1) own += __shfl_xor(own, off);
+  if (lane == 0) wsum[wv] = own;
+  __syncthreads();
+  const uint32_t total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (total == 0) return;  // block-uniform
+  if (threadIdx.x == 0) base = atomicAdd(&sc->boundary, total);
+  __syncthreads();
+  uint32_t run = base;
+  for (uint32_t it = start; it < end; it += 256) {
+    const uint32_t cnt = edge_mult(keys, net, it + threadIdx.x, end, m, lo, hi);
+    uint32_t incl = cnt;  // inclusive prefix over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t o = __shfl_up(incl, off);
+      if (lane >= off) incl += o;
     }
+    __syncthreads();  // the previous iteration's wave sums are read
+    if (lane == 63) wsum[wv] = incl;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < 4; ++w) {
+      before += w < wv ? wsum[w] : 0u;
+      all += wsum[w];
+    }
+    const uint32_t pos = run + before + incl - cnt;
+    for (uint32_t k = 0; k < cnt; ++k)
+      if (pos + k < out_cap) out[pos + k] = m > 0 ? make_int2(lo, hi) : make_int2(hi, lo);
+    run += all;
   }
-  const uint32_t cnt = (uint32_t)(m > 0 ? m : -m);
-  uint32_t incl = cnt;  // inclusive prefix sum over the wave
-  for (int off = 1; off < 64; off <<= 1) {
-    const uint32_t o = __shfl_up(incl, off);
-    if (lane >= off) incl += o;
-  }
-  const uint32_t total = __shfl(incl, 63);
-  if (total == 0) return;  // wave-uniform
-  uint32_t base = 0;
-  if (lane == 63) base = atomicAdd(&sc->boundary, total);
-  base = __shfl(base, 63);
-  const uint32_t pos = base + incl - cnt;
-  for (uint32_t k = 0; k < cnt; ++k)
-    if (pos + k < out_cap) out[pos + k] = m > 0 ? make_int2(lo, hi) : make_int2(hi, lo);
 }
 
 // ============================================================================================
@@ -780,11 +803,18 @@ __global__ __launch_bounds__(256) void k_cap_bounds(const u64 *__restrict__ ek, 
       mn[k] = min(mn[k], (uint32_t)__shfl_xor(mn[k], off));
       mx[k] = max(mx[k], (uint32_t)__shfl_xor(mx[k], off));
     }
+  __shared__ uint32_t red[4][6];
+  const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0)
-    for (int k = 0; k < 3; ++k) {
-      atomicMin(&bounds[k], mn[k]);
-      atomicMax(&bounds[3 + k], mx[k]);
-    }
+    for (int k = 0; k < 3; ++k) red[wv][k] = mn[k], red[wv][3 + k] = mx[k];
+  __syncthreads();
+  if (threadIdx.x < 6) {  // one atomic per block and bound
+    const int k = threadIdx.x;
+    uint32_t v = red[0][k];
+    for (int w = 1; w < 4; ++w) v = k < 3 ? min(v, red[w][k]) : max(v, red[w][k]);
+    if (k < 3) atomicMin(&bounds[k], v);
+    else atomicMax(&bounds[k], v);
+  }
 }
 
 // fixed-point frame of axis k: x -> (x - lo) * 2^32 / (hi - lo), in [0, 2^32]
@@ -1275,52 +1305,6 @@ __device__ __forceinline__ bool column_in_xy_box(float X, float Y, const float4 
          Y >= fminf(fminf(t[0].y, t[1].y), t[2].y) && Y <= fmaxf(fmaxf(t[0].y, t[1].y), t[2].y);
 }
 
-__device__ __forceinline__ void column_crossings(const ColumnArgs &a, int x, int y) {
-  const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
-  const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
-  const size_t col = (size_t)y * a.g.nx + x;
-  ColumnSink sink{a.cross + col * kColCap, 0};
-  for_column_leaves(a.nodes, a.nnodes, X, Y, a.sc, [&](int leaf) {
-    const float4 *t = a.tris + 3 * leaf;
-    if (!column_in_xy_box(X, Y, t)) return;
-    float z;
-    const int s = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
-    if (s) sink.add(z, s);
-  });
-  for_column_caps(a, X, Y, [&](const float *t) {  // caps enter the closed surface reversed
-    float z;
-    const int s = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
-    if (s) sink.add(z, -s);
-  });
-  a.count[col] = sink.cnt;
-  if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
-}
-
-__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
-  const int x = blockIdx.x * 16 + (threadIdx.x & 15);
-  const int y = blockIdx.y * 16 + (threadIdx.x >> 4);
-  if (x >= a.g.nx || y >= a.g.ny) return;
-  column_crossings(a, x, y);
-}
-
-// the columns of every job of a batch in one launch: block b is 16 x 16 columns of the job whose
-// range of the column-tile prefix holds b
-__global__ __launch_bounds__(256) void k_columns_batch(const SdfArgs *__restrict__ jobs,
-                                                       const uint32_t *__restrict__ prefix, uint32_t njobs) {
-  const uint32_t b = blockIdx.x;
-  int lo = 0, hi = (int)njobs - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
-  }
-  const ColumnArgs &a = jobs[lo].col;
-  const uint32_t local = b - prefix[lo], tx = (uint32_t)(a.g.nx + 15) / 16;
-  const int x = (int)(local % tx) * 16 + (threadIdx.x & 15);
-  const int y = (int)(local / tx) * 16 + (threadIdx.x >> 4);
-  if (x >= a.g.nx || y >= a.g.ny) return;
-  column_crossings(a, x, y);
-}
-
 // winding (integer part) of the closed surface mesh + reversed caps for a voxel whose column
 // overflowed the stored crossings: recount crossings above q.z
 __device__ __noinline__ int column_direct(const ColumnArgs &a, float X, float Y, float qz) {
@@ -1686,10 +1670,6 @@ __device__ __forceinline__ void store_voxel(const SdfArgs &a, F3 q, int x, int y
   const size_t col = (size_t)y * g.nx + x;
   const float d = sqrtf(best);
   const size_t oi = out_index(a, x, y, z);
-  if (a.unsigned_only) {
-    a.out[oi] = d;
-    return;
-  }
   const int cnt = a.col.count[col];
   bool ambiguous = cnt > kColCap;
   bool inside = false;
@@ -1721,7 +1701,7 @@ __device__ __forceinline__ void store_voxel(const SdfArgs &a, F3 q, int x, int y
 template <bool CAPS>
 __device__ __forceinline__ void finish_voxel(const SdfArgs &a, F3 q, int x, int y, int z, float best) {
   float wc = 0.0f, cb = 0.0f;
-  if (CAPS && !a.unsigned_only && a.col.count[(size_t)y * a.col.g.nx + x] <= kColCap)
+  if (CAPS && a.col.count[(size_t)y * a.col.g.nx + x] <= kColCap)
     cap_winding_bounded(a.col, q, wc, cb);
   store_voxel<CAPS>(a, q, x, y, z, best, wc, cb);
 }
@@ -1986,6 +1966,133 @@ union TriRecBits {
   TriRec r;
 };
 
+// The +z crossings of the 8 x 8 columns (x0.., y0..) of a wave's tile: one walk of the tree for the whole
+// wave (a child is entered when its xy box holds any lane's column), node and triangle records read once
+// (scalar loads) and tested by every lane; then the cap fans the same way.  A column's crossings are the
+// same set as its own walk finds (only their order in the list differs).
+__device__ __forceinline__ void column_crossings_wave(const ColumnArgs &a, int x0, int y0, int *sstack) {
+  const int lane = threadIdx.x & 63;
+  const int x = x0 + (lane & 7), y = y0 + (lane >> 3);
+  const bool valid = x < a.g.nx && y < a.g.ny;
+  if (__ballot(valid) == 0) return;
+  const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
+  const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
+  const size_t col = (size_t)y * a.g.nx + x;
+  ColumnSink sink{a.cross + (valid ? col : 0) * kColCap, 0};
+  auto tri = [&](int l) {
+    const CV4 *r = (const CV4 *)(a.tris + 3 * (size_t)l);
+    const V4 p0 = r[0], p1 = r[1], p2 = r[2];
+    const float4 t[3] = {make_float4(p0.x, p0.y, p0.z, p0.w), make_float4(p1.x, p1.y, p1.z, p1.w),
+                         make_float4(p2.x, p2.y, p2.z, p2.w)};
+    if (!valid || !column_in_xy_box(X, Y, t)) return;
+    float z;
+    const int sg = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
+    if (sg) sink.add(z, sg);
+  };
+  int sp = 0, cur = 0;
+  for (int visits = 0; visits <= a.nnodes; ++visits) {
+    const Node nd = load_node_uniform(a.nodes, cur);
+    const bool in0 = valid && X >= nd.lo0[0] && X <= nd.hi0[0] && Y >= nd.lo0[1] && Y <= nd.hi0[1];
+    const bool in1 = valid && X >= nd.lo1[0] && X <= nd.hi1[0] && Y >= nd.lo1[1] && Y <= nd.hi1[1];
+    int next = -1;
+    if (__ballot(in0) != 0) {
+      if (nd.c0 < 0) for_leaf_code(~nd.c0, tri); else next = nd.c0;
+    }
+    if (__ballot(in1) != 0) {
+      if (nd.c1 < 0) {
+        for_leaf_code(~nd.c1, tri);
+      } else if (next < 0) {
+        next = nd.c1;
+      } else if (sp < kStack4) {
+        if (lane == 0) sstack[sp] = nd.c1;
+        ++sp;
+      } else {
+        if (lane == 0) stack_overflow(a.sc);
+      }
+    }
+    if (next >= 0) {
+      cur = next;
+      continue;
+    }
+    if (sp == 0) break;
+    --sp;
+    __builtin_amdgcn_wave_barrier();
+    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
+  }
+  if (a.ncap > 0 && a.ncapnode > 0 && a.cap_nodes) {  // caps enter the closed surface reversed
+    typedef const __attribute__((address_space(4))) float CF;
+    auto fan = [&](int i) {
+      CF *p = (CF *)(a.caps + 9 * (size_t)i);
+      float t[9];
+#pragma unroll
+      for (int j = 0; j < 9; ++j) t[j] = p[j];
+      if (!valid || !(X >= fminf(fminf(t[0], t[3]), t[6]) && X <= fmaxf(fmaxf(t[0], t[3]), t[6]) &&
+                      Y >= fminf(fminf(t[1], t[4]), t[7]) && Y <= fmaxf(fmaxf(t[1], t[4]), t[7])))
+        return;
+      float z;
+      const int sg = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
+      if (sg) sink.add(z, -sg);
+    };
+    sp = 0;
+    cur = 0;
+    for (;;) {
+      if (cur < 0) {
+        fan(~cur);
+      } else {
+        const CapNode nd = load_capnode_uniform(a.cap_nodes, cur);
+        const bool in = valid && X >= nd.lo[0] && X <= nd.hi[0] && Y >= nd.lo[1] && Y <= nd.hi[1];
+        if (__ballot(in) != 0) {
+          if (nd.left < 0 && nd.right < 0) {
+            const int first = -1 - nd.left;
+            for (int i = 0; i < -nd.right; ++i) fan(first + i);
+          } else if (sp < kCapStack) {
+            if (lane == 0) sstack[sp] = nd.right;
+            ++sp;
+            cur = nd.left;
+            continue;
+          } else {
+            if (lane == 0) stack_overflow(a.sc);
+          }
+        }
+      }
+      if (sp == 0) break;
+      --sp;
+      __builtin_amdgcn_wave_barrier();
+      cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
+    }
+  }
+  if (!valid) return;
+  a.count[col] = sink.cnt;
+  if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
+}
+
+// 16 x 16 columns per block, an 8 x 8 tile per wave
+__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
+  __shared__ int sstack[4][kStack4];
+  const int wv = threadIdx.x >> 6;
+  column_crossings_wave(a, blockIdx.x * 16 + (wv & 1) * 8, blockIdx.y * 16 + (wv >> 1) * 8, sstack[wv]);
+}
+
+// the columns of every job of a batch in one launch: block b is 16 x 16 columns of the job whose
+// range of the column-tile prefix holds b
+__global__ __launch_bounds__(256) void k_columns_batch(const SdfArgs *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ prefix, uint32_t njobs) {
+  __shared__ int sstack[4][kStack4];
+  const uint32_t b = blockIdx.x;
+  int lo = 0, hi = (int)njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  const ColumnArgs &a = jobs[lo].col;
+  const uint32_t local = b - prefix[lo], tx = (uint32_t)(a.g.nx + 15) / 16;
+  const int wv = threadIdx.x >> 6;
+  column_crossings_wave(a, (int)(local % tx) * 16 + (wv & 1) * 8, (int)(local / tx) * 16 + (wv >> 1) * 8,
+                        sstack[wv]);
+}
+
+
+
 __device__ __forceinline__ TriRec load_trirec_uniform(const TriRec *p, int i) {
   const CV4 *r = (const CV4 *)(p + i);
   TriRecBits u;
@@ -2130,7 +2237,7 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   }
   float capw[V], capb[V];
   for (int k = 0; k < V; ++k) capw[k] = capb[k] = 0.0f;
-  if (CAPS && !a.unsigned_only) {
+  if (CAPS) {
     bool need[V];
     for (int k = 0; k < V; ++k) need[k] = valid[k] && a.col.count[(size_t)y * g.nx + x] <= kColCap;
     cap_winding_wave<V>(a.col, q, need, capw, capb, sstack);
@@ -2331,7 +2438,7 @@ void launch_weld(const float *xyz, int ntri, int32_t *slots, int32_t *minid, uin
   k_weld_insert<<<nb(nc), 256, 0, s>>>(xyz, nc, slots, vmask, cslot);
   k_weld_min<<<nb(nc), 256, 0, s>>>(cslot, nc, minid);
   k_edge_insert<<<nb(ntri), 256, 0, s>>>(cslot, minid, ntri, ekeys, enet, emask);
-  k_edge_collect<<<nb((uint64_t)emask + 1), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
+  k_edge_collect<<<std::min(nb((uint64_t)emask + 1), 1024u), 256, 0, s>>>(ekeys, enet, emask + 1, sc, bnd, bnd_cap);
 }
 void launch_cap_keys(const int2 *bnd, uint32_t ne, unsigned long long *keys, hipStream_t s) {
   if (ne) k_cap_keys<<<nb(ne), 256, 0, s>>>(bnd, ne, keys);
@@ -2356,7 +2463,7 @@ void launch_iota(uint32_t *v, uint32_t n, hipStream_t s) {
   if (n) k_iota<<<nb(n), 256, 0, s>>>(v, n);
 }
 void launch_cap_bounds(const unsigned long long *ek, uint32_t ne, const float *xyz, uint32_t *bounds, hipStream_t s) {
-  if (ne) k_cap_bounds<<<std::min(nb(ne), 256u), 256, 0, s>>>(ek, ne, xyz, bounds);
+  if (ne) k_cap_bounds<<<std::min(nb(ne), 64u), 256, 0, s>>>(ek, ne, xyz, bounds);
 }
 void launch_cap_sums(const uint32_t *slab, const uint32_t *sedge, uint32_t ne, const unsigned long long *ek,
                      const float *xyz, const uint32_t *bounds, unsigned long long *acc, hipStream_t s) {
@@ -2496,10 +2603,36 @@ void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool sliv
   IB_DISPATCH(launch_batch_t, count, slivers, caps, t, blocks, s);
 }
 
-__global__ __launch_bounds__(256) void k_band_flags(const float *__restrict__ coarse, uint32_t n, float thresh,
-                                                    uint32_t *flags) {
-  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) flags[i] = coarse[i] <= thresh ? 1u : 0u;
+// Brick selection by splatting: every triangle flags the bricks whose centre (a cell-centred grid of
+// spacing H over the grid) lies within thresh of it, by the query's own squared distance (tri_d2 of the
+// triangle's record), so a brick is flagged exactly when sqrt of the least squared distance over all
+// triangles is at most thresh.  Candidates: the centres within thresh of the triangle's box along every
+// axis, one more on each side for rounding.
+template <bool SLIVERS>
+__global__ __launch_bounds__(256) void k_band_splat(const float4 *__restrict__ tris, const TriRec *__restrict__ recs,
+                                                    int n, GridArgs cg, float thresh, uint32_t *flags) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const float4 v0 = tris[3 * t], v1 = tris[3 * t + 1], v2 = tris[3 * t + 2];
+  const float lo[3] = {fminf(fminf(v0.x, v1.x), v2.x), fminf(fminf(v0.y, v1.y), v2.y), fminf(fminf(v0.z, v1.z), v2.z)};
+  const float hi[3] = {fmaxf(fmaxf(v0.x, v1.x), v2.x), fmaxf(fmaxf(v0.y, v1.y), v2.y), fmaxf(fmaxf(v0.z, v1.z), v2.z)};
+  const float org[3] = {cg.ox, cg.oy, cg.oz};
+  const int dim[3] = {cg.nx, cg.ny, cg.nz};
+  int a[3], b[3];
+  for (int k = 0; k < 3; ++k) {
+    a[k] = max(0, (int)floorf((lo[k] - thresh - org[k]) / cg.h - 0.5f) - 1);
+    b[k] = min(dim[k] - 1, (int)ceilf((hi[k] + thresh - org[k]) / cg.h - 0.5f) + 1);
+  }
+  const TriRec r = recs[t];
+  for (int z = a[2]; z <= b[2]; ++z)
+    for (int y = a[1]; y <= b[1]; ++y)
+      for (int x = a[0]; x <= b[0]; ++x) {
+        const size_t i = ((size_t)z * cg.ny + y) * cg.nx + x;
+        if (flags[i]) continue;
+        const F3 q = {cg.ox + ((float)x + 0.5f) * cg.h, cg.oy + ((float)y + 0.5f) * cg.h,
+                      cg.oz + ((float)z + 0.5f) * cg.h};
+        if (sqrtf(tri_d2<SLIVERS>(q, r)) <= thresh) flags[i] = 1u;
+      }
 }
 
 __global__ __launch_bounds__(256) void k_band_scatter(const uint32_t *__restrict__ flags,
@@ -2512,8 +2645,11 @@ __global__ __launch_bounds__(256) void k_band_scatter(const uint32_t *__restrict
   if (i == n - 1) *count = slots[i] + flags[i];
 }
 
-void launch_band_flags(const float *coarse, uint32_t nbrick, float thresh, uint32_t *flags, hipStream_t s) {
-  if (nbrick) k_band_flags<<<nb(nbrick), 256, 0, s>>>(coarse, nbrick, thresh, flags);
+void launch_band_splat(const float4 *tris, const TriRec *recs, int n, const GridArgs &cg, float thresh, bool slivers,
+                       uint32_t *flags, hipStream_t s) {
+  if (n <= 0) return;
+  if (slivers) k_band_splat<true><<<nb(n), 256, 0, s>>>(tris, recs, n, cg, thresh, flags);
+  else k_band_splat<false><<<nb(n), 256, 0, s>>>(tris, recs, n, cg, thresh, flags);
 }
 
 void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t nbrick, int32_t *index,

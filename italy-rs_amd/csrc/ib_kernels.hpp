@@ -72,7 +72,6 @@ struct SdfArgs {
   float *out;
   const int32_t *band_index;    // narrow band: slot of every 8^3 brick of the grid (-1: none); null: dense out
   const uint32_t *band_bricks;  // narrow band: brick linear index of every slot (2 block-tiles per brick)
-  int unsigned_only;            // 1: store |d| only (brick selection pass; no columns needed)
   uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
   uint32_t amb_cap;    // capacity of amb_list
   uint32_t sing_cap;     // voxels on a hole cap summed by the chunked launches (0: by the resolving wave)
@@ -163,7 +162,9 @@ void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, 
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s);
 // narrow band: bricks whose centre distance (from the coarse pass) is <= thresh
-void launch_band_flags(const float *coarse, uint32_t nbrick, float thresh, uint32_t *flags, hipStream_t s);
+// flags[b] = 1 for every brick centre of the coarse grid cg within thresh of some triangle (flags cleared first)
+void launch_band_splat(const float4 *tris, const TriRec *recs, int n, const GridArgs &cg, float thresh, bool slivers,
+                       uint32_t *flags, hipStream_t s);
 void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t nbrick, int32_t *index,
                          uint32_t *bricks, uint32_t *count, hipStream_t s);
 // FP64 winding recheck, parity tie-break and column-overflow recount for the listed voxels
