@@ -1739,7 +1739,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     SdfArgs sa;
     const uint32_t amb_cap = (uint32_t)std::min<uint64_t>(nvox, nvox / 16 + 4096);
     IB_TRY(prepare_query(c, mesh[j], jb.sdf, jb.d_distances, scs + j, &arena, amb_cap, sa, s, false));
-    col_prefix.push_back(col_prefix.back() + column_tiles(sa.col.g));
+    col_prefix.push_back(col_prefix.back() + column_blocks(sa.col));
     uint32_t tiles;
     if (band) {
       sa.band_index = jb.d_brick_index;

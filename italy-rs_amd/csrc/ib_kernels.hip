@@ -1966,130 +1966,95 @@ union TriRecBits {
   TriRec r;
 };
 
-// The +z crossings of the 8 x 8 columns (x0.., y0..) of a wave's tile: one walk of the tree for the whole
-// wave (a child is entered when its xy box holds any lane's column), node and triangle records read once
-// (scalar loads) and tested by every lane; then the cap fans the same way.  A column's crossings are the
-// same set as its own walk finds (only their order in the list differs).
-__device__ __forceinline__ void column_crossings_wave(const ColumnArgs &a, int x0, int y0, int *sstack) {
-  const int lane = threadIdx.x & 63;
-  const int x = x0 + (lane & 7), y = y0 + (lane >> 3);
-  const bool valid = x < a.g.nx && y < a.g.ny;
-  if (__ballot(valid) == 0) return;
+// Column crossings by rasterising the triangles: every mesh triangle and every cap fan (reversed) tests the
+// column centres inside its xy box (one more column on each side for rounding) with the crossing test of the
+// column walk, and appends each crossing to its column's list (an atomic slot; the list is a set: every use
+// sums the crossings above a point, in any order).  A triangle over more than 16 columns is shared by the
+// wave's lanes.  Counts start at zero.
+__device__ __forceinline__ void column_append(const ColumnArgs &a, int x, int y, F3 p0, F3 p1, F3 p2, int sign) {
   const float X = a.g.ox + ((float)x + 0.5f) * a.g.h;
   const float Y = a.g.oy + ((float)y + 0.5f) * a.g.h;
+  if (!(X >= fminf(fminf(p0.x, p1.x), p2.x) && X <= fmaxf(fmaxf(p0.x, p1.x), p2.x) &&
+        Y >= fminf(fminf(p0.y, p1.y), p2.y) && Y <= fmaxf(fmaxf(p0.y, p1.y), p2.y)))
+    return;
+  float z;
+  const int sg = column_cross(X, Y, p0, p1, p2, &z);
+  if (!sg) return;
   const size_t col = (size_t)y * a.g.nx + x;
-  ColumnSink sink{a.cross + (valid ? col : 0) * kColCap, 0};
-  auto tri = [&](int l) {
-    const CV4 *r = (const CV4 *)(a.tris + 3 * (size_t)l);
-    const V4 p0 = r[0], p1 = r[1], p2 = r[2];
-    const float4 t[3] = {make_float4(p0.x, p0.y, p0.z, p0.w), make_float4(p1.x, p1.y, p1.z, p1.w),
-                         make_float4(p2.x, p2.y, p2.z, p2.w)};
-    if (!valid || !column_in_xy_box(X, Y, t)) return;
-    float z;
-    const int sg = column_cross(X, Y, f3(t[0]), f3(t[1]), f3(t[2]), &z);
-    if (sg) sink.add(z, sg);
-  };
-  int sp = 0, cur = 0;
-  for (int visits = 0; visits <= a.nnodes; ++visits) {
-    const Node nd = load_node_uniform(a.nodes, cur);
-    const bool in0 = valid && X >= nd.lo0[0] && X <= nd.hi0[0] && Y >= nd.lo0[1] && Y <= nd.hi0[1];
-    const bool in1 = valid && X >= nd.lo1[0] && X <= nd.hi1[0] && Y >= nd.lo1[1] && Y <= nd.hi1[1];
-    int next = -1;
-    if (__ballot(in0) != 0) {
-      if (nd.c0 < 0) for_leaf_code(~nd.c0, tri); else next = nd.c0;
-    }
-    if (__ballot(in1) != 0) {
-      if (nd.c1 < 0) {
-        for_leaf_code(~nd.c1, tri);
-      } else if (next < 0) {
-        next = nd.c1;
-      } else if (sp < kStack4) {
-        if (lane == 0) sstack[sp] = nd.c1;
-        ++sp;
-      } else {
-        if (lane == 0) stack_overflow(a.sc);
-      }
-    }
-    if (next >= 0) {
-      cur = next;
-      continue;
-    }
-    if (sp == 0) break;
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
-  }
-  if (a.ncap > 0 && a.ncapnode > 0 && a.cap_nodes) {  // caps enter the closed surface reversed
-    typedef const __attribute__((address_space(4))) float CF;
-    auto fan = [&](int i) {
-      CF *p = (CF *)(a.caps + 9 * (size_t)i);
-      float t[9];
-#pragma unroll
-      for (int j = 0; j < 9; ++j) t[j] = p[j];
-      if (!valid || !(X >= fminf(fminf(t[0], t[3]), t[6]) && X <= fmaxf(fmaxf(t[0], t[3]), t[6]) &&
-                      Y >= fminf(fminf(t[1], t[4]), t[7]) && Y <= fmaxf(fmaxf(t[1], t[4]), t[7])))
-        return;
-      float z;
-      const int sg = column_cross(X, Y, {t[0], t[1], t[2]}, {t[3], t[4], t[5]}, {t[6], t[7], t[8]}, &z);
-      if (sg) sink.add(z, -sg);
-    };
-    sp = 0;
-    cur = 0;
-    for (;;) {
-      if (cur < 0) {
-        fan(~cur);
-      } else {
-        const CapNode nd = load_capnode_uniform(a.cap_nodes, cur);
-        const bool in = valid && X >= nd.lo[0] && X <= nd.hi[0] && Y >= nd.lo[1] && Y <= nd.hi[1];
-        if (__ballot(in) != 0) {
-          if (nd.left < 0 && nd.right < 0) {
-            const int first = -1 - nd.left;
-            for (int i = 0; i < -nd.right; ++i) fan(first + i);
-          } else if (sp < kCapStack) {
-            if (lane == 0) sstack[sp] = nd.right;
-            ++sp;
-            cur = nd.left;
-            continue;
-          } else {
-            if (lane == 0) stack_overflow(a.sc);
-          }
-        }
-      }
-      if (sp == 0) break;
-      --sp;
-      __builtin_amdgcn_wave_barrier();
-      cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
-    }
-  }
-  if (!valid) return;
-  a.count[col] = sink.cnt;
-  if (sink.cnt > kColCap) atomicAdd(&a.sc->col_overflow, 1u);
+  const int k = atomicAdd(&a.count[col], 1);
+  if (k < kColCap) a.cross[col * kColCap + k] = make_float2(z, (float)(sg * sign));
+  else if (k == kColCap) atomicAdd(&a.sc->col_overflow, 1u);
 }
 
-// 16 x 16 columns per block, an 8 x 8 tile per wave
-__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) {
-  __shared__ int sstack[4][kStack4];
-  const int wv = threadIdx.x >> 6;
-  column_crossings_wave(a, blockIdx.x * 16 + (wv & 1) * 8, blockIdx.y * 16 + (wv >> 1) * 8, sstack[wv]);
+// first and last column index whose centre can lie in [lo, hi] along an axis of origin o, spacing h, n columns
+__device__ __forceinline__ void column_span(float lo, float hi, float o, float h, int n, int &a, int &b) {
+  const float fa = fminf(fmaxf(floorf((lo - o) / h - 0.5f), -2.0f), (float)n + 2.0f);
+  const float fb = fminf(fmaxf(ceilf((hi - o) / h - 0.5f), -2.0f), (float)n + 2.0f);
+  a = max((int)fa - 1, 0);
+  b = min((int)fb + 1, n - 1);
 }
 
-// the columns of every job of a batch in one launch: block b is 16 x 16 columns of the job whose
-// range of the column-tile prefix holds b
+// triangle or fan i (i >= ntri: fan i - ntri) of the job, one per lane
+__device__ __forceinline__ void column_splat(const ColumnArgs &a, int i) {
+  const int lane = threadIdx.x & 63;
+  const bool have = i < a.ntri + a.ncap;
+  F3 p0 = {0.0f, 0.0f, 0.0f}, p1 = p0, p2 = p0;
+  int sign = 1;
+  if (have) {
+    if (i < a.ntri) {
+      p0 = f3(a.tris[3 * (size_t)i]);
+      p1 = f3(a.tris[3 * (size_t)i + 1]);
+      p2 = f3(a.tris[3 * (size_t)i + 2]);
+    } else {
+      const float *t = a.caps + 9 * (size_t)(i - a.ntri);
+      p0 = {t[0], t[1], t[2]};
+      p1 = {t[3], t[4], t[5]};
+      p2 = {t[6], t[7], t[8]};
+      sign = -1;  // caps enter the closed surface reversed
+    }
+  }
+  int x0, x1, y0, y1;
+  column_span(fminf(fminf(p0.x, p1.x), p2.x), fmaxf(fmaxf(p0.x, p1.x), p2.x), a.g.ox, a.g.h, a.g.nx, x0, x1);
+  column_span(fminf(fminf(p0.y, p1.y), p2.y), fmaxf(fmaxf(p0.y, p1.y), p2.y), a.g.oy, a.g.h, a.g.ny, y0, y1);
+  const int w = x1 - x0 + 1, hgt = y1 - y0 + 1;
+  const int ncols = (have && w > 0 && hgt > 0) ? w * hgt : 0;
+  const bool big = ncols > 16;
+  if (!big)
+    for (int c = 0; c < ncols; ++c) column_append(a, x0 + c % w, y0 + c / w, p0, p1, p2, sign);
+  unsigned long long m = __ballot(big);
+  while (m) {
+    const int b = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    const F3 q0 = {__shfl(p0.x, b), __shfl(p0.y, b), __shfl(p0.z, b)};
+    const F3 q1 = {__shfl(p1.x, b), __shfl(p1.y, b), __shfl(p1.z, b)};
+    const F3 q2 = {__shfl(p2.x, b), __shfl(p2.y, b), __shfl(p2.z, b)};
+    const int bx = __shfl(x0, b), by = __shfl(y0, b), bw = __shfl(w, b), bn = __shfl(ncols, b), bs = __shfl(sign, b);
+    for (int c = lane; c < bn; c += 64) column_append(a, bx + c % bw, by + c / bw, q0, q1, q2, bs);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_columns(ColumnArgs a) { column_splat(a, blockIdx.x * 256 + threadIdx.x); }
+
+// the columns of every job of a batch in one launch: block b rasterises 256 triangles and fans of the job
+// whose range of the block prefix holds b
 __global__ __launch_bounds__(256) void k_columns_batch(const SdfArgs *__restrict__ jobs,
                                                        const uint32_t *__restrict__ prefix, uint32_t njobs) {
-  __shared__ int sstack[4][kStack4];
   const uint32_t b = blockIdx.x;
   int lo = 0, hi = (int)njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
     if (prefix[mid] <= b) lo = mid; else hi = mid - 1;
   }
-  const ColumnArgs &a = jobs[lo].col;
-  const uint32_t local = b - prefix[lo], tx = (uint32_t)(a.g.nx + 15) / 16;
-  const int wv = threadIdx.x >> 6;
-  column_crossings_wave(a, (int)(local % tx) * 16 + (wv & 1) * 8, (int)(local / tx) * 16 + (wv >> 1) * 8,
-                        sstack[wv]);
+  column_splat(jobs[lo].col, (int)(b - prefix[lo]) * 256 + threadIdx.x);
 }
+
+// zero every job's column counts (blockIdx.y = job)
+__global__ __launch_bounds__(256) void k_zero_counts_batch(const SdfArgs *__restrict__ jobs) {
+  const ColumnArgs &a = jobs[blockIdx.y].col;
+  const size_t ncol = (size_t)a.g.nx * a.g.ny;
+  for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < ncol; i += (size_t)gridDim.x * 256) a.count[i] = 0;
+}
+
 
 
 
@@ -2527,15 +2492,18 @@ void launch_vox_out(const unsigned long long *keys, const uint32_t *perm, uint32
                     int32_t *cells, int32_t *first, float *colors, hipStream_t s) {
   if (n) k_vox_out<<<nb(n), 256, 0, s>>>(keys, perm, n, nx, ny, rgb, cells, first, colors);
 }
-uint32_t column_tiles(const GridArgs &g) { return nb(g.nx, 16) * nb(g.ny, 16); }
-void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t tiles,
+void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t blocks,
                           hipStream_t s) {
-  if (tiles) k_columns_batch<<<tiles, 256, 0, s>>>(jobs, prefix, njobs);
+  if (!njobs) return;
+  k_zero_counts_batch<<<dim3(64, njobs), 256, 0, s>>>(jobs);
+  if (blocks) k_columns_batch<<<blocks, 256, 0, s>>>(jobs, prefix, njobs);
 }
 void launch_columns(const ColumnArgs &a, hipStream_t s) {
-  dim3 grid(nb(a.g.nx, 16), nb(a.g.ny, 16));
-  k_columns<<<grid, 256, 0, s>>>(a);
+  fill_u32(reinterpret_cast<uint32_t *>(a.count), 0u, (uint64_t)a.g.nx * a.g.ny, s);
+  const uint64_t n = (uint64_t)a.ntri + (uint64_t)a.ncap;
+  if (n) k_columns<<<nb(n), 256, 0, s>>>(a);
 }
+uint32_t column_blocks(const ColumnArgs &a) { return nb((uint64_t)a.ntri + (uint64_t)a.ncap); }
 static const int kShapeHost[kNumShapes][3] = {{4, 4, 1}, {8, 4, 1}, {8, 8, 1}, {16, 4, 1}, {4, 4, 2}};
 
 void tile_dims(const GridArgs &g, int shape, uint32_t d[3]) {

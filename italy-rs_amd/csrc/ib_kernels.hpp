@@ -138,8 +138,8 @@ void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *chil
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 // 16 x 16 column tiles of a grid, and the columns of a whole batch in one launch (prefix: first
 // column tile of each job, njobs + 1 entries on the device)
-uint32_t column_tiles(const GridArgs &g);
-void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t tiles, hipStream_t s);
+uint32_t column_blocks(const ColumnArgs &a);  // blocks of the column rasteriser for one job
+void launch_columns_batch(const SdfArgs *jobs, const uint32_t *prefix, uint32_t njobs, uint32_t blocks, hipStream_t s);
 void launch_columns(const ColumnArgs &a, hipStream_t s);
 // a batch of query jobs drained by one persistent launch
 struct BatchArgs {
