@@ -71,8 +71,13 @@ static_assert(sizeof(CapNode) == 64, "cap node record is one 64-byte line");
 // 30-bit codes split by sorted position), so a traversal that keeps one pending sibling per level fits 64
 constexpr int kCapLeaf = 4;
 constexpr int kCapStack = 64;
-// a cap node farther than this bound on |its solid angle| / 4pi is not evaluated, only bounded
+// a cap node whose far-field error bound (winding units) is at most kCapTau is not evaluated: its first-order
+// term is added and its bound counted
 constexpr float kCapTau = 1e-3f;
+// ... and a node whose bound is at most kCapTauHi is not evaluated either while the bounds of the nodes not
+// evaluated so far stay within kCapBudget (the sign needs |w - 1/2| >= 0.05 + bound)
+constexpr float kCapTauHi = 1e-2f;
+constexpr float kCapBudget = 0.2f;
 
 // ---------------------------------------------------------------------------------------------
 // triangle / AABB separating axis test: same expression trees as src/convert/tri_box_overlap.h

@@ -1351,9 +1351,28 @@ __device__ __forceinline__ float solid_angle_f32(F3 q, const float *t, bool &sin
   return 2.0f * atan2f(det, den);
 }
 
+// Far field of a cap node seen from q: the bound in winding units, the term w1 in steradians.  The node's
+// fans lie within r of c and have total area `area` and area vector `dip`.  First-order term w1 = dip . (c - q) / |c - q|^3:
+// the integrand n . v / |v|^3 changes by at most 2 |dx| / |v|^3 per unit move dx of the point on the fan (the
+// operator norm of the derivative of v / |v|^3), so |Omega - w1| <= 2 area r / (d - r)^3, d = |c - q|.
+// Without the term, |Omega| <= area / (d - r)^2.  Returns the smaller bound (with w1 = 0 for the second) plus
+// a relative margin for the FP32 evaluation of w1; infinite when q lies within r of c.
+__device__ __forceinline__ float cap_far(const CapNode &nd, F3 q, float &w1) {
+  const float inv4pi = (float)(1.0 / (4.0 * M_PI));
+  const float dx = nd.c[0] - q.x, dy = nd.c[1] - q.y, dz = nd.c[2] - q.z;
+  const float d2 = dx * dx + dy * dy + dz * dz, d = sqrtf(d2), dmin = d - nd.r;
+  w1 = 0.0f;
+  if (!(dmin > 0.0f)) return INFINITY;
+  const float b0 = nd.area * inv4pi / (dmin * dmin);
+  const float b1 = b0 * (2.0f * nd.r / dmin);
+  if (!(b1 < b0)) return b0;
+  w1 = (nd.dip[0] * dx + nd.dip[1] * dy + nd.dip[2] * dz) / (d2 * d);  // steradians, like the fans' sum
+  return b1 + 1e-5f * b0;
+}
+
 // Cap winding with a rigorous error bound: caps near q are summed exactly (FP32 solid angles,
-// Neumaier), every farther node only contributes its bound |Omega| <= area / dmin^2 (a surface of
-// that area seen from at least dmin) to `bound`, in winding units.
+// Neumaier), every farther node contributes its first-order term and its error bound (cap_far) to
+// `bound`, in winding units.
 __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, float &w, float &bound) {
   w = 0.0f;
   bound = 0.0f;
@@ -1375,14 +1394,14 @@ __device__ __forceinline__ void cap_winding_bounded(const ColumnArgs &a, F3 q, f
       continue;
     }
     const CapNode nd = a.cap_nodes[code];
-    const float dx = q.x - nd.c[0], dy = q.y - nd.c[1], dz = q.z - nd.c[2];
-    const float dmin = sqrtf(dx * dx + dy * dy + dz * dz) - nd.r;
-    if (dmin > 0.0f) {
-      const float b = nd.area * inv4pi / (dmin * dmin);
-      if (b <= kCapTau) {
-        ub += b;
-        continue;
-      }
+    float w1;
+    const float b = cap_far(nd, q, w1);
+    if (b <= kCapTau || (b <= kCapTauHi && ub + b <= kCapBudget)) {
+      ub += b;
+      const float t = sum + w1;
+      cmp += (fabsf(sum) >= fabsf(w1)) ? (sum - t) + w1 : (w1 - t) + sum;
+      sum = t;
+      continue;
     }
     if (nd.left < 0 && nd.right < 0) {
       const int first = -1 - nd.left;
@@ -1887,8 +1906,8 @@ __device__ __forceinline__ CapNode load_capnode_uniform(const CapNode *p, int i)
 }
 
 // The caps' winding number at every needed voxel of a wave's brick (V per lane), with cap_winding_bounded's
-// bound, by one walk of the cap tree for the whole wave: a node is bounded (not entered) when its bound
-// area / (4pi dmin^2) is at most kCapTau for every needed voxel, else entered by all; fans are read once
+// bound, by one walk of the cap tree for the whole wave: a node is not entered when its far field (cap_far)
+// is within the thresholds (kCapTau, kCapTauHi) for every needed voxel, else entered by all; fans are read once
 // (scalar loads) and evaluated by every lane.  The stack is the wave's LDS stack, free after the
 // closest-triangle walk.
 template <int V>
@@ -1927,16 +1946,19 @@ __device__ __forceinline__ void cap_winding_wave(const ColumnArgs &a, const F3 *
       fan(~cur);
     } else {
       const CapNode nd = load_capnode_uniform(a.cap_nodes, cur);
-      float b[V];
+      float b[V], w1[V];
       bool open = false;
       for (int k = 0; k < V; ++k) {
-        const float dx = q[k].x - nd.c[0], dy = q[k].y - nd.c[1], dz = q[k].z - nd.c[2];
-        const float dmin = sqrtf(dx * dx + dy * dy + dz * dz) - nd.r;
-        b[k] = dmin > 0.0f ? nd.area * inv4pi / (dmin * dmin) : INFINITY;
-        open = open || (need[k] && !(b[k] <= kCapTau));
+        b[k] = cap_far(nd, q[k], w1[k]);
+        open = open || (need[k] && !(b[k] <= kCapTau || (b[k] <= kCapTauHi && ub[k] + b[k] <= kCapBudget)));
       }
       if (__ballot(open) == 0) {
-        for (int k = 0; k < V; ++k) ub[k] += b[k];
+        for (int k = 0; k < V; ++k) {
+          ub[k] += b[k];
+          const float s = sum[k] + w1[k];
+          cmp[k] += (fabsf(sum[k]) >= fabsf(w1[k])) ? (sum[k] - s) + w1[k] : (w1[k] - s) + sum[k];
+          sum[k] = s;
+        }
       } else if (nd.left < 0 && nd.right < 0) {
         const int first = -1 - nd.left;
         for (int i = 0; i < -nd.right; ++i) fan(first + i);
