@@ -48,7 +48,7 @@ struct DevCache {
     const size_t step = 16ull << 20;
     return (n + step - 1) / step * step;
   }
-  uint64_t failures = 0;  // hipMalloc calls that failed even after the cache was returned
+  std::atomic<uint64_t> failures{0};  // hipMalloc calls that failed even after the cache was returned (read unlocked)
   void *get(size_t n) {
     const size_t k = klass(std::max<size_t>(n, 1));
     {
@@ -65,7 +65,6 @@ struct DevCache {
     if (hipMalloc(&p, k) == hipSuccess) return p;
     clear();  // out of device memory: give the cached blocks back and try once more
     if (hipMalloc(&p, k) == hipSuccess) return p;
-    std::lock_guard<std::mutex> lk(mu);
     ++failures;
     return nullptr;
   }
@@ -500,7 +499,8 @@ ib_status ib_server_submit(ib_server *s, const ib_request *r, uint64_t *ticket) 
   Ticket t;
   t.req = *r;
   t.t_submit = now_s();
-  ib_server::estimate(*r, t.est, t.resident);
+  uint64_t out_estimate = 0;  // what the result will hold; counted (as t.resident) only once it exists
+  ib_server::estimate(*r, t.est, out_estimate);
   std::lock_guard<std::mutex> lk(s->mu);
   const uint64_t id = s->next_id++;
   s->tickets.emplace(id, t);

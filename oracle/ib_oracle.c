@@ -738,7 +738,7 @@ int orc_bvh_build(const float *xyz, size_t ntri, orc_bvh *b) {
   /* stable sort by code: sort (code<<32 | index) */
   uint64_t *kv = (uint64_t *)malloc(sizeof(uint64_t) * n);
   for (int i = 0; i < n; ++i) kv[i] = ((uint64_t)codes[i] << 32) | (uint32_t)i;
-  qsort(kv, n, sizeof(uint64_t), cmp_pair);
+  if (n > 1) qsort(kv, n, sizeof(uint64_t), cmp_pair);
   b->order = (int32_t *)malloc(sizeof(int32_t) * n);
   b->codes = (uint32_t *)malloc(sizeof(uint32_t) * n);
   for (int i = 0; i < n; ++i) {
@@ -963,7 +963,7 @@ static int column_crossings(const qctx *c, float x, float y, crossing **out, int
     (*out)[cnt].s = -s;
     cnt++;
   }
-  qsort(*out, cnt, sizeof(crossing), cmp_cross);
+  if (cnt > 1) qsort(*out, (size_t)cnt, sizeof(crossing), cmp_cross);
   return cnt;
 }
 
@@ -1806,7 +1806,7 @@ int orc_watertight(const float *xyz, size_t ntri, const float bmin[3], const flo
       e[ne++] = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
     }
   }
-  qsort(e, ne, sizeof(uint64_t), cmp_u64);
+  if (ne > 1) qsort(e, ne, sizeof(uint64_t), cmp_u64);
   for (size_t i = 0; i < ne;) {
     size_t j = i;
     while (j < ne && e[j] == e[i]) ++j;
@@ -2112,7 +2112,8 @@ static int capset_build(const float *xyz, size_t ntri, capset *cs) {
     free(caps);
     return -1;
   }
-  cs->n = nc;
+  cs->n = nc;  /* nc >= 1: every unmatched edge gives one fan */
+  if (nc == 0) return 0;
   cs->sorted = (float *)malloc(sizeof(float) * 9 * nc);
   for (size_t i = 0; i < nc; ++i) memcpy(cs->sorted + 9 * i, caps + 9 * (size_t)cs->b.order[i], 9 * sizeof(float));
   free(caps);
