@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -65,6 +66,8 @@ struct ib_context {
   std::unordered_map<std::string, Buf> bufs;
   std::unordered_map<void *, size_t> user;
   Scalars *host_sc = nullptr;  // pinned
+  char *staging = nullptr;     // pinned staging of large device -> host results (grown on demand)
+  size_t staging_bytes = 0;
   hipEvent_t ev[EV_COUNT] = {};
   bool ev_valid[EV_COUNT] = {};
   ib_stats stats{};
@@ -618,6 +621,64 @@ ib_status query_dense(ib_context *c, const MeshDev &m, const ib_grid &g, float *
   return IB_OK;
 }
 
+// Device -> host copies of large results into the caller's (pageable) buffers: one DMA of all ranges into a
+// pinned staging buffer the context keeps (grown to the largest result), then the host copies the staging
+// out on several threads.  A pageable destination otherwise goes through the runtime's own small bounce
+// buffers at a fraction of the link's rate.  Small results, and a failed staging allocation, copy directly.
+// Waits for the stream.
+struct HostCopy {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+ib_status download(ib_context *c, const HostCopy *cp, int n, hipStream_t s) {
+  size_t total = 0;
+  for (int i = 0; i < n; ++i) total += (cp[i].bytes + 255) & ~size_t(255);
+  if (total >= (8u << 20) && c->staging_bytes < total) {
+    if (c->staging) (void)hipHostFree(c->staging);
+    c->staging = nullptr;
+    c->staging_bytes = 0;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, total, hipHostMallocDefault) == hipSuccess) {
+      c->staging = static_cast<char *>(p);
+      c->staging_bytes = total;
+    }
+  }
+  if (total < (8u << 20) || !c->staging) {
+    for (int i = 0; i < n; ++i)
+      if (cp[i].bytes) IB_HIP(hipMemcpyAsync(cp[i].dst, cp[i].src, cp[i].bytes, hipMemcpyDeviceToHost, s));
+    IB_HIP(hipStreamSynchronize(s));
+    return IB_OK;
+  }
+  std::vector<size_t> off(n + 1, 0);
+  for (int i = 0; i < n; ++i) {
+    off[i + 1] = off[i] + ((cp[i].bytes + 255) & ~size_t(255));
+    if (cp[i].bytes)
+      IB_HIP(hipMemcpyAsync(c->staging + off[i], cp[i].src, cp[i].bytes, hipMemcpyDeviceToHost, s));
+  }
+  IB_HIP(hipStreamSynchronize(s));
+  // the bytes of all ranges, cut into equal slices, one per thread
+  size_t sum = 0;
+  for (int i = 0; i < n; ++i) sum += cp[i].bytes;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nt = (int)std::min<size_t>(std::min(8u, hw), std::max<size_t>(1, sum >> 23));
+  auto work = [&](int t) {
+    size_t lo = sum * t / nt, hi = sum * (t + 1) / nt, base = 0;
+    for (int i = 0; i < n && lo < hi; ++i) {
+      const size_t b0 = base, b1 = base + cp[i].bytes;
+      base = b1;
+      if (hi <= b0 || lo >= b1) continue;
+      const size_t a = std::max(lo, b0), e = std::min(hi, b1);
+      std::memcpy(static_cast<char *>(cp[i].dst) + (a - b0), c->staging + off[i] + (a - b0), e - a);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto &x : th) x.join();
+  return IB_OK;
+}
+
 // empty mesh: no surface anywhere
 ib_status fill_empty(ib_context *c, const ib_grid *sdf, float *d_dist, const ib_grid *occ, uint32_t *d_occ,
                      hipStream_t s) {
@@ -819,6 +880,7 @@ void ib_destroy(ib_context *c) {
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   (void)hipHostFree(c->host_sc);
   if (c->host_batch_sc) (void)hipHostFree(c->host_batch_sc);
+  if (c->staging) (void)hipHostFree(c->staging);
   for (int i = 0; i < ib_context::kSide; ++i) {
     if (c->side[i]) (void)hipStreamSynchronize(c->side[i]);
     if (c->side[i]) (void)hipStreamDestroy(c->side[i]);
@@ -909,14 +971,14 @@ ib_status ib_bake_dense(ib_context *c, const float *xyz, uint64_t ntri, const ib
   if (occ) IB_TRY(c->scratch("out_occ", nwords, &d_occ));
   if (ntri) IB_HIP(hipMemcpyAsync(d_xyz, xyz, 9 * ntri * sizeof(float), hipMemcpyHostToDevice, s));
   IB_TRY(bake_device(c, d_xyz, ntri, sdf, d_dist, occ, d_occ, s));
-  if (sdf) IB_HIP(hipMemcpyAsync(dist, d_dist, nvox * sizeof(float), hipMemcpyDeviceToHost, s));
-  if (occ) IB_HIP(hipMemcpyAsync(occ_words, d_occ, nwords * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   if (ntri && sdf) {
     Scalars *sc;
     IB_TRY(c->scratch("sc", 1, &sc));
     IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   }
-  IB_HIP(hipStreamSynchronize(s));
+  const HostCopy cp[2] = {{dist, d_dist, sdf ? nvox * sizeof(float) : 0},
+                          {occ_words, d_occ, occ ? nwords * sizeof(uint32_t) : 0}};
+  IB_TRY(download(c, cp, 2, s));
   if (ntri && sdf && c->host_sc->stack_overflow)
     return fail(IB_ERR_INTERNAL, "a traversal stack overflowed (the tree exceeds its depth bound)");
   return IB_OK;
@@ -1169,11 +1231,10 @@ ib_status ib_splat_materials(ib_context *c, const float *xyz, uint64_t ntri, con
   }
   // the device buffer already has the host layout: rgb interleaved per cell (std::vector<glm::vec3>),
   // then the metallic and roughness planes
-  IB_HIP(hipMemcpyAsync(base_color, d_out, 3 * ncell * sizeof(float), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipMemcpyAsync(metallic, d_out + 3 * ncell, ncell * sizeof(float), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipMemcpyAsync(roughness, d_out + 4 * ncell, ncell * sizeof(float), hipMemcpyDeviceToHost, s));
-  IB_HIP(hipStreamSynchronize(s));
-  return IB_OK;
+  const HostCopy cp[3] = {{base_color, d_out, 3 * ncell * sizeof(float)},
+                          {metallic, d_out + 3 * ncell, ncell * sizeof(float)},
+                          {roughness, d_out + 4 * ncell, ncell * sizeof(float)}};
+  return download(c, cp, 3, s);
 }
 
 // ---- mesh ingestion gate (src/io/mesh_weld.cpp:26-48, src/io/mesh_validate.cpp:21-59) ---------------
@@ -1492,10 +1553,8 @@ ib_status ib_bake_analytic(ib_context *c, const ib_csg_op *prog, uint32_t nops, 
   IB_TRY(c->scratch("an_dist", n, &d_dist));
   if (branch) IB_TRY(c->scratch("an_branch", n, &d_branch));
   IB_TRY(bake_analytic(c, prog, nops, grid, d_dist, d_branch, s));
-  IB_HIP(hipMemcpyAsync(dist, d_dist, n * sizeof(float), hipMemcpyDeviceToHost, s));
-  if (branch) IB_HIP(hipMemcpyAsync(branch, d_branch, n, hipMemcpyDeviceToHost, s));
-  IB_HIP(hipStreamSynchronize(s));
-  return IB_OK;
+  const HostCopy cp[2] = {{dist, d_dist, n * sizeof(float)}, {branch, d_branch, branch ? n : 0}};
+  return download(c, cp, 2, s);
 }
 
 static ib_status grid_view(const ib_grid *grid, const float *d_distances, GridView &v) {
