@@ -149,7 +149,7 @@ int orc_bake_dense_strided(const float *xyz, size_t ntri, const float origin[3],
  * <= h (band + 3.5 sqrt 3)); brick_index (nx/8 * ny/8 * nz/8, x fastest) gets each brick's slot in order of
  * its linear index, or -1, and *nbricks the count.  Every brick_stride-th slot is baked into pool
  * (slot * 512 + ((z%8)*8 + y%8)*8 + x%8; the other slots are left as they are): FP32 closest-triangle
- * distance, sign by the closure identity with a single cone cap over all holes and an FP64 cap winding
+ * distance, sign by the closure identity with a fan cap over each boundary loop and an FP64 cap winding
  * number (bounded far field, full sum near 1/2, the mesh's own winding number on the cap, parity on ties).
  * Returns 0, -1 if pool_bricks < *nbricks (nothing baked), -2 for an empty mesh or grid dims not
  * divisible by 8. */
