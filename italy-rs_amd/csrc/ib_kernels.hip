@@ -1352,22 +1352,28 @@ __device__ __forceinline__ float solid_angle_f32(F3 q, const float *t, bool &sin
 }
 
 // Far field of a cap node seen from q: the bound in winding units, the term w1 in steradians.  The node's
-// fans lie within r of c and have total area `area` and area vector `dip`.  First-order term w1 = dip . (c - q) / |c - q|^3:
-// the integrand n . v / |v|^3 changes by at most 2 |dx| / |v|^3 per unit move dx of the point on the fan (the
-// operator norm of the derivative of v / |v|^3), so |Omega - w1| <= 2 area r / (d - r)^3, d = |c - q|.
-// Without the term, |Omega| <= area / (d - r)^2.  Returns the smaller bound (with w1 = 0 for the second) plus
-// a relative margin for the FP32 evaluation of w1; infinite when q lies within r of c.
+// fans lie within r of c and have total area `area` and area vector `dip`.  First-order term w1 = dip . (c - q) /
+// |c - q|^3: the integrand n . v / |v|^3 changes by at most 2 |dx| / |v|^3 per unit move dx of the point on the
+// fan (the operator norm of the derivative of v / |v|^3), so |Omega - w1| <= 2 area r / (d - r)^3, d = |c - q|.
+// Without the term, |Omega| <= area / (d - r)^2.  Returns the smaller bound (with w1 = 0 for the second);
+// infinite when q lies within r of c.  Hardware reciprocal square root and reciprocal (1 ulp each) instead of
+// correctly rounded sqrt and divides: d is rounded down by 2^-20 relative before r is subtracted, 1 / (d - r)
+// up by 2^-20, and the bound up by 2^-18, which covers every rounding of the chain; the evaluation of w1 errs
+// by less than 2^-21 |w1| <= 2^-21 area / (d - r)^2, inside the 1e-5 * area / (d - r)^2 added to its bound.
 __device__ __forceinline__ float cap_far(const CapNode &nd, F3 q, float &w1) {
   const float inv4pi = (float)(1.0 / (4.0 * M_PI));
   const float dx = nd.c[0] - q.x, dy = nd.c[1] - q.y, dz = nd.c[2] - q.z;
-  const float d2 = dx * dx + dy * dy + dz * dz, d = sqrtf(d2), dmin = d - nd.r;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  const float rd = __builtin_amdgcn_rsqf(d2);  // 1 / d (infinite at 0: then dmin is NaN)
+  const float dmin = d2 * rd * (1.0f - 0x1p-20f) - nd.r;
   w1 = 0.0f;
   if (!(dmin > 0.0f)) return INFINITY;
-  const float b0 = nd.area * inv4pi / (dmin * dmin);
-  const float b1 = b0 * (2.0f * nd.r / dmin);
-  if (!(b1 < b0)) return b0;
-  w1 = (nd.dip[0] * dx + nd.dip[1] * dy + nd.dip[2] * dz) / (d2 * d);  // steradians, like the fans' sum
-  return b1 + 1e-5f * b0;
+  const float id = __builtin_amdgcn_rcpf(dmin) * (1.0f + 0x1p-20f);
+  const float b0 = nd.area * inv4pi * id * id;
+  const float b1 = b0 * (2.0f * nd.r * id);
+  if (!(b1 < b0)) return b0 * (1.0f + 0x1p-18f);
+  w1 = (nd.dip[0] * dx + nd.dip[1] * dy + nd.dip[2] * dz) * (rd * rd * rd);  // steradians, like the fans' sum
+  return (b1 + 1e-5f * b0) * (1.0f + 0x1p-18f);
 }
 
 // Cap winding with a rigorous error bound: caps near q are summed exactly (FP32 solid angles,
