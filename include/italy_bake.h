@@ -74,6 +74,7 @@ typedef struct {
   double ms_query;       /* the closest-triangle query kernel alone (dense bakes; HIP events around its launch) */
   uint64_t obb_tests;    /* point/oriented-box tests of leaf blocks in the per-brick query (counting mode) */
   uint64_t on_cap;       /* rechecked voxels lying on a hole cap, decided by the mesh's own FP64 winding number */
+  uint64_t rechecked_full; /* rechecked voxels that the cap tree's FP64 bound left undecided: every cap fan summed */
 } ib_stats;
 
 /* Options for ib_set_option. */
@@ -88,8 +89,11 @@ typedef enum {
   IB_OPT_TILE_ORDER = 5,    /* launch order of a dense per-brick query's tiles: 0 grid order, 1 costliest
                                first (estimated from the column crossings), 2 (default) half-octave cost
                                buckets, costliest first, Morton order inside a bucket.  Speed only. */
-  IB_OPT_LEAF_BLOCK = 6     /* LBVH subtrees of at most this many triangles (1..32, default 8) are evaluated
+  IB_OPT_LEAF_BLOCK = 6,    /* LBVH subtrees of at most this many triangles (1..32, default 8) are evaluated
                                as one block from their parent.  Speed only; applies to later builds. */
+  IB_OPT_RESOLVE_FULL = 7   /* value != 0: every FP64 recheck of an open mesh sums all hole-cap fans instead
+                               of first bounding the far caps through the cap tree.  Signs are identical;
+                               for tests and measurement. */
 } ib_option;
 
 /* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy

@@ -79,6 +79,7 @@ struct ib_context {
   int blocks_per_cu = 8;       // persistent blocks per CU (autotuned for batches)
   int leaf_block = 8;          // LBVH subtrees of at most this many triangles are evaluated as one leaf block
   int tile_order = 2;          // dense per-brick queries: 0 grid order, 1 costliest first, 2 cost buckets + Morton
+  bool resolve_full = false;   // FP64 rechecks sum every cap fan (no cap-tree bound first)
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
   uint32_t batch_sc_cap = 0;
@@ -288,6 +289,7 @@ struct MeshDev {
   int ncap = 0;
   CapNode *cap_nodes = nullptr;
   int ncapnode = 0;
+  int4 *cap_cut = nullptr;
   int2 *bnd = nullptr;  // the weld's unmatched edges, until the caps are built
   int depth = 0;
   bool slivers = false;
@@ -455,6 +457,7 @@ ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint3
   m.ncap = (int)ne;
   m.ncapnode = ne > 1 ? (int)ne - 1 : 1;
   IB_TRY(take(c, arena, "cap_nodes", (size_t)m.ncapnode, &m.cap_nodes));
+  IB_TRY(take(c, arena, "cap_cut", (size_t)kCapCut, &m.cap_cut));
   // vertex ids are corner indices below 3n: keys of 32 + bits(3n - 1) bits
   int idbits = 1;
   while (idbits < 32 && ((uint64_t)3 * m.n - 1) >> idbits) ++idbits;
@@ -514,7 +517,7 @@ ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint3
   // 6. the tree (children, parents and ranges in acc's space)
   int2 *child = reinterpret_cast<int2 *>(w + 40 * st), *ranges = reinterpret_cast<int2 *>(w + 48 * st);
   int32_t *parent = reinterpret_cast<int32_t *>(w + 56 * st);
-  launch_cap_tree(m.caps, codes_s, (int)ne, child, parent, ranges, m.cap_nodes, s);
+  launch_cap_tree(m.caps, codes_s, (int)ne, child, parent, ranges, m.cap_nodes, m.cap_cut, s);
   return IB_OK;
 }
 
@@ -565,6 +568,7 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   ca.caps = m.caps;
   ca.ncap = m.ncap;
   ca.cap_nodes = m.cap_nodes;
+  ca.cap_cut = m.cap_cut;
   ca.ncapnode = m.ncapnode;
   ca.g = to_args(g);
   ca.sc = sc;
@@ -575,11 +579,13 @@ ib_status prepare_query(ib_context *c, const MeshDev &m, const ib_grid &g, float
   sa.col = ca;
   sa.out = d_out;
   sa.amb_cap = amb_cap;
+  sa.resolve_full = c->resolve_full ? 1u : 0u;
   IB_TRY(take(c, arena, "amb_list", amb_cap, &sa.amb_list));
-  if (m.ncap > 0) {  // only hole caps can put a voxel on a cap
+  if (m.ncap > 0) {  // only hole caps put a voxel on a cap or leave its FP64 recheck to the chunked sums
     sa.sing_cap = std::min<uint32_t>(amb_cap, kSingCap);
-    sa.sing_chunks = (uint32_t)((m.n + kSingChunk - 1) / kSingChunk);
+    sa.sing_chunks = (uint32_t)((std::max(m.n, m.ncap) + kSingChunk - 1) / kSingChunk);
     IB_TRY(take(c, arena, "sing_list", sa.sing_cap, &sa.sing));
+    IB_TRY(take(c, arena, "sing_kind", sa.sing_cap, &sa.sing_kind));
     IB_TRY(take(c, arena, "sing_part", (size_t)sa.sing_cap * sa.sing_chunks, &sa.sing_part));
   }
   return IB_OK;
@@ -1638,7 +1644,8 @@ ib_status ib_last_stats(ib_context *c, ib_stats *out) {
     st.tri_tests = h.tri_tests;
     st.stack_overflows = h.stack_overflow;
     st.obb_tests = h.obb_tests;
-    st.on_cap = h.singular;
+    st.on_cap = h.on_cap;
+    st.rechecked_full = h.rechecked_full;
   }
   *out = st;
   return IB_OK;
@@ -1849,7 +1856,8 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.box_tests += h.box_tests;
     c->stats.tri_tests += h.tri_tests;
     c->stats.obb_tests += h.obb_tests;
-    c->stats.on_cap += h.singular;
+    c->stats.on_cap += h.on_cap;
+    c->stats.rechecked_full += h.rechecked_full;
     c->stats.stack_overflows += h.stack_overflow;
     if (h.stack_overflow)
       return fail(IB_ERR_INTERNAL, "job " + std::to_string(j) + ": a traversal stack overflowed");
@@ -1968,6 +1976,9 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
       if (value < 1 || value > kMaxLeafBlock)
         return fail(IB_ERR_INVALID_ARGUMENT, "leaf block must be 1.." + std::to_string(kMaxLeafBlock));
       c->leaf_block = (int)value;
+      return IB_OK;
+    case IB_OPT_RESOLVE_FULL:
+      c->resolve_full = value != 0;
       return IB_OK;
     case IB_OPT_TILE_ORDER:
       if (value < 0 || value > 2) return fail(IB_ERR_INVALID_ARGUMENT, "tile order must be 0..2");

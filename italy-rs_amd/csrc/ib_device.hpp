@@ -78,6 +78,14 @@ constexpr float kCapTau = 1e-3f;
 // evaluated so far stay within kCapBudget (the sign needs |w - 1/2| >= 0.05 + bound)
 constexpr float kCapTauHi = 1e-2f;
 constexpr float kCapBudget = 0.2f;
+// the FP64 recheck sums the caps over a cut of their tree first: a node of the cut whose far-field error
+// bound is at most kCapTau64 adds its first-order term and its bound; the voxel is decided there when |w|
+// stays on one side of 1/2 and of the tie band's edges by more than the total bound, and every fan is
+// summed otherwise
+constexpr double kCapTau64 = 1e-4;
+// the cut of the cap tree that the FP64 recheck tests: the nodes (or fans) kCapCutBits levels below the root
+constexpr int kCapCutBits = 10;
+constexpr int kCapCut = 1 << kCapCutBits;
 
 // ---------------------------------------------------------------------------------------------
 // triangle / AABB separating axis test: same expression trees as src/convert/tri_box_overlap.h

@@ -990,6 +990,38 @@ __global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ cap
   nodes[rec] = nd;
 }
 
+// The cut of the cap tree at depth kCapCutBits: slot s follows the path that its bits spell from the root
+// (most significant first, 0 = left); a node or fan reached in fewer steps belongs to the slot whose
+// remaining bits are zero, and the other slots on that path stay empty.  Each slot holds (code, first fan,
+// fan count, 0): the slots partition the fans into contiguous ranges of the sorted cap soup.
+__global__ __launch_bounds__(256) void k_cap_cut(const CapNode *__restrict__ nodes, const int2 *__restrict__ ranges,
+                                                 int4 *cut) {
+  const int slot = blockIdx.x * 256 + threadIdx.x;
+  if (slot >= kCapCut) return;
+  int code = 0;  // record 0 is the root (a leaf when the caps have at most kCapLeaf fans)
+  bool mine = true;
+  for (int k = kCapCutBits - 1; k >= 0; --k) {
+    const int bit = (slot >> k) & 1;
+    const bool inner = code >= 0 && !(nodes[code].left < 0 && nodes[code].right < 0);
+    if (inner) code = bit ? nodes[code].right : nodes[code].left;
+    else if (bit) mine = false;
+  }
+  int first = 0, count = 0;
+  if (mine) {
+    if (code < 0) {
+      first = ~code;
+      count = 1;
+    } else if (nodes[code].left < 0 && nodes[code].right < 0) {
+      first = -1 - nodes[code].left;
+      count = -nodes[code].right;
+    } else {
+      first = ranges[code].x;
+      count = ranges[code].y - ranges[code].x + 1;
+    }
+  }
+  cut[slot] = make_int4(code, first, count, 0);
+}
+
 // ============================================================================================
 // surface occupancy: voxelize.cpp:64-100 per triangle, bit-packed with atomicOr
 // ============================================================================================
@@ -1461,20 +1493,87 @@ __device__ __forceinline__ double wave_sum_f64(double sum, double comp) {
   return __shfl(v + e, 0);
 }
 
-// FP64 winding number of the caps at q, computed by a whole wave: cap i goes to lane i % 64, each
-// lane keeps a Neumaier-compensated sum, and the 64 partial sums are combined by wave_sum_f64.
-// `singular` (wave-uniform) is set when q lies on or next to a cap triangle.
-__device__ __noinline__ double cap_winding_f64_wave(F3 q, const float *caps, int ncap, bool &singular) {
+// FP64 sum of the caps' solid angles [c0, c1) at q (steradians), computed by a whole wave: cap i goes to
+// lane i % 64, each lane keeps a Neumaier-compensated sum, and the 64 partial sums are combined by
+// wave_sum_f64.  `singular` (wave-uniform) is set when q lies on or next to one of those cap triangles.
+__device__ __noinline__ double cap_solid_angle_f64_wave(F3 q, const float *caps, int c0, int c1, bool &singular) {
   const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0;
   bool sing = false;
-  for (int i = lane; i < ncap; i += 64) {
-    const double om = solid_angle_f64(q, caps + 9 * i, sing);
+  for (int i = c0 + lane; i < c1; i += 64) {
+    const double om = solid_angle_f64(q, caps + 9 * (size_t)i, sing);
     const double s = sum + om;
     comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
     sum = s;
   }
   singular = __ballot(sing) != 0;
+  return wave_sum_f64(sum, comp);
+}
+__device__ __forceinline__ double cap_winding_f64_wave(F3 q, const float *caps, int ncap, bool &singular) {
+  return cap_solid_angle_f64_wave(q, caps, 0, ncap, singular) * (1.0 / (4.0 * M_PI));
+}
+
+__device__ __forceinline__ void neumaier_add(double &sum, double &comp, double v) {
+  const double t = sum + v;
+  comp += (fabs(sum) >= fabs(v)) ? (sum - t) + v : (v - t) + sum;
+  sum = t;
+}
+
+// Far field of a cap node seen from q in FP64, with cap_far's bounds: returns the error bound (winding
+// units), w1 the first-order term (steradians); infinite when q lies within r of c.  The node's fields are
+// floats, r and area rounded up; the area vector's rounding (at most 2^-24 |dip| <= 2^-24 area per
+// component) moves w1 by less than 1.1e-7 of the zero-order bound, inside the 1e-6 of it added.
+__device__ __forceinline__ double cap_far_f64(const CapNode &nd, F3 q, double &w1) {
+  const double dx = (double)nd.c[0] - q.x, dy = (double)nd.c[1] - q.y, dz = (double)nd.c[2] - q.z;
+  const double d = sqrt(dx * dx + dy * dy + dz * dz);
+  const double dmin = d - (double)nd.r;
+  w1 = 0.0;
+  if (!(dmin > 0.0)) return INFINITY;
+  const double b0 = (double)nd.area / (4.0 * M_PI * dmin * dmin);
+  const double b1 = b0 * (2.0 * (double)nd.r / dmin);
+  if (!(b1 < b0)) return b0 * (1.0 + 1e-12);
+  w1 = ((double)nd.dip[0] * dx + (double)nd.dip[1] * dy + (double)nd.dip[2] * dz) / (d * d * d);
+  return (b1 + 1e-6 * b0) * (1.0 + 1e-12);
+}
+
+// FP64 winding number of the caps at q with a rigorous error bound, by a whole wave, over the cut of the
+// cap tree (k_cap_cut: kCapCut slots, each a node, a fan or nothing, partitioning the fans).  Lane l tests
+// slots l * kCapCut / 64 ..: a node whose bound (cap_far_f64) is at most kCapTau64 adds its first-order
+// term and its bound; the fans of every other slot are summed exactly by the whole wave, slot by slot in
+// slot order.  Returns w_C; `bound` is the total error bound in winding units (the sums' FP64 rounding
+// included); `singular` as for cap_winding_f64_wave: a fan inside a bounded node lies farther from q than
+// the node's diameter (or its solid angle is below area / d^2 < pi), where the solid angle is far from
+// singular.  Sums run in a fixed order: per lane, then wave_sum_f64.
+constexpr int kCutPerLane = kCapCut / 64;
+__device__ __noinline__ double cap_winding_f64_cut(F3 q, const ColumnArgs &a, double &bound, bool &singular) {
+  const int lane = threadIdx.x & 63;
+  double sum = 0.0, comp = 0.0, ub = 0.0;
+  bool sing = false;
+  uint32_t near = 0;  // bit j: slot lane * kCutPerLane + j is summed exactly
+  for (int j = 0; j < kCutPerLane; ++j) {
+    const int4 e = a.cap_cut[lane * kCutPerLane + j];
+    if (e.z == 0) continue;
+    if (e.x >= 0) {
+      double w1;
+      const double b = cap_far_f64(a.cap_nodes[e.x], q, w1);
+      if (b <= kCapTau64) {
+        ub += b;
+        neumaier_add(sum, comp, w1);
+        continue;
+      }
+    }
+    near |= 1u << j;
+  }
+  for (uint64_t lanes = __ballot(near != 0); lanes; lanes &= lanes - 1) {
+    const int l = __builtin_ctzll(lanes);
+    for (uint32_t bits = (uint32_t)__shfl((int)near, l); bits; bits &= bits - 1) {
+      const int4 e = a.cap_cut[l * kCutPerLane + __builtin_ctz(bits)];
+      for (int t = e.y + lane; t < e.y + e.z; t += 64)
+        neumaier_add(sum, comp, solid_angle_f64(q, a.caps + 9 * (size_t)t, sing));
+    }
+  }
+  singular = __ballot(sing) != 0;
+  bound = wave_sum(ub) * (1.0 + 1e-9) + 1e-10;
   return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
 }
 
@@ -1741,99 +1840,120 @@ __device__ __forceinline__ bool decide_w64(const SdfArgs &a, double w64, F3 q) {
   return parity_inside(a.col.nodes, a.col.nnodes, a.col.tris, q, a.col.sc);
 }
 
+// the signed count of +z crossings above q on its column (the closure identity's integer)
+__device__ __forceinline__ int column_wi(const ColumnArgs &c, int x, int y, F3 q) {
+  const size_t col = (size_t)y * c.g.nx + x;
+  const int cnt = c.count[col];
+  if (cnt > kColCap) return column_direct(c, q.x, q.y, q.z);
+  int wi = 0;
+  const float2 *cr = c.cross + col * kColCap;
+  for (int k = 0; k < cnt; ++k)
+    if (cr[k].x > q.z) wi += (int)cr[k].y;
+  return wi;
+}
+
+__device__ __forceinline__ F3 voxel_centre(const GridArgs &g, uint32_t vox, int &x, int &y, int &z) {
+  x = (int)(vox % (uint32_t)g.nx);
+  y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
+  z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
+  return {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+}
+
+__device__ __forceinline__ void store_sign(const SdfArgs &a, int x, int y, int z, bool inside) {
+  const size_t oi = out_index(a, x, y, z);
+  const float d = a.out[oi];
+  a.out[oi] = inside ? -d : d;
+}
+
+// FP64 sums handed to the chunked launches (k_singular_parts / k_singular_finish): kind 0, a voxel on a
+// hole cap (the mesh's own winding number over its triangles); kind 1, a voxel the cut's bound left
+// undecided (the column's integer plus every cap fan).  Returns false when the list is full.
+__device__ __forceinline__ bool list_for_chunks(const SdfArgs &a, uint32_t vox, uint32_t kind) {
+  uint32_t slot = 0;
+  if ((threadIdx.x & 63) == 0) slot = atomicAdd(&a.col.sc->singular, 1u);
+  slot = __shfl(slot, 0);
+  if (slot >= a.sing_cap) return false;
+  if ((threadIdx.x & 63) == 0) {
+    a.sing[slot] = vox;
+    a.sing_kind[slot] = kind;
+  }
+  return true;
+}
+
 __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
-  const GridArgs &g = a.col.g;
   const int lane = threadIdx.x & 63;
   for (uint32_t i = wave; i < n; i += nwaves) {
     const uint32_t vox = a.amb_list[i];
-    const int x = (int)(vox % (uint32_t)g.nx);
-    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
-    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
-    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
-    const size_t col = (size_t)y * g.nx + x;
-    const int cnt = a.col.count[col];
-    int wi = 0;
-    if (cnt <= kColCap) {
-      const float2 *cr = a.col.cross + col * kColCap;
-      for (int k = 0; k < cnt; ++k)
-        if (cr[k].x > q.z) wi += (int)cr[k].y;
-    } else {
-      wi = column_direct(a.col, q.x, q.y, q.z);
-    }
+    int x, y, z;
+    const F3 q = voxel_centre(a.col.g, vox, x, y, z);
+    const int wi = column_wi(a.col, x, y, q);
     bool inside;
     if (a.col.ncap == 0) {
       inside = wi != 0;
     } else {
       if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
-      bool on_cap = false;
-      double w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
+      // the cap tree's cut with its bound first: it decides when |w| stays on one side of 1/2 and of the
+      // tie band's edges by more than the bound, and it finds every voxel on a cap; the others sum every
+      // fan, in chunks across the chip (or here, when the chunk list is full or IB_OPT_RESOLVE_FULL is set)
+      bool on_cap = false, decided = false;
+      double w64 = 0.0;
+      if (!a.resolve_full) {
+        double bound;
+        w64 = (double)wi + cap_winding_f64_cut(q, a.col, bound, on_cap);
+        const double m = fabs(fabs(w64) - 0.5);
+        decided = on_cap || m >= kTieEps + bound || m <= kTieEps - bound;
+      }
+      if (!decided) {
+        if (lane == 0) atomicAdd(&a.col.sc->rechecked_full, 1u);
+        if (!a.resolve_full && list_for_chunks(a, vox, 1u)) continue;
+        w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
+      }
       if (on_cap) {
-        // the mesh's own winding number: summed in chunks by every wave of the next launch (k_singular),
-        // or here when the list is full
-        uint32_t slot = 0;
-        if (lane == 0) slot = atomicAdd(&a.col.sc->singular, 1u);
-        slot = __shfl(slot, 0);
-        if (slot < a.sing_cap) {
-          if (lane == 0) a.sing[slot] = vox;
-          continue;
-        }
+        if (lane == 0) atomicAdd(&a.col.sc->on_cap, 1u);
+        if (list_for_chunks(a, vox, 0u)) continue;
         w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
       }
       inside = decide_w64(a, w64, q);
     }
-    if (lane == 0) {
-      const size_t oi = out_index(a, x, y, z);
-      const float d = a.out[oi];
-      a.out[oi] = inside ? -d : d;
-    }
+    if (lane == 0) store_sign(a, x, y, z, inside);
   }
 }
 
-// Voxels on a hole cap (listed by resolve_range): partial FP64 solid-angle sums of the mesh over chunks of
-// kSingChunk triangles, one (voxel, chunk) item per wave (grid-stride), then per voxel the chunks' sum in
-// chunk order and the decision.  The partials depend only on the item, not on the wave that sums it.
+// The FP64 sums listed by resolve_range, over chunks of kSingChunk triangles (kind 0) or cap fans (kind
+// 1): one (voxel, chunk) item per wave (grid-stride), then per voxel the chunks' sum in chunk order and the
+// decision.  The partials depend only on the item, not on the wave that sums it.
 __device__ __forceinline__ void singular_parts(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t ns = min(a.col.sc->singular, a.sing_cap), nc = a.sing_chunks;
   const uint32_t items = ns * nc;
-  const GridArgs &g = a.col.g;
   for (uint32_t it = wave; it < items; it += nwaves) {
     const uint32_t slot = it / nc, chunk = it % nc;
-    const uint32_t vox = a.sing[slot];
-    const int x = (int)(vox % (uint32_t)g.nx);
-    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
-    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
-    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
-    const int t0 = (int)(chunk * kSingChunk), t1 = min(a.col.ntri, t0 + kSingChunk);
-    const double part = mesh_solid_angle_f64_wave(q, a.col.tris, t0, t1);
+    int x, y, z;
+    const F3 q = voxel_centre(a.col.g, a.sing[slot], x, y, z);
+    const int t0 = (int)(chunk * kSingChunk);
+    double part;
+    if (a.sing_kind[slot] == 0) {
+      part = mesh_solid_angle_f64_wave(q, a.col.tris, t0, min(a.col.ntri, t0 + kSingChunk));
+    } else {
+      bool unused = false;
+      part = cap_solid_angle_f64_wave(q, a.col.caps, t0, min(a.col.ncap, t0 + kSingChunk), unused);
+    }
     if ((threadIdx.x & 63) == 0) a.sing_part[(size_t)slot * nc + chunk] = part;
   }
 }
 
 __device__ __forceinline__ void singular_finish(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t ns = min(a.col.sc->singular, a.sing_cap), nc = a.sing_chunks;
-  const GridArgs &g = a.col.g;
   const int lane = threadIdx.x & 63;
   for (uint32_t slot = wave; slot < ns; slot += nwaves) {
-    const uint32_t vox = a.sing[slot];
-    const int x = (int)(vox % (uint32_t)g.nx);
-    const int y = (int)((vox / (uint32_t)g.nx) % (uint32_t)g.ny);
-    const int z = (int)(vox / ((uint32_t)g.nx * (uint32_t)g.ny));
-    const F3 q = {g.ox + ((float)x + 0.5f) * g.h, g.oy + ((float)y + 0.5f) * g.h, g.oz + ((float)z + 0.5f) * g.h};
+    int x, y, z;
+    const F3 q = voxel_centre(a.col.g, a.sing[slot], x, y, z);
     double sum = 0.0, comp = 0.0;
-    for (uint32_t k = lane; k < nc; k += 64) {
-      const double om = a.sing_part[(size_t)slot * nc + k];
-      const double s = sum + om;
-      comp += (fabs(sum) >= fabs(om)) ? (sum - s) + om : (om - s) + sum;
-      sum = s;
-    }
-    const double w64 = wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+    for (uint32_t k = lane; k < nc; k += 64) neumaier_add(sum, comp, a.sing_part[(size_t)slot * nc + k]);
+    double w64 = wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+    if (a.sing_kind[slot] != 0) w64 += (double)column_wi(a.col, x, y, q);
     const bool inside = decide_w64(a, w64, q);
-    if (lane == 0) {
-      const size_t oi = out_index(a, x, y, z);
-      const float d = a.out[oi];
-      a.out[oi] = inside ? -d : d;
-    }
+    if (lane == 0) store_sign(a, x, y, z, inside);
   }
 }
 
@@ -2473,10 +2593,11 @@ void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, floa
   if (ne) k_cap_gather<<<nb(ne), 256, 0, s>>>(tmp, order, ne, caps);
 }
 void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *child, int32_t *parent, int2 *ranges,
-                     CapNode *nodes, hipStream_t s) {
+                     CapNode *nodes, int4 *cut, hipStream_t s) {
   if (n <= 0) return;
   if (n > 1) k_cap_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, child, parent, ranges);
   k_cap_nodes<<<nb(n > 1 ? n - 1 : 1, 4), 256, 0, s>>>(caps, child, parent, ranges, n, nodes);
+  k_cap_cut<<<nb(kCapCut), 256, 0, s>>>(nodes, ranges, cut);
 }
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
   k_occupancy<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, words);

@@ -33,11 +33,12 @@ struct Scalars {
   uint32_t col_overflow;  // columns with more than kColCap crossings
   uint32_t rechecked;     // voxels re-evaluated in FP64
   uint32_t ties;          // voxels decided by ray parity
-  uint32_t singular;      // FP64 rechecks lying on a hole cap (the mesh's own winding number decides)
+  uint32_t singular;      // entries claimed in the chunked FP64 list (voxels on a cap, undecided rechecks)
   uint32_t slivers;       // triangles taking the FP64 distance path
   uint32_t ambiguous;     // voxels handed to the resolve kernel (column overflow or |w| near 0.5)
   uint32_t stack_overflow;  // traversal pushes that found the stack full (0 by the depth bound; checked by the host)
-  uint32_t pad[2];
+  uint32_t rechecked_full;  // FP64 rechecks the cap tree's bound left undecided: every fan summed
+  uint32_t on_cap;          // FP64 rechecks lying on a hole cap (decided by the mesh's own winding number)
   unsigned long long box_tests;  // counting mode only
   unsigned long long tri_tests;
   unsigned long long obb_tests;
@@ -60,6 +61,7 @@ struct ColumnArgs {
   const float *caps;  // 9 floats per cap triangle, in cap-hierarchy leaf order
   int ncap;
   const CapNode *cap_nodes;
+  const int4 *cap_cut;  // the cap tree's cut (kCapCut slots: code, first fan, fan count) for the FP64 recheck
   int ncapnode;
   GridArgs g;
   float2 *cross;      // kColCap per column: (z, +-1)
@@ -74,10 +76,12 @@ struct SdfArgs {
   const uint32_t *band_bricks;  // narrow band: brick linear index of every slot (2 block-tiles per brick)
   uint32_t *amb_list;  // voxels whose sign the resolve kernel decides
   uint32_t amb_cap;    // capacity of amb_list
-  uint32_t sing_cap;     // voxels on a hole cap summed by the chunked launches (0: by the resolving wave)
-  uint32_t sing_chunks;  // chunks of kSingChunk triangles
+  uint32_t sing_cap;     // voxels whose FP64 sum the chunked launches make (0: the resolving wave sums)
+  uint32_t sing_chunks;  // chunks of kSingChunk triangles or cap fans (the larger count)
   uint32_t *sing;        // those voxels (sing_cap)
+  uint32_t *sing_kind;   // 0: on a hole cap, the mesh's own winding number; 1: the column's integer + every cap fan
   double *sing_part;     // their chunk sums (sing_cap x sing_chunks)
+  uint32_t resolve_full; // nonzero: the FP64 recheck sums every cap fan instead of walking the cap tree first
 };
 constexpr int kSingChunk = 2048;
 
@@ -134,7 +138,7 @@ void launch_cap_tris(const unsigned long long *ek, const uint32_t *lab, uint32_t
                      const uint32_t cmax[3], float *tris, uint32_t *codes, int32_t *idx, hipStream_t s);
 void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, float *caps, hipStream_t s);
 void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *child, int32_t *parent, int2 *ranges,
-                     CapNode *nodes, hipStream_t s);
+                     CapNode *nodes, int4 *cut, hipStream_t s);
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s);
 // 16 x 16 column tiles of a grid, and the columns of a whole batch in one launch (prefix: first
 // column tile of each job, njobs + 1 entries on the device)

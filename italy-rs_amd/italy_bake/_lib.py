@@ -71,7 +71,7 @@ class Stats(ctypes.Structure):
         ("column_overflows", ctypes.c_uint64), ("rechecked", ctypes.c_uint64), ("ties", ctypes.c_uint64),
         ("box_tests", ctypes.c_uint64), ("tri_tests", ctypes.c_uint64), ("stack_overflows", ctypes.c_uint64),
         ("ms_caps", ctypes.c_double), ("ms_query", ctypes.c_double), ("obb_tests", ctypes.c_uint64),
-        ("on_cap", ctypes.c_uint64),
+        ("on_cap", ctypes.c_uint64), ("rechecked_full", ctypes.c_uint64),
     ]
 
     def as_dict(self):
