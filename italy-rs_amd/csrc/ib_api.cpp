@@ -340,13 +340,15 @@ ib_status enqueue_weld(ib_context *c, const float *d_xyz, int n, Scalars *sc, in
 }
 
 ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
-                       hipStream_t s, Arena *out);
+                       hipStream_t s, Arena *out, bool side_depth);
 
 // ingest + LBVH of one mesh whose soup is on the device.  Outputs and temporaries come from `arena`
 // when given (a batch builds its meshes on several streams at once, so each needs its own), else
-// from the context scratch (reuse is ordered by the stream).  Nothing is read back here.
+// from the context scratch (reuse is ordered by the stream).  With `overlap` (a plain bake, or the only
+// mesh of a batch) the weld and the depth walk run on side streams beside the tree build.  Nothing is
+// read back here.
 ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
-                        hipStream_t s, Arena *keep = nullptr) {
+                        hipStream_t s, Arena *keep = nullptr, bool overlap = false) {
   Arena *out = keep ? keep : arena;  // the tree and records outlive the temporaries when `keep` is given
   // the scalars (cmin first, all ones; the rest zero) and the weld tables, cleared in one launch
   static_assert(offsetof(Scalars, cmin) == 0, "cmin leads the scalars");
@@ -356,10 +358,11 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   l.add(reinterpret_cast<uint32_t *>(sc) + 3, 0u, sizeof(Scalars) / 4 - 3);
   IB_TRY(take_weld(c, n, arena, w, l));
   fill_multi(l, s);
-  // a plain bake welds on a side stream beside the tree build (they share only the cleared scalars);
-  // the build's stream waits for the weld before it returns, on every path
+  // the weld runs on a side stream beside the tree build (they share only the cleared scalars); the
+  // build's stream waits for the weld before it returns, on every path
+  overlap = overlap || !arena;
   hipStream_t ws = s;
-  if (!arena) {
+  if (overlap) {
     IB_HIP(hipEventRecord(c->fork, s));
     IB_HIP(hipStreamWaitEvent(c->side[1], c->fork, 0));
     ws = c->side[1];
@@ -367,19 +370,19 @@ ib_status enqueue_build(ib_context *c, const float *d_xyz, int n, Scalars *sc, M
   launch_weld(d_xyz, n, w.slots, w.minid, w.vcap - 1, w.cslot, w.ekeys, w.enet, w.ecap - 1, sc, w.bnd, 3u * n, ws);
   m.bnd = w.bnd;
   ib_status st = IB_OK;
-  if (!arena) {
-    c->record(EV_INGEST, ws);
+  if (overlap) {
+    if (!arena) c->record(EV_INGEST, ws);
     IB_HIP(hipEventRecord(c->join[1], ws));
   }
-  if (st == IB_OK) st = enqueue_tree(c, d_xyz, n, sc, m, arena, s, out);
-  if (!arena) IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
+  if (st == IB_OK) st = enqueue_tree(c, d_xyz, n, sc, m, arena, s, out, overlap);
+  if (overlap) IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
   if (st == IB_OK && !arena) c->record(EV_BVH, s);
   return st;
 }
 
 // Morton codes, sort, Karras tree, records, refit, 4-wide view
 ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, MeshDev &m, Arena *arena,
-                       hipStream_t s, Arena *out) {
+                       hipStream_t s, Arena *out, bool side_depth) {
   uint32_t *codes_in, *codes, *visits;
   int32_t *idx, *order, *leaf_parent;
   float *root_box;
@@ -409,8 +412,7 @@ ib_status enqueue_tree(ib_context *c, const float *d_xyz, int n, Scalars *sc, Me
   int2 *ranges;
   IB_TRY(take(c, arena, "node_ranges", std::max(n - 1, 1), &ranges));
   launch_karras(codes, n, m.nodes, leaf_parent, ranges, s);
-  // a plain bake walks the parent chains for the depth on a side stream beside the records and the refit
-  const bool side_depth = !arena;
+  // the parent chains are walked for the depth on a side stream beside the records and the refit
   if (side_depth) {
     IB_HIP(hipEventRecord(c->fork, s));
     IB_HIP(hipStreamWaitEvent(c->side[2], c->fork, 0));
@@ -804,7 +806,7 @@ ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void 
   IB_TRY(mh->keep.get(9 * ntri, &mh->d_xyz));
   IB_HIP(hipMemcpyAsync(mh->d_xyz, d_xyz, 9 * ntri * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int n = (int)ntri;
-  IB_TRY(enqueue_build(c, mh->d_xyz, n, sc, mh->m, &temp, s, &mh->keep));
+  IB_TRY(enqueue_build(c, mh->d_xyz, n, sc, mh->m, &temp, s, &mh->keep, true));
   IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
   const Scalars h = *c->host_sc;
@@ -1702,19 +1704,23 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   //    most of the GPU idle one at a time), joined into s; then one readback of all scalars
   IB_HIP(hipEventRecord(c->fork, s));
   for (int i = 0; i < ib_context::kSide; ++i) IB_HIP(hipStreamWaitEvent(c->side[i], c->fork, 0));
+  // a batch of one mesh builds on s with the weld and the depth walk beside it, occupancy on side[0]
+  const bool alone = njobs == 1;
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
-    hipStream_t sj = c->side[j % ib_context::kSide];
+    hipStream_t sj = alone ? s : c->side[j % ib_context::kSide];
+    hipStream_t so = alone ? c->side[0] : sj;
     if (jb.ntri == 0 || jb.mesh) {  // nothing to build: the query counters still start at zero
       fill_u32(reinterpret_cast<uint32_t *>(scs + j), 0u, sizeof(Scalars) / 4, sj);
       if (jb.mesh) {
         mesh[j] = jb.mesh->m;
-        if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
+        if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, so);
       }
       continue;
     }
-    IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj));
-    if (jb.occ.nx) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
+    if (jb.occ.nx && alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, so);
+    IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, nullptr, alone));
+    if (jb.occ.nx && !alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
   }
   for (int i = 0; i < ib_context::kSide; ++i) {
     IB_HIP(hipEventRecord(c->join[i], c->side[i]));
