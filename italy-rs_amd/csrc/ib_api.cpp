@@ -66,6 +66,7 @@ struct ib_context {
   std::unordered_map<std::string, Buf> bufs;
   std::unordered_map<void *, size_t> user;
   Scalars *host_sc = nullptr;  // pinned
+  Scalars *host_sc_early = nullptr;  // pinned: the weld's counters, read back before the tree is done
   char *staging = nullptr;     // pinned staging of large device -> host results (grown on demand)
   size_t staging_bytes = 0;
   hipEvent_t ev[EV_COUNT] = {};
@@ -291,6 +292,7 @@ struct MeshDev {
   int ncapnode = 0;
   int4 *cap_cut = nullptr;
   int2 *bnd = nullptr;  // the weld's unmatched edges, until the caps are built
+  bool caps_built = false;  // the caps were built beside the tree (early_caps)
   int depth = 0;
   bool slivers = false;
 };
@@ -441,7 +443,7 @@ void enqueue_occupancy(const float *d_xyz, int n, const ib_grid &occ, uint32_t *
 // tree come from `arena` (or the context's scratch); the temporaries from the context's scratch, one work
 // region carved per step (offsets in units of the edge count rounded up to 64), free again once the stream
 // has run the steps (every user of the scratch runs on the same stream or after it).
-ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint32_t ne, const Scalars &h, MeshDev &m,
+ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint32_t ne, MeshDev &m,
                        Arena *arena, hipStream_t s) {
   if (ne == 0) return IB_OK;
   const size_t st = ((size_t)ne + 63) & ~size_t(63);
@@ -510,7 +512,7 @@ ib_status enqueue_caps(ib_context *c, const float *d_xyz, const int2 *bnd, uint3
   // 5. one fan triangle per edge, sorted by the Morton code of its centroid, into the mesh's cap soup
   uint32_t *codes = iv0;
   int32_t *idx = reinterpret_cast<int32_t *>(iv1);
-  launch_cap_tris(ek1, label, ne, d_xyz, bounds, acc, h.cmin, h.cmax, tmp, codes, idx, s);
+  launch_cap_tris(ek1, label, ne, d_xyz, bounds, acc, tmp, codes, idx, s);
   uint32_t *codes_s = u32at(24);
   int32_t *order = reinterpret_cast<int32_t *>(u32at(28));
   tbx = tb;
@@ -534,16 +536,42 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
   if (m.depth > kMaxTreeDepth)
     return fail(IB_ERR_INTERNAL, "BVH depth " + std::to_string(m.depth) + " exceeds the bound of the key length (" +
                                      std::to_string(kMaxTreeDepth) + ")");
+  if (m.caps_built) return IB_OK;
   m.ncap = 0;
   m.caps = nullptr;
   if (h.boundary == 0) return IB_OK;
   if (!c->ev_valid[EV_CAP0]) c->record(EV_CAP0, s);
   if (!m.bnd) return fail(IB_ERR_INTERNAL, "the weld's unmatched edges are gone");
   const uint32_t ne = std::min<uint32_t>(h.boundary, 3u * n);
-  IB_TRY(enqueue_caps(c, d_xyz, m.bnd, ne, h, m, arena, s));
+  IB_TRY(enqueue_caps(c, d_xyz, m.bnd, ne, m, arena, s));
   m.bnd = nullptr;  // temporaries: not kept with a resident mesh
   c->record(EV_CAP1, s);
   return IB_OK;
+}
+
+// Hole caps of a mesh built alone (a plain bake, a batch of one, a resident mesh; enqueue_build with
+// `overlap`, whose weld ran on side[1]): the weld's boundary count comes back as soon as the weld is done,
+// and the caps are built on side[1] beside the rest of the tree build; `s` waits for them.  The caps need
+// only the soup and the unmatched edges.  finish_build then leaves them as they are.
+ib_status early_caps(ib_context *c, const float *d_xyz, int n, const Scalars *sc, MeshDev &m, Arena *arena,
+                     hipStream_t s) {
+  hipStream_t ws = c->side[1];
+  IB_HIP(hipMemcpyAsync(c->host_sc_early, sc, sizeof(Scalars), hipMemcpyDeviceToHost, ws));
+  IB_HIP(hipStreamSynchronize(ws));
+  const uint32_t boundary = c->host_sc_early->boundary;
+  m.caps_built = true;
+  m.ncap = 0;
+  m.caps = nullptr;
+  if (boundary == 0) return IB_OK;
+  if (!m.bnd) return fail(IB_ERR_INTERNAL, "the weld's unmatched edges are gone");
+  if (!c->ev_valid[EV_CAP0]) c->record(EV_CAP0, ws);
+  const uint32_t ne = std::min<uint32_t>(boundary, 3u * n);
+  const ib_status st = enqueue_caps(c, d_xyz, m.bnd, ne, m, arena, ws);
+  m.bnd = nullptr;
+  if (st == IB_OK) c->record(EV_CAP1, ws);
+  IB_HIP(hipEventRecord(c->join[1], ws));
+  IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
+  return st;
 }
 
 // brick selection: a brick is in the band when its centre is within band * h + 3.5 * sqrt(3) * h of the
@@ -733,6 +761,7 @@ ib_status bake_device(ib_context *c, const float *d_xyz, uint64_t ntri, const ib
   }
   const ib_status st = [&]() -> ib_status {
     IB_TRY(enqueue_build(c, d_xyz, (int)ntri, sc, m, nullptr, s));
+    IB_TRY(early_caps(c, d_xyz, (int)ntri, sc, m, nullptr, s));
     c->record(EV_OCC, s);
     IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
     IB_HIP(hipStreamSynchronize(s));
@@ -807,6 +836,7 @@ ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void 
   IB_HIP(hipMemcpyAsync(mh->d_xyz, d_xyz, 9 * ntri * sizeof(float), hipMemcpyDeviceToDevice, s));
   const int n = (int)ntri;
   IB_TRY(enqueue_build(c, mh->d_xyz, n, sc, mh->m, &temp, s, &mh->keep, true));
+  IB_TRY(early_caps(c, mh->d_xyz, n, sc, mh->m, &mh->keep, s));
   IB_HIP(hipMemcpyAsync(c->host_sc, sc, sizeof(Scalars), hipMemcpyDeviceToHost, s));
   IB_HIP(hipStreamSynchronize(s));
   const Scalars h = *c->host_sc;
@@ -850,7 +880,8 @@ ib_status ib_create(int device, uint64_t budget, ib_context **out) {
   }
   c->budget = budget ? budget : (uint64_t)(0.9 * (double)freeb);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipHostMalloc((void **)&c->host_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
+      hipHostMalloc((void **)&c->host_sc, sizeof(Scalars), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void **)&c->host_sc_early, sizeof(Scalars), hipHostMallocDefault) != hipSuccess) {
     delete c;
     return fail(IB_ERR_DEVICE, "stream / pinned allocation failed");
   }
@@ -887,6 +918,7 @@ void ib_destroy(ib_context *c) {
   c->release_cache();
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   (void)hipHostFree(c->host_sc);
+  if (c->host_sc_early) (void)hipHostFree(c->host_sc_early);
   if (c->host_batch_sc) (void)hipHostFree(c->host_batch_sc);
   if (c->staging) (void)hipHostFree(c->staging);
   for (int i = 0; i < ib_context::kSide; ++i) {
@@ -1720,6 +1752,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     }
     if (jb.occ.nx && alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, so);
     IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, nullptr, alone));
+    if (alone) IB_TRY(early_caps(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj));
     if (jb.occ.nx && !alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
   }
   for (int i = 0; i < ib_context::kSide; ++i) {

@@ -858,16 +858,13 @@ __global__ __launch_bounds__(256) void k_cap_sums(const uint32_t *__restrict__ s
     for (int k = 0; k < 4; ++k) atomicAdd(&acc[4 * (size_t)L + k], v[k]);
 }
 
-struct CapCodeArgs {
-  uint32_t cmin[3], cmax[3];  // the mesh's centroid bounds (ordered-int encoded)
-};
-
 // The fan triangle of every edge: its loop's apex (the mean start vertex), the edge's start, its end;
-// and the Morton code of the triangle's centroid over the mesh's centroid bounds.
+// and the Morton code of the triangle's centroid over the bounds of the boundary's vertices (every fan
+// lies inside them), so that the caps need nothing from the mesh's own tree.
 __global__ __launch_bounds__(256) void k_cap_tris(const u64 *__restrict__ ek, const uint32_t *__restrict__ lab,
                                                   uint32_t ne, const float *__restrict__ xyz,
                                                   const uint32_t *__restrict__ bounds, const u64 *__restrict__ acc,
-                                                  CapCodeArgs ca, float *tris, uint32_t *codes, int32_t *idx) {
+                                                  float *tris, uint32_t *codes, int32_t *idx) {
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   if (e >= ne) return;
   const uint32_t L = lab[e];
@@ -885,7 +882,7 @@ __global__ __launch_bounds__(256) void k_cap_tris(const u64 *__restrict__ ek, co
     t[6 + k] = b[k];
   }
   for (int k = 0; k < 9; ++k) tris[9 * (size_t)e + k] = t[k];
-  codes[e] = centroid_code(t, ca.cmin, ca.cmax);
+  codes[e] = centroid_code(t, bounds, bounds + 3);
   idx[e] = (int32_t)e;
 }
 
@@ -2583,11 +2580,9 @@ void launch_cap_sums(const uint32_t *slab, const uint32_t *sedge, uint32_t ne, c
   if (ne) k_cap_sums<<<nb(ne), 256, 0, s>>>(slab, sedge, ne, ek, xyz, bounds, acc);
 }
 void launch_cap_tris(const unsigned long long *ek, const uint32_t *lab, uint32_t ne, const float *xyz,
-                     const uint32_t *bounds, const unsigned long long *acc, const uint32_t cmin[3],
-                     const uint32_t cmax[3], float *tris, uint32_t *codes, int32_t *idx, hipStream_t s) {
-  CapCodeArgs ca;
-  for (int k = 0; k < 3; ++k) ca.cmin[k] = cmin[k], ca.cmax[k] = cmax[k];
-  if (ne) k_cap_tris<<<nb(ne), 256, 0, s>>>(ek, lab, ne, xyz, bounds, acc, ca, tris, codes, idx);
+                     const uint32_t *bounds, const unsigned long long *acc, float *tris, uint32_t *codes,
+                     int32_t *idx, hipStream_t s) {
+  if (ne) k_cap_tris<<<nb(ne), 256, 0, s>>>(ek, lab, ne, xyz, bounds, acc, tris, codes, idx);
 }
 void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, float *caps, hipStream_t s) {
   if (ne) k_cap_gather<<<nb(ne), 256, 0, s>>>(tmp, order, ne, caps);

@@ -134,8 +134,8 @@ void launch_cap_bounds(const unsigned long long *ek, uint32_t ne, const float *x
 void launch_cap_sums(const uint32_t *slab, const uint32_t *sedge, uint32_t ne, const unsigned long long *ek,
                      const float *xyz, const uint32_t *bounds, unsigned long long *acc, hipStream_t s);
 void launch_cap_tris(const unsigned long long *ek, const uint32_t *lab, uint32_t ne, const float *xyz,
-                     const uint32_t *bounds, const unsigned long long *acc, const uint32_t cmin[3],
-                     const uint32_t cmax[3], float *tris, uint32_t *codes, int32_t *idx, hipStream_t s);
+                     const uint32_t *bounds, const unsigned long long *acc, float *tris, uint32_t *codes,
+                     int32_t *idx, hipStream_t s);
 void launch_cap_gather(const float *tmp, const int32_t *order, uint32_t ne, float *caps, hipStream_t s);
 void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *child, int32_t *parent, int2 *ranges,
                      CapNode *nodes, int4 *cut, hipStream_t s);
