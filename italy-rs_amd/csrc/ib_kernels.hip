@@ -429,16 +429,13 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-// Oriented box of every live internal node covering more than `block` and at most kNodeObbMax triangles,
-// at the node's index (those are the internal entries k_build4 marks): block_obb's construction with the
-// node's triangles dealt to the 64 lanes of a wave and the sums, minima and maxima combined across the wave.
-// Any frame gives a box that holds the triangles; the sums' order only moves the frame.
-__global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ ranges, const float4 *__restrict__ tris,
-                                                   int ninternal, int block, Obb *node_obbs) {
-  const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (i >= ninternal) return;  // wave-uniform
+// Oriented box of internal node i (block_obb's construction) by the whole wave: the node's triangles dealt
+// to the 64 lanes of a wave and the sums, minima and maxima combined across the wave.  Any frame gives a
+// box that holds the triangles; the sums' order only moves the frame.
+__device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, const float4 *__restrict__ tris, int i,
+                                           Obb *node_obbs) {
+  const int lane = threadIdx.x & 63;
   const int first = ranges[i].x, size = ranges[i].y - ranges[i].x + 1;
-  if (size <= block || size > kNodeObbMax) return;
   double n[3] = {0.0, 0.0, 0.0};
   for (int t = first + lane; t < first + size; t += 64) {
     const float4 a = tris[3 * t], p = tris[3 * t + 1], q = tris[3 * t + 2];
@@ -519,6 +516,21 @@ __global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ rang
   b.ex = ef[0], b.ey = ef[1], b.ez = ef[2];
   b.pad = 0.0f;
   node_obbs[i] = b;
+}
+
+// Oriented box of every live internal node covering more than `block` and at most kNodeObbMax triangles,
+// at the node's index (those are the internal entries k_build4 marks).  Each wave strides over groups of 64
+// consecutive nodes, finds the group's qualifying nodes with one ballot and boxes them one after another
+// (most nodes qualify not, and a wave per node would spend the launch on them).
+__global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ ranges, const float4 *__restrict__ tris,
+                                                   int ninternal, int block, Obb *node_obbs) {
+  const int lane = threadIdx.x & 63;
+  for (int grp = blockIdx.x * 4 + (threadIdx.x >> 6); grp * 64 < ninternal; grp += gridDim.x * 4) {
+    const int i = grp * 64 + lane;
+    const int size = i < ninternal ? ranges[i].y - ranges[i].x + 1 : 0;
+    for (uint64_t m = __ballot(size > block && size <= kNodeObbMax); m; m &= m - 1)
+      node_obb_wave(ranges, tris, grp * 64 + __builtin_ctzll(m), node_obbs);
+  }
 }
 
 __global__ __launch_bounds__(256) void k_build4(const Node *__restrict__ nodes, const int2 *__restrict__ ranges,
@@ -2534,7 +2546,7 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   if (block > 1) {
     k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
     k_block_obbs<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, tris, n - 1, block, obbs);
-    k_node_obbs<<<nb(n - 1, 4), 256, 0, s>>>(ranges, tris, n - 1, block, node_obbs);
+    k_node_obbs<<<std::min(nb(n - 1, 256), 2048u), 256, 0, s>>>(ranges, tris, n - 1, block, node_obbs);
   }
   k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block, nodes4);
 }
