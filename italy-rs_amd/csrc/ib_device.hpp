@@ -56,14 +56,15 @@ static_assert(sizeof(Obb) == 64, "oriented box record is one 64-byte line");
 
 // Node of the hole-cap hierarchy.  Leaf (left < 0 and right < 0): fans [-1 - left, -1 - left - right)
 // of the sorted cap soup.  Inner node: left / right are child codes, a node index (>= 0) or a single fan
-// ~index (< 0), at most one of them a fan.  (c, r) bounds every vertex below; lo / hi is the box;
-// area = sum of |fan area|; dip = sum of area vectors.
+// ~index (< 0), at most one of them a fan.  (c, r) bounds every vertex below; lo / hi is the box in x and
+// y; area = sum of |fan area|; dip = sum of area vectors; open2: a point closer to c than sqrt(open2) sees
+// a far-field bound above kCapTauHi (cap_far), so the walks open the node without evaluating it.
 struct alignas(16) CapNode {
   float c[3], r;
   float dip[3], area;
-  float lo[3];
+  float lo[2], open2;
   int32_t left;
-  float hi[3];
+  float hi[2], pad;
   int32_t right;
 };
 static_assert(sizeof(CapNode) == 64, "cap node record is one 64-byte line");

@@ -982,12 +982,22 @@ __global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ cap
   for (int k = 0; k < 3; ++k) {
     nd.c[k] = (float)c[k];
     nd.dip[k] = (float)dip[k];
+  }
+  for (int k = 0; k < 2; ++k) {
     nd.lo[k] = nextafterf((float)bl[k], -INFINITY);
     nd.hi[k] = nextafterf((float)bh[k], INFINITY);
   }
+  nd.pad = 0.0f;
   // the float centre moved by at most one ulp per axis: widen r by that, then by a relative margin
   nd.r = (float)((sqrt(r2) + 1e-6 * (fabs(c[0]) + fabs(c[1]) + fabs(c[2]))) * (1.0 + 1e-5));
   nd.area = (float)(area * (1.0 + 1e-5));
+  // cap_far's bound exceeds kCapTauHi while d - r < min(D0, D1), D0 = sqrt(area / (4 pi tau)) (zero order),
+  // D1 = cbrt(2 area r / (4 pi tau)) (first order): below (r + min(D0, D1))^2, less a relative margin, no
+  // voxel can accept the node
+  const double tau = (double)kCapTauHi, ar = (double)nd.area, rr = (double)nd.r;
+  const double d0 = sqrt(ar / (4.0 * M_PI * tau)), d1 = cbrt(2.0 * ar * rr / (4.0 * M_PI * tau));
+  const double dopen = rr + fmin(d0, d1);
+  nd.open2 = (float)(dopen * dopen * (1.0 - 1e-4));
   const int size = hi - lo + 1;
   if (size <= kCapLeaf) {
     nd.left = -1 - lo;
@@ -1401,10 +1411,8 @@ __device__ __forceinline__ float solid_angle_f32(F3 q, const float *t, bool &sin
 // correctly rounded sqrt and divides: d is rounded down by 2^-20 relative before r is subtracted, 1 / (d - r)
 // up by 2^-20, and the bound up by 2^-18, which covers every rounding of the chain; the evaluation of w1 errs
 // by less than 2^-21 |w1| <= 2^-21 area / (d - r)^2, inside the 1e-5 * area / (d - r)^2 added to its bound.
-__device__ __forceinline__ float cap_far(const CapNode &nd, F3 q, float &w1) {
+__device__ __forceinline__ float cap_far_at(const CapNode &nd, float dx, float dy, float dz, float d2, float &w1) {
   const float inv4pi = (float)(1.0 / (4.0 * M_PI));
-  const float dx = nd.c[0] - q.x, dy = nd.c[1] - q.y, dz = nd.c[2] - q.z;
-  const float d2 = dx * dx + dy * dy + dz * dz;
   const float rd = __builtin_amdgcn_rsqf(d2);  // 1 / d (infinite at 0: then dmin is NaN)
   const float dmin = d2 * rd * (1.0f - 0x1p-20f) - nd.r;
   w1 = 0.0f;
@@ -1415,6 +1423,18 @@ __device__ __forceinline__ float cap_far(const CapNode &nd, F3 q, float &w1) {
   if (!(b1 < b0)) return b0 * (1.0f + 0x1p-18f);
   w1 = (nd.dip[0] * dx + nd.dip[1] * dy + nd.dip[2] * dz) * (rd * rd * rd);  // steradians, like the fans' sum
   return (b1 + 1e-5f * b0) * (1.0f + 0x1p-18f);
+}
+// the offset c - q and its squared length, as cap_far evaluates them
+__device__ __forceinline__ float cap_d2(const CapNode &nd, F3 q, float &dx, float &dy, float &dz) {
+  dx = nd.c[0] - q.x;
+  dy = nd.c[1] - q.y;
+  dz = nd.c[2] - q.z;
+  return dx * dx + dy * dy + dz * dz;
+}
+__device__ __forceinline__ float cap_far(const CapNode &nd, F3 q, float &w1) {
+  float dx, dy, dz;
+  const float d2 = cap_d2(nd, q, dx, dy, dz);
+  return cap_far_at(nd, dx, dy, dz, d2, w1);
 }
 
 // Cap winding with a rigorous error bound: caps near q are summed exactly (FP32 solid angles,
@@ -2081,11 +2101,19 @@ __device__ __forceinline__ void cap_winding_wave(const ColumnArgs &a, const F3 *
       fan(~cur);
     } else {
       const CapNode nd = load_capnode_uniform(a.cap_nodes, cur);
-      float b[V], w1[V];
-      bool open = false;
+      float b[V], w1[V], dx[V], dy[V], dz[V], d2[V];
+      // a needed voxel inside the node's opening radius opens it: no far field to evaluate
+      bool near = false;
       for (int k = 0; k < V; ++k) {
-        b[k] = cap_far(nd, q[k], w1[k]);
-        open = open || (need[k] && !(b[k] <= kCapTau || (b[k] <= kCapTauHi && ub[k] + b[k] <= kCapBudget)));
+        d2[k] = cap_d2(nd, q[k], dx[k], dy[k], dz[k]);
+        near = near || (need[k] && d2[k] < nd.open2);
+      }
+      bool open = __ballot(near) != 0;
+      if (!open) {
+        for (int k = 0; k < V; ++k) {
+          b[k] = cap_far_at(nd, dx[k], dy[k], dz[k], d2[k], w1[k]);
+          open = open || (need[k] && !(b[k] <= kCapTau || (b[k] <= kCapTauHi && ub[k] + b[k] <= kCapBudget)));
+        }
       }
       if (__ballot(open) == 0) {
         for (int k = 0; k < V; ++k) {
