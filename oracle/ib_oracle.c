@@ -1964,6 +1964,9 @@ typedef struct {
   float *sorted;     /* 9 floats each, in the cap LBVH's leaf order */
   orc_bvh b;
   double *area;      /* per LBVH node (internal, then leaves): sum of triangle areas below */
+  double *dip;       /* optional (triset_moments), 3 per node: sum of the area vectors below */
+  double *ctr;       /* 3 per node: area-weighted centre of the triangles below */
+  double *rad;       /* per node: largest distance from ctr to a vertex below */
 } capset;
 
 /* Unmatched directed edges (with multiplicity) after the exact-position weld, as (from, to) vertex ids
@@ -2050,6 +2053,7 @@ static double tri_area_f64(const float *t) {
  * leaving as entering it); each loop, walked from an unused edge by any unused edge leaving the current
  * vertex until it returns to its start, is closed by a fan from the mean of its vertices: triangles
  * (P, u, v) for its edges u -> v, whose boundary is the loop. */
+static int triset_build(const float *tris, size_t nc, capset *cs);
 static int capset_build(const float *xyz, size_t ntri, capset *cs) {
   memset(cs, 0, sizeof(*cs));
   int64_t *e;
@@ -2108,15 +2112,19 @@ static int capset_build(const float *xyz, size_t ntri, capset *cs) {
   free(loop);
   free(e);
   free(vp);
-  if (orc_bvh_build(caps, nc, &cs->b) != 0) {
-    free(caps);
-    return -1;
-  }
-  cs->n = nc;  /* nc >= 1: every unmatched edge gives one fan */
+  const int st = triset_build(caps, nc, cs); /* nc >= 1: every unmatched edge gives one fan */
+  free(caps);
+  return st;
+}
+
+/* A tree over any triangle set, in the form the bounded winding number walks: the LBVH, the triangles in
+ * its leaf order, and every node's total area. */
+static int triset_build(const float *tris, size_t nc, capset *cs) {
+  if (orc_bvh_build(tris, nc, &cs->b) != 0) return -1;
+  cs->n = nc;
   if (nc == 0) return 0;
   cs->sorted = (float *)malloc(sizeof(float) * 9 * nc);
-  for (size_t i = 0; i < nc; ++i) memcpy(cs->sorted + 9 * i, caps + 9 * (size_t)cs->b.order[i], 9 * sizeof(float));
-  free(caps);
+  for (size_t i = 0; i < nc; ++i) memcpy(cs->sorted + 9 * i, tris + 9 * (size_t)cs->b.order[i], 9 * sizeof(float));
   const int ni = (int)nc - 1;
   cs->area = (double *)calloc(2 * nc - 1, sizeof(double));
   for (size_t i = 0; i < nc; ++i) {
@@ -2130,6 +2138,50 @@ static void capset_free(capset *cs) {
   if (cs->n) orc_bvh_free(&cs->b);
   free(cs->sorted);
   free(cs->area);
+  free(cs->dip);
+  free(cs->ctr);
+  free(cs->rad);
+}
+
+/* First moments of every node of a triset: area vectors, area-weighted centres (the box centre for a node
+ * of zero area) and the radius about that centre, each triangle carried up its parent chain. */
+static void triset_moments(capset *cs) {
+  const size_t n = cs->n, nn = 2 * n - 1;
+  const int ni = (int)n - 1;
+  const orc_bvh *b = &cs->b;
+  cs->dip = (double *)calloc(3 * nn, sizeof(double));
+  cs->ctr = (double *)calloc(3 * nn, sizeof(double));
+  cs->rad = (double *)calloc(nn, sizeof(double));
+  double *acc = (double *)calloc(3 * nn, sizeof(double));
+  for (size_t i = 0; i < n; ++i) {
+    const float *t = cs->sorted + 9 * i;
+    const double e1[3] = {(double)t[3] - t[0], (double)t[4] - t[1], (double)t[5] - t[2]};
+    const double e2[3] = {(double)t[6] - t[0], (double)t[7] - t[1], (double)t[8] - t[2]};
+    const double nv[3] = {0.5 * (e1[1] * e2[2] - e1[2] * e2[1]), 0.5 * (e1[2] * e2[0] - e1[0] * e2[2]),
+                          0.5 * (e1[0] * e2[1] - e1[1] * e2[0])};
+    const double a = sqrt(dot3d(nv, nv));
+    for (int x = ni + (int)i; x >= 0; x = b->parent[x])
+      for (int k = 0; k < 3; ++k) {
+        cs->dip[3 * x + k] += nv[k];
+        acc[3 * x + k] += a * (((double)t[k] + t[3 + k]) + t[6 + k]) / 3.0;
+      }
+  }
+  for (size_t x = 0; x < nn; ++x)
+    for (int k = 0; k < 3; ++k)
+      cs->ctr[3 * x + k] = cs->area[x] > 0.0 ? acc[3 * x + k] / cs->area[x]
+                                             : 0.5 * ((double)b->lo[3 * x + k] + b->hi[3 * x + k]);
+  free(acc);
+  for (size_t i = 0; i < n; ++i) {
+    const float *t = cs->sorted + 9 * i;
+    for (int x = ni + (int)i; x >= 0; x = b->parent[x])
+      for (int j = 0; j < 3; ++j) {
+        const double d[3] = {(double)t[3 * j] - cs->ctr[3 * x], (double)t[3 * j + 1] - cs->ctr[3 * x + 1],
+                             (double)t[3 * j + 2] - cs->ctr[3 * x + 2]};
+        const double r = sqrt(dot3d(d, d));
+        if (r > cs->rad[x]) cs->rad[x] = r;
+      }
+  }
+  for (size_t x = 0; x < nn; ++x) cs->rad[x] *= 1.0 + 1e-12;
 }
 
 /* FP64 solid angle of t at q, and whether q lies on or next to t (as the device's singular test) */
@@ -2197,6 +2249,107 @@ static double cap_winding_full_f64(const capset *cs, const double q[3], int *sin
   double sum = 0.0, comp = 0.0;
   for (size_t i = 0; i < cs->n; ++i) neumaier(&sum, &comp, solid_angle_chk(q, cs->sorted + 9 * i, singular));
   return (sum + comp) / (4.0 * M_PI);
+}
+
+/* FP64 winding numbers of a whole soup at many points, exact where the decision needs it: a tree over the
+ * soup is walked with tri_winding_dipole_f64; when | |w| - 1/2 | exceeds its bound by more than the tie band
+ * 1e-6, |w| > 1/2 and whether w lies within 1e-6 of 1/2 are those of the exact sum and the estimate is kept,
+ * else the point takes the brute-force sum orc_gwn_f64 (exact[i] = 1), as does a point lying on a
+ * triangle. */
+/* Winding number of a triset with moments at q, with a first-order far field: the triangles of a node lie
+ * within rad of ctr; seen from d = |ctr - q| > rad they subtend dip . (ctr - q) / d^3 steradians within
+ * 2 area rad / (d - rad)^3 (the integrand n . v / |v|^3 changes by at most 2 |dx| / |v|^3 per unit move dx
+ * of the point on the triangle), and at most area / (d - rad)^2 in all.  A node whose smaller bound is at
+ * most 1e-4 in winding units adds that term (or nothing) and the bound; the other triangles are summed
+ * exactly.  *bound: the total bound (FP64 rounding included); *singular as solid_angle_chk. */
+static double tri_winding_dipole_f64(const capset *cs, const double q[3], double *bound, int *singular) {
+  const orc_bvh *b = &cs->b;
+  const int ni = (int)cs->n - 1;
+  double sum = 0.0, comp = 0.0, ub = 0.0;
+  int stack[512];
+  int sp = 0;
+  stack[sp++] = ni > 0 ? 0 : ~0;
+  while (sp) {
+    const int x = stack[--sp];
+    if (x < 0) {
+      neumaier(&sum, &comp, solid_angle_chk(q, cs->sorted + 9 * (size_t)(~x), singular));
+      continue;
+    }
+    const double v[3] = {cs->ctr[3 * x] - q[0], cs->ctr[3 * x + 1] - q[1], cs->ctr[3 * x + 2] - q[2]};
+    const double d = sqrt(dot3d(v, v)), dmin = d - cs->rad[x];
+    if (dmin > 0.0) {
+      const double b0 = cs->area[x] / (4.0 * M_PI * dmin * dmin);
+      const double b1 = b0 * 2.0 * cs->rad[x] / dmin;
+      if (b1 < b0 && b1 <= 1e-4) {
+        ub += b1;
+        neumaier(&sum, &comp, dot3d(cs->dip + 3 * x, v) / (d * d * d));
+        continue;
+      }
+      if (b0 <= 1e-4) {
+        ub += b0;
+        continue;
+      }
+    }
+    if (sp + 2 <= 512) {
+      stack[sp++] = b->left[x];
+      stack[sp++] = b->right[x];
+    } else {
+      ub += 1.0;
+    }
+  }
+  *bound = ub * (1.0 + 1e-9) + 1e-12;
+  return (sum + comp) / (4.0 * M_PI);
+}
+
+typedef struct {
+  const capset *cs;
+  const float *xyz;
+  size_t ntri;
+  const double *pts;
+  size_t npts;
+  double *w;
+  uint8_t *exact;
+  size_t next;
+  pthread_mutex_t *mu;
+} gwn_job;
+
+static void *gwn_fn(void *arg) {
+  gwn_job *j = (gwn_job *)arg;
+  for (;;) {
+    pthread_mutex_lock(j->mu);
+    const size_t i0 = j->next;
+    j->next += 64;
+    pthread_mutex_unlock(j->mu);
+    if (i0 >= j->npts) break;
+    const size_t i1 = i0 + 64 < j->npts ? i0 + 64 : j->npts;
+    for (size_t i = i0; i < i1; ++i) {
+      const double *q = j->pts + 3 * i;
+      double bound = INFINITY, w = 0.0;
+      int sing = 0;
+      if (j->cs->n) w = tri_winding_dipole_f64(j->cs, q, &bound, &sing);
+      const int exact = sing || !(fabs(fabs(w) - 0.5) > bound + 1e-6 + 1e-12);
+      j->w[i] = exact ? orc_gwn_f64(q, j->xyz, j->ntri) : w;
+      j->exact[i] = (uint8_t)exact;
+    }
+  }
+  return NULL;
+}
+
+int orc_gwn_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_w,
+                       uint8_t *out_exact) {
+  capset cs;
+  memset(&cs, 0, sizeof(cs));
+  if (ntri && triset_build(xyz, ntri, &cs) != 0) return -1;
+  if (ntri) triset_moments(&cs);
+  if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  gwn_job j = {&cs, xyz, ntri, pts, npts, out_w, out_exact, 0, &mu};
+  pthread_t *tid = (pthread_t *)malloc(sizeof(pthread_t) * threads);
+  for (int i = 0; i < threads; ++i) pthread_create(&tid[i], NULL, gwn_fn, &j);
+  for (int i = 0; i < threads; ++i) pthread_join(tid[i], NULL);
+  free(tid);
+  capset_free(&cs);
+  return 0;
 }
 
 /* +z crossings of the reversed cap on the column through (x, y): appended to *out with negated signs */

@@ -80,6 +80,11 @@ double orc_tri_dist2_f64(const double q[3], const float *tri9);
 /* ---- winding number ----------------------------------------------------------- */
 /* FP64 brute force generalized winding number: sum of solid angles / 4pi */
 double orc_gwn_f64(const double q[3], const float *xyz, size_t ntri);
+/* FP64 winding numbers at npts points: a bounded hierarchical estimate where it decides |w| > 1/2 and the
+ * 1e-6 tie band as the exact sum does, else orc_gwn_f64 (out_exact[i] = 1).  Returns 0, or -1 when the tree
+ * cannot be built. */
+int orc_gwn_f64_points(const float *xyz, size_t ntri, const double *pts, size_t npts, int threads, double *out_w,
+                       uint8_t *out_exact);
 /* Signed crossing count of the +z ray from (x,y,z) with the triangles (Woop edge functions with
  * exact zero handling and a half-open tie rule).  For a closed consistently oriented mesh this
  * equals the winding number of the point. */

@@ -98,6 +98,8 @@ def _load():
                                            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, _f32p, ctypes.POINTER(BakeStats)]
     lib.orc_sdf_f64_points.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p, _f64p]
+    lib.orc_gwn_f64_points.argtypes = [_f32p, ctypes.c_size_t, _f64p, ctypes.c_size_t, ctypes.c_int, _f64p,
+                                       ctypes.POINTER(ctypes.c_uint8)]
     lib.orc_udf2_f32_brute.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, _f32p]
     lib.orc_udf_points_f32.argtypes = [_f32p, ctypes.c_size_t, _f32p, ctypes.c_size_t, ctypes.c_int, _f32p]
     lib.orc_boundary_edge_count.argtypes = [_f32p, ctypes.c_size_t]
@@ -447,6 +449,20 @@ def sdf_f64_points(xyz, pts, threads=0):
     lib().orc_sdf_f64_points(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f64p), len(pts), threads, _p(d, _f64p),
                              _p(w, _f64p))
     return d, w
+
+
+def gwn_f64_points(xyz, pts, threads=0):
+    """FP64 winding numbers at points (orc_gwn_f64_points): a bounded hierarchical estimate where it decides
+    |w| > 0.5 and the 1e-6 tie band as the exact sum does, else the brute-force sum.  Returns (w, exact)."""
+    xyz = _f32(xyz)
+    pts = np.ascontiguousarray(pts, np.float64).reshape(-1, 3)
+    w = np.zeros(len(pts), np.float64)
+    exact = np.zeros(len(pts), np.uint8)
+    rc = lib().orc_gwn_f64_points(_p(xyz, _f32p), xyz.size // 9, _p(pts, _f64p), len(pts), threads, _p(w, _f64p),
+                                  exact.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)))
+    if rc != 0:
+        raise RuntimeError("orc_gwn_f64_points: the tree could not be built")
+    return w, exact.astype(bool)
 
 
 def boundary_edge_count(xyz):

@@ -3,12 +3,15 @@
 - 5,120-triangle 64^3 bake: every voxel against the brute-force FP64 distance and winding number.
 - 327,680-triangle 256^3 bake: all 16,777,216 voxels bit for bit against the oracle's dense bake (whose sign is
   the exact winding number of the closed mesh, its signed +z crossing count); 1,048,576 seeded voxels against
-  the FP64 distance through the oracle's LBVH (equal to the brute force bit for bit); 16,384 of them (half
-  within 3 voxels of the surface) against the brute-force FP64 winding number.
+  the FP64 distance through the oracle's LBVH (equal to the brute force bit for bit); 131,072 of them (102,400
+  within 3 voxels of the surface) against the FP64 winding number.
 - Precision-stress narrow bands (513,803 and 1,053,195 triangles, 384^3, +-4 voxels): every band voxel (every
   4th brick for the larger) bit for bit against the oracle's band bake (FP64 winding number by the closure
-  identity), 1,048,576 voxels against the FP64 distance, 8,192 (4,096) against the brute-force FP64 winding
-  number.
+  identity), 1,048,576 voxels against the FP64 distance, 131,072 (65,536) against the FP64 winding number,
+  102,400 (49,152) of them within 3 voxels of the surface.
+The FP64 winding numbers of the large samples are the oracle's orc_gwn_f64_points: a first-order tree walk with
+a rigorous bound where it decides |w| > 1/2 and the tie band as the sum over every triangle does, that sum
+elsewhere (pinned against the brute force in test_oracle.py).
 Max and RMS |d - d64| (bound 1e-5 * extent) and sign agreement on every checked voxel with |d64| > 1e-4 * extent
 whose |w64| is not within 1e-6 of 0.5.  The numbers are written as JSON to the file the environment variable
 IB_ACCURACY_JSON names, when it is set."""
@@ -29,9 +32,10 @@ def _write():
             json.dump(REPORT, f, indent=1)
 
 
-def _record(name, dist, pts, xyz, extent, oracle, sign_sample=None):
+def _record(name, dist, pts, xyz, extent, oracle, sign_sample=None, h=None):
     """dist: GPU signed distances at pts.  sign_sample: indices whose sign is checked against the FP64
-    winding number (None: every point, with brute-force FP64 distances too)."""
+    winding number (None: every point, with brute-force FP64 distances too); h: the voxel size (the count of
+    checked signs within 3 voxels of the surface is reported)."""
     pts = pts.astype(np.float64)
     if sign_sample is None:
         d64s, w = oracle.sdf_f64_points(xyz, pts)
@@ -39,14 +43,15 @@ def _record(name, dist, pts, xyz, extent, oracle, sign_sample=None):
         sign_sample = np.arange(len(pts))
     else:
         d64 = oracle.udf_points_f64(xyz, pts)
-        _, w = oracle.sdf_f64_points(xyz, pts[sign_sample])
+        w, exact = oracle.gwn_f64_points(xyz, pts[sign_sample])
     err = np.abs(dist.astype(np.float64)) - d64
     ds = d64[sign_sample]
     decided = (ds > 1e-4 * extent) & (np.abs(np.abs(w) - 0.5) > 1e-6)
     sign_bad = int(((dist[sign_sample] < 0) != (np.abs(w) > 0.5))[decided].sum())
     REPORT[name] = {"samples": int(len(pts)), "max_abs_err": float(np.abs(err).max()),
                     "rms_err": float(np.sqrt(np.mean(err ** 2))), "max_err_over_extent": float(np.abs(err).max() / extent),
-                    "sign_mismatches": sign_bad, "sign_checked": int(decided.sum())}
+                    "sign_mismatches": sign_bad, "sign_checked": int(decided.sum()),
+                    "sign_checked_near_surface": int((decided & (ds < 3 * (h or 0.0))).sum())}
     _write()
     assert np.abs(err).max() <= 1e-5 * extent
     assert sign_bad == 0
@@ -83,9 +88,9 @@ def test_accuracy_plain_large(ctx, oracle, meshgen):
     zyx = _sample(g, 1 << 20, 2)
     zyx = _near_half(zyx, np.argwhere(np.abs(dist) < 3 * g.h), 3)
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    signs = np.concatenate([np.arange(8192), (1 << 19) + np.arange(8192)])  # near the surface, and anywhere
+    signs = np.concatenate([np.arange(102400), (1 << 19) + np.arange(28672)])  # near the surface, and anywhere
     _record("plain_large_256", dist[zyx[:, 0], zyx[:, 1], zyx[:, 2]], pts, m.xyz, cfg["hi"] - cfg["lo"], oracle,
-            sign_sample=signs)
+            sign_sample=signs, h=g.h)
 
 
 def test_plain_large_every_voxel_equals_oracle(ctx, oracle, meshgen):
@@ -127,8 +132,9 @@ def _band_check(ctx, oracle, name, m, cfg, sign_n, stride):
     zyx[: n // 2] = slots[brick_of_slot[sel[:, 0]]] * 8 + sel[:, 1:]
     vals[: n // 2] = pool[sel[:, 0], sel[:, 1], sel[:, 2], sel[:, 3]]
     pts = oracle.cell_centres(list(g.origin), g.h, g.nx, g.ny, g.nz)[zyx[:, 0], zyx[:, 1], zyx[:, 2]]
-    signs = np.concatenate([np.arange(sign_n // 2), (n // 2) + np.arange(sign_n // 2)])
-    _record(name, vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle, sign_sample=signs)
+    near_n = sign_n * 25 // 32
+    signs = np.concatenate([np.arange(near_n), (n // 2) + np.arange(sign_n - near_n)])
+    _record(name, vals, pts, m.xyz, cfg["hi"] - cfg["lo"], oracle, sign_sample=signs, h=g.h)
     # every (stride-th) band brick against the oracle's band bake
     oi, op, ost = oracle.bake_band(m.xyz, list(g.origin), g.h, g.nx, g.ny, g.nz, cfg["band"], stride=stride)
     assert np.array_equal(oi, index)
@@ -144,10 +150,10 @@ def _band_check(ctx, oracle, name, m, cfg, sign_n, stride):
 
 def test_accuracy_stress_band(ctx, oracle, meshgen):
     m, cfg = meshgen.workload_stress()
-    _band_check(ctx, oracle, "stress_band_384", m, cfg, 8192, 1)
+    _band_check(ctx, oracle, "stress_band_384", m, cfg, 131072, 1)
 
 
 def test_accuracy_stress_band_1m(ctx, oracle, meshgen):
     """The precision-stress union at its stated ~1M triangles (64x64-segment tori, 46x46-quad box faces)."""
     m, cfg = meshgen.workload_stress_1m()
-    _band_check(ctx, oracle, "stress_band_384_1m", m, cfg, 4096, 4)
+    _band_check(ctx, oracle, "stress_band_384_1m", m, cfg, 65536, 4)

@@ -541,3 +541,44 @@ def test_bake_band_ties_on_an_open_face(oracle):
     assert tie.sum() > 0
     assert np.array_equal(np.abs(vals), np.abs(ref))
     assert np.array_equal((vals < 0)[far], (ref < 0)[far])
+
+
+def test_gwn_f64_points_decides_as_the_brute_force_sum(oracle, meshgen):
+    """orc_gwn_f64_points (a bounded tree walk, the brute-force sum where the bound leaves the decision open)
+    against orc_gwn_f64 over every triangle: the same |w| > 1/2 and the same 1e-6 tie band at points far from
+    and near a closed displaced icosphere, near an open stress union, and on and just off the plane of an open
+    cube's missing face (w = 1/2 there); points on a triangle take the brute force."""
+    import shapes
+
+    rng = np.random.default_rng(7)
+    closed = meshgen.displaced(4, 3).xyz
+    open_union = meshgen.stress(seed=9, ncomp=4, remove_frac=0.05).xyz
+    cube = shapes.box_faces(0.5, 3, skip=("+z",))
+    cases = []
+    for xyz in (closed, open_union):
+        t = xyz.reshape(-1, 3, 3).astype(np.float64)
+        c = t.mean(1)
+        n = np.cross(t[:, 1] - t[:, 0], t[:, 2] - t[:, 0])
+        n /= np.linalg.norm(n, axis=1, keepdims=True) + 1e-300
+        pick = rng.choice(len(t), 600, replace=False)
+        off = rng.choice([-3e-2, -1e-3, -1e-5, 1e-5, 1e-3, 3e-2], size=600)[:, None]
+        near = c[pick] + off * n[pick]
+        far = rng.uniform(-1.2, 1.2, (300, 3))
+        on = c[pick[:20]]  # on a triangle
+        cases.append((xyz, np.concatenate([near, far, on])))
+    u = rng.uniform(-0.45, 0.45, (200, 2))
+    dz = rng.choice([0.0, 1e-9, -1e-9, 1e-7, -1e-7, 1e-3, -1e-3, 0.1, -0.1], size=200)
+    cases.append((cube, np.stack([u[:, 0], u[:, 1], 0.5 + dz], 1)))
+    fast = 0
+    total = 0
+    for xyz, pts in cases:
+        w, exact = oracle.gwn_f64_points(xyz, pts, threads=4)
+        _, wb = oracle.sdf_f64_points(xyz, pts, threads=4)
+        assert np.array_equal(np.abs(w) > 0.5, np.abs(wb) > 0.5)
+        assert np.array_equal(np.abs(np.abs(w) - 0.5) < 1e-6, np.abs(np.abs(wb) - 0.5) < 1e-6)
+        assert np.array_equal(w[exact], wb[exact])
+        fast += int((~exact).sum())
+        total += len(pts)
+    assert fast > total // 2  # the tree decides most points on its own
+    w, exact = oracle.gwn_f64_points(cube, np.array([[0.0, 0.0, 0.5]]), threads=1)  # w = 1/2 exactly
+    assert exact[0] and abs(w[0] - 0.5) < 1e-12
