@@ -109,6 +109,14 @@ int main(void) {
   orc_udf2_f32_brute(closed, nc, pts, NP, f2);
   for (int i = 0; i < NP; ++i) CHECK(fout[i] * fout[i] >= 0.0f && f2[i] >= 0.0f);
   orc_sdf_f64_points(closed, nc, dpts, NP, 3, d64, w64);
+  {
+    double wd[NP];
+    uint8_t ex[NP];
+    CHECK(orc_gwn_f64_points(closed, nc, dpts, NP, 3, wd, ex) == 0);
+    for (int i = 0; i < NP; ++i) CHECK((fabs(wd[i]) > 0.5) == (fabs(w64[i]) > 0.5));
+    CHECK(orc_gwn_f64_points(open, no, dpts, NP, 3, wd, ex) == 0);
+    CHECK(orc_gwn_f64_points(open, 0, dpts, NP, 1, wd, ex) == 0);
+  }
   const double q0[3] = {0.01, 0.02, 0.03};
   CHECK(fabs(orc_gwn_f64(q0, closed, nc)) > 0.99);
   const float qf[3] = {0.01f, 0.02f, 0.03f};
