@@ -74,11 +74,12 @@ constexpr int kCapLeaf = 4;
 constexpr int kCapStack = 64;
 // a cap node whose far-field error bound (winding units) is at most kCapTau is not evaluated: its first-order
 // term is added and its bound counted
-constexpr float kCapTau = 1e-3f;
+constexpr float kCapTau = 2e-3f;
 // ... and a node whose bound is at most kCapTauHi is not evaluated either while the bounds of the nodes not
-// evaluated so far stay within kCapBudget (the sign needs |w - 1/2| >= 0.05 + bound)
+// evaluated so far stay within kCapBudget (the sign needs |w - 1/2| >= 0.05 + bound; a voxel whose bound
+// leaves it closer goes to the FP64 recheck, which the cap tree's cut makes cheap)
 constexpr float kCapTauHi = 1e-2f;
-constexpr float kCapBudget = 0.2f;
+constexpr float kCapBudget = 0.3f;
 // the FP64 recheck sums the caps over a cut of their tree first: a node of the cut whose far-field error
 // bound is at most kCapTau64 adds its first-order term and its bound; the voxel is decided there when |w|
 // stays on one side of 1/2 and of the tie band's edges by more than the total bound, and every fan is
