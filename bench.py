@@ -447,7 +447,8 @@ def band_section(ctx, steps=3, cpu=True):
                         f"+-{cfg['band']} voxels in 8^3 FP32 bricks over [-1,1]^3",
             "triangles": mesh.ntri, "bricks": int(n.value), "band_voxels_computed": int(n.value) * 512,
             "ms_per_bake": ms, "ms_spread": [1e3 * min(times), 1e3 * max(times)], "voxel_queries_per_s": rate,
-            "boundary_edges": st["boundary_edges"], "fp64_rechecked": st["rechecked"], "fp64_on_cap": st["on_cap"], "parity_ties": st["ties"],
+            "boundary_edges": st["boundary_edges"], "fp64_rechecked": st["rechecked"],
+            "fp64_summed_over_every_fan": st["rechecked_full"], "fp64_on_cap": st["on_cap"], "parity_ties": st["ties"],
             "ms_hole_caps": st["ms_caps"], "ms_query_launch": st["ms_sdf"],
             "query_executed_work": {"flops": exec_flops, "TFLOP/s": exec_flops / (st["ms_sdf"] * 1e-3) / 1e12,
                                     "note": "executed FP32 work of the band's persistent query launch (counting "
