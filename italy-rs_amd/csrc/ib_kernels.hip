@@ -2035,6 +2035,15 @@ __device__ __forceinline__ Node4 load_node4_uniform(const Node4 *p, int i) {
   return u.r;
 }
 
+// only the children, count and oriented-box mask of node i (its last 32 bytes); the boxes stay unread
+__device__ __forceinline__ Node4 load_node4_links_uniform(const Node4 *p, int i) {
+  const CV4 *r = (const CV4 *)(p + i);
+  Node4Bits u;
+  u.v[6] = r[6];
+  u.v[7] = r[7];
+  return u.r;
+}
+
 union ObbBits {
   V4 v[4];
   Obb r;
@@ -2251,10 +2260,24 @@ __device__ __forceinline__ TriRec load_trirec_uniform(const TriRec *p, int i) {
   return u.r;
 }
 
+// Squared distance from a voxel to box e of the visited node, from the terms that lanes e * 16 + j hold
+// (see wave_brick): the x-y term at byte address axy and the z term at byte address az of entry 0.
+__device__ __forceinline__ float sep_box_d2(int axy, int az, int e, float fxy, float fz) {
+  const float dxy = __int_as_float(__builtin_amdgcn_ds_bpermute(axy + 64 * e, __float_as_int(fxy)));
+  const float dz = __int_as_float(__builtin_amdgcn_ds_bpermute(az + 64 * e, __float_as_int(fz)));
+  return dxy + dz;
+}
+
 // One wave = one brick of sx * sy * sz voxels with corner (bx, by, bz), V voxels per lane: lane l
 // holds voxels (l % sx, (l / sx) % sy, l / (sx * sy) + k * sz / V) for k < V.  sstack is the wave's
 // LDS stack.  Every lane's voxels are queried against every triangle the brick visits.
-template <bool COUNT, bool SLIVERS, bool CAPS, int V>
+// SEP (4 x 4 x sz bricks, sz <= 16): the squared distance from a voxel to an axis-aligned box is a sum of
+// per-axis terms, and the brick's voxels take only 16 distinct (x, y) pairs and sz distinct z coordinates.
+// At every node, lane e * 16 + j loads box e (a vector load) and evaluates its x-y term at the brick's
+// j-th (x, y) pair and its z term at the j-th z coordinate; each lane then gathers its voxels' two terms
+// per entry (ds_bpermute) and adds them.  One set of VALU instructions evaluates the terms of all four
+// entries, where testing entry by entry repeats them per entry.
+template <bool COUNT, bool SLIVERS, bool CAPS, int V, bool SEP>
 __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int bz, int sx, int sy, int *sstack) {
   const int lane = threadIdx.x & 63;
   const GridArgs &g = a.col.g;
@@ -2283,6 +2306,23 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
   const int centre = __builtin_amdgcn_readfirstlane((sx >> 1) + (sy >> 1) * sx + (lz >> 1) * sx * sy);
   const Node4 *__restrict__ nodes4 = a.col.nodes4;
   const TriRec *__restrict__ recs = a.col.recs;
+  // SEP roles: this lane's entry (lane >> 4), the coordinates of its terms (the brick's (x, y) pair and z
+  // coordinate number lane & 15) and, per voxel, the byte addresses of the lanes holding that voxel's
+  // terms of entry 0 (a voxel shadowing the first one reads the first pair and coordinate)
+  int fbox = 0;
+  float fcx = 0.0f, fcy = 0.0f, fcz = 0.0f;
+  int gxy[V], gz[V];
+  if (SEP) {
+    const int fj = lane & 15;
+    fbox = 3 * (lane >> 4);
+    fcx = g.ox + ((float)(bx + (fj & 3)) + 0.5f) * g.h;
+    fcy = g.oy + ((float)(by + (fj >> 2)) + 0.5f) * g.h;
+    fcz = g.oz + ((float)(bz + fj) + 0.5f) * g.h;
+    for (int k = 0; k < V; ++k) {
+      gxy[k] = valid[k] ? 4 * (lane & 15) : 0;
+      gz[k] = valid[k] ? 4 * (z[k] - bz) : 0;
+    }
+  }
   WorkCount wc;
   int sp = 0;
   int cur = 0;
@@ -2302,21 +2342,44 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
       cb = tc;
     }
   };
+  // SEP: this lane's bounds of the node visited next, and that node's links, loaded one visit ahead so
+  // that the loads overlap the leaf blocks evaluated at the current node
+  float plo[3] = {0.0f, 0.0f, 0.0f}, phi[3] = {0.0f, 0.0f, 0.0f};
+  Node4 nd_next;
+  auto fetch = [&](int node) {
+    const float *nf = reinterpret_cast<const float *>(nodes4 + node) + fbox;
+    for (int i = 0; i < 3; ++i) plo[i] = nf[i], phi[i] = nf[12 + i];
+    nd_next = load_node4_links_uniform(nodes4, node);
+  };
+  if (SEP) fetch(0);
   for (int visits = 0; visits <= a.col.nnodes; ++visits) {
-    const Node4 nd = load_node4_uniform(nodes4, cur);
+    const Node4 nd = SEP ? nd_next : load_node4_uniform(nodes4, cur);
+    // SEP: the distances to all four boxes at once, so that the gathers overlap (an unused entry's
+    // are not read); every lane takes part in them
+    float dsep[4][V];
+    if (SEP) {
+      const float tx = fmaxf(fmaxf(plo[0] - fcx, fcx - phi[0]), 0.0f);
+      const float ty = fmaxf(fmaxf(plo[1] - fcy, fcy - phi[1]), 0.0f);
+      const float tz = fmaxf(fmaxf(plo[2] - fcz, fcz - phi[2]), 0.0f);
+      const float fxy = fmaf(ty, ty, tx * tx), fz = tz * tz;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        for (int k = 0; k < V; ++k) dsep[e][k] = sep_box_d2(gxy[k], gz[k], e, fxy, fz);
+    }
     // per entry: does any lane reach its box, and the centre lane's distance to it (ordering key)
     uint32_t key[4];
     int ch[4];
-    bool leaf[4];
+    int lcode[4];  // leaf-block code of a leaf entry whose box some lane reaches, else -1
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       key[e] = 0xffffffffu;
       ch[e] = -1;
-      leaf[e] = false;
+      lcode[e] = -1;
       if (e >= nd.count) continue;  // wave-uniform: unused entries cost no box test
-      const float d = box_d2(q[0], nd.lo[e], nd.hi[e]);
+      const float d = SEP ? dsep[e][0] : box_d2(q[0], nd.lo[e], nd.hi[e]);
       bool need = d <= bound[0];
-      for (int k = 1; k < V; ++k) need = need || box_d2(q[k], nd.lo[e], nd.hi[e]) <= bound[k];
+      for (int k = 1; k < V; ++k)
+        need = need || (SEP ? dsep[e][k] : box_d2(q[k], nd.lo[e], nd.hi[e])) <= bound[k];
       if (__ballot(need) == 0) continue;
       const int c = nd.child[e];
       if ((nd.obb_mask >> e) & 1) {  // a small subtree: its oriented box must reach some lane too
@@ -2326,8 +2389,9 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
         if (COUNT) wc.obb += V;
         if (__ballot(near) == 0) continue;
       }
-      leaf[e] = c < 0;
-      if (c >= 0) {
+      if (c < 0) {
+        lcode[e] = ~c;
+      } else {
         ch[e] = c;
         key[e] = (uint32_t)__builtin_amdgcn_readlane(__float_as_int(d), centre);
       }
@@ -2349,11 +2413,19 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
           stack_overflow(a.col.sc);
         }
       }
+    // the node visited next (-1: none), chosen before the leaf blocks are evaluated
+    int next = ch[0];
+    if (next < 0 && sp > 0) {
+      --sp;
+      __builtin_amdgcn_wave_barrier();
+      next = __builtin_amdgcn_readfirstlane(sstack[sp]);
+    }
+    if (SEP && next >= 0) fetch(next);
     bool evaluated = false;
 #pragma unroll
     for (int e = 0; e < 4; ++e)
-      if (leaf[e]) {
-        const int code = ~nd.child[e];
+      if (lcode[e] >= 0) {
+        const int code = lcode[e];
         if (code >> kBlockShift) {  // a block of 2+ triangles: its oriented box must reach some lane too
           const Obb b = load_obb_uniform(obbs, code & kBlockMask);
           bool need = false;
@@ -2367,14 +2439,8 @@ __device__ __forceinline__ void wave_brick(const SdfArgs &a, int bx, int by, int
       }
     if (evaluated)
       for (int k = 0; k < V; ++k) bound[k] = prune_bound(best[k], slack);
-    if (ch[0] >= 0) {
-      cur = ch[0];
-      continue;
-    }
-    if (sp == 0) break;
-    --sp;
-    __builtin_amdgcn_wave_barrier();
-    cur = __builtin_amdgcn_readfirstlane(sstack[sp]);
+    if (next < 0) break;
+    cur = next;
   }
   if (COUNT) {
     int nv = 0;
@@ -2403,7 +2469,7 @@ __device__ __constant__ const int kShapes[kNumShapes][3] = {{4, 4, 1}, {8, 4, 1}
 
 // One block per block-tile (2 x 2 bricks); `order`, when given, lists the tiles in launch order
 // (most expensive first, from k_tile_keys), so the long-running tiles do not form the tail.
-template <bool COUNT, bool SLIVERS, bool CAPS, int V>
+template <bool COUNT, bool SLIVERS, bool CAPS, int V, bool SEP>
 __global__ __launch_bounds__(256, 8) void k_sdf_wave(SdfArgs a, int shape, const uint32_t *__restrict__ order,
                                                   uint32_t tx_n, uint32_t ty_n) {
   __shared__ int sstack[4][kStack4];
@@ -2413,7 +2479,7 @@ __global__ __launch_bounds__(256, 8) void k_sdf_wave(SdfArgs a, int shape, const
   const int bx = (int)(tile % tx_n) * 2 * sx + (wv & 1) * sx;
   const int by = (int)((tile / tx_n) % ty_n) * 2 * sy + (wv >> 1) * sy;
   const int bz = (int)(tile / (tx_n * ty_n)) * sz;
-  wave_brick<COUNT, SLIVERS, CAPS, V>(a, bx, by, bz, sx, sy, sstack[wv]);
+  wave_brick<COUNT, SLIVERS, CAPS, V, SEP>(a, bx, by, bz, sx, sy, sstack[wv]);
 }
 
 // Launch-order key of every block-tile: an estimate of its query cost from the column crossings
@@ -2476,8 +2542,9 @@ __device__ __forceinline__ int job_of_tile(const BatchArgs &t, uint32_t tile) {
 
 
 // Persistent variant for a batch of meshes: blocks take block-tiles from one device work queue
-// (tile_prefix[j] = first tile of job j) until the queue is empty.
-template <bool COUNT, bool SLIVERS, bool CAPS, int V>
+// (tile_prefix[j] = first tile of job j) until the queue is empty.  SEP: the batch's brick shape is
+// 4 x 4 x sz, and band jobs use the separable box tests too.
+template <bool COUNT, bool SLIVERS, bool CAPS, int V, bool SEP>
 __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
   __shared__ int sstack[4][kStack4];
   __shared__ uint32_t s_tile;
@@ -2502,14 +2569,14 @@ __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
       const int bx = (int)(b % nbx) * 8 + (sub & 1) * 4;
       const int by = (int)((b / nbx) % nby) * 8 + ((sub >> 1) & 1) * 4;
       const int bz = (int)(b / (nbx * nby)) * 8 + (sub >> 2) * 4;
-      wave_brick<COUNT, SLIVERS, CAPS, 1>(a, bx, by, bz, 4, 4, sstack[wv]);
+      wave_brick<COUNT, SLIVERS, CAPS, 1, SEP>(a, bx, by, bz, 4, 4, sstack[wv]);
       continue;
     }
     const uint32_t tx = (g.nx + 2 * sx - 1) / (2 * sx), ty = (g.ny + 2 * sy - 1) / (2 * sy);
     const int bx = (int)(local % tx) * 2 * sx + (wv & 1) * sx;
     const int by = (int)((local / tx) % ty) * 2 * sy + (wv >> 1) * sy;
     const int bz = (int)(local / (tx * ty)) * sz;
-    wave_brick<COUNT, SLIVERS, CAPS, V>(a, bx, by, bz, sx, sy, sstack[wv]);
+    wave_brick<COUNT, SLIVERS, CAPS, V, SEP>(a, bx, by, bz, sx, sy, sstack[wv]);
   }
 }
 
@@ -2710,10 +2777,18 @@ static void launch_sdf_t(const SdfArgs &a, int variant, int shape, const uint32_
   uint32_t d[3];
   tile_dims(a.col.g, shape, d);
   if (variant == 1) {
-    if (kShapeHost[shape][2] == 2)
-      k_sdf_wave<COUNT, SLIVERS, CAPS, 2><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
-    else
-      k_sdf_wave<COUNT, SLIVERS, CAPS, 1><<<d[0] * d[1] * d[2], 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    // shape {4, 4, 2} is the only one with two voxels per lane.  4 x 4 shapes take the separable box tests
+    // where the kernel keeps every value in registers (the FP64 sliver path, and the cap winding with two
+    // voxels per lane, would spill them)
+    const unsigned n = d[0] * d[1] * d[2];
+    if (kShapeHost[shape][2] == 2) {
+      if (!SLIVERS && !CAPS) k_sdf_wave<COUNT, SLIVERS, CAPS, 2, true><<<n, 256, 0, s>>>(a, shape, order, d[0], d[1]);
+      else k_sdf_wave<COUNT, SLIVERS, CAPS, 2, false><<<n, 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    } else if (!SLIVERS && kShapeHost[shape][0] * kShapeHost[shape][1] == 16) {
+      k_sdf_wave<COUNT, SLIVERS, CAPS, 1, true><<<n, 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    } else {
+      k_sdf_wave<COUNT, SLIVERS, CAPS, 1, false><<<n, 256, 0, s>>>(a, shape, order, d[0], d[1]);
+    }
     return;
   }
   dim3 grid(nb(a.col.g.nx, 8), nb(a.col.g.ny, 8), nb(a.col.g.nz, 4));
@@ -2747,8 +2822,8 @@ void launch_resolve(const SdfArgs &a, hipStream_t s) {
 
 template <bool COUNT, bool SLIVERS, bool CAPS>
 static void launch_batch_t(const BatchArgs &t, uint32_t blocks, hipStream_t s) {
-  if (kShapeHost[t.shape][2] == 2) k_sdf_batch<COUNT, SLIVERS, CAPS, 2><<<blocks, 256, 0, s>>>(t);
-  else k_sdf_batch<COUNT, SLIVERS, CAPS, 1><<<blocks, 256, 0, s>>>(t);
+  if (kShapeHost[t.shape][2] == 2) k_sdf_batch<COUNT, SLIVERS, CAPS, 2, false><<<blocks, 256, 0, s>>>(t);
+  else k_sdf_batch<COUNT, SLIVERS, CAPS, 1, false><<<blocks, 256, 0, s>>>(t);
 }
 
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s) {
