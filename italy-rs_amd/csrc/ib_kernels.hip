@@ -416,17 +416,37 @@ __global__ __launch_bounds__(256) void k_block_obbs(const Node *__restrict__ nod
   }
 }
 
+// Wave reductions of doubles through DPP lane moves (no LDS round trips), for a wave whose 64 lanes are all
+// active: pairs, quads (quad_perm), 8 and 16 lanes (half and full row mirrors), then rows 0+1 and 2+3
+// (row_bcast:15) and all four rows (row_bcast:31); lane 63 ends with the total, which every lane reads.
+// A lane the DPP move does not write keeps `id`, the operation's identity.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v, double id) {
+  const long long a = __double_as_longlong(v), o = __double_as_longlong(id);
+  const int lo = __builtin_amdgcn_update_dpp((int)o, (int)a, CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(o >> 32), (int)(a >> 32), CTRL, ROWS, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+template <class Op>
+__device__ __forceinline__ double wave_reduce(double v, double id, Op op) {
+  v = op(v, dpp_move<0xB1, 0xF>(v, id));   // quad_perm [1, 0, 3, 2]
+  v = op(v, dpp_move<0x4E, 0xF>(v, id));   // quad_perm [2, 3, 0, 1]
+  v = op(v, dpp_move<0x141, 0xF>(v, id));  // row_half_mirror
+  v = op(v, dpp_move<0x140, 0xF>(v, id));  // row_mirror
+  v = op(v, dpp_move<0x142, 0xA>(v, id));  // row_bcast:15 into rows 1 and 3
+  v = op(v, dpp_move<0x143, 0xC>(v, id));  // row_bcast:31 into rows 2 and 3
+  const long long a = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)a, 63), hi = __builtin_amdgcn_readlane((int)(a >> 32), 63);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
-  return v;
+  return wave_reduce(v, 0.0, [](double x, double y) { return x + y; });
 }
 __device__ __forceinline__ double wave_min(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v = fmin(v, __shfl_xor(v, off));
-  return v;
+  return wave_reduce(v, INFINITY, [](double x, double y) { return fmin(x, y); });
 }
 __device__ __forceinline__ double wave_max(double v) {
-  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-  return v;
+  return wave_reduce(v, -INFINITY, [](double x, double y) { return fmax(x, y); });
 }
 
 // Oriented box of internal node i (block_obb's construction) by the whole wave: the node's triangles dealt
@@ -491,18 +511,20 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
     }
   }
   for (int k = 0; k < 3; ++k) lo[k] = wave_min(lo[k]), hi[k] = wave_max(hi[k]);
+  if (lane != 0) return;
   double cc[3] = {o.x, o.y, o.z};
   for (int k = 0; k < 3; ++k)
     for (int j = 0; j < 3; ++j) cc[j] += 0.5 * (lo[k] + hi[k]) * ax[k][j];
   const float cf[3] = {(float)cc[0], (float)cc[1], (float)cc[2]};
-  double e[3] = {0.0, 0.0, 0.0};
-  for (int c = lane; c < 3 * size; c += 64) {
-    const float4 p = tris[3 * first + c];
-    const double d[3] = {(double)p.x - cf[0], (double)p.y - cf[1], (double)p.z - cf[2]};
-    for (int k = 0; k < 3; ++k) e[k] = fmax(e[k], fabs(ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2]));
+  // half extents about the rounded centre from the projections' range: axis . (v - cf) is
+  // axis . (v - o) - axis . (cf - o).  The margin covers the FP64 roundings of both sides, a few units in
+  // the 16th digit of |v - o| (at most mag, the sum of the largest projections) and of |cf - o|.
+  double e[3], mag = 0.0;
+  for (int k = 0; k < 3; ++k) mag += fmax(fabs(lo[k]), fabs(hi[k]));
+  for (int k = 0; k < 3; ++k) {
+    const double sc = ax[k][0] * ((double)cf[0] - o.x) + ax[k][1] * ((double)cf[1] - o.y) + ax[k][2] * ((double)cf[2] - o.z);
+    e[k] = fmax(hi[k] - sc, sc - lo[k]) + 1e-12 * (mag + fabs(sc));
   }
-  for (int k = 0; k < 3; ++k) e[k] = wave_max(e[k]);
-  if (lane != 0) return;
   Obb b;
   float ef[3];
   for (int k = 0; k < 3; ++k) {
