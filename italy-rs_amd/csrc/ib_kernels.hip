@@ -2638,7 +2638,7 @@ void fill_multi(const FillList &l, hipStream_t s) {
 }
 hipError_t sort_pairs(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                       int32_t *vout, int n, hipStream_t s) {
-  return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, (unsigned)n, 0u, 30u, s);
+  return sort_pairs_u32(temp, temp_bytes, kin, kout, vin, vout, (uint32_t)n, 30, s);
 }
 
 void launch_karras(const uint32_t *codes, int n, Node *nodes, int32_t *leaf_parent, int2 *ranges, hipStream_t s) {
@@ -2737,8 +2737,13 @@ void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_
                        int32_t *tri, hipStream_t s) {
   k_splat_emit<<<nb(n, 4), 256, 0, s>>>(xyz, n, g, offsets, cell, tri);
 }
+// rocPRIM merge-sorts 2^17 .. 2^20 keys in about twenty launches; there the three digit passes of
+// radix_sort_u32 take 0.6x its time (measured at 2^18 .. 2^19 keys).  Fewer keys sort in rocPRIM's few block
+// launches, more in its one-sweep radix sort, both quicker than the digit passes.
+constexpr uint32_t kRadixMin = 1u << 17, kRadixMax = 1u << 20;
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                           int32_t *vout, uint32_t n, int end_bit, hipStream_t s) {
+  if (n >= kRadixMin && n < kRadixMax) return radix_sort_u32(temp, temp_bytes, kin, kout, vin, vout, n, end_bit, s);
   return rocprim::radix_sort_pairs(temp, temp_bytes, kin, kout, vin, vout, n, 0u, (unsigned)end_bit, s);
 }
 void launch_splat_reduce(const uint32_t *cell, const int32_t *tri, uint32_t npairs, const float *attr, int k,

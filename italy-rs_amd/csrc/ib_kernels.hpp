@@ -190,6 +190,10 @@ hipError_t exclusive_scan_u32(void *temp, size_t &temp_bytes, const uint32_t *in
                               hipStream_t s);
 void launch_splat_emit(const float *xyz, int n, const GridArgs &g, const uint32_t *offsets, uint32_t *cell,
                        int32_t *tri, hipStream_t s);
+// stable LSD radix sort of (key, value) pairs by the keys' low end_bit bits (ib_sort.hip); temp as rocPRIM's:
+// a call with temp == nullptr returns the bytes needed in temp_bytes
+hipError_t radix_sort_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
+                          int32_t *vout, uint32_t n, int end_bit, hipStream_t s);
 hipError_t sort_pairs_u32(void *temp, size_t &temp_bytes, const uint32_t *kin, uint32_t *kout, const int32_t *vin,
                           int32_t *vout, uint32_t n, int end_bit, hipStream_t s);
 // per-cell means of the k attributes (k <= 8) of the (cell, triangle) pairs sorted by cell: attributes 0..2
