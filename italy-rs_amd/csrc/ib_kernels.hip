@@ -1551,6 +1551,7 @@ __device__ __noinline__ double cap_solid_angle_f64_wave(F3 q, const float *caps,
   const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0;
   bool sing = false;
+#pragma unroll 1
   for (int i = c0 + lane; i < c1; i += 64) {
     const double om = solid_angle_f64(q, caps + 9 * (size_t)i, sing);
     const double s = sum + om;
@@ -1600,9 +1601,52 @@ __device__ __noinline__ double cap_winding_f64_cut(F3 q, const ColumnArgs &a, do
   const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0, ub = 0.0;
   bool sing = false;
-  uint32_t near = 0;  // bit j: slot lane * kCutPerLane + j is summed exactly
+  static_assert(kCutPerLane <= 64, "one bit per slot of a lane");
+  uint64_t near = 0;  // bit j: slot lane * kCutPerLane + j is summed exactly
   for (int j = 0; j < kCutPerLane; ++j) {
     const int4 e = a.cap_cut[lane * kCutPerLane + j];
+    if (e.z == 0) continue;
+    if (e.x >= 0) {
+      double w1;
+      const double b = cap_far_f64(a.cap_nodes[e.x], q, w1);
+      if (b <= kCapTau64) {
+        ub += b;
+        neumaier_add(sum, comp, w1);
+        continue;
+      }
+    }
+    near |= 1ull << j;
+  }
+  for (uint64_t lanes = __ballot(near != 0); lanes; lanes &= lanes - 1) {
+    const int l = __builtin_ctzll(lanes);
+    for (uint64_t bits = __shfl((unsigned long long)near, l); bits; bits &= bits - 1) {
+      const int4 e = a.cap_cut[l * kCutPerLane + __builtin_ctzll(bits)];
+#pragma unroll 1
+      for (int t = e.y + lane; t < e.y + e.z; t += 64)
+        neumaier_add(sum, comp, solid_angle_f64(q, a.caps + 9 * (size_t)t, sing));
+    }
+  }
+  singular = __ballot(sing) != 0;
+  bound = wave_sum(ub) * (1.0 + 1e-9) + 1e-10;
+  return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+}
+
+// cap_winding_f64_cut by the 4 waves of a 256-thread block, for one q: thread t tests slots t, t + 256, ..
+// (a q's near slots are neighbours in the cut's Morton order, so they spread over all four waves), each wave
+// sums the fans of its threads' near slots, and the four waves' sums, bounds and flags are combined in wave
+// order (the result is the same in every thread).  Every thread of the block calls it.
+constexpr int kCutPerThread = kCapCut / 256;
+static_assert(kCapCut % 256 == 0 && kCutPerThread <= 32, "whole slots per thread, one bit each");
+__device__ __noinline__ double cap_winding_f64_cut_block(F3 q, const ColumnArgs &a, double &bound,
+                                                         bool &singular) {
+  __shared__ double s_sum[4], s_ub[4];
+  __shared__ int s_sing[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double sum = 0.0, comp = 0.0, ub = 0.0;
+  bool sing = false;
+  uint32_t near = 0;  // bit j: slot threadIdx.x + 256 * j is summed exactly
+  for (int j = 0; j < kCutPerThread; ++j) {
+    const int4 e = a.cap_cut[threadIdx.x + 256 * j];
     if (e.z == 0) continue;
     if (e.x >= 0) {
       double w1;
@@ -1618,14 +1662,31 @@ __device__ __noinline__ double cap_winding_f64_cut(F3 q, const ColumnArgs &a, do
   for (uint64_t lanes = __ballot(near != 0); lanes; lanes &= lanes - 1) {
     const int l = __builtin_ctzll(lanes);
     for (uint32_t bits = (uint32_t)__shfl((int)near, l); bits; bits &= bits - 1) {
-      const int4 e = a.cap_cut[l * kCutPerLane + __builtin_ctz(bits)];
+      const int4 e = a.cap_cut[wv * 64 + l + 256 * __builtin_ctz(bits)];
+#pragma unroll 1
       for (int t = e.y + lane; t < e.y + e.z; t += 64)
         neumaier_add(sum, comp, solid_angle_f64(q, a.caps + 9 * (size_t)t, sing));
     }
   }
-  singular = __ballot(sing) != 0;
-  bound = wave_sum(ub) * (1.0 + 1e-9) + 1e-10;
-  return wave_sum_f64(sum, comp) * (1.0 / (4.0 * M_PI));
+  const double ws = wave_sum_f64(sum, comp), wub = wave_sum(ub);
+  const bool wsing = __ballot(sing) != 0;
+  if (lane == 0) {
+    s_sum[wv] = ws;
+    s_ub[wv] = wub;
+    s_sing[wv] = wsing ? 1 : 0;
+  }
+  __syncthreads();
+  double t = 0.0, c = 0.0, u = 0.0;
+  bool sg = false;
+  for (int k = 0; k < 4; ++k) {
+    neumaier_add(t, c, s_sum[k]);
+    u += s_ub[k];
+    sg = sg || s_sing[k] != 0;
+  }
+  __syncthreads();  // the next call rewrites the sums
+  singular = sg;
+  bound = u * (1.0 + 1e-9) + 1e-10;
+  return (t + c) * (1.0 / (4.0 * M_PI));
 }
 
 // FP64 winding number of the mesh itself at q (no caps), by the whole wave over its ntri triangles: the
@@ -1635,6 +1696,7 @@ __device__ __noinline__ double mesh_solid_angle_f64_wave(F3 q, const float4 *__r
   const int lane = threadIdx.x & 63;
   double sum = 0.0, comp = 0.0;
   bool unused = false;
+#pragma unroll 1
   for (int i = t0 + lane; i < t1; i += 64) {
     const float4 *t = tris + 3 * i;
     const float v[9] = {t[0].x, t[0].y, t[0].z, t[1].x, t[1].y, t[1].z, t[2].x, t[2].y, t[2].z};
@@ -1931,42 +1993,51 @@ __device__ __forceinline__ bool list_for_chunks(const SdfArgs &a, uint32_t vox, 
   return true;
 }
 
-__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
+// The voxels the query left ambiguous, by 256-thread blocks (grid-stride over blk).  Without hole caps the
+// column's crossing count is the sign, one voxel per wave.  With caps, one voxel per block: the four waves
+// test the cap tree's cut together (cap_winding_f64_cut_block) and the rest, rare, is wave 0's.
+__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t blk, uint32_t nblk) {
   const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
-  const int lane = threadIdx.x & 63;
-  for (uint32_t i = wave; i < n; i += nwaves) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (a.col.ncap == 0) {
+    for (uint32_t i = blk * 4 + wv; i < n; i += nblk * 4) {
+      const uint32_t vox = a.amb_list[i];
+      int x, y, z;
+      const F3 q = voxel_centre(a.col.g, vox, x, y, z);
+      const int wi = column_wi(a.col, x, y, q);
+      if (lane == 0) store_sign(a, x, y, z, wi != 0);
+    }
+    return;
+  }
+  for (uint32_t i = blk; i < n; i += nblk) {  // the same voxels in every wave of the block
     const uint32_t vox = a.amb_list[i];
     int x, y, z;
     const F3 q = voxel_centre(a.col.g, vox, x, y, z);
     const int wi = column_wi(a.col, x, y, q);
-    bool inside;
-    if (a.col.ncap == 0) {
-      inside = wi != 0;
-    } else {
-      if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
-      // the cap tree's cut with its bound first: it decides when |w| stays on one side of 1/2 and of the
-      // tie band's edges by more than the bound, and it finds every voxel on a cap; the others sum every
-      // fan, in chunks across the chip (or here, when the chunk list is full or IB_OPT_RESOLVE_FULL is set)
-      bool on_cap = false, decided = false;
-      double w64 = 0.0;
-      if (!a.resolve_full) {
-        double bound;
-        w64 = (double)wi + cap_winding_f64_cut(q, a.col, bound, on_cap);
-        const double m = fabs(fabs(w64) - 0.5);
-        decided = on_cap || m >= kTieEps + bound || m <= kTieEps - bound;
-      }
-      if (!decided) {
-        if (lane == 0) atomicAdd(&a.col.sc->rechecked_full, 1u);
-        if (!a.resolve_full && list_for_chunks(a, vox, 1u)) continue;
-        w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
-      }
-      if (on_cap) {
-        if (lane == 0) atomicAdd(&a.col.sc->on_cap, 1u);
-        if (list_for_chunks(a, vox, 0u)) continue;
-        w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
-      }
-      inside = decide_w64(a, w64, q);
+    if (threadIdx.x == 0) atomicAdd(&a.col.sc->rechecked, 1u);
+    // the cap tree's cut with its bound first: it decides when |w| stays on one side of 1/2 and of the
+    // tie band's edges by more than the bound, and it finds every voxel on a cap; the others sum every
+    // fan, in chunks across the chip (or here, when the chunk list is full or IB_OPT_RESOLVE_FULL is set)
+    bool on_cap = false, decided = false;
+    double w64 = 0.0;
+    if (!a.resolve_full) {
+      double bound;
+      w64 = (double)wi + cap_winding_f64_cut_block(q, a.col, bound, on_cap);
+      const double m = fabs(fabs(w64) - 0.5);
+      decided = on_cap || m >= kTieEps + bound || m <= kTieEps - bound;
     }
+    if (wv != 0) continue;
+    if (!decided) {
+      if (lane == 0) atomicAdd(&a.col.sc->rechecked_full, 1u);
+      if (!a.resolve_full && list_for_chunks(a, vox, 1u)) continue;
+      w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
+    }
+    if (on_cap) {
+      if (lane == 0) atomicAdd(&a.col.sc->on_cap, 1u);
+      if (list_for_chunks(a, vox, 0u)) continue;
+      w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
+    }
+    const bool inside = decide_w64(a, w64, q);
     if (lane == 0) store_sign(a, x, y, z, inside);
   }
 }
@@ -2008,10 +2079,10 @@ __device__ __forceinline__ void singular_finish(const SdfArgs &a, uint32_t wave,
   }
 }
 
-__global__ __launch_bounds__(256) void k_resolve(SdfArgs a) {
-  resolve_range(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+__global__ __launch_bounds__(256, 4) void k_resolve(SdfArgs a) {
+  resolve_range(a, blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(256) void k_singular_parts(SdfArgs a) {
+__global__ __launch_bounds__(256, 4) void k_singular_parts(SdfArgs a) {
   singular_parts(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 __global__ __launch_bounds__(256) void k_singular_finish(SdfArgs a) {
@@ -2603,10 +2674,10 @@ __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
 }
 
 // resolve for every job of a batch: blockIdx.y = job
-__global__ __launch_bounds__(256) void k_resolve_batch(const SdfArgs *jobs) {
-  resolve_range(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
+__global__ __launch_bounds__(256, 4) void k_resolve_batch(const SdfArgs *jobs) {
+  resolve_range(jobs[blockIdx.y], blockIdx.x, gridDim.x);
 }
-__global__ __launch_bounds__(256) void k_singular_parts_batch(const SdfArgs *jobs) {
+__global__ __launch_bounds__(256, 4) void k_singular_parts_batch(const SdfArgs *jobs) {
   singular_parts(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 __global__ __launch_bounds__(256) void k_singular_finish_batch(const SdfArgs *jobs) {
@@ -2840,7 +2911,7 @@ void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int sha
 }
 
 void launch_resolve(const SdfArgs &a, hipStream_t s) {
-  k_resolve<<<512, 256, 0, s>>>(a);
+  k_resolve<<<2048, 256, 0, s>>>(a);  // one voxel per block on a mesh with caps
   if (a.sing_cap) {
     k_singular_parts<<<1024, 256, 0, s>>>(a);
     k_singular_finish<<<64, 256, 0, s>>>(a);
@@ -2913,8 +2984,9 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
 
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s) {
   if (!njobs) return;
-  // a few jobs (bands, served requests) get the whole chip; a large batch a share per job
-  const unsigned per_job = njobs <= 4 ? 512u : 128u;
+  // a few jobs (bands, served requests) get the whole chip (one voxel per block on meshes with caps); a large
+  // batch a share per job
+  const unsigned per_job = njobs <= 4 ? 2048u : 128u;
   k_resolve_batch<<<dim3(per_job, njobs), 256, 0, s>>>(jobs);
   if (singular) {
     k_singular_parts_batch<<<dim3(njobs <= 4 ? 1024u : 64u, njobs), 256, 0, s>>>(jobs);
