@@ -551,10 +551,11 @@ ib_status finish_build(ib_context *c, const float *d_xyz, int n, const Scalars &
 
 // Hole caps of a mesh built alone (a plain bake, a batch of one, a resident mesh; enqueue_build with
 // `overlap`, whose weld ran on side[1]): the weld's boundary count comes back as soon as the weld is done,
-// and the caps are built on side[1] beside the rest of the tree build; `s` waits for them.  The caps need
+// and the caps are built on side[1] beside the rest of the tree build; `s` waits for them (join), or, with
+// join false, the caller makes `s` wait for c->join[1] before the first use of the caps.  The caps need
 // only the soup and the unmatched edges.  finish_build then leaves them as they are.
 ib_status early_caps(ib_context *c, const float *d_xyz, int n, const Scalars *sc, MeshDev &m, Arena *arena,
-                     hipStream_t s) {
+                     hipStream_t s, bool join = true) {
   hipStream_t ws = c->side[1];
   IB_HIP(hipMemcpyAsync(c->host_sc_early, sc, sizeof(Scalars), hipMemcpyDeviceToHost, ws));
   IB_HIP(hipStreamSynchronize(ws));
@@ -570,7 +571,7 @@ ib_status early_caps(ib_context *c, const float *d_xyz, int n, const Scalars *sc
   m.bnd = nullptr;
   if (st == IB_OK) c->record(EV_CAP1, ws);
   IB_HIP(hipEventRecord(c->join[1], ws));
-  IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
+  if (join) IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));
   return st;
 }
 
@@ -1752,10 +1753,14 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     }
     if (jb.occ.nx && alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, so);
     IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, nullptr, alone));
-    if (alone) IB_TRY(early_caps(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj));
+    // a mesh built alone: its caps run on side[1] until the columns need them (below), beside the brick
+    // selection of a narrow band
+    if (alone) IB_TRY(early_caps(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, false));
     if (jb.occ.nx && !alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
   }
+  const bool caps_side = alone && jobs[0].ntri && !jobs[0].mesh;  // early_caps recorded c->join[1]
   for (int i = 0; i < ib_context::kSide; ++i) {
+    if (caps_side && i == 1) continue;
     IB_HIP(hipEventRecord(c->join[i], c->side[i]));
     IB_HIP(hipStreamWaitEvent(s, c->join[i], 0));
   }
@@ -1832,6 +1837,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     }
     if (short_pool) return fail(IB_ERR_CAPACITY, "a brick pool is smaller than its narrow band (see nbricks)");
   }
+  if (caps_side) IB_HIP(hipStreamWaitEvent(s, c->join[1], 0));  // the caps, for the columns and the query
   // 3. columns + per-job query blocks, tile prefix
   std::vector<SdfArgs> args;
   std::vector<uint32_t> owner, prefix(1, 0), col_prefix(1, 0);
