@@ -1996,7 +1996,7 @@ __device__ __forceinline__ bool list_for_chunks(const SdfArgs &a, uint32_t vox, 
 // The voxels the query left ambiguous, by 256-thread blocks (grid-stride over blk).  Without hole caps the
 // column's crossing count is the sign, one voxel per wave.  With caps, one voxel per block: the four waves
 // test the cap tree's cut together (cap_winding_f64_cut_block) and the rest, rare, is wave 0's.
-__device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t blk, uint32_t nblk) {
+__device__ __forceinline__ void resolve_range_block(const SdfArgs &a, uint32_t blk, uint32_t nblk) {
   const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   if (a.col.ncap == 0) {
@@ -2042,9 +2042,51 @@ __device__ __forceinline__ void resolve_range(const SdfArgs &a, uint32_t blk, ui
   }
 }
 
-// The FP64 sums listed by resolve_range, over chunks of kSingChunk triangles (kind 0) or cap fans (kind
-// 1): one (voxel, chunk) item per wave (grid-stride), then per voxel the chunks' sum in chunk order and the
-// decision.  The partials depend only on the item, not on the wave that sums it.
+// resolve_range_block's work with one voxel per wave (wave = the global wave index): the form for batches
+// of several jobs, whose resolve launches share the chip with each other and with other streams' work
+__device__ __forceinline__ void resolve_range_wave(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
+  const uint32_t n = min(a.col.sc->ambiguous, a.amb_cap);
+  const int lane = threadIdx.x & 63;
+  for (uint32_t i = wave; i < n; i += nwaves) {
+    const uint32_t vox = a.amb_list[i];
+    int x, y, z;
+    const F3 q = voxel_centre(a.col.g, vox, x, y, z);
+    const int wi = column_wi(a.col, x, y, q);
+    bool inside;
+    if (a.col.ncap == 0) {
+      inside = wi != 0;
+    } else {
+      if (lane == 0) atomicAdd(&a.col.sc->rechecked, 1u);
+      // the cap tree's cut with its bound first: it decides when |w| stays on one side of 1/2 and of the
+      // tie band's edges by more than the bound, and it finds every voxel on a cap; the others sum every
+      // fan, in chunks across the chip (or here, when the chunk list is full or IB_OPT_RESOLVE_FULL is set)
+      bool on_cap = false, decided = false;
+      double w64 = 0.0;
+      if (!a.resolve_full) {
+        double bound;
+        w64 = (double)wi + cap_winding_f64_cut(q, a.col, bound, on_cap);
+        const double m = fabs(fabs(w64) - 0.5);
+        decided = on_cap || m >= kTieEps + bound || m <= kTieEps - bound;
+      }
+      if (!decided) {
+        if (lane == 0) atomicAdd(&a.col.sc->rechecked_full, 1u);
+        if (!a.resolve_full && list_for_chunks(a, vox, 1u)) continue;
+        w64 = (double)wi + cap_winding_f64_wave(q, a.col.caps, a.col.ncap, on_cap);
+      }
+      if (on_cap) {
+        if (lane == 0) atomicAdd(&a.col.sc->on_cap, 1u);
+        if (list_for_chunks(a, vox, 0u)) continue;
+        w64 = mesh_winding_f64_wave(q, a.col.tris, a.col.ntri);
+      }
+      inside = decide_w64(a, w64, q);
+    }
+    if (lane == 0) store_sign(a, x, y, z, inside);
+  }
+}
+
+// The FP64 sums listed by resolve_range_block / resolve_range_wave, over chunks of kSingChunk triangles
+// (kind 0) or cap fans (kind 1): one (voxel, chunk) item per wave (grid-stride), then per voxel the chunks'
+// sum in chunk order and the decision.  The partials depend only on the item, not on the wave that sums it.
 __device__ __forceinline__ void singular_parts(const SdfArgs &a, uint32_t wave, uint32_t nwaves) {
   const uint32_t ns = min(a.col.sc->singular, a.sing_cap), nc = a.sing_chunks;
   const uint32_t items = ns * nc;
@@ -2080,7 +2122,7 @@ __device__ __forceinline__ void singular_finish(const SdfArgs &a, uint32_t wave,
 }
 
 __global__ __launch_bounds__(256, 4) void k_resolve(SdfArgs a) {
-  resolve_range(a, blockIdx.x, gridDim.x);
+  resolve_range_block(a, blockIdx.x, gridDim.x);
 }
 __global__ __launch_bounds__(256, 4) void k_singular_parts(SdfArgs a) {
   singular_parts(a, blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
@@ -2674,8 +2716,11 @@ __global__ __launch_bounds__(256, 8) void k_sdf_batch(BatchArgs t) {
 }
 
 // resolve for every job of a batch: blockIdx.y = job
-__global__ __launch_bounds__(256, 4) void k_resolve_batch(const SdfArgs *jobs) {
-  resolve_range(jobs[blockIdx.y], blockIdx.x, gridDim.x);
+__global__ __launch_bounds__(256, 4) void k_resolve_batch(const SdfArgs *jobs, int per_voxel_blocks) {
+  if (per_voxel_blocks)
+    resolve_range_block(jobs[blockIdx.y], blockIdx.x, gridDim.x);
+  else
+    resolve_range_wave(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
 }
 __global__ __launch_bounds__(256, 4) void k_singular_parts_batch(const SdfArgs *jobs) {
   singular_parts(jobs[blockIdx.y], blockIdx.x * 4 + (threadIdx.x >> 6), gridDim.x * 4);
@@ -2984,10 +3029,15 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
 
 void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s) {
   if (!njobs) return;
-  // a few jobs (bands, served requests) get the whole chip (one voxel per block on meshes with caps); a large
-  // batch a share per job
-  const unsigned per_job = njobs <= 4 ? 2048u : 128u;
-  k_resolve_batch<<<dim3(per_job, njobs), 256, 0, s>>>(jobs);
+  // a single job: one voxel per block on a mesh with caps, over the whole chip; several jobs: one voxel per
+  // wave (measured: the served mix's capacity is higher so, with other streams' work on the chip), a few
+  // jobs the whole chip, a large batch a share per job
+  if (njobs == 1) {
+    k_resolve_batch<<<dim3(2048, 1), 256, 0, s>>>(jobs, 1);
+  } else {
+    const unsigned per_job = njobs <= 4 ? 512u : 128u;
+    k_resolve_batch<<<dim3(per_job, njobs), 256, 0, s>>>(jobs, 0);
+  }
   if (singular) {
     k_singular_parts_batch<<<dim3(njobs <= 4 ? 1024u : 64u, njobs), 256, 0, s>>>(jobs);
     k_singular_finish_batch<<<dim3(16, njobs), 256, 0, s>>>(jobs);
