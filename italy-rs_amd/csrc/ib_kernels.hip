@@ -541,17 +541,18 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
 }
 
 // Oriented box of every live internal node covering more than `block` and at most kNodeObbMax triangles,
-// at the node's index (those are the internal entries k_build4 marks).  Each wave strides over groups of 64
-// consecutive nodes, finds the group's qualifying nodes with one ballot and boxes them one after another
-// (most nodes qualify not, and a wave per node would spend the launch on them).
+// at the node's index (those are the internal entries k_build4 marks).  Each wave strides over groups of
+// `group` (<= 64) consecutive nodes, finds the group's qualifying nodes with one ballot and boxes them one
+// after another (most nodes qualify not, and a wave per node would spend the launch on them; a small mesh
+// takes small groups, so that its few waves do not box many nodes each).
 __global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ ranges, const float4 *__restrict__ tris,
-                                                   int ninternal, int block, Obb *node_obbs) {
+                                                   int ninternal, int block, int group, Obb *node_obbs) {
   const int lane = threadIdx.x & 63;
-  for (int grp = blockIdx.x * 4 + (threadIdx.x >> 6); grp * 64 < ninternal; grp += gridDim.x * 4) {
-    const int i = grp * 64 + lane;
-    const int size = i < ninternal ? ranges[i].y - ranges[i].x + 1 : 0;
+  for (int grp = blockIdx.x * 4 + (threadIdx.x >> 6); grp * group < ninternal; grp += gridDim.x * 4) {
+    const int i = grp * group + lane;
+    const int size = lane < group && i < ninternal ? ranges[i].y - ranges[i].x + 1 : 0;
     for (uint64_t m = __ballot(size > block && size <= kNodeObbMax); m; m &= m - 1)
-      node_obb_wave(ranges, tris, grp * 64 + __builtin_ctzll(m), node_obbs);
+      node_obb_wave(ranges, tris, grp * group + __builtin_ctzll(m), node_obbs);
   }
 }
 
@@ -2779,7 +2780,8 @@ void launch_refit(const float4 *tris, int n, Node *nodes, const int32_t *leaf_pa
   if (block > 1) {
     k_collapse_blocks<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block);
     k_block_obbs<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, tris, n - 1, block, obbs);
-    k_node_obbs<<<std::min(nb(n - 1, 256), 2048u), 256, 0, s>>>(ranges, tris, n - 1, block, node_obbs);
+    const int group = n >= (1 << 16) ? 64 : 8;  // groups of 8 nodes: 1.6 % off the 512-mesh batch
+    k_node_obbs<<<std::min(nb(n - 1, 4 * group), 2048u), 256, 0, s>>>(ranges, tris, n - 1, block, group, node_obbs);
   }
   k_build4<<<nb(n - 1), 256, 0, s>>>(nodes, ranges, n - 1, block, nodes4);
 }
