@@ -315,8 +315,8 @@ __global__ __launch_bounds__(256) void k_collapse_blocks(Node *nodes, const int2
 // nodes, each block in exactly one slot), written at the block's first triangle.  Frame: z the normalised
 // sum of the triangles' area vectors (the world z if that sum vanishes), x the first triangle's longest
 // edge with its z part removed (a world axis if that vanishes), y = z x x, all in FP64 and rounded to FP32.
-// The centre halves the vertices' span along the rounded axes; the half extents are the FP64 maxima of
-// |axis . (v - c)| with the rounded axes and centre, rounded up to FP32.
+// The centre halves the vertices' span along the rounded axes; the half extents bound |axis . (v - c)| with
+// the rounded axes and centre (from the span, with a margin for the FP64 roundings), rounded up to FP32.
 __device__ void block_obb(const float4 *__restrict__ tris, int first, int count, Obb &b) {
   double n[3] = {0.0, 0.0, 0.0};
   for (int t = first; t < first + count; ++t) {
@@ -376,13 +376,13 @@ __device__ void block_obb(const float4 *__restrict__ tris, int first, int count,
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < 3; ++i) c[i] += 0.5 * (lo[k] + hi[k]) * ax[k][i];
   const float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
-  double e[3] = {0.0, 0.0, 0.0};
-  for (int t = first; t < first + count; ++t)
-    for (int v = 0; v < 3; ++v) {
-      const float4 p = tris[3 * t + v];
-      const double d[3] = {(double)p.x - cf[0], (double)p.y - cf[1], (double)p.z - cf[2]};
-      for (int k = 0; k < 3; ++k) e[k] = fmax(e[k], fabs(ax[k][0] * d[0] + ax[k][1] * d[1] + ax[k][2] * d[2]));
-    }
+  // half extents about the rounded centre from the projections' range, as node_obb_wave derives them
+  double e[3], mag = 0.0;
+  for (int k = 0; k < 3; ++k) mag += fmax(fabs(lo[k]), fabs(hi[k]));
+  for (int k = 0; k < 3; ++k) {
+    const double sc = ax[k][0] * ((double)cf[0] - o.x) + ax[k][1] * ((double)cf[1] - o.y) + ax[k][2] * ((double)cf[2] - o.z);
+    e[k] = fmax(hi[k] - sc, sc - lo[k]) + 1e-12 * (mag + fabs(sc));
+  }
   float ef[3];
   for (int k = 0; k < 3; ++k) {
     ef[k] = (float)e[k];
