@@ -91,9 +91,14 @@ typedef enum {
                                buckets, costliest first, Morton order inside a bucket.  Speed only. */
   IB_OPT_LEAF_BLOCK = 6,    /* LBVH subtrees of at most this many triangles (1..32, default 8) are evaluated
                                as one block from their parent.  Speed only; applies to later builds. */
-  IB_OPT_RESOLVE_FULL = 7   /* value != 0: every FP64 recheck of an open mesh sums all hole-cap fans instead
+  IB_OPT_RESOLVE_FULL = 7,  /* value != 0: every FP64 recheck of an open mesh sums all hole-cap fans instead
                                of first bounding the far caps through the cap tree.  Signs are identical;
                                for tests and measurement. */
+  IB_OPT_SHARED_CHIP = 8    /* value != 0: the context's bakes share the GPU with other streams' work (the bake
+                               server sets it on its workers): the FP64 recheck of a single-job batch takes one
+                               voxel per wave, as a batch of several jobs does, instead of one voxel per
+                               256-thread block, which finishes sooner on an otherwise idle GPU.  Speed only;
+                               default 0. */
 } ib_option;
 
 /* One mesh of a batch: device soup, optional SDF grid (sdf.nx == 0: none) and optional occupancy

@@ -81,6 +81,7 @@ struct ib_context {
   int leaf_block = 8;          // LBVH subtrees of at most this many triangles are evaluated as one leaf block
   int tile_order = 2;          // dense per-brick queries: 0 grid order, 1 costliest first, 2 cost buckets + Morton
   bool resolve_full = false;   // FP64 rechecks sum every cap fan (no cap-tree bound first)
+  bool shared_chip = false;    // bakes share the GPU with other streams' work (IB_OPT_SHARED_CHIP)
   int cus = 0;
   Scalars *host_batch_sc = nullptr;  // pinned, batch_sc_cap entries
   uint32_t batch_sc_cap = 0;
@@ -1886,7 +1887,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     BatchArgs t{d_args, d_prefix, (uint32_t)args.size(), d_head, c->brick_shape, nullptr};
     const uint32_t blocks = std::min<uint32_t>(prefix.back(), (uint32_t)(c->cus * c->blocks_per_cu));
     launch_sdf_batch(t, std::max(blocks, 1u), c->count_work, any_slivers, any_caps, s);
-    launch_resolve_batch(d_args, (uint32_t)args.size(), any_caps, s);
+    launch_resolve_batch(d_args, (uint32_t)args.size(), any_caps, !c->shared_chip, s);
   }
   c->record(EV_SDF, s);
   IB_HIP(hipMemcpyAsync(c->host_batch_sc, scs, sizeof(Scalars) * njobs, hipMemcpyDeviceToHost, s));
@@ -1925,7 +1926,7 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
       BatchArgs t{d_one, d_pf, 1, d_hd, jb.band_voxels > 0 ? 0 : c->brick_shape};
       launch_sdf_batch(t, std::max(1u, std::min<uint32_t>(pf[1], (uint32_t)(c->cus * c->blocks_per_cu))), false,
                        any_slivers, any_caps, s);
-      launch_resolve_batch(d_one, 1, any_caps, s);
+      launch_resolve_batch(d_one, 1, any_caps, !c->shared_chip, s);
       IB_HIP(hipStreamSynchronize(s));
     }
   }
@@ -2024,6 +2025,9 @@ ib_status ib_set_option(ib_context *c, int32_t option, int64_t value) {
       return IB_OK;
     case IB_OPT_RESOLVE_FULL:
       c->resolve_full = value != 0;
+      return IB_OK;
+    case IB_OPT_SHARED_CHIP:
+      c->shared_chip = value != 0;
       return IB_OK;
     case IB_OPT_TILE_ORDER:
       if (value < 0 || value > 2) return fail(IB_ERR_INVALID_ARGUMENT, "tile order must be 0..2");

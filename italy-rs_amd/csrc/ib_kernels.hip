@@ -3029,12 +3029,12 @@ void launch_band_scatter(const uint32_t *flags, const uint32_t *slots, uint32_t 
   if (nbrick) k_band_scatter<<<nb(nbrick), 256, 0, s>>>(flags, slots, nbrick, index, bricks, count);
 }
 
-void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s) {
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, bool block_single, hipStream_t s) {
   if (!njobs) return;
-  // a single job: one voxel per block on a mesh with caps, over the whole chip; several jobs: one voxel per
-  // wave (measured: the served mix's capacity is higher so, with other streams' work on the chip), a few
+  // a single job on an otherwise idle chip: one voxel per block on a mesh with caps; several jobs, or a job
+  // beside other streams' work: one voxel per wave (measured: the served mix's capacity is higher so), a few
   // jobs the whole chip, a large batch a share per job
-  if (njobs == 1) {
+  if (njobs == 1 && block_single) {
     k_resolve_batch<<<dim3(2048, 1), 256, 0, s>>>(jobs, 1);
   } else {
     const unsigned per_job = njobs <= 4 ? 512u : 128u;

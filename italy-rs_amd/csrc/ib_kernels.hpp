@@ -164,7 +164,8 @@ void launch_sdf(const SdfArgs &a, bool count, int variant, bool slivers, int sha
 // launch-order keys (ascending = costliest first) and tile ids of the per-brick query's block-tiles
 void launch_tile_keys(const ColumnArgs &a, int shape, int mode, uint32_t *keys, int32_t *ids, hipStream_t s);
 void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool slivers, bool caps, hipStream_t s);
-void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, hipStream_t s);
+// block_single: a single job's voxels take a 256-thread block each (else one wave each)
+void launch_resolve_batch(const SdfArgs *jobs, uint32_t njobs, bool singular, bool block_single, hipStream_t s);
 // narrow band: bricks whose centre distance (from the coarse pass) is <= thresh
 // flags[b] = 1 for every brick centre of the coarse grid cg within thresh of some triangle (flags cleared first)
 void launch_band_splat(const float4 *tris, const TriRec *recs, int n, const GridArgs &cg, float thresh, bool slivers,

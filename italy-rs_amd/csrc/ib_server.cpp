@@ -453,8 +453,10 @@ ib_status ib_server_create(int device, uint64_t hbm_budget, int32_t workers, int
     // every worker context gets the whole budget (not 90% of free memory each): the server's admission
     // is what divides the budget between the workers' batches
     ib_context *c = nullptr;
-    const ib_status st = ib_create(device, s->budget, &c);
+    ib_status st = ib_create(device, s->budget, &c);
+    if (st == IB_OK) st = ib_set_option(c, IB_OPT_SHARED_CHIP, 1);  // the workers' batches run side by side
     if (st != IB_OK) {
+      if (c) ib_destroy(c);
       for (auto *x : s->ctxs) ib_destroy(x);
       delete s;
       return sfail(st, ib_last_error());

@@ -21,9 +21,10 @@ OPT_BLOCKS_PER_CU = 4
 OPT_TILE_ORDER = 5
 OPT_LEAF_BLOCK = 6
 OPT_RESOLVE_FULL = 7
+OPT_SHARED_CHIP = 8
 # the library's defaults (ib_create), per option
 DEFAULT_OPTIONS = {OPT_COUNT_WORK: 0, OPT_QUERY_VARIANT: 1, OPT_BRICK_SHAPE: 0, OPT_BLOCKS_PER_CU: 8,
-                   OPT_TILE_ORDER: 2, OPT_LEAF_BLOCK: 8, OPT_RESOLVE_FULL: 0}
+                   OPT_TILE_ORDER: 2, OPT_LEAF_BLOCK: 8, OPT_RESOLVE_FULL: 0, OPT_SHARED_CHIP: 0}
 
 
 def _f3(v):

@@ -72,6 +72,10 @@ ib_status ib_create(int device, uint64_t, ib_context **out) {
   return IB_OK;
 }
 void ib_destroy(ib_context *c) { delete c; }
+ib_status ib_set_option(ib_context *c, int32_t opt, int64_t) {
+  if (!c) return IB_ERR_INVALID_ARGUMENT;
+  return opt == IB_OPT_SHARED_CHIP ? IB_OK : IB_ERR_INVALID_ARGUMENT;  // the only option the server sets
+}
 
 ib_status ib_mesh_create(ib_context *c, const float *d_xyz, uint64_t ntri, void *, ib_mesh **out) {
   std::lock_guard<std::mutex> lk(c->mu);

@@ -67,7 +67,8 @@ def test_fp64_recheck_through_the_cap_tree_equals_the_full_sum(ctx, meshgen):
     decides when |w| stays clear of 1/2 and of the tie band by more than that bound; otherwise, or with
     IB_OPT_RESOLVE_FULL set, it sums every cap fan.  Both give the same bits on every band voxel of the
     513,803-triangle precision-stress union and on a dense open-mesh bake, and the tree decides most
-    rechecks on its own."""
+    rechecks on its own.  With IB_OPT_SHARED_CHIP set a single bake's rechecks take one voxel per wave
+    instead of one per block: the same bits again."""
     import shapes
     from italy_bake import Grid, bake
 
@@ -77,8 +78,9 @@ def test_fp64_recheck_through_the_cap_tree_equals_the_full_sum(ctx, meshgen):
     gd = Grid.make((-0.7, -0.66, -0.69), 0.021, 64, 64, 64)
     runs = []
     try:
-        for full in (0, 1):
+        for full, shared in ((0, 0), (1, 0), (0, 1)):
             ctx.set_option(bake.OPT_RESOLVE_FULL, full)
+            ctx.set_option(bake.OPT_SHARED_CHIP, shared)
             index, pool = ctx.bake_band(m.xyz, g, c["band"])
             st_band = ctx.stats()
             dense, _ = ctx.bake_dense(box, gd, None)
@@ -86,10 +88,14 @@ def test_fp64_recheck_through_the_cap_tree_equals_the_full_sum(ctx, meshgen):
             runs.append((index, pool, st_band, dense, st_dense))
     finally:
         ctx.set_option(bake.OPT_RESOLVE_FULL, 0)
-    (i0, p0, b0, d0, s0), (i1, p1, b1, d1, s1) = runs
-    assert np.array_equal(i0, i1)
-    assert np.array_equal(p0.view(np.uint32), p1.view(np.uint32))
-    assert np.array_equal(d0.view(np.uint32), d1.view(np.uint32))
+        ctx.set_option(bake.OPT_SHARED_CHIP, 0)
+    (i0, p0, b0, d0, s0), (i1, p1, b1, d1, s1), (i2, p2, b2, d2, s2) = runs
+    for i, p, d in ((i1, p1, d1), (i2, p2, d2)):
+        assert np.array_equal(i0, i)
+        assert np.array_equal(p0.view(np.uint32), p.view(np.uint32))
+        assert np.array_equal(d0.view(np.uint32), d.view(np.uint32))
+    for tree, wave in ((b0, b2), (s0, s2)):
+        assert tree["rechecked"] == wave["rechecked"] and tree["rechecked_full"] == wave["rechecked_full"]
     for tree, full in ((b0, b1), (s0, s1)):
         assert tree["rechecked"] == full["rechecked"] > 0
         assert full["rechecked_full"] == full["rechecked"]
