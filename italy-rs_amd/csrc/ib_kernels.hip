@@ -944,26 +944,23 @@ __global__ __launch_bounds__(256) void k_cap_karras(const uint32_t *__restrict__
   if (i == 0) parent[0] = -1;
 }
 
-// One wave per internal node the traversal reaches (the root and every node whose parent covers more
-// than kCapLeaf fans): FP64 sums over the node's fans, bounds rounded outwards so that the FP32 far-field
-// bound area / dmin^2 stays an upper bound.  A node of at most kCapLeaf fans becomes a leaf (left =
-// -1 - first, right = -count); an inner node keeps its children's codes, of which at most one is a single
-// fan (~index).  With one fan, record 0 is a leaf of that fan.
-__global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ caps, const int2 *__restrict__ child,
-                                                   const int32_t *__restrict__ parent,
-                                                   const int2 *__restrict__ ranges, int n, CapNode *nodes) {
-  const int rec = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (rec >= (n > 1 ? n - 1 : 1)) return;  // wave-uniform
-  int lo = 0, hi = 0;
-  if (n > 1) {
-    lo = ranges[rec].x;
-    hi = ranges[rec].y;
-    const int p = parent[rec];
-    if (p >= 0 && ranges[p].y - ranges[p].x + 1 <= kCapLeaf) return;
-  }
+// Per internal node the traversal reaches (the root and every node whose parent covers more than kCapLeaf
+// fans): FP64 sums over the node's fans, bounds rounded outwards so that the FP32 far-field bound
+// area / dmin^2 stays an upper bound.  A node of at most kCapLeaf fans becomes a leaf (left = -1 - first,
+// right = -count); an inner node keeps its children's codes, of which at most one is a single fan (~index).
+// With one fan, record 0 is a leaf of that fan.  W waves (thread tid of 64 W) take the node's fans in
+// strides; with W > 1 their partial results combine through `red` in wave order, so the sums do not depend
+// on timing.
+constexpr int kCapBigFans = 1024;  // nodes above this many fans take a block of kCapBigWaves waves
+constexpr int kCapBigWaves = 8;
+
+template <int W>
+__device__ __forceinline__ void cap_node(const float *__restrict__ caps, const int2 *__restrict__ child, int rec,
+                                         int lo, int hi, int tid, double (*red)[13], CapNode *nodes) {
+  const int lane = tid & 63, wv = tid >> 6;
   double area = 0.0, ac[3] = {0.0, 0.0, 0.0}, dip[3] = {0.0, 0.0, 0.0};
   double bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int t = lo + lane; t <= hi; t += 64) {
+  for (int t = lo + tid; t <= hi; t += 64 * W) {
     const float *v = caps + 9 * (size_t)t;
     double e1[3], e2[3], sum[3];
     for (int k = 0; k < 3; ++k) {
@@ -989,10 +986,39 @@ __global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ cap
     bl[k] = wave_min(bl[k]);
     bh[k] = wave_max(bh[k]);
   }
+  if constexpr (W > 1) {
+    if (lane == 0) {
+      red[wv][0] = area;
+      for (int k = 0; k < 3; ++k) {
+        red[wv][1 + k] = ac[k];
+        red[wv][4 + k] = dip[k];
+        red[wv][7 + k] = bl[k];
+        red[wv][10 + k] = bh[k];
+      }
+    }
+    __syncthreads();
+    area = red[0][0];
+    for (int k = 0; k < 3; ++k) {
+      ac[k] = red[0][1 + k];
+      dip[k] = red[0][4 + k];
+      bl[k] = red[0][7 + k];
+      bh[k] = red[0][10 + k];
+    }
+    for (int w = 1; w < W; ++w) {
+      area += red[w][0];
+      for (int k = 0; k < 3; ++k) {
+        ac[k] += red[w][1 + k];
+        dip[k] += red[w][4 + k];
+        bl[k] = fmin(bl[k], red[w][7 + k]);
+        bh[k] = fmax(bh[k], red[w][10 + k]);
+      }
+    }
+    __syncthreads();
+  }
   double c[3];
   for (int k = 0; k < 3; ++k) c[k] = area > 0.0 ? ac[k] / area : 0.5 * (bl[k] + bh[k]);
   double r2 = 0.0;
-  for (int t = lo + lane; t <= hi; t += 64) {
+  for (int t = lo + tid; t <= hi; t += 64 * W) {
     const float *v = caps + 9 * (size_t)t;
     for (int j = 0; j < 3; ++j) {
       const double dx = v[3 * j] - c[0], dy = v[3 * j + 1] - c[1], dz = v[3 * j + 2] - c[2];
@@ -1000,7 +1026,13 @@ __global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ cap
     }
   }
   r2 = wave_max(r2);
-  if (lane != 0) return;
+  if constexpr (W > 1) {
+    if (lane == 0) red[wv][0] = r2;
+    __syncthreads();
+    for (int w = 0; w < W; ++w) r2 = fmax(r2, red[w][0]);
+    __syncthreads();
+  }
+  if (tid != 0) return;
   CapNode nd;
   for (int k = 0; k < 3; ++k) {
     nd.c[k] = (float)c[k];
@@ -1030,6 +1062,42 @@ __global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ cap
     nd.right = child[rec].y;
   }
   nodes[rec] = nd;
+}
+
+// the node's fan range, or false when the traversal never reaches it (its parent is a leaf)
+__device__ __forceinline__ bool cap_node_range(const int32_t *__restrict__ parent, const int2 *__restrict__ ranges,
+                                               int n, int rec, int &lo, int &hi) {
+  lo = hi = 0;
+  if (n <= 1) return true;
+  lo = ranges[rec].x;
+  hi = ranges[rec].y;
+  const int p = parent[rec];
+  return !(p >= 0 && ranges[p].y - ranges[p].x + 1 <= kCapLeaf);
+}
+
+// one wave per node of at most kCapBigFans fans
+__global__ __launch_bounds__(256) void k_cap_nodes(const float *__restrict__ caps, const int2 *__restrict__ child,
+                                                   const int32_t *__restrict__ parent,
+                                                   const int2 *__restrict__ ranges, int n, CapNode *nodes) {
+  const int rec = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (rec >= (n > 1 ? n - 1 : 1)) return;  // wave-uniform
+  int lo, hi;
+  if (!cap_node_range(parent, ranges, n, rec, lo, hi) || hi - lo + 1 > kCapBigFans) return;
+  cap_node<1>(caps, child, rec, lo, hi, threadIdx.x & 63, nullptr, nodes);
+}
+
+// one block per node of more than kCapBigFans fans (the few top levels), grid-stride over the nodes
+__global__ __launch_bounds__(64 * kCapBigWaves) void k_cap_nodes_big(const float *__restrict__ caps,
+                                                                     const int2 *__restrict__ child,
+                                                                     const int32_t *__restrict__ parent,
+                                                                     const int2 *__restrict__ ranges, int n,
+                                                                     CapNode *nodes) {
+  __shared__ double red[kCapBigWaves][13];
+  for (int rec = blockIdx.x; rec < n - 1; rec += gridDim.x) {  // block-uniform
+    int lo, hi;
+    if (!cap_node_range(parent, ranges, n, rec, lo, hi) || hi - lo + 1 <= kCapBigFans) continue;
+    cap_node<kCapBigWaves>(caps, child, rec, lo, hi, threadIdx.x, red, nodes);
+  }
 }
 
 // The cut of the cap tree at depth kCapCutBits: slot s follows the path that its bits spell from the root
@@ -2839,6 +2907,8 @@ void launch_cap_tree(const float *caps, const uint32_t *codes, int n, int2 *chil
   if (n <= 0) return;
   if (n > 1) k_cap_karras<<<nb(n - 1), 256, 0, s>>>(codes, n, child, parent, ranges);
   k_cap_nodes<<<nb(n > 1 ? n - 1 : 1, 4), 256, 0, s>>>(caps, child, parent, ranges, n, nodes);
+  if (n > kCapBigFans)
+    k_cap_nodes_big<<<(n - 1) < 256 ? (n - 1) : 256, 64 * kCapBigWaves, 0, s>>>(caps, child, parent, ranges, n, nodes);
   k_cap_cut<<<nb(kCapCut), 256, 0, s>>>(nodes, ranges, cut);
 }
 void launch_occupancy(const float *xyz, int n, const GridArgs &g, uint32_t *words, hipStream_t s) {
@@ -2979,7 +3049,9 @@ void launch_sdf_batch(const BatchArgs &t, uint32_t blocks, bool count, bool sliv
 // spacing H over the grid) lies within thresh of it, by the query's own squared distance (tri_d2 of the
 // triangle's record), so a brick is flagged exactly when sqrt of the least squared distance over all
 // triangles is at most thresh.  Candidates: the centres within thresh of the triangle's box along every
-// axis, one more on each side for rounding.
+// axis, one more on each side for rounding.  A candidate farther than thresh + slack from the triangle's box
+// is skipped before the exact test: the FP32 distance is within slack = 2^-17 (Mq + 3 Mp) of the exact one
+// (tri_d2_f32's bound, for this triangle's vertices and these centres), so its root exceeds thresh too.
 template <bool SLIVERS>
 __global__ __launch_bounds__(256) void k_band_splat(const float4 *__restrict__ tris, const TriRec *__restrict__ recs,
                                                     int n, GridArgs cg, float thresh, uint32_t *flags) {
@@ -2995,16 +3067,32 @@ __global__ __launch_bounds__(256) void k_band_splat(const float4 *__restrict__ t
     a[k] = max(0, (int)floorf((lo[k] - thresh - org[k]) / cg.h - 0.5f) - 1);
     b[k] = min(dim[k] - 1, (int)ceilf((hi[k] + thresh - org[k]) / cg.h - 0.5f) + 1);
   }
+  float mq = 0.0f, mp = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    mq = fmaxf(mq, fmaxf(fabsf(org[k] + ((float)a[k] + 0.5f) * cg.h), fabsf(org[k] + ((float)b[k] + 0.5f) * cg.h)));
+    mp = fmaxf(mp, fmaxf(fabsf(lo[k]), fabsf(hi[k])));
+  }
+  const float reach = thresh + (mq + 3.0f * mp) * 7.62939453125e-6f;  // 2^-17
+  const float lim = reach * reach * 1.0000077f;  // covers the roundings of the box distance and of lim
   const TriRec r = recs[t];
-  for (int z = a[2]; z <= b[2]; ++z)
-    for (int y = a[1]; y <= b[1]; ++y)
+  for (int z = a[2]; z <= b[2]; ++z) {
+    const float qz = cg.oz + ((float)z + 0.5f) * cg.h;
+    const float gz = fmaxf(fmaxf(lo[2] - qz, qz - hi[2]), 0.0f);
+    for (int y = a[1]; y <= b[1]; ++y) {
+      const float qy = cg.oy + ((float)y + 0.5f) * cg.h;
+      const float gy = fmaxf(fmaxf(lo[1] - qy, qy - hi[1]), 0.0f);
+      const float gyz = fmaf(gy, gy, gz * gz);
+      if (gyz > lim) continue;
       for (int x = a[0]; x <= b[0]; ++x) {
+        const float qx = cg.ox + ((float)x + 0.5f) * cg.h;
+        const float gx = fmaxf(fmaxf(lo[0] - qx, qx - hi[0]), 0.0f);
+        if (fmaf(gx, gx, gyz) > lim) continue;
         const size_t i = ((size_t)z * cg.ny + y) * cg.nx + x;
         if (flags[i]) continue;
-        const F3 q = {cg.ox + ((float)x + 0.5f) * cg.h, cg.oy + ((float)y + 0.5f) * cg.h,
-                      cg.oz + ((float)z + 0.5f) * cg.h};
-        if (sqrtf(tri_d2<SLIVERS>(q, r)) <= thresh) flags[i] = 1u;
+        if (sqrtf(tri_d2<SLIVERS>({qx, qy, qz}, r)) <= thresh) flags[i] = 1u;
       }
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void k_band_scatter(const uint32_t *__restrict__ flags,
