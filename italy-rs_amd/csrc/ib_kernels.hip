@@ -376,7 +376,7 @@ __device__ void block_obb(const float4 *__restrict__ tris, int first, int count,
   for (int k = 0; k < 3; ++k)
     for (int i = 0; i < 3; ++i) c[i] += 0.5 * (lo[k] + hi[k]) * ax[k][i];
   const float cf[3] = {(float)c[0], (float)c[1], (float)c[2]};
-  // half extents about the rounded centre from the projections' range, as node_obb_wave derives them
+  // half extents about the rounded centre from the projections' range, as node_obb_lanes derives them
   double e[3], mag = 0.0;
   for (int k = 0; k < 3; ++k) mag += fmax(fabs(lo[k]), fabs(hi[k]));
   for (int k = 0; k < 3; ++k) {
@@ -449,15 +449,36 @@ __device__ __forceinline__ double wave_max(double v) {
   return wave_reduce(v, -INFINITY, [](double x, double y) { return fmax(x, y); });
 }
 
-// Oriented box of internal node i (block_obb's construction) by the whole wave: the node's triangles dealt
-// to the 64 lanes of a wave and the sums, minima and maxima combined across the wave.  Any frame gives a
-// box that holds the triangles; the sums' order only moves the frame.
-__device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, const float4 *__restrict__ tris, int i,
-                                           Obb *node_obbs) {
-  const int lane = threadIdx.x & 63;
-  const int first = ranges[i].x, size = ranges[i].y - ranges[i].x + 1;
+// Reduction over the L lanes of a node: the wave (L = 64, every lane receives the total) or one DPP row of
+// 16 lanes (L = 16: pairs, quads, the half-row and row mirrors leave every lane with its row's total).
+// Every lane of the wave must be active.
+template <int L, class Op>
+__device__ __forceinline__ double lanes_reduce(double v, double id, Op op) {
+  if constexpr (L == 64) {
+    return wave_reduce(v, id, op);
+  } else {
+    static_assert(L == 16, "a node takes the wave or one row");
+    v = op(v, dpp_move<0xB1, 0xF>(v, id));
+    v = op(v, dpp_move<0x4E, 0xF>(v, id));
+    v = op(v, dpp_move<0x141, 0xF>(v, id));
+    return op(v, dpp_move<0x140, 0xF>(v, id));
+  }
+}
+
+// Oriented box of internal node i (block_obb's construction) by L lanes: the node's triangles dealt to the
+// L lanes (the wave, or the 16 lanes of a DPP row, four nodes to a wave) and the sums, minima and maxima
+// combined across them.  i < 0: the lanes box nothing (they still take part in the reductions).  Any frame
+// gives a box that holds the triangles; the sums' order only moves the frame.
+template <int L>
+__device__ __noinline__ void node_obb_lanes(const int2 *__restrict__ ranges, const float4 *__restrict__ tris, int i,
+                                            Obb *node_obbs) {
+  const int lane = threadIdx.x & (L - 1);
+  const auto sum = [](double a, double b) { return a + b; };
+  const auto mn = [](double a, double b) { return fmin(a, b); };
+  const auto mx = [](double a, double b) { return fmax(a, b); };
+  const int first = i >= 0 ? ranges[i].x : 0, size = i >= 0 ? ranges[i].y - ranges[i].x + 1 : 0;
   double n[3] = {0.0, 0.0, 0.0};
-  for (int t = first + lane; t < first + size; t += 64) {
+  for (int t = first + lane; t < first + size; t += L) {
     const float4 a = tris[3 * t], p = tris[3 * t + 1], q = tris[3 * t + 2];
     const double e1[3] = {(double)p.x - a.x, (double)p.y - a.y, (double)p.z - a.z};
     const double e2[3] = {(double)q.x - a.x, (double)q.y - a.y, (double)q.z - a.z};
@@ -465,7 +486,7 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
     n[1] += e1[2] * e2[0] - e1[0] * e2[2];
     n[2] += e1[0] * e2[1] - e1[1] * e2[0];
   }
-  for (int k = 0; k < 3; ++k) n[k] = wave_sum(n[k]);
+  for (int k = 0; k < 3; ++k) n[k] = lanes_reduce<L>(n[k], 0.0, sum);
   double z[3] = {0.0, 0.0, 1.0};
   const double nl = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
   if (nl > 0.0 && nl < INFINITY) z[0] = n[0] / nl, z[1] = n[1] / nl, z[2] = n[2] / nl;
@@ -501,7 +522,7 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
                           {(float)z[0], (float)z[1], (float)z[2]}};
   const float4 o = tris[3 * first];
   double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  for (int c = lane; c < 3 * size; c += 64) {
+  for (int c = lane; c < 3 * size; c += L) {
     const float4 p = tris[3 * first + c];
     const double d[3] = {(double)p.x - o.x, (double)p.y - o.y, (double)p.z - o.z};
     for (int k = 0; k < 3; ++k) {
@@ -510,8 +531,8 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
       hi[k] = fmax(hi[k], s);
     }
   }
-  for (int k = 0; k < 3; ++k) lo[k] = wave_min(lo[k]), hi[k] = wave_max(hi[k]);
-  if (lane != 0) return;
+  for (int k = 0; k < 3; ++k) lo[k] = lanes_reduce<L>(lo[k], INFINITY, mn), hi[k] = lanes_reduce<L>(hi[k], -INFINITY, mx);
+  if (lane != 0 || i < 0) return;
   double cc[3] = {o.x, o.y, o.z};
   for (int k = 0; k < 3; ++k)
     for (int j = 0; j < 3; ++j) cc[j] += 0.5 * (lo[k] + hi[k]) * ax[k][j];
@@ -544,15 +565,24 @@ __device__ __noinline__ void node_obb_wave(const int2 *__restrict__ ranges, cons
 // at the node's index (those are the internal entries k_build4 marks).  Each wave strides over groups of
 // `group` (<= 64) consecutive nodes, finds the group's qualifying nodes with one ballot and boxes them one
 // after another (most nodes qualify not, and a wave per node would spend the launch on them; a small mesh
-// takes small groups, so that its few waves do not box many nodes each).
+// takes small groups, so that its few waves do not box many nodes each): a node of more than kNodeObbRow
+// triangles with the whole wave, smaller ones (most of them) four at a time, one per row of 16 lanes.
+constexpr int kNodeObbRow = 64;
 __global__ __launch_bounds__(256) void k_node_obbs(const int2 *__restrict__ ranges, const float4 *__restrict__ tris,
                                                    int ninternal, int block, int group, Obb *node_obbs) {
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, row = lane >> 4;
   for (int grp = blockIdx.x * 4 + (threadIdx.x >> 6); grp * group < ninternal; grp += gridDim.x * 4) {
     const int i = grp * group + lane;
     const int size = lane < group && i < ninternal ? ranges[i].y - ranges[i].x + 1 : 0;
-    for (uint64_t m = __ballot(size > block && size <= kNodeObbMax); m; m &= m - 1)
-      node_obb_wave(ranges, tris, grp * group + __builtin_ctzll(m), node_obbs);
+    const bool live = size > block && size <= kNodeObbMax;
+    for (uint64_t m = __ballot(live && size > kNodeObbRow); m; m &= m - 1)
+      node_obb_lanes<64>(ranges, tris, grp * group + __builtin_ctzll(m), node_obbs);
+    for (uint64_t m = __ballot(live && size <= kNodeObbRow); m;) {  // wave-uniform
+      int pick = -1;  // row r takes the r-th of the remaining nodes
+      for (int r = 0; r < 4 && m; ++r, m &= m - 1)
+        if (r == row) pick = grp * group + __builtin_ctzll(m);
+      node_obb_lanes<16>(ranges, tris, pick, node_obbs);
+    }
   }
 }
 
