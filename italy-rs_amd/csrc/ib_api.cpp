@@ -1740,6 +1740,39 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
   for (int i = 0; i < ib_context::kSide; ++i) IB_HIP(hipStreamWaitEvent(c->side[i], c->fork, 0));
   // a batch of one mesh builds on s with the weld and the depth walk beside it, occupancy on side[0]
   const bool alone = njobs == 1;
+  // narrow band: the bricks whose centre (a cell-centred grid of spacing 8h) lies within
+  // band*h + 3.5*sqrt(3)*h of the surface (1-Lipschitz: every voxel of the band), flagged by the triangles
+  std::vector<uint32_t> band_jobs;
+  std::vector<uint32_t *> band_bricks(njobs, nullptr);
+  std::vector<char> band_selected(njobs, 0);
+  for (uint32_t j = 0; j < njobs; ++j)
+    if (jobs[j].band_voxels > 0 && jobs[j].ntri) band_jobs.push_back(j);
+  uint32_t *d_counts = nullptr;
+  if (!band_jobs.empty()) IB_TRY(arena.get(band_jobs.size(), &d_counts));
+  const auto select_band = [&](size_t k, bool slivers) -> ib_status {
+    const uint32_t j = band_jobs[k];
+    const ib_grid &g = jobs[j].sdf;
+    ib_grid cg = g;
+    cg.h = g.h * 8.0f;
+    cg.nx = g.nx / 8;
+    cg.ny = g.ny / 8;
+    cg.nz = g.nz / 8;
+    const uint32_t nbrick = (uint32_t)((g.nx / 8) * (g.ny / 8) * (g.nz / 8));
+    uint32_t *flags, *slots;
+    IB_TRY(arena.get(nbrick + 1, &flags));
+    IB_TRY(arena.get(nbrick + 1, &slots));
+    IB_TRY(arena.get(nbrick, &band_bricks[j]));
+    fill_u32(flags, 0u, nbrick, s);
+    launch_band_splat(mesh[j].tris, mesh[j].recs, mesh[j].n, to_args(cg), band_thresh(jobs[j]), slivers, flags, s);
+    size_t tb = 0;
+    IB_HIP(exclusive_scan_u32(nullptr, tb, flags, slots, nbrick, s));
+    char *temp;
+    IB_TRY(c->scratch("band_scan", tb, &temp));
+    IB_HIP(exclusive_scan_u32(temp, tb, flags, slots, nbrick, s));
+    launch_band_scatter(flags, slots, nbrick, jobs[j].d_brick_index, band_bricks[j], d_counts + k, s);
+    band_selected[j] = 1;
+    return IB_OK;
+  };
   for (uint32_t j = 0; j < njobs; ++j) {
     const ib_job &jb = jobs[j];
     hipStream_t sj = alone ? s : c->side[j % ib_context::kSide];
@@ -1754,8 +1787,11 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     }
     if (jb.occ.nx && alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, so);
     IB_TRY(enqueue_build(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, nullptr, alone));
-    // a mesh built alone: its caps run on side[1] until the columns need them (below), beside the brick
-    // selection of a narrow band
+    // a mesh built alone: its brick selection follows the tree on s before the host enqueues the caps (a
+    // chain of small launches), whose kernels then run on side[1] beside it until the columns need them
+    // (below).  The mesh's sliver count is not on the host yet: the splat takes the instance that tests
+    // each triangle's sliver flag, which gives the same flags on a mesh without slivers.
+    if (alone && !band_jobs.empty()) IB_TRY(select_band(0, true));
     if (alone) IB_TRY(early_caps(c, jb.d_xyz, (int)jb.ntri, scs + j, mesh[j], &arena, sj, false));
     if (jb.occ.nx && !alone) enqueue_occupancy(jb.d_xyz, (int)jb.ntri, jb.occ, jb.d_occ_words, sj);
   }
@@ -1794,39 +1830,11 @@ ib_status ib_bake_batch_device(ib_context *c, ib_job *jobs, uint32_t njobs, void
     c->stats.bvh_depth = std::max(c->stats.bvh_depth, mesh[j].depth);
   }
   c->record(EV_OCC, s);
-  // 2. narrow band: the bricks whose centre (a cell-centred grid of spacing 8h) lies within
-  //    band*h + 3.5*sqrt(3)*h of the surface (1-Lipschitz: every voxel of the band), flagged by the triangles
-  std::vector<uint32_t> band_jobs;
-  std::vector<uint32_t *> band_bricks(njobs, nullptr), band_count(njobs, nullptr);
-  for (uint32_t j = 0; j < njobs; ++j)
-    if (jobs[j].band_voxels > 0 && jobs[j].ntri) band_jobs.push_back(j);
+  // 2. narrow band: the brick selection of every band job not selected beside its build, and the counts
   if (!band_jobs.empty()) {
     std::vector<uint32_t> host_counts(band_jobs.size());
-    uint32_t *d_counts;
-    IB_TRY(arena.get(band_jobs.size(), &d_counts));
-    for (size_t k = 0; k < band_jobs.size(); ++k) {
-      const uint32_t j = band_jobs[k];
-      const ib_grid &g = jobs[j].sdf;
-      ib_grid cg = g;
-      cg.h = g.h * 8.0f;
-      cg.nx = g.nx / 8;
-      cg.ny = g.ny / 8;
-      cg.nz = g.nz / 8;
-      const uint32_t nbrick = (uint32_t)((g.nx / 8) * (g.ny / 8) * (g.nz / 8));
-      uint32_t *flags, *slots;
-      IB_TRY(arena.get(nbrick + 1, &flags));
-      IB_TRY(arena.get(nbrick + 1, &slots));
-      IB_TRY(arena.get(nbrick, &band_bricks[j]));
-      fill_u32(flags, 0u, nbrick, s);
-      launch_band_splat(mesh[j].tris, mesh[j].recs, mesh[j].n, to_args(cg), band_thresh(jobs[j]), mesh[j].slivers,
-                        flags, s);
-      size_t tb = 0;
-      IB_HIP(exclusive_scan_u32(nullptr, tb, flags, slots, nbrick, s));
-      char *temp;
-      IB_TRY(c->scratch("band_scan", tb, &temp));
-      IB_HIP(exclusive_scan_u32(temp, tb, flags, slots, nbrick, s));
-      launch_band_scatter(flags, slots, nbrick, jobs[j].d_brick_index, band_bricks[j], d_counts + k, s);
-    }
+    for (size_t k = 0; k < band_jobs.size(); ++k)
+      if (!band_selected[band_jobs[k]]) IB_TRY(select_band(k, mesh[band_jobs[k]].slivers));
     IB_HIP(hipMemcpyAsync(host_counts.data(), d_counts, host_counts.size() * sizeof(uint32_t), hipMemcpyDeviceToHost,
                           s));
     IB_HIP(hipStreamSynchronize(s));
